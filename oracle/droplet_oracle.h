@@ -1,0 +1,134 @@
+/*
+ * droplet_oracle.h -- CPU restatement of the droplet solver's per-time-step
+ * path (farzan-t/3D_Droplet_simulation_2007, main.f).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: the GPU product
+ * (libdroplet_gpu.so) never links, loads or calls it.  Only tests/, the
+ * smoke() entry point and bench.py's cpu_baseline leg may use it.
+ *
+ * Every routine states the main.f line range it restates.  Arrays follow the
+ * reference layout: FP64, column-major (nx,ny,nz) with one ghost layer, the
+ * x index unit-stride, Fortran 1-based indices in the code.
+ */
+#ifndef DROPLET_ORACLE_H
+#define DROPLET_ORACLE_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OR_MAXLEV 16
+
+typedef struct or_level {
+    int nx, ny, nz;
+    double *rhs, *res, *u, *cc, *a, *b, *c, *ei;   /* memoryaloc, main.f:305-318 */
+} or_level;
+
+typedef struct or_state {
+    /* /DIMENSION/ (input.h:29-30) */
+    int nx, ny, nz;
+    /* inputdata.in, in readinput order (main.f:188-234) */
+    int tmax, nfilter, cycles;
+    double omega;
+    int npre, ntop, npost, restart;
+    double maxcfl;
+    int iturb, convectype, hswitch;
+    double xvel1_si, yvel1_si, zvel1_si, xvel2_si, yvel2_si, zvel2_si;
+    double rho2_si, rho1_si, sigma_si, visc2_si, visc1_si, grav_si, grav1_si;
+    int icase, ibc;
+    double length_si, radius_si, y0_si, z0_si;
+    int tprint, ntecplot, tsave, surfacet;
+    /* derived switches (main.f:243-268) */
+    int ijet, idrop, ibubble, isheet;
+    /* multigrid heap (memoryaloc, main.f:274-322); lev[0] = Fortran level 1 */
+    int ng;
+    or_level lev[OR_MAXLEV];
+    /* /DINEM/ */
+    double tau_si, timescale, lengthscale, lengthsize, vscale;
+    /* /PHYSICS/ scalars */
+    double tau, time, radius, h, rho1, rho2, mu1, mu2, sigma, g, g1;
+    double y0, z0, xvel1, yvel1, zvel1, xvel2, yvel2, zvel2;
+    /* /DIFF/: tauh2 is declared INTEGER in input.h, so every assignment
+     * tauh2 = tau/(h*h) truncates toward zero (main.f:389,2164,2171) */
+    double tauh;
+    int tauh2;
+    int tswap;
+    /* /SURFCOM/ */
+    double *u, *v, *w, *s11, *s22, *s33, *s12, *s13, *s23;
+    /* /PHYSICS/ arrays and /TURBULENCE/ arrays used on the path */
+    double *rhon, *mun, *mut, *ktur, *eps, *cmu;
+    /* pressure solver selection: 0 = reference FMG schedule (mglin),
+     * 1 = FP64 V-cycles to a relative-residual tolerance,
+     * 2 = FP32 V-cycles inside an FP64 defect-correction loop */
+    int mg_mode;
+    int mg_nu1, mg_nu2, mg_maxcycles, mg_guess;   /* guess: 0 zero, 1 previous p */
+    double mg_tol;
+    int mg_cycles_last;
+    double mg_relres_last, mg_bnorm_last;
+    double *p_prev;
+    /* last step diagnostics */
+    double cfl_last;
+    /* tolerance-mode options: outer 0 = defect correction, 1 = (flexible)
+     * preconditioned CG; prolong 0 = injection + 8-average restriction,
+     * 1 = trilinear + its transpose */
+    int mg_outer, mg_prolong;
+} or_state;
+
+/* field ids shared with the GPU library's dg_field ids */
+enum {
+    OR_F_CC = 0, OR_F_U, OR_F_V, OR_F_W, OR_F_P,
+    OR_F_S11, OR_F_S22, OR_F_S33, OR_F_S12, OR_F_S13, OR_F_S23,
+    OR_F_RHON, OR_F_MUN, OR_F_MUT, OR_F_KTUR, OR_F_EPS,
+    OR_F_A, OR_F_B, OR_F_C, OR_F_EI, OR_F_RHS, OR_F_RES,
+    OR_F_PPREV,
+    OR_NFIELDS
+};
+
+/* ---- set-up: readinput + memoryaloc + nondim + IC (main.f:121-322,341-446,5678-5752) */
+or_state *or_create(const char *inputdata_path, char *errbuf, int errlen);
+void      or_destroy(or_state *st);
+double   *or_field(or_state *st, int id);
+double   *or_level_array(or_state *st, int level, int which);  /* which: 0 rhs,1 res,2 u,3 cc,4 a,5 b,6 c,7 ei */
+int       or_level_dims(or_state *st, int level, int *dims3);
+
+/* ---- one time step and its parts (main.f:19-73) */
+void or_step(or_state *st, int t);
+void or_vofmethod(or_state *st, int t);           /* main.f:795-824 */
+void or_kepsmodel(or_state *st, int t);           /* main.f:1888-2026 */
+void or_checktime(or_state *st, int t);           /* main.f:2156-2175 */
+void or_csf(or_state *st);                        /* main.f:2363-2481 */
+void or_solvelpres(or_state *st, int t);          /* main.f:3236-3439 */
+void or_calcrho(or_state *st);                    /* main.f:2177-2190 */
+double or_cfl(or_state *st);                      /* main.f:4848-4881 */
+
+/* ---- pure kernels with the reference's argument lists */
+double or_al3d(double b1, double b2, double b3, double cc);                         /* main.f:4562 */
+double or_fl3d(double m1, double m2, double m3, double alpha, double r0, double dr0); /* main.f:4639 */
+void or_relax(double *u, double lengthsize, const double *rhs, const double *a, const double *b,
+              const double *c, const double *ei, int nx, int ny, int nz, double omega);      /* main.f:3956 */
+void or_resid(double lengthsize, double *res, const double *u, const double *rhs, const double *a,
+              const double *b, const double *c, int nx, int ny, int nz);                      /* main.f:3902 */
+void or_rstrct(double *uc, const double *uf, int ncx, int ncy, int ncz);                   /* main.f:3765 */
+void or_interp(double *uf, const double *uc, int nfx, int nfy, int nfz);                   /* main.f:3866 */
+void or_addint(double *uf, const double *uc, double *res, int nfx, int nfy, int nfz);      /* main.f:3934 */
+void or_acfromac(double *a, double *b, double *c, double *ei, const double *a2, const double *b2,
+                 const double *c2, int nx, int ny, int nz);                                /* main.f:3793 */
+void or_slvsml(double *u, const double *rhs, const double *a, const double *b, const double *c,
+               const double *ei, int nx, int ny, int nz, int ntop, double omega, double lengthsize); /* main.f:3846 */
+void or_abcei(double *a, double *b, double *c, double *ei, const double *cc, double r1, double r2,
+              int nx, int ny, int nz);                                                      /* main.f:3582 */
+void or_bc_scalar(double *s, int nx, int ny, int nz);                                      /* main.f:4298 */
+void or_filterc27(const double *cc, double *cf, int nx, int ny, int nz);                   /* main.f:5473 */
+void or_mglin(or_state *st);                                                               /* main.f:3645 */
+
+/* ---- tolerance-mode pressure solve (no reference counterpart): solves the
+ * finest-level system held in lev[ng-1] (rhs, a, b, c, ei) into lev[ng-1].u */
+int  or_mg_solve_tol(or_state *st);
+double or_resid_norm(or_state *st, double *bnorm_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

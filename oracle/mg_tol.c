@@ -1,0 +1,398 @@
+/*
+ * mg_tol.c -- CPU statement of the tolerance-mode pressure solver
+ * (TEST INFRASTRUCTURE ONLY: checker and CPU baseline for the GPU path).
+ *
+ * The reference's mglin (main.f:3645-3763) runs a fixed full-multigrid
+ * schedule with no stopping test.  This solver iterates until the FP64
+ * relative residual ||b - A p||_2 / ||b||_2 of the finest-level system is
+ * below mg_tol.  It keeps the reference's discrete operator exactly:
+ *   (A p)_ijk = h^-2 sum_f a_f (p_nb - p)          (resid, main.f:3917-3923)
+ * with bc_scalar ghosts (main.f:4298-4329): p_nb = p across every boundary
+ * face except x-max, whose ghost value is 0.  Coarse operators are the
+ * reference's face-coefficient averages (acfromac, main.f:3793-3844).
+ *
+ * Method (no reference counterpart; the converged answer is fixed by A and
+ * b, only the iteration count depends on these choices):
+ *  - outer loop: preconditioned conjugate gradients in FP64 (mg_outer=1,
+ *    flexible Polak-Ribiere beta) or plain defect correction (mg_outer=0);
+ *  - preconditioner: one symmetric V(nu1,nu2)-cycle: red-black Gauss-Seidel
+ *    with the operator's own diagonal (red->black before, black->red after
+ *    the coarse correction), prolongation trilinear (mg_prolong=1, weights
+ *    3/4,1/4 per axis) with restriction its transpose/8, or injection with
+ *    the 8-child average (mg_prolong=0, rstrct main.f:3765-3791);
+ *    coarsest level solved directly (dense Cholesky, FP64);
+ *  - mg_mode 2 runs the V-cycle in FP32 (coefficients, smoother,
+ *    transfers) while every outer-loop vector and residual stays FP64.
+ */
+#include "droplet_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define IDX(i, j, k) (((size_t)(i) - 1) + (size_t)nx * (((size_t)(j) - 1) + (size_t)ny * ((size_t)(k) - 1)))
+
+/* 1-D transfer weight between fine index i (2..nf-1) and coarse index c
+ * (2..nc-1) for the trilinear prolongation with bc_scalar ghost folding.
+ * dirichlet_hi: the high ghost is 0 (x axis), else it copies the last cell. */
+static double tw1(int i, int c, int nc, int dirichlet_hi)
+{
+    int par = i / 2 + 1, s = (i & 1) ? 1 : -1, nb = par + s;
+    double w = 0.0;
+    if (par == c) w += 0.75;
+    if (nb == 1) nb = 2;                        /* low ghost copies cell 2 */
+    if (nb == nc) nb = dirichlet_hi ? -1 : nc - 1;
+    if (nb == c) w += 0.25;
+    return w;
+}
+
+#define DEFINE_LEVEL_KERNELS(T, SFX)                                                                              \
+    static inline T nb_##SFX(const T *u, int nx, int ny, int nz, int i, int j, int k, int di, int dj, int dk)      \
+    {                                                                                                             \
+        int ii = i + di, jj = j + dj, kk = k + dk;                                                                \
+        if (ii == nx) return (T)0;                                                                                \
+        if (ii == 1) ii = 2;                                                                                      \
+        if (jj == 1) jj = 2;                                                                                      \
+        if (jj == ny) jj = ny - 1;                                                                                \
+        if (kk == 1) kk = 2;                                                                                      \
+        if (kk == nz) kk = nz - 1;                                                                                \
+        return u[IDX(ii, jj, kk)];                                                                                \
+    }                                                                                                             \
+    /* one Gauss-Seidel colour pass (col 0: i+j+k even) */                                                       \
+    static void gs_colour_##SFX(T *u, const T *rhs, const T *a, const T *b, const T *c, int nx, int ny, int nz,    \
+                                T h2, T om, int col)                                                              \
+    {                                                                                                             \
+        for (int k = 2; k <= nz - 1; k++)                                                                         \
+            for (int j = 2; j <= ny - 1; j++)                                                                     \
+                for (int i = 2 + ((col + j + k) & 1); i <= nx - 1; i += 2) {                                      \
+                    T aw = a[IDX(i - 1, j, k)], ae = a[IDX(i, j, k)];                                             \
+                    T bs = b[IDX(i, j - 1, k)], bn = b[IDX(i, j, k)];                                             \
+                    T cb = c[IDX(i, j, k - 1)], ct = c[IDX(i, j, k)];                                             \
+                    T s = (T)0, d = ae;                                                                           \
+                    if (i < nx - 1) s += ae * u[IDX(i + 1, j, k)];                                                \
+                    if (i > 2) { s += aw * u[IDX(i - 1, j, k)]; d += aw; }                                        \
+                    if (j < ny - 1) { s += bn * u[IDX(i, j + 1, k)]; d += bn; }                                   \
+                    if (j > 2) { s += bs * u[IDX(i, j - 1, k)]; d += bs; }                                        \
+                    if (k < nz - 1) { s += ct * u[IDX(i, j, k + 1)]; d += ct; }                                   \
+                    if (k > 2) { s += cb * u[IDX(i, j, k - 1)]; d += cb; }                                        \
+                    T g = (s - h2 * rhs[IDX(i, j, k)]) / d;                                                       \
+                    u[IDX(i, j, k)] = om * g + ((T)1 - om) * u[IDX(i, j, k)];                                     \
+                }                                                                                                 \
+    }                                                                                                             \
+    static void resid_##SFX(T *res, const T *u, const T *rhs, const T *a, const T *b, const T *c, int nx, int ny,  \
+                            int nz, T h2i)                                                                        \
+    {                                                                                                             \
+        for (int k = 2; k <= nz - 1; k++)                                                                         \
+            for (int j = 2; j <= ny - 1; j++)                                                                     \
+                for (int i = 2; i <= nx - 1; i++) {                                                               \
+                    T u0 = u[IDX(i, j, k)];                                                                       \
+                    T s = a[IDX(i, j, k)] * (nb_##SFX(u, nx, ny, nz, i, j, k, 1, 0, 0) - u0) +                    \
+                          a[IDX(i - 1, j, k)] * (nb_##SFX(u, nx, ny, nz, i, j, k, -1, 0, 0) - u0) +               \
+                          b[IDX(i, j, k)] * (nb_##SFX(u, nx, ny, nz, i, j, k, 0, 1, 0) - u0) +                    \
+                          b[IDX(i, j - 1, k)] * (nb_##SFX(u, nx, ny, nz, i, j, k, 0, -1, 0) - u0) +               \
+                          c[IDX(i, j, k)] * (nb_##SFX(u, nx, ny, nz, i, j, k, 0, 0, 1) - u0) +                    \
+                          c[IDX(i, j, k - 1)] * (nb_##SFX(u, nx, ny, nz, i, j, k, 0, 0, -1) - u0);                \
+                    res[IDX(i, j, k)] = rhs[IDX(i, j, k)] - h2i * s;                                              \
+                }                                                                                                 \
+    }                                                                                                             \
+    static void restrict_##SFX(T *uc, const T *uf, int ncx, int ncy, int ncz, int tri)                           \
+    {                                                                                                             \
+        int nx = 2 * ncx - 2, ny = 2 * ncy - 2, nz = 2 * ncz - 2;                                                 \
+        for (int kc = 2; kc <= ncz - 1; kc++)                                                                     \
+            for (int jc = 2; jc <= ncy - 1; jc++)                                                                 \
+                for (int ic = 2; ic <= ncx - 1; ic++) {                                                           \
+                    T s = (T)0;                                                                                   \
+                    if (!tri) {                                                                                   \
+                        int i = 2 * ic - 2, j = 2 * jc - 2, k = 2 * kc - 2;                                       \
+                        s = uf[IDX(i, j, k)] + uf[IDX(i + 1, j, k)] + uf[IDX(i, j + 1, k)] +                      \
+                            uf[IDX(i + 1, j + 1, k)] + uf[IDX(i, j, k + 1)] + uf[IDX(i + 1, j, k + 1)] +          \
+                            uf[IDX(i, j + 1, k + 1)] + uf[IDX(i + 1, j + 1, k + 1)];                              \
+                    } else {                                                                                      \
+                        for (int k = 2 * kc - 3; k <= 2 * kc; k++) {                                              \
+                            if (k < 2 || k > nz - 1) continue;                                                    \
+                            T wk = (T)tw1(k, kc, ncz, 0);                                                         \
+                            for (int j = 2 * jc - 3; j <= 2 * jc; j++) {                                          \
+                                if (j < 2 || j > ny - 1) continue;                                                \
+                                T wj = (T)tw1(j, jc, ncy, 0);                                                     \
+                                for (int i = 2 * ic - 3; i <= 2 * ic; i++) {                                      \
+                                    if (i < 2 || i > nx - 1) continue;                                            \
+                                    s += (T)tw1(i, ic, ncx, 1) * wj * wk * uf[IDX(i, j, k)];                      \
+                                }                                                                                 \
+                            }                                                                                     \
+                        }                                                                                         \
+                    }                                                                                             \
+                    uc[((size_t)ic - 1) + (size_t)ncx * (((size_t)jc - 1) + (size_t)ncy * ((size_t)kc - 1))] =   \
+                        (T)0.125 * s;                                                                             \
+                }                                                                                                 \
+    }                                                                                                             \
+    static void prolong_add_##SFX(T *uf, const T *uc, int nfx, int nfy, int nfz, int tri)                        \
+    {                                                                                                             \
+        int ncx = nfx / 2 + 1, ncy = nfy / 2 + 1, ncz = nfz / 2 + 1;                                              \
+        for (int k = 2; k <= nfz - 1; k++)                                                                        \
+            for (int j = 2; j <= nfy - 1; j++)                                                                    \
+                for (int i = 2; i <= nfx - 1; i++) {                                                              \
+                    int ic = i / 2 + 1, jc = j / 2 + 1, kc = k / 2 + 1;                                           \
+                    T acc;                                                                                        \
+                    if (!tri) {                                                                                   \
+                        acc = nb_##SFX(uc, ncx, ncy, ncz, ic, jc, kc, 0, 0, 0);                                   \
+                    } else {                                                                                      \
+                        int si = (i & 1) ? 1 : -1, sj = (j & 1) ? 1 : -1, sk = (k & 1) ? 1 : -1;                  \
+                        acc = (T)0;                                                                               \
+                        for (int q = 0; q < 8; q++) {                                                             \
+                            int oi = (q & 1) ? si : 0, oj = (q & 2) ? sj : 0, ok = (q & 4) ? sk : 0;              \
+                            T w = ((q & 1) ? (T)0.25 : (T)0.75) * ((q & 2) ? (T)0.25 : (T)0.75) *                 \
+                                  ((q & 4) ? (T)0.25 : (T)0.75);                                                  \
+                            acc += w * nb_##SFX(uc, ncx, ncy, ncz, ic, jc, kc, oi, oj, ok);                       \
+                        }                                                                                         \
+                    }                                                                                             \
+                    int nx = nfx, ny = nfy;                                                                       \
+                    uf[IDX(i, j, k)] += acc;                                                                      \
+                }                                                                                                 \
+    }
+
+DEFINE_LEVEL_KERNELS(double, d)
+DEFINE_LEVEL_KERNELS(float, f)
+
+/* dense Cholesky solve on the coarsest level: -A u = -rhs is SPD */
+static void coarse_direct(double *u, const double *rhs, const double *a, const double *b, const double *c, int nx,
+                          int ny, int nz, double h2i)
+{
+    int mx = nx - 2, my = ny - 2, mz = nz - 2, n = mx * my * mz;
+    double *M = calloc((size_t)n * n, sizeof(double));
+    double *r = malloc(sizeof(double) * n);
+#define CID(i, j, k) (((i) - 2) + mx * (((j) - 2) + my * ((k) - 2)))
+    for (int k = 2; k <= nz - 1; k++)
+        for (int j = 2; j <= ny - 1; j++)
+            for (int i = 2; i <= nx - 1; i++) {
+                int row = CID(i, j, k);
+                double d = 0.0;
+                double cf[6] = {a[IDX(i, j, k)], a[IDX(i - 1, j, k)], b[IDX(i, j, k)],
+                                b[IDX(i, j - 1, k)], c[IDX(i, j, k)], c[IDX(i, j, k - 1)]};
+                int ni[6] = {i + 1, i - 1, i, i, i, i}, nj[6] = {j, j, j + 1, j - 1, j, j}, nk[6] = {k, k, k, k, k + 1, k - 1};
+                for (int f = 0; f < 6; f++) {
+                    if (ni[f] == nx) { d += cf[f]; continue; }
+                    if (ni[f] == 1 || nj[f] == 1 || nj[f] == ny || nk[f] == 1 || nk[f] == nz) continue;
+                    d += cf[f];
+                    M[(size_t)row * n + CID(ni[f], nj[f], nk[f])] -= cf[f] * h2i;
+                }
+                M[(size_t)row * n + row] += d * h2i;
+                r[row] = -rhs[IDX(i, j, k)];
+            }
+    for (int p = 0; p < n; p++) {
+        double s = M[(size_t)p * n + p];
+        for (int q = 0; q < p; q++) s -= M[(size_t)p * n + q] * M[(size_t)p * n + q];
+        double l = sqrt(s);
+        M[(size_t)p * n + p] = l;
+        for (int rr = p + 1; rr < n; rr++) {
+            double t = M[(size_t)rr * n + p];
+            for (int q = 0; q < p; q++) t -= M[(size_t)rr * n + q] * M[(size_t)p * n + q];
+            M[(size_t)rr * n + p] = t / l;
+        }
+    }
+    for (int p = 0; p < n; p++) {
+        double t = r[p];
+        for (int q = 0; q < p; q++) t -= M[(size_t)p * n + q] * r[q];
+        r[p] = t / M[(size_t)p * n + p];
+    }
+    for (int p = n - 1; p >= 0; p--) {
+        double t = r[p];
+        for (int q = p + 1; q < n; q++) t -= M[(size_t)q * n + p] * r[q];
+        r[p] = t / M[(size_t)p * n + p];
+    }
+    for (int k = 2; k <= nz - 1; k++)
+        for (int j = 2; j <= ny - 1; j++)
+            for (int i = 2; i <= nx - 1; i++) u[IDX(i, j, k)] = r[CID(i, j, k)];
+#undef CID
+    free(M);
+    free(r);
+}
+
+typedef struct tol_ws {
+    int ng;
+    int nx[OR_MAXLEV], ny[OR_MAXLEV], nz[OR_MAXLEV];
+    double *u[OR_MAXLEV], *r[OR_MAXLEV], *f[OR_MAXLEV];
+    float *uf[OR_MAXLEV], *rf[OR_MAXLEV], *ff[OR_MAXLEV];
+    float *af[OR_MAXLEV], *bf[OR_MAXLEV], *cf[OR_MAXLEV];
+} tol_ws;
+
+static double lev_h2i(or_state *st, int nx) { double h = st->lengthsize / (double)(nx - 2); return 1.0 / (h * h); }
+
+/* symmetric V-cycle on levels l..0 in FP64: approximately solves A e = f */
+static void vcycle_d(or_state *st, tol_ws *W, int l)
+{
+    or_level *L = &st->lev[l];
+    int nx = W->nx[l], ny = W->ny[l], nz = W->nz[l];
+    double h2i = lev_h2i(st, nx), h2 = 1.0 / h2i;
+    if (l == 0) {
+        coarse_direct(W->u[0], W->f[0], L->a, L->b, L->c, nx, ny, nz, h2i);
+        return;
+    }
+    for (int s = 0; s < st->mg_nu1; s++) {
+        gs_colour_d(W->u[l], W->f[l], L->a, L->b, L->c, nx, ny, nz, h2, 1.0, 0);
+        gs_colour_d(W->u[l], W->f[l], L->a, L->b, L->c, nx, ny, nz, h2, 1.0, 1);
+    }
+    resid_d(W->r[l], W->u[l], W->f[l], L->a, L->b, L->c, nx, ny, nz, h2i);
+    restrict_d(W->f[l - 1], W->r[l], W->nx[l - 1], W->ny[l - 1], W->nz[l - 1], st->mg_prolong);
+    memset(W->u[l - 1], 0, sizeof(double) * (size_t)W->nx[l - 1] * W->ny[l - 1] * W->nz[l - 1]);
+    vcycle_d(st, W, l - 1);
+    prolong_add_d(W->u[l], W->u[l - 1], nx, ny, nz, st->mg_prolong);
+    for (int s = 0; s < st->mg_nu2; s++) {
+        gs_colour_d(W->u[l], W->f[l], L->a, L->b, L->c, nx, ny, nz, h2, 1.0, 1);
+        gs_colour_d(W->u[l], W->f[l], L->a, L->b, L->c, nx, ny, nz, h2, 1.0, 0);
+    }
+}
+
+static void vcycle_f(or_state *st, tol_ws *W, int l)
+{
+    int nx = W->nx[l], ny = W->ny[l], nz = W->nz[l];
+    double h2i = lev_h2i(st, nx);
+    if (l == 0) {
+        size_t n = (size_t)nx * ny * nz;
+        for (size_t p = 0; p < n; p++) W->f[0][p] = (double)W->ff[0][p];
+        or_level *L = &st->lev[0];
+        coarse_direct(W->u[0], W->f[0], L->a, L->b, L->c, nx, ny, nz, h2i);
+        for (size_t p = 0; p < n; p++) W->uf[0][p] = (float)W->u[0][p];
+        return;
+    }
+    float h2if = (float)h2i, h2f = (float)(1.0 / h2i);
+    for (int s = 0; s < st->mg_nu1; s++) {
+        gs_colour_f(W->uf[l], W->ff[l], W->af[l], W->bf[l], W->cf[l], nx, ny, nz, h2f, 1.0f, 0);
+        gs_colour_f(W->uf[l], W->ff[l], W->af[l], W->bf[l], W->cf[l], nx, ny, nz, h2f, 1.0f, 1);
+    }
+    resid_f(W->rf[l], W->uf[l], W->ff[l], W->af[l], W->bf[l], W->cf[l], nx, ny, nz, h2if);
+    restrict_f(W->ff[l - 1], W->rf[l], W->nx[l - 1], W->ny[l - 1], W->nz[l - 1], st->mg_prolong);
+    memset(W->uf[l - 1], 0, sizeof(float) * (size_t)W->nx[l - 1] * W->ny[l - 1] * W->nz[l - 1]);
+    vcycle_f(st, W, l - 1);
+    prolong_add_f(W->uf[l], W->uf[l - 1], nx, ny, nz, st->mg_prolong);
+    for (int s = 0; s < st->mg_nu2; s++) {
+        gs_colour_f(W->uf[l], W->ff[l], W->af[l], W->bf[l], W->cf[l], nx, ny, nz, h2f, 1.0f, 1);
+        gs_colour_f(W->uf[l], W->ff[l], W->af[l], W->bf[l], W->cf[l], nx, ny, nz, h2f, 1.0f, 0);
+    }
+}
+
+static double dot_interior(const double *x, const double *y, int nx, int ny, int nz)
+{
+    double s = 0.0;
+    for (int k = 2; k <= nz - 1; k++)
+        for (int j = 2; j <= ny - 1; j++)
+            for (int i = 2; i <= nx - 1; i++) s += x[IDX(i, j, k)] * y[IDX(i, j, k)];
+    return s;
+}
+
+/* z = M r: one V-cycle on the finest level from a zero initial guess */
+static void precond(or_state *st, tol_ws *W, const double *r, double *z)
+{
+    int top = W->ng - 1;
+    size_t n = (size_t)W->nx[top] * W->ny[top] * W->nz[top];
+    if (st->mg_mode == 2) {
+        for (size_t q = 0; q < n; q++) { W->ff[top][q] = (float)r[q]; W->uf[top][q] = 0.0f; }
+        vcycle_f(st, W, top);
+        for (size_t q = 0; q < n; q++) z[q] = (double)W->uf[top][q];
+    } else {
+        memcpy(W->f[top], r, sizeof(double) * n);
+        memset(W->u[top], 0, sizeof(double) * n);
+        vcycle_d(st, W, top);
+        memcpy(z, W->u[top], sizeof(double) * n);
+    }
+}
+
+double or_resid_norm(or_state *st, double *bnorm_out)
+{
+    or_level *L = &st->lev[st->ng - 1];
+    int nx = L->nx, ny = L->ny, nz = L->nz;
+    double *r = calloc((size_t)nx * ny * nz, sizeof(double));
+    resid_d(r, L->u, L->rhs, L->a, L->b, L->c, nx, ny, nz, lev_h2i(st, nx));
+    double rn = sqrt(dot_interior(r, r, nx, ny, nz));
+    if (bnorm_out) *bnorm_out = sqrt(dot_interior(L->rhs, L->rhs, nx, ny, nz));
+    free(r);
+    return rn;
+}
+
+int or_mg_solve_tol(or_state *st)
+{
+    int ng = st->ng;
+    for (int g = ng - 1; g >= 1; g--) {
+        or_level *C = &st->lev[g - 1], *F = &st->lev[g];
+        or_acfromac(C->a, C->b, C->c, C->ei, F->a, F->b, F->c, C->nx, C->ny, C->nz);
+    }
+    tol_ws W;
+    memset(&W, 0, sizeof W);
+    W.ng = ng;
+    for (int g = 0; g < ng; g++) {
+        or_level *L = &st->lev[g];
+        W.nx[g] = L->nx; W.ny[g] = L->ny; W.nz[g] = L->nz;
+        size_t n = (size_t)L->nx * L->ny * L->nz;
+        W.u[g] = calloc(n, sizeof(double)); W.r[g] = calloc(n, sizeof(double)); W.f[g] = calloc(n, sizeof(double));
+        if (st->mg_mode == 2) {
+            W.uf[g] = calloc(n, sizeof(float)); W.rf[g] = calloc(n, sizeof(float)); W.ff[g] = calloc(n, sizeof(float));
+            W.af[g] = malloc(n * sizeof(float)); W.bf[g] = malloc(n * sizeof(float)); W.cf[g] = malloc(n * sizeof(float));
+            for (size_t p = 0; p < n; p++) {
+                W.af[g][p] = (float)L->a[p]; W.bf[g][p] = (float)L->b[p]; W.cf[g][p] = (float)L->c[p];
+            }
+        }
+    }
+    or_level *L = &st->lev[ng - 1];
+    int nx = L->nx, ny = L->ny, nz = L->nz;
+    size_t n = (size_t)nx * ny * nz;
+    double *x = L->u;
+    if (st->mg_guess == 1) memcpy(x, st->p_prev, sizeof(double) * n);
+    else memset(x, 0, sizeof(double) * n);
+    double h2i = lev_h2i(st, nx);
+    double bn = sqrt(dot_interior(L->rhs, L->rhs, nx, ny, nz));
+    if (bn == 0.0) bn = 1.0;
+    double *r = calloc(n, sizeof(double)), *z = calloc(n, sizeof(double)), *p = calloc(n, sizeof(double)),
+           *q = calloc(n, sizeof(double));
+    resid_d(r, x, L->rhs, L->a, L->b, L->c, nx, ny, nz, h2i);
+    double rel = sqrt(dot_interior(r, r, nx, ny, nz)) / bn;
+    int it = 0;
+    if (st->mg_outer == 1) {
+        double rz = 0.0, alpha = 0.0;
+        while (rel >= st->mg_tol && it < st->mg_maxcycles) {
+            precond(st, &W, r, z);
+            double rz_new = dot_interior(r, z, nx, ny, nz);
+            if (it == 0) {
+                memcpy(p, z, sizeof(double) * n);
+            } else {
+                /* flexible (Polak-Ribiere) beta = z_new.(r_new - r_old) / rz_old,
+                 * with r_new - r_old = -alpha q from the previous iteration */
+                double beta = -alpha * dot_interior(z, q, nx, ny, nz) / rz;
+                for (size_t s = 0; s < n; s++) p[s] = z[s] + beta * p[s];
+            }
+            rz = rz_new;
+            /* q = A p (ghosts of p folded in the operator) */
+            resid_d(q, p, z, L->a, L->b, L->c, nx, ny, nz, h2i); /* q = z - A p */
+            for (int k = 2; k <= nz - 1; k++)
+                for (int j = 2; j <= ny - 1; j++)
+                    for (int i = 2; i <= nx - 1; i++) q[IDX(i, j, k)] = z[IDX(i, j, k)] - q[IDX(i, j, k)];
+            double pq = dot_interior(p, q, nx, ny, nz);
+            alpha = rz / pq;
+            for (int k = 2; k <= nz - 1; k++)
+                for (int j = 2; j <= ny - 1; j++)
+                    for (int i = 2; i <= nx - 1; i++) {
+                        x[IDX(i, j, k)] += alpha * p[IDX(i, j, k)];
+                        r[IDX(i, j, k)] -= alpha * q[IDX(i, j, k)];
+                    }
+            rel = sqrt(dot_interior(r, r, nx, ny, nz)) / bn;
+            it++;
+        }
+    } else {
+        while (rel >= st->mg_tol && it < st->mg_maxcycles) {
+            precond(st, &W, r, z);
+            for (size_t s = 0; s < n; s++) x[s] += z[s];
+            resid_d(r, x, L->rhs, L->a, L->b, L->c, nx, ny, nz, h2i);
+            rel = sqrt(dot_interior(r, r, nx, ny, nz)) / bn;
+            it++;
+        }
+    }
+    st->mg_cycles_last = it;
+    st->mg_relres_last = rel;
+    st->mg_bnorm_last = bn;
+    or_bc_scalar(x, nx, ny, nz);
+    memcpy(st->p_prev, x, sizeof(double) * n);
+    free(r); free(z); free(p); free(q);
+    for (int g = 0; g < ng; g++) {
+        free(W.u[g]); free(W.r[g]); free(W.f[g]);
+        free(W.uf[g]); free(W.rf[g]); free(W.ff[g]); free(W.af[g]); free(W.bf[g]); free(W.cf[g]);
+    }
+    return it;
+}

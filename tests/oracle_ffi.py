@@ -1,0 +1,158 @@
+"""ctypes binding of the CPU restatement (oracle/_build/liboracle.so).
+
+Test infrastructure only: the checker for the GPU product and the CPU
+baseline of bench.py.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+ORACLE_SO = os.path.join(REPO, "oracle", "_build", "liboracle.so")
+
+FIELD_IDS = dict(cc=0, u=1, v=2, w=3, p=4, s11=5, s22=6, s33=7, s12=8, s13=9, s23=10,
+                 rhon=11, mun=12, mut=13, ktur=14, eps=15, a=16, b=17, c=18, ei=19, rhs=20, res=21,
+                 pprev=22)
+
+_lib = None
+
+
+def build_oracle():
+    subprocess.run(["make", "-s", "-C", os.path.join(REPO, "oracle")], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            build_oracle()
+        L = ctypes.CDLL(ORACLE_SO)
+        L.or_create.restype = ctypes.c_void_p
+        L.or_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
+        L.or_destroy.argtypes = [ctypes.c_void_p]
+        L.or_field.restype = ctypes.POINTER(ctypes.c_double)
+        L.or_field.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.or_level_array.restype = ctypes.POINTER(ctypes.c_double)
+        L.or_level_array.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        L.or_level_dims.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        for nm in ("or_step", "or_vofmethod", "or_kepsmodel", "or_checktime", "or_solvelpres"):
+            getattr(L, nm).argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.or_csf.argtypes = [ctypes.c_void_p]
+        L.or_calcrho.argtypes = [ctypes.c_void_p]
+        L.or_cfl.argtypes = [ctypes.c_void_p]
+        L.or_cfl.restype = ctypes.c_double
+        L.or_al3d.restype = ctypes.c_double
+        L.or_al3d.argtypes = [ctypes.c_double] * 4
+        L.or_fl3d.restype = ctypes.c_double
+        L.or_fl3d.argtypes = [ctypes.c_double] * 6
+        L.or_mg_solve_tol.argtypes = [ctypes.c_void_p]
+        L.or_mglin.argtypes = [ctypes.c_void_p]
+        L.or_resid_norm.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
+        L.or_resid_norm.restype = ctypes.c_double
+        _lib = L
+    return _lib
+
+
+class OrState(ctypes.Structure):
+    """Mirror of or_state's leading scalar members (droplet_oracle.h)."""
+
+
+def _state_fields():
+    d, i, p = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
+
+    class Level(ctypes.Structure):
+        _fields_ = [("nx", i), ("ny", i), ("nz", i)] + [(n, p) for n in ("rhs", "res", "u", "cc", "a", "b", "c", "ei")]
+
+    f = [("nx", i), ("ny", i), ("nz", i), ("tmax", i), ("nfilter", i), ("cycles", i), ("omega", d),
+         ("npre", i), ("ntop", i), ("npost", i), ("restart", i), ("maxcfl", d), ("iturb", i), ("convectype", i),
+         ("hswitch", i)]
+    f += [(n, d) for n in ("xvel1_si", "yvel1_si", "zvel1_si", "xvel2_si", "yvel2_si", "zvel2_si", "rho2_si",
+                           "rho1_si", "sigma_si", "visc2_si", "visc1_si", "grav_si", "grav1_si")]
+    f += [("icase", i), ("ibc", i)]
+    f += [(n, d) for n in ("length_si", "radius_si", "y0_si", "z0_si")]
+    f += [(n, i) for n in ("tprint", "ntecplot", "tsave", "surfacet", "ijet", "idrop", "ibubble", "isheet", "ng")]
+    f += [("lev", Level * 16)]
+    f += [(n, d) for n in ("tau_si", "timescale", "lengthscale", "lengthsize", "vscale", "tau", "time", "radius",
+                           "h", "rho1", "rho2", "mu1", "mu2", "sigma", "g", "g1", "y0", "z0", "xvel1", "yvel1",
+                           "zvel1", "xvel2", "yvel2", "zvel2", "tauh")]
+    f += [("tauh2", i), ("tswap", i)]
+    f += [(n, p) for n in ("u", "v", "w", "s11", "s22", "s33", "s12", "s13", "s23", "rhon", "mun", "mut", "ktur",
+                           "eps", "cmu")]
+    f += [("mg_mode", i), ("mg_nu1", i), ("mg_nu2", i), ("mg_maxcycles", i), ("mg_guess", i), ("mg_tol", d),
+          ("mg_cycles_last", i), ("mg_relres_last", d), ("mg_bnorm_last", d), ("p_prev", p), ("cfl_last", d), ("mg_outer", i), ("mg_prolong", i)]
+    return f
+
+
+OrState._fields_ = _state_fields()
+
+
+class Oracle:
+    """One CPU-restatement instance created from inputdata.in text."""
+
+    def __init__(self, inputdata: str):
+        L = lib()
+        with tempfile.NamedTemporaryFile("w", suffix=".in", delete=False) as f:
+            f.write(inputdata)
+            path = f.name
+        err = ctypes.create_string_buffer(256)
+        try:
+            self.h = L.or_create(path.encode(), err, 256)
+        finally:
+            os.unlink(path)
+        if not self.h:
+            raise RuntimeError(err.value.decode())
+        self.st = ctypes.cast(self.h, ctypes.POINTER(OrState)).contents
+        self.nx, self.ny, self.nz = self.st.nx, self.st.ny, self.st.nz
+
+    def close(self):
+        if self.h:
+            lib().or_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def field(self, name):
+        ptr = lib().or_field(self.h, FIELD_IDS[name])
+        n = self.nx * self.ny * self.nz
+        return np.ctypeslib.as_array(ptr, shape=(n,)).reshape(self.nz, self.ny, self.nx)
+
+    def level(self, lev, which):
+        dims = (ctypes.c_int * 3)()
+        lib().or_level_dims(self.h, lev, dims)
+        ptr = lib().or_level_array(self.h, lev, which)
+        nx, ny, nz = dims
+        return np.ctypeslib.as_array(ptr, shape=(nx * ny * nz,)).reshape(nz, ny, nx)
+
+    def step(self, t):
+        lib().or_step(self.h, int(t))
+
+    def run(self, nsteps, t0=0):
+        hist = []
+        for t in range(t0, t0 + nsteps):
+            self.step(t)
+            hist.append(dict(t=t, tau=self.st.tau, time=self.st.time, tauh=self.st.tauh, tauh2=self.st.tauh2,
+                             npre=self.st.npre, npost=self.st.npost, ntop=self.st.ntop,
+                             vol=float(self.field("cc")[1:-1, 1:-1, 1:-1].sum()), tswap=self.st.tswap,
+                             mg_cycles=self.st.mg_cycles_last, relres=self.st.mg_relres_last))
+        return hist
+
+
+def inputdata_text(nx=18, ny=18, nz=18, tmax=5, cycles=2, omega=1.5, npre=2, ntop=4, npost=2, maxcfl=0.25,
+                   iturb=0, convectype=2, hswitch=0, xvel1=0.0, yvel1=-1.0, zvel1=0.0, xvel2=0.0, yvel2=0.0,
+                   zvel2=0.0, rho2=1.2, rho1=1000.0, sigma=0.072, visc2=1.8e-5, visc1=1.0e-3, grav=-9.81,
+                   grav1=0.0, length=0.004, radius=0.0008, y0=0.0025, z0=1000.0, surfacet=2):
+    """inputdata.in in the reference's 38-line order (main.f:188-234)."""
+    vals = [nx, ny, nz, tmax, 0, cycles, omega, npre, ntop, npost, 0, maxcfl, iturb, convectype, hswitch,
+            xvel1, yvel1, zvel1, xvel2, yvel2, zvel2, rho2, rho1, sigma, visc2, visc1, grav, grav1, 2, 2,
+            length, radius, y0, z0, 100000, 100000, 100000, surfacet]
+    return "".join(f"{v!r}\n" if isinstance(v, float) else f"{v}\n" for v in vals)

@@ -1,0 +1,255 @@
+"""Python host side of libdroplet_gpu.so.
+
+Mirrors the reference's driver (PROGRAM THREE DIM, main.f:2-76) and its
+subroutine interfaces over the C-ABI declared in include/droplet_gpu.h.  The
+GPU library is required: nothing here computes the physics on the CPU, and
+every entry point raises if the library is missing or reports an error.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libdroplet_gpu.so")
+
+FIELD_IDS = dict(cc=0, u=1, v=2, w=3, p=4, s11=5, s22=6, s33=7, s12=8, s13=9, s23=10,
+                 rhon=11, mun=12, mut=13, ktur=14, eps=15, a=16, b=17, c=18, ei=19, rhs=20, res=21,
+                 pprev=22)
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+class DgState(ctypes.Structure):
+    _fields_ = [("tau", ctypes.c_double), ("time", ctypes.c_double), ("tauh", ctypes.c_double),
+                ("tauh2", ctypes.c_int), ("tswap", ctypes.c_int), ("npre", ctypes.c_int),
+                ("ntop", ctypes.c_int), ("npost", ctypes.c_int),
+                ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("ng", ctypes.c_int),
+                ("mg_iters", ctypes.c_int), ("mg_relres", ctypes.c_double), ("cfl", ctypes.c_double),
+                ("epsdiv", ctypes.c_double)]
+
+
+class DropletError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Loads the in-tree HIP library (fails loudly when it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise DropletError(f"{LIBPATH} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIBPATH)
+        L.dg_last_error.restype = ctypes.c_char_p
+        L.dg_init.argtypes = [ctypes.c_char_p]
+        L.dg_get_field.argtypes = [_ip, _dp]
+        L.dg_set_field.argtypes = [_ip, _dp]
+        L.dg_get_state.argtypes = [ctypes.POINTER(DgState)]
+        L.dg_set_state.argtypes = [ctypes.POINTER(DgState)]
+        _lib = L
+    return _lib
+
+
+def _check(rc, what):
+    if rc != 0:
+        msg = lib().dg_last_error().decode(errors="replace")
+        raise DropletError(f"{what}: {msg}")
+
+
+def _i(v):
+    return ctypes.byref(ctypes.c_int(int(v)))
+
+
+def _d(v):
+    return ctypes.byref(ctypes.c_double(float(v)))
+
+
+def _p(a):
+    if a.dtype != np.float64 or not a.flags["C_CONTIGUOUS"]:
+        raise DropletError("arrays must be C-contiguous float64 (reference layout (nz,ny,nx) in numpy order)")
+    return a.ctypes.data_as(_dp)
+
+
+def make_inputdata(nx=66, ny=66, nz=66, tmax=49, cycles=2, omega=1.5, npre=2, ntop=4, npost=2, maxcfl=0.25,
+                   iturb=0, convectype=2, hswitch=0, xvel1=0.0, yvel1=-1.0, zvel1=0.0, xvel2=0.0, yvel2=0.0,
+                   zvel2=0.0, rho2=1.2, rho1=1000.0, sigma=0.072, visc2=1.8e-5, visc1=1.0e-3, grav=-9.81,
+                   grav1=0.0, length=0.004, radius=0.0008, y0=0.0025, z0=1000.0, tprint=100000,
+                   ntecplot=100000, tsave=100000, surfacet=2):
+    """Text of an inputdata.in file in the reference's 38-line order
+    (readinput, main.f:188-234).  Defaults: a water droplet in air on a 4 mm
+    cube falling at 1 m/s along the gravity axis (the reference's y)."""
+    vals = [nx, ny, nz, tmax, 0, cycles, omega, npre, ntop, npost, 0, maxcfl, iturb, convectype, hswitch,
+            xvel1, yvel1, zvel1, xvel2, yvel2, zvel2, rho2, rho1, sigma, visc2, visc1, grav, grav1, 2, 2,
+            length, radius, y0, z0, tprint, ntecplot, tsave, surfacet]
+    return "".join(f"{v!r}\n" if isinstance(v, float) else f"{v}\n" for v in vals)
+
+
+class DropletGPU:
+    """The resident device state of one run (the reference's COMMON blocks).
+
+    Only one instance is active per process, as in the reference."""
+
+    def __init__(self, inputdata_path=None, inputdata_text=None):
+        L = lib()
+        if inputdata_text is not None:
+            with tempfile.NamedTemporaryFile("w", suffix=".in", delete=False) as f:
+                f.write(inputdata_text)
+                path = f.name
+            try:
+                _check(L.dg_init(path.encode()), "dg_init")
+            finally:
+                os.unlink(path)
+        else:
+            _check(L.dg_init(os.fspath(inputdata_path).encode()), "dg_init")
+        st = self.state()
+        self.nx, self.ny, self.nz = st["nx"], st["ny"], st["nz"]
+
+    def close(self):
+        lib().dg_finalize()
+
+    # ---- the reference's per-step operators
+    def step(self, t):
+        _check(lib().dg_step(_i(t)), "dg_step")
+
+    def run(self, nsteps, t0=0):
+        _check(lib().dg_run(_i(t0), _i(nsteps)), "dg_run")
+
+    def vofmethod(self, t):
+        _check(lib().dg_vofmethod(_i(t)), "dg_vofmethod")
+
+    def kepsmodel(self, t):
+        _check(lib().dg_kepsmodel(_i(t)), "dg_kepsmodel")
+
+    def checktime(self, t):
+        _check(lib().dg_checktime(_i(t)), "dg_checktime")
+
+    def csf(self):
+        _check(lib().dg_csf(), "dg_csf")
+
+    def solvelpres(self, t):
+        _check(lib().dg_solvelpres(_i(t)), "dg_solvelpres")
+
+    def calcrho(self):
+        _check(lib().dg_calcrho(), "dg_calcrho")
+
+    def clip_cc(self):
+        _check(lib().dg_clip_cc(), "dg_clip_cc")
+
+    def set_solver(self, mode=0, tol=1e-8, nu1=2, nu2=2, maxit=200, guess=0):
+        _check(lib().dg_set_solver(_i(mode), _d(tol), _i(nu1), _i(nu2), _i(maxit), _i(guess)), "dg_set_solver")
+
+    def set_ordered_sums(self, on):
+        _check(lib().dg_set_ordered_sums(_i(1 if on else 0)), "dg_set_ordered_sums")
+
+    # ---- state exchange
+    def get(self, name):
+        out = np.empty((self.nz, self.ny, self.nx), dtype=np.float64)
+        _check(lib().dg_get_field(ctypes.byref(ctypes.c_int(FIELD_IDS[name])), _p(out)), "dg_get_field")
+        return out
+
+    def set(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.nz, self.ny, self.nx)
+        _check(lib().dg_set_field(ctypes.byref(ctypes.c_int(FIELD_IDS[name])), _p(a)), "dg_set_field")
+
+    def state(self):
+        s = DgState()
+        _check(lib().dg_get_state(ctypes.byref(s)), "dg_get_state")
+        return {k: getattr(s, k) for k, _ in DgState._fields_}
+
+    def set_state(self, **kw):
+        s = DgState()
+        _check(lib().dg_get_state(ctypes.byref(s)), "dg_get_state")
+        for k, v in kw.items():
+            setattr(s, k, v)
+        _check(lib().dg_set_state(ctypes.byref(s)), "dg_set_state")
+
+
+# ---- host-pointer kernels with the reference's argument lists -------------
+
+def relax(u, lengthsize, rhs, a, b, c, ei, omega):
+    nz, ny, nx = u.shape
+    _check(lib().dg_relax(_p(u), _d(lengthsize), _p(rhs), _p(a), _p(b), _p(c), _p(ei), _i(2), _i(nx), _i(ny),
+                          _i(nz), _d(omega)), "dg_relax")
+
+
+def resid(lengthsize, res, u, rhs, a, b, c):
+    nz, ny, nx = u.shape
+    _check(lib().dg_resid(_d(lengthsize), _p(res), _p(u), _p(rhs), _p(a), _p(b), _p(c), _i(2), _i(nx), _i(ny),
+                          _i(nz)), "dg_resid")
+
+
+def rstrct(uc, uf):
+    ncz, ncy, ncx = uc.shape
+    _check(lib().dg_rstrct(_p(uc), _p(uf), _i(2), _i(ncx), _i(ncy), _i(ncz)), "dg_rstrct")
+
+
+def interp(uf, uc):
+    nz, ny, nx = uf.shape
+    _check(lib().dg_interp(_p(uf), _p(uc), _i(2), _i(nx), _i(ny), _i(nz)), "dg_interp")
+
+
+def addint(uf, uc, res):
+    nz, ny, nx = uf.shape
+    _check(lib().dg_addint(_p(uf), _p(uc), _p(res), _i(2), _i(nx), _i(ny), _i(nz)), "dg_addint")
+
+
+def acfromac(a, b, c, ei, a2, b2, c2):
+    nz, ny, nx = a.shape
+    _check(lib().dg_acfromac(_p(a), _p(b), _p(c), _p(ei), _p(a2), _p(b2), _p(c2), _i(nx), _i(ny), _i(nz)),
+           "dg_acfromac")
+
+
+def slvsml(u, rhs, a, b, c, ei, ntop, omega, lengthsize):
+    nz, ny, nx = u.shape
+    _check(lib().dg_slvsml(_p(u), _p(rhs), _p(a), _p(b), _p(c), _p(ei), _i(2), _i(nx), _i(ny), _i(nz), _i(ntop),
+                           _d(omega), _d(lengthsize)), "dg_slvsml")
+
+
+def abcei(a, b, c, ei, cc, r1, r2):
+    nz, ny, nx = a.shape
+    _check(lib().dg_abcei(_p(a), _p(b), _p(c), _p(ei), _p(cc), _d(r1), _d(r2), _i(2), _i(nx), _i(ny), _i(nz)),
+           "dg_abcei")
+
+
+def bc_scalar(s):
+    nz, ny, nx = s.shape
+    _check(lib().dg_bc_scalar(_p(s), _i(2), _i(nx), _i(ny), _i(nz)), "dg_bc_scalar")
+
+
+def filterc27(cc, cf):
+    nz, ny, nx = cc.shape
+    _check(lib().dg_filterc27(_p(cc), _p(cf), _i(nx), _i(ny), _i(nz)), "dg_filterc27")
+
+
+def al3d(b1, b2, b3, cc):
+    b1, b2, b3, cc = (np.ascontiguousarray(x, dtype=np.float64) for x in (b1, b2, b3, cc))
+    out = np.empty_like(b1)
+    _check(lib().dg_al3d_batch(_i(b1.size), _p(b1), _p(b2), _p(b3), _p(cc), _p(out)), "dg_al3d_batch")
+    return out
+
+
+def fl3d(m1, m2, m3, alpha, r0, dr0):
+    m1, m2, m3, alpha, r0, dr0 = (np.ascontiguousarray(x, dtype=np.float64) for x in (m1, m2, m3, alpha, r0, dr0))
+    out = np.empty_like(m1)
+    _check(lib().dg_fl3d_batch(_i(m1.size), _p(m1), _p(m2), _p(m3), _p(alpha), _p(r0), _p(dr0), _p(out)),
+           "dg_fl3d_batch")
+    return out
+
+
+def poisson_solve(rhs, a, b, c, p0=None, lengthsize=1.0, mode=2, tol=1e-8, maxit=200):
+    """Tolerance-mode solve of resid's operator; returns (p, iterations, relres)."""
+    nz, ny, nx = rhs.shape
+    p = np.zeros_like(rhs) if p0 is None else np.array(p0, dtype=np.float64, copy=True)
+    it = ctypes.c_int(0)
+    rel = ctypes.c_double(0.0)
+    _check(lib().dg_poisson_solve(_i(nx), _i(ny), _i(nz), _d(lengthsize), _p(rhs), _p(a), _p(b), _p(c), _p(p),
+                                  _i(mode), _d(tol), _i(maxit), ctypes.byref(it), ctypes.byref(rel)),
+           "dg_poisson_solve")
+    return p, it.value, rel.value

@@ -1,0 +1,718 @@
+// dg_api.hip -- C-ABI of libdroplet_gpu.so: lifecycle, host set-up
+// (readinput / memoryaloc / nondim of the reference), the time-step
+// operators and the host-pointer kernel entry points.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "droplet_gpu.h"
+#include "dg_ctx.h"
+#include "dg_util.h"
+
+namespace dg {
+
+static char g_err[1024] = "";
+static int g_status = 0;
+
+void set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    g_status = -1;
+}
+const char *last_error() { return g_err; }
+int status() { return g_status; }
+
+Ctx *ctx()
+{
+    static Ctx c;
+    return &c;
+}
+
+static bool g_ready = false;
+
+// ---- allocation -----------------------------------------------------------
+
+static int dalloc(double **p, size_t n)
+{
+    DG_HIP(hipMalloc(p, n * sizeof(double)));
+    DG_HIP(hipMemset(*p, 0, n * sizeof(double)));
+    return 0;
+}
+
+// level sizes: memoryaloc (main.f:274-322) halves n -> n/2+1 while the
+// smallest extent exceeds 4.  Extents of the form m*2^k+2 are accepted for
+// the tolerance-mode solver (halving stops when an interior extent turns odd).
+static int plan_levels(int nx, int ny, int nz, bool strict, int dims[MAXLEV][3], int *ng_out)
+{
+    int nn = nx < ny ? nx : ny;
+    if (nz < nn) nn = nz;
+    if (strict) {
+        int j = nn - 2, ng = 1;
+        while (j > 2) { j = j / 2; ng++; }
+        if ((1 << ng) + 2 != nn) {
+            set_error("STOP: nn .ne. 2**ng + 2 (nn=%d)", nn);
+            return -1;
+        }
+    }
+    int lx = nx, ly = ny, lz = nz, cnt = 0;
+    int tmp[MAXLEV][3];
+    for (;;) {
+        if (cnt >= MAXLEV) { set_error("too many multigrid levels"); return -1; }
+        tmp[cnt][0] = lx; tmp[cnt][1] = ly; tmp[cnt][2] = lz;
+        cnt++;
+        int m = lx < ly ? lx : ly;
+        if (lz < m) m = lz;
+        if (m <= 4) break;
+        if (((lx - 2) & 1) || ((ly - 2) & 1) || ((lz - 2) & 1)) break;
+        lx = lx / 2 + 1; ly = ly / 2 + 1; lz = lz / 2 + 1;
+    }
+    for (int l = 0; l < cnt; l++)
+        for (int d = 0; d < 3; d++) dims[l][d] = tmp[cnt - 1 - l][d];  // dims[0] = coarsest
+    *ng_out = cnt;
+    return 0;
+}
+
+static int alloc_levels(Ctx &c, int nx, int ny, int nz, bool strict)
+{
+    int dims[MAXLEV][3], ng;
+    if (plan_levels(nx, ny, nz, strict, dims, &ng)) return -1;
+    c.ng = ng;
+    for (int l = 0; l < ng; l++) {
+        Level &L = c.lev[l];
+        L.g = make_grid(dims[l][0], dims[l][1], dims[l][2]);
+        const size_t n = L.g.n();
+        if (dalloc(&L.rhs, n) || dalloc(&L.res, n) || dalloc(&L.u, n) || dalloc(&L.cc, n) || dalloc(&L.a, n) ||
+            dalloc(&L.b, n) || dalloc(&L.c, n) || dalloc(&L.ei, n))
+            return -1;
+    }
+    c.g = c.lev[ng - 1].g;
+    // reduction partials: RED_BLOCKS pairs or two per interior block
+    const dim3 ig = interior_grid(c.g);
+    size_t np = 2 * (size_t)ig.x * ig.y * ig.z;
+    if (np < 2 * (size_t)RED_BLOCKS) np = 2 * RED_BLOCKS;
+    if (dalloc(&c.red_partial, np)) return -1;
+    if (dalloc(&c.dscal, 64)) return -1;
+    DG_HIP(hipHostMalloc((void **)&c.hscal, 64 * sizeof(double), hipHostMallocDefault));
+    memset(c.hscal, 0, 64 * sizeof(double));
+    if (!c.stream) DG_HIP(hipStreamCreate(&c.stream));
+    return 0;
+}
+
+static int alloc_fields(Ctx &c)
+{
+    const size_t n = c.g.n();
+    double **arrs[] = {&c.u, &c.v, &c.w, &c.s11, &c.s22, &c.s33, &c.s12, &c.s13, &c.s23, &c.rhon,
+                       &c.mun, &c.mut, &c.ktur, &c.eps, &c.cmu, &c.ccf, &c.pprev};
+    for (double **p : arrs)
+        if (dalloc(p, n)) return -1;
+    return 0;
+}
+
+static void free_ctx(Ctx &c)
+{
+    for (int l = 0; l < c.ng; l++) {
+        Level &L = c.lev[l];
+        void *ps[] = {L.rhs, L.res, L.u, L.cc, L.a, L.b, L.c, L.ei, L.a32, L.b32, L.c32,
+                      L.u32, L.f32, L.r32, L.u64, L.f64, L.r64};
+        for (void *p : ps)
+            if (p) (void)hipFree(p);
+        L = Level();
+    }
+    void *fs[] = {c.u, c.v, c.w, c.s11, c.s22, c.s33, c.s12, c.s13, c.s23, c.rhon, c.mun, c.mut, c.ktur,
+                  c.eps, c.cmu, c.ccf, c.pprev, c.pcg_r, c.pcg_z, c.pcg_p, c.pcg_q, c.red_partial, c.dscal};
+    for (void *p : fs)
+        if (p) (void)hipFree(p);
+    if (c.hscal) (void)hipHostFree(c.hscal);
+    if (c.stream) (void)hipStreamDestroy(c.stream);
+    c = Ctx();
+}
+
+// ---- readinput (main.f:185-272): first token of each of the 38 lines ----
+
+struct Input {
+    int nx, ny, nz, tmax, nfilter, cycles;
+    double omega;
+    int npre, ntop, npost, restart;
+    double maxcfl;
+    int iturb, convectype, hswitch;
+    double xvel1, yvel1, zvel1, xvel2, yvel2, zvel2, rho2, rho1, sigma, visc2, visc1, grav, grav1;
+    int icase, ibc;
+    double length, radius, y0, z0;
+    int tprint, ntecplot, tsave, surfacet;
+};
+
+static int read_input(const char *path, Input &in)
+{
+    FILE *f = fopen(path, "r");
+    if (!f) {
+        set_error("cannot open %s", path);
+        return -1;
+    }
+    double v[38];
+    char line[1024];
+    for (int q = 0; q < 38; q++) {
+        if (!fgets(line, sizeof line, f)) {
+            fclose(f);
+            set_error("%s: missing value on line %d", path, q + 1);
+            return -1;
+        }
+        char *p = line;
+        while (*p == ' ' || *p == '\t') p++;
+        char *end = nullptr;
+        v[q] = strtod(p, &end);
+        if (end == p) {
+            fclose(f);
+            set_error("%s: no number on line %d", path, q + 1);
+            return -1;
+        }
+    }
+    fclose(f);
+    int q = 0;
+    in.nx = (int)v[q++]; in.ny = (int)v[q++]; in.nz = (int)v[q++]; in.tmax = (int)v[q++];
+    in.nfilter = (int)v[q++]; in.cycles = (int)v[q++]; in.omega = v[q++];
+    in.npre = (int)v[q++]; in.ntop = (int)v[q++]; in.npost = (int)v[q++]; in.restart = (int)v[q++];
+    in.maxcfl = v[q++]; in.iturb = (int)v[q++]; in.convectype = (int)v[q++]; in.hswitch = (int)v[q++];
+    in.xvel1 = v[q++]; in.yvel1 = v[q++]; in.zvel1 = v[q++];
+    in.xvel2 = v[q++]; in.yvel2 = v[q++]; in.zvel2 = v[q++];
+    in.rho2 = v[q++]; in.rho1 = v[q++]; in.sigma = v[q++]; in.visc2 = v[q++]; in.visc1 = v[q++];
+    in.grav = v[q++]; in.grav1 = v[q++]; in.icase = (int)v[q++]; in.ibc = (int)v[q++];
+    in.length = v[q++]; in.radius = v[q++]; in.y0 = v[q++]; in.z0 = v[q++];
+    in.tprint = (int)v[q++]; in.ntecplot = (int)v[q++]; in.tsave = (int)v[q++]; in.surfacet = (int)v[q++];
+    return 0;
+}
+
+// nondim (main.f:341-446): host arithmetic in the reference's order
+static void nondim(const Input &in, Scalars &s)
+{
+    const double L = in.length;
+    s.lengthscale = L;
+    double vs = sqrt((in.xvel1 - in.xvel2) * (in.xvel1 - in.xvel2) + (in.yvel1 - in.yvel2) * (in.yvel1 - in.yvel2) +
+                     (in.zvel1 - in.zvel2) * (in.zvel1 - in.zvel2));
+    double cdrag = 1.0;
+    const double vel2 = sqrt(in.xvel2 * in.xvel2 + in.yvel2 * in.yvel2 + in.zvel2 * in.zvel2);
+    const double gtot = sqrt(in.grav * in.grav + in.grav1 * in.grav1);
+    if (vs == 0.0 && gtot != 0.0) {
+        vs = sqrt((fabs(gtot) * L) * fabs(in.rho1 - in.rho2) / in.rho2 * (1.0 / cdrag));
+        const double re2 = vel2 * L * in.rho2 / in.visc2;
+        if (re2 <= 10.0 && vs == 0.0) cdrag = 10.0 / re2;
+        vs = sqrt((fabs(gtot) * L) * fabs(in.rho1 - in.rho2) / in.rho2 * (1.0 / cdrag));
+    }
+    if (vs == 0.0) vs = 1.0;
+    s.vscale = vs;
+    s.timescale = L / vs;
+    s.lengthsize = in.length / L;
+    s.h = s.lengthsize / (double)(in.nx - 2);
+    s.time = 0.0;
+    s.tau_si = in.maxcfl * DG_F4(0.8) * (in.length / (double)(in.nx - 2)) / vs;
+    s.tau = s.tau_si / s.timescale;
+    s.tauh = s.tau / s.h;
+    s.tauh2 = (int)(s.tau / (s.h * s.h));
+    s.tswap = 0;
+    s.radius = in.radius / L;
+    s.xvel1 = in.xvel1 / vs; s.yvel1 = in.yvel1 / vs; s.zvel1 = in.zvel1 / vs;
+    s.xvel2 = in.xvel2 / vs; s.yvel2 = in.yvel2 / vs; s.zvel2 = in.zvel2 / vs;
+    s.y0 = (double)(in.ny - 2) * s.h / 2.0;
+    s.z0 = (double)(in.nz - 2) * s.h / 2.0;
+    s.rho1 = 1.0;
+    s.rho2 = in.rho2 / in.rho1;
+    double denom = in.rho1 * (L * L) / s.timescale;
+    s.mu1 = in.visc1 / denom;
+    s.mu2 = in.visc2 / denom;
+    denom = in.rho1 * (L * L * L) / (s.timescale * s.timescale);
+    s.sigma = in.sigma / denom;
+    denom = L / (s.timescale * s.timescale);
+    s.g = in.grav / denom;
+    s.g1 = in.grav1 / denom;
+    // smooth2's droplet centre (main.f:5687-5690)
+    if (in.z0 != 1000.0) s.z0 = in.z0 / in.length;
+    if (in.y0 != 1000.0) s.y0 = in.y0 / in.length;
+    s.y0 = s.y0 + s.h / 2.0;
+    s.maxcfl = in.maxcfl;
+    s.omega = in.omega;
+    s.cycles = in.cycles; s.npre = in.npre; s.ntop = in.ntop; s.npost = in.npost;
+    s.iturb = in.iturb; s.convectype = in.convectype; s.surfacet = in.surfacet; s.hswitch = in.hswitch;
+    s.tprint = in.tprint;
+}
+
+static double *field_ptr(Ctx &c, int id)
+{
+    Level &L = c.lev[c.ng - 1];
+    switch (id) {
+    case DG_F_CC: return L.cc;
+    case DG_F_U: return c.u;
+    case DG_F_V: return c.v;
+    case DG_F_W: return c.w;
+    case DG_F_P: return L.u;
+    case DG_F_S11: return c.s11;
+    case DG_F_S22: return c.s22;
+    case DG_F_S33: return c.s33;
+    case DG_F_S12: return c.s12;
+    case DG_F_S13: return c.s13;
+    case DG_F_S23: return c.s23;
+    case DG_F_RHON: return c.rhon;
+    case DG_F_MUN: return c.mun;
+    case DG_F_MUT: return c.mut;
+    case DG_F_KTUR: return c.ktur;
+    case DG_F_EPS: return c.eps;
+    case DG_F_A: return L.a;
+    case DG_F_B: return L.b;
+    case DG_F_C: return L.c;
+    case DG_F_EI: return L.ei;
+    case DG_F_RHS: return L.rhs;
+    case DG_F_RES: return L.res;
+    case DG_F_PPREV: return c.pprev;
+    default: return nullptr;
+    }
+}
+
+static int require_ready()
+{
+    if (!g_ready) {
+        set_error("dg_init has not been called");
+        return -1;
+    }
+    return 0;
+}
+
+// ---- scoped device buffers for the host-pointer entry points --------------
+struct DevBufs {
+    std::vector<double *> ptrs;
+    ~DevBufs()
+    {
+        for (double *p : ptrs) (void)hipFree(p);
+    }
+    double *up(const double *h, size_t n)
+    {
+        double *d = nullptr;
+        if (hipMalloc(&d, n * sizeof(double)) != hipSuccess) return nullptr;
+        ptrs.push_back(d);
+        if (h && hipMemcpy(d, h, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+        if (!h && hipMemset(d, 0, n * sizeof(double)) != hipSuccess) return nullptr;
+        return d;
+    }
+};
+
+static int down(double *h, const double *d, size_t n)
+{
+    DG_HIP(hipMemcpy(h, d, n * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+static int check_dims(int nx, int ny, int nz)
+{
+    if (nx < 3 || ny < 3 || nz < 3 || (long long)nx * ny * nz > (1LL << 31) - 1) {
+        set_error("invalid extents %d x %d x %d", nx, ny, nz);
+        return -1;
+    }
+    return 0;
+}
+
+static hipStream_t scratch_stream() { return nullptr; }
+
+}  // namespace dg
+
+using namespace dg;
+
+extern "C" {
+
+int dg_status(void) { return g_status; }
+const char *dg_last_error(void) { return g_err; }
+
+int dg_init(const char *path)
+{
+    Ctx &c = *ctx();
+    if (g_ready) {
+        free_ctx(c);
+        g_ready = false;
+    }
+    g_status = 0;
+    g_err[0] = 0;
+    Input in;
+    if (read_input(path, in)) return -1;
+    if (in.icase != 2) {
+        set_error("icase=%d: only the droplet case (icase=2, smooth2) is on this path", in.icase);
+        return -1;
+    }
+    if (in.restart != 0) {
+        set_error("restart=%d: restart files (iniget) are not on this path", in.restart);
+        return -1;
+    }
+    if (in.hswitch != 0) {
+        set_error("hswitch=%d: interface heights (CalcH3d) are not on this path", in.hswitch);
+        return -1;
+    }
+    if (in.surfacet != 2) {
+        set_error("surfacet=%d: only CSF (surfacet=2) is on this path", in.surfacet);
+        return -1;
+    }
+    if (in.iturb != 0 && in.iturb != 3) {
+        set_error("iturb=%d: only laminar (0) and k-epsilon (3) are on this path", in.iturb);
+        return -1;
+    }
+    if (check_dims(in.nx, in.ny, in.nz)) return -1;
+    if (alloc_levels(c, in.nx, in.ny, in.nz, true)) return -1;
+    if (alloc_fields(c)) return -1;
+    nondim(in, c.s);
+    // cmu = 0.09 (main.f:347-353); mut, turbst and the Reynolds stresses start at 0
+    if (mg_fill(c.g.n(), c.stream, c.cmu, 0.09)) return -1;
+    g_ready = true;
+    if (op_smooth2(c) || op_bc_c(c, c.lev[c.ng - 1].cc) || op_bc_v(c) || op_calcrho(c)) return -1;
+    DG_HIP(hipStreamSynchronize(c.stream));
+    return 0;
+}
+
+int dg_finalize(void)
+{
+    Ctx &c = *ctx();
+    if (g_ready) free_ctx(c);
+    g_ready = false;
+    return 0;
+}
+
+int dg_vofmethod(const int *t)
+{
+    if (require_ready()) return -1;
+    return op_vofmethod(*ctx(), *t);
+}
+int dg_kepsmodel(const int *t)
+{
+    if (require_ready()) return -1;
+    return op_kepsmodel(*ctx(), *t);
+}
+int dg_checktime(const int *t)
+{
+    if (require_ready()) return -1;
+    return op_checktime(*ctx(), *t);
+}
+int dg_csf(void)
+{
+    if (require_ready()) return -1;
+    return op_csf(*ctx());
+}
+int dg_solvelpres(const int *t)
+{
+    if (require_ready()) return -1;
+    return op_solvelpres(*ctx(), *t);
+}
+int dg_calcrho(void)
+{
+    if (require_ready()) return -1;
+    return op_calcrho(*ctx());
+}
+int dg_mglin(void)
+{
+    if (require_ready()) return -1;
+    return op_mglin(*ctx());
+}
+
+int dg_step(const int *t)
+{
+    if (require_ready()) return -1;
+    Ctx &c = *ctx();
+    if (op_vofmethod(c, *t)) return -1;
+    if (c.s.iturb == 3 && op_kepsmodel(c, *t)) return -1;
+    if (op_checktime(c, *t)) return -1;
+    if (c.s.surfacet == 2 && op_csf(c)) return -1;
+    if (op_solvelpres(c, *t)) return -1;
+    return 0;
+}
+
+int dg_run(const int *t0, const int *nsteps)
+{
+    for (int t = *t0; t < *t0 + *nsteps; t++)
+        if (dg_step(&t)) return -1;
+    Ctx &c = *ctx();
+    DG_HIP(hipStreamSynchronize(c.stream));
+    return 0;
+}
+
+int dg_get_field(const int *id, double *host)
+{
+    if (require_ready()) return -1;
+    Ctx &c = *ctx();
+    double *d = field_ptr(c, *id);
+    if (!d) { set_error("unknown field id %d", *id); return -1; }
+    DG_HIP(hipStreamSynchronize(c.stream));
+    DG_HIP(hipMemcpy(host, d, c.g.n() * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int dg_set_field(const int *id, const double *host)
+{
+    if (require_ready()) return -1;
+    Ctx &c = *ctx();
+    double *d = field_ptr(c, *id);
+    if (!d) { set_error("unknown field id %d", *id); return -1; }
+    DG_HIP(hipStreamSynchronize(c.stream));
+    DG_HIP(hipMemcpy(d, host, c.g.n() * sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int dg_get_state(dg_state *out)
+{
+    if (require_ready()) return -1;
+    Ctx &c = *ctx();
+    DG_HIP(hipStreamSynchronize(c.stream));
+    out->tau = c.s.tau; out->time = c.s.time; out->tauh = c.s.tauh;
+    out->tauh2 = c.s.tauh2; out->tswap = c.s.tswap;
+    out->npre = c.s.npre; out->ntop = c.s.ntop; out->npost = c.s.npost;
+    out->nx = c.g.nx; out->ny = c.g.ny; out->nz = c.g.nz; out->ng = c.ng;
+    out->mg_iters = c.mg_iters_last; out->mg_relres = c.mg_relres_last;
+    out->cfl = c.cfl_last; out->epsdiv = c.epsdiv_last;
+    return 0;
+}
+
+int dg_set_state(const dg_state *in)
+{
+    if (require_ready()) return -1;
+    Ctx &c = *ctx();
+    c.s.tau = in->tau; c.s.time = in->time; c.s.tauh = in->tauh; c.s.tauh2 = in->tauh2; c.s.tswap = in->tswap;
+    c.s.npre = in->npre; c.s.ntop = in->ntop; c.s.npost = in->npost;
+    return 0;
+}
+
+int dg_clip_cc(void)
+{
+    if (require_ready()) return -1;
+    Ctx &c = *ctx();
+    double *cc = c.lev[c.ng - 1].cc;
+    return launch_linear(c.g.n(), c.stream, [=] __device__(size_t p) {
+        double v = cc[p];
+        cc[p] = v > 1.0 ? 1.0 : (v < 0.0 ? 0.0 : v);
+    });
+}
+
+int dg_set_solver(const int *mode, const double *tol, const int *nu1, const int *nu2, const int *maxit,
+                  const int *guess)
+{
+    if (require_ready()) return -1;
+    Ctx &c = *ctx();
+    if (*mode < 0 || *mode > 2 || *nu1 < 0 || *nu2 < 0 || *maxit < 1 || !(*tol > 0.0)) {
+        set_error("dg_set_solver: invalid arguments");
+        return -1;
+    }
+    c.mg_mode = *mode; c.mg_tol = *tol; c.mg_nu1 = *nu1; c.mg_nu2 = *nu2; c.mg_maxit = *maxit; c.mg_guess = *guess;
+    return 0;
+}
+
+int dg_set_ordered_sums(const int *on)
+{
+    if (require_ready()) return -1;
+    ctx()->ordered_sums = *on ? 1 : 0;
+    return 0;
+}
+
+// ---- host-pointer kernels ----------------------------------------------------
+
+int dg_relax(double *u, const double *lengthsize, const double *rhs, const double *a, const double *b,
+             const double *c, const double *ei, const int *ibc, const int *nx, const int *ny, const int *nz,
+             const double *omega)
+{
+    (void)ibc;
+    if (check_dims(*nx, *ny, *nz)) return -1;
+    const Grid g = make_grid(*nx, *ny, *nz);
+    const size_t n = g.n();
+    DevBufs B;
+    double *du = B.up(u, n), *dr = B.up(rhs, n), *da = B.up(a, n), *db = B.up(b, n), *dc = B.up(c, n),
+           *de = B.up(ei, n);
+    if (!du || !dr || !da || !db || !dc || !de) { set_error("dg_relax: device allocation failed"); return -1; }
+    if (mg_relax(g, scratch_stream(), du, *lengthsize, dr, da, db, dc, de, *omega)) return -1;
+    return down(u, du, n);
+}
+
+int dg_resid(const double *lengthsize, double *res, const double *u, const double *rhs, const double *a,
+             const double *b, const double *c, const int *ibc, const int *nx, const int *ny, const int *nz)
+{
+    (void)ibc;
+    if (check_dims(*nx, *ny, *nz)) return -1;
+    const Grid g = make_grid(*nx, *ny, *nz);
+    const size_t n = g.n();
+    DevBufs B;
+    double *dres = B.up(res, n), *du = B.up(u, n), *dr = B.up(rhs, n), *da = B.up(a, n), *db = B.up(b, n),
+           *dc = B.up(c, n);
+    if (!dres || !du || !dr || !da || !db || !dc) { set_error("dg_resid: device allocation failed"); return -1; }
+    if (mg_resid(g, scratch_stream(), *lengthsize, dres, du, dr, da, db, dc)) return -1;
+    return down(res, dres, n);
+}
+
+int dg_rstrct(double *uc, const double *uf, const int *ibc, const int *ncx, const int *ncy, const int *ncz)
+{
+    (void)ibc;
+    if (check_dims(*ncx, *ncy, *ncz)) return -1;
+    const Grid gc = make_grid(*ncx, *ncy, *ncz);
+    const Grid gf = make_grid(2 * *ncx - 2, 2 * *ncy - 2, 2 * *ncz - 2);
+    DevBufs B;
+    double *duc = B.up(uc, gc.n()), *duf = B.up(uf, gf.n());
+    if (!duc || !duf) { set_error("dg_rstrct: device allocation failed"); return -1; }
+    if (mg_rstrct(gc, scratch_stream(), duc, duf)) return -1;
+    return down(uc, duc, gc.n());
+}
+
+int dg_interp(double *uf, const double *uc, const int *ibc, const int *nfx, const int *nfy, const int *nfz)
+{
+    (void)ibc;
+    if (check_dims(*nfx, *nfy, *nfz)) return -1;
+    const Grid gf = make_grid(*nfx, *nfy, *nfz);
+    const Grid gc = make_grid(*nfx / 2 + 1, *nfy / 2 + 1, *nfz / 2 + 1);
+    DevBufs B;
+    double *duf = B.up(uf, gf.n()), *duc = B.up(uc, gc.n());
+    if (!duf || !duc) { set_error("dg_interp: device allocation failed"); return -1; }
+    if (mg_interp(gf, scratch_stream(), duf, duc)) return -1;
+    return down(uf, duf, gf.n());
+}
+
+int dg_addint(double *uf, const double *uc, double *res, const int *ibc, const int *nfx, const int *nfy,
+              const int *nfz)
+{
+    (void)ibc;
+    if (check_dims(*nfx, *nfy, *nfz)) return -1;
+    const Grid gf = make_grid(*nfx, *nfy, *nfz);
+    const Grid gc = make_grid(*nfx / 2 + 1, *nfy / 2 + 1, *nfz / 2 + 1);
+    DevBufs B;
+    double *duf = B.up(uf, gf.n()), *duc = B.up(uc, gc.n()), *dres = B.up(res, gf.n());
+    if (!duf || !duc || !dres) { set_error("dg_addint: device allocation failed"); return -1; }
+    if (mg_addint(gf, scratch_stream(), duf, duc, dres)) return -1;
+    if (down(uf, duf, gf.n())) return -1;
+    return down(res, dres, gf.n());
+}
+
+int dg_acfromac(double *a, double *b, double *c, double *ei, const double *a2, const double *b2, const double *c2,
+                const int *nx, const int *ny, const int *nz)
+{
+    if (check_dims(*nx, *ny, *nz)) return -1;
+    const Grid gc = make_grid(*nx, *ny, *nz);
+    const Grid gf = make_grid(2 * *nx - 2, 2 * *ny - 2, 2 * *nz - 2);
+    DevBufs B;
+    double *da = B.up(a, gc.n()), *db = B.up(b, gc.n()), *dc = B.up(c, gc.n()), *de = B.up(ei, gc.n());
+    double *da2 = B.up(a2, gf.n()), *db2 = B.up(b2, gf.n()), *dc2 = B.up(c2, gf.n());
+    if (!da || !db || !dc || !de || !da2 || !db2 || !dc2) { set_error("dg_acfromac: device allocation failed"); return -1; }
+    if (mg_acfromac(gc, scratch_stream(), da, db, dc, de, da2, db2, dc2)) return -1;
+    if (down(a, da, gc.n()) || down(b, db, gc.n()) || down(c, dc, gc.n())) return -1;
+    return down(ei, de, gc.n());
+}
+
+int dg_slvsml(double *u, const double *rhs, const double *a, const double *b, const double *c, const double *ei,
+              const int *ibc, const int *nx, const int *ny, const int *nz, const int *ntop, const double *omega,
+              const double *lengthsize)
+{
+    (void)ibc;
+    if (check_dims(*nx, *ny, *nz)) return -1;
+    const Grid g = make_grid(*nx, *ny, *nz);
+    const size_t n = g.n();
+    DevBufs B;
+    double *du = B.up(u, n), *dr = B.up(rhs, n), *da = B.up(a, n), *db = B.up(b, n), *dc = B.up(c, n),
+           *de = B.up(ei, n);
+    if (!du || !dr || !da || !db || !dc || !de) { set_error("dg_slvsml: device allocation failed"); return -1; }
+    for (int q = 1; q <= *ntop; q++)
+        if (mg_relax(g, scratch_stream(), du, *lengthsize, dr, da, db, dc, de, *omega)) return -1;
+    return down(u, du, n);
+}
+
+int dg_abcei(double *a, double *b, double *c, double *ei, const double *cc, const double *r1, const double *r2,
+             const int *ibc, const int *nx, const int *ny, const int *nz)
+{
+    (void)ibc;
+    if (check_dims(*nx, *ny, *nz)) return -1;
+    const Grid g = make_grid(*nx, *ny, *nz);
+    const size_t n = g.n();
+    DevBufs B;
+    double *da = B.up(a, n), *db = B.up(b, n), *dc = B.up(c, n), *de = B.up(ei, n), *dcc = B.up(cc, n);
+    if (!da || !db || !dc || !de || !dcc) { set_error("dg_abcei: device allocation failed"); return -1; }
+    if (mg_abcei(g, scratch_stream(), da, db, dc, de, dcc, *r1, *r2)) return -1;
+    if (down(a, da, n) || down(b, db, n) || down(c, dc, n)) return -1;
+    return down(ei, de, n);
+}
+
+int dg_bc_scalar(double *scal, const int *ibc, const int *nx, const int *ny, const int *nz)
+{
+    (void)ibc;
+    if (check_dims(*nx, *ny, *nz)) return -1;
+    const Grid g = make_grid(*nx, *ny, *nz);
+    DevBufs B;
+    double *d = B.up(scal, g.n());
+    if (!d) { set_error("dg_bc_scalar: device allocation failed"); return -1; }
+    if (op_bc_scalar(g, scratch_stream(), d)) return -1;
+    return down(scal, d, g.n());
+}
+
+int dg_filterc27(const double *cc, double *cfilt, const int *nx, const int *ny, const int *nz)
+{
+    if (check_dims(*nx, *ny, *nz)) return -1;
+    const Grid g = make_grid(*nx, *ny, *nz);
+    DevBufs B;
+    double *dcc = B.up(cc, g.n()), *dcf = B.up(cfilt, g.n());
+    if (!dcc || !dcf) { set_error("dg_filterc27: device allocation failed"); return -1; }
+    if (mg_filterc27(g, scratch_stream(), dcc, dcf)) return -1;
+    return down(cfilt, dcf, g.n());
+}
+
+int dg_al3d_batch(const int *n, const double *b1, const double *b2, const double *b3, const double *cc, double *out)
+{
+    if (*n <= 0) return 0;
+    DevBufs B;
+    double *d1 = B.up(b1, *n), *d2 = B.up(b2, *n), *d3 = B.up(b3, *n), *dc = B.up(cc, *n), *dout = B.up(nullptr, *n);
+    if (!d1 || !d2 || !d3 || !dc || !dout) { set_error("dg_al3d_batch: device allocation failed"); return -1; }
+    if (plic_al3d_batch(*n, scratch_stream(), d1, d2, d3, dc, dout)) return -1;
+    return down(out, dout, *n);
+}
+
+int dg_fl3d_batch(const int *n, const double *m1, const double *m2, const double *m3, const double *alpha,
+                  const double *r0, const double *dr0, double *out)
+{
+    if (*n <= 0) return 0;
+    DevBufs B;
+    double *d1 = B.up(m1, *n), *d2 = B.up(m2, *n), *d3 = B.up(m3, *n), *da = B.up(alpha, *n), *dr = B.up(r0, *n),
+           *dd = B.up(dr0, *n), *dout = B.up(nullptr, *n);
+    if (!d1 || !d2 || !d3 || !da || !dr || !dd || !dout) { set_error("dg_fl3d_batch: device allocation failed"); return -1; }
+    if (plic_fl3d_batch(*n, scratch_stream(), d1, d2, d3, da, dr, dd, dout)) return -1;
+    return down(out, dout, *n);
+}
+
+int dg_poisson_solve(const int *nx, const int *ny, const int *nz, const double *lengthsize, const double *rhs,
+                     const double *a, const double *b, const double *c, double *p, const int *mode, const double *tol,
+                     const int *maxit, int *iters, double *relres)
+{
+    if (check_dims(*nx, *ny, *nz)) return -1;
+    if (*mode < 1 || *mode > 2) {
+        set_error("dg_poisson_solve: mode must be 1 (FP64) or 2 (FP32 V-cycle)");
+        return -1;
+    }
+    Ctx lc;
+    if (alloc_levels(lc, *nx, *ny, *nz, false)) { free_ctx(lc); return -1; }
+    lc.pprev = nullptr;
+    const size_t n = lc.g.n();
+    if (dalloc(&lc.pprev, n)) { free_ctx(lc); return -1; }
+    Level &F = lc.lev[lc.ng - 1];
+    int rc = 0;
+    rc |= hipMemcpy(F.rhs, rhs, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess;
+    rc |= hipMemcpy(F.a, a, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess;
+    rc |= hipMemcpy(F.b, b, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess;
+    rc |= hipMemcpy(F.c, c, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess;
+    rc |= hipMemcpy(lc.pprev, p, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess;
+    if (rc) { set_error("dg_poisson_solve: upload failed"); free_ctx(lc); return -1; }
+    lc.s.lengthsize = *lengthsize;
+    lc.mg_mode = *mode; lc.mg_tol = *tol; lc.mg_maxit = *maxit; lc.mg_guess = 1;
+    if (op_mg_solve_tol(lc)) { free_ctx(lc); return -1; }
+    if (hipStreamSynchronize(lc.stream) != hipSuccess ||
+        hipMemcpy(p, F.u, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+        set_error("dg_poisson_solve: download failed");
+        free_ctx(lc);
+        return -1;
+    }
+    *iters = lc.mg_iters_last;
+    *relres = lc.mg_relres_last;
+    free_ctx(lc);
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,106 @@
+// dg_ctx.h -- device-resident solver state and the operator entry points
+// shared between the kernel files and the C-ABI layer.
+#pragma once
+
+#include "dg_internal.h"
+
+namespace dg {
+
+constexpr int MAXLEV = 16;
+
+// One multigrid level.  The eight FP64 arrays mirror the reference's heap
+// slots (memoryaloc, main.f:305-318), including their reuse as scratch by
+// VOF (vof1/2/3 = ei/res/rhs, main.f:806-816), csf (w11/w22/w33 =
+// u/res/rhs, main.f:52) and solvelpres (w1/w2/w3 = u/res/rhs, main.f:68).
+struct Level {
+    Grid g;
+    double *rhs = nullptr, *res = nullptr, *u = nullptr, *cc = nullptr;
+    double *a = nullptr, *b = nullptr, *c = nullptr, *ei = nullptr;
+    // tolerance-mode work arrays
+    float *a32 = nullptr, *b32 = nullptr, *c32 = nullptr;
+    float *u32 = nullptr, *f32 = nullptr, *r32 = nullptr;
+    double *u64 = nullptr, *f64 = nullptr, *r64 = nullptr;
+};
+
+struct Scalars {
+    // /PHYSICS/, /DIFF/, /DINEM/ values used on the path
+    double tau, time, h, tauh;
+    int tauh2;  // INTEGER in input.h: tau/(h*h) truncated
+    int tswap;
+    double rho1, rho2, mu1, mu2, sigma, g, g1;
+    double xvel1, yvel1, zvel1, xvel2, yvel2, zvel2, radius, y0, z0;
+    double lengthscale, lengthsize, timescale, vscale, tau_si;
+    double maxcfl, omega;
+    int cycles, npre, ntop, npost, iturb, convectype, surfacet, hswitch, tprint;
+};
+
+struct Ctx {
+    Grid g{};
+    int ng = 0;
+    Level lev[MAXLEV];
+    double *u = nullptr, *v = nullptr, *w = nullptr;
+    double *s11 = nullptr, *s22 = nullptr, *s33 = nullptr, *s12 = nullptr, *s13 = nullptr, *s23 = nullptr;
+    double *rhon = nullptr, *mun = nullptr, *mut = nullptr, *ktur = nullptr, *eps = nullptr, *cmu = nullptr;
+    double *ccf = nullptr;     // csf's filtered volume fraction (automatic array in csf)
+    double *pprev = nullptr;   // pressure of the previous step (tolerance-mode guess)
+    double *pcg_r = nullptr, *pcg_z = nullptr, *pcg_p = nullptr, *pcg_q = nullptr;
+    double *red_partial = nullptr;  // RED_BLOCKS doubles
+    double *dscal = nullptr;        // small device scalar scratch (64 doubles)
+    double *hscal = nullptr;        // pinned host mirror of dscal
+    Scalars s{};
+    // pressure-solver options
+    int mg_mode = 0;       // 0 reference FMG schedule, 1 FP64 PCG, 2 FP32-preconditioned FP64 PCG
+    int mg_nu1 = 2, mg_nu2 = 2, mg_maxit = 200, mg_guess = 0;
+    double mg_tol = 1e-8;
+    int ordered_sums = 1;  // reproduce the reference's sequential outflow sums
+    // diagnostics of the last step
+    int mg_iters_last = 0;
+    double mg_relres_last = 0.0, cfl_last = 0.0, epsdiv_last = 0.0;
+    hipStream_t stream = nullptr;
+};
+
+Ctx *ctx();  // the process-wide context (the reference keeps one COMMON state)
+
+// ---- operators (return 0 on success, -1 on error with set_error) ----
+int op_bc_c(Ctx &c, double *cc);
+int op_bc_v(Ctx &c);
+int op_bc_w(Ctx &c, double *w1, double *w2, double *w3);
+int op_bc_tensor(Ctx &c);
+int op_bc_scalar(const Grid &g, hipStream_t s, double *a);
+int op_bc_out(const Grid &g, hipStream_t s, double *a);
+int op_bc_resid(const Grid &g, hipStream_t s, double *a);
+int op_calcrho(Ctx &c);
+int op_vofmethod(Ctx &c, int t);
+int op_kepsmodel(Ctx &c, int t);
+int op_cfl(Ctx &c, double *out_host);
+int op_checktime(Ctx &c, int t);
+int op_csf(Ctx &c);
+int op_solvelpres(Ctx &c, int t);
+int op_smooth2(Ctx &c);
+
+// multigrid, reference schedule (main.f:3645-3997)
+int mg_relax(const Grid &g, hipStream_t s, double *u, double lengthsize, const double *rhs, const double *a,
+             const double *b, const double *c, const double *ei, double omega);
+int mg_resid(const Grid &g, hipStream_t s, double lengthsize, double *res, const double *u, const double *rhs,
+             const double *a, const double *b, const double *c);
+int mg_rstrct(const Grid &gc, hipStream_t s, double *uc, const double *uf);
+int mg_interp(const Grid &gf, hipStream_t s, double *uf, const double *uc);
+int mg_addint(const Grid &gf, hipStream_t s, double *uf, const double *uc, double *res);
+int mg_acfromac(const Grid &gc, hipStream_t s, double *a, double *b, double *c, double *ei, const double *a2,
+                const double *b2, const double *c2);
+int mg_abcei(const Grid &g, hipStream_t s, double *a, double *b, double *c, double *ei, const double *cc, double r1,
+             double r2);
+int mg_filterc27(const Grid &g, hipStream_t s, const double *cc, double *cf);
+int mg_fill(size_t n, hipStream_t s, double *a, double v);
+int op_mglin(Ctx &c);
+
+// tolerance-mode pressure solve of the finest-level system (no reference counterpart)
+int op_mg_solve_tol(Ctx &c);
+
+// PLIC functions on batches (device)
+int plic_al3d_batch(int n, hipStream_t s, const double *b1, const double *b2, const double *b3, const double *cc,
+                    double *out);
+int plic_fl3d_batch(int n, hipStream_t s, const double *m1, const double *m2, const double *m3, const double *al,
+                    const double *r0, const double *dr0, double *out);
+
+}  // namespace dg

@@ -1,0 +1,769 @@
+// dg_mg.hip -- pressure Poisson solvers.
+//
+// 1. The reference's full-multigrid schedule (mglin, main.f:3645-3763) with
+//    its kernels relax/resid/rstrct/interp/addint/acfromac/slvsml, FP64,
+//    same arithmetic order (parity mode).
+// 2. The tolerance mode (no reference counterpart): preconditioned
+//    conjugate gradients on the reference's finest-level operator, stopped
+//    at ||b - A p||_2 / ||b||_2 < tol (FP64), preconditioned by one
+//    symmetric V-cycle (red-black Gauss-Seidel, 8-child average restriction,
+//    injection prolongation, acfromac coarse operators, FP64 Cholesky on the
+//    coarsest level).  mg_mode 2 runs the V-cycle in FP32.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "dg_ctx.h"
+#include "dg_util.h"
+
+namespace dg {
+
+// ---------------------------------------------------------------------------
+// reference kernels
+
+// one red-black colour pass of relax (main.f:3966-3991); colour 0 updates
+// cells with i+j+k even (the first pass, ksw=1)
+__global__ void __launch_bounds__(256) k_relax_colour(Grid g, double *u, const double *rhs, const double *a,
+                                                      const double *b, const double *c, const double *ei, double h2,
+                                                      double omega, int colour)
+{
+    const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + 2;
+    const int k = (int)blockIdx.z + 2;
+    const int i = 2 * (int)(blockIdx.x * blockDim.x + threadIdx.x) + 2 + ((colour + j + k) & 1);
+    if (i > g.nx - 1 || j > g.ny - 1) return;
+    const int o = g.at(i, j, k), sy = g.sy, sz = g.sz;
+    double temp = (a[o] * u[o + 1] + a[o - 1] * u[o - 1] + b[o] * u[o + sy] + b[o - sy] * u[o - sy] + c[o] * u[o + sz] +
+                   c[o - sz] * u[o - sz] - h2 * rhs[o]) *
+                  ei[o];
+    u[o] = omega * temp + (1.0 - omega) * u[o];
+}
+
+int mg_relax(const Grid &g, hipStream_t s, double *u, double lengthsize, const double *rhs, const double *a,
+             const double *b, const double *c, const double *ei, double omega)
+{
+    const double h1 = lengthsize / (double)(g.nx - 2);
+    const double h2 = h1 * h1;
+    const int half = (g.nx - 2 + 1) / 2;
+    dim3 grid((unsigned)((half + BX - 1) / BX), (unsigned)((g.ny - 2 + BY - 1) / BY), (unsigned)(g.nz - 2));
+    for (int pass = 0; pass < 2; pass++) {
+        if (op_bc_scalar(g, s, u)) return -1;
+        hipLaunchKernelGGL(k_relax_colour, grid, interior_block(), 0, s, g, u, rhs, a, b, c, ei, h2, omega, pass);
+        DG_CHECK_LAUNCH();
+    }
+    return op_bc_scalar(g, s, u);
+}
+
+int mg_resid(const Grid &g, hipStream_t s, double lengthsize, double *res, const double *u, const double *rhs,
+             const double *a, const double *b, const double *c)
+{
+    const double h1 = lengthsize / (double)(g.nx - 2);
+    const double h2i = 1.0 / (h1 * h1);
+    const int sy = g.sy, sz = g.sz;
+    if (launch_interior(g, s, [=] __device__(int i, int j, int k) {
+            const int o = g.at(i, j, k);
+            const double u0 = u[o];
+            res[o] = -h2i * (a[o] * (u[o + 1] - u0) + a[o - 1] * (u[o - 1] - u0) + b[o] * (u[o + sy] - u0) +
+                             b[o - sy] * (u[o - sy] - u0) + c[o] * (u[o + sz] - u0) + c[o - sz] * (u[o - sz] - u0)) +
+                     rhs[o];
+        }))
+        return -1;
+    return op_bc_resid(g, s, res);
+}
+
+int mg_rstrct(const Grid &gc, hipStream_t s, double *uc, const double *uf)
+{
+    const Grid gf = make_grid(2 * gc.nx - 2, 2 * gc.ny - 2, 2 * gc.nz - 2);
+    if (launch_interior(gc, s, [=] __device__(int ic, int jc, int kc) {
+            const int f = gf.at(2 * ic - 2, 2 * jc - 2, 2 * kc - 2), sy = gf.sy, sz = gf.sz;
+            const double t = uf[f] + uf[f + 1] + uf[f + sy] + uf[f + 1 + sy] + uf[f + sz] + uf[f + 1 + sz] +
+                             uf[f + sy + sz] + uf[f + 1 + sy + sz];
+            uc[gc.at(ic, jc, kc)] = .125 * t;
+        }))
+        return -1;
+    return op_bc_scalar(gc, s, uc);
+}
+
+int mg_interp(const Grid &gf, hipStream_t s, double *uf, const double *uc)
+{
+    const Grid gc = make_grid(gf.nx / 2 + 1, gf.ny / 2 + 1, gf.nz / 2 + 1);
+    if (launch_interior(gc, s, [=] __device__(int ic, int jc, int kc) {
+            const double v = uc[gc.at(ic, jc, kc)];
+            const int f = gf.at(2 * ic - 2, 2 * jc - 2, 2 * kc - 2), sy = gf.sy, sz = gf.sz;
+            uf[f] = v; uf[f + sy] = v; uf[f + 1] = v; uf[f + 1 + sy] = v;
+            uf[f + sz] = v; uf[f + sy + sz] = v; uf[f + 1 + sz] = v; uf[f + 1 + sy + sz] = v;
+        }))
+        return -1;
+    return op_bc_scalar(gf, s, uf);
+}
+
+int mg_addint(const Grid &gf, hipStream_t s, double *uf, const double *uc, double *res)
+{
+    if (mg_interp(gf, s, res, uc)) return -1;
+    return launch_linear(gf.n(), s, [=] __device__(size_t p) { uf[p] = uf[p] + res[p]; });
+}
+
+int mg_acfromac(const Grid &gc, hipStream_t s, double *a, double *b, double *c, double *ei, const double *a2,
+                const double *b2, const double *c2)
+{
+    const Grid gf = make_grid(2 * gc.nx - 2, 2 * gc.ny - 2, 2 * gc.nz - 2);
+    const int nx = gc.nx, ny = gc.ny, nz = gc.nz;
+    // a: i = 1..nx-1, b: j = 1..ny-1, c: k = 1..nz-1 (interior in the others)
+    const Grid ga = make_grid(nx + 1, ny, nz);  // index range i = 1..nx-1 through i+... handled below
+    (void)ga;
+    if (launch_face((nx - 1) * (ny - 2), nz - 2, s, [=] __device__(int p, int k0) {
+            int i = p % (nx - 1) + 1, j = p / (nx - 1) + 2, k = k0 + 2;
+            a[gc.at(i, j, k)] = 0.25 * (a2[gf.at(2 * i - 1, 2 * j - 1, 2 * k - 1)] + a2[gf.at(2 * i - 1, 2 * j - 2, 2 * k - 1)] +
+                                        a2[gf.at(2 * i - 1, 2 * j - 1, 2 * k - 2)] + a2[gf.at(2 * i - 1, 2 * j - 2, 2 * k - 2)]);
+        }))
+        return -1;
+    if (launch_face((nx - 2) * (ny - 1), nz - 2, s, [=] __device__(int p, int k0) {
+            int i = p % (nx - 2) + 2, j = p / (nx - 2) + 1, k = k0 + 2;
+            b[gc.at(i, j, k)] = 0.25 * (b2[gf.at(2 * i - 1, 2 * j - 1, 2 * k - 1)] + b2[gf.at(2 * i - 2, 2 * j - 1, 2 * k - 1)] +
+                                        b2[gf.at(2 * i - 1, 2 * j - 1, 2 * k - 2)] + b2[gf.at(2 * i - 2, 2 * j - 1, 2 * k - 2)]);
+        }))
+        return -1;
+    if (launch_face((nx - 2) * (ny - 2), nz - 1, s, [=] __device__(int p, int k0) {
+            int i = p % (nx - 2) + 2, j = p / (nx - 2) + 2, k = k0 + 1;
+            c[gc.at(i, j, k)] = 0.25 * (c2[gf.at(2 * i - 1, 2 * j - 1, 2 * k - 1)] + c2[gf.at(2 * i - 2, 2 * j - 1, 2 * k - 1)] +
+                                        c2[gf.at(2 * i - 1, 2 * j - 2, 2 * k - 1)] + c2[gf.at(2 * i - 2, 2 * j - 2, 2 * k - 1)]);
+        }))
+        return -1;
+    const int sy = gc.sy, sz = gc.sz;
+    return launch_interior(gc, s, [=] __device__(int i, int j, int k) {
+        const int o = gc.at(i, j, k);
+        ei[o] = 1.0 / (a[o] + a[o - 1] + b[o] + b[o - sy] + c[o] + c[o - sz]);
+    });
+}
+
+static int slvsml(Ctx &c, double lengthsize)
+{
+    Level &L = c.lev[0];
+    for (int q = 1; q <= c.s.ntop; q++)
+        if (mg_relax(L.g, c.stream, L.u, lengthsize, L.rhs, L.a, L.b, L.c, L.ei, c.s.omega)) return -1;
+    return 0;
+}
+
+int op_mglin(Ctx &c)
+{
+    hipStream_t s = c.stream;
+    const int ng = c.ng;
+    for (int gi = ng - 1; gi >= 1; gi--) {
+        Level &C = c.lev[gi - 1], &F = c.lev[gi];
+        if (mg_rstrct(C.g, s, C.rhs, F.rhs)) return -1;
+        if (mg_acfromac(C.g, s, C.a, C.b, C.c, C.ei, F.a, F.b, F.c)) return -1;
+    }
+    Level &L1 = c.lev[0];
+    if (mg_fill(L1.g.n(), s, L1.u, 0.0)) return -1;
+    // the first coarsest solve is given lengthscale, not lengthsize (main.f:3689)
+    if (slvsml(c, c.s.lengthscale)) return -1;
+    const double ls = c.s.lengthsize;
+    for (int j = 2; j <= ng; j++) {
+        Level &J = c.lev[j - 1];
+        if (mg_interp(J.g, s, J.u, c.lev[j - 2].u)) return -1;
+        for (int jc = 1; jc <= c.s.cycles; jc++) {
+            for (int jj = j; jj >= 2; jj--) {
+                Level &F = c.lev[jj - 1], &C = c.lev[jj - 2];
+                for (int q = 1; q <= c.s.npre; q++)
+                    if (mg_relax(F.g, s, F.u, ls, F.rhs, F.a, F.b, F.c, F.ei, c.s.omega)) return -1;
+                if (mg_resid(F.g, s, ls, F.res, F.u, F.rhs, F.a, F.b, F.c)) return -1;
+                if (mg_rstrct(C.g, s, C.rhs, F.res)) return -1;
+                if (mg_fill(C.g.n(), s, C.u, 0.0)) return -1;
+            }
+            if (slvsml(c, ls)) return -1;
+            for (int jj = 2; jj <= j; jj++) {
+                Level &F = c.lev[jj - 1], &C = c.lev[jj - 2];
+                if (mg_addint(F.g, s, F.u, C.u, F.res)) return -1;
+                for (int q = 1; q <= c.s.npost; q++)
+                    if (mg_relax(F.g, s, F.u, ls, F.rhs, F.a, F.b, F.c, F.ei, c.s.omega)) return -1;
+            }
+        }
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// tolerance mode
+
+// bc_scalar ghost folding for the operator: across x-min, y and z faces the
+// neighbour equals the cell itself (the face drops out), across x-max the
+// ghost is 0.
+
+// Gauss-Seidel colour pass with the operator's own diagonal, in place
+template <typename T>
+__global__ void __launch_bounds__(256) k_gs_colour(Grid g, T *u, const T *f, const T *a, const T *b, const T *c, T h2,
+                                                   int colour, int zero_init)
+{
+    const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + 2;
+    const int k = (int)blockIdx.z + 2;
+    const int i = 2 * (int)(blockIdx.x * blockDim.x + threadIdx.x) + 2 + ((colour + j + k) & 1);
+    if (i > g.nx - 1 || j > g.ny - 1) return;
+    const int o = g.at(i, j, k), sy = g.sy, sz = g.sz;
+    const T ae = a[o], aw = a[o - 1], bn = b[o], bs = b[o - sy], ct = c[o], cb = c[o - sz];
+    T s = (T)0, d = ae;
+    if (!zero_init) {
+        if (i < g.nx - 1) s += ae * u[o + 1];
+        if (i > 2) s += aw * u[o - 1];
+        if (j < g.ny - 1) s += bn * u[o + sy];
+        if (j > 2) s += bs * u[o - sy];
+        if (k < g.nz - 1) s += ct * u[o + sz];
+        if (k > 2) s += cb * u[o - sz];
+    }
+    if (i > 2) d += aw;
+    if (j < g.ny - 1) d += bn;
+    if (j > 2) d += bs;
+    if (k < g.nz - 1) d += ct;
+    if (k > 2) d += cb;
+    u[o] = (s - h2 * f[o]) / d;
+}
+
+// r = f - A u
+template <typename T>
+__global__ void __launch_bounds__(256) k_resid_fold(Grid g, T *r, const T *u, const T *f, const T *a, const T *b,
+                                                    const T *c, T h2i)
+{
+    DG_INTERIOR_IJK(g);
+    const int o = g.at(i, j, k), sy = g.sy, sz = g.sz;
+    const T u0 = u[o];
+    const T ue = i < g.nx - 1 ? u[o + 1] : (T)0;
+    const T uw = i > 2 ? u[o - 1] : u0;
+    const T un = j < g.ny - 1 ? u[o + sy] : u0;
+    const T us = j > 2 ? u[o - sy] : u0;
+    const T ut = k < g.nz - 1 ? u[o + sz] : u0;
+    const T ub = k > 2 ? u[o - sz] : u0;
+    const T s = a[o] * (ue - u0) + a[o - 1] * (uw - u0) + b[o] * (un - u0) + b[o - sy] * (us - u0) + c[o] * (ut - u0) +
+                c[o - sz] * (ub - u0);
+    r[o] = f[o] - h2i * s;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_restrict_avg(Grid gc, Grid gf, T *fc, const T *rf)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 2;
+    const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + 2;
+    const int k = (int)blockIdx.z + 2;
+    if (i > gc.nx - 1 || j > gc.ny - 1) return;
+    const int f = gf.at(2 * i - 2, 2 * j - 2, 2 * k - 2), sy = gf.sy, sz = gf.sz;
+    const T t = rf[f] + rf[f + 1] + rf[f + sy] + rf[f + 1 + sy] + rf[f + sz] + rf[f + 1 + sz] + rf[f + sy + sz] +
+                rf[f + 1 + sy + sz];
+    fc[gc.at(i, j, k)] = (T)0.125 * t;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_prolong_inj_add(Grid gf, Grid gc, T *uf, const T *uc)
+{
+    DG_INTERIOR_IJK(gf);
+    uf[gf.at(i, j, k)] += uc[gc.at(i / 2 + 1, j / 2 + 1, k / 2 + 1)];
+}
+
+template <typename T>
+__global__ void k_zero(size_t n, T *a)
+{
+    size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (p < n) a[p] = (T)0;
+}
+
+__global__ void k_to_f32(size_t n, const double *src, float *dst)
+{
+    size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (p < n) dst[p] = (float)src[p];
+}
+
+// coarsest level: assemble -A (SPD) densely and factor it (Cholesky), one
+// block; the factor is kept for the forward/back substitutions of every
+// V-cycle of the solve
+__global__ void __launch_bounds__(256) k_coarse_factor(Grid g, const double *a, const double *b, const double *c,
+                                                      double h2i, double *M, int *info)
+{
+    const int mx = g.nx - 2, my = g.ny - 2, mz = g.nz - 2, n = mx * my * mz;
+    for (int p = threadIdx.x; p < n * n; p += blockDim.x) M[p] = 0.0;
+    __syncthreads();
+    for (int row = threadIdx.x; row < n; row += blockDim.x) {
+        const int i = row % mx + 2, j = (row / mx) % my + 2, k = row / (mx * my) + 2;
+        const int o = g.at(i, j, k);
+        double d = a[o];  // x-max ghost (0) or real neighbour: the face always counts
+        if (i < g.nx - 1) M[row * n + row + 1] -= a[o] * h2i;
+        if (i > 2) { d += a[o - 1]; M[row * n + row - 1] -= a[o - 1] * h2i; }
+        if (j < g.ny - 1) { d += b[o]; M[row * n + row + mx] -= b[o] * h2i; }
+        if (j > 2) { d += b[o - g.sy]; M[row * n + row - mx] -= b[o - g.sy] * h2i; }
+        if (k < g.nz - 1) { d += c[o]; M[row * n + row + mx * my] -= c[o] * h2i; }
+        if (k > 2) { d += c[o - g.sz]; M[row * n + row - mx * my] -= c[o - g.sz] * h2i; }
+        M[row * n + row] += d * h2i;
+    }
+    __syncthreads();
+    // right-looking Cholesky, lower triangle
+    for (int p = 0; p < n; p++) {
+        if (threadIdx.x == 0) {
+            double dpp = M[p * n + p];
+            if (!(dpp > 0.0)) *info = p + 1;
+            M[p * n + p] = sqrt(dpp);
+        }
+        __syncthreads();
+        const double l = M[p * n + p];
+        for (int r = p + 1 + threadIdx.x; r < n; r += blockDim.x) M[r * n + p] /= l;
+        __syncthreads();
+        for (int q = threadIdx.x; q < (n - p - 1) * (n - p - 1); q += blockDim.x) {
+            int r = p + 1 + q / (n - p - 1), cc = p + 1 + q % (n - p - 1);
+            if (cc <= r) M[r * n + cc] -= M[r * n + p] * M[cc * n + p];
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64) k_coarse_solve(Grid g, const double *M, const T *f, T *u)
+{
+    // one wave: substitutions with the rhs staged in LDS
+    __shared__ double y[512];
+    const int mx = g.nx - 2, my = g.ny - 2, mz = g.nz - 2, n = mx * my * mz;
+    for (int row = threadIdx.x; row < n; row += 64) {
+        const int i = row % mx + 2, j = (row / mx) % my + 2, k = row / (mx * my) + 2;
+        y[row] = -(double)f[g.at(i, j, k)];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int p = 0; p < n; p++) {
+            double t = y[p];
+            for (int q = 0; q < p; q++) t -= M[p * n + q] * y[q];
+            y[p] = t / M[p * n + p];
+        }
+        for (int p = n - 1; p >= 0; p--) {
+            double t = y[p];
+            for (int q = p + 1; q < n; q++) t -= M[q * n + p] * y[q];
+            y[p] = t / M[p * n + p];
+        }
+    }
+    __syncthreads();
+    for (int row = threadIdx.x; row < n; row += 64) {
+        const int i = row % mx + 2, j = (row / mx) % my + 2, k = row / (mx * my) + 2;
+        u[g.at(i, j, k)] = (T)y[row];
+    }
+}
+
+struct TolWork {
+    double *M = nullptr;
+    int *info = nullptr;
+    size_t msize = 0;
+};
+static TolWork g_tw;
+
+static inline double lev_h2i(const Ctx &c, const Grid &g)
+{
+    const double h = c.s.lengthsize / (double)(g.nx - 2);
+    return 1.0 / (h * h);
+}
+
+template <typename T>
+static void level_ptrs(Level &L, T *&u, T *&f, T *&r, const T *&a, const T *&b, const T *&cc);
+template <>
+void level_ptrs<double>(Level &L, double *&u, double *&f, double *&r, const double *&a, const double *&b,
+                        const double *&cc)
+{
+    u = L.u64; f = L.f64; r = L.r64; a = L.a; b = L.b; cc = L.c;
+}
+template <>
+void level_ptrs<float>(Level &L, float *&u, float *&f, float *&r, const float *&a, const float *&b, const float *&cc)
+{
+    u = L.u32; f = L.f32; r = L.r32; a = L.a32; b = L.b32; cc = L.c32;
+}
+
+template <typename T>
+static int gs_sweep(Ctx &c, const Grid &g, T *u, const T *f, const T *a, const T *b, const T *cc, T h2, int first,
+                    int second, int zero_first)
+{
+    const int half = (g.nx - 2 + 1) / 2;
+    dim3 grid((unsigned)((half + BX - 1) / BX), (unsigned)((g.ny - 2 + BY - 1) / BY), (unsigned)(g.nz - 2));
+    hipLaunchKernelGGL(k_gs_colour<T>, grid, interior_block(), 0, c.stream, g, u, f, a, b, cc, h2, first, zero_first);
+    DG_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_gs_colour<T>, grid, interior_block(), 0, c.stream, g, u, f, a, b, cc, h2, second, 0);
+    DG_CHECK_LAUNCH();
+    return 0;
+}
+
+// symmetric V-cycle from a zero guess on level l: u_l ~ A_l^-1 f_l
+template <typename T>
+static int vcycle(Ctx &c, int l)
+{
+    Level &L = c.lev[l];
+    const Grid g = L.g;
+    T *u, *f, *r;
+    const T *a, *b, *cc;
+    level_ptrs<T>(L, u, f, r, a, b, cc);
+    if (l == 0) {
+        hipLaunchKernelGGL(k_coarse_solve<T>, dim3(1), dim3(64), 0, c.stream, g, g_tw.M, f, u);
+        DG_CHECK_LAUNCH();
+        return 0;
+    }
+    const double h2i = lev_h2i(c, g);
+    const T h2 = (T)(1.0 / h2i), th2i = (T)h2i;
+    for (int sidx = 0; sidx < c.mg_nu1; sidx++)
+        if (gs_sweep<T>(c, g, u, f, a, b, cc, h2, 0, 1, sidx == 0)) return -1;
+    if (c.mg_nu1 == 0) {
+        hipLaunchKernelGGL(k_zero<T>, dim3((unsigned)((g.n() + 255) / 256)), dim3(256), 0, c.stream, g.n(), u);
+        DG_CHECK_LAUNCH();
+    }
+    hipLaunchKernelGGL(k_resid_fold<T>, interior_grid(g), interior_block(), 0, c.stream, g, r, u, f, a, b, cc, th2i);
+    DG_CHECK_LAUNCH();
+    Level &C = c.lev[l - 1];
+    T *uc, *fc, *rc;
+    const T *ac, *bc, *ccc;
+    level_ptrs<T>(C, uc, fc, rc, ac, bc, ccc);
+    hipLaunchKernelGGL(k_restrict_avg<T>, interior_grid(C.g), interior_block(), 0, c.stream, C.g, g, fc, r);
+    DG_CHECK_LAUNCH();
+    if (vcycle<T>(c, l - 1)) return -1;
+    hipLaunchKernelGGL(k_prolong_inj_add<T>, interior_grid(g), interior_block(), 0, c.stream, g, C.g, u, uc);
+    DG_CHECK_LAUNCH();
+    for (int sidx = 0; sidx < c.mg_nu2; sidx++)
+        if (gs_sweep<T>(c, g, u, f, a, b, cc, h2, 1, 0, 0)) return -1;
+    return 0;
+}
+
+// ---- PCG vector kernels on the finest level (FP64) ----
+
+// K1: partial sums of r.z and z.q (z = the V-cycle result, type T)
+template <typename T>
+__global__ void __launch_bounds__(256) k_dot_rz_zq(Grid g, const double *r, const T *z, const double *q, double *partial)
+{
+    __shared__ double lds4[4];
+    const int mx = g.nx - 2, my = g.ny - 2, mz = g.nz - 2;
+    const long long tot = (long long)mx * my * mz;
+    double a1 = 0.0, a2 = 0.0;
+    for (long long p = (long long)blockIdx.x * 256 + threadIdx.x; p < tot; p += (long long)gridDim.x * 256) {
+        int i = (int)(p % mx) + 2;
+        long long t = p / mx;
+        int j = (int)(t % my) + 2, k = (int)(t / my) + 2;
+        const int o = g.at(i, j, k);
+        const double zz = (double)z[o];
+        a1 += r[o] * zz;
+        a2 += zz * q[o];
+    }
+    double s1 = block_reduce256<false>(a1, lds4);
+    double s2 = block_reduce256<false>(a2, lds4);
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = s1;
+        partial[2 * blockIdx.x + 1] = s2;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_final2(const double *partial, int nb, double *out)
+{
+    __shared__ double lds4[4];
+    double a1 = 0.0, a2 = 0.0;
+    for (int p = threadIdx.x; p < nb; p += 256) {
+        a1 += partial[2 * p];
+        a2 += partial[2 * p + 1];
+    }
+    double s1 = block_reduce256<false>(a1, lds4);
+    double s2 = block_reduce256<false>(a2, lds4);
+    if (threadIdx.x == 0) {
+        out[0] = s1;
+        out[1] = s2;
+    }
+}
+
+// device scalar slots
+enum { S_RZ = 16, S_ZQ, S_RZOLD, S_ALPHA, S_PQ, S_RR, S_BB, S_FIRST };
+
+// K2: p_new = z + beta p_old (beta from device scalars), q = A p_new,
+// partial sums of p_new . q
+template <typename T>
+__global__ void __launch_bounds__(256) k_pcg_dir(Grid g, const T *z, const double *pold, double *pnew, double *q,
+                                                 const double *a, const double *b, const double *c, double h2i,
+                                                 const double *ds, double *partial)
+{
+    __shared__ double lds4[4];
+    const int bidx = (int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 2;
+    const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + 2;
+    const int k = (int)blockIdx.z + 2;
+    const int tid = (int)(threadIdx.x + blockDim.x * threadIdx.y);
+    double contrib = 0.0;
+    const bool first = ds[S_FIRST] != 0.0;
+    const double beta = first ? 0.0 : -ds[S_ALPHA] * ds[S_ZQ] / ds[S_RZOLD];
+    if (i <= g.nx - 1 && j <= g.ny - 1) {
+        const int o = g.at(i, j, k), sy = g.sy, sz = g.sz;
+        auto P = [&](int oo) { return (double)z[oo] + beta * pold[oo]; };
+        const double p0 = P(o);
+        const double pe = i < g.nx - 1 ? P(o + 1) : 0.0;
+        const double pw = i > 2 ? P(o - 1) : p0;
+        const double pn = j < g.ny - 1 ? P(o + sy) : p0;
+        const double ps = j > 2 ? P(o - sy) : p0;
+        const double pt = k < g.nz - 1 ? P(o + sz) : p0;
+        const double pb = k > 2 ? P(o - sz) : p0;
+        const double s = a[o] * (pe - p0) + a[o - 1] * (pw - p0) + b[o] * (pn - p0) + b[o - sy] * (ps - p0) +
+                         c[o] * (pt - p0) + c[o - sz] * (pb - p0);
+        const double qq = h2i * s;
+        pnew[o] = p0;
+        q[o] = qq;
+        contrib = p0 * qq;
+    }
+    // block of 64x4: reduce
+    double v = wave_sum(contrib);
+    if ((tid & 63) == 0) lds4[tid >> 6] = v;
+    __syncthreads();
+    if (tid == 0) partial[bidx] = lds4[0] + lds4[1] + lds4[2] + lds4[3];
+}
+
+// K3: alpha = rz/pq; x += alpha p; r -= alpha q; partial sums of r.r
+__global__ void __launch_bounds__(256) k_pcg_update(Grid g, double *x, double *r, const double *p, const double *q,
+                                                    double *ds, double *partial)
+{
+    __shared__ double lds4[4];
+    const int bidx = (int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 2;
+    const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + 2;
+    const int k = (int)blockIdx.z + 2;
+    const int tid = (int)(threadIdx.x + blockDim.x * threadIdx.y);
+    const double alpha = ds[S_RZ] / ds[S_PQ];
+    double contrib = 0.0;
+    if (i <= g.nx - 1 && j <= g.ny - 1) {
+        const int o = g.at(i, j, k);
+        x[o] = x[o] + alpha * p[o];
+        const double rn = r[o] - alpha * q[o];
+        r[o] = rn;
+        contrib = rn * rn;
+    }
+    double v = wave_sum(contrib);
+    if ((tid & 63) == 0) lds4[tid >> 6] = v;
+    __syncthreads();
+    if (tid == 0) partial[bidx] = lds4[0] + lds4[1] + lds4[2] + lds4[3];
+}
+
+__global__ void k_final_sum(const double *partial, int nb, double *out)
+{
+    __shared__ double lds4[4];
+    double a1 = 0.0;
+    for (int p = threadIdx.x; p < nb; p += 256) a1 += partial[p];
+    double s1 = block_reduce256<false>(a1, lds4);
+    if (threadIdx.x == 0) *out = s1;
+}
+
+// scalar bookkeeping after K3: alpha_old = rz/pq, rz_old = rz, first = 0
+__global__ void k_pcg_scalars(double *ds)
+{
+    ds[S_ALPHA] = ds[S_RZ] / ds[S_PQ];
+    ds[S_RZOLD] = ds[S_RZ];
+    ds[S_FIRST] = 0.0;
+}
+
+// r = b - A x with folded ghosts; partial sums of r.r and b.b
+__global__ void __launch_bounds__(256) k_init_resid(Grid g, double *r, const double *x, const double *bvec,
+                                                    const double *a, const double *b, const double *c, double h2i,
+                                                    double *partial)
+{
+    __shared__ double lds4[4];
+    __shared__ double lds4b[4];
+    const int bidx = (int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 2;
+    const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + 2;
+    const int k = (int)blockIdx.z + 2;
+    const int tid = (int)(threadIdx.x + blockDim.x * threadIdx.y);
+    double c1 = 0.0, c2 = 0.0;
+    if (i <= g.nx - 1 && j <= g.ny - 1) {
+        const int o = g.at(i, j, k), sy = g.sy, sz = g.sz;
+        const double u0 = x[o];
+        const double ue = i < g.nx - 1 ? x[o + 1] : 0.0;
+        const double uw = i > 2 ? x[o - 1] : u0;
+        const double un = j < g.ny - 1 ? x[o + sy] : u0;
+        const double us = j > 2 ? x[o - sy] : u0;
+        const double ut = k < g.nz - 1 ? x[o + sz] : u0;
+        const double ub = k > 2 ? x[o - sz] : u0;
+        const double s = a[o] * (ue - u0) + a[o - 1] * (uw - u0) + b[o] * (un - u0) + b[o - sy] * (us - u0) +
+                         c[o] * (ut - u0) + c[o - sz] * (ub - u0);
+        const double bb = bvec[o];
+        const double rr = bb - h2i * s;
+        r[o] = rr;
+        c1 = rr * rr;
+        c2 = bb * bb;
+    }
+    double v1 = wave_sum(c1), v2 = wave_sum(c2);
+    if ((tid & 63) == 0) { lds4[tid >> 6] = v1; lds4b[tid >> 6] = v2; }
+    __syncthreads();
+    if (tid == 0) {
+        partial[2 * bidx] = lds4[0] + lds4[1] + lds4[2] + lds4[3];
+        partial[2 * bidx + 1] = lds4b[0] + lds4b[1] + lds4b[2] + lds4b[3];
+    }
+}
+
+__global__ void k_copy_cast(size_t n, const double *src, double *dst)
+{
+    size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (p < n) dst[p] = src[p];
+}
+
+static int ensure_tol_buffers(Ctx &c)
+{
+    for (int l = 0; l < c.ng; l++) {
+        Level &L = c.lev[l];
+        const size_t n = L.g.n();
+        if (c.mg_mode == 2 && !L.u32) {
+            DG_HIP(hipMalloc(&L.a32, n * sizeof(float)));
+            DG_HIP(hipMalloc(&L.b32, n * sizeof(float)));
+            DG_HIP(hipMalloc(&L.c32, n * sizeof(float)));
+            DG_HIP(hipMalloc(&L.u32, n * sizeof(float)));
+            DG_HIP(hipMalloc(&L.f32, n * sizeof(float)));
+            DG_HIP(hipMalloc(&L.r32, n * sizeof(float)));
+            DG_HIP(hipMemset(L.u32, 0, n * sizeof(float)));
+            DG_HIP(hipMemset(L.f32, 0, n * sizeof(float)));
+            DG_HIP(hipMemset(L.r32, 0, n * sizeof(float)));
+        }
+        if (c.mg_mode == 1 && !L.u64) {
+            DG_HIP(hipMalloc(&L.u64, n * sizeof(double)));
+            DG_HIP(hipMalloc(&L.f64, n * sizeof(double)));
+            DG_HIP(hipMalloc(&L.r64, n * sizeof(double)));
+            DG_HIP(hipMemset(L.u64, 0, n * sizeof(double)));
+            DG_HIP(hipMemset(L.f64, 0, n * sizeof(double)));
+            DG_HIP(hipMemset(L.r64, 0, n * sizeof(double)));
+        }
+    }
+    const size_t nfine = c.g.n();
+    if (!c.pcg_r) {
+        DG_HIP(hipMalloc(&c.pcg_r, nfine * sizeof(double)));
+        DG_HIP(hipMalloc(&c.pcg_z, nfine * sizeof(double)));
+        DG_HIP(hipMalloc(&c.pcg_p, nfine * sizeof(double)));
+        DG_HIP(hipMalloc(&c.pcg_q, nfine * sizeof(double)));
+        DG_HIP(hipMemset(c.pcg_r, 0, nfine * sizeof(double)));
+        DG_HIP(hipMemset(c.pcg_z, 0, nfine * sizeof(double)));
+        DG_HIP(hipMemset(c.pcg_p, 0, nfine * sizeof(double)));
+        DG_HIP(hipMemset(c.pcg_q, 0, nfine * sizeof(double)));
+    }
+    const Grid g0 = c.lev[0].g;
+    const size_t n0 = (size_t)(g0.nx - 2) * (g0.ny - 2) * (g0.nz - 2);
+    if (n0 > 512) {
+        set_error("coarsest multigrid level has %zu unknowns (limit 512)", n0);
+        return -1;
+    }
+    if (g_tw.msize < n0 * n0) {
+        if (g_tw.M) (void)hipFree(g_tw.M);
+        DG_HIP(hipMalloc(&g_tw.M, n0 * n0 * sizeof(double)));
+        g_tw.msize = n0 * n0;
+        if (!g_tw.info) DG_HIP(hipMalloc(&g_tw.info, sizeof(int)));
+    }
+    return 0;
+}
+
+template <typename T>
+static int pcg_solve(Ctx &c)
+{
+    hipStream_t s = c.stream;
+    Level &F = c.lev[c.ng - 1];
+    const Grid g = F.g;
+    const double h2i = lev_h2i(c, g);
+    double *x = F.u, *bvec = F.rhs, *r = c.pcg_r, *q = c.pcg_q;
+    double *pa = c.pcg_p, *pb = c.pcg_z;  // ping-pong search directions
+    double *ds = c.dscal;
+    const dim3 ig = interior_grid(g), ib = interior_block();
+    const int nblk = (int)(ig.x * ig.y * ig.z);
+    // partial buffers: need nblk*2 doubles
+    double *partial = c.red_partial;
+    // initial guess
+    if (c.mg_guess == 1) {
+        DG_HIP(hipMemcpyAsync(x, c.pprev, g.n() * sizeof(double), hipMemcpyDeviceToDevice, s));
+    } else {
+        if (mg_fill(g.n(), s, x, 0.0)) return -1;
+    }
+    hipLaunchKernelGGL(k_init_resid, ig, ib, 0, s, g, r, x, bvec, F.a, F.b, F.c, h2i, partial);
+    DG_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_final2, dim3(1), dim3(256), 0, s, partial, nblk, ds + S_RR);  // RR, then PQ slot gets bb
+    DG_CHECK_LAUNCH();
+    DG_HIP(hipMemcpyAsync(c.hscal + S_RR, ds + S_RR, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    DG_HIP(hipStreamSynchronize(s));
+    double bb = c.hscal[S_RR + 1];
+    double bnorm = sqrt(bb);
+    if (bnorm == 0.0) bnorm = 1.0;
+    double rel = sqrt(c.hscal[S_RR]) / bnorm;
+    // device scalar FIRST = 1
+    {
+        double one = 1.0;
+        DG_HIP(hipMemcpyAsync(ds + S_FIRST, &one, sizeof(double), hipMemcpyHostToDevice, s));
+        DG_HIP(hipStreamSynchronize(s));
+    }
+    // coarse factorisation for this solve
+    {
+        Level &L0 = c.lev[0];
+        hipLaunchKernelGGL(k_coarse_factor, dim3(1), dim3(256), 0, s, L0.g, L0.a, L0.b, L0.c, lev_h2i(c, L0.g), g_tw.M,
+                           g_tw.info);
+        DG_CHECK_LAUNCH();
+    }
+    Level &top = c.lev[c.ng - 1];
+    int it = 0;
+    while (rel >= c.mg_tol && it < c.mg_maxit) {
+        // z = M r
+        T *u, *f, *rr_;
+        const T *a_, *b_, *c_;
+        level_ptrs<T>(top, u, f, rr_, a_, b_, c_);
+        if (sizeof(T) == 4) {
+            hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((g.n() + 255) / 256)), dim3(256), 0, s, g.n(), r,
+                               (float *)f);
+            DG_CHECK_LAUNCH();
+        } else {
+            hipLaunchKernelGGL(k_copy_cast, dim3((unsigned)((g.n() + 255) / 256)), dim3(256), 0, s, g.n(), r,
+                               (double *)f);
+            DG_CHECK_LAUNCH();
+        }
+        if (vcycle<T>(c, c.ng - 1)) return -1;
+        // rz and zq
+        hipLaunchKernelGGL(k_dot_rz_zq<T>, dim3(RED_BLOCKS), dim3(256), 0, s, g, r, (const T *)u, q, partial);
+        DG_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_final2, dim3(1), dim3(256), 0, s, partial, RED_BLOCKS, ds + S_RZ);
+        DG_CHECK_LAUNCH();
+        // direction and q = A p
+        hipLaunchKernelGGL(k_pcg_dir<T>, ig, ib, 0, s, g, (const T *)u, pa, pb, q, F.a, F.b, F.c, h2i, ds, partial);
+        DG_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(256), 0, s, partial, nblk, ds + S_PQ);
+        DG_CHECK_LAUNCH();
+        // update
+        hipLaunchKernelGGL(k_pcg_update, ig, ib, 0, s, g, x, r, pb, q, ds, partial);
+        DG_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(256), 0, s, partial, nblk, ds + S_RR);
+        DG_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(1), 0, s, ds);
+        DG_CHECK_LAUNCH();
+        double *tmp = pa; pa = pb; pb = tmp;
+        DG_HIP(hipMemcpyAsync(c.hscal + S_RR, ds + S_RR, sizeof(double), hipMemcpyDeviceToHost, s));
+        DG_HIP(hipStreamSynchronize(s));
+        rel = sqrt(c.hscal[S_RR]) / bnorm;
+        it++;
+    }
+    c.mg_iters_last = it;
+    c.mg_relres_last = rel;
+    if (op_bc_scalar(g, s, x)) return -1;
+    DG_HIP(hipMemcpyAsync(c.pprev, x, g.n() * sizeof(double), hipMemcpyDeviceToDevice, s));
+    return 0;
+}
+
+int op_mg_solve_tol(Ctx &c)
+{
+    hipStream_t s = c.stream;
+    if (ensure_tol_buffers(c)) return -1;
+    const Grid g = c.g;
+    // partial-sum buffer must hold 2 values per interior block
+    {
+        const dim3 ig = interior_grid(g);
+        const size_t need = 2 * (size_t)ig.x * ig.y * ig.z;
+        if (need > 2 * (size_t)RED_BLOCKS * 64) {
+            set_error("grid too large for the reduction buffer");
+            return -1;
+        }
+    }
+    for (int gi = c.ng - 1; gi >= 1; gi--) {
+        Level &C = c.lev[gi - 1], &F = c.lev[gi];
+        if (mg_acfromac(C.g, s, C.a, C.b, C.c, C.ei, F.a, F.b, F.c)) return -1;
+    }
+    if (c.mg_mode == 2) {
+        for (int l = 0; l < c.ng; l++) {
+            Level &L = c.lev[l];
+            const size_t n = L.g.n();
+            const unsigned nb = (unsigned)((n + 255) / 256);
+            hipLaunchKernelGGL(k_to_f32, dim3(nb), dim3(256), 0, s, n, L.a, L.a32);
+            hipLaunchKernelGGL(k_to_f32, dim3(nb), dim3(256), 0, s, n, L.b, L.b32);
+            hipLaunchKernelGGL(k_to_f32, dim3(nb), dim3(256), 0, s, n, L.c, L.c32);
+            DG_CHECK_LAUNCH();
+        }
+        return pcg_solve<float>(c);
+    }
+    return pcg_solve<double>(c);
+}
+
+}  // namespace dg

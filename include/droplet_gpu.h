@@ -1,0 +1,131 @@
+/*
+ * droplet_gpu.h -- C-ABI of libdroplet_gpu.so, the MI355X (gfx950) drop-in
+ * for the per-time-step path of farzan-t/3D_Droplet_simulation_2007.
+ *
+ * Conventions (Fortran-friendly, so the reference's Fortran-77 host can bind
+ * them through ISO_C_BINDING interface blocks):
+ *  - every scalar argument is passed by reference (const int*, const double*);
+ *  - arrays are FP64, column-major (nx,ny,nz) with one ghost layer, exactly
+ *    the reference's layout (input.h:33), passed as plain host pointers;
+ *  - every function returns 0 on success and -1 on failure; dg_last_error()
+ *    then names the failure.  The reference has no error codes (it STOPs);
+ *    failures that the reference STOPs on are reported the same way here.
+ *  - like the reference's COMMON state, the device state is process-wide and
+ *    calls are not re-entrant.  Field state stays resident in HBM between
+ *    calls; only dg_get_field/dg_set_field and the host-pointer kernels move
+ *    data across PCIe.
+ */
+#ifndef DROPLET_GPU_H
+#define DROPLET_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* field ids for dg_get_field / dg_set_field */
+enum {
+    DG_F_CC = 0, DG_F_U, DG_F_V, DG_F_W, DG_F_P,
+    DG_F_S11, DG_F_S22, DG_F_S33, DG_F_S12, DG_F_S13, DG_F_S23,
+    DG_F_RHON, DG_F_MUN, DG_F_MUT, DG_F_KTUR, DG_F_EPS,
+    DG_F_A, DG_F_B, DG_F_C, DG_F_EI, DG_F_RHS, DG_F_RES,
+    DG_F_PPREV,
+    DG_NFIELDS
+};
+
+/* Scalars the host reads or sets between calls: /PHYSICS/ tau, time;
+ * /DIFF/ tauh, tauh2 (INTEGER in input.h), tswap; /CYCLING/ NPRE, NTOP,
+ * NPOST (input.h:35-60) plus the last step's solver diagnostics. */
+typedef struct dg_state {
+    double tau, time, tauh;
+    int tauh2, tswap, npre, ntop, npost;
+    int nx, ny, nz, ng;
+    int mg_iters;        /* PCG iterations of the last tolerance-mode solve */
+    double mg_relres;    /* its final ||b-Ap||/||b|| */
+    double cfl;          /* CFL number computed by the last checktime */
+    double epsdiv;       /* h*max|div| of the last adaptive-smoothing check */
+} dg_state;
+
+/* ---- lifecycle ------------------------------------------------------- */
+
+/* Replaces initialize (main.f:121-183): readinput (main.f:185-272) of the
+ * unchanged inputdata.in format, memoryaloc (main.f:274-322), nondim
+ * (main.f:341-446), the droplet IC smooth2 (main.f:5678-5752), bc_c, bc_v,
+ * calcrho.  Allocates every field in HBM. */
+int dg_init(const char *inputdata_path);
+/* Frees device memory (the reference has no counterpart; process exit). */
+int dg_finalize(void);
+/* 0 if the library is usable, -1 after an error. */
+int dg_status(void);
+const char *dg_last_error(void);
+
+/* ---- the time-step operators (device resident) ------------------------ */
+int dg_step(const int *t);          /* loop body main.f:21-71 (printout excluded) */
+int dg_run(const int *t0, const int *nsteps);
+int dg_vofmethod(const int *t);     /* VOFmethod main.f:795-824 */
+int dg_kepsmodel(const int *t);     /* kepsmodel main.f:1888-2026 (iturb=3) */
+int dg_checktime(const int *t);     /* checktime main.f:2156-2175 */
+int dg_csf(void);                   /* csf main.f:2363-2481 (surfacet=2) */
+int dg_solvelpres(const int *t);    /* solvelpres main.f:3236-3439 */
+int dg_calcrho(void);               /* calcrho main.f:2177-2190 */
+int dg_mglin(void);                 /* mglin main.f:3645-3763 on the resident finest level */
+
+/* ---- state exchange (replaces reading/writing the COMMON blocks) ------ */
+int dg_get_field(const int *id, double *host);
+int dg_set_field(const int *id, const double *host);
+int dg_get_state(dg_state *out);
+int dg_set_state(const dg_state *in);   /* sets tau,time,tauh,tauh2,tswap,npre,ntop,npost */
+/* Clips the resident volume fraction to [0,1] (smooth2 can
+ * leave 1+4e-13, which sends the reference's AL3D to NaN). */
+int dg_clip_cc(void);
+
+/* Pressure-solver selection (no reference counterpart):
+ * mode 0 = the reference's mglin schedule (default, bit-parity path),
+ * mode 1 = FP64 PCG with an FP64 V-cycle preconditioner to ||r||/||b|| < tol,
+ * mode 2 = FP64 PCG with an FP32 V-cycle preconditioner to ||r||/||b|| < tol.
+ * guess: 0 zero, 1 previous step's pressure. */
+int dg_set_solver(const int *mode, const double *tol, const int *nu1, const int *nu2, const int *maxit,
+                  const int *guess);
+/* 1 (default): outflow sums in the reference's sequential order
+ * (main.f:4123-4131); 0: parallel tree sums (different rounding). */
+int dg_set_ordered_sums(const int *on);
+
+/* ---- host-pointer kernels with the reference's argument lists --------- */
+/* Each uploads its arrays, runs on the GPU and downloads the outputs. */
+int dg_relax(double *u, const double *lengthsize, const double *rhs, const double *a, const double *b,
+             const double *c, const double *ei, const int *ibc, const int *nx, const int *ny, const int *nz,
+             const double *omega);                                           /* relax main.f:3956 */
+int dg_resid(const double *lengthsize, double *res, const double *u, const double *rhs, const double *a,
+             const double *b, const double *c, const int *ibc, const int *nx, const int *ny,
+             const int *nz);                                                 /* resid main.f:3902 */
+int dg_rstrct(double *uc, const double *uf, const int *ibc, const int *ncx, const int *ncy,
+              const int *ncz);                                               /* rstrct main.f:3765 */
+int dg_interp(double *uf, const double *uc, const int *ibc, const int *nfx, const int *nfy,
+              const int *nfz);                                               /* interp main.f:3866 */
+int dg_addint(double *uf, const double *uc, double *res, const int *ibc, const int *nfx, const int *nfy,
+              const int *nfz);                                               /* addint main.f:3934 */
+int dg_acfromac(double *a, double *b, double *c, double *ei, const double *a2, const double *b2,
+                const double *c2, const int *nx, const int *ny, const int *nz); /* acfromac main.f:3793 */
+int dg_slvsml(double *u, const double *rhs, const double *a, const double *b, const double *c, const double *ei,
+              const int *ibc, const int *nx, const int *ny, const int *nz, const int *ntop, const double *omega,
+              const double *lengthsize);                                     /* slvsml main.f:3846 */
+int dg_abcei(double *a, double *b, double *c, double *ei, const double *cc, const double *r1, const double *r2,
+             const int *ibc, const int *nx, const int *ny, const int *nz);   /* abcei main.f:3582 */
+int dg_bc_scalar(double *scal, const int *ibc, const int *nx, const int *ny, const int *nz); /* main.f:4298 */
+int dg_filterc27(const double *cc, double *cfilt, const int *nx, const int *ny, const int *nz); /* main.f:5473 */
+/* AL3D (main.f:4562) and FL3D (main.f:4639) over n independent inputs */
+int dg_al3d_batch(const int *n, const double *b1, const double *b2, const double *b3, const double *cc,
+                  double *out);
+int dg_fl3d_batch(const int *n, const double *m1, const double *m2, const double *m3, const double *alpha,
+                  const double *r0, const double *dr0, double *out);
+
+/* Tolerance-mode solve of resid's operator (main.f:3902-3931) on host arrays
+ * (no reference counterpart): p is the initial guess on entry and the
+ * solution on return (ghosts filled as bc_scalar).  a,b,c as built by abcei. */
+int dg_poisson_solve(const int *nx, const int *ny, const int *nz, const double *lengthsize, const double *rhs,
+                     const double *a, const double *b, const double *c, double *p, const int *mode,
+                     const double *tol, const int *maxit, int *iters, double *relres);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

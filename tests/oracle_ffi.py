@@ -55,8 +55,87 @@ def lib():
         L.or_mglin.argtypes = [ctypes.c_void_p]
         L.or_resid_norm.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
         L.or_resid_norm.restype = ctypes.c_double
+        dp, i_, d_ = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        L.or_relax.argtypes = [dp, d_, dp, dp, dp, dp, dp, i_, i_, i_, d_]
+        L.or_resid.argtypes = [d_, dp, dp, dp, dp, dp, dp, i_, i_, i_]
+        L.or_rstrct.argtypes = [dp, dp, i_, i_, i_]
+        L.or_interp.argtypes = [dp, dp, i_, i_, i_]
+        L.or_addint.argtypes = [dp, dp, dp, i_, i_, i_]
+        L.or_acfromac.argtypes = [dp, dp, dp, dp, dp, dp, dp, i_, i_, i_]
+        L.or_slvsml.argtypes = [dp, dp, dp, dp, dp, dp, i_, i_, i_, i_, d_, d_]
+        L.or_abcei.argtypes = [dp, dp, dp, dp, dp, d_, d_, i_, i_, i_]
+        L.or_bc_scalar.argtypes = [dp, i_, i_, i_]
+        L.or_filterc27.argtypes = [dp, dp, i_, i_, i_]
         _lib = L
     return _lib
+
+
+def _ptr(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class OracleKernels:
+    """The restatement's explicit-argument kernels, same call shapes as
+    refrun.RefKernels and the GPU package's host-pointer kernels."""
+
+    @staticmethod
+    def al3d(b1, b2, b3, cc):
+        return lib().or_al3d(b1, b2, b3, cc)
+
+    @staticmethod
+    def fl3d(m1, m2, m3, alpha, r0, dr0):
+        return lib().or_fl3d(m1, m2, m3, alpha, r0, dr0)
+
+    @staticmethod
+    def relax(u, lengthsize, rhs, a, b, c, ei, omega):
+        nz, ny, nx = u.shape
+        lib().or_relax(_ptr(u), lengthsize, _ptr(rhs), _ptr(a), _ptr(b), _ptr(c), _ptr(ei), nx, ny, nz, omega)
+
+    @staticmethod
+    def resid(lengthsize, res, u, rhs, a, b, c):
+        nz, ny, nx = u.shape
+        lib().or_resid(lengthsize, _ptr(res), _ptr(u), _ptr(rhs), _ptr(a), _ptr(b), _ptr(c), nx, ny, nz)
+
+    @staticmethod
+    def rstrct(uc, uf):
+        ncz, ncy, ncx = uc.shape
+        lib().or_rstrct(_ptr(uc), _ptr(uf), ncx, ncy, ncz)
+
+    @staticmethod
+    def interp(uf, uc):
+        nz, ny, nx = uf.shape
+        lib().or_interp(_ptr(uf), _ptr(uc), nx, ny, nz)
+
+    @staticmethod
+    def addint(uf, uc, res):
+        nz, ny, nx = uf.shape
+        lib().or_addint(_ptr(uf), _ptr(uc), _ptr(res), nx, ny, nz)
+
+    @staticmethod
+    def acfromac(a, b, c, ei, a2, b2, c2):
+        nz, ny, nx = a.shape
+        lib().or_acfromac(_ptr(a), _ptr(b), _ptr(c), _ptr(ei), _ptr(a2), _ptr(b2), _ptr(c2), nx, ny, nz)
+
+    @staticmethod
+    def slvsml(u, rhs, a, b, c, ei, ntop, omega, lengthsize):
+        nz, ny, nx = u.shape
+        lib().or_slvsml(_ptr(u), _ptr(rhs), _ptr(a), _ptr(b), _ptr(c), _ptr(ei), nx, ny, nz, ntop, omega, lengthsize)
+
+    @staticmethod
+    def abcei(a, b, c, ei, cc, r1, r2):
+        nz, ny, nx = a.shape
+        lib().or_abcei(_ptr(a), _ptr(b), _ptr(c), _ptr(ei), _ptr(cc), r1, r2, nx, ny, nz)
+
+    @staticmethod
+    def bc_scalar(s):
+        nz, ny, nx = s.shape
+        lib().or_bc_scalar(_ptr(s), nx, ny, nz)
+
+    @staticmethod
+    def filterc27(cc, cf):
+        nz, ny, nx = cc.shape
+        lib().or_filterc27(_ptr(cc), _ptr(cf), nx, ny, nz)
 
 
 class OrState(ctypes.Structure):

@@ -64,7 +64,38 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_fortran_host(verbose)
     return LIB
+
+
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+HOST_DIR = os.path.join(HERE, "host")
+HOST_BIN = os.path.join(HERE, "droplet_host")
+
+
+def build_fortran_host(verbose=False):
+    """The Fortran host (host/*.f90) over the C-ABI, linked with an $ORIGIN
+    rpath so it finds libdroplet_gpu.so next to it."""
+    if not os.path.exists(FLANG):
+        if verbose:
+            print("flang not found: Fortran host not built")
+        return None
+    os.makedirs(OBJDIR, exist_ok=True)
+    mod_o = os.path.join(OBJDIR, "droplet_gpu_mod.o")
+    host_o = os.path.join(OBJDIR, "droplet_host.o")
+    steps = [
+        [FLANG, "-O2", "-J", OBJDIR, "-c", os.path.join(HOST_DIR, "droplet_gpu_mod.f90"), "-o", mod_o],
+        [FLANG, "-O2", "-J", OBJDIR, "-c", os.path.join(HOST_DIR, "droplet_host.f90"), "-o", host_o],
+        [FLANG, "-o", HOST_BIN + ".tmp", host_o, mod_o, f"-L{HERE}", "-ldroplet_gpu", "-Wl,-rpath,$ORIGIN"],
+    ]
+    for cmd in steps:
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"flang failed: {' '.join(cmd)}\n{r.stderr[-4000:]}")
+    os.replace(HOST_BIN + ".tmp", HOST_BIN)
+    if verbose:
+        print(f"built {HOST_BIN}")
+    return HOST_BIN
 
 
 if __name__ == "__main__":

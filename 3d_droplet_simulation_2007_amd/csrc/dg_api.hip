@@ -14,6 +14,7 @@
 
 #include "droplet_gpu.h"
 #include "dg_ctx.h"
+#include "dg_prof.h"
 #include "dg_util.h"
 
 namespace dg {
@@ -36,6 +37,12 @@ Ctx *ctx()
 {
     static Ctx c;
     return &c;
+}
+
+Prof &prof()
+{
+    static Prof p;
+    return p;
 }
 
 static bool g_ready = false;
@@ -712,6 +719,38 @@ int dg_poisson_solve(const int *nx, const int *ny, const int *nz, const double *
     *iters = lc.mg_iters_last;
     *relres = lc.mg_relres_last;
     free_ctx(lc);
+    return 0;
+}
+
+int dg_profile_start(const int *kernel_id)
+{
+    Prof &p = prof();
+    if (p.used) {
+        Ctx &c = *ctx();
+        if (c.stream) (void)hipStreamSynchronize(c.stream);
+    }
+    p.kernel = *kernel_id;
+    p.used = 0;
+    p.bytes = 0.0;
+    p.launches = 0;
+    return 0;
+}
+
+int dg_profile_stop(double *total_ms, int *launches, double *total_bytes)
+{
+    Prof &p = prof();
+    DG_HIP(hipDeviceSynchronize());
+    double ms = 0.0;
+    for (size_t q = 0; q + 1 < p.used; q += 2) {
+        float e = 0.0f;
+        DG_HIP(hipEventElapsedTime(&e, p.ev[q], p.ev[q + 1]));
+        ms += e;
+    }
+    *total_ms = ms;
+    *launches = (int)p.launches;
+    *total_bytes = p.bytes;
+    p.kernel = PK_NONE;
+    p.used = 0;
     return 0;
 }
 

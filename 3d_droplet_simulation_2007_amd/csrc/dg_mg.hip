@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "dg_ctx.h"
+#include "dg_prof.h"
 #include "dg_util.h"
 
 namespace dg {
@@ -49,7 +50,9 @@ int mg_relax(const Grid &g, hipStream_t s, double *u, double lengthsize, const d
     dim3 grid((unsigned)((half + BX - 1) / BX), (unsigned)((g.ny - 2 + BY - 1) / BY), (unsigned)(g.nz - 2));
     for (int pass = 0; pass < 2; pass++) {
         if (op_bc_scalar(g, s, u)) return -1;
+        prof_begin(PK_RELAX, s);
         hipLaunchKernelGGL(k_relax_colour, grid, interior_block(), 0, s, g, u, rhs, a, b, c, ei, h2, omega, pass);
+        prof_end(PK_RELAX, s, 3.5 * 8.0 * (double)(g.nx - 2) * (g.ny - 2) * (g.nz - 2));
         DG_CHECK_LAUNCH();
     }
     return op_bc_scalar(g, s, u);
@@ -374,9 +377,16 @@ static int gs_sweep(Ctx &c, const Grid &g, T *u, const T *f, const T *a, const T
 {
     const int half = (g.nx - 2 + 1) / 2;
     dim3 grid((unsigned)((half + BX - 1) / BX), (unsigned)((g.ny - 2 + BY - 1) / BY), (unsigned)(g.nz - 2));
+    // algorithmic bytes: one red-black sweep moves 7 words per cell (relax's
+    // a,b,c,ei,rhs,u -> u); one colour pass is half a sweep
+    const double bytes = 3.5 * (double)sizeof(T) * (double)(g.nx - 2) * (g.ny - 2) * (g.nz - 2);
+    prof_begin(PK_GS_COLOUR, c.stream);
     hipLaunchKernelGGL(k_gs_colour<T>, grid, interior_block(), 0, c.stream, g, u, f, a, b, cc, h2, first, zero_first);
+    prof_end(PK_GS_COLOUR, c.stream, bytes);
     DG_CHECK_LAUNCH();
+    prof_begin(PK_GS_COLOUR, c.stream);
     hipLaunchKernelGGL(k_gs_colour<T>, grid, interior_block(), 0, c.stream, g, u, f, a, b, cc, h2, second, 0);
+    prof_end(PK_GS_COLOUR, c.stream, bytes);
     DG_CHECK_LAUNCH();
     return 0;
 }
@@ -403,7 +413,9 @@ static int vcycle(Ctx &c, int l)
         hipLaunchKernelGGL(k_zero<T>, dim3((unsigned)((g.n() + 255) / 256)), dim3(256), 0, c.stream, g.n(), u);
         DG_CHECK_LAUNCH();
     }
+    prof_begin(PK_RESID, c.stream);
     hipLaunchKernelGGL(k_resid_fold<T>, interior_grid(g), interior_block(), 0, c.stream, g, r, u, f, a, b, cc, th2i);
+    prof_end(PK_RESID, c.stream, 6.0 * (double)sizeof(T) * (double)(g.nx - 2) * (g.ny - 2) * (g.nz - 2));
     DG_CHECK_LAUNCH();
     Level &C = c.lev[l - 1];
     T *uc, *fc, *rc;
@@ -709,12 +721,17 @@ static int pcg_solve(Ctx &c)
         hipLaunchKernelGGL(k_final2, dim3(1), dim3(256), 0, s, partial, RED_BLOCKS, ds + S_RZ);
         DG_CHECK_LAUNCH();
         // direction and q = A p
+        const double ncell = (double)(g.nx - 2) * (g.ny - 2) * (g.nz - 2);
+        prof_begin(PK_PCG_DIR, s);
         hipLaunchKernelGGL(k_pcg_dir<T>, ig, ib, 0, s, g, (const T *)u, pa, pb, q, F.a, F.b, F.c, h2i, ds, partial);
+        prof_end(PK_PCG_DIR, s, ((double)sizeof(T) + 6.0 * 8.0) * ncell);
         DG_CHECK_LAUNCH();
         hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(256), 0, s, partial, nblk, ds + S_PQ);
         DG_CHECK_LAUNCH();
         // update
+        prof_begin(PK_PCG_UPDATE, s);
         hipLaunchKernelGGL(k_pcg_update, ig, ib, 0, s, g, x, r, pb, q, ds, partial);
+        prof_end(PK_PCG_UPDATE, s, 6.0 * 8.0 * ncell);
         DG_CHECK_LAUNCH();
         hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(256), 0, s, partial, nblk, ds + S_RR);
         DG_CHECK_LAUNCH();

@@ -1,0 +1,170 @@
+"""Benchmark: time steps/s of the droplet solver at 256^3 with every step's
+pressure Poisson solve taken to a 1e-8 FP64 relative residual.
+
+Workload (one MI355X): 258^3 grid with ghosts (256^3 cells) on a 6 mm cube,
+a water droplet (r = 1 mm) in air 1.5 mm above the no-slip wall (reference
+axis y is vertical), impact velocity -3 m/s, gravity -9.81, CSF surface
+tension, laminar.  Synthetic: the droplet initial condition of the
+reference's smooth2, no dataset.
+
+Measured paths, each timed over K steps after W warm-up steps:
+  target       FP64 fields, PCG with an FP32 V-cycle preconditioner (FP64
+               residual/update), previous-step pressure as initial guess;
+  denominator  the same PCG with an FP64 V-cycle (the plain FP64 multigrid
+               solver), previous-step guess.
+Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = "time steps/s at 256^3 to 1e-8 Poisson residual: learned+mixed MG vs FP64 MG"
+HBM_PEAK_GBS = 8000.0
+
+
+def splash_inputdata(dg, n):
+    # 6 mm cube; droplet radius 1 mm, its lowest point 1.5 mm above the wall
+    # (centre 2.5 mm); the reference centres z and forces x0 = L/4.
+    return dg.make_inputdata(nx=n, ny=n, nz=n, tmax=200, maxcfl=0.25, iturb=0, convectype=2, yvel1=-3.0,
+                             rho2=1.2, rho1=1000.0, sigma=0.072, visc2=1.8e-5, visc1=1.0e-3, grav=-9.81,
+                             length=0.006, radius=0.001, y0=0.0025, z0=1000.0, cycles=2, npre=2, ntop=4, npost=2)
+
+
+def run_path(dg, txt, mode, steps, warmup, tol, profile_kernel=None):
+    g = dg.DropletGPU(inputdata_text=txt)
+    g.clip_cc()
+    g.set_ordered_sums(False)
+    g.set_solver(mode=mode, tol=tol, nu1=2, nu2=2, maxit=200, guess=1)
+    vol0 = float(g.get("cc")[1:-1, 1:-1, 1:-1].sum())
+    iters = []
+    for t in range(warmup):
+        g.step(t)
+        iters.append(g.state()["mg_iters"])
+    lib = dg.lib()
+    import ctypes
+    if profile_kernel is not None:
+        lib.dg_profile_start(ctypes.byref(ctypes.c_int(profile_kernel)))
+    g.state()  # synchronises the stream
+    t0 = time.perf_counter()
+    timed_iters = []
+    for t in range(warmup, warmup + steps):
+        g.step(t)
+        timed_iters.append(g.state()["mg_iters"])
+    st = g.state()  # synchronises
+    t1 = time.perf_counter()
+    prof = None
+    if profile_kernel is not None:
+        ms, nl, by = ctypes.c_double(), ctypes.c_int(), ctypes.c_double()
+        lib.dg_profile_stop(ctypes.byref(ms), ctypes.byref(nl), ctypes.byref(by))
+        prof = dict(total_ms=ms.value, launches=nl.value, bytes=by.value)
+    vol1 = float(g.get("cc")[1:-1, 1:-1, 1:-1].sum())
+    p = g.get("p")
+    g.close()
+    return dict(seconds=t1 - t0, steps=steps, iters=timed_iters, relres=st["mg_relres"], tau=st["tau"],
+                vol0=vol0, vol1=vol1, prof=prof, p=p)
+
+
+def cpu_baseline(n, steps, tol):
+    """The CPU restatement (oracle), one thread, FP64 PCG with an FP64 V-cycle
+    (the same algorithm as the denominator), on `steps` steps of the same
+    workload; initialisation excluded."""
+    sys.path.insert(0, os.path.join(HERE, "tests"))
+    import oracle_ffi
+    dg = importlib.import_module("3d_droplet_simulation_2007_amd")
+    txt = splash_inputdata(dg, n)
+    o = oracle_ffi.Oracle(txt)
+    cc = o.field("cc")
+    cc[...] = cc.clip(0.0, 1.0)
+    o.st.mg_mode, o.st.mg_outer, o.st.mg_prolong = 1, 1, 0
+    o.st.mg_tol, o.st.mg_nu1, o.st.mg_nu2, o.st.mg_maxcycles, o.st.mg_guess = tol, 2, 2, 200, 1
+    t0 = time.perf_counter()
+    for t in range(steps):
+        o.step(t)
+    dt = time.perf_counter() - t0
+    o.close()
+    return steps / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--grid", type=int, default=258, help="extent with ghosts (2^k+2)")
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-denominator", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU decomposition is a design in DESIGN.md, not a measured path)", file=sys.stderr)
+        sys.exit(2)
+    dg = importlib.import_module("3d_droplet_simulation_2007_amd")
+    dg.lib()
+    n = args.grid
+    txt = splash_inputdata(dg, n)
+    # dominant kernel: the finest-level FP32 Gauss-Seidel colour pass (id 1
+    # covers every level; the finest carries ~7/8 of its bytes)
+    tgt = run_path(dg, txt, 2, args.steps, args.warmup, args.tol, profile_kernel=1)
+    import numpy as np
+    out = {}
+    den = None
+    if not args.no_denominator:
+        den = run_path(dg, txt, 1, args.steps, args.warmup, args.tol)
+        a, b = tgt["p"][1:-1, 1:-1, 1:-1], den["p"][1:-1, 1:-1, 1:-1]
+        out["p_rel_l2_vs_fp64"] = float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+    value = tgt["steps"] / tgt["seconds"]
+    pr = tgt["prof"]
+    avg_ms = pr["total_ms"] / max(pr["launches"], 1)
+    achieved = (pr["bytes"] / max(pr["launches"], 1)) / (avg_ms * 1e-3) / 1e9
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "time steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * tgt["seconds"] / tgt["steps"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 fields and PCG, f32 V-cycle",
+        "data": "synthetic (reference smooth2 droplet IC)",
+        "config": {"workload": f"{n - 2}^3 droplet-on-wall splash, 6 mm cube, r=1 mm, v=-3 m/s, CSF, "
+                               f"each step's Poisson solve to {args.tol:g} relative residual",
+                   "grid_with_ghosts": [n, n, n], "solver": "FP64 PCG + FP32 V(2,2) preconditioner",
+                   "initial_guess": "previous step pressure", "parallelism": "single GPU"},
+        "mg_iters_per_step": float(np.mean(tgt["iters"])),
+        "final_relres": tgt["relres"],
+        "volume_drift_rel": (tgt["vol1"] - tgt["vol0"]) / tgt["vol0"],
+        "roofline": {"kernel": "k_gs_colour<float> (tolerance-mode red-black Gauss-Seidel colour pass)",
+                     "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "avg_launch_us": 1e3 * avg_ms, "launches": pr["launches"],
+                     "algorithmic_bytes_per_launch": pr["bytes"] / max(pr["launches"], 1)},
+    }
+    if den is not None:
+        dv = den["steps"] / den["seconds"]
+        line["denominator_fp64"] = {"value": dv, "ms_per_step": 1e3 * den["seconds"] / den["steps"],
+                                    "mg_iters_per_step": float(np.mean(den["iters"]))}
+        line["speedup_vs_fp64"] = value / dv
+    line.update(out)
+    if not args.no_cpu_baseline:
+        v, dt = cpu_baseline(n, args.cpu_steps, args.tol)
+        line["cpu_baseline"] = {"value": v, "unit": "time steps/s", "cores": 1, "kind": "port",
+                                "sample": f"{args.cpu_steps} step(s) of the same {n - 2}^3 workload, FP64 PCG + "
+                                          f"FP64 V-cycle (the denominator's algorithm), {dt:.1f} s"}
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

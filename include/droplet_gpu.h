@@ -125,6 +125,16 @@ int dg_poisson_solve(const int *nx, const int *ny, const int *nz, const double *
                      const double *a, const double *b, const double *c, double *p, const int *mode,
                      const double *tol, const int *maxit, int *iters, double *relres);
 
+/* ---- measurement (no reference counterpart) ---------------------------- */
+/* Times every launch of one kernel family with HIP events recorded on the
+ * library's stream until dg_profile_stop, which returns the summed kernel
+ * time, the launch count and the summed algorithmic bytes of those launches.
+ * kernel ids: 1 tolerance-mode Gauss-Seidel colour pass, 2 its residual,
+ * 3 PCG direction + A.p, 4 PCG update, 5 reference relax colour pass,
+ * 6 fused two-colour smoother sweep. */
+int dg_profile_start(const int *kernel_id);
+int dg_profile_stop(double *total_ms, int *launches, double *total_bytes);
+
 #ifdef __cplusplus
 }
 #endif

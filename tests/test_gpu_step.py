@@ -62,7 +62,9 @@ def test_step_parity_reference_schedule(dg, name, kw, nsteps):
     txt = inputdata_text(**kw)
     o, g, hist, errs = run_pair(dg, txt, nsteps)
     for (t, tau_o, tau_g, ts_o, ts_g, nt_o, nt_g, vol_o, vol_g) in hist:
-        assert tau_o == tau_g, (t, tau_o, tau_g)
+        # tau follows max|u| (checktime): last-bit differences of the device
+        # pow/acos/cos in AL3D reach it, so it is held to 1e-12, not bits
+        assert abs(tau_o - tau_g) <= 1e-12 * tau_o, (t, tau_o, tau_g)
         assert ts_o == ts_g and nt_o == nt_g
         assert abs(vol_o - vol_g) <= 1e-12 * abs(vol_o), (t, vol_o, vol_g)
     print({k: f"{v:.2e}" for k, v in errs.items()}, flush=True)

@@ -253,3 +253,29 @@ def poisson_solve(rhs, a, b, c, p0=None, lengthsize=1.0, mode=2, tol=1e-8, maxit
                                   _i(mode), _d(tol), _i(maxit), ctypes.byref(it), ctypes.byref(rel)),
            "dg_poisson_solve")
     return p, it.value, rel.value
+
+
+def set_tuning(fused=False, graphs=True):
+    """Tolerance-mode execution switches (results are bit-identical):
+    fused -- one-launch two-colour smoother sweep instead of two colour passes;
+    graphs -- replay each PCG iteration as a captured HIP graph."""
+    _check(lib().dg_set_tuning(_i(1 if fused else 0), _i(1 if graphs else 0)), "dg_set_tuning")
+
+
+# kernel families dg_profile_start accepts
+PROFILE_KERNELS = {"gs_colour": 1, "resid": 2, "pcg_dir": 3, "pcg_update": 4, "relax": 5, "sweep": 6}
+
+
+def profile_start(kernel):
+    """Start HIP-event timing of every launch of one kernel family on the
+    library's stream (graph replay is off meanwhile)."""
+    kid = PROFILE_KERNELS[kernel] if isinstance(kernel, str) else int(kernel)
+    _check(lib().dg_profile_start(ctypes.byref(ctypes.c_int(kid))), "dg_profile_start")
+
+
+def profile_stop():
+    """Stop timing; returns dict(total_ms, launches, bytes) where bytes is the
+    algorithmic byte count of the timed launches."""
+    ms, nl, by = ctypes.c_double(), ctypes.c_int(), ctypes.c_double()
+    _check(lib().dg_profile_stop(ctypes.byref(ms), ctypes.byref(nl), ctypes.byref(by)), "dg_profile_stop")
+    return dict(total_ms=ms.value, launches=nl.value, bytes=by.value)

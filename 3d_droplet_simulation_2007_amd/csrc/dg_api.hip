@@ -46,6 +46,7 @@ Prof &prof()
 }
 
 static bool g_ready = false;
+static int g_fused_default = 0, g_graphs_default = 1;
 
 // ---- allocation -----------------------------------------------------------
 
@@ -105,8 +106,10 @@ static int alloc_levels(Ctx &c, int nx, int ny, int nz, bool strict)
     c.g = c.lev[ng - 1].g;
     // reduction partials: RED_BLOCKS pairs or two per interior block
     const dim3 ig = interior_grid(c.g);
+    // two values per interior block, then room for the second-stage sums
     size_t np = 2 * (size_t)ig.x * ig.y * ig.z;
     if (np < 2 * (size_t)RED_BLOCKS) np = 2 * RED_BLOCKS;
+    np += np / 512 + 64;
     if (dalloc(&c.red_partial, np)) return -1;
     if (dalloc(&c.dscal, 64)) return -1;
     DG_HIP(hipHostMalloc((void **)&c.hscal, 64 * sizeof(double), hipHostMallocDefault));
@@ -130,7 +133,7 @@ static void free_ctx(Ctx &c)
     for (int l = 0; l < c.ng; l++) {
         Level &L = c.lev[l];
         void *ps[] = {L.rhs, L.res, L.u, L.cc, L.a, L.b, L.c, L.ei, L.a32, L.b32, L.c32,
-                      L.u32, L.f32, L.r32, L.u64, L.f64, L.r64};
+                      L.u32, L.f32, L.r32, L.u64, L.f64, L.r64, L.w32, L.w64};
         for (void *p : ps)
             if (p) (void)hipFree(p);
         L = Level();
@@ -139,6 +142,8 @@ static void free_ctx(Ctx &c)
                   c.eps, c.cmu, c.ccf, c.pprev, c.pcg_r, c.pcg_z, c.pcg_p, c.pcg_q, c.red_partial, c.dscal};
     for (void *p : fs)
         if (p) (void)hipFree(p);
+    for (int q = 0; q < 2; q++)
+        if (c.pcg_graph[q]) (void)hipGraphExecDestroy(c.pcg_graph[q]);
     if (c.hscal) (void)hipHostFree(c.hscal);
     if (c.stream) (void)hipStreamDestroy(c.stream);
     c = Ctx();
@@ -369,6 +374,8 @@ int dg_init(const char *path)
     if (check_dims(in.nx, in.ny, in.nz)) return -1;
     if (alloc_levels(c, in.nx, in.ny, in.nz, true)) return -1;
     if (alloc_fields(c)) return -1;
+    c.fused_smoother = g_fused_default;
+    c.use_graphs = g_graphs_default;
     nondim(in, c.s);
     // cmu = 0.09 (main.f:347-353); mut, turbst and the Reynolds stresses start at 0
     if (mg_fill(c.g.n(), c.stream, c.cmu, 0.09)) return -1;
@@ -516,6 +523,16 @@ int dg_set_ordered_sums(const int *on)
 {
     if (require_ready()) return -1;
     ctx()->ordered_sums = *on ? 1 : 0;
+    return 0;
+}
+
+int dg_set_tuning(const int *fused, const int *graphs)
+{
+    g_fused_default = *fused ? 1 : 0;
+    g_graphs_default = *graphs ? 1 : 0;
+    Ctx &c = *ctx();
+    c.fused_smoother = g_fused_default;
+    c.use_graphs = g_graphs_default;
     return 0;
 }
 
@@ -709,6 +726,8 @@ int dg_poisson_solve(const int *nx, const int *ny, const int *nz, const double *
     if (rc) { set_error("dg_poisson_solve: upload failed"); free_ctx(lc); return -1; }
     lc.s.lengthsize = *lengthsize;
     lc.mg_mode = *mode; lc.mg_tol = *tol; lc.mg_maxit = *maxit; lc.mg_guess = 1;
+    lc.fused_smoother = g_fused_default;
+    lc.use_graphs = g_graphs_default;
     if (op_mg_solve_tol(lc)) { free_ctx(lc); return -1; }
     if (hipStreamSynchronize(lc.stream) != hipSuccess ||
         hipMemcpy(p, F.u, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
@@ -739,17 +758,33 @@ int dg_profile_start(const int *kernel_id)
 int dg_profile_stop(double *total_ms, int *launches, double *total_bytes)
 {
     Prof &p = prof();
-    DG_HIP(hipDeviceSynchronize());
+    const int id = p.kernel;
+    p.kernel = PK_NONE;
+    *total_ms = 0.0;
+    *launches = 0;
+    *total_bytes = 0.0;
+    hipError_t e0 = hipDeviceSynchronize();
+    if (e0 != hipSuccess) {
+        set_error("dg_profile_stop(kernel %d): hipDeviceSynchronize: %s", id, hipGetErrorString(e0));
+        p.used = 0;
+        return -1;
+    }
     double ms = 0.0;
     for (size_t q = 0; q + 1 < p.used; q += 2) {
         float e = 0.0f;
-        DG_HIP(hipEventElapsedTime(&e, p.ev[q], p.ev[q + 1]));
+        hipError_t r = hipEventSynchronize(p.ev[q + 1]);
+        if (r == hipSuccess) r = hipEventElapsedTime(&e, p.ev[q], p.ev[q + 1]);
+        if (r != hipSuccess) {
+            set_error("dg_profile_stop(kernel %d): event pair %zu of %zu: %s", id, q / 2, p.used / 2,
+                      hipGetErrorString(r));
+            p.used = 0;
+            return -1;
+        }
         ms += e;
     }
     *total_ms = ms;
     *launches = (int)p.launches;
     *total_bytes = p.bytes;
-    p.kernel = PK_NONE;
     p.used = 0;
     return 0;
 }

@@ -18,8 +18,8 @@ struct Level {
     double *a = nullptr, *b = nullptr, *c = nullptr, *ei = nullptr;
     // tolerance-mode work arrays
     float *a32 = nullptr, *b32 = nullptr, *c32 = nullptr;
-    float *u32 = nullptr, *f32 = nullptr, *r32 = nullptr;
-    double *u64 = nullptr, *f64 = nullptr, *r64 = nullptr;
+    float *u32 = nullptr, *f32 = nullptr, *r32 = nullptr, *w32 = nullptr;
+    double *u64 = nullptr, *f64 = nullptr, *r64 = nullptr, *w64 = nullptr;
 };
 
 struct Scalars {
@@ -53,6 +53,10 @@ struct Ctx {
     int mg_nu1 = 2, mg_nu2 = 2, mg_maxit = 200, mg_guess = 0;
     double mg_tol = 1e-8;
     int ordered_sums = 1;  // reproduce the reference's sequential outflow sums
+    int fused_smoother = 0;  // 1: one-pass red-black sweeps (slower on gfx950 so far), 0: two colour passes
+    int use_graphs = 1;      // replay each PCG iteration as a captured HIP graph
+    hipGraphExec_t pcg_graph[2] = {nullptr, nullptr};
+    int pcg_graph_key[6] = {0, 0, 0, 0, 0, 0};
     // diagnostics of the last step
     int mg_iters_last = 0;
     double mg_relres_last = 0.0, cfl_last = 0.0, epsdiv_last = 0.0;

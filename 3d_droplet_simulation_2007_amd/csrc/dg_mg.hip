@@ -193,8 +193,9 @@ int op_mglin(Ctx &c)
 // neighbour equals the cell itself (the face drops out), across x-max the
 // ghost is 0.
 
-// Gauss-Seidel colour pass with the operator's own diagonal, in place
-template <typename T>
+// Gauss-Seidel colour pass with the operator's own diagonal, in place.
+// TOP only separates the finest level's launches into their own symbol.
+template <typename T, bool TOP>
 __global__ void __launch_bounds__(256) k_gs_colour(Grid g, T *u, const T *f, const T *a, const T *b, const T *c, T h2,
                                                    int colour, int zero_init)
 {
@@ -219,6 +220,111 @@ __global__ void __launch_bounds__(256) k_gs_colour(Grid g, T *u, const T *f, con
     if (k < g.nz - 1) d += ct;
     if (k > 2) d += cb;
     u[o] = (s - h2 * f[o]) / d;
+}
+
+// One full red-black sweep (colour `first` then the other) in a single pass,
+// out of place: u_out = sweep(u_in).  Identical arithmetic to two
+// k_gs_colour launches on one array: the first colour's new values depend
+// only on old second-colour values (read from u_in); the second colour's
+// depend only on new first-colour values, which each block computes itself
+// on its tile plus a one-cell halo ring and exchanges through LDS.
+//
+// Block: 64 x 4 threads over an (x,y) tile of 64 x 16 positions (62 x 14
+// owned columns plus the halo ring), marching up a chunk of RB_KC planes.
+// Each thread owns four consecutive rows of one x column; the rows alternate
+// in colour, so at every plane step p each thread computes two first-colour
+// cells (x,y,p) (written to LDS slot p&1) and, after a barrier, finishes two
+// second-colour cells (x,y,p-1) below them, whose six neighbours are all
+// first-colour cells computed at steps p-2 (registers), p-1 (LDS) and p
+// (registers).  Ghost neighbours are folded as bc_scalar: dropped faces on
+// x-min, y and z, value 0 on x-max.
+constexpr int RB_TX = 64, RB_TY = 16, RB_ROWS = 4, RB_KC = 32;
+
+template <typename T, bool ZERO>
+__device__ __forceinline__ T rb_first(const Grid &g, const T *__restrict__ u_in, const T *__restrict__ f,
+                                      const T *__restrict__ a, const T *__restrict__ b, const T *__restrict__ c, T h2,
+                                      int x, int y, int p)
+{
+    const int sy = g.sy, sz = g.sz;
+    const int o = g.at(x, y, p);
+    const T ae = a[o], aw = a[o - 1], bn = b[o], bs = b[o - sy], ct = c[o], cb = c[o - sz];
+    T s = (T)0, d = ae;
+    if (!ZERO) {
+        if (x < g.nx - 1) s += ae * u_in[o + 1];
+        if (x > 2) s += aw * u_in[o - 1];
+        if (y < g.ny - 1) s += bn * u_in[o + sy];
+        if (y > 2) s += bs * u_in[o - sy];
+        if (p < g.nz - 1) s += ct * u_in[o + sz];
+        if (p > 2) s += cb * u_in[o - sz];
+    }
+    if (x > 2) d += aw;
+    if (y < g.ny - 1) d += bn;
+    if (y > 2) d += bs;
+    if (p < g.nz - 1) d += ct;
+    if (p > 2) d += cb;
+    return (s - h2 * f[o]) / d;
+}
+
+template <typename T, bool ZERO>
+__global__ void __launch_bounds__(RB_TX *(RB_TY / RB_ROWS)) k_rbgs_sweep(
+    Grid g, const T *__restrict__ u_in, T *__restrict__ u_out, const T *__restrict__ f, const T *__restrict__ a,
+    const T *__restrict__ b, const T *__restrict__ c, T h2, int first)
+{
+    __shared__ T lds[2][RB_TY][RB_TX];
+    const int tx = (int)threadIdx.x, ty0 = (int)threadIdx.y * RB_ROWS;
+    const int x = 2 + (RB_TX - 2) * (int)blockIdx.x - 1 + tx;
+    const int ybase = 2 + (RB_TY - 2) * (int)blockIdx.y - 1 + ty0;
+    const int k0 = 2 + RB_KC * (int)blockIdx.z;
+    const int kend = min(k0 + RB_KC - 1, g.nz - 1);
+    const bool xin = x >= 2 && x <= g.nx - 1;
+    const bool xown = xin && tx >= 1 && tx <= RB_TX - 2;
+    const int sy = g.sy, sz = g.sz;
+    T a_prev[RB_ROWS], a_new[RB_ROWS];
+#pragma unroll
+    for (int r = 0; r < RB_ROWS; r++) a_prev[r] = (T)0;
+    for (int p = k0 - 1; p <= kend + 1; p++) {
+        const bool pin = p >= 2 && p <= g.nz - 1;
+#pragma unroll
+        for (int r = 0; r < RB_ROWS; r++) {
+            const int y = ybase + r;
+            a_new[r] = (T)0;
+            const bool is_first = ((x + y + p) & 1) == first;
+            if (xin && y >= 2 && y <= g.ny - 1 && pin && is_first) {
+                a_new[r] = rb_first<T, ZERO>(g, u_in, f, a, b, c, h2, x, y, p);
+                lds[p & 1][ty0 + r][tx] = a_new[r];
+                const bool own = xown && ty0 + r >= 1 && ty0 + r <= RB_TY - 2;
+                if (own && p >= k0 && p <= kend) u_out[g.at(x, y, p)] = a_new[r];
+            }
+        }
+        __syncthreads();
+        const int q = p - 1;
+#pragma unroll
+        for (int r = 0; r < RB_ROWS; r++) {
+            const int y = ybase + r, ty = ty0 + r;
+            const bool is_first = ((x + y + p) & 1) == first;
+            const bool own = xown && y >= 2 && y <= g.ny - 1 && ty >= 1 && ty <= RB_TY - 2;
+            if (own && is_first && q >= k0 && q <= kend) {
+                // (x,y,q) has the second colour; its neighbours have the first
+                const int o = g.at(x, y, q);
+                const T ae = a[o], aw = a[o - 1], bn = b[o], bs = b[o - sy], ct = c[o], cb = c[o - sz];
+                T s = (T)0, d = ae;
+                if (x < g.nx - 1) s += ae * lds[q & 1][ty][tx + 1];
+                if (x > 2) s += aw * lds[q & 1][ty][tx - 1];
+                if (y < g.ny - 1) s += bn * lds[q & 1][ty + 1][tx];
+                if (y > 2) s += bs * lds[q & 1][ty - 1][tx];
+                if (q < g.nz - 1) s += ct * a_new[r];
+                if (q > 2) s += cb * a_prev[r];
+                if (x > 2) d += aw;
+                if (y < g.ny - 1) d += bn;
+                if (y > 2) d += bs;
+                if (q < g.nz - 1) d += ct;
+                if (q > 2) d += cb;
+                u_out[o] = (s - h2 * f[o]) / d;
+            }
+            if (is_first) a_prev[r] = a_new[r];
+        }
+        __syncthreads();
+    }
 }
 
 // r = f - A u
@@ -358,34 +464,70 @@ static inline double lev_h2i(const Ctx &c, const Grid &g)
 }
 
 template <typename T>
-static void level_ptrs(Level &L, T *&u, T *&f, T *&r, const T *&a, const T *&b, const T *&cc);
+static void level_ptrs(Level &L, T *&u, T *&f, T *&r, const T *&a, const T *&b, const T *&cc, T *&w);
 template <>
 void level_ptrs<double>(Level &L, double *&u, double *&f, double *&r, const double *&a, const double *&b,
-                        const double *&cc)
+                        const double *&cc, double *&w)
 {
-    u = L.u64; f = L.f64; r = L.r64; a = L.a; b = L.b; cc = L.c;
+    u = L.u64; f = L.f64; r = L.r64; a = L.a; b = L.b; cc = L.c; w = L.w64;
 }
 template <>
-void level_ptrs<float>(Level &L, float *&u, float *&f, float *&r, const float *&a, const float *&b, const float *&cc)
+void level_ptrs<float>(Level &L, float *&u, float *&f, float *&r, const float *&a, const float *&b, const float *&cc,
+                       float *&w)
 {
-    u = L.u32; f = L.f32; r = L.r32; a = L.a32; b = L.b32; cc = L.c32;
+    u = L.u32; f = L.f32; r = L.r32; a = L.a32; b = L.b32; cc = L.c32; w = L.w32;
+}
+
+// one fused red-black sweep u_out = sweep(u_in); the finest level's launches
+// are the profiled kernel family PK_SWEEP
+template <typename T>
+static int fused_sweep(Ctx &c, const Grid &g, const T *u_in, T *u_out, const T *f, const T *a, const T *b,
+                       const T *cc, T h2, int first, bool zero, bool top)
+{
+    dim3 grid((unsigned)((g.nx - 2 + RB_TX - 3) / (RB_TX - 2)), (unsigned)((g.ny - 2 + RB_TY - 3) / (RB_TY - 2)),
+              (unsigned)((g.nz - 2 + RB_KC - 1) / RB_KC));
+    // algorithmic bytes of one red-black sweep: 7 words per cell (relax:
+    // a,b,c,ei,rhs,u -> u)
+    const double bytes = 7.0 * (double)sizeof(T) * (double)(g.nx - 2) * (g.ny - 2) * (g.nz - 2);
+    const bool profiled = top && !zero;  // the zero-guess sweep reads no u and is a different kernel
+    if (profiled) prof_begin(PK_SWEEP, c.stream);
+    if (zero)
+        hipLaunchKernelGGL((k_rbgs_sweep<T, true>), grid, dim3(RB_TX, RB_TY / RB_ROWS), 0, c.stream, g, u_in, u_out, f, a, b, cc,
+                           h2, first);
+    else
+        hipLaunchKernelGGL((k_rbgs_sweep<T, false>), grid, dim3(RB_TX, RB_TY / RB_ROWS), 0, c.stream, g, u_in, u_out, f, a, b,
+                           cc, h2, first);
+    if (profiled) prof_end(PK_SWEEP, c.stream, bytes);
+    DG_CHECK_LAUNCH();
+    return 0;
 }
 
 template <typename T>
 static int gs_sweep(Ctx &c, const Grid &g, T *u, const T *f, const T *a, const T *b, const T *cc, T h2, int first,
-                    int second, int zero_first)
+                    int second, int zero_first, bool top)
 {
     const int half = (g.nx - 2 + 1) / 2;
     dim3 grid((unsigned)((half + BX - 1) / BX), (unsigned)((g.ny - 2 + BY - 1) / BY), (unsigned)(g.nz - 2));
     // algorithmic bytes: one red-black sweep moves 7 words per cell (relax's
     // a,b,c,ei,rhs,u -> u); one colour pass is half a sweep
     const double bytes = 3.5 * (double)sizeof(T) * (double)(g.nx - 2) * (g.ny - 2) * (g.nz - 2);
+    if (!top) {
+        hipLaunchKernelGGL((k_gs_colour<T, false>), grid, interior_block(), 0, c.stream, g, u, f, a, b, cc, h2, first,
+                           zero_first);
+        DG_CHECK_LAUNCH();
+        hipLaunchKernelGGL((k_gs_colour<T, false>), grid, interior_block(), 0, c.stream, g, u, f, a, b, cc, h2, second,
+                           0);
+        DG_CHECK_LAUNCH();
+        return 0;
+    }
+    // the finest level: the profiled kernel family PK_GS_COLOUR
     prof_begin(PK_GS_COLOUR, c.stream);
-    hipLaunchKernelGGL(k_gs_colour<T>, grid, interior_block(), 0, c.stream, g, u, f, a, b, cc, h2, first, zero_first);
+    hipLaunchKernelGGL((k_gs_colour<T, true>), grid, interior_block(), 0, c.stream, g, u, f, a, b, cc, h2, first,
+                       zero_first);
     prof_end(PK_GS_COLOUR, c.stream, bytes);
     DG_CHECK_LAUNCH();
     prof_begin(PK_GS_COLOUR, c.stream);
-    hipLaunchKernelGGL(k_gs_colour<T>, grid, interior_block(), 0, c.stream, g, u, f, a, b, cc, h2, second, 0);
+    hipLaunchKernelGGL((k_gs_colour<T, true>), grid, interior_block(), 0, c.stream, g, u, f, a, b, cc, h2, second, 0);
     prof_end(PK_GS_COLOUR, c.stream, bytes);
     DG_CHECK_LAUNCH();
     return 0;
@@ -397,9 +539,9 @@ static int vcycle(Ctx &c, int l)
 {
     Level &L = c.lev[l];
     const Grid g = L.g;
-    T *u, *f, *r;
+    T *u, *f, *r, *w;
     const T *a, *b, *cc;
-    level_ptrs<T>(L, u, f, r, a, b, cc);
+    level_ptrs<T>(L, u, f, r, a, b, cc, w);
     if (l == 0) {
         hipLaunchKernelGGL(k_coarse_solve<T>, dim3(1), dim3(64), 0, c.stream, g, g_tw.M, f, u);
         DG_CHECK_LAUNCH();
@@ -407,27 +549,56 @@ static int vcycle(Ctx &c, int l)
     }
     const double h2i = lev_h2i(c, g);
     const T h2 = (T)(1.0 / h2i), th2i = (T)h2i;
-    for (int sidx = 0; sidx < c.mg_nu1; sidx++)
-        if (gs_sweep<T>(c, g, u, f, a, b, cc, h2, 0, 1, sidx == 0)) return -1;
+    const bool top = l == c.ng - 1;
+    // cur: buffer holding the level's iterate.  With the fused (out-of-place)
+    // sweeps the buffers alternate; the first one is chosen so that the last
+    // post-smoothing sweep lands in u, where the caller reads the result.
+    T *cur = u;
+    if (c.fused_smoother) {
+        T *target = (c.mg_nu2 & 1) ? w : u;  // buffer that must hold the pre-smoothed iterate
+        T *other = target == u ? w : u;
+        cur = (c.mg_nu1 & 1) ? target : other;  // the first sweep writes here
+        if (c.mg_nu1 == 0) cur = target;
+        T *prev = cur == u ? w : u;  // unread by the zero-guess sweep
+        for (int sidx = 0; sidx < c.mg_nu1; sidx++) {
+            if (sidx > 0) { T *t = prev; prev = cur; cur = t; }
+            if (fused_sweep<T>(c, g, prev, cur, f, a, b, cc, h2, 0, sidx == 0, top)) return -1;
+        }
+    } else {
+        for (int sidx = 0; sidx < c.mg_nu1; sidx++)
+            if (gs_sweep<T>(c, g, u, f, a, b, cc, h2, 0, 1, sidx == 0, top)) return -1;
+    }
     if (c.mg_nu1 == 0) {
-        hipLaunchKernelGGL(k_zero<T>, dim3((unsigned)((g.n() + 255) / 256)), dim3(256), 0, c.stream, g.n(), u);
+        hipLaunchKernelGGL(k_zero<T>, dim3((unsigned)((g.n() + 255) / 256)), dim3(256), 0, c.stream, g.n(), cur);
         DG_CHECK_LAUNCH();
     }
     prof_begin(PK_RESID, c.stream);
-    hipLaunchKernelGGL(k_resid_fold<T>, interior_grid(g), interior_block(), 0, c.stream, g, r, u, f, a, b, cc, th2i);
+    hipLaunchKernelGGL(k_resid_fold<T>, interior_grid(g), interior_block(), 0, c.stream, g, r, cur, f, a, b, cc, th2i);
     prof_end(PK_RESID, c.stream, 6.0 * (double)sizeof(T) * (double)(g.nx - 2) * (g.ny - 2) * (g.nz - 2));
     DG_CHECK_LAUNCH();
     Level &C = c.lev[l - 1];
-    T *uc, *fc, *rc;
+    T *uc, *fc, *rc, *wc;
     const T *ac, *bc, *ccc;
-    level_ptrs<T>(C, uc, fc, rc, ac, bc, ccc);
+    level_ptrs<T>(C, uc, fc, rc, ac, bc, ccc, wc);
     hipLaunchKernelGGL(k_restrict_avg<T>, interior_grid(C.g), interior_block(), 0, c.stream, C.g, g, fc, r);
     DG_CHECK_LAUNCH();
     if (vcycle<T>(c, l - 1)) return -1;
-    hipLaunchKernelGGL(k_prolong_inj_add<T>, interior_grid(g), interior_block(), 0, c.stream, g, C.g, u, uc);
+    hipLaunchKernelGGL(k_prolong_inj_add<T>, interior_grid(g), interior_block(), 0, c.stream, g, C.g, cur, uc);
     DG_CHECK_LAUNCH();
-    for (int sidx = 0; sidx < c.mg_nu2; sidx++)
-        if (gs_sweep<T>(c, g, u, f, a, b, cc, h2, 1, 0, 0)) return -1;
+    if (c.fused_smoother) {
+        for (int sidx = 0; sidx < c.mg_nu2; sidx++) {
+            T *nxt = cur == u ? w : u;
+            if (fused_sweep<T>(c, g, cur, nxt, f, a, b, cc, h2, 1, false, top)) return -1;
+            cur = nxt;
+        }
+        if (cur != u) {
+            set_error("V-cycle buffer bookkeeping");
+            return -1;
+        }
+    } else {
+        for (int sidx = 0; sidx < c.mg_nu2; sidx++)
+            if (gs_sweep<T>(c, g, u, f, a, b, cc, h2, 1, 0, 0, top)) return -1;
+    }
     return 0;
 }
 
@@ -551,6 +722,35 @@ __global__ void k_final_sum(const double *partial, int nb, double *out)
     if (threadIdx.x == 0) *out = s1;
 }
 
+// first stage of a deterministic two-stage sum of `stride`-interleaved
+// partials: block b sums entries [b*RP_CHUNK, (b+1)*RP_CHUNK) of each lane
+constexpr int RP_CHUNK = 2048;
+__global__ void __launch_bounds__(256) k_partial_chunks(const double *partial, int nb, int stride, double *out)
+{
+    __shared__ double lds4[4];
+    const int lo = (int)blockIdx.x * RP_CHUNK, hi = min(lo + RP_CHUNK, nb);
+    for (int comp = 0; comp < stride; comp++) {
+        double a1 = 0.0;
+        for (int p = lo + (int)threadIdx.x; p < hi; p += 256) a1 += partial[stride * p + comp];
+        double s1 = block_reduce256<false>(a1, lds4);
+        if (threadIdx.x == 0) out[stride * blockIdx.x + comp] = s1;
+    }
+}
+
+// sums nb partials (stride 1) or nb pairs (stride 2) into out[0..stride)
+static int sum_partials(hipStream_t s, const double *partial, int nb, int stride, double *scratch, double *out)
+{
+    const int nchunk = (nb + RP_CHUNK - 1) / RP_CHUNK;
+    hipLaunchKernelGGL(k_partial_chunks, dim3((unsigned)nchunk), dim3(256), 0, s, partial, nb, stride, scratch);
+    DG_CHECK_LAUNCH();
+    if (stride == 2)
+        hipLaunchKernelGGL(k_final2, dim3(1), dim3(256), 0, s, scratch, nchunk, out);
+    else
+        hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(256), 0, s, scratch, nchunk, out);
+    DG_CHECK_LAUNCH();
+    return 0;
+}
+
 // scalar bookkeeping after K3: alpha_old = rz/pq, rz_old = rz, first = 0
 __global__ void k_pcg_scalars(double *ds)
 {
@@ -616,6 +816,8 @@ static int ensure_tol_buffers(Ctx &c)
             DG_HIP(hipMalloc(&L.u32, n * sizeof(float)));
             DG_HIP(hipMalloc(&L.f32, n * sizeof(float)));
             DG_HIP(hipMalloc(&L.r32, n * sizeof(float)));
+            DG_HIP(hipMalloc(&L.w32, n * sizeof(float)));
+            DG_HIP(hipMemset(L.w32, 0, n * sizeof(float)));
             DG_HIP(hipMemset(L.u32, 0, n * sizeof(float)));
             DG_HIP(hipMemset(L.f32, 0, n * sizeof(float)));
             DG_HIP(hipMemset(L.r32, 0, n * sizeof(float)));
@@ -624,6 +826,8 @@ static int ensure_tol_buffers(Ctx &c)
             DG_HIP(hipMalloc(&L.u64, n * sizeof(double)));
             DG_HIP(hipMalloc(&L.f64, n * sizeof(double)));
             DG_HIP(hipMalloc(&L.r64, n * sizeof(double)));
+            DG_HIP(hipMalloc(&L.w64, n * sizeof(double)));
+            DG_HIP(hipMemset(L.w64, 0, n * sizeof(double)));
             DG_HIP(hipMemset(L.u64, 0, n * sizeof(double)));
             DG_HIP(hipMemset(L.f64, 0, n * sizeof(double)));
             DG_HIP(hipMemset(L.r64, 0, n * sizeof(double)));
@@ -655,6 +859,86 @@ static int ensure_tol_buffers(Ctx &c)
     return 0;
 }
 
+// Enqueues one PCG iteration on c.stream (no host synchronisation): z = M r
+// (one V-cycle), r.z and z.q_old, p_new = z + beta p_old and q = A p_new,
+// p.q, x += alpha p, r -= alpha q, r.r, scalar bookkeeping, and the copy of
+// r.r to pinned host memory.  pa/pb: search-direction ping-pong buffers.
+template <typename T>
+static int pcg_iteration(Ctx &c, double *pa, double *pb)
+{
+    hipStream_t s = c.stream;
+    Level &F = c.lev[c.ng - 1];
+    const Grid g = F.g;
+    const double h2i = lev_h2i(c, g);
+    double *x = F.u, *r = c.pcg_r, *q = c.pcg_q, *ds = c.dscal, *partial = c.red_partial;
+    const dim3 ig = interior_grid(g), ib = interior_block();
+    const int nblk = (int)(ig.x * ig.y * ig.z);
+    T *u, *f, *rr_, *w_;
+    const T *a_, *b_, *c_;
+    level_ptrs<T>(F, u, f, rr_, a_, b_, c_, w_);
+    if (sizeof(T) == 4)
+        hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((g.n() + 255) / 256)), dim3(256), 0, s, g.n(), r, (float *)f);
+    else
+        hipLaunchKernelGGL(k_copy_cast, dim3((unsigned)((g.n() + 255) / 256)), dim3(256), 0, s, g.n(), r, (double *)f);
+    DG_CHECK_LAUNCH();
+    if (vcycle<T>(c, c.ng - 1)) return -1;
+    hipLaunchKernelGGL(k_dot_rz_zq<T>, dim3(RED_BLOCKS), dim3(256), 0, s, g, r, (const T *)u, q, partial);
+    DG_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_final2, dim3(1), dim3(256), 0, s, partial, RED_BLOCKS, ds + S_RZ);
+    DG_CHECK_LAUNCH();
+    const double ncell = (double)(g.nx - 2) * (g.ny - 2) * (g.nz - 2);
+    prof_begin(PK_PCG_DIR, s);
+    hipLaunchKernelGGL(k_pcg_dir<T>, ig, ib, 0, s, g, (const T *)u, pa, pb, q, F.a, F.b, F.c, h2i, ds, partial);
+    prof_end(PK_PCG_DIR, s, ((double)sizeof(T) + 6.0 * 8.0) * ncell);
+    DG_CHECK_LAUNCH();
+    if (sum_partials(s, partial, nblk, 1, partial + 2 * (size_t)nblk, ds + S_PQ)) return -1;
+    prof_begin(PK_PCG_UPDATE, s);
+    hipLaunchKernelGGL(k_pcg_update, ig, ib, 0, s, g, x, r, pb, q, ds, partial);
+    prof_end(PK_PCG_UPDATE, s, 6.0 * 8.0 * ncell);
+    DG_CHECK_LAUNCH();
+    if (sum_partials(s, partial, nblk, 1, partial + 2 * (size_t)nblk, ds + S_RR)) return -1;
+    hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(1), 0, s, ds);
+    DG_CHECK_LAUNCH();
+    DG_HIP(hipMemcpyAsync(c.hscal + S_RR, ds + S_RR, sizeof(double), hipMemcpyDeviceToHost, s));
+    return 0;
+}
+
+// The iteration's kernel sequence and arguments depend only on the level
+// layout, the buffers and the solver options, so it is captured once per
+// (mode, nu1, nu2, fused) and replayed: one graph launch per iteration
+// instead of ~10 launches per level.  Two graphs: the search-direction
+// buffers swap roles every iteration.
+template <typename T>
+static int pcg_graphs(Ctx &c)
+{
+    const int key[6] = {c.mg_mode, c.mg_nu1, c.mg_nu2, c.fused_smoother, c.ng, (int)c.g.nx};
+    bool ok = c.pcg_graph[0] && c.pcg_graph[1];
+    for (int q = 0; q < 6 && ok; q++) ok = c.pcg_graph_key[q] == key[q];
+    if (ok) return 0;
+    for (int q = 0; q < 2; q++) {
+        if (c.pcg_graph[q]) {
+            (void)hipGraphExecDestroy(c.pcg_graph[q]);
+            c.pcg_graph[q] = nullptr;
+        }
+    }
+    for (int par = 0; par < 2; par++) {
+        double *pa = par == 0 ? c.pcg_p : c.pcg_z, *pb = par == 0 ? c.pcg_z : c.pcg_p;
+        hipGraph_t graph;
+        DG_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+        int rc = pcg_iteration<T>(c, pa, pb);
+        hipError_t e = hipStreamEndCapture(c.stream, &graph);
+        if (rc) return -1;
+        if (e != hipSuccess) {
+            set_error("PCG graph capture: %s", hipGetErrorString(e));
+            return -1;
+        }
+        DG_HIP(hipGraphInstantiate(&c.pcg_graph[par], graph, nullptr, nullptr, 0));
+        DG_HIP(hipGraphDestroy(graph));
+    }
+    for (int q = 0; q < 6; q++) c.pcg_graph_key[q] = key[q];
+    return 0;
+}
+
 template <typename T>
 static int pcg_solve(Ctx &c)
 {
@@ -662,14 +946,11 @@ static int pcg_solve(Ctx &c)
     Level &F = c.lev[c.ng - 1];
     const Grid g = F.g;
     const double h2i = lev_h2i(c, g);
-    double *x = F.u, *bvec = F.rhs, *r = c.pcg_r, *q = c.pcg_q;
-    double *pa = c.pcg_p, *pb = c.pcg_z;  // ping-pong search directions
+    double *x = F.u, *bvec = F.rhs, *r = c.pcg_r;
     double *ds = c.dscal;
     const dim3 ig = interior_grid(g), ib = interior_block();
     const int nblk = (int)(ig.x * ig.y * ig.z);
-    // partial buffers: need nblk*2 doubles
     double *partial = c.red_partial;
-    // initial guess
     if (c.mg_guess == 1) {
         DG_HIP(hipMemcpyAsync(x, c.pprev, g.n() * sizeof(double), hipMemcpyDeviceToDevice, s));
     } else {
@@ -677,68 +958,31 @@ static int pcg_solve(Ctx &c)
     }
     hipLaunchKernelGGL(k_init_resid, ig, ib, 0, s, g, r, x, bvec, F.a, F.b, F.c, h2i, partial);
     DG_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_final2, dim3(1), dim3(256), 0, s, partial, nblk, ds + S_RR);  // RR, then PQ slot gets bb
-    DG_CHECK_LAUNCH();
-    DG_HIP(hipMemcpyAsync(c.hscal + S_RR, ds + S_RR, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-    DG_HIP(hipStreamSynchronize(s));
-    double bb = c.hscal[S_RR + 1];
-    double bnorm = sqrt(bb);
-    if (bnorm == 0.0) bnorm = 1.0;
-    double rel = sqrt(c.hscal[S_RR]) / bnorm;
-    // device scalar FIRST = 1
-    {
-        double one = 1.0;
-        DG_HIP(hipMemcpyAsync(ds + S_FIRST, &one, sizeof(double), hipMemcpyHostToDevice, s));
-        DG_HIP(hipStreamSynchronize(s));
-    }
-    // coarse factorisation for this solve
+    if (sum_partials(s, partial, nblk, 2, partial + 2 * (size_t)nblk, ds + S_RR)) return -1;  // r.r and b.b
+    // coarse factorisation for this solve, FIRST flag
     {
         Level &L0 = c.lev[0];
         hipLaunchKernelGGL(k_coarse_factor, dim3(1), dim3(256), 0, s, L0.g, L0.a, L0.b, L0.c, lev_h2i(c, L0.g), g_tw.M,
                            g_tw.info);
         DG_CHECK_LAUNCH();
+        c.hscal[S_FIRST] = 1.0;
+        DG_HIP(hipMemcpyAsync(ds + S_FIRST, c.hscal + S_FIRST, sizeof(double), hipMemcpyHostToDevice, s));
     }
-    Level &top = c.lev[c.ng - 1];
+    DG_HIP(hipMemcpyAsync(c.hscal + S_RR, ds + S_RR, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    DG_HIP(hipStreamSynchronize(s));
+    double bnorm = sqrt(c.hscal[S_RR + 1]);
+    if (bnorm == 0.0) bnorm = 1.0;
+    double rel = sqrt(c.hscal[S_RR]) / bnorm;
+    const bool graphs = c.use_graphs && prof().kernel == PK_NONE;
+    if (graphs && pcg_graphs<T>(c)) return -1;
     int it = 0;
     while (rel >= c.mg_tol && it < c.mg_maxit) {
-        // z = M r
-        T *u, *f, *rr_;
-        const T *a_, *b_, *c_;
-        level_ptrs<T>(top, u, f, rr_, a_, b_, c_);
-        if (sizeof(T) == 4) {
-            hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((g.n() + 255) / 256)), dim3(256), 0, s, g.n(), r,
-                               (float *)f);
-            DG_CHECK_LAUNCH();
+        if (graphs) {
+            DG_HIP(hipGraphLaunch(c.pcg_graph[it & 1], s));
         } else {
-            hipLaunchKernelGGL(k_copy_cast, dim3((unsigned)((g.n() + 255) / 256)), dim3(256), 0, s, g.n(), r,
-                               (double *)f);
-            DG_CHECK_LAUNCH();
+            double *pa = (it & 1) ? c.pcg_z : c.pcg_p, *pb = (it & 1) ? c.pcg_p : c.pcg_z;
+            if (pcg_iteration<T>(c, pa, pb)) return -1;
         }
-        if (vcycle<T>(c, c.ng - 1)) return -1;
-        // rz and zq
-        hipLaunchKernelGGL(k_dot_rz_zq<T>, dim3(RED_BLOCKS), dim3(256), 0, s, g, r, (const T *)u, q, partial);
-        DG_CHECK_LAUNCH();
-        hipLaunchKernelGGL(k_final2, dim3(1), dim3(256), 0, s, partial, RED_BLOCKS, ds + S_RZ);
-        DG_CHECK_LAUNCH();
-        // direction and q = A p
-        const double ncell = (double)(g.nx - 2) * (g.ny - 2) * (g.nz - 2);
-        prof_begin(PK_PCG_DIR, s);
-        hipLaunchKernelGGL(k_pcg_dir<T>, ig, ib, 0, s, g, (const T *)u, pa, pb, q, F.a, F.b, F.c, h2i, ds, partial);
-        prof_end(PK_PCG_DIR, s, ((double)sizeof(T) + 6.0 * 8.0) * ncell);
-        DG_CHECK_LAUNCH();
-        hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(256), 0, s, partial, nblk, ds + S_PQ);
-        DG_CHECK_LAUNCH();
-        // update
-        prof_begin(PK_PCG_UPDATE, s);
-        hipLaunchKernelGGL(k_pcg_update, ig, ib, 0, s, g, x, r, pb, q, ds, partial);
-        prof_end(PK_PCG_UPDATE, s, 6.0 * 8.0 * ncell);
-        DG_CHECK_LAUNCH();
-        hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(256), 0, s, partial, nblk, ds + S_RR);
-        DG_CHECK_LAUNCH();
-        hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(1), 0, s, ds);
-        DG_CHECK_LAUNCH();
-        double *tmp = pa; pa = pb; pb = tmp;
-        DG_HIP(hipMemcpyAsync(c.hscal + S_RR, ds + S_RR, sizeof(double), hipMemcpyDeviceToHost, s));
         DG_HIP(hipStreamSynchronize(s));
         rel = sqrt(c.hscal[S_RR]) / bnorm;
         it++;
@@ -755,15 +999,7 @@ int op_mg_solve_tol(Ctx &c)
     hipStream_t s = c.stream;
     if (ensure_tol_buffers(c)) return -1;
     const Grid g = c.g;
-    // partial-sum buffer must hold 2 values per interior block
-    {
-        const dim3 ig = interior_grid(g);
-        const size_t need = 2 * (size_t)ig.x * ig.y * ig.z;
-        if (need > 2 * (size_t)RED_BLOCKS * 64) {
-            set_error("grid too large for the reduction buffer");
-            return -1;
-        }
-    }
+    (void)g;  // the partial-sum buffer is sized for this grid by alloc_levels
     for (int gi = c.ng - 1; gi >= 1; gi--) {
         Level &C = c.lev[gi - 1], &F = c.lev[gi];
         if (mg_acfromac(C.g, s, C.a, C.b, C.c, C.ei, F.a, F.b, F.c)) return -1;

@@ -38,20 +38,18 @@ def splash_inputdata(dg, n):
                              length=0.006, radius=0.001, y0=0.0025, z0=1000.0, cycles=2, npre=2, ntop=4, npost=2)
 
 
-def run_path(dg, txt, mode, steps, warmup, tol, profile_kernel=None):
+def run_path(dg, txt, mode, steps, warmup, tol, profile_kernel=None, profile_steps=2):
+    """W warm-up steps, K timed steps (headline, no instrumentation), then,
+    when profile_kernel is given, profile_steps more steps of the same run
+    with HIP events around every launch of that kernel on the library's
+    stream (graph replay is off while events are recorded)."""
     g = dg.DropletGPU(inputdata_text=txt)
     g.clip_cc()
     g.set_ordered_sums(False)
     g.set_solver(mode=mode, tol=tol, nu1=2, nu2=2, maxit=200, guess=1)
     vol0 = float(g.get("cc")[1:-1, 1:-1, 1:-1].sum())
-    iters = []
     for t in range(warmup):
         g.step(t)
-        iters.append(g.state()["mg_iters"])
-    lib = dg.lib()
-    import ctypes
-    if profile_kernel is not None:
-        lib.dg_profile_start(ctypes.byref(ctypes.c_int(profile_kernel)))
     g.state()  # synchronises the stream
     t0 = time.perf_counter()
     timed_iters = []
@@ -60,13 +58,16 @@ def run_path(dg, txt, mode, steps, warmup, tol, profile_kernel=None):
         timed_iters.append(g.state()["mg_iters"])
     st = g.state()  # synchronises
     t1 = time.perf_counter()
-    prof = None
-    if profile_kernel is not None:
-        ms, nl, by = ctypes.c_double(), ctypes.c_int(), ctypes.c_double()
-        lib.dg_profile_stop(ctypes.byref(ms), ctypes.byref(nl), ctypes.byref(by))
-        prof = dict(total_ms=ms.value, launches=nl.value, bytes=by.value)
     vol1 = float(g.get("cc")[1:-1, 1:-1, 1:-1].sum())
     p = g.get("p")
+    prof = None
+    if profile_kernel is not None:
+        dg.profile_start(profile_kernel)
+        for t in range(warmup + steps, warmup + steps + profile_steps):
+            g.step(t)
+        prof = dg.profile_stop()
+        if prof["launches"] == 0:
+            raise RuntimeError(f"profiling recorded no launch of kernel family {profile_kernel!r}")
     g.close()
     return dict(seconds=t1 - t0, steps=steps, iters=timed_iters, relres=st["mg_relres"], tau=st["tau"],
                 vol0=vol0, vol1=vol1, prof=prof, p=p)
@@ -93,6 +94,20 @@ def cpu_baseline(n, steps, tol):
     return steps / dt, dt
 
 
+def copy_calibration(mib):
+    """Five plain device-to-device copies of `mib` MiB: under rocprofv3 --pmc
+    their FETCH_SIZE / WRITE_SIZE per dispatch, divided by `mib` MiB, give the
+    factor that turns those counters into HBM bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.rand(n, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 4, "copy_calibration_bytes_written": n * 4}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,7 +118,12 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-denominator", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (scale for HBM byte counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU decomposition is a design in DESIGN.md, not a measured path)", file=sys.stderr)
@@ -112,9 +132,8 @@ def main():
     dg.lib()
     n = args.grid
     txt = splash_inputdata(dg, n)
-    # dominant kernel: the finest-level FP32 Gauss-Seidel colour pass (id 1
-    # covers every level; the finest carries ~7/8 of its bytes)
-    tgt = run_path(dg, txt, 2, args.steps, args.warmup, args.tol, profile_kernel=1)
+    # dominant kernel: the finest-level FP32 red-black Gauss-Seidel colour pass
+    tgt = run_path(dg, txt, 2, args.steps, args.warmup, args.tol, profile_kernel="gs_colour")
     import numpy as np
     out = {}
     den = None
@@ -146,7 +165,7 @@ def main():
         "mg_iters_per_step": float(np.mean(tgt["iters"])),
         "final_relres": tgt["relres"],
         "volume_drift_rel": (tgt["vol1"] - tgt["vol0"]) / tgt["vol0"],
-        "roofline": {"kernel": "k_gs_colour<float> (tolerance-mode red-black Gauss-Seidel colour pass)",
+        "roofline": {"kernel": "k_gs_colour<float,true> (finest-level red-black Gauss-Seidel colour pass, FP32)",
                      "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "avg_launch_us": 1e3 * avg_ms, "launches": pr["launches"],

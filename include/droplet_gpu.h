@@ -88,6 +88,12 @@ int dg_set_solver(const int *mode, const double *tol, const int *nu1, const int 
 /* 1 (default): outflow sums in the reference's sequential order
  * (main.f:4123-4131); 0: parallel tree sums (different rounding). */
 int dg_set_ordered_sums(const int *on);
+/* Tolerance-mode execution switches, results are bit-identical either way:
+ * fused = 1 (default) one out-of-place pass per red-black sweep, 0 two
+ * in-place colour passes; graphs = 1 (default) each PCG iteration replayed
+ * as a captured HIP graph, 0 launched kernel by kernel.  Applies to the
+ * resident state and to later dg_poisson_solve calls. */
+int dg_set_tuning(const int *fused, const int *graphs);
 
 /* ---- host-pointer kernels with the reference's argument lists --------- */
 /* Each uploads its arrays, runs on the GPU and downloads the outputs. */

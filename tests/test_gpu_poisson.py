@@ -1,0 +1,93 @@
+"""Tolerance-mode pressure solve on the GPU (dg_poisson_solve).
+
+System: the reference's operator (resid, main.f:3917-3923) with face
+coefficients 1/rho from a seeded sphere of water in air (abcei,
+main.f:3582-3615) and a right-hand side from the divergence of a smoothed
+random velocity field, as in the isolated Poisson benchmark.
+"""
+import numpy as np
+import pytest
+
+from oracle_ffi import Oracle, OracleKernels as OK, inputdata_text
+
+pytestmark = pytest.mark.gpu
+
+
+def sphere_system(n, seed):
+    """(rhs, a, b, c) on an n^3 grid with ghosts."""
+    rng = np.random.default_rng(seed)
+    m = n - 2
+    h = 1.0 / m
+    r = rng.uniform(0.2, 0.35)
+    ctr = rng.uniform(0.25, 0.75, size=3)
+    x = (np.arange(n) - 1.5) * h
+    # volume fraction by 4^3 sub-sampling per cell
+    cc = np.zeros((n, n, n))
+    sub = (np.arange(4) + 0.5) / 4 * h - 0.5 * h
+    for dz in sub:
+        for dy in sub:
+            for dx in sub:
+                Z, Y, X = np.meshgrid(x + dz, x + dy, x + dx, indexing="ij")
+                cc += ((X - ctr[0]) ** 2 + (Y - ctr[1]) ** 2 + (Z - ctr[2]) ** 2 < r * r)
+    cc /= 64.0
+    a, b, c, ei = (np.zeros((n, n, n)) for _ in range(4))
+    rho2 = 1.2 / 1000.0
+    OK.abcei(a, b, c, ei, cc, 0.5 * (1.0 - rho2), rho2)
+    # smoothed random velocity, rhs = div/h^2 like solvelpres (main.f:3345-3353)
+    vel = [rng.normal(0.0, 0.5, size=(n, n, n)) for _ in range(3)]
+    for q in range(3):
+        for ax in range(3):
+            for _ in range(2):
+                vel[q] = 0.5 * (vel[q] + np.roll(vel[q], 1, axis=ax))
+    u, v, w = vel
+    rhs = np.zeros((n, n, n))
+    rhs[1:-1, 1:-1, 1:-1] = ((u[1:-1, 1:-1, 2:] - u[1:-1, 1:-1, 1:-1]) + (v[1:-1, 2:, 1:-1] - v[1:-1, 1:-1, 1:-1]) +
+                             (w[2:, 1:-1, 1:-1] - w[1:-1, 1:-1, 1:-1])) / (h * h)
+    return rhs, a, b, c
+
+
+def test_tuning_switches_are_bit_identical(dg):
+    print("case tuning 34^3", flush=True)
+    rhs, a, b, c = sphere_system(34, 1)
+    outs = []
+    for fused, graphs in ((True, True), (False, True), (True, False), (False, False)):
+        dg.set_tuning(fused=fused, graphs=graphs)
+        p, it, rel = dg.poisson_solve(rhs, a, b, c, mode=2, tol=1e-9)
+        outs.append((p, it, rel))
+    dg.set_tuning(True, True)
+    for p, it, rel in outs[1:]:
+        assert it == outs[0][1]
+        assert np.array_equal(p, outs[0][0])
+
+
+@pytest.mark.parametrize("n", [34, 66, 130])
+def test_mixed_matches_fp64_and_converges(dg, n):
+    print("case poisson", n, flush=True)
+    rhs, a, b, c = sphere_system(n, 7)
+    p64, it64, rel64 = dg.poisson_solve(rhs, a, b, c, mode=1, tol=1e-8)
+    p32, it32, rel32 = dg.poisson_solve(rhs, a, b, c, mode=2, tol=1e-8)
+    print("iters fp64", it64, "mixed", it32, flush=True)
+    assert rel64 < 1e-8 and rel32 < 1e-8
+    assert it32 <= it64 + 2
+    i64, i32 = p64[1:-1, 1:-1, 1:-1], p32[1:-1, 1:-1, 1:-1]
+    assert np.linalg.norm(i32 - i64) / np.linalg.norm(i64) < 1e-6
+    # the returned residual is the true FP64 residual of the returned p
+    res = np.zeros_like(rhs)
+    OK.resid(1.0, res, p32, rhs, a, b, c)
+    assert np.linalg.norm(res[1:-1, 1:-1, 1:-1]) / np.linalg.norm(rhs[1:-1, 1:-1, 1:-1]) < 2e-8
+
+
+def test_matches_oracle_solve(dg, oracle):
+    print("case poisson vs oracle 34^3", flush=True)
+    o = Oracle(inputdata_text(nx=34, ny=34, nz=34))
+    rhs, a, b, c = sphere_system(34, 3)
+    for name, arr in (("rhs", rhs), ("a", a), ("b", b), ("c", c)):
+        o.field(name)[...] = arr
+    o.st.mg_mode, o.st.mg_outer, o.st.mg_prolong = 1, 1, 0
+    o.st.mg_tol, o.st.mg_nu1, o.st.mg_nu2, o.st.mg_maxcycles, o.st.mg_guess = 1e-10, 2, 2, 100, 0
+    oracle.lib().or_mg_solve_tol(o.h)
+    pg, it, rel = dg.poisson_solve(rhs, a, b, c, mode=1, tol=1e-10)
+    po = o.field("p")
+    err = np.linalg.norm(pg[1:-1, 1:-1, 1:-1] - po[1:-1, 1:-1, 1:-1]) / np.linalg.norm(po[1:-1, 1:-1, 1:-1])
+    assert err < 1e-8, err
+    assert abs(it - o.st.mg_cycles_last) <= 1

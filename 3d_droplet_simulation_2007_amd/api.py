@@ -148,6 +148,11 @@ class DropletGPU:
     def set_ordered_sums(self, on):
         _check(lib().dg_set_ordered_sums(_i(1 if on else 0)), "dg_set_ordered_sums")
 
+    def set_vof_guard(self, on):
+        """Advect partly filled cells whose Youngs normal vanishes with a plane
+        normal to the sweep (the reference evaluates 0/0 there)."""
+        _check(lib().dg_set_vof_guard(_i(1 if on else 0)), "dg_set_vof_guard")
+
     # ---- state exchange
     def get(self, name):
         out = np.empty((self.nz, self.ny, self.nx), dtype=np.float64)

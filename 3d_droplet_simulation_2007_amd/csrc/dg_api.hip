@@ -526,6 +526,13 @@ int dg_set_ordered_sums(const int *on)
     return 0;
 }
 
+int dg_set_vof_guard(const int *on)
+{
+    if (require_ready()) return -1;
+    ctx()->vof_guard = *on ? 1 : 0;
+    return 0;
+}
+
 int dg_set_tuning(const int *fused, const int *graphs)
 {
     g_fused_default = *fused ? 1 : 0;

@@ -53,6 +53,7 @@ struct Ctx {
     int mg_nu1 = 2, mg_nu2 = 2, mg_maxit = 200, mg_guess = 0;
     double mg_tol = 1e-8;
     int ordered_sums = 1;  // reproduce the reference's sequential outflow sums
+    int vof_guard = 0;     // 1: plane normal to the sweep where the Youngs normal vanishes
     int fused_smoother = 0;  // 1: one-pass red-black sweeps (slower on gfx950 so far), 0: two colour passes
     int use_graphs = 1;      // replay each PCG iteration as a captured HIP graph
     hipGraphExec_t pcg_graph[2] = {nullptr, nullptr};

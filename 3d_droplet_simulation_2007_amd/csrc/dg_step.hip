@@ -442,6 +442,7 @@ static int swp(Ctx &c, const double *us, double *cc, double *vof1, double *vof2,
     const double tauh = c.s.tauh;
     const double tiny = DG_F4(1.e-30);
     const int sd = d == 1 ? 1 : (d == 2 ? g.sy : g.sz);
+    const int guard = c.vof_guard;
     if (launch_interior(g, c.stream, [=] __device__(int i, int j, int k) {
             const int o = g.at(i, j, k);
             const double a1 = us[o] * tauh;
@@ -475,6 +476,11 @@ static int swp(Ctx &c, const double *us, double *cc, double *vof1, double *vof2,
                 if (mx < 0.0) { mx = -mx; invx = -1.0; }
                 if (my < 0.0) { my = -my; invy = -1.0; }
                 if (mz < 0.0) { mz = -mz; invz = -1.0; }
+                if (guard && mx + my + mz == 0.0) {
+                    if (d == 1) mx = 1.0;
+                    else if (d == 2) my = 1.0;
+                    else mz = 1.0;
+                }
                 mm2 = mx + my + mz + tiny;
                 mx = mx / mm2;
                 my = my / mm2;

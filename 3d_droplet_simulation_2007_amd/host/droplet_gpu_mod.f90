@@ -68,6 +68,10 @@ module droplet_gpu
       integer(c_int), intent(in) :: mode, nu1, nu2, maxit, guess
       real(c_double), intent(in) :: tol
     end function
+    integer(c_int) function dg_set_vof_guard(on) bind(C, name="dg_set_vof_guard")
+      import :: c_int
+      integer(c_int), intent(in) :: on
+    end function
     type(c_ptr) function dg_last_error() bind(C, name="dg_last_error")
       import :: c_ptr
     end function

@@ -5,7 +5,14 @@ Workload (one MI355X): 258^3 grid with ghosts (256^3 cells) on a 6 mm cube,
 a water droplet (r = 1 mm) in air 1.5 mm above the no-slip wall (reference
 axis y is vertical), impact velocity -3 m/s, gravity -9.81, CSF surface
 tension, laminar.  Synthetic: the droplet initial condition of the
-reference's smooth2, no dataset.
+reference's smooth2, no dataset.  Two deviations from the reference's
+arithmetic keep this workload finite: the initial volume fraction is clipped
+to [0, 1] (smooth2 leaves values above 1 that AL3D turns into NaN), and the
+VOF guard advects partly filled cells whose Youngs normal vanishes with a
+plane normal to the sweep (the reference evaluates 0/0 there; on this
+workload that first happens within the first ten steps).  A timed step whose
+Poisson solve does not reach the tolerance, or non-finite fields after the
+timed steps, abort the run.
 
 Measured paths, each timed over K steps after W warm-up steps:
   target       FP64 fields, PCG with an FP32 V-cycle preconditioner (FP64
@@ -28,6 +35,11 @@ sys.path.insert(0, HERE)
 
 METRIC = "time steps/s at 256^3 to 1e-8 Poisson residual: learned+mixed MG vs FP64 MG"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the dominant kernel at 258^3, from rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE runs of this script scaled by the same counters
+# on a plain 256 MiB copy (fetch x2.0, write x1.0); keyed by kernel and grid
+# extent, re-measured whenever the kernel changes
+PMC_TRAFFIC = {("k_gs_colour<float,true>", 258): 533.9e6}
 
 
 def splash_inputdata(dg, n):
@@ -43,23 +55,35 @@ def run_path(dg, txt, mode, steps, warmup, tol, profile_kernel=None, profile_ste
     when profile_kernel is given, profile_steps more steps of the same run
     with HIP events around every launch of that kernel on the library's
     stream (graph replay is off while events are recorded)."""
+    import numpy as np
     g = dg.DropletGPU(inputdata_text=txt)
     g.clip_cc()
     g.set_ordered_sums(False)
+    g.set_vof_guard(True)
     g.set_solver(mode=mode, tol=tol, nu1=2, nu2=2, maxit=200, guess=1)
     vol0 = float(g.get("cc")[1:-1, 1:-1, 1:-1].sum())
     for t in range(warmup):
         g.step(t)
     g.state()  # synchronises the stream
     t0 = time.perf_counter()
-    timed_iters = []
+    timed = []
     for t in range(warmup, warmup + steps):
         g.step(t)
-        timed_iters.append(g.state()["mg_iters"])
+        s = g.state()
+        timed.append((s["mg_iters"], s["mg_relres"]))
     st = g.state()  # synchronises
     t1 = time.perf_counter()
+    # a step whose solve did not reach the tolerance (NaN residual, zero or
+    # exhausted iterations) makes the measurement invalid
+    bad = [(warmup + q, it, rr) for q, (it, rr) in enumerate(timed) if not (it >= 1 and rr < tol)]
+    if bad:
+        raise RuntimeError(f"mode {mode}: timed steps whose Poisson solve did not converge: {bad}")
     vol1 = float(g.get("cc")[1:-1, 1:-1, 1:-1].sum())
     p = g.get("p")
+    for name in ("u", "v", "w", "cc"):
+        if not np.isfinite(g.get(name)).all():
+            raise RuntimeError(f"mode {mode}: non-finite {name} after the timed steps")
+    timed_iters = [it for it, _ in timed]
     prof = None
     if profile_kernel is not None:
         dg.profile_start(profile_kernel)
@@ -84,7 +108,7 @@ def cpu_baseline(n, steps, tol):
     o = oracle_ffi.Oracle(txt)
     cc = o.field("cc")
     cc[...] = cc.clip(0.0, 1.0)
-    o.st.mg_mode, o.st.mg_outer, o.st.mg_prolong = 1, 1, 0
+    o.st.mg_mode, o.st.mg_outer, o.st.mg_prolong, o.st.vof_guard = 1, 1, 0, 1
     o.st.mg_tol, o.st.mg_nu1, o.st.mg_nu2, o.st.mg_maxcycles, o.st.mg_guess = tol, 2, 2, 200, 1
     t0 = time.perf_counter()
     for t in range(steps):
@@ -167,7 +191,8 @@ def main():
         "volume_drift_rel": (tgt["vol1"] - tgt["vol0"]) / tgt["vol0"],
         "roofline": {"kernel": "k_gs_colour<float,true> (finest-level red-black Gauss-Seidel colour pass, FP32)",
                      "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": PMC_TRAFFIC.get(("k_gs_colour<float,true>", n)),
                      "avg_launch_us": 1e3 * avg_ms, "launches": pr["launches"],
                      "algorithmic_bytes_per_launch": pr["bytes"] / max(pr["launches"], 1)},
     }

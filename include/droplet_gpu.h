@@ -88,8 +88,14 @@ int dg_set_solver(const int *mode, const double *tol, const int *nu1, const int 
 /* 1 (default): outflow sums in the reference's sequential order
  * (main.f:4123-4131); 0: parallel tree sums (different rounding). */
 int dg_set_ordered_sums(const int *on);
+/* 1: a partly filled cell whose Youngs normal vanishes (no filled
+ * neighbour, or a symmetric neighbourhood) is advected with a plane normal
+ * to the sweep direction; the reference's AL3D/FL3D evaluate 0/0 there and
+ * the run turns to NaN (swp main.f:848-1000).  0 (default): the reference's
+ * arithmetic. */
+int dg_set_vof_guard(const int *on);
 /* Tolerance-mode execution switches, results are bit-identical either way:
- * fused = 1 (default) one out-of-place pass per red-black sweep, 0 two
+ * fused = 1 one out-of-place pass per red-black sweep, 0 (default) two
  * in-place colour passes; graphs = 1 (default) each PCG iteration replayed
  * as a captured HIP graph, 0 launched kernel by kernel.  Applies to the
  * resident state and to later dg_poisson_solve calls. */

@@ -472,6 +472,11 @@ static void swp(or_state *st, const double *us, double *cc, double *vof1, double
                     if (mx < 0.0) { mx = -mx; invx = -1.0; }
                     if (my < 0.0) { my = -my; invy = -1.0; }
                     if (mz < 0.0) { mz = -mz; invz = -1.0; }
+                    if (st->vof_guard && mx + my + mz == 0.0) {
+                        if (d == 1) mx = 1.0;
+                        else if (d == 2) my = 1.0;
+                        else mz = 1.0;
+                    }
                     mm2 = mx + my + mz + tiny;
                     mx = mx / mm2;
                     my = my / mm2;

@@ -74,6 +74,11 @@ typedef struct or_state {
      * preconditioned CG; prolong 0 = injection + 8-average restriction,
      * 1 = trilinear + its transpose */
     int mg_outer, mg_prolong;
+    /* 1: a partly filled cell whose Youngs normal vanishes (no filled
+     * neighbour, or a symmetric neighbourhood) is advected with a plane
+     * normal to the sweep direction instead of AL3D/FL3D's 0/0 (NaN) of
+     * swp main.f:848-1000.  0 (default): the reference's arithmetic. */
+    int vof_guard;
 } or_state;
 
 /* field ids shared with the GPU library's dg_field ids */

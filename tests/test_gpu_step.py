@@ -22,9 +22,12 @@ def rel_l2(a, b):
     return np.linalg.norm(a - b) / (den if den > 0 else 1.0)
 
 
-def run_pair(dg, txt, nsteps, mode=0, init=None, tol=1e-8):
+def run_pair(dg, txt, nsteps, mode=0, init=None, tol=1e-8, guard=False):
     o = Oracle(txt)
     g = dg.DropletGPU(inputdata_text=txt)
+    if guard:
+        o.st.vof_guard = 1
+        g.set_vof_guard(True)
     if mode:
         o.st.mg_mode = mode
         o.st.mg_outer = 1
@@ -66,6 +69,24 @@ def test_step_parity_reference_schedule(dg, name, kw, nsteps):
         # pow/acos/cos in AL3D reach it, so it is held to 1e-12, not bits
         assert abs(tau_o - tau_g) <= 1e-12 * tau_o, (t, tau_o, tau_g)
         assert ts_o == ts_g and nt_o == nt_g
+        assert abs(vol_o - vol_g) <= 1e-12 * abs(vol_o), (t, vol_o, vol_g)
+    print({k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    for f, e in errs.items():
+        assert e <= 1e-10, (f, e)
+    g.close()
+
+
+def test_step_parity_vof_guard(dg):
+    """The VOF guard on the case where the reference goes NaN at step 4
+    (34x18x18, upwind, y0 = 2 mm): GPU and restatement agree to the parity
+    bar through the guarded cells, and stay finite."""
+    print("case 34x18x18 upwind y0=2mm with the VOF guard", flush=True)
+    txt = inputdata_text(nx=34, ny=18, nz=18, convectype=1, y0=0.002)
+    o, g, hist, errs = run_pair(dg, txt, 8, guard=True)
+    for f in FIELDS:
+        assert np.isfinite(g.get(f)).all(), f
+    for (t, tau_o, tau_g, ts_o, ts_g, nt_o, nt_g, vol_o, vol_g) in hist:
+        assert abs(tau_o - tau_g) <= 1e-12 * tau_o, (t, tau_o, tau_g)
         assert abs(vol_o - vol_g) <= 1e-12 * abs(vol_o), (t, vol_o, vol_g)
     print({k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
     for f, e in errs.items():

@@ -511,7 +511,8 @@ int dg_set_solver(const int *mode, const double *tol, const int *nu1, const int 
 {
     if (require_ready()) return -1;
     Ctx &c = *ctx();
-    if (*mode < 0 || *mode > 2 || *nu1 < 0 || *nu2 < 0 || *maxit < 1 || !(*tol > 0.0)) {
+    if (*mode < 0 || *mode > 2 || *nu1 < 0 || *nu2 < 0 || *maxit < 1 || !(*tol > 0.0) || *guess < 0 ||
+        *guess > 2) {
         set_error("dg_set_solver: invalid arguments");
         return -1;
     }

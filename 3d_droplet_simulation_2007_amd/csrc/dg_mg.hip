@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "dg_ctx.h"
+#include "dg_nn.h"
 #include "dg_prof.h"
 #include "dg_util.h"
 
@@ -951,7 +952,7 @@ static int pcg_solve(Ctx &c)
     const dim3 ig = interior_grid(g), ib = interior_block();
     const int nblk = (int)(ig.x * ig.y * ig.z);
     double *partial = c.red_partial;
-    if (c.mg_guess == 1) {
+    if (c.mg_guess >= 1) {
         DG_HIP(hipMemcpyAsync(x, c.pprev, g.n() * sizeof(double), hipMemcpyDeviceToDevice, s));
     } else {
         if (mg_fill(g.n(), s, x, 0.0)) return -1;
@@ -959,6 +960,10 @@ static int pcg_solve(Ctx &c)
     hipLaunchKernelGGL(k_init_resid, ig, ib, 0, s, g, r, x, bvec, F.a, F.b, F.c, h2i, partial);
     DG_CHECK_LAUNCH();
     if (sum_partials(s, partial, nblk, 2, partial + 2 * (size_t)nblk, ds + S_RR)) return -1;  // r.r and b.b
+    // learned guess (mg_guess 2) and sample capture start from the
+    // previous-step guess's residual
+    const bool hooks = guess_wanted(c);
+    if (hooks && guess_solve_begin(c, x, r, h2i, c.mg_guess == 2, ds + S_RR)) return -1;
     // coarse factorisation for this solve, FIRST flag
     {
         Level &L0 = c.lev[0];
@@ -990,6 +995,7 @@ static int pcg_solve(Ctx &c)
     c.mg_iters_last = it;
     c.mg_relres_last = rel;
     if (op_bc_scalar(g, s, x)) return -1;
+    if (hooks && guess_solve_end(c, x, h2i)) return -1;
     DG_HIP(hipMemcpyAsync(c.pprev, x, g.n() * sizeof(double), hipMemcpyDeviceToDevice, s));
     return 0;
 }
@@ -1014,9 +1020,10 @@ int op_mg_solve_tol(Ctx &c)
             hipLaunchKernelGGL(k_to_f32, dim3(nb), dim3(256), 0, s, n, L.c, L.c32);
             DG_CHECK_LAUNCH();
         }
-        return pcg_solve<float>(c);
     }
-    return pcg_solve<double>(c);
+    int (*solve)(Ctx &) = c.mg_mode == 2 ? pcg_solve<float> : pcg_solve<double>;
+    if (guess_eval_enabled()) return guess_eval_solves(c, solve);
+    return solve(c);
 }
 
 }  // namespace dg

@@ -1,0 +1,1161 @@
+// dg_nn.hip -- the learned initial guess of the pressure solve (no
+// counterpart in the reference): a 3D encoder-decoder on the next-coarser
+// multigrid level, its BF16 forward and backward passes on the matrix cores,
+// Adam training on snapshots that the solver captures in HBM, and the
+// guess step inside the PCG solve.
+//
+// Network (channels 16/32/64/64, 3x3x3 convolutions, ReLU, 2x2x2 average
+// pooling, nearest upsampling, skip concatenation), on a level of
+// n0 = (nx, ny, nz) cells, each a multiple of 8:
+//   level 0 (n0):   x0(8: 4 features + 4 zero) -c01-> e0a(16) -c02-> e0(16)
+//   level 1 (n0/2): pool(e0) -c11-> e1a(32) -c12-> e1(32)
+//   level 2 (n0/4): pool(e1) -c21-> e2a(64) -c22-> e2(64)
+//   level 3 (n0/8): pool(e2) -c31-> e3a(64) -c32-> e3(64)
+//   level 2: [up(e3), e2](128) -c23-> d2a(64) -c24-> d2(64)
+//   level 1: [up(d2), e1](96)  -c13-> d1a(32) -c14-> d1(32)
+//   level 0: [up(d1), e0](48)  -c03-> d0a(16) -c04-> d0(16) -out-> y(1, FP32)
+// The skip tensors are written straight into the upper channel slices of
+// the concatenation buffers.
+//
+// Input features per coarse cell (averages over its 8 fine children):
+//   0: log10 of the mean face coefficient / 3 (log10 of 1/density in the
+//      pure phases, /3), 1: residual of the previous pressure / rms,
+//   2: right-hand side / rms, 3: previous pressure / rms.
+// Target: (converged pressure - previous pressure) averaged to the coarse
+// cell, divided by rms(r) * h^2 of the finest level.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "dg_conv.h"
+#include "dg_ctx.h"
+#include "dg_nn.h"
+#include "dg_util.h"
+#include "droplet_gpu.h"
+
+namespace dg {
+
+namespace {
+
+constexpr int NL = 15;
+constexpr int FEAT_C = 8;  // input channels (4 used)
+constexpr int NFEAT = 4;
+
+struct LayerSpec {
+    int cin, cinp, cout, coutp, relu;
+};
+const LayerSpec kL[NL] = {
+    {NFEAT, 8, 16, 16, 1},  // 0  c01
+    {16, 16, 16, 16, 1},    // 1  c02
+    {16, 16, 32, 32, 1},    // 2  c11
+    {32, 32, 32, 32, 1},    // 3  c12
+    {32, 32, 64, 64, 1},    // 4  c21
+    {64, 64, 64, 64, 1},    // 5  c22
+    {64, 64, 64, 64, 1},    // 6  c31
+    {64, 64, 64, 64, 1},    // 7  c32
+    {128, 128, 64, 64, 1},  // 8  c23
+    {64, 64, 64, 64, 1},    // 9  c24
+    {96, 96, 32, 32, 1},    // 10 c13
+    {32, 32, 32, 32, 1},    // 11 c14
+    {48, 48, 16, 16, 1},    // 12 c03
+    {16, 16, 16, 16, 1},    // 13 c04
+    {16, 16, 1, 16, 0},     // 14 out
+};
+
+template <typename T>
+int dmalloc(T **p, size_t n)
+{
+    *p = nullptr;
+    if (n == 0) n = 1;
+    DG_HIP(hipMalloc((void **)p, n * sizeof(T)));
+    DG_HIP(hipMemset(*p, 0, n * sizeof(T)));
+    return 0;
+}
+template <typename T>
+void dfree(T *&p)
+{
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+struct Acts {
+    int d[4][3] = {};
+    size_t V[4] = {};
+    bool train = false;
+    bf16_t *x0 = nullptr, *e0a = nullptr, *cat0 = nullptr, *d0a = nullptr, *d0 = nullptr;
+    bf16_t *p1 = nullptr, *e1a = nullptr, *cat1 = nullptr, *d1a = nullptr, *d1 = nullptr;
+    bf16_t *p2 = nullptr, *e2a = nullptr, *cat2 = nullptr, *d2a = nullptr, *d2 = nullptr;
+    bf16_t *p3 = nullptr, *e3a = nullptr, *e3 = nullptr;
+    float *out = nullptr;
+    // gradients
+    bf16_t *g_out = nullptr, *g_d0 = nullptr, *g_d0a = nullptr, *g_cat0 = nullptr, *g_e0a = nullptr;
+    bf16_t *g_d1 = nullptr, *g_d1a = nullptr, *g_cat1 = nullptr, *g_p1 = nullptr, *g_e1a = nullptr;
+    bf16_t *g_d2 = nullptr, *g_d2a = nullptr, *g_cat2 = nullptr, *g_p2 = nullptr, *g_e2a = nullptr;
+    bf16_t *g_e3 = nullptr, *g_e3a = nullptr, *g_p3 = nullptr;
+    float *wscratch = nullptr;
+    size_t wscratch_n = 0;
+    double *loss_part = nullptr;  // RED_BLOCKS + 8
+};
+
+struct Store {
+    int cap = 0, count = 0, next = 0;
+    int d[3] = {0, 0, 0};
+    size_t V = 0;
+    bf16_t *feat = nullptr;  // [cap][V][8]
+    float *tgt = nullptr;    // [cap][V]
+};
+
+struct GuessWork {  // finest-level buffers of the guess step
+    size_t n = 0;
+    double *d = nullptr, *s = nullptr, *psave = nullptr;
+    double *scal = nullptr;     // 32 device doubles
+    double *partial = nullptr;  // RED_BLOCKS doubles
+    double *hs = nullptr;       // pinned mirror (32)
+};
+
+// guess scalar slots
+enum { G_PP = 0, G_SR = 1, G_SB = 2, G_SP = 3, G_DR = 4, G_DS = 5, G_RS = 6, G_SS = 7, G_ALPHA = 8, G_ACC = 9,
+       G_RR = 10, G_LOSS = 11, G_ESCALE = 12 };
+
+struct Net {
+    bool have_params = false;
+    size_t np = 0;
+    size_t woff[NL] = {}, boff[NL] = {};
+    float *P = nullptr, *G = nullptr, *M = nullptr, *Vm = nullptr;
+    bf16_t *wB[NL] = {}, *wD[NL] = {};
+    bool packed = false;
+    long step = 0;
+    Acts a;
+    Store st;
+    GuessWork gw;
+    hipStream_t own = nullptr;
+    bool capture = false;
+    int pending = -1;  // store slot waiting for its target
+    long accepted = 0, rejected = 0;
+    double last_alpha = 0.0;
+    int eval = 0;
+    int eval_iters[3] = {0, 0, 0};
+};
+
+Net &net()
+{
+    static Net n;
+    return n;
+}
+
+size_t layer_wcount(int l) { return (size_t)kL[l].coutp * 27 * kL[l].cinp; }
+
+int ensure_param_storage()
+{
+    Net &N = net();
+    if (N.P) return 0;
+    size_t off = 0;
+    for (int l = 0; l < NL; l++) {
+        N.woff[l] = off;
+        off += layer_wcount(l);
+        N.boff[l] = off;
+        off += kL[l].coutp;
+    }
+    N.np = off;
+    if (dmalloc(&N.P, N.np) || dmalloc(&N.G, N.np) || dmalloc(&N.M, N.np) || dmalloc(&N.Vm, N.np)) return -1;
+    for (int l = 0; l < NL; l++)
+        if (dmalloc(&N.wB[l], layer_wcount(l)) || dmalloc(&N.wD[l], layer_wcount(l))) return -1;
+    return 0;
+}
+
+hipStream_t own_stream()
+{
+    Net &N = net();
+    if (!N.own) (void)hipStreamCreate(&N.own);
+    return N.own;
+}
+
+int pack_all(hipStream_t s)
+{
+    Net &N = net();
+    for (int l = 0; l < NL; l++) {
+        if (conv_pack_fwd(N.P + N.woff[l], kL[l].cinp, kL[l].coutp, N.wB[l], s)) return -1;
+        if (conv_pack_dgrad(N.P + N.woff[l], kL[l].cinp, kL[l].coutp, N.wD[l], s)) return -1;
+    }
+    N.packed = true;
+    return 0;
+}
+
+void free_acts(Acts &a)
+{
+    bf16_t **bs[] = {&a.x0, &a.e0a, &a.cat0, &a.d0a, &a.d0, &a.p1, &a.e1a, &a.cat1, &a.d1a, &a.d1, &a.p2, &a.e2a,
+                     &a.cat2, &a.d2a, &a.d2, &a.p3, &a.e3a, &a.e3, &a.g_out, &a.g_d0, &a.g_d0a, &a.g_cat0,
+                     &a.g_e0a, &a.g_d1, &a.g_d1a, &a.g_cat1, &a.g_p1, &a.g_e1a, &a.g_d2, &a.g_d2a, &a.g_cat2,
+                     &a.g_p2, &a.g_e2a, &a.g_e3, &a.g_e3a, &a.g_p3};
+    for (bf16_t **p : bs) dfree(*p);
+    dfree(a.out);
+    dfree(a.wscratch);
+    dfree(a.loss_part);
+    a = Acts();
+}
+
+int ensure_acts(int nx, int ny, int nz, bool train)
+{
+    Net &N = net();
+    Acts &a = N.a;
+    if (nx <= 0 || ny <= 0 || nz <= 0 || (nx & 7) || (ny & 7) || (nz & 7)) {
+        set_error("learned guess: network level %dx%dx%d is not a multiple of 8 in every direction", nx, ny, nz);
+        return -1;
+    }
+    if (a.d[0][0] == nx && a.d[0][1] == ny && a.d[0][2] == nz && (a.train || !train)) return 0;
+    free_acts(a);
+    for (int l = 0; l < 4; l++) {
+        a.d[l][0] = nx >> l;
+        a.d[l][1] = ny >> l;
+        a.d[l][2] = nz >> l;
+        a.V[l] = (size_t)a.d[l][0] * a.d[l][1] * a.d[l][2];
+    }
+    const size_t V0 = a.V[0], V1 = a.V[1], V2 = a.V[2], V3 = a.V[3];
+    if (dmalloc(&a.x0, V0 * 8) || dmalloc(&a.e0a, V0 * 16) || dmalloc(&a.cat0, V0 * 48) || dmalloc(&a.d0a, V0 * 16) ||
+        dmalloc(&a.d0, V0 * 16) || dmalloc(&a.out, V0) || dmalloc(&a.p1, V1 * 16) || dmalloc(&a.e1a, V1 * 32) ||
+        dmalloc(&a.cat1, V1 * 96) || dmalloc(&a.d1a, V1 * 32) || dmalloc(&a.d1, V1 * 32) ||
+        dmalloc(&a.p2, V2 * 32) || dmalloc(&a.e2a, V2 * 64) || dmalloc(&a.cat2, V2 * 128) ||
+        dmalloc(&a.d2a, V2 * 64) || dmalloc(&a.d2, V2 * 64) || dmalloc(&a.p3, V3 * 64) || dmalloc(&a.e3a, V3 * 64) ||
+        dmalloc(&a.e3, V3 * 64) || dmalloc(&a.loss_part, (size_t)RED_BLOCKS + 8))
+        return -1;
+    if (train) {
+        if (dmalloc(&a.g_out, V0 * 16) || dmalloc(&a.g_d0, V0 * 16) || dmalloc(&a.g_d0a, V0 * 16) ||
+            dmalloc(&a.g_cat0, V0 * 48) || dmalloc(&a.g_e0a, V0 * 16) || dmalloc(&a.g_d1, V1 * 32) ||
+            dmalloc(&a.g_d1a, V1 * 32) || dmalloc(&a.g_cat1, V1 * 96) || dmalloc(&a.g_p1, V1 * 16) ||
+            dmalloc(&a.g_e1a, V1 * 32) || dmalloc(&a.g_d2, V2 * 64) || dmalloc(&a.g_d2a, V2 * 64) ||
+            dmalloc(&a.g_cat2, V2 * 128) || dmalloc(&a.g_p2, V2 * 32) || dmalloc(&a.g_e2a, V2 * 64) ||
+            dmalloc(&a.g_e3, V3 * 64) || dmalloc(&a.g_e3a, V3 * 64) || dmalloc(&a.g_p3, V3 * 64))
+            return -1;
+        // weight-gradient scratch: the largest layer
+        static const int lev_of[NL] = {0, 0, 1, 1, 2, 2, 3, 3, 2, 2, 1, 1, 0, 0, 0};
+        size_t mx = 0;
+        for (int l = 0; l < NL; l++) {
+            const int *d = a.d[lev_of[l]];
+            const size_t q = conv3d_wgrad_scratch(d[0], d[1], d[2], kL[l].cinp, kL[l].coutp);
+            if (q > mx) mx = q;
+        }
+        a.wscratch_n = mx;
+        if (dmalloc(&a.wscratch, mx)) return -1;
+    }
+    a.train = train;
+    return 0;
+}
+
+// ---- forward / backward -------------------------------------------------------
+
+int fwd_layer(int l, const int *d, const bf16_t *x, int xs, bf16_t *y, int ys, float *yf, hipStream_t s)
+{
+    Net &N = net();
+    const LayerSpec &L = kL[l];
+    return conv3d_fwd(d[0], d[1], d[2], x, xs, L.cinp, N.wB[l], L.coutp, N.P + N.boff[l], L.cout, L.relu, y, ys, yf,
+                      1, s);
+}
+
+// weight and bias gradient of layer l; data gradient into gx when given
+int bwd_layer(int l, const int *d, const bf16_t *x, int xs, const bf16_t *gy, int gys, bf16_t *gx, int gxs,
+              hipStream_t s)
+{
+    Net &N = net();
+    Acts &a = N.a;
+    const LayerSpec &L = kL[l];
+    if (conv3d_wgrad(d[0], d[1], d[2], x, xs, L.cinp, gy, gys, L.coutp, N.G + N.woff[l], N.G + N.boff[l], a.wscratch,
+                     a.wscratch_n, s))
+        return -1;
+    if (gx && conv3d_dgrad(d[0], d[1], d[2], gy, gys, L.coutp, N.wD[l], L.cinp, gx, gxs, s)) return -1;
+    return 0;
+}
+
+int net_forward(const bf16_t *x0, hipStream_t s)
+{
+    Net &N = net();
+    Acts &a = N.a;
+    const int *d0 = a.d[0], *d1 = a.d[1], *d2 = a.d[2], *d3 = a.d[3];
+    if (fwd_layer(0, d0, x0, FEAT_C, a.e0a, 16, nullptr, s)) return -1;
+    if (fwd_layer(1, d0, a.e0a, 16, a.cat0 + 32, 48, nullptr, s)) return -1;
+    if (pool2_fwd(d0[0], d0[1], d0[2], 16, a.cat0 + 32, 48, a.p1, 16, s)) return -1;
+    if (fwd_layer(2, d1, a.p1, 16, a.e1a, 32, nullptr, s)) return -1;
+    if (fwd_layer(3, d1, a.e1a, 32, a.cat1 + 64, 96, nullptr, s)) return -1;
+    if (pool2_fwd(d1[0], d1[1], d1[2], 32, a.cat1 + 64, 96, a.p2, 32, s)) return -1;
+    if (fwd_layer(4, d2, a.p2, 32, a.e2a, 64, nullptr, s)) return -1;
+    if (fwd_layer(5, d2, a.e2a, 64, a.cat2 + 64, 128, nullptr, s)) return -1;
+    if (pool2_fwd(d2[0], d2[1], d2[2], 64, a.cat2 + 64, 128, a.p3, 64, s)) return -1;
+    if (fwd_layer(6, d3, a.p3, 64, a.e3a, 64, nullptr, s)) return -1;
+    if (fwd_layer(7, d3, a.e3a, 64, a.e3, 64, nullptr, s)) return -1;
+    if (up2_fwd(d2[0], d2[1], d2[2], 64, a.e3, 64, a.cat2, 128, s)) return -1;
+    if (fwd_layer(8, d2, a.cat2, 128, a.d2a, 64, nullptr, s)) return -1;
+    if (fwd_layer(9, d2, a.d2a, 64, a.d2, 64, nullptr, s)) return -1;
+    if (up2_fwd(d1[0], d1[1], d1[2], 64, a.d2, 64, a.cat1, 96, s)) return -1;
+    if (fwd_layer(10, d1, a.cat1, 96, a.d1a, 32, nullptr, s)) return -1;
+    if (fwd_layer(11, d1, a.d1a, 32, a.d1, 32, nullptr, s)) return -1;
+    if (up2_fwd(d0[0], d0[1], d0[2], 32, a.d1, 32, a.cat0, 48, s)) return -1;
+    if (fwd_layer(12, d0, a.cat0, 48, a.d0a, 16, nullptr, s)) return -1;
+    if (fwd_layer(13, d0, a.d0a, 16, a.d0, 16, nullptr, s)) return -1;
+    if (fwd_layer(14, d0, a.d0, 16, nullptr, 0, a.out, s)) return -1;
+    return 0;
+}
+
+// gradients of every parameter from a.g_out (dL/dy in channel 0 of 16)
+int net_backward(const bf16_t *x0, hipStream_t s)
+{
+    Net &N = net();
+    Acts &a = N.a;
+    const int *d0 = a.d[0], *d1 = a.d[1], *d2 = a.d[2], *d3 = a.d[3];
+    const size_t V0 = a.V[0], V1 = a.V[1], V2 = a.V[2], V3 = a.V[3];
+    if (bwd_layer(14, d0, a.d0, 16, a.g_out, 16, a.g_d0, 16, s)) return -1;
+    if (relu_bwd(V0, 16, a.d0, 16, a.g_d0, 16, s)) return -1;
+    if (bwd_layer(13, d0, a.d0a, 16, a.g_d0, 16, a.g_d0a, 16, s)) return -1;
+    if (relu_bwd(V0, 16, a.d0a, 16, a.g_d0a, 16, s)) return -1;
+    if (bwd_layer(12, d0, a.cat0, 48, a.g_d0a, 16, a.g_cat0, 48, s)) return -1;
+    if (up2_bwd(d0[0], d0[1], d0[2], 32, a.g_cat0, 48, a.g_d1, 32, s)) return -1;
+    if (relu_bwd(V1, 32, a.d1, 32, a.g_d1, 32, s)) return -1;
+    if (bwd_layer(11, d1, a.d1a, 32, a.g_d1, 32, a.g_d1a, 32, s)) return -1;
+    if (relu_bwd(V1, 32, a.d1a, 32, a.g_d1a, 32, s)) return -1;
+    if (bwd_layer(10, d1, a.cat1, 96, a.g_d1a, 32, a.g_cat1, 96, s)) return -1;
+    if (up2_bwd(d1[0], d1[1], d1[2], 64, a.g_cat1, 96, a.g_d2, 64, s)) return -1;
+    if (relu_bwd(V2, 64, a.d2, 64, a.g_d2, 64, s)) return -1;
+    if (bwd_layer(9, d2, a.d2a, 64, a.g_d2, 64, a.g_d2a, 64, s)) return -1;
+    if (relu_bwd(V2, 64, a.d2a, 64, a.g_d2a, 64, s)) return -1;
+    if (bwd_layer(8, d2, a.cat2, 128, a.g_d2a, 64, a.g_cat2, 128, s)) return -1;
+    if (up2_bwd(d2[0], d2[1], d2[2], 64, a.g_cat2, 128, a.g_e3, 64, s)) return -1;
+    if (relu_bwd(V3, 64, a.e3, 64, a.g_e3, 64, s)) return -1;
+    if (bwd_layer(7, d3, a.e3a, 64, a.g_e3, 64, a.g_e3a, 64, s)) return -1;
+    if (relu_bwd(V3, 64, a.e3a, 64, a.g_e3a, 64, s)) return -1;
+    if (bwd_layer(6, d3, a.p3, 64, a.g_e3a, 64, a.g_p3, 64, s)) return -1;
+    // e2 feeds both the pool and the concatenation
+    if (pool2_bwd_add(d2[0], d2[1], d2[2], 64, a.g_p3, 64, a.g_cat2 + 64, 128, s)) return -1;
+    if (relu_bwd(V2, 64, a.cat2 + 64, 128, a.g_cat2 + 64, 128, s)) return -1;
+    if (bwd_layer(5, d2, a.e2a, 64, a.g_cat2 + 64, 128, a.g_e2a, 64, s)) return -1;
+    if (relu_bwd(V2, 64, a.e2a, 64, a.g_e2a, 64, s)) return -1;
+    if (bwd_layer(4, d2, a.p2, 32, a.g_e2a, 64, a.g_p2, 32, s)) return -1;
+    if (pool2_bwd_add(d1[0], d1[1], d1[2], 32, a.g_p2, 32, a.g_cat1 + 64, 96, s)) return -1;
+    if (relu_bwd(V1, 32, a.cat1 + 64, 96, a.g_cat1 + 64, 96, s)) return -1;
+    if (bwd_layer(3, d1, a.e1a, 32, a.g_cat1 + 64, 96, a.g_e1a, 32, s)) return -1;
+    if (relu_bwd(V1, 32, a.e1a, 32, a.g_e1a, 32, s)) return -1;
+    if (bwd_layer(2, d1, a.p1, 16, a.g_e1a, 32, a.g_p1, 16, s)) return -1;
+    if (pool2_bwd_add(d0[0], d0[1], d0[2], 16, a.g_p1, 16, a.g_cat0 + 32, 48, s)) return -1;
+    if (relu_bwd(V0, 16, a.cat0 + 32, 48, a.g_cat0 + 32, 48, s)) return -1;
+    if (bwd_layer(1, d0, a.e0a, 16, a.g_cat0 + 32, 48, a.g_e0a, 16, s)) return -1;
+    if (relu_bwd(V0, 16, a.e0a, 16, a.g_e0a, 16, s)) return -1;
+    if (bwd_layer(0, d0, x0, FEAT_C, a.g_e0a, 16, nullptr, 0, s)) return -1;
+    return 0;
+}
+
+// loss = 0.5 * mean((y - target)^2); g_out[v][0] = (y - target) / V
+__global__ void __launch_bounds__(256) k_loss(size_t V, const float *y, const float *tgt, bf16_t *gout, double *part)
+{
+    __shared__ double lds4[4];
+    const float inv = 1.0f / (float)V;
+    double acc = 0.0;
+    for (size_t v = (size_t)blockIdx.x * 256 + threadIdx.x; v < V; v += (size_t)gridDim.x * 256) {
+        const float e = y[v] - tgt[v];
+        acc += (double)e * (double)e;
+        gout[v * 16] = f2bf(e * inv);
+    }
+    const double r = block_reduce256<false>(acc, lds4);
+    if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+__global__ void k_loss_final(const double *part, int n, double scale, double *out, double *accum)
+{
+    __shared__ double lds4[4];
+    double acc = 0.0;
+    for (int p = threadIdx.x; p < n; p += 256) acc += part[p];
+    const double r = block_reduce256<false>(acc, lds4);
+    if (threadIdx.x == 0) {
+        *out = r * scale;
+        if (accum) *accum += r * scale;
+    }
+}
+
+int net_loss(const float *tgt, double *loss_dev, double *accum, hipStream_t s)
+{
+    Acts &a = net().a;
+    const size_t V = a.V[0];
+    int nb = (int)((V + 255) / 256);
+    if (nb > RED_BLOCKS) nb = RED_BLOCKS;
+    hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, s, V, a.out, tgt, a.g_out, a.loss_part);
+    DG_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(256), 0, s, a.loss_part, nb, 0.5 / (double)V, loss_dev, accum);
+    DG_CHECK_LAUNCH();
+    return 0;
+}
+
+int net_adam(float lr, hipStream_t s)
+{
+    Net &N = net();
+    N.step++;
+    const double b1 = 0.9, b2 = 0.999, eps = 1e-8;
+    const double c1 = 1.0 - pow(b1, (double)N.step), c2 = 1.0 - pow(b2, (double)N.step);
+    const float lr_t = (float)(lr * sqrt(c2) / c1), eps_t = (float)(eps * sqrt(c2));
+    if (adam_update(N.np, N.P, N.G, N.M, N.Vm, (float)b1, (float)b2, lr_t, eps_t, s)) return -1;
+    return pack_all(s);
+}
+
+// ---- features, prolongation, guess step -------------------------------------------
+
+// one thread per coarse cell; fine children (2X+2, 2X+3) in 1-based indices
+__global__ void k_features(Grid g, int cx, int cy, int cz, const double *a, const double *b, const double *c,
+                           const double *r, const double *rhs, const double *pp, const double *ds_rr,
+                           double *gs, double nfine, bf16_t *x0)
+{
+    const size_t V = (size_t)cx * cy * cz;
+    const double sr = sqrt(ds_rr[0] / nfine), sb = sqrt(ds_rr[1] / nfine), sp = sqrt(gs[G_PP] / nfine);
+    const double ir = sr > 0.0 ? 1.0 / sr : 0.0, ib = sb > 0.0 ? 1.0 / sb : 0.0, ip = sp > 0.0 ? 1.0 / sp : 0.0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        gs[G_SR] = sr;
+        gs[G_SB] = sb;
+        gs[G_SP] = sp;
+    }
+    for (size_t v = (size_t)blockIdx.x * 256 + threadIdx.x; v < V; v += (size_t)gridDim.x * 256) {
+        const int X = (int)(v % cx), Y = (int)((v / cx) % cy), Z = (int)(v / ((size_t)cx * cy));
+        double coef = 0.0, rr = 0.0, bb = 0.0, p = 0.0;
+        for (int dz = 0; dz < 2; dz++)
+            for (int dy = 0; dy < 2; dy++)
+                for (int dx = 0; dx < 2; dx++) {
+                    const int o = g.at(2 * X + 2 + dx, 2 * Y + 2 + dy, 2 * Z + 2 + dz);
+                    coef += a[o] + a[o - 1] + b[o] + b[o - g.sy] + c[o] + c[o - g.sz];
+                    rr += r[o];
+                    bb += rhs[o];
+                    p += pp[o];
+                }
+        const double mc = coef / 48.0;
+        bf16_t *q = x0 + v * FEAT_C;
+        q[0] = f2bf((float)(mc > 0.0 ? log10(mc) / 3.0 : 0.0));
+        q[1] = f2bf((float)(rr * 0.125 * ir));
+        q[2] = f2bf((float)(bb * 0.125 * ib));
+        q[3] = f2bf((float)(p * 0.125 * ip));
+        q[4] = q[5] = q[6] = q[7] = 0;
+    }
+}
+
+// trilinear (cell-centred, weights 3/4 and 1/4 per axis, constant beyond
+// the edge) prolongation of the network output to the finest interior,
+// times rms(r) * h^2
+__global__ void k_prolong_guess(Grid g, int cx, int cy, int cz, const float *y, const double *gs, double h2, double *d)
+{
+    const double sc = gs[G_SR] * h2;
+    const int mx = g.nx - 2, my = g.ny - 2, mz = g.nz - 2;
+    const size_t N = (size_t)mx * my * mz;
+    for (size_t p = (size_t)blockIdx.x * 256 + threadIdx.x; p < N; p += (size_t)gridDim.x * 256) {
+        const int fx = (int)(p % mx), fy = (int)((p / mx) % my), fz = (int)(p / ((size_t)mx * my));
+        const int X = fx >> 1, Y = fy >> 1, Z = fz >> 1;
+        int X2 = (fx & 1) ? X + 1 : X - 1, Y2 = (fy & 1) ? Y + 1 : Y - 1, Z2 = (fz & 1) ? Z + 1 : Z - 1;
+        X2 = X2 < 0 ? 0 : (X2 >= cx ? cx - 1 : X2);
+        Y2 = Y2 < 0 ? 0 : (Y2 >= cy ? cy - 1 : Y2);
+        Z2 = Z2 < 0 ? 0 : (Z2 >= cz ? cz - 1 : Z2);
+        const int xs[2] = {X, X2}, ys[2] = {Y, Y2}, zs[2] = {Z, Z2};
+        const double w[2] = {0.75, 0.25};
+        double acc = 0.0;
+        for (int k = 0; k < 2; k++)
+            for (int j = 0; j < 2; j++)
+                for (int i = 0; i < 2; i++)
+                    acc += w[i] * w[j] * w[k] * (double)y[((size_t)zs[k] * cy + ys[j]) * cx + xs[i]];
+        d[g.at(fx + 2, fy + 2, fz + 2)] = acc * sc;
+    }
+}
+
+// s = A d with the solve's operator (k_init_resid's boundary treatment:
+// x-max neighbour 0, other missing neighbours equal to the cell)
+__global__ void k_apply_op(Grid g, const double *d, const double *a, const double *b, const double *c, double h2i,
+                           double *s)
+{
+    const int mx = g.nx - 2, my = g.ny - 2, mz = g.nz - 2;
+    const size_t N = (size_t)mx * my * mz;
+    for (size_t p = (size_t)blockIdx.x * 256 + threadIdx.x; p < N; p += (size_t)gridDim.x * 256) {
+        const int i = (int)(p % mx) + 2, j = (int)((p / mx) % my) + 2, k = (int)(p / ((size_t)mx * my)) + 2;
+        const int o = g.at(i, j, k), sy = g.sy, sz = g.sz;
+        const double u0 = d[o];
+        const double ue = i < g.nx - 1 ? d[o + 1] : 0.0;
+        const double uw = i > 2 ? d[o - 1] : u0;
+        const double un = j < g.ny - 1 ? d[o + sy] : u0;
+        const double us = j > 2 ? d[o - sy] : u0;
+        const double ut = k < g.nz - 1 ? d[o + sz] : u0;
+        const double ub = k > 2 ? d[o - sz] : u0;
+        const double t = a[o] * (ue - u0) + a[o - 1] * (uw - u0) + b[o] * (un - u0) + b[o - sy] * (us - u0) +
+                         c[o] * (ut - u0) + c[o - sz] * (ub - u0);
+        s[o] = h2i * t;
+    }
+}
+
+// alpha = (d.r)/(d.Ad); accept when finite, positive and |r - alpha Ad| < |r|
+__global__ void k_guess_alpha(double *gs, const double *ds_rr)
+{
+    const double dr = gs[G_DR], dAd = gs[G_DS], rs = gs[G_RS], ss = gs[G_SS], rr0 = ds_rr[0];
+    double alpha = dAd != 0.0 ? dr / dAd : 0.0;
+    const double rr1 = rr0 - 2.0 * alpha * rs + alpha * alpha * ss;
+    const bool ok = isfinite(alpha) && alpha > 0.0 && isfinite(rr1) && rr1 < rr0;
+    gs[G_ALPHA] = ok ? alpha : 0.0;
+    gs[G_ACC] = ok ? 1.0 : 0.0;
+}
+
+__global__ void k_guess_update(Grid g, const double *gs, const double *d, const double *s, double *x, double *r)
+{
+    const double alpha = gs[G_ALPHA];
+    if (gs[G_ACC] == 0.0) return;
+    const int mx = g.nx - 2, my = g.ny - 2, mz = g.nz - 2;
+    const size_t N = (size_t)mx * my * mz;
+    for (size_t p = (size_t)blockIdx.x * 256 + threadIdx.x; p < N; p += (size_t)gridDim.x * 256) {
+        const int o = g.at((int)(p % mx) + 2, (int)((p / mx) % my) + 2, (int)(p / ((size_t)mx * my)) + 2);
+        x[o] += alpha * d[o];
+        r[o] -= alpha * s[o];
+    }
+}
+
+// target of a captured sample: coarse average of (x - pprev) / (rms(r) h^2)
+__global__ void k_capture_target(Grid g, int cx, int cy, int cz, const double *x, const double *pp, const double *gs,
+                                 double h2, float *tgt)
+{
+    const double sc = gs[G_SR] * h2;
+    const double isc = sc > 0.0 ? 1.0 / sc : 0.0;
+    const size_t V = (size_t)cx * cy * cz;
+    for (size_t v = (size_t)blockIdx.x * 256 + threadIdx.x; v < V; v += (size_t)gridDim.x * 256) {
+        const int X = (int)(v % cx), Y = (int)((v / cx) % cy), Z = (int)(v / ((size_t)cx * cy));
+        double e = 0.0;
+        for (int dz = 0; dz < 2; dz++)
+            for (int dy = 0; dy < 2; dy++)
+                for (int dx = 0; dx < 2; dx++) {
+                    const int o = g.at(2 * X + 2 + dx, 2 * Y + 2 + dy, 2 * Z + 2 + dz);
+                    e += x[o] - pp[o];
+                }
+        tgt[v] = (float)(e * 0.125 * isc);
+    }
+}
+
+unsigned gridn(size_t n)
+{
+    size_t b = (n + 255) / 256;
+    if (b > 16384) b = 16384;
+    if (b < 1) b = 1;
+    return (unsigned)b;
+}
+
+int ensure_work(size_t n)
+{
+    GuessWork &w = net().gw;
+    if (w.n == n) return 0;
+    dfree(w.d);
+    dfree(w.s);
+    dfree(w.psave);
+    if (!w.scal && dmalloc(&w.scal, 32)) return -1;
+    if (!w.partial && dmalloc(&w.partial, (size_t)RED_BLOCKS + 8)) return -1;
+    if (!w.hs) {
+        DG_HIP(hipHostMalloc((void **)&w.hs, 32 * sizeof(double), hipHostMallocDefault));
+        memset(w.hs, 0, 32 * sizeof(double));
+    }
+    if (dmalloc(&w.d, n) || dmalloc(&w.s, n) || dmalloc(&w.psave, n)) return -1;
+    w.n = n;
+    return 0;
+}
+
+}  // namespace
+
+bool guess_wanted(const Ctx &c)
+{
+    const Net &N = net();
+    return c.mg_guess == 2 || N.capture || N.eval;
+}
+
+int guess_solve_begin(Ctx &c, double *x, double *r, double h2i, bool apply, double *rr_slot)
+{
+    Net &N = net();
+    hipStream_t s = c.stream;
+    if (c.ng < 2) {
+        set_error("learned guess: the grid has a single multigrid level");
+        return -1;
+    }
+    Level &F = c.lev[c.ng - 1];
+    const Grid g = F.g;
+    const int cx = (g.nx - 2) / 2, cy = (g.ny - 2) / 2, cz = (g.nz - 2) / 2;
+    if ((g.nx - 2) & 1 || (g.ny - 2) & 1 || (g.nz - 2) & 1) {
+        set_error("learned guess: odd finest-level extent");
+        return -1;
+    }
+    if (apply && !N.have_params) {
+        set_error("learned guess requested but no network weights are loaded (dg_guess_set_params)");
+        return -1;
+    }
+    const bool capture = N.capture && N.st.cap > 0 && c.mg_guess >= 1;
+    if (N.capture && c.mg_guess == 0) {
+        set_error("sample capture needs the previous-step guess (dg_set_solver guess 1 or 2)");
+        return -1;
+    }
+    if (ensure_param_storage() || ensure_acts(cx, cy, cz, N.a.train) || ensure_work(g.n())) return -1;
+    if (apply && !N.packed && pack_all(s)) return -1;
+    GuessWork &w = N.gw;
+    const double nfine = (double)(g.nx - 2) * (g.ny - 2) * (g.nz - 2);
+    const double *pp = c.pprev;
+    if (reduce_interior<false>(g, s, w.partial, w.scal + G_PP, [=] __device__(int i, int j, int k) {
+            const double v = pp[g.at(i, j, k)];
+            return v * v;
+        }))
+        return -1;
+    hipLaunchKernelGGL(k_features, dim3(gridn((size_t)cx * cy * cz)), dim3(256), 0, s, g, cx, cy, cz, F.a, F.b, F.c,
+                       r, F.rhs, pp, rr_slot, w.scal, nfine, N.a.x0);
+    DG_CHECK_LAUNCH();
+    N.pending = -1;
+    if (capture) {
+        Store &S = N.st;
+        if (S.d[0] != cx || S.d[1] != cy || S.d[2] != cz) {
+            if (S.count > 0) {
+                set_error("learned guess: capture store holds %dx%dx%d samples, this solve is %dx%dx%d", S.d[0],
+                          S.d[1], S.d[2], cx, cy, cz);
+                return -1;
+            }
+            dfree(S.feat);
+            dfree(S.tgt);
+            S.d[0] = cx;
+            S.d[1] = cy;
+            S.d[2] = cz;
+            S.V = (size_t)cx * cy * cz;
+            if (dmalloc(&S.feat, (size_t)S.cap * S.V * FEAT_C) || dmalloc(&S.tgt, (size_t)S.cap * S.V)) return -1;
+        }
+        const int slot = S.next;
+        DG_HIP(hipMemcpyAsync(S.feat + (size_t)slot * S.V * FEAT_C, N.a.x0, S.V * FEAT_C * sizeof(bf16_t),
+                              hipMemcpyDeviceToDevice, s));
+        N.pending = slot;
+    }
+    if (!apply) return 0;
+    if (net_forward(N.a.x0, s)) return -1;
+    const double h2 = 1.0 / h2i;
+    hipLaunchKernelGGL(k_prolong_guess, dim3(gridn((size_t)nfine)), dim3(256), 0, s, g, cx, cy, cz, N.a.out, w.scal,
+                       h2, w.d);
+    DG_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_apply_op, dim3(gridn((size_t)nfine)), dim3(256), 0, s, g, w.d, F.a, F.b, F.c, h2i, w.s);
+    DG_CHECK_LAUNCH();
+    const double *dd = w.d, *sv = w.s, *rv = r;
+    if (reduce_interior<false>(g, s, w.partial, w.scal + G_DR, [=] __device__(int i, int j, int k) {
+            const int o = g.at(i, j, k);
+            return dd[o] * rv[o];
+        }) ||
+        reduce_interior<false>(g, s, w.partial, w.scal + G_DS, [=] __device__(int i, int j, int k) {
+            const int o = g.at(i, j, k);
+            return dd[o] * sv[o];
+        }) ||
+        reduce_interior<false>(g, s, w.partial, w.scal + G_RS, [=] __device__(int i, int j, int k) {
+            const int o = g.at(i, j, k);
+            return rv[o] * sv[o];
+        }) ||
+        reduce_interior<false>(g, s, w.partial, w.scal + G_SS, [=] __device__(int i, int j, int k) {
+            const int o = g.at(i, j, k);
+            return sv[o] * sv[o];
+        }))
+        return -1;
+    hipLaunchKernelGGL(k_guess_alpha, dim3(1), dim3(1), 0, s, w.scal, rr_slot);
+    DG_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_guess_update, dim3(gridn((size_t)nfine)), dim3(256), 0, s, g, w.scal, w.d, w.s, x, r);
+    DG_CHECK_LAUNCH();
+    if (reduce_interior<false>(g, s, w.partial, rr_slot, [=] __device__(int i, int j, int k) {
+            const double v = rv[g.at(i, j, k)];
+            return v * v;
+        }))
+        return -1;
+    DG_HIP(hipMemcpyAsync(w.hs, w.scal, 16 * sizeof(double), hipMemcpyDeviceToHost, s));
+    DG_HIP(hipStreamSynchronize(s));
+    N.last_alpha = w.hs[G_ALPHA];
+    if (w.hs[G_ACC] != 0.0)
+        N.accepted++;
+    else
+        N.rejected++;
+    return 0;
+}
+
+int guess_solve_end(Ctx &c, const double *x, double h2i)
+{
+    Net &N = net();
+    if (N.pending < 0) return 0;
+    Store &S = N.st;
+    Level &F = c.lev[c.ng - 1];
+    const Grid g = F.g;
+    hipLaunchKernelGGL(k_capture_target, dim3(gridn(S.V)), dim3(256), 0, c.stream, g, S.d[0], S.d[1], S.d[2], x,
+                       c.pprev, N.gw.scal, 1.0 / h2i, S.tgt + (size_t)N.pending * S.V);
+    DG_CHECK_LAUNCH();
+    S.next = (S.next + 1) % S.cap;
+    if (S.count < S.cap) S.count++;
+    N.pending = -1;
+    return 0;
+}
+
+bool guess_eval_enabled() { return net().eval != 0; }
+
+int guess_eval_solves(Ctx &c, int (*solve)(Ctx &))
+{
+    Net &N = net();
+    const size_t n = c.g.n();
+    if (ensure_work(n)) return -1;
+    const int guess0 = c.mg_guess;
+    const bool cap0 = N.capture;
+    N.capture = false;  // one system, three solves: no samples
+    DG_HIP(hipMemcpyAsync(N.gw.psave, c.pprev, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    int rc = 0;
+    for (int q = 0; q < 3 && !rc; q++) {
+        if (q > 0 && hipMemcpyAsync(c.pprev, N.gw.psave, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream) !=
+                         hipSuccess) {
+            set_error("guess evaluation: restoring the previous pressure failed");
+            rc = -1;
+            break;
+        }
+        c.mg_guess = q;
+        rc = solve(c);
+        N.eval_iters[q] = c.mg_iters_last;
+    }
+    c.mg_guess = guess0;
+    N.capture = cap0;
+    return rc;
+}
+
+}  // namespace dg
+
+// ---- C-ABI ---------------------------------------------------------------------------
+
+using namespace dg;
+
+namespace {
+
+uint64_t splitmix(uint64_t &s)
+{
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+double unif(uint64_t &s) { return ((double)(splitmix(s) >> 11) + 0.5) / 9007199254740992.0; }
+
+int check_dims(const int *nx, const int *ny, const int *nz)
+{
+    if (*nx <= 0 || *ny <= 0 || *nz <= 0 || (*nx & 7) || (*ny & 7) || (*nz & 7)) {
+        set_error("network extents must be positive multiples of 8 (%d %d %d)", *nx, *ny, *nz);
+        return -1;
+    }
+    return 0;
+}
+
+// host FP32 [V][4] features -> device bf16 [V][8]
+int upload_features(const float *feat, size_t V, bf16_t *x0, hipStream_t s)
+{
+    std::vector<bf16_t> h(V * FEAT_C, 0);
+    for (size_t v = 0; v < V; v++)
+        for (int q = 0; q < NFEAT; q++) h[v * FEAT_C + q] = f2bf(feat[v * NFEAT + q]);
+    DG_HIP(hipMemcpyAsync(x0, h.data(), h.size() * sizeof(bf16_t), hipMemcpyHostToDevice, s));
+    DG_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+int require_params()
+{
+    if (!net().have_params) {
+        set_error("no network weights: call dg_guess_init_random or dg_guess_set_params first");
+        return -1;
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dg_guess_nparams(int *n)
+{
+    if (ensure_param_storage()) return -1;
+    *n = (int)net().np;
+    return 0;
+}
+
+int dg_guess_init_random(const int *seed)
+{
+    if (ensure_param_storage()) return -1;
+    Net &N = net();
+    std::vector<float> h(N.np, 0.f);
+    uint64_t st = 0x5eed0000ull + (uint64_t)(uint32_t)*seed;
+    for (int l = 0; l < NL; l++) {
+        const LayerSpec &L = kL[l];
+        // He-normal for the ReLU layers, a small output layer
+        const double sd = L.relu ? sqrt(2.0 / (27.0 * L.cin)) : 0.1 * sqrt(1.0 / (27.0 * L.cin));
+        for (int co = 0; co < L.cout; co++)
+            for (int t = 0; t < 27; t++)
+                for (int ci = 0; ci < L.cin; ci++) {
+                    const double u1 = unif(st), u2 = unif(st);
+                    const double z = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+                    h[N.woff[l] + ((size_t)co * 27 + t) * L.cinp + ci] = (float)(sd * z);
+                }
+    }
+    hipStream_t s = own_stream();
+    DG_HIP(hipMemcpyAsync(N.P, h.data(), N.np * sizeof(float), hipMemcpyHostToDevice, s));
+    DG_HIP(hipMemsetAsync(N.M, 0, N.np * sizeof(float), s));
+    DG_HIP(hipMemsetAsync(N.Vm, 0, N.np * sizeof(float), s));
+    N.step = 0;
+    if (pack_all(s)) return -1;
+    DG_HIP(hipStreamSynchronize(s));
+    N.have_params = true;
+    return 0;
+}
+
+int dg_guess_set_params(const float *p, const int *n)
+{
+    if (ensure_param_storage()) return -1;
+    Net &N = net();
+    if ((size_t)*n != N.np) {
+        set_error("dg_guess_set_params: %d values given, the network has %zu", *n, N.np);
+        return -1;
+    }
+    // padded entries must stay zero
+    std::vector<float> h(p, p + N.np);
+    for (int l = 0; l < NL; l++) {
+        const LayerSpec &L = kL[l];
+        for (int co = 0; co < L.coutp; co++) {
+            for (int t = 0; t < 27; t++)
+                for (int ci = 0; ci < L.cinp; ci++)
+                    if (co >= L.cout || ci >= L.cin) h[N.woff[l] + ((size_t)co * 27 + t) * L.cinp + ci] = 0.f;
+            if (co >= L.cout) h[N.boff[l] + co] = 0.f;
+        }
+    }
+    hipStream_t s = own_stream();
+    DG_HIP(hipMemcpyAsync(N.P, h.data(), N.np * sizeof(float), hipMemcpyHostToDevice, s));
+    DG_HIP(hipMemsetAsync(N.M, 0, N.np * sizeof(float), s));
+    DG_HIP(hipMemsetAsync(N.Vm, 0, N.np * sizeof(float), s));
+    N.step = 0;
+    if (pack_all(s)) return -1;
+    DG_HIP(hipStreamSynchronize(s));
+    N.have_params = true;
+    return 0;
+}
+
+int dg_guess_get_params(float *p, const int *n)
+{
+    if (require_params()) return -1;
+    Net &N = net();
+    if ((size_t)*n != N.np) {
+        set_error("dg_guess_get_params: room for %d values, the network has %zu", *n, N.np);
+        return -1;
+    }
+    DG_HIP(hipDeviceSynchronize());
+    DG_HIP(hipMemcpy(p, N.P, N.np * sizeof(float), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int dg_guess_forward(const int *nx, const int *ny, const int *nz, const float *feat, float *out)
+{
+    if (check_dims(nx, ny, nz) || require_params()) return -1;
+    Net &N = net();
+    hipStream_t s = own_stream();
+    if (ensure_acts(*nx, *ny, *nz, N.a.train)) return -1;
+    const size_t V = N.a.V[0];
+    if (upload_features(feat, V, N.a.x0, s) || net_forward(N.a.x0, s)) return -1;
+    DG_HIP(hipMemcpyAsync(out, N.a.out, V * sizeof(float), hipMemcpyDeviceToHost, s));
+    DG_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+int dg_guess_gradient(const int *nx, const int *ny, const int *nz, const float *feat, const float *target,
+                      double *loss, float *grad)
+{
+    if (check_dims(nx, ny, nz) || require_params()) return -1;
+    Net &N = net();
+    hipStream_t s = own_stream();
+    if (ensure_acts(*nx, *ny, *nz, true)) return -1;
+    const size_t V = N.a.V[0];
+    float *tg = nullptr;
+    double *ld = nullptr;
+    if (dmalloc(&tg, V) || dmalloc(&ld, 1)) return -1;
+    int rc = -1;
+    do {
+        if (hipMemcpyAsync(tg, target, V * sizeof(float), hipMemcpyHostToDevice, s) != hipSuccess) break;
+        if (upload_features(feat, V, N.a.x0, s) || net_forward(N.a.x0, s) || net_loss(tg, ld, nullptr, s) ||
+            net_backward(N.a.x0, s))
+            break;
+        if (hipMemcpyAsync(grad, N.G, N.np * sizeof(float), hipMemcpyDeviceToHost, s) != hipSuccess) break;
+        if (hipMemcpyAsync(loss, ld, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess) break;
+        if (hipStreamSynchronize(s) != hipSuccess) break;
+        rc = 0;
+    } while (0);
+    if (rc) set_error("dg_guess_gradient failed: %s", last_error());
+    dfree(tg);
+    dfree(ld);
+    return rc;
+}
+
+int dg_guess_train_sample(const int *nx, const int *ny, const int *nz, const float *feat, const float *target,
+                          const float *lr, double *loss)
+{
+    if (check_dims(nx, ny, nz) || require_params()) return -1;
+    Net &N = net();
+    hipStream_t s = own_stream();
+    if (ensure_acts(*nx, *ny, *nz, true)) return -1;
+    const size_t V = N.a.V[0];
+    float *tg = nullptr;
+    double *ld = nullptr;
+    if (dmalloc(&tg, V) || dmalloc(&ld, 1)) return -1;
+    int rc = -1;
+    do {
+        if (hipMemcpyAsync(tg, target, V * sizeof(float), hipMemcpyHostToDevice, s) != hipSuccess) break;
+        if (upload_features(feat, V, N.a.x0, s) || net_forward(N.a.x0, s) || net_loss(tg, ld, nullptr, s) ||
+            net_backward(N.a.x0, s) || net_adam(*lr, s))
+            break;
+        if (hipMemcpyAsync(loss, ld, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess) break;
+        if (hipStreamSynchronize(s) != hipSuccess) break;
+        rc = 0;
+    } while (0);
+    dfree(tg);
+    dfree(ld);
+    return rc;
+}
+
+int dg_guess_capture(const int *capacity)
+{
+    Net &N = net();
+    Store &S = N.st;
+    if (*capacity < 0) {
+        set_error("dg_guess_capture: negative capacity");
+        return -1;
+    }
+    if (*capacity == S.cap && S.cap > 0) {
+        N.capture = true;
+        return 0;
+    }
+    DG_HIP(hipDeviceSynchronize());
+    dfree(S.feat);
+    dfree(S.tgt);
+    S = Store();
+    S.cap = *capacity;
+    N.capture = *capacity > 0;
+    N.pending = -1;
+    return 0;
+}
+
+int dg_guess_store_count(int *n)
+{
+    *n = net().st.count;
+    return 0;
+}
+
+int dg_guess_store_get(const int *i, float *feat, float *target, int *dims)
+{
+    Net &N = net();
+    Store &S = N.st;
+    if (*i < 0 || *i >= S.count) {
+        set_error("dg_guess_store_get: sample %d of %d", *i, S.count);
+        return -1;
+    }
+    dims[0] = S.d[0];
+    dims[1] = S.d[1];
+    dims[2] = S.d[2];
+    if (!feat && !target) return 0;
+    DG_HIP(hipDeviceSynchronize());
+    std::vector<bf16_t> h(S.V * FEAT_C);
+    DG_HIP(hipMemcpy(h.data(), S.feat + (size_t)*i * S.V * FEAT_C, h.size() * sizeof(bf16_t), hipMemcpyDeviceToHost));
+    if (feat)
+        for (size_t v = 0; v < S.V; v++)
+            for (int q = 0; q < NFEAT; q++) feat[v * NFEAT + q] = bf2f(h[v * FEAT_C + q]);
+    if (target) DG_HIP(hipMemcpy(target, S.tgt + (size_t)*i * S.V, S.V * sizeof(float), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// epochs over the stored samples (order reshuffled every epoch from seed),
+// one Adam step per sample; loss_per_epoch[e] = mean loss over the epoch
+int dg_guess_train(const int *epochs, const float *lr, const int *seed, double *loss_per_epoch)
+{
+    if (require_params()) return -1;
+    Net &N = net();
+    Store &S = N.st;
+    if (S.count == 0) {
+        set_error("dg_guess_train: the sample store is empty (dg_guess_capture, then run steps)");
+        return -1;
+    }
+    hipStream_t s = own_stream();
+    DG_HIP(hipDeviceSynchronize());
+    if (ensure_acts(S.d[0], S.d[1], S.d[2], true)) return -1;
+    double *ld = nullptr, *acc = nullptr;
+    if (dmalloc(&ld, 1) || dmalloc(&acc, (size_t)(*epochs > 0 ? *epochs : 1))) return -1;
+    std::vector<int> order(S.count);
+    uint64_t st = 0x7a11ull + (uint64_t)(uint32_t)*seed;
+    int rc = 0;
+    for (int e = 0; e < *epochs && !rc; e++) {
+        for (int q = 0; q < S.count; q++) order[q] = q;
+        for (int q = S.count - 1; q > 0; q--) {
+            const int j = (int)(splitmix(st) % (uint64_t)(q + 1));
+            const int t = order[q];
+            order[q] = order[j];
+            order[j] = t;
+        }
+        for (int q = 0; q < S.count && !rc; q++) {
+            const bf16_t *x0 = S.feat + (size_t)order[q] * S.V * FEAT_C;
+            const float *tg = S.tgt + (size_t)order[q] * S.V;
+            if (net_forward(x0, s) || net_loss(tg, ld, acc + e, s) || net_backward(x0, s) || net_adam(*lr, s)) rc = -1;
+        }
+    }
+    if (!rc) {
+        std::vector<double> h(*epochs > 0 ? *epochs : 1);
+        if (hipMemcpyAsync(h.data(), acc, h.size() * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess) {
+            set_error("dg_guess_train: copy of the losses failed");
+            rc = -1;
+        }
+        for (int e = 0; e < *epochs && !rc; e++) loss_per_epoch[e] = h[e] / S.count;
+    }
+    dfree(ld);
+    dfree(acc);
+    return rc;
+}
+
+int dg_guess_stats(int *accepted, int *rejected, double *last_alpha)
+{
+    Net &N = net();
+    *accepted = (int)N.accepted;
+    *rejected = (int)N.rejected;
+    *last_alpha = N.last_alpha;
+    return 0;
+}
+
+int dg_guess_eval(const int *on)
+{
+    net().eval = *on ? 1 : 0;
+    return 0;
+}
+
+int dg_guess_eval_iters(int *iters)
+{
+    Net &N = net();
+    for (int q = 0; q < 3; q++) iters[q] = N.eval_iters[q];
+    return 0;
+}
+
+int dg_guess_release(void)
+{
+    Net &N = net();
+    (void)hipDeviceSynchronize();
+    free_acts(N.a);
+    dfree(N.st.feat);
+    dfree(N.st.tgt);
+    dfree(N.gw.d);
+    dfree(N.gw.s);
+    dfree(N.gw.psave);
+    dfree(N.gw.scal);
+    dfree(N.gw.partial);
+    if (N.gw.hs) (void)hipHostFree(N.gw.hs);
+    dfree(N.P);
+    dfree(N.G);
+    dfree(N.M);
+    dfree(N.Vm);
+    for (int l = 0; l < NL; l++) {
+        dfree(N.wB[l]);
+        dfree(N.wD[l]);
+    }
+    if (N.own) (void)hipStreamDestroy(N.own);
+    N = Net();
+    return 0;
+}
+
+// ---- single-layer entry points (host pointers, FP32 in/out; operands
+// rounded to bf16 on the way in) ----
+
+int dg_conv3d_fwd(const int *nx, const int *ny, const int *nz, const float *x, const int *cinp, const float *w,
+                  const float *bias, const int *cout, const int *coutp, const int *relu, float *y)
+{
+    const size_t V = (size_t)*nx * *ny * *nz;
+    const size_t nw = (size_t)*coutp * 27 * *cinp;
+    hipStream_t s = own_stream();
+    std::vector<bf16_t> hx(V * *cinp);
+    for (size_t q = 0; q < hx.size(); q++) hx[q] = f2bf(x[q]);
+    bf16_t *dx = nullptr, *dwB = nullptr, *dy = nullptr;
+    float *dw = nullptr, *db = nullptr;
+    int rc = -1;
+    if (dmalloc(&dx, hx.size()) || dmalloc(&dwB, nw) || dmalloc(&dy, V * *coutp) || dmalloc(&dw, nw) ||
+        dmalloc(&db, (size_t)*coutp))
+        goto done;
+    if (hipMemcpy(dx, hx.data(), hx.size() * sizeof(bf16_t), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dw, w, nw * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+        goto done;
+    if (bias && hipMemcpy(db, bias, *coutp * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) goto done;
+    if (conv_pack_fwd(dw, *cinp, *coutp, dwB, s)) goto done;
+    if (conv3d_fwd(*nx, *ny, *nz, dx, *cinp, *cinp, dwB, *coutp, bias ? db : nullptr, *cout, *relu, dy, *coutp, nullptr,
+                   0, s))
+        goto done;
+    if (hipStreamSynchronize(s) != hipSuccess) goto done;
+    {
+        std::vector<bf16_t> hy(V * *coutp);
+        if (hipMemcpy(hy.data(), dy, hy.size() * sizeof(bf16_t), hipMemcpyDeviceToHost) != hipSuccess) goto done;
+        for (size_t q = 0; q < hy.size(); q++) y[q] = bf2f(hy[q]);
+    }
+    rc = 0;
+done:
+    if (rc && !status()) set_error("dg_conv3d_fwd: device call failed");
+    dfree(dx);
+    dfree(dwB);
+    dfree(dy);
+    dfree(dw);
+    dfree(db);
+    return rc;
+}
+
+int dg_conv3d_dgrad(const int *nx, const int *ny, const int *nz, const float *dy, const int *coutp, const float *w,
+                    const int *cinp, float *dx)
+{
+    const size_t V = (size_t)*nx * *ny * *nz;
+    const size_t nw = (size_t)*coutp * 27 * *cinp;
+    hipStream_t s = own_stream();
+    std::vector<bf16_t> hy(V * *coutp);
+    for (size_t q = 0; q < hy.size(); q++) hy[q] = f2bf(dy[q]);
+    bf16_t *ddy = nullptr, *dwD = nullptr, *ddx = nullptr;
+    float *dw = nullptr;
+    int rc = -1;
+    if (dmalloc(&ddy, hy.size()) || dmalloc(&dwD, nw) || dmalloc(&ddx, V * *cinp) || dmalloc(&dw, nw)) goto done;
+    if (hipMemcpy(ddy, hy.data(), hy.size() * sizeof(bf16_t), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dw, w, nw * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+        goto done;
+    if (conv_pack_dgrad(dw, *cinp, *coutp, dwD, s)) goto done;
+    if (conv3d_dgrad(*nx, *ny, *nz, ddy, *coutp, *coutp, dwD, *cinp, ddx, *cinp, s)) goto done;
+    if (hipStreamSynchronize(s) != hipSuccess) goto done;
+    {
+        std::vector<bf16_t> hx(V * *cinp);
+        if (hipMemcpy(hx.data(), ddx, hx.size() * sizeof(bf16_t), hipMemcpyDeviceToHost) != hipSuccess) goto done;
+        for (size_t q = 0; q < hx.size(); q++) dx[q] = bf2f(hx[q]);
+    }
+    rc = 0;
+done:
+    if (rc && !status()) set_error("dg_conv3d_dgrad: device call failed");
+    dfree(ddy);
+    dfree(dwD);
+    dfree(ddx);
+    dfree(dw);
+    return rc;
+}
+
+int dg_conv3d_wgrad(const int *nx, const int *ny, const int *nz, const float *x, const int *cinp, const float *dy,
+                    const int *coutp, float *gw, float *gb)
+{
+    const size_t V = (size_t)*nx * *ny * *nz;
+    const size_t nw = (size_t)*coutp * 27 * *cinp;
+    hipStream_t s = own_stream();
+    std::vector<bf16_t> hx(V * *cinp), hy(V * *coutp);
+    for (size_t q = 0; q < hx.size(); q++) hx[q] = f2bf(x[q]);
+    for (size_t q = 0; q < hy.size(); q++) hy[q] = f2bf(dy[q]);
+    const size_t nsc = conv3d_wgrad_scratch(*nx, *ny, *nz, *cinp, *coutp);
+    bf16_t *dx = nullptr, *ddy = nullptr;
+    float *dgw = nullptr, *dgb = nullptr, *sc = nullptr;
+    int rc = -1;
+    if (dmalloc(&dx, hx.size()) || dmalloc(&ddy, hy.size()) || dmalloc(&dgw, nw) || dmalloc(&dgb, (size_t)*coutp) ||
+        dmalloc(&sc, nsc))
+        goto done;
+    if (hipMemcpy(dx, hx.data(), hx.size() * sizeof(bf16_t), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(ddy, hy.data(), hy.size() * sizeof(bf16_t), hipMemcpyHostToDevice) != hipSuccess)
+        goto done;
+    if (conv3d_wgrad(*nx, *ny, *nz, dx, *cinp, *cinp, ddy, *coutp, *coutp, dgw, dgb, sc, nsc, s)) goto done;
+    if (hipStreamSynchronize(s) != hipSuccess) goto done;
+    if (hipMemcpy(gw, dgw, nw * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) goto done;
+    if (gb && hipMemcpy(gb, dgb, *coutp * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) goto done;
+    rc = 0;
+done:
+    if (rc && !status()) set_error("dg_conv3d_wgrad: device call failed");
+    dfree(dx);
+    dfree(ddy);
+    dfree(dgw);
+    dfree(dgb);
+    dfree(sc);
+    return rc;
+}
+
+}  // extern "C"

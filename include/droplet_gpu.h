@@ -82,7 +82,7 @@ int dg_clip_cc(void);
  * mode 0 = the reference's mglin schedule (default, bit-parity path),
  * mode 1 = FP64 PCG with an FP64 V-cycle preconditioner to ||r||/||b|| < tol,
  * mode 2 = FP64 PCG with an FP32 V-cycle preconditioner to ||r||/||b|| < tol.
- * guess: 0 zero, 1 previous step's pressure. */
+ * guess: 0 zero, 1 previous step's pressure, 2 learned (dg_guess_*). */
 int dg_set_solver(const int *mode, const double *tol, const int *nu1, const int *nu2, const int *maxit,
                   const int *guess);
 /* 1 (default): outflow sums in the reference's sequential order
@@ -136,6 +136,59 @@ int dg_fl3d_batch(const int *n, const double *m1, const double *m2, const double
 int dg_poisson_solve(const int *nx, const int *ny, const int *nz, const double *lengthsize, const double *rhs,
                      const double *a, const double *b, const double *c, double *p, const int *mode,
                      const double *tol, const int *maxit, int *iters, double *relres);
+
+/* ---- learned initial guess (no reference counterpart) ------------------ */
+/* A 3D encoder-decoder (channels 16/32/64/64, 3x3x3 convolutions, BF16
+ * weights and activations, FP32 accumulation on the matrix cores) on the
+ * next-coarser multigrid level predicts the change of pressure over a step
+ * from (log10 mean face coefficient / 3, residual of the previous pressure,
+ * right-hand side, previous pressure), each normalised by its rms.  With
+ * dg_set_solver guess = 2 the tolerance-mode solve starts from
+ * p_prev + alpha * prolong(prediction), alpha = (d.r)/(d.Ad), and falls
+ * back to p_prev when that does not lower |b - A p|.  Network extents
+ * ((n-2)/2 per axis) must be multiples of 8.
+ *
+ * Parameters: one FP32 vector, per layer [coutp][27][cinp] weights then
+ * [coutp] biases (padded rows and columns zero). */
+int dg_guess_nparams(int *n);
+int dg_guess_init_random(const int *seed);
+int dg_guess_set_params(const float *params, const int *n);
+int dg_guess_get_params(float *params, const int *n);
+/* network on host arrays: feat [nz][ny][nx][4] -> out [nz][ny][nx] */
+int dg_guess_forward(const int *nx, const int *ny, const int *nz, const float *feat, float *out);
+/* loss 0.5*mean((out-target)^2) and its gradient for every parameter */
+int dg_guess_gradient(const int *nx, const int *ny, const int *nz, const float *feat, const float *target,
+                      double *loss, float *grad);
+/* one Adam step (b1 0.9, b2 0.999, eps 1e-8) on one sample */
+int dg_guess_train_sample(const int *nx, const int *ny, const int *nz, const float *feat, const float *target,
+                          const float *lr, double *loss);
+/* Sample store in device memory: with capacity > 0 every tolerance-mode
+ * solve (guess 1 or 2) stores its network input and target (ring buffer);
+ * 0 frees it. */
+int dg_guess_capture(const int *capacity);
+int dg_guess_store_count(int *n);
+/* sample i: feat [V][4] and target [V] (either may be NULL), dims[3] */
+int dg_guess_store_get(const int *i, float *feat, float *target, int *dims);
+/* epochs of Adam over the stored samples, reshuffled from seed each epoch */
+int dg_guess_train(const int *epochs, const float *lr, const int *seed, double *loss_per_epoch);
+/* solves that took / rejected the learned step, and the last step length */
+int dg_guess_stats(int *accepted, int *rejected, double *last_alpha);
+/* on = 1: each solve runs from the zero, previous and learned guesses in
+ * turn (same system); dg_guess_eval_iters returns the three counts of the
+ * last solve, whose result is the learned solve's. */
+int dg_guess_eval(const int *on);
+int dg_guess_eval_iters(int *iters);
+int dg_guess_release(void);
+/* single layers on host arrays (operands rounded to BF16; x [V][cinp],
+ * w [coutp][27][cinp], y [V][coutp] (values of the BF16 result)) */
+int dg_conv3d_fwd(const int *nx, const int *ny, const int *nz, const float *x, const int *cinp, const float *w,
+                  const float *bias, const int *cout, const int *coutp, const int *relu, float *y);
+/* dx [V][cinp] = adjoint of the convolution applied to dy [V][coutp] */
+int dg_conv3d_dgrad(const int *nx, const int *ny, const int *nz, const float *dy, const int *coutp, const float *w,
+                    const int *cinp, float *dx);
+/* gw [coutp][27][cinp] and gb [coutp] (FP32) from x [V][cinp], dy [V][coutp] */
+int dg_conv3d_wgrad(const int *nx, const int *ny, const int *nz, const float *x, const int *cinp, const float *dy,
+                    const int *coutp, float *gw, float *gb);
 
 /* ---- measurement (no reference counterpart) ---------------------------- */
 /* Times every launch of one kernel family with HIP events recorded on the

@@ -1,0 +1,134 @@
+"""The learned guess inside the tolerance-mode pressure solve.
+
+* Captured samples: the network input and target the library stores for a
+  solve equal a numpy computation from the step's fields (coefficients,
+  right-hand side, previous and new pressure) to BF16 / FP32 rounding.
+* Guess mode 2 with any weights reaches the same tolerance and the same
+  fields as the previous-step guess (the learned step is only taken when it
+  lowers the residual, so the solve cannot be made worse than its fallback).
+* Evaluation mode reports the three iteration counts of one system.
+"""
+import importlib
+
+import numpy as np
+import pytest
+
+from oracle_ffi import inputdata_text
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gs(dg):
+    return importlib.import_module("3d_droplet_simulation_2007_amd.guess")
+
+
+def apply_op(p, a, b, c, h2i):
+    """The solve's operator at interior cells (k_init_resid's boundaries:
+    x-max neighbour 0, other missing neighbours equal to the cell)."""
+    u0 = p[1:-1, 1:-1, 1:-1]
+    ue = p[1:-1, 1:-1, 2:].copy()
+    ue[:, :, -1] = 0.0
+    uw = p[1:-1, 1:-1, :-2].copy()
+    uw[:, :, 0] = u0[:, :, 0]
+    un = p[1:-1, 2:, 1:-1].copy()
+    un[:, -1, :] = u0[:, -1, :]
+    us = p[1:-1, :-2, 1:-1].copy()
+    us[:, 0, :] = u0[:, 0, :]
+    ut = p[2:, 1:-1, 1:-1].copy()
+    ut[-1] = u0[-1]
+    ub = p[:-2, 1:-1, 1:-1].copy()
+    ub[0] = u0[0]
+    t = (a[1:-1, 1:-1, 1:-1] * (ue - u0) + a[1:-1, 1:-1, :-2] * (uw - u0) + b[1:-1, 1:-1, 1:-1] * (un - u0) +
+         b[1:-1, :-2, 1:-1] * (us - u0) + c[1:-1, 1:-1, 1:-1] * (ut - u0) + c[:-2, 1:-1, 1:-1] * (ub - u0))
+    return h2i * t
+
+
+def restrict(f):
+    nz, ny, nx = f.shape
+    return f.reshape(nz // 2, 2, ny // 2, 2, nx // 2, 2).mean(axis=(1, 3, 5))
+
+
+def test_captured_sample_matches_fields(dg, gs):
+    print("case 34^3 capture", flush=True)
+    txt = inputdata_text(nx=34, ny=34, nz=34)
+    g = dg.DropletGPU(inputdata_text=txt)
+    g.set_solver(mode=1, tol=1e-10, guess=1)
+    gs.capture(8)
+    try:
+        for t in range(2):
+            g.step(t)
+        p_old = g.get("pprev")
+        g.step(2)
+        assert gs.store_count() == 3
+        feat, tgt = gs.store_get(2)
+        a, b, c, rhs, p_new = (g.get(k) for k in ("a", "b", "c", "rhs", "p"))
+        n = g.nx - 2
+        h2i = float(n) ** 2  # lengthsize 1 (nondimensional)
+        r0 = rhs[1:-1, 1:-1, 1:-1] - apply_op(p_old, a, b, c, h2i)
+        rms = lambda v: np.sqrt(np.mean(v * v))
+        coef = (a[1:-1, 1:-1, 1:-1] + a[1:-1, 1:-1, :-2] + b[1:-1, 1:-1, 1:-1] + b[1:-1, :-2, 1:-1] +
+                c[1:-1, 1:-1, 1:-1] + c[:-2, 1:-1, 1:-1]) / 6.0
+        ref = np.stack([np.log10(restrict(coef)) / 3.0,
+                        restrict(r0) / rms(r0),
+                        restrict(rhs[1:-1, 1:-1, 1:-1]) / rms(rhs[1:-1, 1:-1, 1:-1]),
+                        restrict(p_old[1:-1, 1:-1, 1:-1]) / rms(p_old[1:-1, 1:-1, 1:-1])], axis=-1)
+        scale = np.abs(ref).max(axis=(0, 1, 2))
+        for q in range(4):
+            err = np.abs(feat[..., q] - ref[..., q]).max()
+            print("feature", q, "max err", err, "scale", scale[q], flush=True)
+            assert err <= 2.0 ** -8 * scale[q] + 1e-6, q
+        tref = restrict(p_new[1:-1, 1:-1, 1:-1] - p_old[1:-1, 1:-1, 1:-1]) / (rms(r0) / h2i)
+        assert np.abs(tgt - tref).max() <= 1e-5 * np.abs(tref).max()
+    finally:
+        gs.capture(0)
+        g.close()
+
+
+def test_learned_guess_reaches_tolerance(dg, gs):
+    """Random weights: every solve still reaches 1e-8 and the fields match
+    the previous-step-guess run to the tolerance-mode bar."""
+    print("case 34^3 learned guess (random weights)", flush=True)
+    txt = inputdata_text(nx=34, ny=34, nz=34)
+    gs.init_random(7)
+    out = {}
+    for guess in (1, 2):
+        g = dg.DropletGPU(inputdata_text=txt)
+        g.set_solver(mode=2, tol=1e-8, guess=guess)
+        before = gs.stats()
+        for t in range(5):
+            g.step(t)
+            st = g.state()
+            assert st["mg_relres"] < 1e-8 and st["mg_iters"] >= 0
+        after = gs.stats()
+        if guess == 2:
+            n = after["accepted"] + after["rejected"] - before["accepted"] - before["rejected"]
+            assert n == 5
+        out[guess] = {k: g.get(k) for k in ("u", "v", "w", "p", "cc")}
+        g.close()
+    for k in out[1]:
+        a, b = out[2][k][1:-1, 1:-1, 1:-1], out[1][k][1:-1, 1:-1, 1:-1]
+        e = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+        print(k, f"{e:.2e}", flush=True)
+        assert e <= 1e-6, (k, e)
+
+
+def test_eval_mode_reports_three_counts(dg, gs):
+    print("case 34^3 evaluation mode", flush=True)
+    txt = inputdata_text(nx=34, ny=34, nz=34)
+    gs.init_random(8)
+    g = dg.DropletGPU(inputdata_text=txt)
+    g.set_solver(mode=2, tol=1e-8, guess=1)
+    try:
+        for t in range(3):
+            g.step(t)
+        gs.eval_mode(True)
+        g.step(3)
+        it = gs.eval_iters()
+        print(it, flush=True)
+        assert it["zero"] > 0 and it["previous"] > 0 and it["learned"] >= 0
+        assert it["zero"] >= it["previous"]
+        assert g.state()["mg_relres"] < 1e-8
+    finally:
+        gs.eval_mode(False)
+        g.close()

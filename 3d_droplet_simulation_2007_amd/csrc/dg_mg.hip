@@ -995,7 +995,7 @@ static int pcg_solve(Ctx &c)
     c.mg_iters_last = it;
     c.mg_relres_last = rel;
     if (op_bc_scalar(g, s, x)) return -1;
-    if (hooks && guess_solve_end(c, x, h2i)) return -1;
+    if (hooks && guess_solve_end(c, x, h2i, rel < c.mg_tol)) return -1;
     DG_HIP(hipMemcpyAsync(c.pprev, x, g.n() * sizeof(double), hipMemcpyDeviceToDevice, s));
     return 0;
 }

@@ -16,10 +16,10 @@ namespace dg {
 // the previous-step guess).  rr_slot receives the new r.r.  When capture is
 // on, the input is also written to the sample store.
 int guess_solve_begin(Ctx &c, double *x, double *r, double h2i, bool apply, double *rr_slot);
-// Called after convergence, before x is copied to the previous pressure:
+// Called after the solve, before x is copied to the previous pressure:
 // completes a captured sample with its target (x - previous pressure on
-// the coarse level, normalised).
-int guess_solve_end(Ctx &c, const double *x, double h2i);
+// the coarse level, normalised); a solve that did not converge drops it.
+int guess_solve_end(Ctx &c, const double *x, double h2i, bool converged);
 // whether the solve must call the hooks (learned guess or capture active)
 bool guess_wanted(const Ctx &c);
 // Evaluation mode (dg_guess_eval): the step's system is solved three times,

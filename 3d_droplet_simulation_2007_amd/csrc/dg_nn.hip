@@ -19,10 +19,12 @@
 //
 // Input features per coarse cell (averages over its 8 fine children):
 //   0: log10 of the mean face coefficient / 3 (log10 of 1/density in the
-//      pure phases, /3), 1: residual of the previous pressure / rms,
-//   2: right-hand side / rms, 3: previous pressure / rms.
+//      pure phases, /3), 1: asinh(100 * residual of the previous pressure
+//      / rms) / 10, 2: asinh(100 * right-hand side / rms) / 10,
+//   3: previous pressure / rms.
 // Target: (converged pressure - previous pressure) averaged to the coarse
-// cell, divided by rms(r) * h^2 of the finest level.
+// cell, scaled to unit rms per sample (the solve's line search sets the
+// length of the step, the network supplies its direction).
 
 #include <hip/hip_runtime.h>
 
@@ -45,6 +47,9 @@ namespace {
 constexpr int NL = 15;
 constexpr int FEAT_C = 8;  // input channels (4 used)
 constexpr int NFEAT = 4;
+// residual and right-hand side are spiky (the interface cells hold most of
+// their energy): they enter as asinh(FEAT_GAIN * x / rms) / FEAT_DIV
+constexpr double FEAT_GAIN = 100.0, FEAT_DIV = 10.0;
 
 struct LayerSpec {
     int cin, cinp, cout, coutp, relu;
@@ -120,7 +125,7 @@ struct GuessWork {  // finest-level buffers of the guess step
 
 // guess scalar slots
 enum { G_PP = 0, G_SR = 1, G_SB = 2, G_SP = 3, G_DR = 4, G_DS = 5, G_RS = 6, G_SS = 7, G_ALPHA = 8, G_ACC = 9,
-       G_RR = 10, G_LOSS = 11, G_ESCALE = 12 };
+       G_RR = 10, G_LOSS = 11, G_ESCALE = 12, G_RR1 = 13 };
 
 struct Net {
     bool have_params = false;
@@ -136,8 +141,11 @@ struct Net {
     hipStream_t own = nullptr;
     bool capture = false;
     int pending = -1;  // store slot waiting for its target
-    long accepted = 0, rejected = 0;
-    double last_alpha = 0.0;
+    long accepted = 0, rejected = 0, skipped = 0;
+    double last_alpha = 0.0, last_gain = 1.0, gain_sum = 0.0;
+    // fallback controller
+    int probe_every = 8, skip_left = 0;
+    double min_gain = 4.0;
     int eval = 0;
     int eval_iters[3] = {0, 0, 0};
 };
@@ -426,8 +434,8 @@ __global__ void k_features(Grid g, int cx, int cy, int cz, const double *a, cons
         const double mc = coef / 48.0;
         bf16_t *q = x0 + v * FEAT_C;
         q[0] = f2bf((float)(mc > 0.0 ? log10(mc) / 3.0 : 0.0));
-        q[1] = f2bf((float)(rr * 0.125 * ir));
-        q[2] = f2bf((float)(bb * 0.125 * ib));
+        q[1] = f2bf((float)(asinh(rr * 0.125 * ir * FEAT_GAIN) / FEAT_DIV));
+        q[2] = f2bf((float)(asinh(bb * 0.125 * ib * FEAT_GAIN) / FEAT_DIV));
         q[3] = f2bf((float)(p * 0.125 * ip));
         q[4] = q[5] = q[6] = q[7] = 0;
     }
@@ -491,6 +499,8 @@ __global__ void k_guess_alpha(double *gs, const double *ds_rr)
     const bool ok = isfinite(alpha) && alpha > 0.0 && isfinite(rr1) && rr1 < rr0;
     gs[G_ALPHA] = ok ? alpha : 0.0;
     gs[G_ACC] = ok ? 1.0 : 0.0;
+    gs[G_RR] = rr0;
+    gs[G_RR1] = ok ? rr1 : rr0;
 }
 
 __global__ void k_guess_update(Grid g, const double *gs, const double *d, const double *s, double *x, double *r)
@@ -506,7 +516,25 @@ __global__ void k_guess_update(Grid g, const double *gs, const double *d, const 
     }
 }
 
-// target of a captured sample: coarse average of (x - pprev) / (rms(r) h^2)
+// target of a captured sample: coarse average of (x - pprev) / (rms(r) h^2),
+// then (k_target_scale) divided by its own rms
+__global__ void __launch_bounds__(256) k_sumsq_f(size_t n, const float *v, double *part)
+{
+    __shared__ double lds4[4];
+    double acc = 0.0;
+    for (size_t p = (size_t)blockIdx.x * 256 + threadIdx.x; p < n; p += (size_t)gridDim.x * 256)
+        acc += (double)v[p] * (double)v[p];
+    const double r = block_reduce256<false>(acc, lds4);
+    if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+__global__ void k_target_scale(size_t n, const double *gs, float *tgt)
+{
+    const double ms = gs[G_ESCALE] / (double)n;
+    const float s = ms > 0.0 ? (float)(1.0 / sqrt(ms)) : 0.0f;
+    for (size_t p = (size_t)blockIdx.x * 256 + threadIdx.x; p < n; p += (size_t)gridDim.x * 256) tgt[p] *= s;
+}
+
 __global__ void k_capture_target(Grid g, int cx, int cy, int cz, const double *x, const double *pp, const double *gs,
                                  double h2, float *tgt)
 {
@@ -584,6 +612,15 @@ int guess_solve_begin(Ctx &c, double *x, double *r, double h2i, bool apply, doub
         set_error("sample capture needs the previous-step guess (dg_set_solver guess 1 or 2)");
         return -1;
     }
+    // fallback controller: after a learned step that cut |r|^2 by less than
+    // min_gain, the next probe_every solves use the previous-step guess
+    // alone (evaluation mode always runs the network)
+    if (apply && !N.eval && N.skip_left > 0) {
+        N.skip_left--;
+        N.skipped++;
+        apply = false;
+    }
+    if (!apply && !capture) return 0;
     if (ensure_param_storage() || ensure_acts(cx, cy, cz, N.a.train) || ensure_work(g.n())) return -1;
     if (apply && !N.packed && pack_all(s)) return -1;
     GuessWork &w = N.gw;
@@ -657,22 +694,39 @@ int guess_solve_begin(Ctx &c, double *x, double *r, double h2i, bool apply, doub
     DG_HIP(hipMemcpyAsync(w.hs, w.scal, 16 * sizeof(double), hipMemcpyDeviceToHost, s));
     DG_HIP(hipStreamSynchronize(s));
     N.last_alpha = w.hs[G_ALPHA];
-    if (w.hs[G_ACC] != 0.0)
+    const bool acc = w.hs[G_ACC] != 0.0;
+    N.last_gain = acc && w.hs[G_RR1] > 0.0 ? w.hs[G_RR] / w.hs[G_RR1] : 1.0;
+    N.gain_sum += N.last_gain;
+    if (acc)
         N.accepted++;
     else
         N.rejected++;
+    if (!N.eval && N.probe_every > 0 && N.last_gain < N.min_gain) N.skip_left = N.probe_every;
     return 0;
 }
 
-int guess_solve_end(Ctx &c, const double *x, double h2i)
+int guess_solve_end(Ctx &c, const double *x, double h2i, bool converged)
 {
     Net &N = net();
     if (N.pending < 0) return 0;
+    if (!converged) {
+        N.pending = -1;
+        return 0;
+    }
     Store &S = N.st;
     Level &F = c.lev[c.ng - 1];
     const Grid g = F.g;
+    float *tg = S.tgt + (size_t)N.pending * S.V;
     hipLaunchKernelGGL(k_capture_target, dim3(gridn(S.V)), dim3(256), 0, c.stream, g, S.d[0], S.d[1], S.d[2], x,
-                       c.pprev, N.gw.scal, 1.0 / h2i, S.tgt + (size_t)N.pending * S.V);
+                       c.pprev, N.gw.scal, 1.0 / h2i, tg);
+    DG_CHECK_LAUNCH();
+    int nb = (int)((S.V + 255) / 256);
+    if (nb > RED_BLOCKS) nb = RED_BLOCKS;
+    hipLaunchKernelGGL(k_sumsq_f, dim3(nb), dim3(256), 0, c.stream, S.V, tg, N.gw.partial);
+    DG_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_reduce_final<false>, dim3(1), dim3(256), 0, c.stream, N.gw.partial, nb, N.gw.scal + G_ESCALE);
+    DG_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_target_scale, dim3(gridn(S.V)), dim3(256), 0, c.stream, S.V, N.gw.scal, tg);
     DG_CHECK_LAUNCH();
     S.next = (S.next + 1) % S.cap;
     if (S.count < S.cap) S.count++;
@@ -1001,12 +1055,28 @@ int dg_guess_train(const int *epochs, const float *lr, const int *seed, double *
     return rc;
 }
 
-int dg_guess_stats(int *accepted, int *rejected, double *last_alpha)
+int dg_guess_stats(int *accepted, int *rejected, int *skipped, double *last_alpha, double *mean_gain)
 {
     Net &N = net();
     *accepted = (int)N.accepted;
     *rejected = (int)N.rejected;
+    *skipped = (int)N.skipped;
     *last_alpha = N.last_alpha;
+    const long n = N.accepted + N.rejected;
+    *mean_gain = n > 0 ? N.gain_sum / (double)n : 0.0;
+    return 0;
+}
+
+int dg_guess_set_controller(const int *probe_every, const double *min_gain)
+{
+    if (*probe_every < 0 || !(*min_gain >= 0.0)) {
+        set_error("dg_guess_set_controller: probe_every >= 0 and min_gain >= 0 required");
+        return -1;
+    }
+    Net &N = net();
+    N.probe_every = *probe_every;
+    N.min_gain = *min_gain;
+    N.skip_left = 0;
     return 0;
 }
 

@@ -131,9 +131,18 @@ def train(epochs, lr, seed=0):
 
 
 def stats():
-    a, r, al = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_double(0.0)
-    _check(lib().dg_guess_stats(ctypes.byref(a), ctypes.byref(r), ctypes.byref(al)), "dg_guess_stats")
-    return dict(accepted=a.value, rejected=r.value, last_alpha=al.value)
+    a, r, sk = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+    al, mg = ctypes.c_double(0.0), ctypes.c_double(0.0)
+    _check(lib().dg_guess_stats(ctypes.byref(a), ctypes.byref(r), ctypes.byref(sk), ctypes.byref(al),
+                                ctypes.byref(mg)), "dg_guess_stats")
+    return dict(accepted=a.value, rejected=r.value, skipped=sk.value, last_alpha=al.value, mean_gain=mg.value)
+
+
+def set_controller(probe_every=8, min_gain=4.0):
+    """Fallback controller: a learned step dividing |r|^2 by less than
+    min_gain sends the next probe_every solves to the previous-step guess."""
+    pe, mg = ctypes.c_int(int(probe_every)), ctypes.c_double(float(min_gain))
+    _check(lib().dg_guess_set_controller(ctypes.byref(pe), ctypes.byref(mg)), "dg_guess_set_controller")
 
 
 def eval_mode(on):

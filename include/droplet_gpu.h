@@ -140,9 +140,11 @@ int dg_poisson_solve(const int *nx, const int *ny, const int *nz, const double *
 /* ---- learned initial guess (no reference counterpart) ------------------ */
 /* A 3D encoder-decoder (channels 16/32/64/64, 3x3x3 convolutions, BF16
  * weights and activations, FP32 accumulation on the matrix cores) on the
- * next-coarser multigrid level predicts the change of pressure over a step
- * from (log10 mean face coefficient / 3, residual of the previous pressure,
- * right-hand side, previous pressure), each normalised by its rms.  With
+ * next-coarser multigrid level predicts the direction of the change of
+ * pressure over a step from (log10 mean face coefficient / 3,
+ * asinh(100 r/rms(r))/10 of the previous pressure's residual r,
+ * asinh(100 b/rms(b))/10 of the right-hand side b, previous pressure /
+ * rms); captured targets are scaled to unit rms per sample.  With
  * dg_set_solver guess = 2 the tolerance-mode solve starts from
  * p_prev + alpha * prolong(prediction), alpha = (d.r)/(d.Ad), and falls
  * back to p_prev when that does not lower |b - A p|.  Network extents
@@ -171,8 +173,14 @@ int dg_guess_store_count(int *n);
 int dg_guess_store_get(const int *i, float *feat, float *target, int *dims);
 /* epochs of Adam over the stored samples, reshuffled from seed each epoch */
 int dg_guess_train(const int *epochs, const float *lr, const int *seed, double *loss_per_epoch);
-/* solves that took / rejected the learned step, and the last step length */
-int dg_guess_stats(int *accepted, int *rejected, double *last_alpha);
+/* solves that took / rejected the learned step, solves the controller ran
+ * without the network, the last step length and the mean of
+ * |r|^2 before / after the learned step over the solves that ran it */
+int dg_guess_stats(int *accepted, int *rejected, int *skipped, double *last_alpha, double *mean_gain);
+/* Fallback controller: a learned step that divides |r|^2 by less than
+ * min_gain sends the next probe_every solves to the previous-step guess
+ * alone (probe_every 0: the network runs on every solve).  Default 8, 4. */
+int dg_guess_set_controller(const int *probe_every, const double *min_gain);
 /* on = 1: each solve runs from the zero, previous and learned guesses in
  * turn (same system); dg_guess_eval_iters returns the three counts of the
  * last solve, whose result is the learned solve's. */

@@ -70,15 +70,16 @@ def test_captured_sample_matches_fields(dg, gs):
         coef = (a[1:-1, 1:-1, 1:-1] + a[1:-1, 1:-1, :-2] + b[1:-1, 1:-1, 1:-1] + b[1:-1, :-2, 1:-1] +
                 c[1:-1, 1:-1, 1:-1] + c[:-2, 1:-1, 1:-1]) / 6.0
         ref = np.stack([np.log10(restrict(coef)) / 3.0,
-                        restrict(r0) / rms(r0),
-                        restrict(rhs[1:-1, 1:-1, 1:-1]) / rms(rhs[1:-1, 1:-1, 1:-1]),
+                        np.arcsinh(100.0 * restrict(r0) / rms(r0)) / 10.0,
+                        np.arcsinh(100.0 * restrict(rhs[1:-1, 1:-1, 1:-1]) / rms(rhs[1:-1, 1:-1, 1:-1])) / 10.0,
                         restrict(p_old[1:-1, 1:-1, 1:-1]) / rms(p_old[1:-1, 1:-1, 1:-1])], axis=-1)
         scale = np.abs(ref).max(axis=(0, 1, 2))
         for q in range(4):
             err = np.abs(feat[..., q] - ref[..., q]).max()
             print("feature", q, "max err", err, "scale", scale[q], flush=True)
             assert err <= 2.0 ** -8 * scale[q] + 1e-6, q
-        tref = restrict(p_new[1:-1, 1:-1, 1:-1] - p_old[1:-1, 1:-1, 1:-1]) / (rms(r0) / h2i)
+        tref = restrict(p_new[1:-1, 1:-1, 1:-1] - p_old[1:-1, 1:-1, 1:-1])
+        tref = tref / rms(tref)
         assert np.abs(tgt - tref).max() <= 1e-5 * np.abs(tref).max()
     finally:
         gs.capture(0)
@@ -91,21 +92,25 @@ def test_learned_guess_reaches_tolerance(dg, gs):
     print("case 34^3 learned guess (random weights)", flush=True)
     txt = inputdata_text(nx=34, ny=34, nz=34)
     gs.init_random(7)
+    gs.set_controller(0, 0.0)  # the network on every solve
     out = {}
-    for guess in (1, 2):
-        g = dg.DropletGPU(inputdata_text=txt)
-        g.set_solver(mode=2, tol=1e-8, guess=guess)
-        before = gs.stats()
-        for t in range(5):
-            g.step(t)
-            st = g.state()
-            assert st["mg_relres"] < 1e-8 and st["mg_iters"] >= 0
-        after = gs.stats()
-        if guess == 2:
-            n = after["accepted"] + after["rejected"] - before["accepted"] - before["rejected"]
-            assert n == 5
-        out[guess] = {k: g.get(k) for k in ("u", "v", "w", "p", "cc")}
-        g.close()
+    try:
+        for guess in (1, 2):
+            g = dg.DropletGPU(inputdata_text=txt)
+            g.set_solver(mode=2, tol=1e-8, guess=guess)
+            before = gs.stats()
+            for t in range(5):
+                g.step(t)
+                st = g.state()
+                assert st["mg_relres"] < 1e-8 and st["mg_iters"] >= 0
+            after = gs.stats()
+            if guess == 2:
+                n = after["accepted"] + after["rejected"] - before["accepted"] - before["rejected"]
+                assert n == 5
+            out[guess] = {k: g.get(k) for k in ("u", "v", "w", "p", "cc")}
+            g.close()
+    finally:
+        gs.set_controller()
     for k in out[1]:
         a, b = out[2][k][1:-1, 1:-1, 1:-1], out[1][k][1:-1, 1:-1, 1:-1]
         e = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
@@ -131,4 +136,28 @@ def test_eval_mode_reports_three_counts(dg, gs):
         assert g.state()["mg_relres"] < 1e-8
     finally:
         gs.eval_mode(False)
+        g.close()
+
+
+def test_fallback_controller_skips_after_a_weak_step(dg, gs):
+    """With min_gain out of reach every learned step counts as weak: the
+    network runs on solves 1, 4, 7, ... (probe_every = 2) and the others use
+    the previous-step guess alone."""
+    print("case 34^3 fallback controller", flush=True)
+    txt = inputdata_text(nx=34, ny=34, nz=34)
+    gs.init_random(9)
+    gs.set_controller(2, 1e300)
+    g = dg.DropletGPU(inputdata_text=txt)
+    g.set_solver(mode=2, tol=1e-8, guess=2)
+    try:
+        before = gs.stats()
+        for t in range(7):
+            g.step(t)
+            assert g.state()["mg_relres"] < 1e-8
+        after = gs.stats()
+        ran = after["accepted"] + after["rejected"] - before["accepted"] - before["rejected"]
+        skipped = after["skipped"] - before["skipped"]
+        assert (ran, skipped) == (3, 4), (ran, skipped)
+    finally:
+        gs.set_controller()
         g.close()

@@ -260,15 +260,22 @@ def poisson_solve(rhs, a, b, c, p0=None, lengthsize=1.0, mode=2, tol=1e-8, maxit
     return p, it.value, rel.value
 
 
-def set_tuning(fused=False, graphs=True):
-    """Tolerance-mode execution switches (results are bit-identical):
-    fused -- one-launch two-colour smoother sweep instead of two colour passes;
-    graphs -- replay each PCG iteration as a captured HIP graph."""
-    _check(lib().dg_set_tuning(_i(1 if fused else 0), _i(1 if graphs else 0)), "dg_set_tuning")
+def set_tuning(fused=False, graphs=True, split=True):
+    """Tolerance-mode execution switches:
+    fused -- one-launch two-colour smoother sweep instead of two colour passes
+             (bit-identical);
+    graphs -- replay each PCG iteration as a captured HIP graph (bit-identical);
+    split -- mode 2's FP32 V-cycle on red-black split storage with FP16
+             coefficients where the densities allow (same tolerance reached);
+             False natural storage, "fp32" / "fp16" split with that
+             coefficient type regardless of the densities."""
+    sv = {False: 0, True: 1, "fp32": 2, "fp16": 3}[split]
+    _check(lib().dg_set_tuning(_i(1 if fused else 0), _i(1 if graphs else 0), _i(sv)), "dg_set_tuning")
 
 
 # kernel families dg_profile_start accepts
-PROFILE_KERNELS = {"gs_colour": 1, "resid": 2, "pcg_dir": 3, "pcg_update": 4, "relax": 5, "sweep": 6}
+PROFILE_KERNELS = {"gs_colour": 1, "resid": 2, "pcg_dir": 3, "pcg_update": 4, "relax": 5, "sweep": 6,
+                   "rb_colour": 7}
 
 
 def profile_start(kernel):

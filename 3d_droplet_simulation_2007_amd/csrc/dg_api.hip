@@ -46,7 +46,7 @@ Prof &prof()
 }
 
 static bool g_ready = false;
-static int g_fused_default = 0, g_graphs_default = 1;
+static int g_fused_default = 0, g_graphs_default = 1, g_split_default = 1;
 
 // ---- allocation -----------------------------------------------------------
 
@@ -133,7 +133,7 @@ static void free_ctx(Ctx &c)
     for (int l = 0; l < c.ng; l++) {
         Level &L = c.lev[l];
         void *ps[] = {L.rhs, L.res, L.u, L.cc, L.a, L.b, L.c, L.ei, L.a32, L.b32, L.c32,
-                      L.u32, L.f32, L.r32, L.u64, L.f64, L.r64, L.w32, L.w64};
+                      L.u32, L.f32, L.r32, L.u64, L.f64, L.r64, L.w32, L.w64, L.rb_u, L.rb_f, L.rb_coef};
         for (void *p : ps)
             if (p) (void)hipFree(p);
         L = Level();
@@ -376,7 +376,14 @@ int dg_init(const char *path)
     if (alloc_fields(c)) return -1;
     c.fused_smoother = g_fused_default;
     c.use_graphs = g_graphs_default;
+    c.rb_split = g_split_default;
     nondim(in, c.s);
+    {
+        // face coefficients lie between 1/rho of the two fluids (rho1 = 1
+        // after nondim): FP16 holds them when that range is moderate
+        const double rlo = c.s.rho2 < 1.0 ? c.s.rho2 : 1.0, rhi = c.s.rho2 > 1.0 ? c.s.rho2 : 1.0;
+        c.coef16_ok = (rlo > 0.0 && 1.0 / rlo <= 3.0e4 && 1.0 / rhi >= 1.0e-3) ? 1 : 0;
+    }
     // cmu = 0.09 (main.f:347-353); mut, turbst and the Reynolds stresses start at 0
     if (mg_fill(c.g.n(), c.stream, c.cmu, 0.09)) return -1;
     g_ready = true;
@@ -534,13 +541,19 @@ int dg_set_vof_guard(const int *on)
     return 0;
 }
 
-int dg_set_tuning(const int *fused, const int *graphs)
+int dg_set_tuning(const int *fused, const int *graphs, const int *split)
 {
     g_fused_default = *fused ? 1 : 0;
     g_graphs_default = *graphs ? 1 : 0;
+    if (*split < 0 || *split > 3) {
+        set_error("dg_set_tuning: split must be 0..3");
+        return -1;
+    }
+    g_split_default = *split;
     Ctx &c = *ctx();
     c.fused_smoother = g_fused_default;
     c.use_graphs = g_graphs_default;
+    c.rb_split = g_split_default;
     return 0;
 }
 
@@ -736,6 +749,8 @@ int dg_poisson_solve(const int *nx, const int *ny, const int *nz, const double *
     lc.mg_mode = *mode; lc.mg_tol = *tol; lc.mg_maxit = *maxit; lc.mg_guess = 1;
     lc.fused_smoother = g_fused_default;
     lc.use_graphs = g_graphs_default;
+    lc.rb_split = g_split_default;
+    lc.coef16_ok = 0;  // arbitrary coefficients: FP32 in the split V-cycle
     if (op_mg_solve_tol(lc)) { free_ctx(lc); return -1; }
     if (hipStreamSynchronize(lc.stream) != hipSuccess ||
         hipMemcpy(p, F.u, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {

@@ -20,6 +20,11 @@ struct Level {
     float *a32 = nullptr, *b32 = nullptr, *c32 = nullptr;
     float *u32 = nullptr, *f32 = nullptr, *r32 = nullptr, *w32 = nullptr;
     double *u64 = nullptr, *f64 = nullptr, *r64 = nullptr, *w64 = nullptr;
+    // red-black split storage of the FP32 V-cycle (levels above the
+    // coarsest): u and f per colour, a/b/c per colour in FP16 or FP32
+    float *rb_u = nullptr, *rb_f = nullptr;
+    void *rb_coef = nullptr;
+    int rb_coef16 = -1;  // type rb_coef was allocated for (-1: none)
 };
 
 struct Scalars {
@@ -56,8 +61,10 @@ struct Ctx {
     int vof_guard = 0;     // 1: plane normal to the sweep where the Youngs normal vanishes
     int fused_smoother = 0;  // 1: one-pass red-black sweeps (slower on gfx950 so far), 0: two colour passes
     int use_graphs = 1;      // replay each PCG iteration as a captured HIP graph
+    int rb_split = 1;        // mode 2: red-black split storage of the V-cycle levels
+    int coef16_ok = 0;       // face coefficients fit FP16 (set from the fluids' densities)
     hipGraphExec_t pcg_graph[2] = {nullptr, nullptr};
-    int pcg_graph_key[6] = {0, 0, 0, 0, 0, 0};
+    int pcg_graph_key[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     // diagnostics of the last step
     int mg_iters_last = 0;
     double mg_relres_last = 0.0, cfl_last = 0.0, epsdiv_last = 0.0;

@@ -18,6 +18,7 @@
 #include "dg_ctx.h"
 #include "dg_nn.h"
 #include "dg_prof.h"
+#include "dg_rb.h"
 #include "dg_util.h"
 
 namespace dg {
@@ -458,6 +459,15 @@ struct TolWork {
 };
 static TolWork g_tw;
 
+// coarsest-level FP32 solve (natural layout f32 -> u32) for the split V-cycle
+int coarse_solve_f32(Ctx &c)
+{
+    Level &L = c.lev[0];
+    hipLaunchKernelGGL(k_coarse_solve<float>, dim3(1), dim3(64), 0, c.stream, L.g, g_tw.M, L.f32, L.u32);
+    DG_CHECK_LAUNCH();
+    return 0;
+}
+
 static inline double lev_h2i(const Ctx &c, const Grid &g)
 {
     const double h = c.s.lengthsize / (double)(g.nx - 2);
@@ -605,9 +615,10 @@ static int vcycle(Ctx &c, int l)
 
 // ---- PCG vector kernels on the finest level (FP64) ----
 
-// K1: partial sums of r.z and z.q (z = the V-cycle result, type T)
-template <typename T>
-__global__ void __launch_bounds__(256) k_dot_rz_zq(Grid g, const double *r, const T *z, const double *q, double *partial)
+// K1: partial sums of r.z and z.q (z = the V-cycle result, read through an
+// accessor: natural layout of type T, or the red-black split FP32 layout)
+template <class Z>
+__global__ void __launch_bounds__(256) k_dot_rz_zq(Grid g, const double *r, Z z, const double *q, double *partial)
 {
     __shared__ double lds4[4];
     const int mx = g.nx - 2, my = g.ny - 2, mz = g.nz - 2;
@@ -618,7 +629,7 @@ __global__ void __launch_bounds__(256) k_dot_rz_zq(Grid g, const double *r, cons
         long long t = p / mx;
         int j = (int)(t % my) + 2, k = (int)(t / my) + 2;
         const int o = g.at(i, j, k);
-        const double zz = (double)z[o];
+        const double zz = z(i, j, k, o);
         a1 += r[o] * zz;
         a2 += zz * q[o];
     }
@@ -651,8 +662,8 @@ enum { S_RZ = 16, S_ZQ, S_RZOLD, S_ALPHA, S_PQ, S_RR, S_BB, S_FIRST };
 
 // K2: p_new = z + beta p_old (beta from device scalars), q = A p_new,
 // partial sums of p_new . q
-template <typename T>
-__global__ void __launch_bounds__(256) k_pcg_dir(Grid g, const T *z, const double *pold, double *pnew, double *q,
+template <class Z>
+__global__ void __launch_bounds__(256) k_pcg_dir(Grid g, Z z, const double *pold, double *pnew, double *q,
                                                  const double *a, const double *b, const double *c, double h2i,
                                                  const double *ds, double *partial)
 {
@@ -667,14 +678,14 @@ __global__ void __launch_bounds__(256) k_pcg_dir(Grid g, const T *z, const doubl
     const double beta = first ? 0.0 : -ds[S_ALPHA] * ds[S_ZQ] / ds[S_RZOLD];
     if (i <= g.nx - 1 && j <= g.ny - 1) {
         const int o = g.at(i, j, k), sy = g.sy, sz = g.sz;
-        auto P = [&](int oo) { return (double)z[oo] + beta * pold[oo]; };
-        const double p0 = P(o);
-        const double pe = i < g.nx - 1 ? P(o + 1) : 0.0;
-        const double pw = i > 2 ? P(o - 1) : p0;
-        const double pn = j < g.ny - 1 ? P(o + sy) : p0;
-        const double ps = j > 2 ? P(o - sy) : p0;
-        const double pt = k < g.nz - 1 ? P(o + sz) : p0;
-        const double pb = k > 2 ? P(o - sz) : p0;
+        auto P = [&](int ii, int jj, int kk, int oo) { return z(ii, jj, kk, oo) + beta * pold[oo]; };
+        const double p0 = P(i, j, k, o);
+        const double pe = i < g.nx - 1 ? P(i + 1, j, k, o + 1) : 0.0;
+        const double pw = i > 2 ? P(i - 1, j, k, o - 1) : p0;
+        const double pn = j < g.ny - 1 ? P(i, j + 1, k, o + sy) : p0;
+        const double ps = j > 2 ? P(i, j - 1, k, o - sy) : p0;
+        const double pt = k < g.nz - 1 ? P(i, j, k + 1, o + sz) : p0;
+        const double pb = k > 2 ? P(i, j, k - 1, o - sz) : p0;
         const double s = a[o] * (pe - p0) + a[o - 1] * (pw - p0) + b[o] * (pn - p0) + b[o - sy] * (ps - p0) +
                          c[o] * (pt - p0) + c[o - sz] * (pb - p0);
         const double qq = h2i * s;
@@ -877,21 +888,37 @@ static int pcg_iteration(Ctx &c, double *pa, double *pb)
     T *u, *f, *rr_, *w_;
     const T *a_, *b_, *c_;
     level_ptrs<T>(F, u, f, rr_, a_, b_, c_, w_);
-    if (sizeof(T) == 4)
-        hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((g.n() + 255) / 256)), dim3(256), 0, s, g.n(), r, (float *)f);
-    else
-        hipLaunchKernelGGL(k_copy_cast, dim3((unsigned)((g.n() + 255) / 256)), dim3(256), 0, s, g.n(), r, (double *)f);
-    DG_CHECK_LAUNCH();
-    if (vcycle<T>(c, c.ng - 1)) return -1;
-    hipLaunchKernelGGL(k_dot_rz_zq<T>, dim3(RED_BLOCKS), dim3(256), 0, s, g, r, (const T *)u, q, partial);
-    DG_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_final2, dim3(1), dim3(256), 0, s, partial, RED_BLOCKS, ds + S_RZ);
-    DG_CHECK_LAUNCH();
     const double ncell = (double)(g.nx - 2) * (g.ny - 2) * (g.nz - 2);
-    prof_begin(PK_PCG_DIR, s);
-    hipLaunchKernelGGL(k_pcg_dir<T>, ig, ib, 0, s, g, (const T *)u, pa, pb, q, F.a, F.b, F.c, h2i, ds, partial);
-    prof_end(PK_PCG_DIR, s, ((double)sizeof(T) + 6.0 * 8.0) * ncell);
-    DG_CHECK_LAUNCH();
+    if (sizeof(T) == 4 && rb_enabled(c)) {
+        // z = M r through the red-black split V-cycle
+        if (rb_vcycle(c, r)) return -1;
+        const ZSplit z = rb_top_z(c);
+        hipLaunchKernelGGL(k_dot_rz_zq<ZSplit>, dim3(RED_BLOCKS), dim3(256), 0, s, g, r, z, q, partial);
+        DG_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_final2, dim3(1), dim3(256), 0, s, partial, RED_BLOCKS, ds + S_RZ);
+        DG_CHECK_LAUNCH();
+        prof_begin(PK_PCG_DIR, s);
+        hipLaunchKernelGGL(k_pcg_dir<ZSplit>, ig, ib, 0, s, g, z, pa, pb, q, F.a, F.b, F.c, h2i, ds, partial);
+        prof_end(PK_PCG_DIR, s, (4.0 + 6.0 * 8.0) * ncell);
+        DG_CHECK_LAUNCH();
+    } else {
+        if (sizeof(T) == 4)
+            hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((g.n() + 255) / 256)), dim3(256), 0, s, g.n(), r, (float *)f);
+        else
+            hipLaunchKernelGGL(k_copy_cast, dim3((unsigned)((g.n() + 255) / 256)), dim3(256), 0, s, g.n(), r,
+                               (double *)f);
+        DG_CHECK_LAUNCH();
+        if (vcycle<T>(c, c.ng - 1)) return -1;
+        const ZNatural<T> z{(const T *)u, g};
+        hipLaunchKernelGGL(k_dot_rz_zq<ZNatural<T>>, dim3(RED_BLOCKS), dim3(256), 0, s, g, r, z, q, partial);
+        DG_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_final2, dim3(1), dim3(256), 0, s, partial, RED_BLOCKS, ds + S_RZ);
+        DG_CHECK_LAUNCH();
+        prof_begin(PK_PCG_DIR, s);
+        hipLaunchKernelGGL(k_pcg_dir<ZNatural<T>>, ig, ib, 0, s, g, z, pa, pb, q, F.a, F.b, F.c, h2i, ds, partial);
+        prof_end(PK_PCG_DIR, s, ((double)sizeof(T) + 6.0 * 8.0) * ncell);
+        DG_CHECK_LAUNCH();
+    }
     if (sum_partials(s, partial, nblk, 1, partial + 2 * (size_t)nblk, ds + S_PQ)) return -1;
     prof_begin(PK_PCG_UPDATE, s);
     hipLaunchKernelGGL(k_pcg_update, ig, ib, 0, s, g, x, r, pb, q, ds, partial);
@@ -912,9 +939,11 @@ static int pcg_iteration(Ctx &c, double *pa, double *pb)
 template <typename T>
 static int pcg_graphs(Ctx &c)
 {
-    const int key[6] = {c.mg_mode, c.mg_nu1, c.mg_nu2, c.fused_smoother, c.ng, (int)c.g.nx};
+    const int rb = sizeof(T) == 4 && rb_enabled(c);
+    const int key[8] = {c.mg_mode, c.mg_nu1, c.mg_nu2, c.fused_smoother, c.ng, (int)c.g.nx,
+                        rb, rb ? c.lev[c.ng - 1].rb_coef16 : -1};
     bool ok = c.pcg_graph[0] && c.pcg_graph[1];
-    for (int q = 0; q < 6 && ok; q++) ok = c.pcg_graph_key[q] == key[q];
+    for (int q = 0; q < 8 && ok; q++) ok = c.pcg_graph_key[q] == key[q];
     if (ok) return 0;
     for (int q = 0; q < 2; q++) {
         if (c.pcg_graph[q]) {
@@ -936,7 +965,7 @@ static int pcg_graphs(Ctx &c)
         DG_HIP(hipGraphInstantiate(&c.pcg_graph[par], graph, nullptr, nullptr, 0));
         DG_HIP(hipGraphDestroy(graph));
     }
-    for (int q = 0; q < 6; q++) c.pcg_graph_key[q] = key[q];
+    for (int q = 0; q < 8; q++) c.pcg_graph_key[q] = key[q];
     return 0;
 }
 
@@ -1010,7 +1039,9 @@ int op_mg_solve_tol(Ctx &c)
         Level &C = c.lev[gi - 1], &F = c.lev[gi];
         if (mg_acfromac(C.g, s, C.a, C.b, C.c, C.ei, F.a, F.b, F.c)) return -1;
     }
-    if (c.mg_mode == 2) {
+    if (c.mg_mode == 2 && rb_enabled(c)) {
+        if (rb_prepare(c)) return -1;
+    } else if (c.mg_mode == 2) {
         for (int l = 0; l < c.ng; l++) {
             Level &L = c.lev[l];
             const size_t n = L.g.n();

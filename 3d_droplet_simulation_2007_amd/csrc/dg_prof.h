@@ -17,6 +17,7 @@ enum ProfKernel {
     PK_PCG_UPDATE = 4,  // PCG x/r update (finest level, FP64)
     PK_RELAX = 5,       // reference relax colour pass (parity mode)
     PK_SWEEP = 6,       // fused two-colour smoother sweep
+    PK_RB_COLOUR = 7,   // finest-level colour pass of the red-black split FP32 V-cycle
 };
 
 struct Prof {

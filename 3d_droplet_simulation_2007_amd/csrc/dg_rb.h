@@ -1,0 +1,74 @@
+// dg_rb.h -- red-black split storage of the FP32 V-cycle (tolerance mode 2).
+//
+// On every level above the coarsest, the unknowns u, the right-hand side f
+// and the face coefficients a, b, c of the interior cells are stored per
+// colour, colour = (i+j+k)&1 (1-based indices), in two halves of one array:
+//   X[colour * half + ((k-2)*my + (j-2))*mh + ((i-2)>>1)],
+//   mx = nx-2 (even), mh = mx/2, my = ny-2, mz = nz-2, half = mh*my*mz.
+// Along a row (j,k) the cells of one colour are every other i, so a colour
+// pass reads its own f and coefficients and the other colour's u and
+// coefficients as contiguous streams and writes its own u contiguously: no
+// half-used cache lines, and the other colour's unknowns are half the size
+// of a full array.  Coefficients are stored in FP16 when their range allows
+// (face values 1/rho of the two fluids) -- the V-cycle is a preconditioner
+// of the FP64 PCG, which keeps the exact operator.
+#pragma once
+
+#include <hip/hip_fp16.h>
+
+#include "dg_ctx.h"
+
+namespace dg {
+
+struct RbGeom {
+    int nx, ny, nz;  // extents with ghosts
+    int mh, my, mz;  // half row, interior y, interior z
+    size_t half;     // cells of one colour
+    __host__ __device__ inline size_t idx(int i, int j, int k) const
+    {
+        return ((size_t)(k - 2) * my + (j - 2)) * mh + ((i - 2) >> 1);
+    }
+    __host__ __device__ inline size_t at(int i, int j, int k) const
+    {
+        return (size_t)((i + j + k) & 1) * half + idx(i, j, k);
+    }
+};
+
+inline RbGeom rb_geom(const Grid &g)
+{
+    RbGeom r;
+    r.nx = g.nx;
+    r.ny = g.ny;
+    r.nz = g.nz;
+    r.mh = (g.nx - 2) / 2;
+    r.my = g.ny - 2;
+    r.mz = g.nz - 2;
+    r.half = (size_t)r.mh * r.my * r.mz;
+    return r;
+}
+
+// z accessors for the PCG kernels: the V-cycle result at interior (i,j,k)
+template <typename T>
+struct ZNatural {
+    const T *z;
+    Grid g;
+    __device__ inline double operator()(int i, int j, int k, int o) const { return (double)z[o]; }
+};
+struct ZSplit {
+    const float *z;
+    RbGeom r;
+    __device__ inline double operator()(int i, int j, int k, int o) const { return (double)z[r.at(i, j, k)]; }
+};
+
+// whether the split path serves this context's mode-2 solves
+bool rb_enabled(const Ctx &c);
+// allocates the split arrays and builds the split coefficients of every
+// level above the coarsest for this solve (after the coarse operators and
+// their FP32 copies exist)
+int rb_prepare(Ctx &c);
+// z = V-cycle(r) from a zero guess; r: FP64 finest-level residual (natural
+// layout); the result stays in the finest level's split u (rb_top_z)
+int rb_vcycle(Ctx &c, const double *r);
+ZSplit rb_top_z(const Ctx &c);
+
+}  // namespace dg

@@ -97,6 +97,7 @@ def test_learned_guess_reaches_tolerance(dg, gs):
     try:
         for guess in (1, 2):
             g = dg.DropletGPU(inputdata_text=txt)
+            g.set_vof_guard(True)  # keeps the run continuous in the solver's rounding
             g.set_solver(mode=2, tol=1e-8, guess=guess)
             before = gs.stats()
             for t in range(5):

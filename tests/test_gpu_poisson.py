@@ -46,18 +46,26 @@ def sphere_system(n, seed):
     return rhs, a, b, c
 
 
-def test_tuning_switches_are_bit_identical(dg):
-    print("case tuning 34^3", flush=True)
-    rhs, a, b, c = sphere_system(34, 1)
+@pytest.mark.parametrize("n", [34, 66])
+def test_tuning_switches_are_bit_identical(dg, n):
+    """Fused or two-pass sweeps, graph replay or not, red-black split or
+    natural storage: the same FP32 arithmetic in the same order (the
+    standalone solve keeps FP32 coefficients), so the same bits."""
+    print("case tuning", n, flush=True)
+    rhs, a, b, c = sphere_system(n, 1)
     outs = []
-    for fused, graphs in ((True, True), (False, True), (True, False), (False, False)):
-        dg.set_tuning(fused=fused, graphs=graphs)
-        p, it, rel = dg.poisson_solve(rhs, a, b, c, mode=2, tol=1e-9)
-        outs.append((p, it, rel))
-    dg.set_tuning(True, True)
-    for p, it, rel in outs[1:]:
-        assert it == outs[0][1]
-        assert np.array_equal(p, outs[0][0])
+    combos = [(False, True, True), (False, False, True), (False, True, False), (True, True, False),
+              (True, False, False), (False, False, False)]
+    try:
+        for fused, graphs, split in combos:
+            dg.set_tuning(fused=fused, graphs=graphs, split=split)
+            p, it, rel = dg.poisson_solve(rhs, a, b, c, mode=2, tol=1e-9)
+            outs.append((p, it, rel))
+    finally:
+        dg.set_tuning()
+    for (p, it, rel), combo in zip(outs[1:], combos[1:]):
+        assert it == outs[0][1], combo
+        assert np.array_equal(p, outs[0][0]), combo
 
 
 @pytest.mark.parametrize("n", [34, 66, 130])

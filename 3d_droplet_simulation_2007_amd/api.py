@@ -260,6 +260,12 @@ def poisson_solve(rhs, a, b, c, p0=None, lengthsize=1.0, mode=2, tol=1e-8, maxit
     return p, it.value, rel.value
 
 
+def set_smoother(omega=1.0, nu_coarse=0):
+    """Tolerance-mode smoother: SOR weight of the red-black passes and the
+    sweep count on the levels below the finest (0: the solver's nu1/nu2)."""
+    _check(lib().dg_set_smoother(_d(omega), _i(nu_coarse)), "dg_set_smoother")
+
+
 def set_tuning(fused=False, graphs=True, split=True):
     """Tolerance-mode execution switches:
     fused -- one-launch two-colour smoother sweep instead of two colour passes

@@ -47,6 +47,8 @@ Prof &prof()
 
 static bool g_ready = false;
 static int g_fused_default = 0, g_graphs_default = 1, g_split_default = 1;
+static double g_omega_default = 1.0;  // tolerance-mode smoother (dg_set_smoother)
+static int g_nuc_default = 0;
 
 // ---- allocation -----------------------------------------------------------
 
@@ -377,6 +379,8 @@ int dg_init(const char *path)
     c.fused_smoother = g_fused_default;
     c.use_graphs = g_graphs_default;
     c.rb_split = g_split_default;
+    c.mg_omega = g_omega_default;
+    c.mg_nuc = g_nuc_default;
     nondim(in, c.s);
     {
         // face coefficients lie between 1/rho of the two fluids (rho1 = 1
@@ -531,6 +535,20 @@ int dg_set_ordered_sums(const int *on)
 {
     if (require_ready()) return -1;
     ctx()->ordered_sums = *on ? 1 : 0;
+    return 0;
+}
+
+int dg_set_smoother(const double *omega, const int *nu_coarse)
+{
+    if (!(*omega > 0.0 && *omega < 2.0) || *nu_coarse < 0) {
+        set_error("dg_set_smoother: 0 < omega < 2 and nu_coarse >= 0 required");
+        return -1;
+    }
+    g_omega_default = *omega;
+    g_nuc_default = *nu_coarse;
+    Ctx &c = *ctx();
+    c.mg_omega = g_omega_default;
+    c.mg_nuc = g_nuc_default;
     return 0;
 }
 
@@ -751,6 +769,8 @@ int dg_poisson_solve(const int *nx, const int *ny, const int *nz, const double *
     lc.use_graphs = g_graphs_default;
     lc.rb_split = g_split_default;
     lc.coef16_ok = 0;  // arbitrary coefficients: FP32 in the split V-cycle
+    lc.mg_omega = g_omega_default;
+    lc.mg_nuc = g_nuc_default;
     if (op_mg_solve_tol(lc)) { free_ctx(lc); return -1; }
     if (hipStreamSynchronize(lc.stream) != hipSuccess ||
         hipMemcpy(p, F.u, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {

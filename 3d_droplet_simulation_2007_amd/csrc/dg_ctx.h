@@ -57,6 +57,8 @@ struct Ctx {
     int mg_mode = 0;       // 0 reference FMG schedule, 1 FP64 PCG, 2 FP32-preconditioned FP64 PCG
     int mg_nu1 = 2, mg_nu2 = 2, mg_maxit = 200, mg_guess = 0;
     double mg_tol = 1e-8;
+    double mg_omega = 1.0;  // SOR weight of the tolerance-mode smoother (1: Gauss-Seidel)
+    int mg_nuc = 0;         // sweeps (pre and post) below the finest level; 0: mg_nu1 / mg_nu2
     int ordered_sums = 1;  // reproduce the reference's sequential outflow sums
     int vof_guard = 0;     // 1: plane normal to the sweep where the Youngs normal vanishes
     int fused_smoother = 0;  // 1: one-pass red-black sweeps (slower on gfx950 so far), 0: two colour passes

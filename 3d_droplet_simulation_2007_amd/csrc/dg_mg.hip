@@ -195,11 +195,14 @@ int op_mglin(Ctx &c)
 // neighbour equals the cell itself (the face drops out), across x-max the
 // ghost is 0.
 
-// Gauss-Seidel colour pass with the operator's own diagonal, in place.
-// TOP only separates the finest level's launches into their own symbol.
+// Gauss-Seidel colour pass with the operator's own diagonal, in place, with
+// SOR weight om: u = om * gs + (1 - om) * u (om = 1: Gauss-Seidel).  From a
+// zero iterate the first pass reads no neighbour (zero_init) and the first
+// two passes no old value of their own cells (zero_own).  TOP only
+// separates the finest level's launches into their own symbol.
 template <typename T, bool TOP>
 __global__ void __launch_bounds__(256) k_gs_colour(Grid g, T *u, const T *f, const T *a, const T *b, const T *c, T h2,
-                                                   int colour, int zero_init)
+                                                   int colour, int zero_init, T om, int zero_own)
 {
     const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + 2;
     const int k = (int)blockIdx.z + 2;
@@ -221,7 +224,9 @@ __global__ void __launch_bounds__(256) k_gs_colour(Grid g, T *u, const T *f, con
     if (j > 2) d += bs;
     if (k < g.nz - 1) d += ct;
     if (k > 2) d += cb;
-    u[o] = (s - h2 * f[o]) / d;
+    const T gsv = (s - h2 * f[o]) / d;
+    const T uold = (zero_own || om == (T)1) ? (T)0 : u[o];  // Gauss-Seidel needs no old value
+    u[o] = om * gsv + ((T)1 - om) * uold;
 }
 
 // One full red-black sweep (colour `first` then the other) in a single pass,
@@ -515,7 +520,7 @@ static int fused_sweep(Ctx &c, const Grid &g, const T *u_in, T *u_out, const T *
 
 template <typename T>
 static int gs_sweep(Ctx &c, const Grid &g, T *u, const T *f, const T *a, const T *b, const T *cc, T h2, int first,
-                    int second, int zero_first, bool top)
+                    int second, int zero_first, bool top, T om)
 {
     const int half = (g.nx - 2 + 1) / 2;
     dim3 grid((unsigned)((half + BX - 1) / BX), (unsigned)((g.ny - 2 + BY - 1) / BY), (unsigned)(g.nz - 2));
@@ -524,21 +529,22 @@ static int gs_sweep(Ctx &c, const Grid &g, T *u, const T *f, const T *a, const T
     const double bytes = 3.5 * (double)sizeof(T) * (double)(g.nx - 2) * (g.ny - 2) * (g.nz - 2);
     if (!top) {
         hipLaunchKernelGGL((k_gs_colour<T, false>), grid, interior_block(), 0, c.stream, g, u, f, a, b, cc, h2, first,
-                           zero_first);
+                           zero_first, om, zero_first);
         DG_CHECK_LAUNCH();
         hipLaunchKernelGGL((k_gs_colour<T, false>), grid, interior_block(), 0, c.stream, g, u, f, a, b, cc, h2, second,
-                           0);
+                           0, om, zero_first);
         DG_CHECK_LAUNCH();
         return 0;
     }
     // the finest level: the profiled kernel family PK_GS_COLOUR
     prof_begin(PK_GS_COLOUR, c.stream);
     hipLaunchKernelGGL((k_gs_colour<T, true>), grid, interior_block(), 0, c.stream, g, u, f, a, b, cc, h2, first,
-                       zero_first);
+                       zero_first, om, zero_first);
     prof_end(PK_GS_COLOUR, c.stream, bytes);
     DG_CHECK_LAUNCH();
     prof_begin(PK_GS_COLOUR, c.stream);
-    hipLaunchKernelGGL((k_gs_colour<T, true>), grid, interior_block(), 0, c.stream, g, u, f, a, b, cc, h2, second, 0);
+    hipLaunchKernelGGL((k_gs_colour<T, true>), grid, interior_block(), 0, c.stream, g, u, f, a, b, cc, h2, second, 0,
+                       om, zero_first);
     prof_end(PK_GS_COLOUR, c.stream, bytes);
     DG_CHECK_LAUNCH();
     return 0;
@@ -561,6 +567,13 @@ static int vcycle(Ctx &c, int l)
     const double h2i = lev_h2i(c, g);
     const T h2 = (T)(1.0 / h2i), th2i = (T)h2i;
     const bool top = l == c.ng - 1;
+    const int nu1 = (top || c.mg_nuc <= 0) ? c.mg_nu1 : c.mg_nuc;
+    const int nu2 = (top || c.mg_nuc <= 0) ? c.mg_nu2 : c.mg_nuc;
+    const T om = (T)c.mg_omega;
+    if (c.fused_smoother && (c.mg_omega != 1.0 || nu1 != c.mg_nu1)) {
+        set_error("the fused smoother sweep supports Gauss-Seidel with equal sweep counts on all levels only");
+        return -1;
+    }
     // cur: buffer holding the level's iterate.  With the fused (out-of-place)
     // sweeps the buffers alternate; the first one is chosen so that the last
     // post-smoothing sweep lands in u, where the caller reads the result.
@@ -576,10 +589,10 @@ static int vcycle(Ctx &c, int l)
             if (fused_sweep<T>(c, g, prev, cur, f, a, b, cc, h2, 0, sidx == 0, top)) return -1;
         }
     } else {
-        for (int sidx = 0; sidx < c.mg_nu1; sidx++)
-            if (gs_sweep<T>(c, g, u, f, a, b, cc, h2, 0, 1, sidx == 0, top)) return -1;
+        for (int sidx = 0; sidx < nu1; sidx++)
+            if (gs_sweep<T>(c, g, u, f, a, b, cc, h2, 0, 1, sidx == 0, top, om)) return -1;
     }
-    if (c.mg_nu1 == 0) {
+    if (nu1 == 0) {
         hipLaunchKernelGGL(k_zero<T>, dim3((unsigned)((g.n() + 255) / 256)), dim3(256), 0, c.stream, g.n(), cur);
         DG_CHECK_LAUNCH();
     }
@@ -607,8 +620,8 @@ static int vcycle(Ctx &c, int l)
             return -1;
         }
     } else {
-        for (int sidx = 0; sidx < c.mg_nu2; sidx++)
-            if (gs_sweep<T>(c, g, u, f, a, b, cc, h2, 1, 0, 0, top)) return -1;
+        for (int sidx = 0; sidx < nu2; sidx++)
+            if (gs_sweep<T>(c, g, u, f, a, b, cc, h2, 1, 0, 0, top, om)) return -1;
     }
     return 0;
 }
@@ -701,8 +714,10 @@ __global__ void __launch_bounds__(256) k_pcg_dir(Grid g, Z z, const double *pold
 }
 
 // K3: alpha = rz/pq; x += alpha p; r -= alpha q; partial sums of r.r
+// (fs != null: also the FP32 red-black split copy of the new r, the next
+// V-cycle's right-hand side)
 __global__ void __launch_bounds__(256) k_pcg_update(Grid g, double *x, double *r, const double *p, const double *q,
-                                                    double *ds, double *partial)
+                                                    double *ds, double *partial, float *fs, RbGeom rg)
 {
     __shared__ double lds4[4];
     const int bidx = (int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
@@ -717,6 +732,7 @@ __global__ void __launch_bounds__(256) k_pcg_update(Grid g, double *x, double *r
         x[o] = x[o] + alpha * p[o];
         const double rn = r[o] - alpha * q[o];
         r[o] = rn;
+        if (fs) fs[rg.at(i, j, k)] = (float)rn;
         contrib = rn * rn;
     }
     double v = wave_sum(contrib);
@@ -889,9 +905,12 @@ static int pcg_iteration(Ctx &c, double *pa, double *pb)
     const T *a_, *b_, *c_;
     level_ptrs<T>(F, u, f, rr_, a_, b_, c_, w_);
     const double ncell = (double)(g.nx - 2) * (g.ny - 2) * (g.nz - 2);
-    if (sizeof(T) == 4 && rb_enabled(c)) {
-        // z = M r through the red-black split V-cycle
-        if (rb_vcycle(c, r)) return -1;
+    const bool split = sizeof(T) == 4 && rb_enabled(c);
+    if (split) {
+        // z = M r through the red-black split V-cycle; its right-hand side
+        // (the FP32 split copy of r) was written by the previous update or
+        // by the solve's set-up
+        if (rb_vcycle(c, nullptr)) return -1;
         const ZSplit z = rb_top_z(c);
         hipLaunchKernelGGL(k_dot_rz_zq<ZSplit>, dim3(RED_BLOCKS), dim3(256), 0, s, g, r, z, q, partial);
         DG_CHECK_LAUNCH();
@@ -921,8 +940,9 @@ static int pcg_iteration(Ctx &c, double *pa, double *pb)
     }
     if (sum_partials(s, partial, nblk, 1, partial + 2 * (size_t)nblk, ds + S_PQ)) return -1;
     prof_begin(PK_PCG_UPDATE, s);
-    hipLaunchKernelGGL(k_pcg_update, ig, ib, 0, s, g, x, r, pb, q, ds, partial);
-    prof_end(PK_PCG_UPDATE, s, 6.0 * 8.0 * ncell);
+    hipLaunchKernelGGL(k_pcg_update, ig, ib, 0, s, g, x, r, pb, q, ds, partial, split ? F.rb_f : nullptr,
+                       rb_geom(g));
+    prof_end(PK_PCG_UPDATE, s, (6.0 * 8.0 + (split ? 4.0 : 0.0)) * ncell);
     DG_CHECK_LAUNCH();
     if (sum_partials(s, partial, nblk, 1, partial + 2 * (size_t)nblk, ds + S_RR)) return -1;
     hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(1), 0, s, ds);
@@ -993,6 +1013,8 @@ static int pcg_solve(Ctx &c)
     // previous-step guess's residual
     const bool hooks = guess_wanted(c);
     if (hooks && guess_solve_begin(c, x, r, h2i, c.mg_guess == 2, ds + S_RR)) return -1;
+    // the split V-cycle's right-hand side of the first iteration
+    if (sizeof(T) == 4 && rb_enabled(c) && rb_load_rhs(c, r)) return -1;
     // coarse factorisation for this solve, FIRST flag
     {
         Level &L0 = c.lev[0];

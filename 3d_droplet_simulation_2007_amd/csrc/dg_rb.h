@@ -66,8 +66,11 @@ bool rb_enabled(const Ctx &c);
 // level above the coarsest for this solve (after the coarse operators and
 // their FP32 copies exist)
 int rb_prepare(Ctx &c);
+// finest-level split right-hand side (FP32) from r (FP64, natural layout)
+int rb_load_rhs(Ctx &c, const double *r);
 // z = V-cycle(r) from a zero guess; r: FP64 finest-level residual (natural
-// layout); the result stays in the finest level's split u (rb_top_z)
+// layout), or null when the split right-hand side already holds it; the
+// result stays in the finest level's split u (rb_top_z)
 int rb_vcycle(Ctx &c, const double *r);
 ZSplit rb_top_z(const Ctx &c);
 

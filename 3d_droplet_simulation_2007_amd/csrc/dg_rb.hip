@@ -64,7 +64,7 @@ __global__ void k_rb_in(Grid g, RbGeom r, const double *src, float *dst)
 // TOP only separates the finest level's launches into their own symbol.
 template <typename CT, bool ZERO, bool TOP>
 __global__ void __launch_bounds__(256) k_rb_colour(RbGeom r, float *__restrict__ u, const float *__restrict__ f,
-                                                   Coef<CT> cf, float h2, int colour)
+                                                   Coef<CT> cf, float h2, int colour, float om, int zown)
 {
     const int m = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + 2;
@@ -95,7 +95,113 @@ __global__ void __launch_bounds__(256) k_rb_colour(RbGeom r, float *__restrict__
     if (hs) d += bs;
     if (ht) d += ct;
     if (hb) d += cb;
-    u[own] = (s - h2 * f[own]) / d;
+    const float gsv = (s - h2 * f[own]) / d;
+    const float uold = (ZERO || zown || om == 1.f) ? 0.f : u[own];  // Gauss-Seidel needs no old value
+    u[own] = om * gsv + (1.f - om) * uold;
+}
+
+// four consecutive entries (16-byte aligned for FP32, 8-byte for FP16)
+__device__ inline void ld4(const float *p, float v[4])
+{
+    const float4 q = *reinterpret_cast<const float4 *>(p);
+    v[0] = q.x;
+    v[1] = q.y;
+    v[2] = q.z;
+    v[3] = q.w;
+}
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+__device__ inline void ld4(const _Float16 *p, float v[4])
+{
+    const h4 q = *reinterpret_cast<const h4 *>(p);
+    v[0] = (float)q[0];
+    v[1] = (float)q[1];
+    v[2] = (float)q[2];
+    v[3] = (float)q[3];
+}
+
+// The colour pass with four consecutive cells of a row per thread (rows of
+// a multiple of 4 cells): one vector load per neighbour row instead of
+// four scalar loads, same arithmetic in the same order per cell.
+template <typename CT, bool ZERO, bool TOP>
+__global__ void __launch_bounds__(256) k_rb_colour4(RbGeom r, float *__restrict__ u, const float *__restrict__ f,
+                                                    Coef<CT> cf, float h2, int colour, float om, int zown)
+{
+    const int m0 = 4 * (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + 2;
+    const int k = (int)blockIdx.z + 2;
+    if (m0 >= r.mh || j > r.ny - 1) return;
+    const int p = (colour + j + k) & 1;
+    const size_t row = ((size_t)(k - 2) * r.my + (j - 2)) * r.mh;
+    const size_t own = (size_t)colour * r.half + row + m0;
+    const size_t oth = (size_t)(1 - colour) * r.half + row + m0;  // other colour, same row, m0
+    const size_t rs = (size_t)r.mh, ps = (size_t)r.mh * r.my;
+    const bool hn = j < r.ny - 1, hs = j > 2, ht = k < r.nz - 1, hb = k > 2;
+    float ae[4], bn[4], ct[4], aw[4] = {0.f, 0.f, 0.f, 0.f}, bs[4] = {0.f, 0.f, 0.f, 0.f},
+                                 cb[4] = {0.f, 0.f, 0.f, 0.f};
+    ld4(cf.p + own, ae);
+    ld4(cf.p + cf.n2 + own, bn);
+    ld4(cf.p + 2 * cf.n2 + own, ct);
+    // west faces: the other colour's a at m+p-1
+    {
+        float t[4];
+        ld4(cf.p + oth, t);
+        if (p) {
+            for (int q = 0; q < 4; q++) aw[q] = t[q];
+        } else {
+            aw[0] = m0 > 0 ? cf.a(oth - 1) : 0.f;
+            aw[1] = t[0];
+            aw[2] = t[1];
+            aw[3] = t[2];
+        }
+    }
+    if (hs) ld4(cf.p + cf.n2 + oth - rs, bs);
+    if (hb) ld4(cf.p + 2 * cf.n2 + oth - ps, cb);
+    float ue[4] = {0.f, 0.f, 0.f, 0.f}, uw[4] = {0.f, 0.f, 0.f, 0.f}, un[4], us[4], ut[4], ub[4];
+    if (!ZERO) {
+        float t[4];
+        ld4(u + oth, t);
+        if (p) {  // west m .. , east m+1 ..
+            for (int q = 0; q < 4; q++) uw[q] = t[q];
+            ue[0] = t[1];
+            ue[1] = t[2];
+            ue[2] = t[3];
+            ue[3] = m0 + 4 < r.mh ? u[oth + 4] : 0.f;
+        } else {  // west m-1 .., east m ..
+            for (int q = 0; q < 4; q++) ue[q] = t[q];
+            uw[0] = m0 > 0 ? u[oth - 1] : 0.f;
+            uw[1] = t[0];
+            uw[2] = t[1];
+            uw[3] = t[2];
+        }
+        if (hn) ld4(u + oth + rs, un);
+        if (hs) ld4(u + oth - rs, us);
+        if (ht) ld4(u + oth + ps, ut);
+        if (hb) ld4(u + oth - ps, ub);
+    }
+    float fv[4], out[4], uo[4] = {0.f, 0.f, 0.f, 0.f};
+    ld4(f + own, fv);
+    if (!ZERO && !zown && om != 1.f) ld4(u + own, uo);  // Gauss-Seidel needs no old value
+    for (int q = 0; q < 4; q++) {
+        const int i = 2 + p + 2 * (m0 + q);
+        const bool hw = i > 2, he = i < r.nx - 1;
+        float s = 0.f, d = ae[q];
+        if (!ZERO) {
+            if (he) s += ae[q] * ue[q];
+            if (hw) s += aw[q] * uw[q];
+            if (hn) s += bn[q] * un[q];
+            if (hs) s += bs[q] * us[q];
+            if (ht) s += ct[q] * ut[q];
+            if (hb) s += cb[q] * ub[q];
+        }
+        if (hw) d += aw[q];
+        if (hn) d += bn[q];
+        if (hs) d += bs[q];
+        if (ht) d += ct[q];
+        if (hb) d += cb[q];
+        const float gsv = (s - h2 * fv[q]) / d;
+        out[q] = om * gsv + (1.f - om) * uo[q];
+    }
+    *reinterpret_cast<float4 *>(u + own) = make_float4(out[0], out[1], out[2], out[3]);
 }
 
 // residual of one fine cell (f - A u, boundary folding of the natural resid)
@@ -172,9 +278,12 @@ double lev_h2i(const Ctx &c, const Grid &g)
     return 1.0 / (h * h);
 }
 
+// zero: the iterate is zero on entry; zown: this pass's own cells are zero
+// (the second pass of the first sweep)
 template <typename CT>
-int colour_pass(Ctx &c, int l, int colour, bool zero)
+int colour_pass(Ctx &c, int l, int colour, bool zero, int zown)
 {
+    const float om = (float)c.mg_omega;
     Level &L = c.lev[l];
     const RbGeom r = rb_geom(L.g);
     const Coef<CT> cf{(const CT *)L.rb_coef, 2 * r.half};
@@ -184,16 +293,26 @@ int colour_pass(Ctx &c, int l, int colour, bool zero)
     // algorithmic bytes: own f read and u write (one colour, FP32), the other
     // colour's u (not read from a zero iterate) and all coefficients
     const double cells = (double)r.half;
-    const double bytes = cells * (zero ? 8.0 : 12.0) + 6.0 * cells * (double)sizeof(CT);
+    const double bytes = cells * (zero ? 8.0 : (om == 1.f || zown ? 12.0 : 16.0)) + 6.0 * cells * (double)sizeof(CT);
     if (top) prof_begin(PK_RB_COLOUR, c.stream);
-    if (top && zero)
-        hipLaunchKernelGGL((k_rb_colour<CT, true, true>), grid, interior_block(), 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, colour);
+    if (r.mh % 4 == 0) {
+        const dim3 g4((unsigned)((r.mh / 4 + BX - 1) / BX), (unsigned)((r.my + BY - 1) / BY), (unsigned)r.mz);
+        if (top && zero)
+            hipLaunchKernelGGL((k_rb_colour4<CT, true, true>), g4, interior_block(), 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, colour, om, zown);
+        else if (top)
+            hipLaunchKernelGGL((k_rb_colour4<CT, false, true>), g4, interior_block(), 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, colour, om, zown);
+        else if (zero)
+            hipLaunchKernelGGL((k_rb_colour4<CT, true, false>), g4, interior_block(), 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, colour, om, zown);
+        else
+            hipLaunchKernelGGL((k_rb_colour4<CT, false, false>), g4, interior_block(), 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, colour, om, zown);
+    } else if (top && zero)
+        hipLaunchKernelGGL((k_rb_colour<CT, true, true>), grid, interior_block(), 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, colour, om, zown);
     else if (top)
-        hipLaunchKernelGGL((k_rb_colour<CT, false, true>), grid, interior_block(), 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, colour);
+        hipLaunchKernelGGL((k_rb_colour<CT, false, true>), grid, interior_block(), 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, colour, om, zown);
     else if (zero)
-        hipLaunchKernelGGL((k_rb_colour<CT, true, false>), grid, interior_block(), 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, colour);
+        hipLaunchKernelGGL((k_rb_colour<CT, true, false>), grid, interior_block(), 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, colour, om, zown);
     else
-        hipLaunchKernelGGL((k_rb_colour<CT, false, false>), grid, interior_block(), 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, colour);
+        hipLaunchKernelGGL((k_rb_colour<CT, false, false>), grid, interior_block(), 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, colour, om, zown);
     if (top) prof_end(PK_RB_COLOUR, c.stream, bytes);
     DG_CHECK_LAUNCH();
     return 0;
@@ -203,9 +322,12 @@ template <typename CT>
 int vcycle_rb(Ctx &c, int l)
 {
     Level &L = c.lev[l];
-    for (int q = 0; q < c.mg_nu1; q++) {
-        if (colour_pass<CT>(c, l, 0, q == 0)) return -1;
-        if (colour_pass<CT>(c, l, 1, false)) return -1;
+    const bool top = l == c.ng - 1;
+    const int nu1 = (top || c.mg_nuc <= 0) ? c.mg_nu1 : c.mg_nuc;
+    const int nu2 = (top || c.mg_nuc <= 0) ? c.mg_nu2 : c.mg_nuc;
+    for (int q = 0; q < nu1; q++) {
+        if (colour_pass<CT>(c, l, 0, q == 0, q == 0)) return -1;
+        if (colour_pass<CT>(c, l, 1, false, q == 0)) return -1;
     }
     const RbGeom rf = rb_geom(L.g);
     const Coef<CT> cf{(const CT *)L.rb_coef, 2 * rf.half};
@@ -224,9 +346,9 @@ int vcycle_rb(Ctx &c, int l)
     hipLaunchKernelGGL(k_rb_prolong, dim3(gridn(2 * rf.half)), dim3(256), 0, c.stream, rf, L.rb_u, rc,
                        csplit ? C.rb_u : nullptr, C.g, csplit ? nullptr : C.u32);
     DG_CHECK_LAUNCH();
-    for (int q = 0; q < c.mg_nu2; q++) {
-        if (colour_pass<CT>(c, l, 1, false)) return -1;
-        if (colour_pass<CT>(c, l, 0, false)) return -1;
+    for (int q = 0; q < nu2; q++) {
+        if (colour_pass<CT>(c, l, 1, false, 0)) return -1;
+        if (colour_pass<CT>(c, l, 0, false, 0)) return -1;
     }
     return 0;
 }
@@ -235,7 +357,7 @@ int vcycle_rb(Ctx &c, int l)
 
 bool rb_enabled(const Ctx &c)
 {
-    if (!c.rb_split || c.mg_mode != 2 || c.ng < 2 || c.mg_nu1 < 1) return false;
+    if (!c.rb_split || c.mg_mode != 2 || c.ng < 2 || c.mg_nu1 < 1 || c.mg_nuc < 0) return false;
     for (int l = 1; l < c.ng; l++) {
         const Grid &g = c.lev[l].g;
         if ((g.nx - 2) & 1 || (g.ny - 2) & 1 || (g.nz - 2) & 1) return false;
@@ -275,13 +397,20 @@ int rb_prepare(Ctx &c)
     return 0;
 }
 
-int rb_vcycle(Ctx &c, const double *r)
+int rb_load_rhs(Ctx &c, const double *r)
 {
     Level &F = c.lev[c.ng - 1];
     const RbGeom rg = rb_geom(F.g);
     const size_t N = (size_t)(F.g.nx - 2) * (F.g.ny - 2) * (F.g.nz - 2);
     hipLaunchKernelGGL(k_rb_in, dim3(gridn(N)), dim3(256), 0, c.stream, F.g, rg, r, F.rb_f);
     DG_CHECK_LAUNCH();
+    return 0;
+}
+
+int rb_vcycle(Ctx &c, const double *r)
+{
+    Level &F = c.lev[c.ng - 1];
+    if (r && rb_load_rhs(c, r)) return -1;
     return F.rb_coef16 == 1 ? vcycle_rb<_Float16>(c, c.ng - 1) : vcycle_rb<float>(c, c.ng - 1);
 }
 

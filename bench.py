@@ -42,7 +42,8 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE and WRITE_SIZE runs of this script scaled by the same counters
 # on a plain 256 MiB copy (fetch x2.0, write x1.0); keyed by kernel and grid
 # extent, re-measured whenever the kernel changes
-PMC_TRAFFIC = {("k_gs_colour<float,true>", 258): 533.9e6}
+PMC_TRAFFIC = {}
+ROOF_KERNEL = "k_rb_colour<_Float16,false,true>"
 
 
 def splash_inputdata(dg, n):
@@ -108,7 +109,7 @@ def run_path(dg, txt, mode, steps, warmup, tol, guess=1, profile_kernel=None, pr
                 vol0=vol0, vol1=vol1, prof=prof, p=p, guess_stats=guess_stats)
 
 
-def cpu_baseline(n, steps, tol):
+def cpu_baseline(n, steps, tol, omega=1.0, nu_coarse=0):
     """The CPU restatement (oracle), one thread, FP64 PCG with an FP64 V-cycle
     (the same algorithm as the denominator), on `steps` steps of the same
     workload; initialisation excluded."""
@@ -121,6 +122,7 @@ def cpu_baseline(n, steps, tol):
     cc[...] = cc.clip(0.0, 1.0)
     o.st.mg_mode, o.st.mg_outer, o.st.mg_prolong, o.st.vof_guard = 1, 1, 0, 1
     o.st.mg_tol, o.st.mg_nu1, o.st.mg_nu2, o.st.mg_maxcycles, o.st.mg_guess = tol, 2, 2, 200, 1
+    o.st.mg_omega, o.st.mg_nuc = omega, nu_coarse
     t0 = time.perf_counter()
     for t in range(steps):
         o.step(t)
@@ -153,6 +155,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-denominator", action="store_true")
+    ap.add_argument("--omega", type=float, default=1.15, help="SOR weight of the V-cycle smoother (all paths)")
+    ap.add_argument("--nu-coarse", type=int, default=3, help="smoothing sweeps below the finest level (all paths)")
     ap.add_argument("--weights", default=os.path.join(HERE, "3d_droplet_simulation_2007_amd", "weights",
                                                        "guess_unet.npz"), help="learned-guess network weights")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -167,14 +171,16 @@ def main():
         sys.exit(2)
     dg = importlib.import_module("3d_droplet_simulation_2007_amd")
     dg.lib()
+    # one smoother for every path (target, mixed and the FP64 denominator)
+    dg.set_smoother(args.omega, args.nu_coarse)
     n = args.grid
     txt = splash_inputdata(dg, n)
     gs = importlib.import_module("3d_droplet_simulation_2007_amd.guess")
     weights_meta = gs.load_weights(args.weights)
     import numpy as np
     # target: learned guess + FP32-preconditioned FP64 PCG; dominant kernel:
-    # the finest-level FP32 red-black Gauss-Seidel colour pass
-    tgt = run_path(dg, txt, 2, args.steps, args.warmup, args.tol, guess=2, profile_kernel="gs_colour")
+    # the finest-level colour pass of the red-black split FP32 V-cycle
+    tgt = run_path(dg, txt, 2, args.steps, args.warmup, args.tol, guess=2, profile_kernel="rb_colour")
     out = {}
     den = mixed = None
     if not args.no_denominator:
@@ -205,7 +211,8 @@ def main():
         "data": "synthetic (reference smooth2 droplet IC)",
         "config": {"workload": f"{n - 2}^3 droplet-on-wall splash, 6 mm cube, r=1 mm, v=-3 m/s, CSF, "
                                f"each step's Poisson solve to {args.tol:g} relative residual",
-                   "grid_with_ghosts": [n, n, n], "solver": "FP64 PCG + FP32 V(2,2) preconditioner",
+                   "grid_with_ghosts": [n, n, n], "solver": f"FP64 PCG + FP32 V(2,2) preconditioner, red-black SOR omega {args.omega:g}, "
+                                                         f"{args.nu_coarse} sweeps below the finest level",
                    "initial_guess": "learned (encoder-decoder on the next-coarser level) over the previous "
                                     "step's pressure, residual-checked, with the fallback controller",
                    "parallelism": "single GPU"},
@@ -214,10 +221,11 @@ def main():
         "mg_iters_per_step": float(np.mean(tgt["iters"])),
         "final_relres": tgt["relres"],
         "volume_drift_rel": (tgt["vol1"] - tgt["vol0"]) / tgt["vol0"],
-        "roofline": {"kernel": "k_gs_colour<float,true> (finest-level red-black Gauss-Seidel colour pass, FP32)",
+        "roofline": {"kernel": ROOF_KERNEL + " (finest-level Gauss-Seidel colour pass of the red-black split "
+                                           "FP32 V-cycle, FP16 face coefficients)",
                      "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": PMC_TRAFFIC.get(("k_gs_colour<float,true>", n)),
+                     "traffic": PMC_TRAFFIC.get((ROOF_KERNEL, n)),
                      "avg_launch_us": 1e3 * avg_ms, "launches": pr["launches"],
                      "algorithmic_bytes_per_launch": pr["bytes"] / max(pr["launches"], 1)},
     }
@@ -234,7 +242,7 @@ def main():
                                                "speedup_vs_fp64": mv / dv}
     line.update(out)
     if not args.no_cpu_baseline:
-        v, dt = cpu_baseline(n, args.cpu_steps, args.tol)
+        v, dt = cpu_baseline(n, args.cpu_steps, args.tol, args.omega, args.nu_coarse)
         line["cpu_baseline"] = {"value": v, "unit": "time steps/s", "cores": 1, "kind": "port",
                                 "sample": f"{args.cpu_steps} step(s) of the same {n - 2}^3 workload, FP64 PCG + "
                                           f"FP64 V-cycle (the denominator's algorithm), {dt:.1f} s"}

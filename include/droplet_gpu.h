@@ -85,6 +85,12 @@ int dg_clip_cc(void);
  * guess: 0 zero, 1 previous step's pressure, 2 learned (dg_guess_*). */
 int dg_set_solver(const int *mode, const double *tol, const int *nu1, const int *nu2, const int *maxit,
                   const int *guess);
+/* Tolerance-mode smoother (no reference counterpart): SOR weight omega of
+ * the symmetric red-black passes (1, the default: Gauss-Seidel) and the
+ * sweep count nu_coarse before and after the coarse correction on every
+ * level below the finest (0, the default: nu1 / nu2).  Applies to both
+ * precisions, the resident state and later dg_init / dg_poisson_solve. */
+int dg_set_smoother(const double *omega, const int *nu_coarse);
 /* 1 (default): outflow sums in the reference's sequential order
  * (main.f:4123-4131); 0: parallel tree sums (different rounding). */
 int dg_set_ordered_sums(const int *on);

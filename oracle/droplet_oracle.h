@@ -79,6 +79,11 @@ typedef struct or_state {
      * normal to the sweep direction instead of AL3D/FL3D's 0/0 (NaN) of
      * swp main.f:848-1000.  0 (default): the reference's arithmetic. */
     int vof_guard;
+    /* tolerance-mode smoother: SOR weight of the red-black passes (1 =
+     * Gauss-Seidel) and the sweep count (pre and post) on the levels below
+     * the finest (0: mg_nu1 / mg_nu2 there too) */
+    double mg_omega;
+    int mg_nuc;
 } or_state;
 
 /* field ids shared with the GPU library's dg_field ids */

@@ -227,18 +227,22 @@ static void vcycle_d(or_state *st, tol_ws *W, int l)
         coarse_direct(W->u[0], W->f[0], L->a, L->b, L->c, nx, ny, nz, h2i);
         return;
     }
-    for (int s = 0; s < st->mg_nu1; s++) {
-        gs_colour_d(W->u[l], W->f[l], L->a, L->b, L->c, nx, ny, nz, h2, 1.0, 0);
-        gs_colour_d(W->u[l], W->f[l], L->a, L->b, L->c, nx, ny, nz, h2, 1.0, 1);
+    const int top = l == st->ng - 1;
+    const int nu1 = (top || st->mg_nuc <= 0) ? st->mg_nu1 : st->mg_nuc;
+    const int nu2 = (top || st->mg_nuc <= 0) ? st->mg_nu2 : st->mg_nuc;
+    const double om = st->mg_omega > 0.0 ? st->mg_omega : 1.0;
+    for (int s = 0; s < nu1; s++) {
+        gs_colour_d(W->u[l], W->f[l], L->a, L->b, L->c, nx, ny, nz, h2, om, 0);
+        gs_colour_d(W->u[l], W->f[l], L->a, L->b, L->c, nx, ny, nz, h2, om, 1);
     }
     resid_d(W->r[l], W->u[l], W->f[l], L->a, L->b, L->c, nx, ny, nz, h2i);
     restrict_d(W->f[l - 1], W->r[l], W->nx[l - 1], W->ny[l - 1], W->nz[l - 1], st->mg_prolong);
     memset(W->u[l - 1], 0, sizeof(double) * (size_t)W->nx[l - 1] * W->ny[l - 1] * W->nz[l - 1]);
     vcycle_d(st, W, l - 1);
     prolong_add_d(W->u[l], W->u[l - 1], nx, ny, nz, st->mg_prolong);
-    for (int s = 0; s < st->mg_nu2; s++) {
-        gs_colour_d(W->u[l], W->f[l], L->a, L->b, L->c, nx, ny, nz, h2, 1.0, 1);
-        gs_colour_d(W->u[l], W->f[l], L->a, L->b, L->c, nx, ny, nz, h2, 1.0, 0);
+    for (int s = 0; s < nu2; s++) {
+        gs_colour_d(W->u[l], W->f[l], L->a, L->b, L->c, nx, ny, nz, h2, om, 1);
+        gs_colour_d(W->u[l], W->f[l], L->a, L->b, L->c, nx, ny, nz, h2, om, 0);
     }
 }
 
@@ -255,18 +259,22 @@ static void vcycle_f(or_state *st, tol_ws *W, int l)
         return;
     }
     float h2if = (float)h2i, h2f = (float)(1.0 / h2i);
-    for (int s = 0; s < st->mg_nu1; s++) {
-        gs_colour_f(W->uf[l], W->ff[l], W->af[l], W->bf[l], W->cf[l], nx, ny, nz, h2f, 1.0f, 0);
-        gs_colour_f(W->uf[l], W->ff[l], W->af[l], W->bf[l], W->cf[l], nx, ny, nz, h2f, 1.0f, 1);
+    const int top = l == st->ng - 1;
+    const int nu1 = (top || st->mg_nuc <= 0) ? st->mg_nu1 : st->mg_nuc;
+    const int nu2 = (top || st->mg_nuc <= 0) ? st->mg_nu2 : st->mg_nuc;
+    const float om = st->mg_omega > 0.0 ? (float)st->mg_omega : 1.0f;
+    for (int s = 0; s < nu1; s++) {
+        gs_colour_f(W->uf[l], W->ff[l], W->af[l], W->bf[l], W->cf[l], nx, ny, nz, h2f, om, 0);
+        gs_colour_f(W->uf[l], W->ff[l], W->af[l], W->bf[l], W->cf[l], nx, ny, nz, h2f, om, 1);
     }
     resid_f(W->rf[l], W->uf[l], W->ff[l], W->af[l], W->bf[l], W->cf[l], nx, ny, nz, h2if);
     restrict_f(W->ff[l - 1], W->rf[l], W->nx[l - 1], W->ny[l - 1], W->nz[l - 1], st->mg_prolong);
     memset(W->uf[l - 1], 0, sizeof(float) * (size_t)W->nx[l - 1] * W->ny[l - 1] * W->nz[l - 1]);
     vcycle_f(st, W, l - 1);
     prolong_add_f(W->uf[l], W->uf[l - 1], nx, ny, nz, st->mg_prolong);
-    for (int s = 0; s < st->mg_nu2; s++) {
-        gs_colour_f(W->uf[l], W->ff[l], W->af[l], W->bf[l], W->cf[l], nx, ny, nz, h2f, 1.0f, 1);
-        gs_colour_f(W->uf[l], W->ff[l], W->af[l], W->bf[l], W->cf[l], nx, ny, nz, h2f, 1.0f, 0);
+    for (int s = 0; s < nu2; s++) {
+        gs_colour_f(W->uf[l], W->ff[l], W->af[l], W->bf[l], W->cf[l], nx, ny, nz, h2f, om, 1);
+        gs_colour_f(W->uf[l], W->ff[l], W->af[l], W->bf[l], W->cf[l], nx, ny, nz, h2f, om, 0);
     }
 }
 

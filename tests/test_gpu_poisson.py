@@ -99,3 +99,35 @@ def test_matches_oracle_solve(dg, oracle):
     err = np.linalg.norm(pg[1:-1, 1:-1, 1:-1] - po[1:-1, 1:-1, 1:-1]) / np.linalg.norm(po[1:-1, 1:-1, 1:-1])
     assert err < 1e-8, err
     assert abs(it - o.st.mg_cycles_last) <= 1
+
+
+@pytest.mark.parametrize("omega,nuc", [(1.15, 0), (1.15, 4), (1.0, 3)])
+def test_smoother_options_match_oracle(dg, oracle, omega, nuc):
+    """SOR weight and coarse-level sweep count: the FP64 solve follows the
+    CPU statement of the same V-cycle (iterations within one, solutions to
+    1e-8), and the split FP32 V-cycle stays bit-identical to the natural one."""
+    print("case smoother", omega, nuc, flush=True)
+    rhs, a, b, c = sphere_system(34, 5)
+    o = Oracle(inputdata_text(nx=34, ny=34, nz=34))
+    for name, arr in (("rhs", rhs), ("a", a), ("b", b), ("c", c)):
+        o.field(name)[...] = arr
+    o.st.mg_mode, o.st.mg_outer, o.st.mg_prolong = 1, 1, 0
+    o.st.mg_tol, o.st.mg_nu1, o.st.mg_nu2, o.st.mg_maxcycles, o.st.mg_guess = 1e-10, 2, 2, 100, 0
+    o.st.mg_omega, o.st.mg_nuc = omega, nuc
+    oracle.lib().or_mg_solve_tol(o.h)
+    try:
+        dg.set_smoother(omega, nuc)
+        pg, it, rel = dg.poisson_solve(rhs, a, b, c, mode=1, tol=1e-10)
+        outs = []
+        for split in (True, False):
+            dg.set_tuning(split=split)
+            outs.append(dg.poisson_solve(rhs, a, b, c, mode=2, tol=1e-9))
+    finally:
+        dg.set_smoother()
+        dg.set_tuning()
+    po = o.field("p")
+    err = np.linalg.norm(pg[1:-1, 1:-1, 1:-1] - po[1:-1, 1:-1, 1:-1]) / np.linalg.norm(po[1:-1, 1:-1, 1:-1])
+    print("iters gpu", it, "oracle", o.st.mg_cycles_last, "err", err, flush=True)
+    assert err < 1e-8, err
+    assert abs(it - o.st.mg_cycles_last) <= 1
+    assert outs[0][1] == outs[1][1] and np.array_equal(outs[0][0], outs[1][0])

@@ -15,6 +15,7 @@
 #include <cmath>
 #include <vector>
 
+#include "dg_coarse.h"
 #include "dg_ctx.h"
 #include "dg_nn.h"
 #include "dg_prof.h"
@@ -432,29 +433,7 @@ __global__ void __launch_bounds__(64) k_coarse_solve(Grid g, const double *M, co
 {
     // one wave: substitutions with the rhs staged in LDS
     __shared__ double y[512];
-    const int mx = g.nx - 2, my = g.ny - 2, mz = g.nz - 2, n = mx * my * mz;
-    for (int row = threadIdx.x; row < n; row += 64) {
-        const int i = row % mx + 2, j = (row / mx) % my + 2, k = row / (mx * my) + 2;
-        y[row] = -(double)f[g.at(i, j, k)];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int p = 0; p < n; p++) {
-            double t = y[p];
-            for (int q = 0; q < p; q++) t -= M[p * n + q] * y[q];
-            y[p] = t / M[p * n + p];
-        }
-        for (int p = n - 1; p >= 0; p--) {
-            double t = y[p];
-            for (int q = p + 1; q < n; q++) t -= M[q * n + p] * y[q];
-            y[p] = t / M[p * n + p];
-        }
-    }
-    __syncthreads();
-    for (int row = threadIdx.x; row < n; row += 64) {
-        const int i = row % mx + 2, j = (row / mx) % my + 2, k = row / (mx * my) + 2;
-        u[g.at(i, j, k)] = (T)y[row];
-    }
+    coarse_subst<T>(g, M, f, u, y, (int)threadIdx.x, 64);
 }
 
 struct TolWork {
@@ -465,6 +444,8 @@ struct TolWork {
 static TolWork g_tw;
 
 // coarsest-level FP32 solve (natural layout f32 -> u32) for the split V-cycle
+const double *coarse_factor() { return g_tw.M; }
+
 int coarse_solve_f32(Ctx &c)
 {
     Level &L = c.lev[0];
@@ -634,17 +615,18 @@ template <class Z>
 __global__ void __launch_bounds__(256) k_dot_rz_zq(Grid g, const double *r, Z z, const double *q, double *partial)
 {
     __shared__ double lds4[4];
-    const int mx = g.nx - 2, my = g.ny - 2, mz = g.nz - 2;
-    const long long tot = (long long)mx * my * mz;
+    // block b takes the interior rows b, b + gridDim.x, ...; its threads
+    // stride along each row (fixed assignment: deterministic sums)
+    const int my = g.ny - 2, rows = my * (g.nz - 2);
     double a1 = 0.0, a2 = 0.0;
-    for (long long p = (long long)blockIdx.x * 256 + threadIdx.x; p < tot; p += (long long)gridDim.x * 256) {
-        int i = (int)(p % mx) + 2;
-        long long t = p / mx;
-        int j = (int)(t % my) + 2, k = (int)(t / my) + 2;
-        const int o = g.at(i, j, k);
-        const double zz = z(i, j, k, o);
-        a1 += r[o] * zz;
-        a2 += zz * q[o];
+    for (int rr = (int)blockIdx.x; rr < rows; rr += (int)gridDim.x) {
+        const int j = rr % my + 2, k = rr / my + 2;
+        for (int i = 2 + (int)threadIdx.x; i <= g.nx - 1; i += 256) {
+            const int o = g.at(i, j, k);
+            const double zz = z(i, j, k, o);
+            a1 += r[o] * zz;
+            a2 += zz * q[o];
+        }
     }
     double s1 = block_reduce256<false>(a1, lds4);
     double s2 = block_reduce256<false>(a2, lds4);
@@ -675,6 +657,11 @@ enum { S_RZ = 16, S_ZQ, S_RZOLD, S_ALPHA, S_PQ, S_RR, S_BB, S_FIRST };
 
 // K2: p_new = z + beta p_old (beta from device scalars), q = A p_new,
 // partial sums of p_new . q
+// planes per thread of k_pcg_dir: each thread marches up a column of
+// PCG_KCH cells, carrying the plane below's direction value and face
+// coefficient in registers
+constexpr int PCG_KCH = 8;
+
 template <class Z>
 __global__ void __launch_bounds__(256) k_pcg_dir(Grid g, Z z, const double *pold, double *pnew, double *q,
                                                  const double *a, const double *b, const double *c, double h2i,
@@ -684,27 +671,36 @@ __global__ void __launch_bounds__(256) k_pcg_dir(Grid g, Z z, const double *pold
     const int bidx = (int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 2;
     const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + 2;
-    const int k = (int)blockIdx.z + 2;
+    const int k0 = (int)blockIdx.z * PCG_KCH + 2;
+    const int k1 = min(k0 + PCG_KCH, g.nz);  // interior planes k0 .. k1 - 1 (the last is nz - 1)
     const int tid = (int)(threadIdx.x + blockDim.x * threadIdx.y);
     double contrib = 0.0;
     const bool first = ds[S_FIRST] != 0.0;
     const double beta = first ? 0.0 : -ds[S_ALPHA] * ds[S_ZQ] / ds[S_RZOLD];
     if (i <= g.nx - 1 && j <= g.ny - 1) {
-        const int o = g.at(i, j, k), sy = g.sy, sz = g.sz;
+        const int sy = g.sy, sz = g.sz;
         auto P = [&](int ii, int jj, int kk, int oo) { return z(ii, jj, kk, oo) + beta * pold[oo]; };
-        const double p0 = P(i, j, k, o);
-        const double pe = i < g.nx - 1 ? P(i + 1, j, k, o + 1) : 0.0;
-        const double pw = i > 2 ? P(i - 1, j, k, o - 1) : p0;
-        const double pn = j < g.ny - 1 ? P(i, j + 1, k, o + sy) : p0;
-        const double ps = j > 2 ? P(i, j - 1, k, o - sy) : p0;
-        const double pt = k < g.nz - 1 ? P(i, j, k + 1, o + sz) : p0;
-        const double pb = k > 2 ? P(i, j, k - 1, o - sz) : p0;
-        const double s = a[o] * (pe - p0) + a[o - 1] * (pw - p0) + b[o] * (pn - p0) + b[o - sy] * (ps - p0) +
-                         c[o] * (pt - p0) + c[o - sz] * (pb - p0);
-        const double qq = h2i * s;
-        pnew[o] = p0;
-        q[o] = qq;
-        contrib = p0 * qq;
+        int o = g.at(i, j, k0);
+        double p0 = P(i, j, k0, o);
+        double pb = k0 > 2 ? P(i, j, k0 - 1, o - sz) : p0;
+        double cb = c[o - sz];
+        for (int k = k0; k < k1; k++, o += sz) {
+            const double pt = k < g.nz - 1 ? P(i, j, k + 1, o + sz) : p0;
+            const double pe = i < g.nx - 1 ? P(i + 1, j, k, o + 1) : 0.0;
+            const double pw = i > 2 ? P(i - 1, j, k, o - 1) : p0;
+            const double pn = j < g.ny - 1 ? P(i, j + 1, k, o + sy) : p0;
+            const double ps = j > 2 ? P(i, j - 1, k, o - sy) : p0;
+            const double ct = c[o];
+            const double s = a[o] * (pe - p0) + a[o - 1] * (pw - p0) + b[o] * (pn - p0) + b[o - sy] * (ps - p0) +
+                             ct * (pt - p0) + cb * (pb - p0);
+            const double qq = h2i * s;
+            pnew[o] = p0;
+            q[o] = qq;
+            contrib += p0 * qq;
+            pb = p0;
+            p0 = pt;
+            cb = ct;
+        }
     }
     // block of 64x4: reduce
     double v = wave_sum(contrib);
@@ -732,7 +728,7 @@ __global__ void __launch_bounds__(256) k_pcg_update(Grid g, double *x, double *r
         x[o] = x[o] + alpha * p[o];
         const double rn = r[o] - alpha * q[o];
         r[o] = rn;
-        if (fs) fs[rg.at(i, j, k)] = (float)rn;
+        if (fs) fs[rg.at32(i, j, k)] = (float)rn;
         contrib = rn * rn;
     }
     double v = wave_sum(contrib);
@@ -901,6 +897,8 @@ static int pcg_iteration(Ctx &c, double *pa, double *pb)
     double *x = F.u, *r = c.pcg_r, *q = c.pcg_q, *ds = c.dscal, *partial = c.red_partial;
     const dim3 ig = interior_grid(g), ib = interior_block();
     const int nblk = (int)(ig.x * ig.y * ig.z);
+    const dim3 dgr(ig.x, ig.y, (unsigned)((g.nz - 2 + PCG_KCH - 1) / PCG_KCH));  // k_pcg_dir: columns of planes
+    const int nblk_dir = (int)(dgr.x * dgr.y * dgr.z);
     T *u, *f, *rr_, *w_;
     const T *a_, *b_, *c_;
     level_ptrs<T>(F, u, f, rr_, a_, b_, c_, w_);
@@ -917,7 +915,7 @@ static int pcg_iteration(Ctx &c, double *pa, double *pb)
         hipLaunchKernelGGL(k_final2, dim3(1), dim3(256), 0, s, partial, RED_BLOCKS, ds + S_RZ);
         DG_CHECK_LAUNCH();
         prof_begin(PK_PCG_DIR, s);
-        hipLaunchKernelGGL(k_pcg_dir<ZSplit>, ig, ib, 0, s, g, z, pa, pb, q, F.a, F.b, F.c, h2i, ds, partial);
+        hipLaunchKernelGGL(k_pcg_dir<ZSplit>, dgr, ib, 0, s, g, z, pa, pb, q, F.a, F.b, F.c, h2i, ds, partial);
         prof_end(PK_PCG_DIR, s, (4.0 + 6.0 * 8.0) * ncell);
         DG_CHECK_LAUNCH();
     } else {
@@ -934,11 +932,11 @@ static int pcg_iteration(Ctx &c, double *pa, double *pb)
         hipLaunchKernelGGL(k_final2, dim3(1), dim3(256), 0, s, partial, RED_BLOCKS, ds + S_RZ);
         DG_CHECK_LAUNCH();
         prof_begin(PK_PCG_DIR, s);
-        hipLaunchKernelGGL(k_pcg_dir<ZNatural<T>>, ig, ib, 0, s, g, z, pa, pb, q, F.a, F.b, F.c, h2i, ds, partial);
+        hipLaunchKernelGGL(k_pcg_dir<ZNatural<T>>, dgr, ib, 0, s, g, z, pa, pb, q, F.a, F.b, F.c, h2i, ds, partial);
         prof_end(PK_PCG_DIR, s, ((double)sizeof(T) + 6.0 * 8.0) * ncell);
         DG_CHECK_LAUNCH();
     }
-    if (sum_partials(s, partial, nblk, 1, partial + 2 * (size_t)nblk, ds + S_PQ)) return -1;
+    if (sum_partials(s, partial, nblk_dir, 1, partial + 2 * (size_t)nblk, ds + S_PQ)) return -1;
     prof_begin(PK_PCG_UPDATE, s);
     hipLaunchKernelGGL(k_pcg_update, ig, ib, 0, s, g, x, r, pb, q, ds, partial, split ? F.rb_f : nullptr,
                        rb_geom(g));

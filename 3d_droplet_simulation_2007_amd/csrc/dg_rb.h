@@ -32,6 +32,12 @@ struct RbGeom {
     {
         return (size_t)((i + j + k) & 1) * half + idx(i, j, k);
     }
+    // the same offset in 32-bit arithmetic (levels with 2 * half < 2^31,
+    // which rb_enabled requires of every split level)
+    __device__ inline unsigned at32(int i, int j, int k) const
+    {
+        return (unsigned)((i + j + k) & 1) * (unsigned)half + (unsigned)(((k - 2) * my + (j - 2)) * mh + ((i - 2) >> 1));
+    }
 };
 
 inline RbGeom rb_geom(const Grid &g)
@@ -57,7 +63,7 @@ struct ZNatural {
 struct ZSplit {
     const float *z;
     RbGeom r;
-    __device__ inline double operator()(int i, int j, int k, int o) const { return (double)z[r.at(i, j, k)]; }
+    __device__ inline double operator()(int i, int j, int k, int o) const { return (double)z[r.at32(i, j, k)]; }
 };
 
 // whether the split path serves this context's mode-2 solves

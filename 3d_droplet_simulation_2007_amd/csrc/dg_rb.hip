@@ -7,13 +7,15 @@
 
 #include <hip/hip_runtime.h>
 
+#include "dg_coarse.h"
 #include "dg_prof.h"
 #include "dg_rb.h"
 #include "dg_util.h"
 
 namespace dg {
 
-int coarse_solve_f32(Ctx &c);  // dg_mg.hip
+int coarse_solve_f32(Ctx &c);     // dg_mg.hip
+const double *coarse_factor();  // dg_mg.hip: Cholesky factor of the coarsest level
 
 namespace {
 
@@ -27,31 +29,31 @@ struct Coef {
     __device__ inline float c(size_t e) const { return (float)p[2 * n2 + e]; }
 };
 
+// thread (i, j, k) of the interior, k = blockIdx.z
 template <typename CT>
-__global__ void k_rb_coef(Grid g, RbGeom r, const double *a, const double *b, const double *c, CT *out)
+__global__ void __launch_bounds__(256) k_rb_coef(Grid g, RbGeom r, const double *a, const double *b, const double *c,
+                                                 CT *out)
 {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 2;
+    const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + 2;
+    const int k = (int)blockIdx.z + 2;
+    if (i > g.nx - 1 || j > g.ny - 1) return;
     const size_t n2 = 2 * r.half;
-    const int mx = g.nx - 2, my = g.ny - 2, mz = g.nz - 2;
-    const size_t N = (size_t)mx * my * mz;
-    for (size_t p = (size_t)blockIdx.x * 256 + threadIdx.x; p < N; p += (size_t)gridDim.x * 256) {
-        const int i = (int)(p % mx) + 2, j = (int)((p / mx) % my) + 2, k = (int)(p / ((size_t)mx * my)) + 2;
-        const int o = g.at(i, j, k);
-        const size_t e = r.at(i, j, k);
-        out[e] = (CT)a[o];
-        out[n2 + e] = (CT)b[o];
-        out[2 * n2 + e] = (CT)c[o];
-    }
+    const int o = g.at(i, j, k);
+    const size_t e = r.at(i, j, k);
+    out[e] = (CT)a[o];
+    out[n2 + e] = (CT)b[o];
+    out[2 * n2 + e] = (CT)c[o];
 }
 
 // finest-level right-hand side: f (split FP32) = r (natural FP64)
-__global__ void k_rb_in(Grid g, RbGeom r, const double *src, float *dst)
+__global__ void __launch_bounds__(256) k_rb_in(Grid g, RbGeom r, const double *src, float *dst)
 {
-    const int mx = g.nx - 2, my = g.ny - 2, mz = g.nz - 2;
-    const size_t N = (size_t)mx * my * mz;
-    for (size_t p = (size_t)blockIdx.x * 256 + threadIdx.x; p < N; p += (size_t)gridDim.x * 256) {
-        const int i = (int)(p % mx) + 2, j = (int)((p / mx) % my) + 2, k = (int)(p / ((size_t)mx * my)) + 2;
-        dst[r.at(i, j, k)] = (float)src[g.at(i, j, k)];
-    }
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 2;
+    const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + 2;
+    const int k = (int)blockIdx.z + 2;
+    if (i > g.nx - 1 || j > g.ny - 1) return;
+    dst[r.at(i, j, k)] = (float)src[g.at(i, j, k)];
 }
 
 // One Gauss-Seidel colour pass: thread (m, j, k) updates the colour's cell
@@ -62,14 +64,10 @@ __global__ void k_rb_in(Grid g, RbGeom r, const double *src, float *dst)
 // x-max neighbour is 0 with its face kept in the diagonal.  ZERO: the
 // iterate is zero on entry (first pass of a V-cycle level), u is not read.
 // TOP only separates the finest level's launches into their own symbol.
-template <typename CT, bool ZERO, bool TOP>
-__global__ void __launch_bounds__(256) k_rb_colour(RbGeom r, float *__restrict__ u, const float *__restrict__ f,
-                                                   Coef<CT> cf, float h2, int colour, float om, int zown)
+template <typename CT, bool ZERO>
+__device__ inline void rb_cell(const RbGeom &r, float *u, const float *f, const Coef<CT> &cf, float h2, int colour,
+                               float om, int zown, int m, int j, int k)
 {
-    const int m = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + 2;
-    const int k = (int)blockIdx.z + 2;
-    if (m >= r.mh || j > r.ny - 1) return;
     const int p = (colour + j + k) & 1;
     const int i = 2 + p + 2 * m;
     const size_t row = ((size_t)(k - 2) * r.my + (j - 2)) * r.mh;
@@ -98,6 +96,17 @@ __global__ void __launch_bounds__(256) k_rb_colour(RbGeom r, float *__restrict__
     const float gsv = (s - h2 * f[own]) / d;
     const float uold = (ZERO || zown || om == 1.f) ? 0.f : u[own];  // Gauss-Seidel needs no old value
     u[own] = om * gsv + (1.f - om) * uold;
+}
+
+template <typename CT, bool ZERO, bool TOP>
+__global__ void __launch_bounds__(256) k_rb_colour(RbGeom r, float *__restrict__ u, const float *__restrict__ f,
+                                                   Coef<CT> cf, float h2, int colour, float om, int zown)
+{
+    const int m = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + 2;
+    const int k = (int)blockIdx.z + 2;
+    if (m >= r.mh || j > r.ny - 1) return;
+    rb_cell<CT, ZERO>(r, u, f, cf, h2, colour, om, zown, m, j, k);
 }
 
 // four consecutive entries (16-byte aligned for FP32, 8-byte for FP16)
@@ -209,17 +218,18 @@ template <typename CT>
 __device__ inline float rb_resid_cell(const RbGeom &r, const float *u, const float *f, const Coef<CT> &cf, float h2i,
                                       int i, int j, int k)
 {
-    const size_t e = r.at(i, j, k);
+    auto at = [&](int ii, int jj, int kk) { return r.at32(ii, jj, kk); };
+    const unsigned e = at(i, j, k);
     const float u0 = u[e];
-    const float ue = i < r.nx - 1 ? u[r.at(i + 1, j, k)] : 0.f;
-    const float uw = i > 2 ? u[r.at(i - 1, j, k)] : u0;
-    const float un = j < r.ny - 1 ? u[r.at(i, j + 1, k)] : u0;
-    const float us = j > 2 ? u[r.at(i, j - 1, k)] : u0;
-    const float ut = k < r.nz - 1 ? u[r.at(i, j, k + 1)] : u0;
-    const float ub = k > 2 ? u[r.at(i, j, k - 1)] : u0;
-    const float aw = i > 2 ? cf.a(r.at(i - 1, j, k)) : 0.f;
-    const float bs = j > 2 ? cf.b(r.at(i, j - 1, k)) : 0.f;
-    const float cb = k > 2 ? cf.c(r.at(i, j, k - 1)) : 0.f;
+    const float ue = i < r.nx - 1 ? u[at(i + 1, j, k)] : 0.f;
+    const float uw = i > 2 ? u[at(i - 1, j, k)] : u0;
+    const float un = j < r.ny - 1 ? u[at(i, j + 1, k)] : u0;
+    const float us = j > 2 ? u[at(i, j - 1, k)] : u0;
+    const float ut = k < r.nz - 1 ? u[at(i, j, k + 1)] : u0;
+    const float ub = k > 2 ? u[at(i, j, k - 1)] : u0;
+    const float aw = i > 2 ? cf.a(at(i - 1, j, k)) : 0.f;
+    const float bs = j > 2 ? cf.b(at(i, j - 1, k)) : 0.f;
+    const float cb = k > 2 ? cf.c(at(i, j, k - 1)) : 0.f;
     const float s = cf.a(e) * (ue - u0) + aw * (uw - u0) + cf.b(e) * (un - u0) + bs * (us - u0) + cf.c(e) * (ut - u0) +
                     cb * (ub - u0);
     return f[e] - h2i * s;
@@ -228,48 +238,147 @@ __device__ inline float rb_resid_cell(const RbGeom &r, const float *u, const flo
 // coarse right-hand side = average of the 8 children's residuals; the
 // coarse level is split (fcs) or the natural-layout coarsest level (fcn)
 template <typename CT>
+__device__ inline void rb_restrict_cell(const RbGeom &rf, const float *u, const float *f, const Coef<CT> &cf,
+                                        float h2i, const RbGeom &rc, float *fcs, const Grid &gc, float *fcn, int I,
+                                        int J, int K)
+{
+    float t = 0.f;
+    for (int dz = 0; dz < 2; dz++)
+        for (int dy = 0; dy < 2; dy++)
+            for (int dx = 0; dx < 2; dx++)
+                t += rb_resid_cell<CT>(rf, u, f, cf, h2i, 2 * I + 2 + dx, 2 * J + 2 + dy, 2 * K + 2 + dz);
+    const float v = 0.125f * t;
+    if (fcs)
+        fcs[rc.at(I + 2, J + 2, K + 2)] = v;
+    else
+        fcn[gc.at(I + 2, J + 2, K + 2)] = v;
+}
+
+// thread (I, J, K) of the coarse interior, K = blockIdx.z
+template <typename CT>
 __global__ void __launch_bounds__(256) k_rb_resid_restrict(RbGeom rf, const float *u, const float *f, Coef<CT> cf,
                                                            float h2i, RbGeom rc, float *fcs, Grid gc, float *fcn)
 {
-    const int mx = gc.nx - 2, my = gc.ny - 2, mz = gc.nz - 2;
-    const size_t N = (size_t)mx * my * mz;
-    for (size_t p = (size_t)blockIdx.x * 256 + threadIdx.x; p < N; p += (size_t)gridDim.x * 256) {
-        const int I = (int)(p % mx), J = (int)((p / mx) % my), K = (int)(p / ((size_t)mx * my));
-        float t = 0.f;
-        for (int dz = 0; dz < 2; dz++)
-            for (int dy = 0; dy < 2; dy++)
-                for (int dx = 0; dx < 2; dx++)
-                    t += rb_resid_cell<CT>(rf, u, f, cf, h2i, 2 * I + 2 + dx, 2 * J + 2 + dy, 2 * K + 2 + dz);
-        const float v = 0.125f * t;
-        if (fcs)
-            fcs[rc.at(I + 2, J + 2, K + 2)] = v;
-        else
-            fcn[gc.at(I + 2, J + 2, K + 2)] = v;
+    const int I = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int J = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+    const int K = (int)blockIdx.z;
+    if (I >= gc.nx - 2 || J >= gc.ny - 2) return;
+    rb_restrict_cell<CT>(rf, u, f, cf, h2i, rc, fcs, gc, fcn, I, J, K);
+}
+
+// fine u += coarse u of the parent cell (injection), fine in split order:
+// entry m of row (j, k) in both colours, cells i = 2 + p + 2m with p = 0, 1,
+// both children of coarse (m+2, j', k')
+__device__ inline void rb_prolong_entry(const RbGeom &rf, float *u, const RbGeom &rc, const float *ucs,
+                                        const Grid &gc, const float *ucn, int m, int j, int k)
+{
+    const int ic = m + 2, jc = (j - 2) / 2 + 2, kc = (k - 2) / 2 + 2;
+    const float e = ucs ? ucs[rc.at(ic, jc, kc)] : ucn[gc.at(ic, jc, kc)];
+    const size_t o = ((size_t)(k - 2) * rf.my + (j - 2)) * rf.mh + m;
+    u[o] += e;
+    u[rf.half + o] += e;
+}
+
+__global__ void __launch_bounds__(256) k_rb_prolong(RbGeom rf, float *u, RbGeom rc, const float *ucs, Grid gc,
+                                                    const float *ucn)
+{
+    const int m = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + 2;
+    const int k = (int)blockIdx.z + 2;
+    if (m >= rf.mh || j > rf.ny - 1) return;
+    rb_prolong_entry(rf, u, rc, ucs, gc, ucn, m, j, k);
+}
+
+// The V-cycle below a small level in one workgroup: every colour pass,
+// restriction, the coarsest solve and every prolongation of the levels
+// 1..top as loops of TAIL_THREADS threads separated by barriers, with the
+// per-cell arithmetic of the kernels above (same results, ~4 launches per
+// level fewer).  Levels are indexed as in the context (0 = coarsest, the
+// natural-layout direct solve).
+constexpr int TAIL_THREADS = 1024;
+constexpr int TAIL_MAX_LEVELS = 8;
+constexpr size_t TAIL_MAX_CELLS = 4096;  // largest level taken into the tail
+
+template <typename CT>
+struct RbTail {
+    int top;  // highest level in the tail (>= 1)
+    RbGeom r[TAIL_MAX_LEVELS];
+    float *u[TAIL_MAX_LEVELS], *f[TAIL_MAX_LEVELS];
+    Coef<CT> cf[TAIL_MAX_LEVELS];
+    float h2[TAIL_MAX_LEVELS], h2i[TAIL_MAX_LEVELS];
+    int nu1[TAIL_MAX_LEVELS], nu2[TAIL_MAX_LEVELS];
+    float om;
+    Grid g0;  // coarsest level (natural layout)
+    float *f0, *u0;
+    const double *M;
+};
+
+template <typename CT, bool ZERO>
+__device__ inline void tail_pass(const RbTail<CT> &t, int l, int colour, int zown)
+{
+    const RbGeom &r = t.r[l];
+    const int n = (int)r.half;
+    for (int e = (int)threadIdx.x; e < n; e += TAIL_THREADS) {
+        const int m = e % r.mh, q = e / r.mh;
+        rb_cell<CT, ZERO>(r, t.u[l], t.f[l], t.cf[l], t.h2[l], colour, t.om, zown, m, q % r.my + 2, q / r.my + 2);
+    }
+    __syncthreads();
+}
+
+template <typename CT>
+__global__ void __launch_bounds__(TAIL_THREADS) k_rb_tail(RbTail<CT> t)
+{
+    __shared__ double y[512];
+    for (int l = t.top; l >= 1; l--) {
+        for (int q = 0; q < t.nu1[l]; q++) {
+            if (q == 0)
+                tail_pass<CT, true>(t, l, 0, 1);
+            else
+                tail_pass<CT, false>(t, l, 0, 0);
+            tail_pass<CT, false>(t, l, 1, q == 0);
+        }
+        const RbGeom &rf = t.r[l];
+        const bool csplit = l - 1 >= 1;
+        const int cx = rf.mh, cy = rf.my / 2, n = cx * cy * (rf.mz / 2);  // coarse interior
+        for (int e = (int)threadIdx.x; e < n; e += TAIL_THREADS) {
+            const int I = e % cx, q = e / cx;
+            rb_restrict_cell<CT>(rf, t.u[l], t.f[l], t.cf[l], t.h2i[l], t.r[csplit ? l - 1 : l],
+                                 csplit ? t.f[l - 1] : nullptr, t.g0, csplit ? nullptr : t.f0, I, q % cy, q / cy);
+        }
+        __syncthreads();
+    }
+    coarse_subst<float>(t.g0, t.M, t.f0, t.u0, y, (int)threadIdx.x, TAIL_THREADS);
+    __syncthreads();
+    for (int l = 1; l <= t.top; l++) {
+        const RbGeom &rf = t.r[l];
+        const bool csplit = l - 1 >= 1;
+        const int n = (int)rf.half;
+        for (int e = (int)threadIdx.x; e < n; e += TAIL_THREADS) {
+            const int m = e % rf.mh, q = e / rf.mh;
+            rb_prolong_entry(rf, t.u[l], csplit ? t.r[l - 1] : rf, csplit ? t.u[l - 1] : nullptr, t.g0,
+                             csplit ? nullptr : t.u0, m, q % rf.my + 2, q / rf.my + 2);
+        }
+        __syncthreads();
+        for (int q = 0; q < t.nu2[l]; q++) {
+            tail_pass<CT, false>(t, l, 1, 0);
+            tail_pass<CT, false>(t, l, 0, 0);
+        }
     }
 }
 
-// fine u += coarse u of the parent cell (injection), fine in split order
-__global__ void k_rb_prolong(RbGeom rf, float *u, RbGeom rc, const float *ucs, Grid gc, const float *ucn)
+// 256-thread block shaped to a row of `lanes` threads: x extent the
+// smallest power of two in 8..64 covering it, the rest along y
+dim3 row_block(int lanes)
 {
-    const size_t n2 = 2 * rf.half;
-    for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < n2; e += (size_t)gridDim.x * 256) {
-        const int colour = e >= rf.half ? 1 : 0;
-        const size_t q = e - (size_t)colour * rf.half;
-        const int m = (int)(q % rf.mh);
-        const int j = (int)((q / rf.mh) % rf.my) + 2;
-        const int k = (int)(q / ((size_t)rf.mh * rf.my)) + 2;
-        const int i = 2 + ((colour + j + k) & 1) + 2 * m;
-        const int ic = (i - 2) / 2 + 2, jc = (j - 2) / 2 + 2, kc = (k - 2) / 2 + 2;
-        u[e] += ucs ? ucs[rc.at(ic, jc, kc)] : ucn[gc.at(ic, jc, kc)];
-    }
+    int bx = 64;
+    while (bx > 8 && bx / 2 >= lanes) bx /= 2;
+    return dim3((unsigned)bx, (unsigned)(256 / bx));
 }
 
-unsigned gridn(size_t n)
+dim3 row_grid(const dim3 &b, int lanes, int rows, int planes)
 {
-    size_t b = (n + 255) / 256;
-    if (b > 65535) b = 65535;
-    if (b < 1) b = 1;
-    return (unsigned)b;
+    return dim3((unsigned)((lanes + (int)b.x - 1) / (int)b.x), (unsigned)((rows + (int)b.y - 1) / (int)b.y),
+                (unsigned)planes);
 }
 
 double lev_h2i(const Ctx &c, const Grid &g)
@@ -288,7 +397,7 @@ int colour_pass(Ctx &c, int l, int colour, bool zero, int zown)
     const RbGeom r = rb_geom(L.g);
     const Coef<CT> cf{(const CT *)L.rb_coef, 2 * r.half};
     const float h2 = (float)(1.0 / lev_h2i(c, L.g));
-    const dim3 grid((unsigned)((r.mh + BX - 1) / BX), (unsigned)((r.my + BY - 1) / BY), (unsigned)r.mz);
+    const dim3 b1 = row_block(r.mh), grid = row_grid(b1, r.mh, r.my, r.mz);
     const bool top = l == c.ng - 1;
     // algorithmic bytes: own f read and u write (one colour, FP32), the other
     // colour's u (not read from a zero iterate) and all coefficients
@@ -296,24 +405,64 @@ int colour_pass(Ctx &c, int l, int colour, bool zero, int zown)
     const double bytes = cells * (zero ? 8.0 : (om == 1.f || zown ? 12.0 : 16.0)) + 6.0 * cells * (double)sizeof(CT);
     if (top) prof_begin(PK_RB_COLOUR, c.stream);
     if (r.mh % 4 == 0) {
-        const dim3 g4((unsigned)((r.mh / 4 + BX - 1) / BX), (unsigned)((r.my + BY - 1) / BY), (unsigned)r.mz);
+        const dim3 b4 = row_block(r.mh / 4), g4 = row_grid(b4, r.mh / 4, r.my, r.mz);
         if (top && zero)
-            hipLaunchKernelGGL((k_rb_colour4<CT, true, true>), g4, interior_block(), 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, colour, om, zown);
+            hipLaunchKernelGGL((k_rb_colour4<CT, true, true>), g4, b4, 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, colour, om, zown);
         else if (top)
-            hipLaunchKernelGGL((k_rb_colour4<CT, false, true>), g4, interior_block(), 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, colour, om, zown);
+            hipLaunchKernelGGL((k_rb_colour4<CT, false, true>), g4, b4, 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, colour, om, zown);
         else if (zero)
-            hipLaunchKernelGGL((k_rb_colour4<CT, true, false>), g4, interior_block(), 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, colour, om, zown);
+            hipLaunchKernelGGL((k_rb_colour4<CT, true, false>), g4, b4, 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, colour, om, zown);
         else
-            hipLaunchKernelGGL((k_rb_colour4<CT, false, false>), g4, interior_block(), 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, colour, om, zown);
+            hipLaunchKernelGGL((k_rb_colour4<CT, false, false>), g4, b4, 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, colour, om, zown);
     } else if (top && zero)
-        hipLaunchKernelGGL((k_rb_colour<CT, true, true>), grid, interior_block(), 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, colour, om, zown);
+        hipLaunchKernelGGL((k_rb_colour<CT, true, true>), grid, b1, 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, colour, om, zown);
     else if (top)
-        hipLaunchKernelGGL((k_rb_colour<CT, false, true>), grid, interior_block(), 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, colour, om, zown);
+        hipLaunchKernelGGL((k_rb_colour<CT, false, true>), grid, b1, 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, colour, om, zown);
     else if (zero)
-        hipLaunchKernelGGL((k_rb_colour<CT, true, false>), grid, interior_block(), 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, colour, om, zown);
+        hipLaunchKernelGGL((k_rb_colour<CT, true, false>), grid, b1, 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, colour, om, zown);
     else
-        hipLaunchKernelGGL((k_rb_colour<CT, false, false>), grid, interior_block(), 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, colour, om, zown);
+        hipLaunchKernelGGL((k_rb_colour<CT, false, false>), grid, b1, 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, colour, om, zown);
     if (top) prof_end(PK_RB_COLOUR, c.stream, bytes);
+    DG_CHECK_LAUNCH();
+    return 0;
+}
+
+// highest level taken into the single-workgroup tail, or -1
+int tail_level(const Ctx &c)
+{
+    int lt = -1;
+    for (int l = 1; l < c.ng - 1 && l < TAIL_MAX_LEVELS; l++)
+        if (2 * rb_geom(c.lev[l].g).half <= TAIL_MAX_CELLS) lt = l;
+    return lt;
+}
+
+template <typename CT>
+int launch_tail(Ctx &c, int lt)
+{
+    const Grid &g0 = c.lev[0].g;
+    if ((g0.nx - 2) * (g0.ny - 2) * (g0.nz - 2) > 512) {
+        set_error("split V-cycle tail: coarsest level above 512 unknowns");
+        return -1;
+    }
+    RbTail<CT> t{};
+    t.top = lt;
+    for (int l = 1; l <= lt; l++) {
+        Level &L = c.lev[l];
+        t.r[l] = rb_geom(L.g);
+        t.u[l] = L.rb_u;
+        t.f[l] = L.rb_f;
+        t.cf[l] = Coef<CT>{(const CT *)L.rb_coef, 2 * t.r[l].half};
+        t.h2[l] = (float)(1.0 / lev_h2i(c, L.g));
+        t.h2i[l] = (float)lev_h2i(c, L.g);
+        t.nu1[l] = c.mg_nuc <= 0 ? c.mg_nu1 : c.mg_nuc;
+        t.nu2[l] = c.mg_nuc <= 0 ? c.mg_nu2 : c.mg_nuc;
+    }
+    t.om = (float)c.mg_omega;
+    t.g0 = g0;
+    t.f0 = c.lev[0].f32;
+    t.u0 = c.lev[0].u32;
+    t.M = coarse_factor();
+    hipLaunchKernelGGL(k_rb_tail<CT>, dim3(1), dim3(TAIL_THREADS), 0, c.stream, t);
     DG_CHECK_LAUNCH();
     return 0;
 }
@@ -321,6 +470,7 @@ int colour_pass(Ctx &c, int l, int colour, bool zero, int zown)
 template <typename CT>
 int vcycle_rb(Ctx &c, int l)
 {
+    if (l == tail_level(c)) return launch_tail<CT>(c, l);
     Level &L = c.lev[l];
     const bool top = l == c.ng - 1;
     const int nu1 = (top || c.mg_nuc <= 0) ? c.mg_nu1 : c.mg_nuc;
@@ -334,8 +484,8 @@ int vcycle_rb(Ctx &c, int l)
     Level &C = c.lev[l - 1];
     const bool csplit = l - 1 >= 1;
     const RbGeom rc = rb_geom(C.g);
-    const size_t nc = (size_t)(C.g.nx - 2) * (C.g.ny - 2) * (C.g.nz - 2);
-    hipLaunchKernelGGL(k_rb_resid_restrict<CT>, dim3(gridn(nc)), dim3(256), 0, c.stream, rf, L.rb_u, L.rb_f, cf,
+    const dim3 bc = row_block(C.g.nx - 2);
+    hipLaunchKernelGGL(k_rb_resid_restrict<CT>, row_grid(bc, C.g.nx - 2, C.g.ny - 2, C.g.nz - 2), bc, 0, c.stream, rf, L.rb_u, L.rb_f, cf,
                        (float)lev_h2i(c, L.g), rc, csplit ? C.rb_f : nullptr, C.g, csplit ? nullptr : C.f32);
     DG_CHECK_LAUNCH();
     if (csplit) {
@@ -343,7 +493,8 @@ int vcycle_rb(Ctx &c, int l)
     } else {
         if (coarse_solve_f32(c)) return -1;
     }
-    hipLaunchKernelGGL(k_rb_prolong, dim3(gridn(2 * rf.half)), dim3(256), 0, c.stream, rf, L.rb_u, rc,
+    const dim3 bf = row_block(rf.mh);
+    hipLaunchKernelGGL(k_rb_prolong, row_grid(bf, rf.mh, rf.my, rf.mz), bf, 0, c.stream, rf, L.rb_u, rc,
                        csplit ? C.rb_u : nullptr, C.g, csplit ? nullptr : C.u32);
     DG_CHECK_LAUNCH();
     for (int q = 0; q < nu2; q++) {
@@ -362,6 +513,9 @@ bool rb_enabled(const Ctx &c)
         const Grid &g = c.lev[l].g;
         if ((g.nx - 2) & 1 || (g.ny - 2) & 1 || (g.nz - 2) & 1) return false;
     }
+    // the split kernels index one level with 32-bit offsets
+    const Grid &gt = c.lev[c.ng - 1].g;
+    if ((double)(gt.nx - 2) * (gt.ny - 2) * (gt.nz - 2) >= 2147483648.0) return false;
     return true;
 }
 
@@ -385,13 +539,11 @@ int rb_prepare(Ctx &c)
             DG_HIP(hipMalloc(&L.rb_coef, 3 * n2 * (want16 ? sizeof(_Float16) : sizeof(float))));
             L.rb_coef16 = want16;
         }
-        const size_t N = (size_t)(L.g.nx - 2) * (L.g.ny - 2) * (L.g.nz - 2);
+        const dim3 bl = row_block(L.g.nx - 2), gr = row_grid(bl, L.g.nx - 2, L.g.ny - 2, L.g.nz - 2);
         if (want16)
-            hipLaunchKernelGGL(k_rb_coef<_Float16>, dim3(gridn(N)), dim3(256), 0, c.stream, L.g, r, L.a, L.b, L.c,
-                               (_Float16 *)L.rb_coef);
+            hipLaunchKernelGGL(k_rb_coef<_Float16>, gr, bl, 0, c.stream, L.g, r, L.a, L.b, L.c, (_Float16 *)L.rb_coef);
         else
-            hipLaunchKernelGGL(k_rb_coef<float>, dim3(gridn(N)), dim3(256), 0, c.stream, L.g, r, L.a, L.b, L.c,
-                               (float *)L.rb_coef);
+            hipLaunchKernelGGL(k_rb_coef<float>, gr, bl, 0, c.stream, L.g, r, L.a, L.b, L.c, (float *)L.rb_coef);
         DG_CHECK_LAUNCH();
     }
     return 0;
@@ -401,8 +553,8 @@ int rb_load_rhs(Ctx &c, const double *r)
 {
     Level &F = c.lev[c.ng - 1];
     const RbGeom rg = rb_geom(F.g);
-    const size_t N = (size_t)(F.g.nx - 2) * (F.g.ny - 2) * (F.g.nz - 2);
-    hipLaunchKernelGGL(k_rb_in, dim3(gridn(N)), dim3(256), 0, c.stream, F.g, rg, r, F.rb_f);
+    const dim3 bl = row_block(F.g.nx - 2), gr = row_grid(bl, F.g.nx - 2, F.g.ny - 2, F.g.nz - 2);
+    hipLaunchKernelGGL(k_rb_in, gr, bl, 0, c.stream, F.g, rg, r, F.rb_f);
     DG_CHECK_LAUNCH();
     return 0;
 }

@@ -167,7 +167,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU decomposition is a design in DESIGN.md, not a measured path)", file=sys.stderr)
+              "(the multi-GPU decomposition is a design, not a measured path)", file=sys.stderr)
         sys.exit(2)
     dg = importlib.import_module("3d_droplet_simulation_2007_amd")
     dg.lib()

@@ -3,7 +3,7 @@
  * for the per-time-step path of farzan-t/3D_Droplet_simulation_2007.
  *
  * Conventions (Fortran-friendly, so the reference's Fortran-77 host can bind
- * them through ISO_C_BINDING interface blocks, see INTEGRATION.md):
+ * them through ISO_C_BINDING interface blocks):
  *  - every scalar argument is passed by reference (const int*, const double*);
  *  - arrays are FP64, column-major (nx,ny,nz) with one ghost layer, exactly
  *    the reference's layout (input.h:33), passed as plain host pointers;
@@ -74,8 +74,9 @@ int dg_get_field(const int *id, double *host);
 int dg_set_field(const int *id, const double *host);
 int dg_get_state(dg_state *out);
 int dg_set_state(const dg_state *in);   /* sets tau,time,tauh,tauh2,tswap,npre,ntop,npost */
-/* Clips the resident volume fraction to [0,1] (smooth2 can
- * leave 1+4e-13, which sends the reference's AL3D to NaN). */
+/* Clips the resident volume fraction to [0,1] (a deviation from the
+ * reference: smooth2 can leave 1+4e-13, which sends the reference's AL3D,
+ * main.f:4562-4634, to NaN). */
 int dg_clip_cc(void);
 
 /* Pressure-solver selection (no reference counterpart):

@@ -1,6 +1,6 @@
 """The CPU restatement against the live flang-built reference
 (oracle/_ref/libref_*.so from oracle/build_ref.sh).  Skipped where the
-reference was not built (the GPU box).  Bar: bit-exact."""
+reference was not built (machines without /root/reference).  Bar: bit-exact."""
 import numpy as np
 import pytest
 

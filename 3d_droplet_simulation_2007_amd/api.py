@@ -266,17 +266,25 @@ def set_smoother(omega=1.0, nu_coarse=0):
     _check(lib().dg_set_smoother(_d(omega), _i(nu_coarse)), "dg_set_smoother")
 
 
-def set_tuning(fused=False, graphs=True, split=True):
+def set_tuning(fused=False, graphs=True, split=True, tail="lds"):
     """Tolerance-mode execution switches:
-    fused -- one-launch two-colour smoother sweep instead of two colour passes
-             (bit-identical);
+    fused -- one-launch two-colour smoother sweeps instead of two colour
+             passes (bit-identical): False nowhere, True on the natural layout
+             (Gauss-Seidel only) and the split finest level, "split" on the
+             split finest level only;
     graphs -- replay each PCG iteration as a captured HIP graph (bit-identical);
     split -- mode 2's FP32 V-cycle on red-black split storage with FP16
              coefficients where the densities allow (same tolerance reached);
              False natural storage, "fp32" / "fp16" split with that
-             coefficient type regardless of the densities."""
+             coefficient type regardless of the densities;
+    tail -- with split storage, the levels of at most 4096 cells and the
+            coarsest solve: None one launch per operation, "global" one
+            single-workgroup kernel, "lds" that kernel on LDS copies when
+            they fit (bit-identical)."""
     sv = {False: 0, True: 1, "fp32": 2, "fp16": 3}[split]
-    _check(lib().dg_set_tuning(_i(1 if fused else 0), _i(1 if graphs else 0), _i(sv)), "dg_set_tuning")
+    tv = {None: 0, "global": 1, "lds": 2}[tail]
+    fv = {False: 0, True: 1, "split": 2}[fused]
+    _check(lib().dg_set_tuning(_i(fv), _i(1 if graphs else 0), _i(sv), _i(tv)), "dg_set_tuning")
 
 
 # kernel families dg_profile_start accepts

@@ -46,7 +46,7 @@ Prof &prof()
 }
 
 static bool g_ready = false;
-static int g_fused_default = 0, g_graphs_default = 1, g_split_default = 1;
+static int g_fused_default = 0, g_graphs_default = 1, g_split_default = 1, g_tail_default = 2;
 static double g_omega_default = 1.0;  // tolerance-mode smoother (dg_set_smoother)
 static int g_nuc_default = 0;
 
@@ -97,6 +97,14 @@ static int alloc_levels(Ctx &c, int nx, int ny, int nz, bool strict)
     int dims[MAXLEV][3], ng;
     if (plan_levels(nx, ny, nz, strict, dims, &ng)) return -1;
     c.ng = ng;
+    {
+        // the reference's form nn = 2**ng + 2, which its FMG schedule needs
+        int nn = nx < ny ? nx : ny;
+        if (nz < nn) nn = nz;
+        int j = nn - 2, ngr = 1;
+        while (j > 2) { j = j / 2; ngr++; }
+        c.ref_grid = (1 << ngr) + 2 == nn && ngr == ng;
+    }
     for (int l = 0; l < ng; l++) {
         Level &L = c.lev[l];
         L.g = make_grid(dims[l][0], dims[l][1], dims[l][2]);
@@ -135,7 +143,7 @@ static void free_ctx(Ctx &c)
     for (int l = 0; l < c.ng; l++) {
         Level &L = c.lev[l];
         void *ps[] = {L.rhs, L.res, L.u, L.cc, L.a, L.b, L.c, L.ei, L.a32, L.b32, L.c32,
-                      L.u32, L.f32, L.r32, L.u64, L.f64, L.r64, L.w32, L.w64, L.rb_u, L.rb_f, L.rb_coef};
+                      L.u32, L.f32, L.r32, L.u64, L.f64, L.r64, L.w32, L.w64, L.rb_u, L.rb_f, L.rb_w, L.rb_coef};
         for (void *p : ps)
             if (p) (void)hipFree(p);
         L = Level();
@@ -374,11 +382,14 @@ int dg_init(const char *path)
         return -1;
     }
     if (check_dims(in.nx, in.ny, in.nz)) return -1;
-    if (alloc_levels(c, in.nx, in.ny, in.nz, true)) return -1;
+    // m*2**k + 2 grids are accepted for the tolerance mode; the reference's
+    // 2**ng + 2 check moves to its FMG schedule (op_mglin)
+    if (alloc_levels(c, in.nx, in.ny, in.nz, false)) return -1;
     if (alloc_fields(c)) return -1;
     c.fused_smoother = g_fused_default;
     c.use_graphs = g_graphs_default;
     c.rb_split = g_split_default;
+    c.rb_tail = g_tail_default;
     c.mg_omega = g_omega_default;
     c.mg_nuc = g_nuc_default;
     nondim(in, c.s);
@@ -559,19 +570,21 @@ int dg_set_vof_guard(const int *on)
     return 0;
 }
 
-int dg_set_tuning(const int *fused, const int *graphs, const int *split)
+int dg_set_tuning(const int *fused, const int *graphs, const int *split, const int *tail)
 {
-    g_fused_default = *fused ? 1 : 0;
-    g_graphs_default = *graphs ? 1 : 0;
-    if (*split < 0 || *split > 3) {
-        set_error("dg_set_tuning: split must be 0..3");
+    if (*fused < 0 || *fused > 2 || *split < 0 || *split > 3 || *tail < 0 || *tail > 2) {
+        set_error("dg_set_tuning: fused must be 0..2, split 0..3 and tail 0..2");
         return -1;
     }
+    g_fused_default = *fused;
+    g_graphs_default = *graphs ? 1 : 0;
     g_split_default = *split;
+    g_tail_default = *tail;
     Ctx &c = *ctx();
     c.fused_smoother = g_fused_default;
     c.use_graphs = g_graphs_default;
     c.rb_split = g_split_default;
+    c.rb_tail = g_tail_default;
     return 0;
 }
 
@@ -768,6 +781,7 @@ int dg_poisson_solve(const int *nx, const int *ny, const int *nz, const double *
     lc.fused_smoother = g_fused_default;
     lc.use_graphs = g_graphs_default;
     lc.rb_split = g_split_default;
+    lc.rb_tail = g_tail_default;
     lc.coef16_ok = 0;  // arbitrary coefficients: FP32 in the split V-cycle
     lc.mg_omega = g_omega_default;
     lc.mg_nuc = g_nuc_default;

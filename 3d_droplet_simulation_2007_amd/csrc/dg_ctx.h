@@ -23,6 +23,7 @@ struct Level {
     // red-black split storage of the FP32 V-cycle (levels above the
     // coarsest): u and f per colour, a/b/c per colour in FP16 or FP32
     float *rb_u = nullptr, *rb_f = nullptr;
+    float *rb_w = nullptr;  // finest level: second iterate buffer of the fused (out-of-place) sweep
     void *rb_coef = nullptr;
     int rb_coef16 = -1;  // type rb_coef was allocated for (-1: none)
 };
@@ -61,12 +62,16 @@ struct Ctx {
     int mg_nuc = 0;         // sweeps (pre and post) below the finest level; 0: mg_nu1 / mg_nu2
     int ordered_sums = 1;  // reproduce the reference's sequential outflow sums
     int vof_guard = 0;     // 1: plane normal to the sweep where the Youngs normal vanishes
-    int fused_smoother = 0;  // 1: one-pass red-black sweeps (slower on gfx950 so far), 0: two colour passes
+    int fused_smoother = 0;  // one-pass red-black sweeps: 0 nowhere, 1 natural layout (Gauss-Seidel) and the
+                             // split finest level, 2 the split finest level only (both slower than two
+                             // passes on gfx950 so far: latency-bound plane marching)
     int use_graphs = 1;      // replay each PCG iteration as a captured HIP graph
+    int ref_grid = 1;        // the grid has the reference's form nn = 2**ng + 2 (mglin's schedule may run)
     int rb_split = 1;        // mode 2: red-black split storage of the V-cycle levels
+    int rb_tail = 2;         // split levels of <= 4096 cells: 0 per-level launches, 1 one-workgroup tail, 2 LDS tail
     int coef16_ok = 0;       // face coefficients fit FP16 (set from the fluids' densities)
     hipGraphExec_t pcg_graph[2] = {nullptr, nullptr};
-    int pcg_graph_key[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int pcg_graph_key[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     // diagnostics of the last step
     int mg_iters_last = 0;
     double mg_relres_last = 0.0, cfl_last = 0.0, epsdiv_last = 0.0;

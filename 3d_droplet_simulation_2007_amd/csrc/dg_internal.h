@@ -61,10 +61,28 @@ inline dim3 interior_grid(const Grid &g)
 }
 inline dim3 interior_block() { return dim3(BX, BY, 1); }
 
-#define DG_INTERIOR_IJK(g)                                         \
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 2; \
-    const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + 2; \
-    const int k = (int)blockIdx.z + 2;                              \
+// The dispatcher deals a launch's workgroups to the 8 XCDs round-robin in
+// launch order, and each XCD has its own L2.  xcd_tile() renumbers the
+// launch so that XCD x runs one contiguous run of tiles (x fastest, then y,
+// then z): the tiles beside a tile, whose halo rows and shared cache lines
+// it reads, then sit in the same L2.  Returns the tile (bx, by, bz) of this
+// workgroup, and its renumbered linear index through *lin.
+__device__ inline uint3 xcd_tile(unsigned *lin = nullptr)
+{
+    const unsigned gx = gridDim.x, gy = gridDim.y;
+    const unsigned n = gx * gy * gridDim.z;
+    unsigned id = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const unsigned per = n / 8;
+    if (id < per * 8) id = (id % 8) * per + id / 8;
+    if (lin) *lin = id;
+    return make_uint3(id % gx, (id / gx) % gy, id / (gx * gy));
+}
+
+#define DG_INTERIOR_IJK(g)                                     \
+    const uint3 tile_ = ::dg::xcd_tile();                       \
+    const int i = (int)(tile_.x * blockDim.x + threadIdx.x) + 2; \
+    const int j = (int)(tile_.y * blockDim.y + threadIdx.y) + 2; \
+    const int k = (int)tile_.z + 2;                             \
     if (i > (g).nx - 1 || j > (g).ny - 1) return;
 
 // 2-D launch over a face: (n1 x n2) points

@@ -33,9 +33,10 @@ __global__ void __launch_bounds__(256) k_relax_colour(Grid g, double *u, const d
                                                       const double *b, const double *c, const double *ei, double h2,
                                                       double omega, int colour)
 {
-    const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + 2;
-    const int k = (int)blockIdx.z + 2;
-    const int i = 2 * (int)(blockIdx.x * blockDim.x + threadIdx.x) + 2 + ((colour + j + k) & 1);
+    const uint3 tile = xcd_tile();
+    const int j = (int)(tile.y * blockDim.y + threadIdx.y) + 2;
+    const int k = (int)tile.z + 2;
+    const int i = 2 * (int)(tile.x * blockDim.x + threadIdx.x) + 2 + ((colour + j + k) & 1);
     if (i > g.nx - 1 || j > g.ny - 1) return;
     const int o = g.at(i, j, k), sy = g.sy, sz = g.sz;
     double temp = (a[o] * u[o + 1] + a[o - 1] * u[o - 1] + b[o] * u[o + sy] + b[o - sy] * u[o - sy] + c[o] * u[o + sz] +
@@ -153,6 +154,10 @@ static int slvsml(Ctx &c, double lengthsize)
 
 int op_mglin(Ctx &c)
 {
+    if (!c.ref_grid) {  // memoryaloc's STOP (main.f:274-322)
+        set_error("STOP: nn .ne. 2**ng + 2 (the reference's multigrid schedule; use a tolerance-mode solver)");
+        return -1;
+    }
     hipStream_t s = c.stream;
     const int ng = c.ng;
     for (int gi = ng - 1; gi >= 1; gi--) {
@@ -205,9 +210,10 @@ template <typename T, bool TOP>
 __global__ void __launch_bounds__(256) k_gs_colour(Grid g, T *u, const T *f, const T *a, const T *b, const T *c, T h2,
                                                    int colour, int zero_init, T om, int zero_own)
 {
-    const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + 2;
-    const int k = (int)blockIdx.z + 2;
-    const int i = 2 * (int)(blockIdx.x * blockDim.x + threadIdx.x) + 2 + ((colour + j + k) & 1);
+    const uint3 tile = xcd_tile();
+    const int j = (int)(tile.y * blockDim.y + threadIdx.y) + 2;
+    const int k = (int)tile.z + 2;
+    const int i = 2 * (int)(tile.x * blockDim.x + threadIdx.x) + 2 + ((colour + j + k) & 1);
     if (i > g.nx - 1 || j > g.ny - 1) return;
     const int o = g.at(i, j, k), sy = g.sy, sz = g.sz;
     const T ae = a[o], aw = a[o - 1], bn = b[o], bs = b[o - sy], ct = c[o], cb = c[o - sz];
@@ -551,7 +557,8 @@ static int vcycle(Ctx &c, int l)
     const int nu1 = (top || c.mg_nuc <= 0) ? c.mg_nu1 : c.mg_nuc;
     const int nu2 = (top || c.mg_nuc <= 0) ? c.mg_nu2 : c.mg_nuc;
     const T om = (T)c.mg_omega;
-    if (c.fused_smoother && (c.mg_omega != 1.0 || nu1 != c.mg_nu1)) {
+    const bool fused = c.fused_smoother == 1;  // natural layout: one-pass sweeps on request only
+    if (fused && (c.mg_omega != 1.0 || nu1 != c.mg_nu1)) {
         set_error("the fused smoother sweep supports Gauss-Seidel with equal sweep counts on all levels only");
         return -1;
     }
@@ -559,7 +566,7 @@ static int vcycle(Ctx &c, int l)
     // sweeps the buffers alternate; the first one is chosen so that the last
     // post-smoothing sweep lands in u, where the caller reads the result.
     T *cur = u;
-    if (c.fused_smoother) {
+    if (fused) {
         T *target = (c.mg_nu2 & 1) ? w : u;  // buffer that must hold the pre-smoothed iterate
         T *other = target == u ? w : u;
         cur = (c.mg_nu1 & 1) ? target : other;  // the first sweep writes here
@@ -590,7 +597,7 @@ static int vcycle(Ctx &c, int l)
     if (vcycle<T>(c, l - 1)) return -1;
     hipLaunchKernelGGL(k_prolong_inj_add<T>, interior_grid(g), interior_block(), 0, c.stream, g, C.g, cur, uc);
     DG_CHECK_LAUNCH();
-    if (c.fused_smoother) {
+    if (fused) {
         for (int sidx = 0; sidx < c.mg_nu2; sidx++) {
             T *nxt = cur == u ? w : u;
             if (fused_sweep<T>(c, g, cur, nxt, f, a, b, cc, h2, 1, false, top)) return -1;
@@ -657,39 +664,69 @@ enum { S_RZ = 16, S_ZQ, S_RZOLD, S_ALPHA, S_PQ, S_RR, S_BB, S_FIRST };
 
 // K2: p_new = z + beta p_old (beta from device scalars), q = A p_new,
 // partial sums of p_new . q
-// planes per thread of k_pcg_dir: each thread marches up a column of
-// PCG_KCH cells, carrying the plane below's direction value and face
-// coefficient in registers
+// planes per workgroup of k_pcg_dir
 constexpr int PCG_KCH = 8;
 
+// A workgroup of 64 x 4 threads owns a 64 x 4 column tile and marches up
+// PCG_KCH planes.  Per plane it forms p_new = z + beta p_old once for the
+// tile and a one-cell ring around it in LDS (two plane buffers: the
+// current plane and the next), so the in-plane neighbours come from LDS;
+// each thread carries its column's plane-below value and c-coefficient in
+// registers.  The per-cell arithmetic is the stencil of resid
+// (main.f:3917-3923) on p_new with bc_scalar folding.
 template <class Z>
 __global__ void __launch_bounds__(256) k_pcg_dir(Grid g, Z z, const double *pold, double *pnew, double *q,
                                                  const double *a, const double *b, const double *c, double h2i,
                                                  const double *ds, double *partial)
 {
+    constexpr int TW = 66, TH = 6;  // tile + ring
+    __shared__ double pl[2][TH][TW];
     __shared__ double lds4[4];
-    const int bidx = (int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 2;
-    const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + 2;
-    const int k0 = (int)blockIdx.z * PCG_KCH + 2;
+    unsigned lin;
+    const uint3 tile = xcd_tile(&lin);
+    const int bidx = (int)lin;
+    const int tx = (int)threadIdx.x, ty = (int)threadIdx.y, tid = tx + 64 * ty;
+    const int i0 = (int)tile.x * 64 + 2, j0 = (int)tile.y * 4 + 2;
+    const int i = i0 + tx, j = j0 + ty;
+    const int k0 = (int)tile.z * PCG_KCH + 2;
     const int k1 = min(k0 + PCG_KCH, g.nz);  // interior planes k0 .. k1 - 1 (the last is nz - 1)
-    const int tid = (int)(threadIdx.x + blockDim.x * threadIdx.y);
-    double contrib = 0.0;
     const bool first = ds[S_FIRST] != 0.0;
     const double beta = first ? 0.0 : -ds[S_ALPHA] * ds[S_ZQ] / ds[S_RZOLD];
-    if (i <= g.nx - 1 && j <= g.ny - 1) {
-        const int sy = g.sy, sz = g.sz;
-        auto P = [&](int ii, int jj, int kk, int oo) { return z(ii, jj, kk, oo) + beta * pold[oo]; };
-        int o = g.at(i, j, k0);
-        double p0 = P(i, j, k0, o);
-        double pb = k0 > 2 ? P(i, j, k0 - 1, o - sz) : p0;
-        double cb = c[o - sz];
-        for (int k = k0; k < k1; k++, o += sz) {
-            const double pt = k < g.nz - 1 ? P(i, j, k + 1, o + sz) : p0;
-            const double pe = i < g.nx - 1 ? P(i + 1, j, k, o + 1) : 0.0;
-            const double pw = i > 2 ? P(i - 1, j, k, o - 1) : p0;
-            const double pn = j < g.ny - 1 ? P(i, j + 1, k, o + sy) : p0;
-            const double ps = j > 2 ? P(i, j - 1, k, o - sy) : p0;
+    const int sy = g.sy, sz = g.sz;
+    auto P = [&](int ii, int jj, int kk) {
+        const int oo = g.at(ii, jj, kk);
+        return z(ii, jj, kk, oo) + beta * pold[oo];
+    };
+    // p_new of plane kk on the tile and its ring (0 outside the interior,
+    // where no stencil reads it)
+    auto fill = [&](int buf, int kk) {
+        for (int e = tid; e < TH * TW; e += 256) {
+            const int yy = e / TW, xx = e - yy * TW;
+            const int ii = i0 - 1 + xx, jj = j0 - 1 + yy;
+            pl[buf][yy][xx] = (ii >= 2 && ii <= g.nx - 1 && jj >= 2 && jj <= g.ny - 1) ? P(ii, jj, kk) : 0.0;
+        }
+    };
+    const bool in = i <= g.nx - 1 && j <= g.ny - 1;
+    double contrib = 0.0, p0 = 0.0, pb = 0.0, cb = 0.0;
+    fill(0, k0);
+    __syncthreads();
+    if (in) {
+        const int o = g.at(i, j, k0);
+        p0 = pl[0][ty + 1][tx + 1];
+        pb = k0 > 2 ? P(i, j, k0 - 1) : p0;
+        cb = c[o - sz];
+    }
+    int cur = 0;
+    for (int k = k0; k < k1; k++) {
+        if (k < g.nz - 1) fill(cur ^ 1, k + 1);
+        __syncthreads();
+        if (in) {
+            const int o = g.at(i, j, k);
+            const double pt = k < g.nz - 1 ? pl[cur ^ 1][ty + 1][tx + 1] : p0;
+            const double pe = i < g.nx - 1 ? pl[cur][ty + 1][tx + 2] : 0.0;
+            const double pw = i > 2 ? pl[cur][ty + 1][tx] : p0;
+            const double pn = j < g.ny - 1 ? pl[cur][ty + 2][tx + 1] : p0;
+            const double ps = j > 2 ? pl[cur][ty][tx + 1] : p0;
             const double ct = c[o];
             const double s = a[o] * (pe - p0) + a[o - 1] * (pw - p0) + b[o] * (pn - p0) + b[o - sy] * (ps - p0) +
                              ct * (pt - p0) + cb * (pb - p0);
@@ -701,6 +738,8 @@ __global__ void __launch_bounds__(256) k_pcg_dir(Grid g, Z z, const double *pold
             p0 = pt;
             cb = ct;
         }
+        __syncthreads();
+        cur ^= 1;
     }
     // block of 64x4: reduce
     double v = wave_sum(contrib);
@@ -716,10 +755,12 @@ __global__ void __launch_bounds__(256) k_pcg_update(Grid g, double *x, double *r
                                                     double *ds, double *partial, float *fs, RbGeom rg)
 {
     __shared__ double lds4[4];
-    const int bidx = (int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 2;
-    const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + 2;
-    const int k = (int)blockIdx.z + 2;
+    unsigned lin;
+    const uint3 tile = xcd_tile(&lin);
+    const int bidx = (int)lin;
+    const int i = (int)(tile.x * blockDim.x + threadIdx.x) + 2;
+    const int j = (int)(tile.y * blockDim.y + threadIdx.y) + 2;
+    const int k = (int)tile.z + 2;
     const int tid = (int)(threadIdx.x + blockDim.x * threadIdx.y);
     const double alpha = ds[S_RZ] / ds[S_PQ];
     double contrib = 0.0;
@@ -790,10 +831,12 @@ __global__ void __launch_bounds__(256) k_init_resid(Grid g, double *r, const dou
 {
     __shared__ double lds4[4];
     __shared__ double lds4b[4];
-    const int bidx = (int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 2;
-    const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + 2;
-    const int k = (int)blockIdx.z + 2;
+    unsigned lin;
+    const uint3 tile = xcd_tile(&lin);
+    const int bidx = (int)lin;
+    const int i = (int)(tile.x * blockDim.x + threadIdx.x) + 2;
+    const int j = (int)(tile.y * blockDim.y + threadIdx.y) + 2;
+    const int k = (int)tile.z + 2;
     const int tid = (int)(threadIdx.x + blockDim.x * threadIdx.y);
     double c1 = 0.0, c2 = 0.0;
     if (i <= g.nx - 1 && j <= g.ny - 1) {
@@ -958,10 +1001,12 @@ template <typename T>
 static int pcg_graphs(Ctx &c)
 {
     const int rb = sizeof(T) == 4 && rb_enabled(c);
-    const int key[8] = {c.mg_mode, c.mg_nu1, c.mg_nu2, c.fused_smoother, c.ng, (int)c.g.nx,
-                        rb, rb ? c.lev[c.ng - 1].rb_coef16 : -1};
+    // everything that changes the captured kernel sequence or its arguments
+    const int key[12] = {c.mg_mode, c.mg_nu1, c.mg_nu2, c.fused_smoother, c.ng, (int)c.g.nx,
+                         rb, rb ? c.lev[c.ng - 1].rb_coef16 : -1, rb ? c.rb_tail : -1, c.mg_nuc,
+                         (int)llround(c.mg_omega * 1e9), (int)c.g.nz};
     bool ok = c.pcg_graph[0] && c.pcg_graph[1];
-    for (int q = 0; q < 8 && ok; q++) ok = c.pcg_graph_key[q] == key[q];
+    for (int q = 0; q < 12 && ok; q++) ok = c.pcg_graph_key[q] == key[q];
     if (ok) return 0;
     for (int q = 0; q < 2; q++) {
         if (c.pcg_graph[q]) {
@@ -983,7 +1028,7 @@ static int pcg_graphs(Ctx &c)
         DG_HIP(hipGraphInstantiate(&c.pcg_graph[par], graph, nullptr, nullptr, 0));
         DG_HIP(hipGraphDestroy(graph));
     }
-    for (int q = 0; q < 8; q++) c.pcg_graph_key[q] = key[q];
+    for (int q = 0; q < 12; q++) c.pcg_graph_key[q] = key[q];
     return 0;
 }
 

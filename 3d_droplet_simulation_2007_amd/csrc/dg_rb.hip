@@ -29,14 +29,15 @@ struct Coef {
     __device__ inline float c(size_t e) const { return (float)p[2 * n2 + e]; }
 };
 
-// thread (i, j, k) of the interior, k = blockIdx.z
+// thread (i, j, k) of the interior, one z plane per tile
 template <typename CT>
 __global__ void __launch_bounds__(256) k_rb_coef(Grid g, RbGeom r, const double *a, const double *b, const double *c,
                                                  CT *out)
 {
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 2;
-    const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + 2;
-    const int k = (int)blockIdx.z + 2;
+    const uint3 tile = xcd_tile();
+    const int i = (int)(tile.x * blockDim.x + threadIdx.x) + 2;
+    const int j = (int)(tile.y * blockDim.y + threadIdx.y) + 2;
+    const int k = (int)tile.z + 2;
     if (i > g.nx - 1 || j > g.ny - 1) return;
     const size_t n2 = 2 * r.half;
     const int o = g.at(i, j, k);
@@ -49,9 +50,10 @@ __global__ void __launch_bounds__(256) k_rb_coef(Grid g, RbGeom r, const double 
 // finest-level right-hand side: f (split FP32) = r (natural FP64)
 __global__ void __launch_bounds__(256) k_rb_in(Grid g, RbGeom r, const double *src, float *dst)
 {
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 2;
-    const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + 2;
-    const int k = (int)blockIdx.z + 2;
+    const uint3 tile = xcd_tile();
+    const int i = (int)(tile.x * blockDim.x + threadIdx.x) + 2;
+    const int j = (int)(tile.y * blockDim.y + threadIdx.y) + 2;
+    const int k = (int)tile.z + 2;
     if (i > g.nx - 1 || j > g.ny - 1) return;
     dst[r.at(i, j, k)] = (float)src[g.at(i, j, k)];
 }
@@ -102,9 +104,10 @@ template <typename CT, bool ZERO, bool TOP>
 __global__ void __launch_bounds__(256) k_rb_colour(RbGeom r, float *__restrict__ u, const float *__restrict__ f,
                                                    Coef<CT> cf, float h2, int colour, float om, int zown)
 {
-    const int m = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + 2;
-    const int k = (int)blockIdx.z + 2;
+    const uint3 tile = xcd_tile();
+    const int m = (int)(tile.x * blockDim.x + threadIdx.x);
+    const int j = (int)(tile.y * blockDim.y + threadIdx.y) + 2;
+    const int k = (int)tile.z + 2;
     if (m >= r.mh || j > r.ny - 1) return;
     rb_cell<CT, ZERO>(r, u, f, cf, h2, colour, om, zown, m, j, k);
 }
@@ -128,21 +131,30 @@ __device__ inline void ld4(const _Float16 *p, float v[4])
     v[3] = (float)q[3];
 }
 
-// The colour pass with four consecutive cells of a row per thread (rows of
-// a multiple of 4 cells): one vector load per neighbour row instead of
-// four scalar loads, same arithmetic in the same order per cell.
-template <typename CT, bool ZERO, bool TOP>
-__global__ void __launch_bounds__(256) k_rb_colour4(RbGeom r, float *__restrict__ u, const float *__restrict__ f,
-                                                    Coef<CT> cf, float h2, int colour, float om, int zown)
+// Other-colour values around a group of four entries m0..m0+3 of row
+// (j, k): row4(dj, dk) the four entries of row j+dj, plane k+dk; one(dm)
+// the entry m0+dm of the group's own row (dm = -1 or 4).  NbGlobal reads
+// the iterate in HBM; NbRing reads a plane ring in LDS (the fused sweep).
+struct NbGlobal {
+    const float *o;  // other colour, row (j, k), entry m0
+    size_t rs, ps;   // row and plane strides of one colour
+    __device__ inline void row4(int dj, int dk, float v[4]) const
+    {
+        ld4(o + (ptrdiff_t)dj * (ptrdiff_t)rs + (ptrdiff_t)dk * (ptrdiff_t)ps, v);
+    }
+    __device__ inline float one(int dm) const { return o[dm]; }
+};
+
+// The arithmetic of one colour update of four consecutive entries: the
+// same operations in the same order as rb_cell, per cell.  own / oth: the
+// group's offsets in its own and in the other colour's half (coefficients
+// of the group's faces and of the other colour's west/south/bottom faces).
+template <typename CT, bool ZERO, class NB>
+__device__ inline void rb_group4(const RbGeom &r, const NB &nb, const float *f, const float *uold_own,
+                                 const Coef<CT> &cf, size_t own, size_t oth, float h2, int colour, float om, int zown,
+                                 int m0, int j, int k, float out[4])
 {
-    const int m0 = 4 * (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + 2;
-    const int k = (int)blockIdx.z + 2;
-    if (m0 >= r.mh || j > r.ny - 1) return;
     const int p = (colour + j + k) & 1;
-    const size_t row = ((size_t)(k - 2) * r.my + (j - 2)) * r.mh;
-    const size_t own = (size_t)colour * r.half + row + m0;
-    const size_t oth = (size_t)(1 - colour) * r.half + row + m0;  // other colour, same row, m0
     const size_t rs = (size_t)r.mh, ps = (size_t)r.mh * r.my;
     const bool hn = j < r.ny - 1, hs = j > 2, ht = k < r.nz - 1, hb = k > 2;
     float ae[4], bn[4], ct[4], aw[4] = {0.f, 0.f, 0.f, 0.f}, bs[4] = {0.f, 0.f, 0.f, 0.f},
@@ -168,28 +180,28 @@ __global__ void __launch_bounds__(256) k_rb_colour4(RbGeom r, float *__restrict_
     float ue[4] = {0.f, 0.f, 0.f, 0.f}, uw[4] = {0.f, 0.f, 0.f, 0.f}, un[4], us[4], ut[4], ub[4];
     if (!ZERO) {
         float t[4];
-        ld4(u + oth, t);
+        nb.row4(0, 0, t);
         if (p) {  // west m .. , east m+1 ..
             for (int q = 0; q < 4; q++) uw[q] = t[q];
             ue[0] = t[1];
             ue[1] = t[2];
             ue[2] = t[3];
-            ue[3] = m0 + 4 < r.mh ? u[oth + 4] : 0.f;
+            ue[3] = m0 + 4 < r.mh ? nb.one(4) : 0.f;
         } else {  // west m-1 .., east m ..
             for (int q = 0; q < 4; q++) ue[q] = t[q];
-            uw[0] = m0 > 0 ? u[oth - 1] : 0.f;
+            uw[0] = m0 > 0 ? nb.one(-1) : 0.f;
             uw[1] = t[0];
             uw[2] = t[1];
             uw[3] = t[2];
         }
-        if (hn) ld4(u + oth + rs, un);
-        if (hs) ld4(u + oth - rs, us);
-        if (ht) ld4(u + oth + ps, ut);
-        if (hb) ld4(u + oth - ps, ub);
+        if (hn) nb.row4(1, 0, un);
+        if (hs) nb.row4(-1, 0, us);
+        if (ht) nb.row4(0, 1, ut);
+        if (hb) nb.row4(0, -1, ub);
     }
-    float fv[4], out[4], uo[4] = {0.f, 0.f, 0.f, 0.f};
+    float fv[4], uo[4] = {0.f, 0.f, 0.f, 0.f};
     ld4(f + own, fv);
-    if (!ZERO && !zown && om != 1.f) ld4(u + own, uo);  // Gauss-Seidel needs no old value
+    if (!ZERO && !zown && om != 1.f) ld4(uold_own + own, uo);  // Gauss-Seidel needs no old value
     for (int q = 0; q < 4; q++) {
         const int i = 2 + p + 2 * (m0 + q);
         const bool hw = i > 2, he = i < r.nx - 1;
@@ -210,7 +222,103 @@ __global__ void __launch_bounds__(256) k_rb_colour4(RbGeom r, float *__restrict_
         const float gsv = (s - h2 * fv[q]) / d;
         out[q] = om * gsv + (1.f - om) * uo[q];
     }
+}
+
+// The colour pass with four consecutive cells of a row per thread (rows of
+// a multiple of 4 cells): one vector load per neighbour row instead of
+// four scalar loads, same arithmetic in the same order per cell.
+template <typename CT, bool ZERO, bool TOP>
+__global__ void __launch_bounds__(256) k_rb_colour4(RbGeom r, float *__restrict__ u, const float *__restrict__ f,
+                                                    Coef<CT> cf, float h2, int colour, float om, int zown)
+{
+    const uint3 tile = xcd_tile();
+    const int m0 = 4 * (int)(tile.x * blockDim.x + threadIdx.x);
+    const int j = (int)(tile.y * blockDim.y + threadIdx.y) + 2;
+    const int k = (int)tile.z + 2;
+    if (m0 >= r.mh || j > r.ny - 1) return;
+    const size_t row = ((size_t)(k - 2) * r.my + (j - 2)) * r.mh;
+    const size_t own = (size_t)colour * r.half + row + m0;
+    const size_t oth = (size_t)(1 - colour) * r.half + row + m0;  // other colour, same row, m0
+    const NbGlobal nb{u + oth, (size_t)r.mh, (size_t)r.mh * r.my};
+    float out[4];
+    rb_group4<CT, ZERO>(r, nb, f, u, cf, own, oth, h2, colour, om, zown, m0, j, k, out);
     *reinterpret_cast<float4 *>(u + own) = make_float4(out[0], out[1], out[2], out[3]);
+}
+
+// One red-black sweep of the finest level in one pass (colour ca, then the
+// other colour cb), out of place: X holds the iterate before the sweep, Y
+// receives it after.  A workgroup of TX x (TY + 2) threads owns TY whole
+// rows (TX groups of four entries each, TX = mh / 4) and a chunk of KC
+// planes, and marches up the planes: at step p every thread row updates
+// colour ca on plane p for one of the TY rows or the halo row on either
+// side (from X, exactly the first colour pass), keeps those values in a
+// three-plane LDS ring, and then the inner TY thread rows update colour cb
+// on plane p - 1 from the ring (exactly the second colour pass, which reads
+// the new ca values).  The ca planes just outside the chunk and the halo
+// rows are computed again by the neighbouring workgroups, which write them;
+// every cell is written by one workgroup.  Same results as the two colour
+// passes; the sweep reads f, X and the coefficients once and writes Y once.
+// ZERO: X is the zero iterate (the first sweep of a V-cycle: colour ca reads
+// no neighbour, colour cb no old own value).
+struct NbRing {
+    const float *ring;  // [3][TY + 2][mh]
+    int mh, rows;       // entries per row, rows per ring plane
+    int slot0, slot1, slotm;  // ring slots of planes k, k + 1, k - 1
+    int rr;             // ring row of the group's row
+    int m0;
+    __device__ inline const float *at(int dj, int dk) const
+    {
+        const int sl = dk == 0 ? slot0 : (dk > 0 ? slot1 : slotm);
+        return ring + ((size_t)sl * rows + (rr + dj)) * mh + m0;
+    }
+    __device__ inline void row4(int dj, int dk, float v[4]) const { ld4(at(dj, dk), v); }
+    __device__ inline float one(int dm) const { return at(0, 0)[dm]; }
+};
+
+constexpr int SWEEP_MAX_THREADS = 384;
+
+template <typename CT, bool ZERO>
+__global__ void __launch_bounds__(SWEEP_MAX_THREADS) k_rb_sweep4(RbGeom r, const float *__restrict__ X,
+                                                                 float *__restrict__ Y, const float *__restrict__ f,
+                                                                 Coef<CT> cf, float h2, int ca, float om, int kc)
+{
+    extern __shared__ __align__(16) float ring[];
+    const int rows = (int)blockDim.y, TY = rows - 2;
+    const uint3 tile = xcd_tile();
+    const int m0 = 4 * (int)threadIdx.x;  // blockDim.x * 4 == mh
+    const int j0 = (int)tile.y * TY + 2;
+    const int k0 = (int)tile.z * kc + 2, k1 = min(k0 + kc, r.nz);  // own planes k0 .. k1 - 1
+    const int cb = 1 - ca;
+    const size_t rs = (size_t)r.mh, ps = (size_t)r.mh * r.my;
+    for (int pl = k0 - 1; pl <= k1; pl++) {
+        // colour ca on plane pl: thread row rr takes grid row j0 - 1 + rr
+        const int rr = (int)threadIdx.y, ja = j0 - 1 + rr;
+        if (pl >= 2 && pl <= r.nz - 1 && ja >= 2 && ja <= r.ny - 1) {
+            const size_t row = ((size_t)(pl - 2) * r.my + (ja - 2)) * r.mh;
+            const size_t own = (size_t)ca * r.half + row + m0;
+            const size_t oth = (size_t)cb * r.half + row + m0;
+            const NbGlobal nb{X + oth, rs, ps};
+            float out[4];
+            rb_group4<CT, ZERO>(r, nb, f, X, cf, own, oth, h2, ca, om, ZERO ? 1 : 0, m0, ja, pl, out);
+            float *dst = ring + ((size_t)(pl % 3) * rows + rr) * r.mh + m0;
+            *reinterpret_cast<float4 *>(dst) = make_float4(out[0], out[1], out[2], out[3]);
+            if (rr >= 1 && rr <= TY && pl >= k0 && pl < k1)
+                *reinterpret_cast<float4 *>(Y + own) = make_float4(out[0], out[1], out[2], out[3]);
+        }
+        __syncthreads();
+        // colour cb on plane pl - 1, the inner thread rows, from the ring
+        const int q = pl - 1, j = ja;
+        if (rr >= 1 && rr <= TY && q >= k0 && q < k1 && j <= r.ny - 1) {
+            const size_t row = ((size_t)(q - 2) * r.my + (j - 2)) * r.mh;
+            const size_t own = (size_t)cb * r.half + row + m0;
+            const size_t oth = (size_t)ca * r.half + row + m0;
+            const NbRing nb{ring, r.mh, rows, q % 3, (q + 1) % 3, (q + 2) % 3, rr, m0};
+            float out[4];
+            rb_group4<CT, false>(r, nb, f, X, cf, own, oth, h2, cb, om, ZERO ? 1 : 0, m0, j, q, out);
+            *reinterpret_cast<float4 *>(Y + own) = make_float4(out[0], out[1], out[2], out[3]);
+        }
+        __syncthreads();
+    }
 }
 
 // residual of one fine cell (f - A u, boundary folding of the natural resid)
@@ -254,14 +362,15 @@ __device__ inline void rb_restrict_cell(const RbGeom &rf, const float *u, const 
         fcn[gc.at(I + 2, J + 2, K + 2)] = v;
 }
 
-// thread (I, J, K) of the coarse interior, K = blockIdx.z
+// thread (I, J, K) of the coarse interior, one z plane per tile
 template <typename CT>
 __global__ void __launch_bounds__(256) k_rb_resid_restrict(RbGeom rf, const float *u, const float *f, Coef<CT> cf,
                                                            float h2i, RbGeom rc, float *fcs, Grid gc, float *fcn)
 {
-    const int I = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    const int J = (int)(blockIdx.y * blockDim.y + threadIdx.y);
-    const int K = (int)blockIdx.z;
+    const uint3 tile = xcd_tile();
+    const int I = (int)(tile.x * blockDim.x + threadIdx.x);
+    const int J = (int)(tile.y * blockDim.y + threadIdx.y);
+    const int K = (int)tile.z;
     if (I >= gc.nx - 2 || J >= gc.ny - 2) return;
     rb_restrict_cell<CT>(rf, u, f, cf, h2i, rc, fcs, gc, fcn, I, J, K);
 }
@@ -282,9 +391,10 @@ __device__ inline void rb_prolong_entry(const RbGeom &rf, float *u, const RbGeom
 __global__ void __launch_bounds__(256) k_rb_prolong(RbGeom rf, float *u, RbGeom rc, const float *ucs, Grid gc,
                                                     const float *ucn)
 {
-    const int m = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + 2;
-    const int k = (int)blockIdx.z + 2;
+    const uint3 tile = xcd_tile();
+    const int m = (int)(tile.x * blockDim.x + threadIdx.x);
+    const int j = (int)(tile.y * blockDim.y + threadIdx.y) + 2;
+    const int k = (int)tile.z + 2;
     if (m >= rf.mh || j > rf.ny - 1) return;
     rb_prolong_entry(rf, u, rc, ucs, gc, ucn, m, j, k);
 }
@@ -294,7 +404,10 @@ __global__ void __launch_bounds__(256) k_rb_prolong(RbGeom rf, float *u, RbGeom 
 // 1..top as loops of TAIL_THREADS threads separated by barriers, with the
 // per-cell arithmetic of the kernels above (same results, ~4 launches per
 // level fewer).  Levels are indexed as in the context (0 = coarsest, the
-// natural-layout direct solve).
+// natural-layout direct solve).  LDS form: the levels' u, f and face
+// coefficients and the coarsest level's f and u are staged in LDS (loaded
+// once per launch: the top level's f and every coefficient; the top level's
+// u written back at the end); the global form works in place.
 constexpr int TAIL_THREADS = 1024;
 constexpr int TAIL_MAX_LEVELS = 8;
 constexpr size_t TAIL_MAX_CELLS = 4096;  // largest level taken into the tail
@@ -313,56 +426,171 @@ struct RbTail {
     const double *M;
 };
 
-template <typename CT, bool ZERO>
-__device__ inline void tail_pass(const RbTail<CT> &t, int l, int colour, int zown)
+// LDS bytes of the staged tail: per level u, f (FP32) and 3 coefficient
+// planes, then the coarsest level's f and u, 16-byte aligned pieces
+template <typename CT>
+__host__ __device__ inline size_t tail_lds_bytes(const RbTail<CT> &t)
 {
-    const RbGeom &r = t.r[l];
-    const int n = (int)r.half;
-    for (int e = (int)threadIdx.x; e < n; e += TAIL_THREADS) {
-        const int m = e % r.mh, q = e / r.mh;
-        rb_cell<CT, ZERO>(r, t.u[l], t.f[l], t.cf[l], t.h2[l], colour, t.om, zown, m, q % r.my + 2, q / r.my + 2);
+    size_t b = 0;
+    for (int l = 1; l <= t.top; l++) {
+        const size_t n2 = 2 * t.r[l].half;
+        b += 2 * ((n2 * sizeof(float) + 15) & ~(size_t)15) + ((3 * n2 * sizeof(CT) + 15) & ~(size_t)15);
     }
+    const size_t n0 = (size_t)t.g0.nx * t.g0.ny * t.g0.nz;
+    return b + 2 * ((n0 * sizeof(float) + 15) & ~(size_t)15);
+}
+
+// where level l's arrays live for one form of the tail
+template <typename CT, bool LDS>
+struct TailMem {
+    const RbTail<CT> &t;
+    unsigned char *lds;
+    __device__ inline size_t base(int l) const  // LDS offset of level l's u
+    {
+        size_t b = 0;
+        for (int q = 1; q < l; q++) {
+            const size_t n2 = 2 * t.r[q].half;
+            b += 2 * ((n2 * sizeof(float) + 15) & ~(size_t)15) + ((3 * n2 * sizeof(CT) + 15) & ~(size_t)15);
+        }
+        return b;
+    }
+    __device__ inline size_t fl(int l) const { return (2 * t.r[l].half * sizeof(float) + 15) & ~(size_t)15; }
+    __device__ inline float *u(int l) const { return LDS ? (float *)(lds + base(l)) : t.u[l]; }
+    __device__ inline float *f(int l) const { return LDS ? (float *)(lds + base(l) + fl(l)) : t.f[l]; }
+    __device__ inline Coef<CT> cf(int l) const
+    {
+        return LDS ? Coef<CT>{(const CT *)(lds + base(l) + 2 * fl(l)), 2 * t.r[l].half} : t.cf[l];
+    }
+    __device__ inline float *f0() const { return LDS ? (float *)(lds + base(t.top + 1)) : t.f0; }
+    __device__ inline float *u0() const
+    {
+        const size_t n0 = (size_t)t.g0.nx * t.g0.ny * t.g0.nz;
+        return LDS ? (float *)(lds + base(t.top + 1) + ((n0 * sizeof(float) + 15) & ~(size_t)15)) : t.u0;
+    }
+};
+
+// this thread's entries (m, j, k) of one colour of a tail level, decoded
+// once per visit of the level instead of once per pass
+constexpr int TAIL_CPT = (int)((TAIL_MAX_CELLS / 2 + TAIL_THREADS - 1) / TAIL_THREADS);
+struct TailCells {
+    int n;
+    int m[TAIL_CPT], j[TAIL_CPT], k[TAIL_CPT];
+};
+
+__device__ inline TailCells tail_cells(const RbGeom &r)
+{
+    TailCells c;
+    c.n = 0;
+#pragma unroll
+    for (int s = 0; s < TAIL_CPT; s++) {
+        const int e = (int)threadIdx.x + s * TAIL_THREADS;
+        c.m[s] = c.j[s] = c.k[s] = 0;
+        if (e < (int)r.half) {
+            const int q = e / r.mh;
+            c.m[s] = e - q * r.mh;
+            c.j[s] = q % r.my + 2;
+            c.k[s] = q / r.my + 2;
+            c.n = s + 1;
+        }
+    }
+    return c;
+}
+
+template <typename CT, bool LDS, bool ZERO>
+__device__ inline void tail_pass(const TailMem<CT, LDS> &mem, const TailCells &cells, int l, int colour, int zown)
+{
+    const RbTail<CT> &t = mem.t;
+    const RbGeom &r = t.r[l];
+    float *u = mem.u(l);
+    const float *f = mem.f(l);
+    const Coef<CT> cf = mem.cf(l);
+#pragma unroll
+    for (int s = 0; s < TAIL_CPT; s++)
+        if (s < cells.n) rb_cell<CT, ZERO>(r, u, f, cf, t.h2[l], colour, t.om, zown, cells.m[s], cells.j[s], cells.k[s]);
     __syncthreads();
+}
+
+template <typename CT, bool LDS>
+__device__ inline void tail_body(const TailMem<CT, LDS> &mem, double *y)
+{
+    const RbTail<CT> &t = mem.t;
+    for (int l = t.top; l >= 1; l--) {
+        const TailCells cells = tail_cells(t.r[l]);
+        for (int q = 0; q < t.nu1[l]; q++) {
+            if (q == 0)
+                tail_pass<CT, LDS, true>(mem, cells, l, 0, 1);
+            else
+                tail_pass<CT, LDS, false>(mem, cells, l, 0, 0);
+            tail_pass<CT, LDS, false>(mem, cells, l, 1, q == 0);
+        }
+        const RbGeom &rf = t.r[l];
+        const bool csplit = l - 1 >= 1;
+        const float *u = mem.u(l), *f = mem.f(l);
+        const Coef<CT> cf = mem.cf(l);
+        float *fc = csplit ? mem.f(l - 1) : mem.f0();
+        const int cx = rf.mh, cy = rf.my / 2, n = cx * cy * (rf.mz / 2);  // coarse interior
+        for (int e = (int)threadIdx.x; e < n; e += TAIL_THREADS) {
+            const int I = e % cx, q = e / cx;
+            rb_restrict_cell<CT>(rf, u, f, cf, t.h2i[l], t.r[csplit ? l - 1 : l], csplit ? fc : nullptr, t.g0,
+                                 csplit ? nullptr : fc, I, q % cy, q / cy);
+        }
+        __syncthreads();
+    }
+    coarse_subst<float>(t.g0, t.M, mem.f0(), mem.u0(), y, (int)threadIdx.x, TAIL_THREADS);
+    __syncthreads();
+    for (int l = 1; l <= t.top; l++) {
+        const RbGeom &rf = t.r[l];
+        const bool csplit = l - 1 >= 1;
+        float *u = mem.u(l);
+        const float *uc = csplit ? mem.u(l - 1) : mem.u0();
+        const int n = (int)rf.half;
+        for (int e = (int)threadIdx.x; e < n; e += TAIL_THREADS) {
+            const int m = e % rf.mh, q = e / rf.mh;
+            rb_prolong_entry(rf, u, csplit ? t.r[l - 1] : rf, csplit ? uc : nullptr, t.g0, csplit ? nullptr : uc, m,
+                             q % rf.my + 2, q / rf.my + 2);
+        }
+        __syncthreads();
+        const TailCells cells = tail_cells(rf);
+        for (int q = 0; q < t.nu2[l]; q++) {
+            tail_pass<CT, LDS, false>(mem, cells, l, 1, 0);
+            tail_pass<CT, LDS, false>(mem, cells, l, 0, 0);
+        }
+    }
 }
 
 template <typename CT>
 __global__ void __launch_bounds__(TAIL_THREADS) k_rb_tail(RbTail<CT> t)
 {
     __shared__ double y[512];
-    for (int l = t.top; l >= 1; l--) {
-        for (int q = 0; q < t.nu1[l]; q++) {
-            if (q == 0)
-                tail_pass<CT, true>(t, l, 0, 1);
-            else
-                tail_pass<CT, false>(t, l, 0, 0);
-            tail_pass<CT, false>(t, l, 1, q == 0);
-        }
-        const RbGeom &rf = t.r[l];
-        const bool csplit = l - 1 >= 1;
-        const int cx = rf.mh, cy = rf.my / 2, n = cx * cy * (rf.mz / 2);  // coarse interior
-        for (int e = (int)threadIdx.x; e < n; e += TAIL_THREADS) {
-            const int I = e % cx, q = e / cx;
-            rb_restrict_cell<CT>(rf, t.u[l], t.f[l], t.cf[l], t.h2i[l], t.r[csplit ? l - 1 : l],
-                                 csplit ? t.f[l - 1] : nullptr, t.g0, csplit ? nullptr : t.f0, I, q % cy, q / cy);
-        }
-        __syncthreads();
+    const TailMem<CT, false> mem{t, nullptr};
+    tail_body<CT, false>(mem, y);
+}
+
+template <typename CT>
+__global__ void __launch_bounds__(TAIL_THREADS) k_rb_tail_lds(RbTail<CT> t)
+{
+    extern __shared__ __align__(16) unsigned char tail_lds[];
+    __shared__ double y[512];
+    const TailMem<CT, true> mem{t, tail_lds};
+    // stage the top level's right-hand side and every level's coefficients
+    {
+        const int l = t.top;
+        const int n2 = (int)(2 * t.r[l].half);
+        float *fd = mem.f(l);
+        for (int e = (int)threadIdx.x; e < n2; e += TAIL_THREADS) fd[e] = t.f[l][e];
     }
-    coarse_subst<float>(t.g0, t.M, t.f0, t.u0, y, (int)threadIdx.x, TAIL_THREADS);
-    __syncthreads();
     for (int l = 1; l <= t.top; l++) {
-        const RbGeom &rf = t.r[l];
-        const bool csplit = l - 1 >= 1;
-        const int n = (int)rf.half;
-        for (int e = (int)threadIdx.x; e < n; e += TAIL_THREADS) {
-            const int m = e % rf.mh, q = e / rf.mh;
-            rb_prolong_entry(rf, t.u[l], csplit ? t.r[l - 1] : rf, csplit ? t.u[l - 1] : nullptr, t.g0,
-                             csplit ? nullptr : t.u0, m, q % rf.my + 2, q / rf.my + 2);
-        }
-        __syncthreads();
-        for (int q = 0; q < t.nu2[l]; q++) {
-            tail_pass<CT, false>(t, l, 1, 0);
-            tail_pass<CT, false>(t, l, 0, 0);
-        }
+        const int n = (int)(6 * t.r[l].half);
+        CT *cd = (CT *)mem.cf(l).p;
+        for (int e = (int)threadIdx.x; e < n; e += TAIL_THREADS) cd[e] = t.cf[l].p[e];
+    }
+    __syncthreads();
+    tail_body<CT, true>(mem, y);
+    {
+        const int l = t.top;
+        const int n2 = (int)(2 * t.r[l].half);
+        const float *us = mem.u(l);
+        for (int e = (int)threadIdx.x; e < n2; e += TAIL_THREADS) t.u[l][e] = us[e];
     }
 }
 
@@ -462,7 +690,65 @@ int launch_tail(Ctx &c, int lt)
     t.f0 = c.lev[0].f32;
     t.u0 = c.lev[0].u32;
     t.M = coarse_factor();
-    hipLaunchKernelGGL(k_rb_tail<CT>, dim3(1), dim3(TAIL_THREADS), 0, c.stream, t);
+    // LDS form when the device's per-workgroup LDS holds the staged levels
+    // next to the 4 KB substitution buffer
+    static int lds_max = -1;
+    if (lds_max < 0) {
+        int dev = 0;
+        DG_HIP(hipGetDevice(&dev));
+        DG_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    }
+    const size_t lds = tail_lds_bytes(t);
+    if (c.rb_tail == 2 && lds + 512 * sizeof(double) <= (size_t)lds_max) {
+        if (lds > 48 * 1024)
+            DG_HIP(hipFuncSetAttribute((const void *)k_rb_tail_lds<CT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds));
+        hipLaunchKernelGGL(k_rb_tail_lds<CT>, dim3(1), dim3(TAIL_THREADS), lds, c.stream, t);
+    } else {
+        hipLaunchKernelGGL(k_rb_tail<CT>, dim3(1), dim3(TAIL_THREADS), 0, c.stream, t);
+    }
+    DG_CHECK_LAUNCH();
+    return 0;
+}
+
+// planes per workgroup of the fused sweep
+constexpr int SWEEP_KC = 4;
+
+// whether level l runs fused sweeps: the finest level, rows of 8..64 groups
+// of four entries (one workgroup spans whole rows)
+bool sweep_fused_ok(const Ctx &c, int l)
+{
+    if (c.fused_smoother == 0 || l != c.ng - 1) return false;
+    const RbGeom r = rb_geom(c.lev[l].g);
+    const int tx = r.mh / 4;
+    return r.mh % 4 == 0 && (tx == 8 || tx == 16 || tx == 32 || tx == 64) && c.lev[l].rb_w &&
+           tx * (256 / tx + 2) <= SWEEP_MAX_THREADS;
+}
+
+// one fused red-black sweep of level l: colour ca then the other, X -> Y
+template <typename CT>
+int sweep_fused(Ctx &c, int l, const float *X, float *Y, int ca, bool zero)
+{
+    Level &L = c.lev[l];
+    const RbGeom r = rb_geom(L.g);
+    const Coef<CT> cf{(const CT *)L.rb_coef, 2 * r.half};
+    const float h2 = (float)(1.0 / lev_h2i(c, L.g));
+    const float om = (float)c.mg_omega;
+    const int tx = r.mh / 4, ty = 256 / tx;  // ty owned rows, ty + 2 thread rows
+    const dim3 blk((unsigned)tx, (unsigned)(ty + 2));
+    const dim3 grd(1, (unsigned)((r.my + ty - 1) / ty), (unsigned)((r.mz + SWEEP_KC - 1) / SWEEP_KC));
+    const size_t lds = 3 * (size_t)(ty + 2) * r.mh * sizeof(float);
+    // algorithmic bytes: f, Y (and X unless zero) 4 B per cell, coefficients
+    const double cells = 2.0 * (double)r.half;
+    const double bytes = cells * (zero ? 8.0 : 12.0) + 3.0 * cells * (double)sizeof(CT);
+    prof_begin(PK_RB_COLOUR, c.stream);
+    if (zero)
+        hipLaunchKernelGGL((k_rb_sweep4<CT, true>), grd, blk, lds, c.stream, r, X, Y, L.rb_f, cf, h2, ca, om,
+                           SWEEP_KC);
+    else
+        hipLaunchKernelGGL((k_rb_sweep4<CT, false>), grd, blk, lds, c.stream, r, X, Y, L.rb_f, cf, h2, ca, om,
+                           SWEEP_KC);
+    prof_end(PK_RB_COLOUR, c.stream, bytes);
     DG_CHECK_LAUNCH();
     return 0;
 }
@@ -470,14 +756,28 @@ int launch_tail(Ctx &c, int lt)
 template <typename CT>
 int vcycle_rb(Ctx &c, int l)
 {
-    if (l == tail_level(c)) return launch_tail<CT>(c, l);
+    if (c.rb_tail > 0 && l == tail_level(c)) return launch_tail<CT>(c, l);
     Level &L = c.lev[l];
     const bool top = l == c.ng - 1;
     const int nu1 = (top || c.mg_nuc <= 0) ? c.mg_nu1 : c.mg_nuc;
     const int nu2 = (top || c.mg_nuc <= 0) ? c.mg_nu2 : c.mg_nuc;
-    for (int q = 0; q < nu1; q++) {
-        if (colour_pass<CT>(c, l, 0, q == 0, q == 0)) return -1;
-        if (colour_pass<CT>(c, l, 1, false, q == 0)) return -1;
+    // fused sweeps alternate between rb_u and rb_w; the first one writes the
+    // buffer that makes the last one land in rb_u
+    const bool fused = sweep_fused_ok(c, l);
+    float *cur = L.rb_u;
+    auto other = [&](float *b) { return b == L.rb_u ? L.rb_w : L.rb_u; };
+    if (fused) {
+        float *w1 = ((nu1 + nu2) & 1) ? L.rb_u : L.rb_w;
+        for (int q = 0; q < nu1; q++) {
+            float *y = q == 0 ? w1 : other(cur);
+            if (sweep_fused<CT>(c, l, cur, y, 0, q == 0)) return -1;
+            cur = y;
+        }
+    } else {
+        for (int q = 0; q < nu1; q++) {
+            if (colour_pass<CT>(c, l, 0, q == 0, q == 0)) return -1;
+            if (colour_pass<CT>(c, l, 1, false, q == 0)) return -1;
+        }
     }
     const RbGeom rf = rb_geom(L.g);
     const Coef<CT> cf{(const CT *)L.rb_coef, 2 * rf.half};
@@ -485,7 +785,7 @@ int vcycle_rb(Ctx &c, int l)
     const bool csplit = l - 1 >= 1;
     const RbGeom rc = rb_geom(C.g);
     const dim3 bc = row_block(C.g.nx - 2);
-    hipLaunchKernelGGL(k_rb_resid_restrict<CT>, row_grid(bc, C.g.nx - 2, C.g.ny - 2, C.g.nz - 2), bc, 0, c.stream, rf, L.rb_u, L.rb_f, cf,
+    hipLaunchKernelGGL(k_rb_resid_restrict<CT>, row_grid(bc, C.g.nx - 2, C.g.ny - 2, C.g.nz - 2), bc, 0, c.stream, rf, cur, L.rb_f, cf,
                        (float)lev_h2i(c, L.g), rc, csplit ? C.rb_f : nullptr, C.g, csplit ? nullptr : C.f32);
     DG_CHECK_LAUNCH();
     if (csplit) {
@@ -494,12 +794,24 @@ int vcycle_rb(Ctx &c, int l)
         if (coarse_solve_f32(c)) return -1;
     }
     const dim3 bf = row_block(rf.mh);
-    hipLaunchKernelGGL(k_rb_prolong, row_grid(bf, rf.mh, rf.my, rf.mz), bf, 0, c.stream, rf, L.rb_u, rc,
+    hipLaunchKernelGGL(k_rb_prolong, row_grid(bf, rf.mh, rf.my, rf.mz), bf, 0, c.stream, rf, cur, rc,
                        csplit ? C.rb_u : nullptr, C.g, csplit ? nullptr : C.u32);
     DG_CHECK_LAUNCH();
-    for (int q = 0; q < nu2; q++) {
-        if (colour_pass<CT>(c, l, 1, false, 0)) return -1;
-        if (colour_pass<CT>(c, l, 0, false, 0)) return -1;
+    if (fused) {
+        for (int q = 0; q < nu2; q++) {
+            float *y = other(cur);
+            if (sweep_fused<CT>(c, l, cur, y, 1, false)) return -1;
+            cur = y;
+        }
+        if (cur != L.rb_u) {
+            set_error("split V-cycle: fused sweep buffer bookkeeping");
+            return -1;
+        }
+    } else {
+        for (int q = 0; q < nu2; q++) {
+            if (colour_pass<CT>(c, l, 1, false, 0)) return -1;
+            if (colour_pass<CT>(c, l, 0, false, 0)) return -1;
+        }
     }
     return 0;
 }
@@ -532,6 +844,10 @@ int rb_prepare(Ctx &c)
             DG_HIP(hipMalloc(&L.rb_f, n2 * sizeof(float)));
             DG_HIP(hipMemset(L.rb_u, 0, n2 * sizeof(float)));
             DG_HIP(hipMemset(L.rb_f, 0, n2 * sizeof(float)));
+        }
+        if (l == c.ng - 1 && c.fused_smoother != 0 && !L.rb_w) {
+            DG_HIP(hipMalloc(&L.rb_w, n2 * sizeof(float)));
+            DG_HIP(hipMemset(L.rb_w, 0, n2 * sizeof(float)));
         }
         if (L.rb_coef16 != want16) {
             if (L.rb_coef) (void)hipFree(L.rb_coef);

@@ -443,7 +443,7 @@ static int swp(Ctx &c, const double *us, double *cc, double *vof1, double *vof2,
     const double tiny = DG_F4(1.e-30);
     const int sd = d == 1 ? 1 : (d == 2 ? g.sy : g.sz);
     const int guard = c.vof_guard;
-    if (launch_interior(g, c.stream, [=] __device__(int i, int j, int k) {
+    if (launch_interior_rr(g, c.stream, [=] __device__(int i, int j, int k) {
             const int o = g.at(i, j, k);
             const double a1 = us[o] * tauh;
             const double a2 = us[o + sd] * tauh;
@@ -1074,7 +1074,7 @@ int op_smooth2(Ctx &c)
     const double h = s.h;
     const double x0 = (double)(g.nx - 2) * h / 4.0;
     const double y0 = s.y0, z0 = s.z0, r2 = s.radius * s.radius;
-    if (launch_interior(g, c.stream, [=] __device__(int i, int j, int k) {
+    if (launch_interior_rr(g, c.stream, [=] __device__(int i, int j, int k) {
             const double xa = ((double)(i - 1) - 1.5) * h, xb = ((double)i - 1.5) * h;
             const double ya = ((double)(j - 1) - 1.5) * h, yb = ((double)j - 1.5) * h;
             const double za = ((double)(k - 1) - 1.5) * h, zb = ((double)k - 1.5) * h;

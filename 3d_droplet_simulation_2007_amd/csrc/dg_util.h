@@ -13,6 +13,20 @@ __global__ void __launch_bounds__(256) k_interior(Grid g, F f)
     f(i, j, k);
 }
 
+// the same in the dispatcher's own order (tiles dealt round-robin over the
+// XCDs), for kernels whose work concentrates in a few tiles (the interface
+// cells of PLIC and of the initial condition's sub-sampling): contiguous
+// runs would leave most XCDs idle
+template <class F>
+__global__ void __launch_bounds__(256) k_interior_rr(Grid g, F f)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 2;
+    const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + 2;
+    const int k = (int)blockIdx.z + 2;
+    if (i > g.nx - 1 || j > g.ny - 1) return;
+    f(i, j, k);
+}
+
 // f(a,b) over a 2-D index range [0,n1) x [0,n2)
 template <class F>
 __global__ void __launch_bounds__(256) k_face(int n1, int n2, F f)
@@ -35,6 +49,15 @@ inline int launch_interior(const Grid &g, hipStream_t s, F f)
 {
     if (g.nx < 3 || g.ny < 3 || g.nz < 3) return 0;
     hipLaunchKernelGGL(k_interior<F>, interior_grid(g), interior_block(), 0, s, g, f);
+    DG_CHECK_LAUNCH();
+    return 0;
+}
+
+template <class F>
+inline int launch_interior_rr(const Grid &g, hipStream_t s, F f)
+{
+    if (g.nx < 3 || g.ny < 3 || g.nz < 3) return 0;
+    hipLaunchKernelGGL(k_interior_rr<F>, interior_grid(g), interior_block(), 0, s, g, f);
     DG_CHECK_LAUNCH();
     return 0;
 }

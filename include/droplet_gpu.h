@@ -101,17 +101,23 @@ int dg_set_ordered_sums(const int *on);
  * the run turns to NaN (swp main.f:848-1000).  0 (default): the reference's
  * arithmetic. */
 int dg_set_vof_guard(const int *on);
-/* Tolerance-mode execution switches.  Bit-identical either way: fused = 1
- * one out-of-place pass per red-black sweep, 0 (default) two in-place
- * colour passes; graphs = 1 (default) each PCG iteration replayed as a
+/* Tolerance-mode execution switches.  Bit-identical either way: fused
+ * selects one out-of-place pass per red-black sweep instead of two in-place
+ * colour passes: 0 (default) nowhere, 1 on the natural layout
+ * (Gauss-Seidel, equal sweep counts) and the split finest level, 2 on the
+ * split finest level only; graphs = 1 (default) each PCG iteration replayed as a
  * captured HIP graph, 0 launched kernel by kernel.  split = 1 (default):
  * mode 2's FP32 V-cycle keeps each level above the coarsest in red-black
  * split storage with FP16 face coefficients when the densities allow
  * (different rounding of the preconditioner, same tolerance reached);
  * 2: split storage, FP32 coefficients (bit-identical to 0); 3: split, FP16
- * coefficients always; 0: natural layout, FP32 coefficients.  Applies to
- * the resident state and to later dg_init / dg_poisson_solve calls. */
-int dg_set_tuning(const int *fused, const int *graphs, const int *split);
+ * coefficients always; 0: natural layout, FP32 coefficients.  tail (split
+ * storage only; bit-identical either way): the levels of at most 4096 cells
+ * and the coarsest solve run as 0 one launch per operation, 1 one
+ * single-workgroup kernel in global memory, 2 (default) the same kernel on
+ * LDS copies of those levels when they fit.  Applies to the resident state
+ * and to later dg_init / dg_poisson_solve calls. */
+int dg_set_tuning(const int *fused, const int *graphs, const int *split, const int *tail);
 
 /* ---- host-pointer kernels with the reference's argument lists --------- */
 /* Each uploads its arrays, runs on the GPU and downloads the outputs. */

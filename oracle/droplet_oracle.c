@@ -927,6 +927,10 @@ void or_abcei(double *a, double *b, double *c, double *ei, const double *cc, dou
 /* mglin, main.f:3645-3763: full-multigrid schedule, fixed cycle counts */
 void or_mglin(or_state *st)
 {
+    if (!st->ref_grid) {  /* memoryaloc's STOP (main.f:274-322) */
+        fprintf(stderr, "STOP: nn .ne. 2**ng + 2\n");
+        exit(1);
+    }
     or_level *lv = st->lev;
     int ng = st->ng;
     /* restrict rhs and coefficients down to the coarsest level */
@@ -1352,15 +1356,31 @@ or_state *or_create(const char *path, char *errbuf, int errlen)
         return NULL;
     }
     st->idrop = 1;
-    /* memoryaloc */
+    /* memoryaloc: levels n -> n/2+1 while the smallest extent exceeds 4.
+     * The reference STOPs unless nn = 2**ng + 2; that check moves to mglin
+     * (or_mglin), its only user, so that m*2**k + 2 grids serve the
+     * tolerance-mode solver: their halving stops where an interior extent
+     * turns odd. */
     int nn = st->nx < st->ny ? st->nx : st->ny;
     if (st->nz < nn) nn = st->nz;
-    int j = nn - 2, ng = 1;
-    while (j > 2) { j = j / 2; ng++; }
-    if ((1 << ng) + 2 != nn || ng > OR_MAXLEV) {
-        if (errbuf) snprintf(errbuf, errlen, "STOP: nn .ne. 2**ng + 2 (%d)", nn);
-        free(st);
-        return NULL;
+    int ng = 0;
+    {
+        int lx = st->nx, ly = st->ny, lz = st->nz;
+        for (;;) {
+            if (ng >= OR_MAXLEV) {
+                if (errbuf) snprintf(errbuf, errlen, "too many multigrid levels (%d)", nn);
+                free(st);
+                return NULL;
+            }
+            ng++;
+            int m = lx < ly ? lx : ly;
+            if (lz < m) m = lz;
+            if (m <= 4 || ((lx - 2) & 1) || ((ly - 2) & 1) || ((lz - 2) & 1)) break;
+            lx = lx / 2 + 1; ly = ly / 2 + 1; lz = lz / 2 + 1;
+        }
+        int j = nn - 2, ngr = 1;
+        while (j > 2) { j = j / 2; ngr++; }
+        st->ref_grid = (1 << ngr) + 2 == nn && ngr == ng;
     }
     st->ng = ng;
     int lx = st->nx, ly = st->ny, lz = st->nz;

@@ -84,6 +84,10 @@ typedef struct or_state {
      * the finest (0: mg_nu1 / mg_nu2 there too) */
     double mg_omega;
     int mg_nuc;
+    /* 1: the grid has the reference's form nn = 2**ng + 2 (memoryaloc's
+     * check, main.f:274-322), so the FMG schedule (mglin) may run; 0: an
+     * m*2**k + 2 grid, tolerance mode only */
+    int ref_grid;
 } or_state;
 
 /* field ids shared with the GPU library's dg_field ids */

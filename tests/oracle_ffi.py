@@ -164,7 +164,7 @@ def _state_fields():
     f += [(n, p) for n in ("u", "v", "w", "s11", "s22", "s33", "s12", "s13", "s23", "rhon", "mun", "mut", "ktur",
                            "eps", "cmu")]
     f += [("mg_mode", i), ("mg_nu1", i), ("mg_nu2", i), ("mg_maxcycles", i), ("mg_guess", i), ("mg_tol", d),
-          ("mg_cycles_last", i), ("mg_relres_last", d), ("mg_bnorm_last", d), ("p_prev", p), ("cfl_last", d), ("mg_outer", i), ("mg_prolong", i), ("vof_guard", i), ("mg_omega", d), ("mg_nuc", i)]
+          ("mg_cycles_last", i), ("mg_relres_last", d), ("mg_bnorm_last", d), ("p_prev", p), ("cfl_last", d), ("mg_outer", i), ("mg_prolong", i), ("vof_guard", i), ("mg_omega", d), ("mg_nuc", i), ("ref_grid", i)]
     return f
 
 

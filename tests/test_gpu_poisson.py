@@ -48,17 +48,21 @@ def sphere_system(n, seed):
 
 @pytest.mark.parametrize("n", [34, 66])
 def test_tuning_switches_are_bit_identical(dg, n):
-    """Fused or two-pass sweeps, graph replay or not, red-black split or
-    natural storage: the same FP32 arithmetic in the same order (the
-    standalone solve keeps FP32 coefficients), so the same bits."""
+    """One-pass (fused) or two-pass sweeps on either layout, graph replay or
+    not, red-black split or natural storage, the small levels as one
+    workgroup (LDS or global memory) or one launch per operation: the same
+    FP32 arithmetic in the same order (the standalone solve keeps FP32
+    coefficients), so the same bits."""
     print("case tuning", n, flush=True)
     rhs, a, b, c = sphere_system(n, 1)
     outs = []
-    combos = [(False, True, True), (False, False, True), (False, True, False), (True, True, False),
-              (True, False, False), (False, False, False)]
+    combos = [("split", True, True, "lds"), (False, True, True, "lds"), ("split", False, True, "lds"),
+              (False, False, True, "lds"), ("split", True, True, "global"), (False, True, True, None),
+              ("split", False, True, None), (False, True, False, "lds"), (True, True, False, "lds"),
+              (True, False, False, "lds"), (False, False, False, "lds")]
     try:
-        for fused, graphs, split in combos:
-            dg.set_tuning(fused=fused, graphs=graphs, split=split)
+        for fused, graphs, split, tail in combos:
+            dg.set_tuning(fused=fused, graphs=graphs, split=split, tail=tail)
             p, it, rel = dg.poisson_solve(rhs, a, b, c, mode=2, tol=1e-9)
             outs.append((p, it, rel))
     finally:
@@ -105,7 +109,8 @@ def test_matches_oracle_solve(dg, oracle):
 def test_smoother_options_match_oracle(dg, oracle, omega, nuc):
     """SOR weight and coarse-level sweep count: the FP64 solve follows the
     CPU statement of the same V-cycle (iterations within one, solutions to
-    1e-8), and the split FP32 V-cycle stays bit-identical to the natural one."""
+    1e-8), and the split FP32 V-cycle, in two colour passes or one fused
+    pass per sweep, stays bit-identical to the natural one."""
     print("case smoother", omega, nuc, flush=True)
     rhs, a, b, c = sphere_system(34, 5)
     o = Oracle(inputdata_text(nx=34, ny=34, nz=34))
@@ -119,8 +124,8 @@ def test_smoother_options_match_oracle(dg, oracle, omega, nuc):
         dg.set_smoother(omega, nuc)
         pg, it, rel = dg.poisson_solve(rhs, a, b, c, mode=1, tol=1e-10)
         outs = []
-        for split in (True, False):
-            dg.set_tuning(split=split)
+        for split, fused in ((True, False), (True, "split"), (False, False)):
+            dg.set_tuning(split=split, fused=fused)
             outs.append(dg.poisson_solve(rhs, a, b, c, mode=2, tol=1e-9))
     finally:
         dg.set_smoother()
@@ -130,4 +135,5 @@ def test_smoother_options_match_oracle(dg, oracle, omega, nuc):
     print("iters gpu", it, "oracle", o.st.mg_cycles_last, "err", err, flush=True)
     assert err < 1e-8, err
     assert abs(it - o.st.mg_cycles_last) <= 1
-    assert outs[0][1] == outs[1][1] and np.array_equal(outs[0][0], outs[1][0])
+    for o2 in outs[1:]:
+        assert outs[0][1] == o2[1] and np.array_equal(outs[0][0], o2[0])

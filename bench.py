@@ -42,11 +42,12 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the dominant kernel family at 258^3, from
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this script scaled by
 # the same counters on a plain 256 MiB copy (fetch x1.9998, write x1.0):
-# 233.4 MB per SOR pass (7 of the 8 finest passes of a V(2,2) cycle) and
-# 167.7 MB per zero-iterate pass, averaged as bench.py times them; keyed by
-# kernel and grid extent, re-measured whenever the kernel changes
+# 234.4 MB per non-zero-iterate pass (7 of the 8 finest passes of a V(2,2)
+# cycle) and 167.7 MB per zero-iterate pass, averaged as bench.py times
+# them; keyed by kernel and grid extent, re-measured whenever the kernel
+# changes
 ROOF_KERNEL = "k_rb_colour4<_Float16,*,true>"
-PMC_TRAFFIC = {(ROOF_KERNEL, 258): (7 * 233.4e6 + 167.7e6) / 8}
+PMC_TRAFFIC = {(ROOF_KERNEL, 258): (7 * 234.4e6 + 167.7e6) / 8}
 
 
 def splash_inputdata(dg, n):

@@ -824,6 +824,23 @@ __global__ void k_pcg_scalars(double *ds)
     ds[S_FIRST] = 0.0;
 }
 
+// the previous-step guess is dropped for the zero guess when its residual
+// is no smaller than b (r.r >= b.b): x = 0, r = b
+__global__ void __launch_bounds__(256) k_guess_check(Grid g, double *x, double *r, const double *bvec,
+                                                     const double *ds)
+{
+    if (ds[S_RR] < ds[S_BB]) return;
+    DG_INTERIOR_IJK(g);
+    const int o = g.at(i, j, k);
+    x[o] = 0.0;
+    r[o] = bvec[o];
+}
+
+__global__ void k_guess_check_scalar(double *ds)
+{
+    if (ds[S_RR] >= ds[S_BB]) ds[S_RR] = ds[S_BB];
+}
+
 // r = b - A x with folded ghosts; partial sums of r.r and b.b
 __global__ void __launch_bounds__(256) k_init_resid(Grid g, double *r, const double *x, const double *bvec,
                                                     const double *a, const double *b, const double *c, double h2i,
@@ -1044,7 +1061,9 @@ static int pcg_solve(Ctx &c)
     const dim3 ig = interior_grid(g), ib = interior_block();
     const int nblk = (int)(ig.x * ig.y * ig.z);
     double *partial = c.red_partial;
-    if (c.mg_guess >= 1) {
+    // guess 1: the previous pressure (residual-checked below); guess 0 and
+    // the learned guess 2: zero (the network predicts the pressure itself)
+    if (c.mg_guess == 1) {
         DG_HIP(hipMemcpyAsync(x, c.pprev, g.n() * sizeof(double), hipMemcpyDeviceToDevice, s));
     } else {
         if (mg_fill(g.n(), s, x, 0.0)) return -1;
@@ -1052,8 +1071,14 @@ static int pcg_solve(Ctx &c)
     hipLaunchKernelGGL(k_init_resid, ig, ib, 0, s, g, r, x, bvec, F.a, F.b, F.c, h2i, partial);
     DG_CHECK_LAUNCH();
     if (sum_partials(s, partial, nblk, 2, partial + 2 * (size_t)nblk, ds + S_RR)) return -1;  // r.r and b.b
-    // learned guess (mg_guess 2) and sample capture start from the
-    // previous-step guess's residual
+    if (c.mg_guess == 1) {
+        // residual check of the previous-step guess against the zero guess
+        hipLaunchKernelGGL(k_guess_check, ig, ib, 0, s, g, x, r, bvec, ds);
+        DG_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_guess_check_scalar, dim3(1), dim3(1), 0, s, ds);
+        DG_CHECK_LAUNCH();
+    }
+    // learned guess (mg_guess 2) and sample capture
     const bool hooks = guess_wanted(c);
     if (hooks && guess_solve_begin(c, x, r, h2i, c.mg_guess == 2, ds + S_RR)) return -1;
     // the split V-cycle's right-hand side of the first iteration

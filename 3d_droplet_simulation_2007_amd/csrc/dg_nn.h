@@ -6,19 +6,19 @@
 
 namespace dg {
 
-// Called by the PCG solve after the initial residual r = b - A x of the
-// previous-step guess x and its sums (ds[S_RR] = r.r, ds[S_RR+1] = b.b)
-// are on the stream.  Builds the network input on the next-coarser level
-// from (coefficients, r, b, previous pressure); when `apply` is set, runs
-// the network, prolongs its correction d to the finest level and, with the
-// energy-optimal step alpha = (d.r)/(d.Ad), replaces x by x + alpha d and r
-// by r - alpha A d if that lowers |r| (otherwise x, r stay: the fallback is
-// the previous-step guess).  rr_slot receives the new r.r.  When capture is
-// on, the input is also written to the sample store.
+// Called by the PCG solve after the initial residual r = b - A x of its
+// guess x (zero for the learned guess) and its sums (ds[S_RR] = r.r,
+// ds[S_RR+1] = b.b) are on the stream.  Builds the network input on the
+// next-coarser level from (coefficients, b, previous pressure); when
+// `apply` is set, runs the network, prolongs its prediction d to the finest
+// level and, with the energy-optimal step alpha = (d.r)/(d.Ad), replaces x
+// by x + alpha d and r by r - alpha A d if that lowers |r| (otherwise x, r
+// stay: the fallback is the guess x).  rr_slot receives the new r.r.  When
+// capture is on, the input is also written to the sample store.
 int guess_solve_begin(Ctx &c, double *x, double *r, double h2i, bool apply, double *rr_slot);
-// Called after the solve, before x is copied to the previous pressure:
-// completes a captured sample with its target (x - previous pressure on
-// the coarse level, normalised); a solve that did not converge drops it.
+// Called after the solve: completes a captured sample with its target (x
+// averaged to the coarse level, normalised); a solve that did not converge
+// drops it.
 int guess_solve_end(Ctx &c, const double *x, double h2i, bool converged);
 // whether the solve must call the hooks (learned guess or capture active)
 bool guess_wanted(const Ctx &c);

@@ -19,12 +19,15 @@
 //
 // Input features per coarse cell (averages over its 8 fine children):
 //   0: log10 of the mean face coefficient / 3 (log10 of 1/density in the
-//      pure phases, /3), 1: asinh(100 * residual of the previous pressure
-//      / rms) / 10, 2: asinh(100 * right-hand side / rms) / 10,
-//   3: previous pressure / rms.
-// Target: (converged pressure - previous pressure) averaged to the coarse
-// cell, scaled to unit rms per sample (the solve's line search sets the
-// length of the step, the network supplies its direction).
+//      pure phases, /3), 1: asinh(right-hand side / rms), near-linear,
+//   2: asinh(100 * right-hand side / rms) / 10, which resolves the small
+//      values between the interface spikes, 3: previous pressure / rms.
+// Target: the converged pressure averaged to the coarse cell, scaled to
+// unit rms per sample.  The guess step starts from the zero guess: the
+// solve's line search sets the length of the step, the network supplies
+// its direction.  (The step-to-step change of the pressure is as large as
+// the pressure on the droplet workloads, so the previous pressure is an
+// input, not a base.)
 
 #include <hip/hip_runtime.h>
 
@@ -47,8 +50,9 @@ namespace {
 constexpr int NL = 15;
 constexpr int FEAT_C = 8;  // input channels (4 used)
 constexpr int NFEAT = 4;
-// residual and right-hand side are spiky (the interface cells hold most of
-// their energy): they enter as asinh(FEAT_GAIN * x / rms) / FEAT_DIV
+// the right-hand side is spiky (the interface cells hold most of its
+// energy): it enters once near-linearly and once as
+// asinh(FEAT_GAIN * x / rms) / FEAT_DIV
 constexpr double FEAT_GAIN = 100.0, FEAT_DIV = 10.0;
 
 struct LayerSpec {
@@ -407,34 +411,34 @@ int net_adam(float lr, hipStream_t s)
 // ---- features, prolongation, guess step -------------------------------------------
 
 // one thread per coarse cell; fine children (2X+2, 2X+3) in 1-based indices
+// (bb_slot: b.b of the solve's right-hand side)
 __global__ void k_features(Grid g, int cx, int cy, int cz, const double *a, const double *b, const double *c,
-                           const double *r, const double *rhs, const double *pp, const double *ds_rr,
-                           double *gs, double nfine, bf16_t *x0)
+                           const double *rhs, const double *pp, const double *bb_slot, double *gs, double nfine,
+                           bf16_t *x0)
 {
     const size_t V = (size_t)cx * cy * cz;
-    const double sr = sqrt(ds_rr[0] / nfine), sb = sqrt(ds_rr[1] / nfine), sp = sqrt(gs[G_PP] / nfine);
-    const double ir = sr > 0.0 ? 1.0 / sr : 0.0, ib = sb > 0.0 ? 1.0 / sb : 0.0, ip = sp > 0.0 ? 1.0 / sp : 0.0;
+    const double sb = sqrt(bb_slot[0] / nfine), sp = sqrt(gs[G_PP] / nfine);
+    const double ib = sb > 0.0 ? 1.0 / sb : 0.0, ip = sp > 0.0 ? 1.0 / sp : 0.0;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        gs[G_SR] = sr;
+        gs[G_SR] = sb;  // output scale: rms(b) h^2
         gs[G_SB] = sb;
         gs[G_SP] = sp;
     }
     for (size_t v = (size_t)blockIdx.x * 256 + threadIdx.x; v < V; v += (size_t)gridDim.x * 256) {
         const int X = (int)(v % cx), Y = (int)((v / cx) % cy), Z = (int)(v / ((size_t)cx * cy));
-        double coef = 0.0, rr = 0.0, bb = 0.0, p = 0.0;
+        double coef = 0.0, bb = 0.0, p = 0.0;
         for (int dz = 0; dz < 2; dz++)
             for (int dy = 0; dy < 2; dy++)
                 for (int dx = 0; dx < 2; dx++) {
                     const int o = g.at(2 * X + 2 + dx, 2 * Y + 2 + dy, 2 * Z + 2 + dz);
                     coef += a[o] + a[o - 1] + b[o] + b[o - g.sy] + c[o] + c[o - g.sz];
-                    rr += r[o];
                     bb += rhs[o];
                     p += pp[o];
                 }
         const double mc = coef / 48.0;
         bf16_t *q = x0 + v * FEAT_C;
         q[0] = f2bf((float)(mc > 0.0 ? log10(mc) / 3.0 : 0.0));
-        q[1] = f2bf((float)(asinh(rr * 0.125 * ir * FEAT_GAIN) / FEAT_DIV));
+        q[1] = f2bf((float)asinh(bb * 0.125 * ib));
         q[2] = f2bf((float)(asinh(bb * 0.125 * ib * FEAT_GAIN) / FEAT_DIV));
         q[3] = f2bf((float)(p * 0.125 * ip));
         q[4] = q[5] = q[6] = q[7] = 0;
@@ -535,8 +539,8 @@ __global__ void k_target_scale(size_t n, const double *gs, float *tgt)
     for (size_t p = (size_t)blockIdx.x * 256 + threadIdx.x; p < n; p += (size_t)gridDim.x * 256) tgt[p] *= s;
 }
 
-__global__ void k_capture_target(Grid g, int cx, int cy, int cz, const double *x, const double *pp, const double *gs,
-                                 double h2, float *tgt)
+__global__ void k_capture_target(Grid g, int cx, int cy, int cz, const double *x, const double *gs, double h2,
+                                 float *tgt)
 {
     const double sc = gs[G_SR] * h2;
     const double isc = sc > 0.0 ? 1.0 / sc : 0.0;
@@ -548,7 +552,7 @@ __global__ void k_capture_target(Grid g, int cx, int cy, int cz, const double *x
             for (int dy = 0; dy < 2; dy++)
                 for (int dx = 0; dx < 2; dx++) {
                     const int o = g.at(2 * X + 2 + dx, 2 * Y + 2 + dy, 2 * Z + 2 + dz);
-                    e += x[o] - pp[o];
+                    e += x[o];
                 }
         tgt[v] = (float)(e * 0.125 * isc);
     }
@@ -607,14 +611,10 @@ int guess_solve_begin(Ctx &c, double *x, double *r, double h2i, bool apply, doub
         set_error("learned guess requested but no network weights are loaded (dg_guess_set_params)");
         return -1;
     }
-    const bool capture = N.capture && N.st.cap > 0 && c.mg_guess >= 1;
-    if (N.capture && c.mg_guess == 0) {
-        set_error("sample capture needs the previous-step guess (dg_set_solver guess 1 or 2)");
-        return -1;
-    }
+    const bool capture = N.capture && N.st.cap > 0;
     // fallback controller: after a learned step that cut |r|^2 by less than
-    // min_gain, the next probe_every solves use the previous-step guess
-    // alone (evaluation mode always runs the network)
+    // min_gain, the next probe_every solves use the plain guess alone
+    // (evaluation mode always runs the network)
     if (apply && !N.eval && N.skip_left > 0) {
         N.skip_left--;
         N.skipped++;
@@ -632,7 +632,7 @@ int guess_solve_begin(Ctx &c, double *x, double *r, double h2i, bool apply, doub
         }))
         return -1;
     hipLaunchKernelGGL(k_features, dim3(gridn((size_t)cx * cy * cz)), dim3(256), 0, s, g, cx, cy, cz, F.a, F.b, F.c,
-                       r, F.rhs, pp, rr_slot, w.scal, nfine, N.a.x0);
+                       F.rhs, pp, rr_slot + 1, w.scal, nfine, N.a.x0);
     DG_CHECK_LAUNCH();
     N.pending = -1;
     if (capture) {
@@ -718,7 +718,7 @@ int guess_solve_end(Ctx &c, const double *x, double h2i, bool converged)
     const Grid g = F.g;
     float *tg = S.tgt + (size_t)N.pending * S.V;
     hipLaunchKernelGGL(k_capture_target, dim3(gridn(S.V)), dim3(256), 0, c.stream, g, S.d[0], S.d[1], S.d[2], x,
-                       c.pprev, N.gw.scal, 1.0 / h2i, tg);
+                       N.gw.scal, 1.0 / h2i, tg);
     DG_CHECK_LAUNCH();
     int nb = (int)((S.V + 255) / 256);
     if (nb > RED_BLOCKS) nb = RED_BLOCKS;

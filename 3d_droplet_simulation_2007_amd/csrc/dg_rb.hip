@@ -321,40 +321,48 @@ __global__ void __launch_bounds__(SWEEP_MAX_THREADS) k_rb_sweep4(RbGeom r, const
     }
 }
 
-// residual of one fine cell (f - A u, boundary folding of the natural resid)
-template <typename CT>
-__device__ inline float rb_resid_cell(const RbGeom &r, const float *u, const float *f, const Coef<CT> &cf, float h2i,
-                                      int i, int j, int k)
-{
-    auto at = [&](int ii, int jj, int kk) { return r.at32(ii, jj, kk); };
-    const unsigned e = at(i, j, k);
-    const float u0 = u[e];
-    const float ue = i < r.nx - 1 ? u[at(i + 1, j, k)] : 0.f;
-    const float uw = i > 2 ? u[at(i - 1, j, k)] : u0;
-    const float un = j < r.ny - 1 ? u[at(i, j + 1, k)] : u0;
-    const float us = j > 2 ? u[at(i, j - 1, k)] : u0;
-    const float ut = k < r.nz - 1 ? u[at(i, j, k + 1)] : u0;
-    const float ub = k > 2 ? u[at(i, j, k - 1)] : u0;
-    const float aw = i > 2 ? cf.a(at(i - 1, j, k)) : 0.f;
-    const float bs = j > 2 ? cf.b(at(i, j - 1, k)) : 0.f;
-    const float cb = k > 2 ? cf.c(at(i, j, k - 1)) : 0.f;
-    const float s = cf.a(e) * (ue - u0) + aw * (uw - u0) + cf.b(e) * (un - u0) + bs * (us - u0) + cf.c(e) * (ut - u0) +
-                    cb * (ub - u0);
-    return f[e] - h2i * s;
-}
-
 // coarse right-hand side = average of the 8 children's residuals; the
-// coarse level is split (fcs) or the natural-layout coarsest level (fcn)
+// coarse level is split (fcs) or the natural-layout coarsest level (fcn).
+// The children of coarse (I, J, K) start at the even fine corner (2I+2,
+// 2J+2, 2K+2), so fine cell (i0+dx, j0+dy, k0+dz) has colour (dx+dy+dz)&1
+// and entry I + (dx>>1) of its row: every offset is the corner's row offset
+// plus constants.  Each child's residual f - A u folds the boundary like
+// the natural-layout resid (k_resid_fold), in the same operation order.
 template <typename CT>
 __device__ inline void rb_restrict_cell(const RbGeom &rf, const float *u, const float *f, const Coef<CT> &cf,
                                         float h2i, const RbGeom &rc, float *fcs, const Grid &gc, float *fcn, int I,
                                         int J, int K)
 {
+    const int i0 = 2 * I + 2, j0 = 2 * J + 2, k0 = 2 * K + 2;
+    const int rs = rf.mh, ps = rf.mh * rf.my;
+    const unsigned half = (unsigned)rf.half;
+    const unsigned base = (unsigned)(((k0 - 2) * rf.my + (j0 - 2)) * rf.mh + I);
+    auto at = [&](int dx, int dy, int dz) {
+        return (unsigned)((dx + dy + dz) & 1) * half + base + (unsigned)(dy * rs + dz * ps + (dx >> 1));
+    };
     float t = 0.f;
+#pragma unroll
     for (int dz = 0; dz < 2; dz++)
+#pragma unroll
         for (int dy = 0; dy < 2; dy++)
-            for (int dx = 0; dx < 2; dx++)
-                t += rb_resid_cell<CT>(rf, u, f, cf, h2i, 2 * I + 2 + dx, 2 * J + 2 + dy, 2 * K + 2 + dz);
+#pragma unroll
+            for (int dx = 0; dx < 2; dx++) {
+                const int i = i0 + dx, j = j0 + dy, k = k0 + dz;
+                const unsigned e = at(dx, dy, dz);
+                const float u0 = u[e];
+                const float ue = i < rf.nx - 1 ? u[at(dx + 1, dy, dz)] : 0.f;
+                const float uw = i > 2 ? u[at(dx - 1, dy, dz)] : u0;
+                const float un = j < rf.ny - 1 ? u[at(dx, dy + 1, dz)] : u0;
+                const float us = j > 2 ? u[at(dx, dy - 1, dz)] : u0;
+                const float ut = k < rf.nz - 1 ? u[at(dx, dy, dz + 1)] : u0;
+                const float ub = k > 2 ? u[at(dx, dy, dz - 1)] : u0;
+                const float aw = i > 2 ? cf.a(at(dx - 1, dy, dz)) : 0.f;
+                const float bs = j > 2 ? cf.b(at(dx, dy - 1, dz)) : 0.f;
+                const float cb = k > 2 ? cf.c(at(dx, dy, dz - 1)) : 0.f;
+                const float s = cf.a(e) * (ue - u0) + aw * (uw - u0) + cf.b(e) * (un - u0) + bs * (us - u0) +
+                                cf.c(e) * (ut - u0) + cb * (ub - u0);
+                t += f[e] - h2i * s;
+            }
     const float v = 0.125f * t;
     if (fcs)
         fcs[rc.at(I + 2, J + 2, K + 2)] = v;

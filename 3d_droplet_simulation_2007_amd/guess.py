@@ -140,7 +140,7 @@ def stats():
 
 def set_controller(probe_every=8, min_gain=4.0):
     """Fallback controller: a learned step dividing |r|^2 by less than
-    min_gain sends the next probe_every solves to the previous-step guess."""
+    min_gain sends the next probe_every solves to the zero guess alone."""
     pe, mg = ctypes.c_int(int(probe_every)), ctypes.c_double(float(min_gain))
     _check(lib().dg_guess_set_controller(ctypes.byref(pe), ctypes.byref(mg)), "dg_guess_set_controller")
 

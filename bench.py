@@ -160,7 +160,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-denominator", action="store_true")
     ap.add_argument("--omega", type=float, default=1.15, help="SOR weight of the V-cycle smoother (all paths)")
-    ap.add_argument("--nu-coarse", type=int, default=3, help="smoothing sweeps below the finest level (all paths)")
+    ap.add_argument("--nu-coarse", type=int, default=4, help="smoothing sweeps below the finest level (all paths)")
     ap.add_argument("--weights", default=os.path.join(HERE, "3d_droplet_simulation_2007_amd", "weights",
                                                        "guess_unet.npz"), help="learned-guess network weights")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",

@@ -83,7 +83,8 @@ int dg_clip_cc(void);
  * mode 0 = the reference's mglin schedule (default, bit-parity path),
  * mode 1 = FP64 PCG with an FP64 V-cycle preconditioner to ||r||/||b|| < tol,
  * mode 2 = FP64 PCG with an FP32 V-cycle preconditioner to ||r||/||b|| < tol.
- * guess: 0 zero, 1 previous step's pressure, 2 learned (dg_guess_*). */
+ * guess: 0 zero, 1 previous step's pressure (dropped for zero when its
+ * residual is no smaller than ||b||), 2 learned (dg_guess_*) from zero. */
 int dg_set_solver(const int *mode, const double *tol, const int *nu1, const int *nu2, const int *maxit,
                   const int *guess);
 /* Tolerance-mode smoother (no reference counterpart): SOR weight omega of
@@ -158,14 +159,13 @@ int dg_poisson_solve(const int *nx, const int *ny, const int *nz, const double *
 /* ---- learned initial guess (no reference counterpart) ------------------ */
 /* A 3D encoder-decoder (channels 16/32/64/64, 3x3x3 convolutions, BF16
  * weights and activations, FP32 accumulation on the matrix cores) on the
- * next-coarser multigrid level predicts the direction of the change of
- * pressure over a step from (log10 mean face coefficient / 3,
- * asinh(100 r/rms(r))/10 of the previous pressure's residual r,
+ * next-coarser multigrid level predicts the direction of the step's
+ * pressure from (log10 mean face coefficient / 3, asinh(b/rms(b)) and
  * asinh(100 b/rms(b))/10 of the right-hand side b, previous pressure /
- * rms); captured targets are scaled to unit rms per sample.  With
- * dg_set_solver guess = 2 the tolerance-mode solve starts from
- * p_prev + alpha * prolong(prediction), alpha = (d.r)/(d.Ad), and falls
- * back to p_prev when that does not lower |b - A p|.  Network extents
+ * rms); captured targets (the converged pressure) are scaled to unit rms
+ * per sample.  With dg_set_solver guess = 2 the tolerance-mode solve starts
+ * from alpha * prolong(prediction), alpha = (d.b)/(d.Ad), and falls back to
+ * the zero guess when that does not lower |b - A p|.  Network extents
  * ((n-2)/2 per axis) must be multiples of 8.
  *
  * Parameters: one FP32 vector, per layer [coutp][27][cinp] weights then
@@ -183,8 +183,7 @@ int dg_guess_gradient(const int *nx, const int *ny, const int *nz, const float *
 int dg_guess_train_sample(const int *nx, const int *ny, const int *nz, const float *feat, const float *target,
                           const float *lr, double *loss);
 /* Sample store in device memory: with capacity > 0 every tolerance-mode
- * solve (guess 1 or 2) stores its network input and target (ring buffer);
- * 0 frees it. */
+ * solve stores its network input and target (ring buffer); 0 frees it. */
 int dg_guess_capture(const int *capacity);
 int dg_guess_store_count(int *n);
 /* sample i: feat [V][4] and target [V] (either may be NULL), dims[3] */
@@ -196,8 +195,8 @@ int dg_guess_train(const int *epochs, const float *lr, const int *seed, double *
  * |r|^2 before / after the learned step over the solves that ran it */
 int dg_guess_stats(int *accepted, int *rejected, int *skipped, double *last_alpha, double *mean_gain);
 /* Fallback controller: a learned step that divides |r|^2 by less than
- * min_gain sends the next probe_every solves to the previous-step guess
- * alone (probe_every 0: the network runs on every solve).  Default 8, 4. */
+ * min_gain sends the next probe_every solves to the zero guess alone
+ * (probe_every 0: the network runs on every solve).  Default 8, 4. */
 int dg_guess_set_controller(const int *probe_every, const double *min_gain);
 /* on = 1: each solve runs from the zero, previous and learned guesses in
  * turn (same system); dg_guess_eval_iters returns the three counts of the

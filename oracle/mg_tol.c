@@ -351,6 +351,15 @@ int or_mg_solve_tol(or_state *st)
     double *r = calloc(n, sizeof(double)), *z = calloc(n, sizeof(double)), *p = calloc(n, sizeof(double)),
            *q = calloc(n, sizeof(double));
     resid_d(r, x, L->rhs, L->a, L->b, L->c, nx, ny, nz, h2i);
+    if (st->mg_guess == 1 && dot_interior(r, r, nx, ny, nz) >= bn * bn) {
+        /* the previous pressure is no better than zero by residual: x = 0, r = b */
+        for (int k = 2; k <= nz - 1; k++)
+            for (int j = 2; j <= ny - 1; j++)
+                for (int i = 2; i <= nx - 1; i++) {
+                    x[IDX(i, j, k)] = 0.0;
+                    r[IDX(i, j, k)] = L->rhs[IDX(i, j, k)];
+                }
+    }
     double rel = sqrt(dot_interior(r, r, nx, ny, nz)) / bn;
     int it = 0;
     if (st->mg_outer == 1) {

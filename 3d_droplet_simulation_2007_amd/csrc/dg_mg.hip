@@ -673,9 +673,11 @@ constexpr int PCG_KCH = 8;
 // current plane and the next), so the in-plane neighbours come from LDS;
 // each thread carries its column's plane-below value and c-coefficient in
 // registers.  The per-cell arithmetic is the stencil of resid
-// (main.f:3917-3923) on p_new with bc_scalar folding.
-template <class Z>
-__global__ void __launch_bounds__(256) k_pcg_dir(Grid g, Z z, const double *pold, double *pnew, double *q,
+// (main.f:3917-3923) on p_new with bc_scalar folding.  PT: the storage type
+// of the search direction; p_new is rounded to it before the stencil, so q
+// is A applied to the stored direction and x, r stay consistent.
+template <class Z, typename PT>
+__global__ void __launch_bounds__(256) k_pcg_dir(Grid g, Z z, const PT *pold, PT *pnew, double *q,
                                                  const double *a, const double *b, const double *c, double h2i,
                                                  const double *ds, double *partial)
 {
@@ -693,9 +695,11 @@ __global__ void __launch_bounds__(256) k_pcg_dir(Grid g, Z z, const double *pold
     const bool first = ds[S_FIRST] != 0.0;
     const double beta = first ? 0.0 : -ds[S_ALPHA] * ds[S_ZQ] / ds[S_RZOLD];
     const int sy = g.sy, sz = g.sz;
+    // the first iteration's direction is z itself (p_old is not read)
     auto P = [&](int ii, int jj, int kk) {
         const int oo = g.at(ii, jj, kk);
-        return z(ii, jj, kk, oo) + beta * pold[oo];
+        const double zz = z(ii, jj, kk, oo);
+        return (double)(PT)(first ? zz : zz + beta * (double)pold[oo]);
     };
     // p_new of plane kk on the tile and its ring (0 outside the interior,
     // where no stencil reads it)
@@ -731,7 +735,7 @@ __global__ void __launch_bounds__(256) k_pcg_dir(Grid g, Z z, const double *pold
             const double s = a[o] * (pe - p0) + a[o - 1] * (pw - p0) + b[o] * (pn - p0) + b[o - sy] * (ps - p0) +
                              ct * (pt - p0) + cb * (pb - p0);
             const double qq = h2i * s;
-            pnew[o] = p0;
+            pnew[o] = (PT)p0;
             q[o] = qq;
             contrib += p0 * qq;
             pb = p0;
@@ -751,7 +755,8 @@ __global__ void __launch_bounds__(256) k_pcg_dir(Grid g, Z z, const double *pold
 // K3: alpha = rz/pq; x += alpha p; r -= alpha q; partial sums of r.r
 // (fs != null: also the FP32 red-black split copy of the new r, the next
 // V-cycle's right-hand side)
-__global__ void __launch_bounds__(256) k_pcg_update(Grid g, double *x, double *r, const double *p, const double *q,
+template <typename PT>
+__global__ void __launch_bounds__(256) k_pcg_update(Grid g, double *x, double *r, const PT *p, const double *q,
                                                     double *ds, double *partial, float *fs, RbGeom rg)
 {
     __shared__ double lds4[4];
@@ -766,7 +771,7 @@ __global__ void __launch_bounds__(256) k_pcg_update(Grid g, double *x, double *r
     double contrib = 0.0;
     if (i <= g.nx - 1 && j <= g.ny - 1) {
         const int o = g.at(i, j, k);
-        x[o] = x[o] + alpha * p[o];
+        x[o] = x[o] + alpha * (double)p[o];
         const double rn = r[o] - alpha * q[o];
         r[o] = rn;
         if (fs) fs[rg.at32(i, j, k)] = (float)rn;
@@ -964,6 +969,9 @@ static int pcg_iteration(Ctx &c, double *pa, double *pb)
     level_ptrs<T>(F, u, f, rr_, a_, b_, c_, w_);
     const double ncell = (double)(g.nx - 2) * (g.ny - 2) * (g.nz - 2);
     const bool split = sizeof(T) == 4 && rb_enabled(c);
+    // search direction storage: FP32 with the FP32 V-cycle (mode 2), FP64
+    // in the plain FP64 solver (mode 1)
+    using PT = T;
     if (split) {
         // z = M r through the red-black split V-cycle; its right-hand side
         // (the FP32 split copy of r) was written by the previous update or
@@ -975,8 +983,9 @@ static int pcg_iteration(Ctx &c, double *pa, double *pb)
         hipLaunchKernelGGL(k_final2, dim3(1), dim3(256), 0, s, partial, RED_BLOCKS, ds + S_RZ);
         DG_CHECK_LAUNCH();
         prof_begin(PK_PCG_DIR, s);
-        hipLaunchKernelGGL(k_pcg_dir<ZSplit>, dgr, ib, 0, s, g, z, pa, pb, q, F.a, F.b, F.c, h2i, ds, partial);
-        prof_end(PK_PCG_DIR, s, (4.0 + 6.0 * 8.0) * ncell);
+        hipLaunchKernelGGL((k_pcg_dir<ZSplit, PT>), dgr, ib, 0, s, g, z, (const PT *)pa, (PT *)pb, q, F.a, F.b, F.c,
+                           h2i, ds, partial);
+        prof_end(PK_PCG_DIR, s, (4.0 + 2.0 * sizeof(PT) + 4.0 * 8.0) * ncell);
         DG_CHECK_LAUNCH();
     } else {
         if (sizeof(T) == 4)
@@ -992,15 +1001,16 @@ static int pcg_iteration(Ctx &c, double *pa, double *pb)
         hipLaunchKernelGGL(k_final2, dim3(1), dim3(256), 0, s, partial, RED_BLOCKS, ds + S_RZ);
         DG_CHECK_LAUNCH();
         prof_begin(PK_PCG_DIR, s);
-        hipLaunchKernelGGL(k_pcg_dir<ZNatural<T>>, dgr, ib, 0, s, g, z, pa, pb, q, F.a, F.b, F.c, h2i, ds, partial);
-        prof_end(PK_PCG_DIR, s, ((double)sizeof(T) + 6.0 * 8.0) * ncell);
+        hipLaunchKernelGGL((k_pcg_dir<ZNatural<T>, PT>), dgr, ib, 0, s, g, z, (const PT *)pa, (PT *)pb, q, F.a, F.b,
+                           F.c, h2i, ds, partial);
+        prof_end(PK_PCG_DIR, s, ((double)sizeof(T) + 2.0 * sizeof(PT) + 4.0 * 8.0) * ncell);
         DG_CHECK_LAUNCH();
     }
     if (sum_partials(s, partial, nblk_dir, 1, partial + 2 * (size_t)nblk, ds + S_PQ)) return -1;
     prof_begin(PK_PCG_UPDATE, s);
-    hipLaunchKernelGGL(k_pcg_update, ig, ib, 0, s, g, x, r, pb, q, ds, partial, split ? F.rb_f : nullptr,
-                       rb_geom(g));
-    prof_end(PK_PCG_UPDATE, s, (6.0 * 8.0 + (split ? 4.0 : 0.0)) * ncell);
+    hipLaunchKernelGGL(k_pcg_update<PT>, ig, ib, 0, s, g, x, r, (const PT *)pb, q, ds, partial,
+                       split ? F.rb_f : nullptr, rb_geom(g));
+    prof_end(PK_PCG_UPDATE, s, (5.0 * 8.0 + sizeof(PT) + (split ? 4.0 : 0.0)) * ncell);
     DG_CHECK_LAUNCH();
     if (sum_partials(s, partial, nblk, 1, partial + 2 * (size_t)nblk, ds + S_RR)) return -1;
     hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(1), 0, s, ds);

@@ -270,8 +270,8 @@ def set_tuning(fused=False, graphs=True, split=True, tail="lds"):
     """Tolerance-mode execution switches:
     fused -- one-launch two-colour smoother sweeps instead of two colour
              passes (bit-identical): False nowhere, True on the natural layout
-             (Gauss-Seidel only) and the split finest level, "split" on the
-             split finest level only;
+             (Gauss-Seidel only) and every split level, "split" on the split
+             levels below the finest of at most 2^18 cells;
     graphs -- replay each PCG iteration as a captured HIP graph (bit-identical);
     split -- mode 2's FP32 V-cycle on red-black split storage with FP16
              coefficients where the densities allow (same tolerance reached);

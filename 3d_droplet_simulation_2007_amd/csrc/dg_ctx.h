@@ -62,9 +62,9 @@ struct Ctx {
     int mg_nuc = 0;         // sweeps (pre and post) below the finest level; 0: mg_nu1 / mg_nu2
     int ordered_sums = 1;  // reproduce the reference's sequential outflow sums
     int vof_guard = 0;     // 1: plane normal to the sweep where the Youngs normal vanishes
-    int fused_smoother = 0;  // one-pass red-black sweeps: 0 nowhere, 1 natural layout (Gauss-Seidel) and the
-                             // split finest level, 2 the split finest level only (both slower than two
-                             // passes on gfx950 so far: latency-bound plane marching)
+    int fused_smoother = 0;  // one-pass red-black sweeps: 0 nowhere; 1 the natural layout (Gauss-Seidel),
+                             // the split finest level (plane marching) and the split levels below it
+                             // (tiles); 2 the split levels of <= 2^18 cells below the finest only
     int use_graphs = 1;      // replay each PCG iteration as a captured HIP graph
     int ref_grid = 1;        // the grid has the reference's form nn = 2**ng + 2 (mglin's schedule may run)
     int rb_split = 1;        // mode 2: red-black split storage of the V-cycle levels

@@ -66,9 +66,15 @@ __global__ void __launch_bounds__(256) k_rb_in(Grid g, RbGeom r, const double *s
 // x-max neighbour is 0 with its face kept in the diagonal.  ZERO: the
 // iterate is zero on entry (first pass of a V-cycle level), u is not read.
 // TOP only separates the finest level's launches into their own symbol.
-template <typename CT, bool ZERO>
-__device__ inline void rb_cell(const RbGeom &r, float *u, const float *f, const Coef<CT> &cf, float h2, int colour,
-                               float om, int zown, int m, int j, int k)
+// The new value of one cell of colour `colour` (entry m of row (j, k)):
+// nb(m', dj, dk) is the other colour's value at entry m' of row (j+dj,
+// k+dk), uold the cell's own value before the pass.  Every split colour
+// update (the colour passes, the tail, the fused sweeps) goes through this
+// arithmetic, so their results agree to the bit.
+template <typename CT, bool ZERO, class NB>
+__device__ inline float rb_cell_value(const RbGeom &r, const NB &nb, const float *f, const float *uold_arr,
+                                      const Coef<CT> &cf, float h2, int colour, float om, int zown, int m, int j,
+                                      int k)
 {
     const int p = (colour + j + k) & 1;
     const int i = 2 + p + 2 * m;
@@ -83,12 +89,12 @@ __device__ inline void rb_cell(const RbGeom &r, float *u, const float *f, const 
     const float cb = hb ? cf.c(oth + row - ps + m) : 0.f;
     float s = 0.f, d = ae;
     if (!ZERO) {
-        if (i < r.nx - 1) s += ae * u[oth + row + m + p];
-        if (hw) s += aw * u[oth + row + m + p - 1];
-        if (hn) s += bn * u[oth + row + rs + m];
-        if (hs) s += bs * u[oth + row - rs + m];
-        if (ht) s += ct * u[oth + row + ps + m];
-        if (hb) s += cb * u[oth + row - ps + m];
+        if (i < r.nx - 1) s += ae * nb(m + p, 0, 0);
+        if (hw) s += aw * nb(m + p - 1, 0, 0);
+        if (hn) s += bn * nb(m, 1, 0);
+        if (hs) s += bs * nb(m, -1, 0);
+        if (ht) s += ct * nb(m, 0, 1);
+        if (hb) s += cb * nb(m, 0, -1);
     }
     if (hw) d += aw;
     if (hn) d += bn;
@@ -96,8 +102,20 @@ __device__ inline void rb_cell(const RbGeom &r, float *u, const float *f, const 
     if (ht) d += ct;
     if (hb) d += cb;
     const float gsv = (s - h2 * f[own]) / d;
-    const float uold = (ZERO || zown || om == 1.f) ? 0.f : u[own];  // Gauss-Seidel needs no old value
-    u[own] = om * gsv + (1.f - om) * uold;
+    const float uold = (ZERO || zown || om == 1.f) ? 0.f : uold_arr[own];  // Gauss-Seidel needs no old value
+    return om * gsv + (1.f - om) * uold;
+}
+
+// in-place colour update of one cell, neighbours from the iterate itself
+template <typename CT, bool ZERO>
+__device__ inline void rb_cell(const RbGeom &r, float *u, const float *f, const Coef<CT> &cf, float h2, int colour,
+                               float om, int zown, int m, int j, int k)
+{
+    const size_t row = ((size_t)(k - 2) * r.my + (j - 2)) * r.mh;
+    const float *o = u + (size_t)(1 - colour) * r.half + row;
+    const size_t rs = (size_t)r.mh, ps = (size_t)r.mh * r.my;
+    auto nb = [&](int mm, int dj, int dk) { return o[(ptrdiff_t)dj * (ptrdiff_t)rs + (ptrdiff_t)dk * (ptrdiff_t)ps + mm]; };
+    u[(size_t)colour * r.half + row + m] = rb_cell_value<CT, ZERO>(r, nb, f, u, cf, h2, colour, om, zown, m, j, k);
 }
 
 template <typename CT, bool ZERO, bool TOP>
@@ -318,6 +336,52 @@ __global__ void __launch_bounds__(SWEEP_MAX_THREADS) k_rb_sweep4(RbGeom r, const
             *reinterpret_cast<float4 *>(Y + own) = make_float4(out[0], out[1], out[2], out[3]);
         }
         __syncthreads();
+    }
+}
+
+// One red-black sweep of a level below the finest in one launch, out of
+// place (X before the sweep, Y after).  A workgroup owns a tile of the
+// second colour cb (FT_X entries x FT_Y rows x FT_Z planes); it first
+// updates the first colour ca on the tile and a one-entry, one-row,
+// one-plane ring around it from X (exactly the first colour pass) into
+// LDS, then cb on the tile from those values (exactly the second pass).
+// Ring cells are computed again by the workgroups that own them, which
+// write them; every cell is written once.  Two dependent phases per sweep
+// instead of two launches: on the small levels, where a launch costs more
+// than its bytes, that halves the sweep's time.
+constexpr int FT_X = 16, FT_Y = 8, FT_Z = 2;
+static_assert(FT_X * FT_Y * FT_Z == 256, "one thread per cell of the tile");
+
+template <typename CT, bool ZERO>
+__global__ void __launch_bounds__(256) k_rb_sweep_tile(RbGeom r, const float *__restrict__ X, float *__restrict__ Y,
+                                                       const float *__restrict__ f, Coef<CT> cf, float h2, int ca,
+                                                       float om)
+{
+    constexpr int HX = FT_X + 2, HY = FT_Y + 2, HZ = FT_Z + 2, HN = HX * HY * HZ;
+    __shared__ float A[HN];
+    const uint3 tile = xcd_tile();
+    const int m0 = (int)tile.x * FT_X, j0 = (int)tile.y * FT_Y + 2, k0 = (int)tile.z * FT_Z + 2;
+    const int tid = (int)threadIdx.x, cb = 1 - ca;
+    const ptrdiff_t rs = r.mh, ps = (ptrdiff_t)r.mh * r.my;
+    for (int e = tid; e < HN; e += 256) {
+        const int hx = e % HX, t = e / HX, hy = t % HY, hz = t / HY;
+        const int m = m0 - 1 + hx, j = j0 - 1 + hy, k = k0 - 1 + hz;
+        if (m < 0 || m >= r.mh || j < 2 || j > r.ny - 1 || k < 2 || k > r.nz - 1) continue;
+        const size_t row = ((size_t)(k - 2) * r.my + (j - 2)) * r.mh;
+        const float *o = X + (size_t)cb * r.half + row;
+        auto nb = [&](int mm, int dj, int dk) { return o[dj * rs + dk * ps + mm]; };
+        const float v = rb_cell_value<CT, ZERO>(r, nb, f, X, cf, h2, ca, om, ZERO ? 1 : 0, m, j, k);
+        A[e] = v;
+        if (hx >= 1 && hx <= FT_X && hy >= 1 && hy <= FT_Y && hz >= 1 && hz <= FT_Z)
+            Y[(size_t)ca * r.half + row + m] = v;
+    }
+    __syncthreads();
+    const int tx = tid % FT_X, ty = (tid / FT_X) % FT_Y, tz = tid / (FT_X * FT_Y);
+    const int m = m0 + tx, j = j0 + ty, k = k0 + tz;
+    if (m < r.mh && j <= r.ny - 1 && k <= r.nz - 1) {
+        auto nb = [&](int mm, int dj, int dk) { return A[((tz + 1 + dk) * HY + (ty + 1 + dj)) * HX + (mm - m0 + 1)]; };
+        const float v = rb_cell_value<CT, false>(r, nb, f, X, cf, h2, cb, om, ZERO ? 1 : 0, m, j, k);
+        Y[(size_t)cb * r.half + ((size_t)(k - 2) * r.my + (j - 2)) * r.mh + m] = v;
     }
 }
 
@@ -722,26 +786,50 @@ int launch_tail(Ctx &c, int lt)
 // planes per workgroup of the fused sweep
 constexpr int SWEEP_KC = 4;
 
-// whether level l runs fused sweeps: the finest level, rows of 8..64 groups
-// of four entries (one workgroup spans whole rows)
-bool sweep_fused_ok(const Ctx &c, int l)
+// largest level (cells) that the tile sweep serves by default (fused = 2):
+// measured at 256^3, 7.6 us against two 4.4 us passes at 32^3 and 8.4
+// against 2 x 4.5 at 64^3, but 31.5 against 2 x 5.6 at 128^3, where its
+// recomputed ring costs more than the launch it saves
+constexpr size_t TILE_SWEEP_MAX_CELLS = 262144;
+
+// how level l sweeps (levels above the tail): 0 two colour passes, 1 the
+// plane-marching one-pass sweep (finest level, rows of 8..64 groups of four
+// entries, fused = 1), 2 the tile one-pass sweep (levels below the finest:
+// all with fused = 1, up to TILE_SWEEP_MAX_CELLS with fused = 2)
+int fused_kind(const Ctx &c, int l)
 {
-    if (c.fused_smoother == 0 || l != c.ng - 1) return false;
+    if (c.fused_smoother == 0 || !c.lev[l].rb_w) return 0;
+    if (l != c.ng - 1)
+        return (c.fused_smoother == 1 || 2 * rb_geom(c.lev[l].g).half <= TILE_SWEEP_MAX_CELLS) ? 2 : 0;
+    if (c.fused_smoother != 1) return 0;
     const RbGeom r = rb_geom(c.lev[l].g);
     const int tx = r.mh / 4;
-    return r.mh % 4 == 0 && (tx == 8 || tx == 16 || tx == 32 || tx == 64) && c.lev[l].rb_w &&
-           tx * (256 / tx + 2) <= SWEEP_MAX_THREADS;
+    return r.mh % 4 == 0 && (tx == 8 || tx == 16 || tx == 32 || tx == 64) &&
+                   tx * (256 / tx + 2) <= SWEEP_MAX_THREADS
+               ? 1
+               : 0;
 }
 
 // one fused red-black sweep of level l: colour ca then the other, X -> Y
 template <typename CT>
-int sweep_fused(Ctx &c, int l, const float *X, float *Y, int ca, bool zero)
+int sweep_fused(Ctx &c, int l, const float *X, float *Y, int ca, bool zero, int kind)
 {
     Level &L = c.lev[l];
     const RbGeom r = rb_geom(L.g);
     const Coef<CT> cf{(const CT *)L.rb_coef, 2 * r.half};
     const float h2 = (float)(1.0 / lev_h2i(c, L.g));
     const float om = (float)c.mg_omega;
+    if (kind == 2) {
+        const dim3 grd((unsigned)((r.mh + FT_X - 1) / FT_X), (unsigned)((r.my + FT_Y - 1) / FT_Y),
+                       (unsigned)((r.mz + FT_Z - 1) / FT_Z));
+        if (zero)
+            hipLaunchKernelGGL((k_rb_sweep_tile<CT, true>), grd, dim3(256), 0, c.stream, r, X, Y, L.rb_f, cf, h2, ca, om);
+        else
+            hipLaunchKernelGGL((k_rb_sweep_tile<CT, false>), grd, dim3(256), 0, c.stream, r, X, Y, L.rb_f, cf, h2, ca,
+                               om);
+        DG_CHECK_LAUNCH();
+        return 0;
+    }
     const int tx = r.mh / 4, ty = 256 / tx;  // ty owned rows, ty + 2 thread rows
     const dim3 blk((unsigned)tx, (unsigned)(ty + 2));
     const dim3 grd(1, (unsigned)((r.my + ty - 1) / ty), (unsigned)((r.mz + SWEEP_KC - 1) / SWEEP_KC));
@@ -771,14 +859,15 @@ int vcycle_rb(Ctx &c, int l)
     const int nu2 = (top || c.mg_nuc <= 0) ? c.mg_nu2 : c.mg_nuc;
     // fused sweeps alternate between rb_u and rb_w; the first one writes the
     // buffer that makes the last one land in rb_u
-    const bool fused = sweep_fused_ok(c, l);
+    const int fk = fused_kind(c, l);
+    const bool fused = fk != 0;
     float *cur = L.rb_u;
     auto other = [&](float *b) { return b == L.rb_u ? L.rb_w : L.rb_u; };
     if (fused) {
         float *w1 = ((nu1 + nu2) & 1) ? L.rb_u : L.rb_w;
         for (int q = 0; q < nu1; q++) {
             float *y = q == 0 ? w1 : other(cur);
-            if (sweep_fused<CT>(c, l, cur, y, 0, q == 0)) return -1;
+            if (sweep_fused<CT>(c, l, cur, y, 0, q == 0, fk)) return -1;
             cur = y;
         }
     } else {
@@ -808,7 +897,7 @@ int vcycle_rb(Ctx &c, int l)
     if (fused) {
         for (int q = 0; q < nu2; q++) {
             float *y = other(cur);
-            if (sweep_fused<CT>(c, l, cur, y, 1, false)) return -1;
+            if (sweep_fused<CT>(c, l, cur, y, 1, false, fk)) return -1;
             cur = y;
         }
         if (cur != L.rb_u) {
@@ -853,7 +942,7 @@ int rb_prepare(Ctx &c)
             DG_HIP(hipMemset(L.rb_u, 0, n2 * sizeof(float)));
             DG_HIP(hipMemset(L.rb_f, 0, n2 * sizeof(float)));
         }
-        if (l == c.ng - 1 && c.fused_smoother != 0 && !L.rb_w) {
+        if (c.fused_smoother != 0 && !L.rb_w) {
             DG_HIP(hipMalloc(&L.rb_w, n2 * sizeof(float)));
             DG_HIP(hipMemset(L.rb_w, 0, n2 * sizeof(float)));
         }

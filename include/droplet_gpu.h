@@ -104,9 +104,10 @@ int dg_set_ordered_sums(const int *on);
 int dg_set_vof_guard(const int *on);
 /* Tolerance-mode execution switches.  Bit-identical either way: fused
  * selects one out-of-place pass per red-black sweep instead of two in-place
- * colour passes: 0 (default) nowhere, 1 on the natural layout
- * (Gauss-Seidel, equal sweep counts) and the split finest level, 2 on the
- * split finest level only; graphs = 1 (default) each PCG iteration replayed as a
+ * colour passes: 0 (default) nowhere; 1 on the natural layout (Gauss-Seidel,
+ * equal sweep counts), the split finest level and the split levels below
+ * it; 2 on the split levels below the finest of at most 2^18 cells; graphs = 1
+ * (default) each PCG iteration replayed as a
  * captured HIP graph, 0 launched kernel by kernel.  split = 1 (default):
  * mode 2's FP32 V-cycle keeps each level above the coarsest in red-black
  * split storage with FP16 face coefficients when the densities allow

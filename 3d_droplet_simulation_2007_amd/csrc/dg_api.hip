@@ -458,7 +458,7 @@ int dg_step(const int *t)
     if (op_vofmethod(c, *t)) return -1;
     if (c.s.iturb == 3 && op_kepsmodel(c, *t)) return -1;
     if (op_checktime(c, *t)) return -1;
-    if (c.s.surfacet == 2 && op_csf(c)) return -1;
+    if (c.s.surfacet == 2 && op_csf(c, false)) return -1;
     if (op_solvelpres(c, *t)) return -1;
     return 0;
 }

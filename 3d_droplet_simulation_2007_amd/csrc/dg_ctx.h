@@ -93,7 +93,7 @@ int op_vofmethod(Ctx &c, int t);
 int op_kepsmodel(Ctx &c, int t);
 int op_cfl(Ctx &c, double *out_host);
 int op_checktime(Ctx &c, int t);
-int op_csf(Ctx &c);
+int op_csf(Ctx &c, bool w_interior = true);
 int op_solvelpres(Ctx &c, int t);
 int op_smooth2(Ctx &c);
 

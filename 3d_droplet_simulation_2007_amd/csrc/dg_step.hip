@@ -655,7 +655,7 @@ int op_kepsmodel(Ctx &c, int t)
             prod = F(mut, 0, 0, 0) / F(rh, 0, 0, 0) / h2 *
                    (2.0 * (dudx * dudx + dvdy * dvdy + dwdz * dwdz + dvdx * dudy + dwdx * dudz + dvdz * dwdy) +
                     dudy * dudy + dudz * dudz + dvdx * dvdx + dvdz * dvdz + dwdx * dwdx + dwdy * dwdy);
-            kn[o] = F(kt, 0, 0, 0) + tau * (-conv + diff + prod - F(ep, 0, 0, 0));
+            const double knew = F(kt, 0, 0, 0) + tau * (-conv + diff + prod - F(ep, 0, 0, 0));
             conv = (+dmax1(U(1, 0, 0), 0.0) * F(ep, 0, 0, 0) + dmin1(U(1, 0, 0), 0.0) * F(ep, 1, 0, 0) -
                     dmax1(U(0, 0, 0), 0.0) * F(ep, -1, 0, 0) - dmin1(U(0, 0, 0), 0.0) * F(ep, 0, 0, 0) +
                     dmax1(V(0, 1, 0), 0.0) * F(ep, 0, 0, 0) + dmin1(V(0, 1, 0), 0.0) * F(ep, 0, 1, 0) -
@@ -674,8 +674,10 @@ int op_kepsmodel(Ctx &c, int t)
                     cjp * (F(ep, 0, 1, 0) - F(ep, 0, 0, 0)) - cjm * (F(ep, 0, 0, 0) - F(ep, 0, -1, 0)) +
                     ckp * (F(ep, 0, 0, 1) - F(ep, 0, 0, 0)) - ckm * (F(ep, 0, 0, 0) - F(ep, 0, 0, -1)));
             prod = prod * c1 * F(ep, 0, 0, 0) / F(kt, 0, 0, 0);
-            en[o] = F(ep, 0, 0, 0) +
+            const double enew = F(ep, 0, 0, 0) +
                     tau * (-conv + diff + prod - c2 * F(ep, 0, 0, 0) * F(ep, 0, 0, 0) / F(kt, 0, 0, 0));
+            kn[o] = knew;
+            en[o] = enew;
 #undef U
 #undef V
 #undef W
@@ -686,9 +688,10 @@ int op_kepsmodel(Ctx &c, int t)
     const double *cmu = c.cmu;
     if (launch_interior(g, c.stream, [=] __device__(int i, int j, int k) {
             const int o = g.at(i, j, k);
-            ktur[o] = kn[o];
-            epsa[o] = en[o];
-            muta[o] = cmu[o] * rh[o] * ktur[o] * ktur[o] / epsa[o];
+            const double kv = kn[o], ev = en[o];
+            ktur[o] = kv;
+            epsa[o] = ev;
+            muta[o] = cmu[o] * rh[o] * kv * kv / ev;
         }))
         return -1;
     if (op_bc_scalar(g, c.stream, c.ktur)) return -1;
@@ -763,7 +766,40 @@ int mg_filterc27(const Grid &g, hipStream_t s, const double *cc, double *cf)
     });
 }
 
-int op_csf(Ctx &c)
+// dst[q] = src[q] on the ghost shell of g (every cell with i, j or k on the
+// boundary), for up to three array pairs; the edges are copied more than
+// once, which a plain copy allows
+static int copy_shell(const Grid &g, hipStream_t s, double *const dst[3], const double *const src[3], int n)
+{
+    double *d0 = dst[0], *d1 = n > 1 ? dst[1] : nullptr, *d2 = n > 2 ? dst[2] : nullptr;
+    const double *s0 = src[0], *s1 = n > 1 ? src[1] : nullptr, *s2 = n > 2 ? src[2] : nullptr;
+    auto cp = [=] __device__(int o) {
+        d0[o] = s0[o];
+        if (d1) d1[o] = s1[o];
+        if (d2) d2[o] = s2[o];
+    };
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    if (launch_face(ny, nz, s, [=] __device__(int j0, int k0) {
+            cp(g.at(1, j0 + 1, k0 + 1));
+            cp(g.at(nx, j0 + 1, k0 + 1));
+        }))
+        return -1;
+    if (launch_face(nx, nz, s, [=] __device__(int i0, int k0) {
+            cp(g.at(i0 + 1, 1, k0 + 1));
+            cp(g.at(i0 + 1, ny, k0 + 1));
+        }))
+        return -1;
+    return launch_face(nx, ny, s, [=] __device__(int i0, int j0) {
+        cp(g.at(i0 + 1, j0 + 1, 1));
+        cp(g.at(i0 + 1, j0 + 1, nz));
+    });
+}
+
+// csf leaves w11, w22, w33 = s11, s22, s33 on every cell (main.f:2471-2473).
+// With w_interior false only their ghost shells are written: the time step
+// calls csf right before solvelpres, whose predictor rewrites the interior
+// of all three before anything reads it.
+int op_csf(Ctx &c, bool w_interior)
 {
     const Grid g = c.g;
     Level &L = c.lev[c.ng - 1];
@@ -771,15 +807,24 @@ int op_csf(Ctx &c)
     const size_t nb = sizeof(double) * g.n();
     hipStream_t s = c.stream;
     // ccf = filterc27(cc) through w22, then bc_out (main.f:2372-2379)
-    if (mg_filterc27(g, s, ccc, w22)) return -1;
-    DG_HIP(hipMemcpyAsync(ccf, w22, nb, hipMemcpyDeviceToDevice, s));
-    if (op_bc_out(g, s, ccf)) return -1;
     if (c.s.sigma == 0.0) {
+        if (mg_filterc27(g, s, ccc, w22)) return -1;
+        DG_HIP(hipMemcpyAsync(ccf, w22, nb, hipMemcpyDeviceToDevice, s));
+        if (op_bc_out(g, s, ccf)) return -1;
         double *arr[6] = {c.s11, c.s22, c.s33, c.s12, c.s13, c.s23};
         for (double *a : arr)
             if (mg_fill(g.n(), s, a, 0.0)) return -1;
         return 0;
     }
+    // w22 is overwritten below, so the filter writes the interior of ccf
+    // directly; its ghost shell takes w22's, which bc_out's edge copies read
+    {
+        double *dst[3] = {ccf, nullptr, nullptr};
+        const double *src[3] = {w22, nullptr, nullptr};
+        if (copy_shell(g, s, dst, src, 1)) return -1;
+    }
+    if (mg_filterc27(g, s, ccc, ccf)) return -1;
+    if (op_bc_out(g, s, ccf)) return -1;
     const double sigh = 0.5 * (c.s.tau * c.s.tau) * c.s.sigma / (c.s.h * c.s.h * c.s.h);
     const int sy = g.sy, sz = g.sz;
     if (launch_interior(g, s, [=] __device__(int i, int j, int k) {
@@ -810,17 +855,30 @@ int op_csf(Ctx &c)
         return -1;
     if (op_bc_out(g, s, w11)) return -1;
     double *s11 = c.s11, *s22 = c.s22, *s33 = c.s33;
+    // the copies w = s of the interior are made by the kernel that writes s
+    // (each cell reads only its own w11 there)
+    const bool wi = w_interior;
     if (launch_interior(g, s, [=] __device__(int i, int j, int k) {
             const int o = g.at(i, j, k);
             const double cf = ccf[o], wv = w11[o];
-            s11[o] = sigh * (cf - ccf[o - 1]) * wv;
-            s22[o] = sigh * (cf - ccf[o - sy]) * wv;
-            s33[o] = sigh * (cf - ccf[o - sz]) * wv;
+            const double v11 = sigh * (cf - ccf[o - 1]) * wv;
+            const double v22 = sigh * (cf - ccf[o - sy]) * wv;
+            const double v33 = sigh * (cf - ccf[o - sz]) * wv;
+            s11[o] = v11;
+            s22[o] = v22;
+            s33[o] = v33;
+            if (wi) {
+                w11[o] = v11;
+                w22[o] = v22;
+                w33[o] = v33;
+            }
         }))
         return -1;
-    DG_HIP(hipMemcpyAsync(w11, s11, nb, hipMemcpyDeviceToDevice, s));
-    DG_HIP(hipMemcpyAsync(w22, s22, nb, hipMemcpyDeviceToDevice, s));
-    DG_HIP(hipMemcpyAsync(w33, s33, nb, hipMemcpyDeviceToDevice, s));
+    {
+        double *dst[3] = {w11, w22, w33};
+        const double *src[3] = {s11, s22, s33};
+        if (copy_shell(g, s, dst, src, 3)) return -1;
+    }
     return op_bc_tensor(c);
 }
 
@@ -841,18 +899,27 @@ static int stress(Ctx &c)
     const double r1 = 2.0 * tauh2 * (c.s.mu1 - c.s.mu2);
     const double r2 = 2.0 * tauh2 * c.s.mu2;
     const int sy = g.sy, sz = g.sz;
+    // every value is computed before the first store: the compiler cannot
+    // tell the stores from the input arrays, and a store between two loads
+    // would make it reload and serialise them
     return launch_interior(g, c.stream, [=] __device__(int i, int j, int k) {
         const int o = g.at(i, j, k);
         double temp = r2 + r1 * cc[o];
-        s11[o] = -s11[o] + temp * (u[o + 1] - u[o]);
-        s22[o] = -s22[o] + temp * (v[o + sy] - v[o]);
-        s33[o] = -s33[o] + temp * (w[o + sz] - w[o]);
+        const double n11 = -s11[o] + temp * (u[o + 1] - u[o]);
+        const double n22 = -s22[o] + temp * (v[o + sy] - v[o]);
+        const double n33 = -s33[o] + temp * (w[o + sz] - w[o]);
         temp = r20 + r10 * (cc[o] + cc[o - 1] + cc[o - sy] + cc[o - 1 - sy]);
-        s12[o] = -s12[o] + (temp) * (u[o] - u[o - sy] + v[o] - v[o - 1]);
+        const double n12 = -s12[o] + (temp) * (u[o] - u[o - sy] + v[o] - v[o - 1]);
         temp = r20 + r10 * (cc[o] + cc[o - 1] + cc[o - sz] + cc[o - 1 - sz]);
-        s13[o] = -s13[o] + (temp) * (u[o] - u[o - sz] + w[o] - w[o - 1]);
+        const double n13 = -s13[o] + (temp) * (u[o] - u[o - sz] + w[o] - w[o - 1]);
         temp = r20 + r10 * (cc[o] + cc[o - sy] + cc[o - sz] + cc[o - sy - sz]);
-        s23[o] = -s23[o] + (temp) * (v[o] - v[o - sz] + w[o] - w[o - sy]);
+        const double n23 = -s23[o] + (temp) * (v[o] - v[o - sz] + w[o] - w[o - sy]);
+        s11[o] = n11;
+        s22[o] = n22;
+        s33[o] = n33;
+        s12[o] = n12;
+        s13[o] = n13;
+        s23[o] = n23;
     });
 }
 
@@ -862,9 +929,12 @@ int mg_abcei(const Grid &g, hipStream_t s, double *a, double *b, double *c, doub
     const int nx = g.nx, ny = g.ny, nz = g.nz, sy = g.sy, sz = g.sz;
     if (launch_interior(g, s, [=] __device__(int i, int j, int k) {
             const int o = g.at(i, j, k);
-            a[o] = 1.0 / (r1 * (cc[o] + cc[o + 1]) + r2);
-            b[o] = 1.0 / (r1 * (cc[o] + cc[o + sy]) + r2);
-            c[o] = 1.0 / (r1 * (cc[o] + cc[o + sz]) + r2);
+            const double na = 1.0 / (r1 * (cc[o] + cc[o + 1]) + r2);
+            const double nb = 1.0 / (r1 * (cc[o] + cc[o + sy]) + r2);
+            const double nc = 1.0 / (r1 * (cc[o] + cc[o + sz]) + r2);
+            a[o] = na;
+            b[o] = nb;
+            c[o] = nc;
         }))
         return -1;
     // bc_coef (main.f:4465-4494): the three fills write different arrays
@@ -902,9 +972,12 @@ static int predictor(Ctx &c, double *w1, double *w2, double *w3)
     const int sy = g.sy, sz = g.sz;
     return launch_interior(g, c.stream, [=] __device__(int i, int j, int k) {
         const int o = g.at(i, j, k);
-        w1[o] = u[o] + a[o - 1] * (s11[o] - s11[o - 1] + s12[o + sy] - s12[o] + s13[o + sz] - s13[o]) + horgrav;
-        w2[o] = v[o] + b[o - sy] * (s12[o + 1] - s12[o] + s22[o] - s22[o - sy] + s23[o + sz] - s23[o]) + stdgrav;
-        w3[o] = w[o] + cf[o - sz] * (s13[o + 1] - s13[o] + s23[o + sy] - s23[o] + s33[o] - s33[o - sz]);
+        const double n1 = u[o] + a[o - 1] * (s11[o] - s11[o - 1] + s12[o + sy] - s12[o] + s13[o + sz] - s13[o]) + horgrav;
+        const double n2 = v[o] + b[o - sy] * (s12[o + 1] - s12[o] + s22[o] - s22[o - sy] + s23[o + sz] - s23[o]) + stdgrav;
+        const double n3 = w[o] + cf[o - sz] * (s13[o + 1] - s13[o] + s23[o + sy] - s23[o] + s33[o] - s33[o - sz]);
+        w1[o] = n1;
+        w2[o] = n2;
+        w3[o] = n3;
     });
 }
 
@@ -927,7 +1000,7 @@ static int convective(Ctx &c, const double *w1, const double *w2, const double *
             s2 = .5 * (A1(0, 0, 0) + A1(1, 0, 0));
             q1 = .5 * (A3(0, 0, 0) + A3(-1, 0, 0));
             q2 = .5 * (A3(0, 0, 1) + A3(-1, 0, 1));
-            u[o] = A1(0, 0, 0) + tauh * (+dmax1(t1, 0.0) * A1(0, -1, 0) + dmin1(t1, 0.0) * A1(0, 0, 0) -
+            const double nu = A1(0, 0, 0) + tauh * (+dmax1(t1, 0.0) * A1(0, -1, 0) + dmin1(t1, 0.0) * A1(0, 0, 0) -
                                          dmax1(t2, 0.0) * A1(0, 0, 0) - dmin1(t2, 0.0) * A1(0, 1, 0) +
                                          dmax1(s1, 0.0) * A1(-1, 0, 0) + dmin1(s1, 0.0) * A1(0, 0, 0) -
                                          dmax1(s2, 0.0) * A1(0, 0, 0) - dmin1(s2, 0.0) * A1(1, 0, 0) +
@@ -939,7 +1012,7 @@ static int convective(Ctx &c, const double *w1, const double *w2, const double *
             s2 = .5 * (A2(0, 0, 0) + A2(0, 1, 0));
             q1 = .5 * (A3(0, 0, 0) + A3(0, -1, 0));
             q2 = .5 * (A3(0, 0, 1) + A3(0, -1, 1));
-            v[o] = A2(0, 0, 0) + tauh * (+dmax1(t1, 0.0) * A2(-1, 0, 0) + dmin1(t1, 0.0) * A2(0, 0, 0) -
+            const double nv = A2(0, 0, 0) + tauh * (+dmax1(t1, 0.0) * A2(-1, 0, 0) + dmin1(t1, 0.0) * A2(0, 0, 0) -
                                          dmax1(t2, 0.0) * A2(0, 0, 0) - dmin1(t2, 0.0) * A2(1, 0, 0) +
                                          dmax1(s1, 0.0) * A2(0, -1, 0) + dmin1(s1, 0.0) * A2(0, 0, 0) -
                                          dmax1(s2, 0.0) * A2(0, 0, 0) - dmin1(s2, 0.0) * A2(0, 1, 0) +
@@ -951,34 +1024,40 @@ static int convective(Ctx &c, const double *w1, const double *w2, const double *
             s2 = .5 * (A3(0, 0, 0) + A3(0, 0, 1));
             q1 = .5 * (A2(0, 0, 0) + A2(0, 0, -1));
             q2 = .5 * (A2(0, 1, 0) + A2(0, 1, -1));
-            w[o] = A3(0, 0, 0) + tauh * (+dmax1(t1, 0.0) * A3(-1, 0, 0) + dmin1(t1, 0.0) * A3(0, 0, 0) -
+            const double nw = A3(0, 0, 0) + tauh * (+dmax1(t1, 0.0) * A3(-1, 0, 0) + dmin1(t1, 0.0) * A3(0, 0, 0) -
                                          dmax1(t2, 0.0) * A3(0, 0, 0) - dmin1(t2, 0.0) * A3(1, 0, 0) +
                                          dmax1(s1, 0.0) * A3(0, 0, -1) + dmin1(s1, 0.0) * A3(0, 0, 0) -
                                          dmax1(s2, 0.0) * A3(0, 0, 0) - dmin1(s2, 0.0) * A3(0, 0, 1) +
                                          dmax1(q1, 0.0) * A3(0, -1, 0) + dmin1(q1, 0.0) * A3(0, 0, 0) -
                                          dmax1(q2, 0.0) * A3(0, 0, 0) - dmin1(q2, 0.0) * A3(0, 1, 0));
+            u[o] = nu;
+            v[o] = nv;
+            w[o] = nw;
         });
     } else if (c.s.convectype == 2) {
         return launch_interior(g, c.stream, [=] __device__(int i, int j, int k) {
             const int o = g.at(i, j, k);
-            u[o] = A1(0, 0, 0) +
+            const double nu = A1(0, 0, 0) +
                    tauh * (0.25 * (sq(A1(0, 0, 0) + A1(-1, 0, 0)) - sq(A1(1, 0, 0) + A1(0, 0, 0)) +
                                    (A1(0, -1, 0) + A1(0, 0, 0)) * (A2(0, 0, 0) + A2(-1, 0, 0)) -
                                    (A1(0, 1, 0) + A1(0, 0, 0)) * (A2(0, 1, 0) + A2(-1, 1, 0)) +
                                    (A1(0, 0, -1) + A1(0, 0, 0)) * (A3(0, 0, 0) + A3(-1, 0, 0)) -
                                    (A1(0, 0, 1) + A1(0, 0, 0)) * (A3(0, 0, 1) + A3(-1, 0, 1))));
-            v[o] = A2(0, 0, 0) +
+            const double nv = A2(0, 0, 0) +
                    tauh * (0.25 * (sq(A2(0, 0, 0) + A2(0, -1, 0)) - sq(A2(0, 1, 0) + A2(0, 0, 0)) +
                                    (A2(-1, 0, 0) + A2(0, 0, 0)) * (A1(0, 0, 0) + A1(0, -1, 0)) -
                                    (A2(1, 0, 0) + A2(0, 0, 0)) * (A1(1, 0, 0) + A1(1, -1, 0)) +
                                    (A2(0, 0, -1) + A2(0, 0, 0)) * (A3(0, 0, 0) + A3(0, -1, 0)) -
                                    (A2(0, 0, 1) + A2(0, 0, 0)) * (A3(0, 0, 1) + A3(0, -1, 1))));
-            w[o] = A3(0, 0, 0) +
+            const double nw = A3(0, 0, 0) +
                    tauh * (0.25 * (sq(A3(0, 0, 0) + A3(0, 0, -1)) - sq(A3(0, 0, 1) + A3(0, 0, 0)) +
                                    (A3(-1, 0, 0) + A3(0, 0, 0)) * (A1(0, 0, 0) + A1(0, 0, -1)) -
                                    (A3(1, 0, 0) + A3(0, 0, 0)) * (A1(1, 0, 0) + A1(1, 0, -1)) +
                                    (A3(0, -1, 0) + A3(0, 0, 0)) * (A2(0, 0, 0) + A2(0, 0, -1)) -
                                    (A3(0, 1, 0) + A3(0, 0, 0)) * (A2(0, 1, 0) + A2(0, 1, -1))));
+            u[o] = nu;
+            v[o] = nv;
+            w[o] = nw;
 #undef A1
 #undef A2
 #undef A3
@@ -986,9 +1065,10 @@ static int convective(Ctx &c, const double *w1, const double *w2, const double *
     }
     return launch_interior(g, c.stream, [=] __device__(int i, int j, int k) {
         const int o = g.at(i, j, k);
-        u[o] = w1[o];
-        v[o] = w2[o];
-        w[o] = w3[o];
+        const double nu = w1[o], nv = w2[o], nw = w3[o];
+        u[o] = nu;
+        v[o] = nv;
+        w[o] = nw;
     });
 }
 
@@ -1032,9 +1112,12 @@ int op_solvelpres(Ctx &c, int t)
         if (launch_interior(g, c.stream, [=] __device__(int i, int j, int k) {
                 const int o = g.at(i, j, k);
                 const double p0 = w1[o];
-                u[o] = u[o] - (p0 - w1[o - 1]) * a[o - 1];
-                v[o] = v[o] - (p0 - w1[o - sy]) * b[o - sy];
-                w[o] = w[o] - (p0 - w1[o - sz]) * cf[o - sz];
+                const double nu = u[o] - (p0 - w1[o - 1]) * a[o - 1];
+                const double nv = v[o] - (p0 - w1[o - sy]) * b[o - sy];
+                const double nw = w[o] - (p0 - w1[o - sz]) * cf[o - sz];
+                u[o] = nu;
+                v[o] = nv;
+                w[o] = nw;
             }))
             return -1;
     }

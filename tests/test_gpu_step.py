@@ -157,3 +157,43 @@ def test_split_fp16_preconditioner_matches_natural(dg):
         e = rel_l2(out[True][k], out[False][k])
         print(k, f"{e:.2e}", flush=True)
         assert e <= 1e-6, (k, e)
+
+
+def test_operators_one_by_one_match_oracle(dg):
+    """The step's operators called one at a time (dg_vofmethod, dg_checktime,
+    dg_csf, dg_solvelpres) after one whole step.  csf's outputs are compared
+    on every cell, ghosts included: s11, s22, s33, and w11, w22, w33 (the
+    fields p, res, rhs), which csf leaves equal to s11, s22, s33
+    (main.f:2471-2473) and whose ghost shells feed the next operators."""
+    import oracle_ffi
+    print("case 34^3 operators one by one", flush=True)
+    txt = inputdata_text(nx=34, ny=34, nz=34)
+    o = Oracle(txt)
+    g = dg.DropletGPU(inputdata_text=txt)
+    o.step(0)
+    g.step(0)
+    L = oracle_ffi.lib()
+    L.or_vofmethod(o.h, 1)
+    g.vofmethod(1)
+    L.or_checktime(o.h, 1)
+    g.checktime(1)
+    assert abs(g.state()["tau"] - o.st.tau) <= 1e-12 * o.st.tau
+    L.or_csf(o.h)
+    g.csf()
+    worst = {}
+    for f in ("cc", "s11", "s22", "s33", "p", "res", "rhs"):
+        a, b = g.get(f), o.field(f)
+        scale = max(float(np.abs(b).max()), 1e-300)
+        worst[f] = float(np.abs(a - b).max()) / scale
+    print({k: f"{v:.2e}" for k, v in worst.items()}, flush=True)
+    for f, e in worst.items():
+        assert e <= 1e-12, (f, e)
+    for f in ("s11", "s22", "s33"):
+        assert np.array_equal(g.get({"s11": "p", "s22": "res", "s33": "rhs"}[f]), g.get(f)), f
+    L.or_solvelpres(o.h, 1)
+    g.solvelpres(1)
+    errs = {f: rel_l2(g.get(f), o.field(f)) for f in FIELDS}
+    print({k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    for f, e in errs.items():
+        assert e <= 1e-10, (f, e)
+    g.close()

@@ -455,7 +455,7 @@ int dg_step(const int *t)
 {
     if (require_ready()) return -1;
     Ctx &c = *ctx();
-    if (op_vofmethod(c, *t)) return -1;
+    if (op_vofmethod(c, *t, true)) return -1;
     if (c.s.iturb == 3 && op_kepsmodel(c, *t)) return -1;
     if (op_checktime(c, *t)) return -1;
     if (c.s.surfacet == 2 && op_csf(c, false)) return -1;

@@ -89,7 +89,7 @@ int op_bc_scalar(const Grid &g, hipStream_t s, double *a);
 int op_bc_out(const Grid &g, hipStream_t s, double *a);
 int op_bc_resid(const Grid &g, hipStream_t s, double *a);
 int op_calcrho(Ctx &c);
-int op_vofmethod(Ctx &c, int t);
+int op_vofmethod(Ctx &c, int t, bool step = false);
 int op_kepsmodel(Ctx &c, int t);
 int op_cfl(Ctx &c, double *out_host);
 int op_checktime(Ctx &c, int t);

@@ -435,99 +435,145 @@ int plic_fl3d_batch(int n, hipStream_t s, const double *m1, const double *m2, co
 // ---------------------------------------------------------------------------
 // VOF: swp (main.f:848-1000) and VOFmethod (main.f:795-824)
 
+// The fluxes of cell o in sweep direction d (swp main.f:848-1000): f1 leaves
+// through the lower face, f2 stays, f3 leaves through the upper face, with
+// a1 = us(o) tauh, a2 = us(o + sd) tauh.  want selects the fluxes formed
+// (1: f1, 2: f2, 4: f3); each is formed exactly as when all three are, and
+// the Youngs normal and the plane are skipped when no wanted flux needs them.
+__device__ inline void swp_fluxes(const double *cc, const Grid g, int o, double a1, double a2, int d, int guard,
+                                  int want, double &f1, double &f2, double &f3)
+{
+    const double tiny = DG_F4(1.e-30);
+    f1 = 0.0;
+    f2 = 0.0;
+    f3 = 0.0;
+    const double c0 = cc[o];
+#define C(di, dj, dk) cc[o + (di) + g.sy * (dj) + g.sz * (dk)]
+    if (c0 == 1.0) {
+        f1 = dmax1(-a1, 0.0);
+        f2 = 1.0 - dmax1(a1, 0.0) + dmin1(a2, 0.0);
+        f3 = dmax1(a2, 0.0);
+    } else if (c0 > 0.0) {
+        const bool w1 = (want & 1) && a1 < 0.0, w3 = (want & 4) && a2 > 0.0;
+        if (!(w1 || w3 || (want & 2))) return;
+        double mm1, mm2, mx, my, mz;
+        mm1 = C(-1, -1, -1) + C(-1, -1, 1) + C(-1, 1, -1) + C(-1, 1, 1) +
+              2.0 * (C(-1, -1, 0) + C(-1, 1, 0) + C(-1, 0, -1) + C(-1, 0, 1)) + 4.0 * C(-1, 0, 0);
+        mm2 = C(1, -1, -1) + C(1, -1, 1) + C(1, 1, -1) + C(1, 1, 1) +
+              2.0 * (C(1, -1, 0) + C(1, 1, 0) + C(1, 0, -1) + C(1, 0, 1)) + 4.0 * C(1, 0, 0);
+        mx = mm1 - mm2;
+        mm1 = C(-1, -1, -1) + C(-1, -1, 1) + C(1, -1, -1) + C(1, -1, 1) +
+              2.0 * (C(-1, -1, 0) + C(1, -1, 0) + C(0, -1, -1) + C(0, -1, 1)) + 4.0 * C(0, -1, 0);
+        mm2 = C(-1, 1, -1) + C(-1, 1, 1) + C(1, 1, -1) + C(1, 1, 1) +
+              2.0 * (C(-1, 1, 0) + C(1, 1, 0) + C(0, 1, -1) + C(0, 1, 1)) + 4.0 * C(0, 1, 0);
+        my = mm1 - mm2;
+        mm1 = C(-1, -1, -1) + C(-1, 1, -1) + C(1, -1, -1) + C(1, 1, -1) +
+              2.0 * (C(-1, 0, -1) + C(1, 0, -1) + C(0, -1, -1) + C(0, 1, -1)) + 4.0 * C(0, 0, -1);
+        mm2 = C(-1, -1, 1) + C(-1, 1, 1) + C(1, -1, 1) + C(1, 1, 1) +
+              2.0 * (C(-1, 0, 1) + C(1, 0, 1) + C(0, -1, 1) + C(0, 1, 1)) + 4.0 * C(0, 0, 1);
+        mz = mm1 - mm2;
+        double invx = 1.0, invy = 1.0, invz = 1.0;
+        if (mx < 0.0) { mx = -mx; invx = -1.0; }
+        if (my < 0.0) { my = -my; invy = -1.0; }
+        if (mz < 0.0) { mz = -mz; invz = -1.0; }
+        if (guard && mx + my + mz == 0.0) {
+            if (d == 1) mx = 1.0;
+            else if (d == 2) my = 1.0;
+            else mz = 1.0;
+        }
+        mm2 = mx + my + mz + tiny;
+        mx = mx / mm2;
+        my = my / mm2;
+        mz = mz / mm2;
+        double alpha = al3d(mx, my, mz, c0);
+        mx = invx * mx;
+        my = invy * my;
+        mz = invz * mz;
+        alpha = alpha + dmin1(0.0, mx) + dmin1(0.0, my) + dmin1(0.0, mz);
+        mm1 = dmax1(a1, 0.0);
+        mm2 = 1.0 - mm1 + dmin1(0.0, a2);
+        // the advected direction goes first in FL3D's argument list
+        double md, mp, mq;
+        if (d == 1) { md = mx; mp = my; mq = mz; }
+        else if (d == 2) { md = my; mp = mz; mq = mx; }
+        else { md = mz; mp = mx; mq = my; }
+        md = md / (1.0 - a1 + a2);
+        alpha = alpha + md * a1;
+        if (w1) f1 = fl3d(md, mp, mq, alpha, a1, -a1);
+        if (w3) f3 = fl3d(md, mp, mq, alpha, 1.0, a2);
+        if (want & 2) f2 = fl3d(md, mp, mq, alpha, mm1, mm2);
+    }
+#undef C
+}
+
+// One sweep fused into one pass (the time step's path): cell o's new volume
+// fraction vof3(o - sd) + vof2(o) + vof1(o + sd) is formed from the fluxes of
+// its two neighbours along d, recomputed here, instead of three flux arrays
+// written and read back.  Only the interface cells whose neighbour's
+// velocity points at them do PLIC work twice, so the pass moves cc, the
+// velocity and the new cc (24 B per cell against 72).  Out of place: every
+// thread reads the old cc; ghost cells are copied.  The neighbour fluxes
+// across the domain's ends are bc_flux's zeros (main.f:4496-4539).
+template <int D>
+__global__ void __launch_bounds__(256) k_swp_fused(Grid g, const double *us, const double *cc, double *ccn,
+                                                   double tauh, int guard)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1;
+    const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + 1;
+    const int k = (int)blockIdx.z + 1;
+    if (i > g.nx || j > g.ny) return;
+    const int o = g.at(i, j, k);
+    if (i == 1 || i == g.nx || j == 1 || j == g.ny || k == 1 || k == g.nz) {
+        ccn[o] = cc[o];
+        return;
+    }
+    const int sd = D == 1 ? 1 : (D == 2 ? g.sy : g.sz);
+    const int id = D == 1 ? i : (D == 2 ? j : k), nd = D == 1 ? g.nx : (D == 2 ? g.ny : g.nz);
+    const double u0 = us[o] * tauh, up = us[o + sd] * tauh;
+    double f1, f2, f3, fb = 0.0, fa = 0.0, t1, t2;
+    swp_fluxes(cc, g, o, u0, up, D, guard, 2, f1, f2, f3);
+    if (id > 2) swp_fluxes(cc, g, o - sd, us[o - sd] * tauh, u0, D, guard, 4, t1, t2, fb);
+    if (id < nd - 1) swp_fluxes(cc, g, o + sd, up, us[o + 2 * sd] * tauh, D, guard, 1, fa, t1, t2);
+    const double v = fb + f2 + fa;
+    ccn[o] = dmax1(0.0, dmin1(1.0, v));
+}
+
+// bc_flux's zeros (main.f:4496-4539) on the ghost faces of the flux arrays
+static int bc_flux(const Grid &g, hipStream_t s, double *vof1, double *vof3, int d)
+{
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    if (d == 1)
+        return launch_face(ny - 2, nz - 2, s, [=] __device__(int j0, int k0) {
+            vof3[g.at(1, j0 + 2, k0 + 2)] = 0.0;
+            vof1[g.at(nx, j0 + 2, k0 + 2)] = 0.0;
+        });
+    if (d == 2)
+        return launch_face(nx - 2, nz - 2, s, [=] __device__(int i0, int k0) {
+            vof3[g.at(i0 + 2, 1, k0 + 2)] = 0.0;
+            vof1[g.at(i0 + 2, ny, k0 + 2)] = 0.0;
+        });
+    return launch_face(nx - 2, ny - 2, s, [=] __device__(int i0, int j0) {
+        vof3[g.at(i0 + 2, j0 + 2, 1)] = 0.0;
+        vof1[g.at(i0 + 2, j0 + 2, nz)] = 0.0;
+    });
+}
+
 static int swp(Ctx &c, const double *us, double *cc, double *vof1, double *vof2, double *vof3, int d)
 {
     const Grid g = c.g;
-    const int nx = g.nx, ny = g.ny, nz = g.nz;
     const double tauh = c.s.tauh;
-    const double tiny = DG_F4(1.e-30);
     const int sd = d == 1 ? 1 : (d == 2 ? g.sy : g.sz);
     const int guard = c.vof_guard;
     if (launch_interior_rr(g, c.stream, [=] __device__(int i, int j, int k) {
             const int o = g.at(i, j, k);
-            const double a1 = us[o] * tauh;
-            const double a2 = us[o + sd] * tauh;
-            double f1 = 0.0, f2 = 0.0, f3 = 0.0;
-            const double c0 = cc[o];
-#define C(di, dj, dk) cc[o + (di) + g.sy * (dj) + g.sz * (dk)]
-            if (c0 == 1.0) {
-                f1 = dmax1(-a1, 0.0);
-                f2 = 1.0 - dmax1(a1, 0.0) + dmin1(a2, 0.0);
-                f3 = dmax1(a2, 0.0);
-            } else if (c0 > 0.0) {
-                double mm1, mm2, mx, my, mz;
-                mm1 = C(-1, -1, -1) + C(-1, -1, 1) + C(-1, 1, -1) + C(-1, 1, 1) +
-                      2.0 * (C(-1, -1, 0) + C(-1, 1, 0) + C(-1, 0, -1) + C(-1, 0, 1)) + 4.0 * C(-1, 0, 0);
-                mm2 = C(1, -1, -1) + C(1, -1, 1) + C(1, 1, -1) + C(1, 1, 1) +
-                      2.0 * (C(1, -1, 0) + C(1, 1, 0) + C(1, 0, -1) + C(1, 0, 1)) + 4.0 * C(1, 0, 0);
-                mx = mm1 - mm2;
-                mm1 = C(-1, -1, -1) + C(-1, -1, 1) + C(1, -1, -1) + C(1, -1, 1) +
-                      2.0 * (C(-1, -1, 0) + C(1, -1, 0) + C(0, -1, -1) + C(0, -1, 1)) + 4.0 * C(0, -1, 0);
-                mm2 = C(-1, 1, -1) + C(-1, 1, 1) + C(1, 1, -1) + C(1, 1, 1) +
-                      2.0 * (C(-1, 1, 0) + C(1, 1, 0) + C(0, 1, -1) + C(0, 1, 1)) + 4.0 * C(0, 1, 0);
-                my = mm1 - mm2;
-                mm1 = C(-1, -1, -1) + C(-1, 1, -1) + C(1, -1, -1) + C(1, 1, -1) +
-                      2.0 * (C(-1, 0, -1) + C(1, 0, -1) + C(0, -1, -1) + C(0, 1, -1)) + 4.0 * C(0, 0, -1);
-                mm2 = C(-1, -1, 1) + C(-1, 1, 1) + C(1, -1, 1) + C(1, 1, 1) +
-                      2.0 * (C(-1, 0, 1) + C(1, 0, 1) + C(0, -1, 1) + C(0, 1, 1)) + 4.0 * C(0, 0, 1);
-                mz = mm1 - mm2;
-#undef C
-                double invx = 1.0, invy = 1.0, invz = 1.0;
-                if (mx < 0.0) { mx = -mx; invx = -1.0; }
-                if (my < 0.0) { my = -my; invy = -1.0; }
-                if (mz < 0.0) { mz = -mz; invz = -1.0; }
-                if (guard && mx + my + mz == 0.0) {
-                    if (d == 1) mx = 1.0;
-                    else if (d == 2) my = 1.0;
-                    else mz = 1.0;
-                }
-                mm2 = mx + my + mz + tiny;
-                mx = mx / mm2;
-                my = my / mm2;
-                mz = mz / mm2;
-                double alpha = al3d(mx, my, mz, c0);
-                mx = invx * mx;
-                my = invy * my;
-                mz = invz * mz;
-                alpha = alpha + dmin1(0.0, mx) + dmin1(0.0, my) + dmin1(0.0, mz);
-                mm1 = dmax1(a1, 0.0);
-                mm2 = 1.0 - mm1 + dmin1(0.0, a2);
-                // the advected direction goes first in FL3D's argument list
-                double md, mp, mq;
-                if (d == 1) { md = mx; mp = my; mq = mz; }
-                else if (d == 2) { md = my; mp = mz; mq = mx; }
-                else { md = mz; mp = mx; mq = my; }
-                md = md / (1.0 - a1 + a2);
-                alpha = alpha + md * a1;
-                if (a1 < 0.0) f1 = fl3d(md, mp, mq, alpha, a1, -a1);
-                if (a2 > 0.0) f3 = fl3d(md, mp, mq, alpha, 1.0, a2);
-                f2 = fl3d(md, mp, mq, alpha, mm1, mm2);
-            }
+            double f1, f2, f3;
+            swp_fluxes(cc, g, o, us[o] * tauh, us[o + sd] * tauh, d, guard, 7, f1, f2, f3);
             vof1[o] = f1;
             vof2[o] = f2;
             vof3[o] = f3;
         }))
         return -1;
-    // bc_flux (main.f:4496-4539)
-    if (d == 1) {
-        if (launch_face(ny - 2, nz - 2, c.stream, [=] __device__(int j0, int k0) {
-                vof3[g.at(1, j0 + 2, k0 + 2)] = 0.0;
-                vof1[g.at(nx, j0 + 2, k0 + 2)] = 0.0;
-            }))
-            return -1;
-    } else if (d == 2) {
-        if (launch_face(nx - 2, nz - 2, c.stream, [=] __device__(int i0, int k0) {
-                vof3[g.at(i0 + 2, 1, k0 + 2)] = 0.0;
-                vof1[g.at(i0 + 2, ny, k0 + 2)] = 0.0;
-            }))
-            return -1;
-    } else {
-        if (launch_face(nx - 2, ny - 2, c.stream, [=] __device__(int i0, int j0) {
-                vof3[g.at(i0 + 2, j0 + 2, 1)] = 0.0;
-                vof1[g.at(i0 + 2, j0 + 2, nz)] = 0.0;
-            }))
-            return -1;
-    }
+    if (bc_flux(g, c.stream, vof1, vof3, d)) return -1;
     if (launch_interior(g, c.stream, [=] __device__(int i, int j, int k) {
             const int o = g.at(i, j, k);
             double v = vof3[o - sd] + vof2[o] + vof1[o + sd];
@@ -535,6 +581,31 @@ static int swp(Ctx &c, const double *us, double *cc, double *vof1, double *vof2,
         }))
         return -1;
     return op_bc_c(c, cc);
+}
+
+// swp for the time step: one fused pass into csf's ccf array (free until
+// csf rewrites it whole), whose pointer then trades places with cc.  The
+// flux arrays keep the ghost zeros of bc_flux; their interiors are not
+// written (the step rewrites them before reading them: kepsmodel, csf's
+// stress copies, solvelpres's predictor and abcei).
+static int swp_step(Ctx &c, const double *us, double *vof1, double *vof3, int d)
+{
+    const Grid g = c.g;
+    Level &L = c.lev[c.ng - 1];
+    const dim3 grid((unsigned)((g.nx + BX - 1) / BX), (unsigned)((g.ny + BY - 1) / BY), (unsigned)g.nz);
+    const double tauh = c.s.tauh;
+    if (d == 1)
+        hipLaunchKernelGGL(k_swp_fused<1>, grid, interior_block(), 0, c.stream, g, us, L.cc, c.ccf, tauh, c.vof_guard);
+    else if (d == 2)
+        hipLaunchKernelGGL(k_swp_fused<2>, grid, interior_block(), 0, c.stream, g, us, L.cc, c.ccf, tauh, c.vof_guard);
+    else
+        hipLaunchKernelGGL(k_swp_fused<3>, grid, interior_block(), 0, c.stream, g, us, L.cc, c.ccf, tauh, c.vof_guard);
+    DG_CHECK_LAUNCH();
+    if (bc_flux(g, c.stream, vof1, vof3, d)) return -1;
+    double *t = L.cc;
+    L.cc = c.ccf;
+    c.ccf = t;
+    return op_bc_c(c, L.cc);
 }
 
 // fturbdif (main.f:1239-1267): in-place update with k innermost, then j, so
@@ -573,7 +644,7 @@ int op_calcrho(Ctx &c)
     });
 }
 
-int op_vofmethod(Ctx &c, int t)
+int op_vofmethod(Ctx &c, int t, bool step)
 {
     (void)t;
     Level &L = c.lev[c.ng - 1];
@@ -585,7 +656,7 @@ int op_vofmethod(Ctx &c, int t)
     else { order[0] = 1; order[1] = 2; order[2] = 3; }
     for (int q = 0; q < 3; q++) {
         const double *us = order[q] == 1 ? c.u : (order[q] == 2 ? c.v : c.w);
-        if (swp(c, us, L.cc, L.ei, L.res, L.rhs, order[q])) return -1;
+        if (step ? swp_step(c, us, L.ei, L.rhs, order[q]) : swp(c, us, L.cc, L.ei, L.res, L.rhs, order[q])) return -1;
     }
     if (op_bc_c(c, L.cc)) return -1;
     if (c.s.iturb != 0) {

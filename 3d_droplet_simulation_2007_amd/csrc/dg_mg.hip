@@ -847,6 +847,9 @@ __global__ void k_guess_check_scalar(double *ds)
 }
 
 // r = b - A x with folded ghosts; partial sums of r.r and b.b
+// ZERO: x is zero, so A x is +0 on every cell and r = b - h2i * 0 = b bit
+// for bit; x and the coefficients are not read
+template <bool ZERO>
 __global__ void __launch_bounds__(256) k_init_resid(Grid g, double *r, const double *x, const double *bvec,
                                                     const double *a, const double *b, const double *c, double h2i,
                                                     double *partial)
@@ -861,7 +864,12 @@ __global__ void __launch_bounds__(256) k_init_resid(Grid g, double *r, const dou
     const int k = (int)tile.z + 2;
     const int tid = (int)(threadIdx.x + blockDim.x * threadIdx.y);
     double c1 = 0.0, c2 = 0.0;
-    if (i <= g.nx - 1 && j <= g.ny - 1) {
+    if (ZERO && i <= g.nx - 1 && j <= g.ny - 1) {
+        const double bb = bvec[g.at(i, j, k)];
+        r[g.at(i, j, k)] = bb;
+        c1 = bb * bb;
+        c2 = bb * bb;
+    } else if (i <= g.nx - 1 && j <= g.ny - 1) {
         const int o = g.at(i, j, k), sy = g.sy, sz = g.sz;
         const double u0 = x[o];
         const double ue = i < g.nx - 1 ? x[o + 1] : 0.0;
@@ -1078,7 +1086,10 @@ static int pcg_solve(Ctx &c)
     } else {
         if (mg_fill(g.n(), s, x, 0.0)) return -1;
     }
-    hipLaunchKernelGGL(k_init_resid, ig, ib, 0, s, g, r, x, bvec, F.a, F.b, F.c, h2i, partial);
+    if (c.mg_guess == 1)
+        hipLaunchKernelGGL(k_init_resid<false>, ig, ib, 0, s, g, r, x, bvec, F.a, F.b, F.c, h2i, partial);
+    else
+        hipLaunchKernelGGL(k_init_resid<true>, ig, ib, 0, s, g, r, x, bvec, F.a, F.b, F.c, h2i, partial);
     DG_CHECK_LAUNCH();
     if (sum_partials(s, partial, nblk, 2, partial + 2 * (size_t)nblk, ds + S_RR)) return -1;  // r.r and b.b
     if (c.mg_guess == 1) {

@@ -818,13 +818,61 @@ int op_checktime(Ctx &c, int t)
 // ---------------------------------------------------------------------------
 // CSF surface tension, csf (main.f:2363-2481), filterc27 (main.f:5473-5511)
 
+// planes each thread of a column kernel walks
+constexpr int COLUMN_KC = 8;
+
+// f(P, o) on the interior for a stencil within the 3 x 3 x 3 block around
+// the cell, with each thread walking COLUMN_KC planes up one column and
+// keeping the 3 x 3 neighbourhoods of src in the planes below, at and
+// above the cell in registers: P[dk + 1][(dj + 1) * 3 + di + 1] is
+// src(i + di, j + dj, k + dk).  9 loads per cell instead of one per
+// stencil point: the vector memory pipeline, not HBM, bounds the
+// one-cell-per-thread form of such stencils.
+template <class F>
+__global__ void __launch_bounds__(256) k_column27(Grid g, const double *src, F f)
+{
+    const uint3 tile = xcd_tile();
+    const int i = (int)(tile.x * blockDim.x + threadIdx.x) + 2;
+    const int j = (int)(tile.y * blockDim.y + threadIdx.y) + 2;
+    const int k0 = (int)tile.z * COLUMN_KC + 2;
+    if (i > g.nx - 1 || j > g.ny - 1) return;
+    const int k1 = k0 + COLUMN_KC - 1 < g.nz - 1 ? k0 + COLUMN_KC - 1 : g.nz - 1;
+    const int sy = g.sy, sz = g.sz;
+    double P[3][9];
+    int o = g.at(i, j, k0);
+    for (int q = 0; q < 9; q++) {
+        const int d = (q % 3 - 1) + sy * (q / 3 - 1);
+        P[0][q] = src[o - sz + d];
+        P[1][q] = src[o + d];
+    }
+    for (int k = k0; k <= k1; k++, o += sz) {
+        for (int q = 0; q < 9; q++) P[2][q] = src[o + sz + (q % 3 - 1) + sy * (q / 3 - 1)];
+        f(P, o);
+        for (int q = 0; q < 9; q++) {
+            P[0][q] = P[1][q];
+            P[1][q] = P[2][q];
+        }
+    }
+}
+
+template <class F>
+static int launch_column27(const Grid &g, hipStream_t s, const double *src, F f)
+{
+    if (g.nx < 3 || g.ny < 3 || g.nz < 3) return 0;
+    const dim3 grid((unsigned)((g.nx - 2 + BX - 1) / BX), (unsigned)((g.ny - 2 + BY - 1) / BY),
+                    (unsigned)((g.nz - 2 + COLUMN_KC - 1) / COLUMN_KC));
+    hipLaunchKernelGGL(k_column27<F>, grid, interior_block(), 0, s, g, src, f);
+    DG_CHECK_LAUNCH();
+    return 0;
+}
+
+#define DG_P27(di, dj, dk) P[(dk) + 1][((dj) + 1) * 3 + (di) + 1]
+
 int mg_filterc27(const Grid &g, hipStream_t s, const double *cc, double *cf)
 {
-    const int sy = g.sy, sz = g.sz;
-    return launch_interior(g, s, [=] __device__(int i, int j, int k) {
+    return launch_column27(g, s, cc, [=] __device__(const double (&P)[3][9], int o) {
         const double w0 = DG_F4(0.2729), w1 = DG_F4(0.08185), w2 = DG_F4(0.0178), w3 = DG_F4(.0028);
-        const int o = g.at(i, j, k);
-#define C(di, dj, dk) cc[o + (di) + sy * (dj) + sz * (dk)]
+#define C DG_P27
         double t6 = C(-1, 0, 0) + C(1, 0, 0) + C(0, -1, 0) + C(0, 1, 0) + C(0, 0, 1) + C(0, 0, -1);
         double t12 = C(-1, -1, 0) + C(-1, 1, 0) + C(1, -1, 0) + C(1, 1, 0) + C(0, 1, 1) + C(0, -1, 1) + C(1, 0, 1) +
                      C(-1, 0, 1) + C(0, 1, -1) + C(0, -1, -1) + C(1, 0, -1) + C(-1, 0, -1);
@@ -898,10 +946,9 @@ int op_csf(Ctx &c, bool w_interior)
     if (op_bc_out(g, s, ccf)) return -1;
     const double sigh = 0.5 * (c.s.tau * c.s.tau) * c.s.sigma / (c.s.h * c.s.h * c.s.h);
     const int sy = g.sy, sz = g.sz;
-    if (launch_interior(g, s, [=] __device__(int i, int j, int k) {
+    if (launch_column27(g, s, ccf, [=] __device__(const double (&P)[3][9], int o) {
             const double e50 = 1.0e-50;
-            const int o = g.at(i, j, k);
-#define F(di, dj, dk) ccf[o + (di) + sy * (dj) + sz * (dk)]
+#define F DG_P27
             double nxu = sqrt(sq(F(1, 0, 0) - F(0, 0, 0)) + 0.0625 * sq(F(0, 1, 0) - F(0, -1, 0) + F(1, 1, 0) - F(1, -1, 0)) +
                               0.0625 * sq(F(0, 0, 1) - F(0, 0, -1) + F(1, 0, 1) - F(1, 0, -1)) + e50);
             double nxd = sqrt(sq(F(0, 0, 0) - F(-1, 0, 0)) + 0.0625 * sq(F(0, 1, 0) - F(0, -1, 0) + F(-1, 1, 0) - F(-1, -1, 0)) +
@@ -998,14 +1045,24 @@ int mg_abcei(const Grid &g, hipStream_t s, double *a, double *b, double *c, doub
              double r2)
 {
     const int nx = g.nx, ny = g.ny, nz = g.nz, sy = g.sy, sz = g.sz;
+    // a, b, c and ei in one pass.  ei needs the coefficients of the faces
+    // below the cell too; the thread forms them again with the same
+    // expression, and at i = 2 takes bc_coef's a(1,j,k) = a(2,j,k).  The
+    // other two ghost coefficients it reads, b(i,1,k) and c(i,j,1), have the
+    // interior expression (main.f:4465-4494), so ei is what the reference's
+    // separate loops give, bit for bit.
     if (launch_interior(g, s, [=] __device__(int i, int j, int k) {
             const int o = g.at(i, j, k);
             const double na = 1.0 / (r1 * (cc[o] + cc[o + 1]) + r2);
             const double nb = 1.0 / (r1 * (cc[o] + cc[o + sy]) + r2);
             const double nc = 1.0 / (r1 * (cc[o] + cc[o + sz]) + r2);
+            const double aw = i > 2 ? 1.0 / (r1 * (cc[o - 1] + cc[o]) + r2) : na;
+            const double bs = 1.0 / (r1 * (cc[o - sy] + cc[o]) + r2);
+            const double cb = 1.0 / (r1 * (cc[o - sz] + cc[o]) + r2);
             a[o] = na;
             b[o] = nb;
             c[o] = nc;
+            ei[o] = 1.0 / (na + aw + nb + bs + nc + cb);
         }))
         return -1;
     // bc_coef (main.f:4465-4494): the three fills write different arrays
@@ -1020,14 +1077,9 @@ int mg_abcei(const Grid &g, hipStream_t s, double *a, double *b, double *c, doub
             b[g.at(i, 1, k)] = 1.0 / (r1 * (cc[g.at(i, 1, k)] + cc[g.at(i, 2, k)]) + r2);
         }))
         return -1;
-    if (launch_face(nx - 2, ny, s, [=] __device__(int i0, int j0) {
-            int i = i0 + 2, j = j0 + 1;
-            c[g.at(i, j, 1)] = 1.0 / (r1 * (cc[g.at(i, j, 1)] + cc[g.at(i, j, 2)]) + r2);
-        }))
-        return -1;
-    return launch_interior(g, s, [=] __device__(int i, int j, int k) {
-        const int o = g.at(i, j, k);
-        ei[o] = 1.0 / (a[o] + a[o - 1] + b[o] + b[o - sy] + c[o] + c[o - sz]);
+    return launch_face(nx - 2, ny, s, [=] __device__(int i0, int j0) {
+        int i = i0 + 2, j = j0 + 1;
+        c[g.at(i, j, 1)] = 1.0 / (r1 * (cc[g.at(i, j, 1)] + cc[g.at(i, j, 2)]) + r2);
     });
 }
 

@@ -111,36 +111,38 @@ int mg_addint(const Grid &gf, hipStream_t s, double *uf, const double *uc, doubl
     return launch_linear(gf.n(), s, [=] __device__(size_t p) { uf[p] = uf[p] + res[p]; });
 }
 
+// acfromac (main.f:3793-3844) in one pass over the coarse interior: each
+// cell forms its three face averages and, for ei, those of the faces below
+// it again (the same four-term sums in the same order); the cells next to
+// the lower ghost faces also write the ghost-face averages a(1,j,k),
+// b(i,1,k) and c(i,j,1) that the reference's loops include.
 int mg_acfromac(const Grid &gc, hipStream_t s, double *a, double *b, double *c, double *ei, const double *a2,
                 const double *b2, const double *c2)
 {
     const Grid gf = make_grid(2 * gc.nx - 2, 2 * gc.ny - 2, 2 * gc.nz - 2);
-    const int nx = gc.nx, ny = gc.ny, nz = gc.nz;
-    // a: i = 1..nx-1, b: j = 1..ny-1, c: k = 1..nz-1 (interior in the others)
-    const Grid ga = make_grid(nx + 1, ny, nz);  // index range i = 1..nx-1 through i+... handled below
-    (void)ga;
-    if (launch_face((nx - 1) * (ny - 2), nz - 2, s, [=] __device__(int p, int k0) {
-            int i = p % (nx - 1) + 1, j = p / (nx - 1) + 2, k = k0 + 2;
-            a[gc.at(i, j, k)] = 0.25 * (a2[gf.at(2 * i - 1, 2 * j - 1, 2 * k - 1)] + a2[gf.at(2 * i - 1, 2 * j - 2, 2 * k - 1)] +
-                                        a2[gf.at(2 * i - 1, 2 * j - 1, 2 * k - 2)] + a2[gf.at(2 * i - 1, 2 * j - 2, 2 * k - 2)]);
-        }))
-        return -1;
-    if (launch_face((nx - 2) * (ny - 1), nz - 2, s, [=] __device__(int p, int k0) {
-            int i = p % (nx - 2) + 2, j = p / (nx - 2) + 1, k = k0 + 2;
-            b[gc.at(i, j, k)] = 0.25 * (b2[gf.at(2 * i - 1, 2 * j - 1, 2 * k - 1)] + b2[gf.at(2 * i - 2, 2 * j - 1, 2 * k - 1)] +
-                                        b2[gf.at(2 * i - 1, 2 * j - 1, 2 * k - 2)] + b2[gf.at(2 * i - 2, 2 * j - 1, 2 * k - 2)]);
-        }))
-        return -1;
-    if (launch_face((nx - 2) * (ny - 2), nz - 1, s, [=] __device__(int p, int k0) {
-            int i = p % (nx - 2) + 2, j = p / (nx - 2) + 2, k = k0 + 1;
-            c[gc.at(i, j, k)] = 0.25 * (c2[gf.at(2 * i - 1, 2 * j - 1, 2 * k - 1)] + c2[gf.at(2 * i - 2, 2 * j - 1, 2 * k - 1)] +
-                                        c2[gf.at(2 * i - 1, 2 * j - 2, 2 * k - 1)] + c2[gf.at(2 * i - 2, 2 * j - 2, 2 * k - 1)]);
-        }))
-        return -1;
     const int sy = gc.sy, sz = gc.sz;
     return launch_interior(gc, s, [=] __device__(int i, int j, int k) {
+        auto A = [&](int ii) {
+            return 0.25 * (a2[gf.at(2 * ii - 1, 2 * j - 1, 2 * k - 1)] + a2[gf.at(2 * ii - 1, 2 * j - 2, 2 * k - 1)] +
+                           a2[gf.at(2 * ii - 1, 2 * j - 1, 2 * k - 2)] + a2[gf.at(2 * ii - 1, 2 * j - 2, 2 * k - 2)]);
+        };
+        auto B = [&](int jj) {
+            return 0.25 * (b2[gf.at(2 * i - 1, 2 * jj - 1, 2 * k - 1)] + b2[gf.at(2 * i - 2, 2 * jj - 1, 2 * k - 1)] +
+                           b2[gf.at(2 * i - 1, 2 * jj - 1, 2 * k - 2)] + b2[gf.at(2 * i - 2, 2 * jj - 1, 2 * k - 2)]);
+        };
+        auto C = [&](int kk) {
+            return 0.25 * (c2[gf.at(2 * i - 1, 2 * j - 1, 2 * kk - 1)] + c2[gf.at(2 * i - 2, 2 * j - 1, 2 * kk - 1)] +
+                           c2[gf.at(2 * i - 1, 2 * j - 2, 2 * kk - 1)] + c2[gf.at(2 * i - 2, 2 * j - 2, 2 * kk - 1)]);
+        };
         const int o = gc.at(i, j, k);
-        ei[o] = 1.0 / (a[o] + a[o - 1] + b[o] + b[o - sy] + c[o] + c[o - sz]);
+        const double av = A(i), aw = A(i - 1), bv = B(j), bs = B(j - 1), cv = C(k), cb = C(k - 1);
+        a[o] = av;
+        b[o] = bv;
+        c[o] = cv;
+        if (i == 2) a[o - 1] = aw;
+        if (j == 2) b[o - sy] = bs;
+        if (k == 2) c[o - sz] = cb;
+        ei[o] = 1.0 / (av + aw + bv + bs + cv + cb);
     });
 }
 

@@ -440,14 +440,14 @@ int plic_fl3d_batch(int n, hipStream_t s, const double *m1, const double *m2, co
 // a1 = us(o) tauh, a2 = us(o + sd) tauh.  want selects the fluxes formed
 // (1: f1, 2: f2, 4: f3); each is formed exactly as when all three are, and
 // the Youngs normal and the plane are skipped when no wanted flux needs them.
-__device__ inline void swp_fluxes(const double *cc, const Grid g, int o, double a1, double a2, int d, int guard,
-                                  int want, double &f1, double &f2, double &f3)
+// c0 is cc(o).
+__device__ inline void swp_fluxes(const double *cc, const Grid g, int o, double c0, double a1, double a2, int d,
+                                  int guard, int want, double &f1, double &f2, double &f3)
 {
     const double tiny = DG_F4(1.e-30);
     f1 = 0.0;
     f2 = 0.0;
     f3 = 0.0;
-    const double c0 = cc[o];
 #define C(di, dj, dk) cc[o + (di) + g.sy * (dj) + g.sz * (dk)]
     if (c0 == 1.0) {
         f1 = dmax1(-a1, 0.0);
@@ -513,7 +513,10 @@ __device__ inline void swp_fluxes(const double *cc, const Grid g, int o, double 
 // velocity points at them do PLIC work twice, so the pass moves cc, the
 // velocity and the new cc (24 B per cell against 72).  Out of place: every
 // thread reads the old cc; ghost cells are copied.  The neighbour fluxes
-// across the domain's ends are bc_flux's zeros (main.f:4496-4539).
+// across the domain's ends are bc_flux's zeros (main.f:4496-4539).  One
+// cell per thread in the dispatcher's own tile order, which deals the few
+// tiles with interface cells (and their PLIC work) over all XCDs: walking
+// columns in XCD-contiguous tiles fetched 30% less but took 1.7x as long.
 template <int D>
 __global__ void __launch_bounds__(256) k_swp_fused(Grid g, const double *us, const double *cc, double *ccn,
                                                    double tauh, int guard)
@@ -531,9 +534,9 @@ __global__ void __launch_bounds__(256) k_swp_fused(Grid g, const double *us, con
     const int id = D == 1 ? i : (D == 2 ? j : k), nd = D == 1 ? g.nx : (D == 2 ? g.ny : g.nz);
     const double u0 = us[o] * tauh, up = us[o + sd] * tauh;
     double f1, f2, f3, fb = 0.0, fa = 0.0, t1, t2;
-    swp_fluxes(cc, g, o, u0, up, D, guard, 2, f1, f2, f3);
-    if (id > 2) swp_fluxes(cc, g, o - sd, us[o - sd] * tauh, u0, D, guard, 4, t1, t2, fb);
-    if (id < nd - 1) swp_fluxes(cc, g, o + sd, up, us[o + 2 * sd] * tauh, D, guard, 1, fa, t1, t2);
+    swp_fluxes(cc, g, o, cc[o], u0, up, D, guard, 2, f1, f2, f3);
+    if (id > 2) swp_fluxes(cc, g, o - sd, cc[o - sd], us[o - sd] * tauh, u0, D, guard, 4, t1, t2, fb);
+    if (id < nd - 1) swp_fluxes(cc, g, o + sd, cc[o + sd], up, us[o + 2 * sd] * tauh, D, guard, 1, fa, t1, t2);
     const double v = fb + f2 + fa;
     ccn[o] = dmax1(0.0, dmin1(1.0, v));
 }
@@ -567,7 +570,7 @@ static int swp(Ctx &c, const double *us, double *cc, double *vof1, double *vof2,
     if (launch_interior_rr(g, c.stream, [=] __device__(int i, int j, int k) {
             const int o = g.at(i, j, k);
             double f1, f2, f3;
-            swp_fluxes(cc, g, o, us[o] * tauh, us[o + sd] * tauh, d, guard, 7, f1, f2, f3);
+            swp_fluxes(cc, g, o, cc[o], us[o] * tauh, us[o + sd] * tauh, d, guard, 7, f1, f2, f3);
             vof1[o] = f1;
             vof2[o] = f2;
             vof3[o] = f3;
@@ -1019,7 +1022,9 @@ static int stress(Ctx &c)
     const int sy = g.sy, sz = g.sz;
     // every value is computed before the first store: the compiler cannot
     // tell the stores from the input arrays, and a store between two loads
-    // would make it reload and serialise them
+    // would make it reload and serialise them.  (Walking columns with the
+    // plane below in registers fetched 15% less but took longer: fewer
+    // waves in flight for a kernel that streams 16 arrays.)
     return launch_interior(g, c.stream, [=] __device__(int i, int j, int k) {
         const int o = g.at(i, j, k);
         double temp = r2 + r1 * cc[o];
@@ -1104,18 +1109,94 @@ static int predictor(Ctx &c, double *w1, double *w2, double *w3)
     });
 }
 
+// convective's stencil points of w1, w2, w3 within each plane, in window
+// order: w1 (0,0) (-1,0) (1,0) (0,-1) (0,1) (1,-1); w2 (0,0) (-1,0) (1,0)
+// (0,-1) (0,1) (-1,1); w3 (0,0) (-1,0) (1,0) (0,-1) (0,1)
+constexpr int CV_DI[6] = {0, -1, 1, 0, 0, 1}, CV_DJ1[6] = {0, 0, 0, -1, 1, -1}, CV_DI2[6] = {0, -1, 1, 0, 0, -1},
+              CV_DJ2[6] = {0, 0, 0, -1, 1, 1};
+__host__ __device__ constexpr int cv_i1(int di, int dj)
+{
+    return dj == 0 ? (di == 0 ? 0 : di < 0 ? 1 : 2) : dj < 0 ? (di == 0 ? 3 : 5) : 4;
+}
+__host__ __device__ constexpr int cv_i2(int di, int dj)
+{
+    return dj == 0 ? (di == 0 ? 0 : di < 0 ? 1 : 2) : dj < 0 ? 3 : (di == 0 ? 4 : 5);
+}
+__host__ __device__ constexpr int cv_i3(int di, int dj) { return cv_i1(di, dj); }
+static_assert(cv_i1(1, -1) == 5 && cv_i1(0, 1) == 4 && cv_i2(-1, 1) == 5 && cv_i2(0, -1) == 3, "window order");
+
+// the three planes k-1, k, k+1 of the window around the cell
+struct CvWin {
+    double p1[3][6], p2[3][6], p3[3][5];
+};
+
+// f(W, o) on the interior with each thread walking COLUMN_KC planes up one
+// column: the planes below and at the cell stay in registers from the
+// previous step, so each plane of w1, w2, w3 is fetched once per column
+// instead of once per plane that reads it (the one-cell-per-thread form
+// fetched each about three times: a plane's tiles in flight on an XCD need
+// three planes of three arrays, more than its L2)
+template <class F>
+__global__ void __launch_bounds__(256) k_convective_col(Grid g, const double *w1, const double *w2,
+                                                        const double *w3, F f)
+{
+    const uint3 tile = xcd_tile();
+    const int i = (int)(tile.x * blockDim.x + threadIdx.x) + 2;
+    const int j = (int)(tile.y * blockDim.y + threadIdx.y) + 2;
+    const int k0 = (int)tile.z * COLUMN_KC + 2;
+    if (i > g.nx - 1 || j > g.ny - 1) return;
+    const int k1 = k0 + COLUMN_KC - 1 < g.nz - 1 ? k0 + COLUMN_KC - 1 : g.nz - 1;
+    const int sy = g.sy, sz = g.sz;
+    CvWin W;
+    auto load = [&](int slot, int oo) {
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            W.p1[slot][q] = w1[oo + CV_DI[q] + sy * CV_DJ1[q]];
+            W.p2[slot][q] = w2[oo + CV_DI2[q] + sy * CV_DJ2[q]];
+        }
+#pragma unroll
+        for (int q = 0; q < 5; q++) W.p3[slot][q] = w3[oo + CV_DI[q] + sy * CV_DJ1[q]];
+    };
+    int o = g.at(i, j, k0);
+    load(0, o - sz);
+    load(1, o);
+    for (int k = k0; k <= k1; k++, o += sz) {
+        load(2, o + sz);
+        f(W, o);
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            W.p1[0][q] = W.p1[1][q]; W.p1[1][q] = W.p1[2][q];
+            W.p2[0][q] = W.p2[1][q]; W.p2[1][q] = W.p2[2][q];
+        }
+#pragma unroll
+        for (int q = 0; q < 5; q++) {
+            W.p3[0][q] = W.p3[1][q]; W.p3[1][q] = W.p3[2][q];
+        }
+    }
+}
+
+template <class F>
+static int launch_convective_col(const Grid &g, hipStream_t s, const double *w1, const double *w2, const double *w3,
+                                 F f)
+{
+    if (g.nx < 3 || g.ny < 3 || g.nz < 3) return 0;
+    const dim3 grid((unsigned)((g.nx - 2 + BX - 1) / BX), (unsigned)((g.ny - 2 + BY - 1) / BY),
+                    (unsigned)((g.nz - 2 + COLUMN_KC - 1) / COLUMN_KC));
+    hipLaunchKernelGGL(k_convective_col<F>, grid, interior_block(), 0, s, g, w1, w2, w3, f);
+    DG_CHECK_LAUNCH();
+    return 0;
+}
+
 static int convective(Ctx &c, const double *w1, const double *w2, const double *w3)
 {
     const Grid g = c.g;
     double *u = c.u, *v = c.v, *w = c.w;
     const double tauh = c.s.tauh;
-    const int sy = g.sy, sz = g.sz;
     if (c.s.convectype == 1) {
-        return launch_interior(g, c.stream, [=] __device__(int i, int j, int k) {
-            const int o = g.at(i, j, k);
-#define A1(di, dj, dk) w1[o + (di) + sy * (dj) + sz * (dk)]
-#define A2(di, dj, dk) w2[o + (di) + sy * (dj) + sz * (dk)]
-#define A3(di, dj, dk) w3[o + (di) + sy * (dj) + sz * (dk)]
+        return launch_convective_col(g, c.stream, w1, w2, w3, [=] __device__(const CvWin &W, int o) {
+#define A1(di, dj, dk) W.p1[(dk) + 1][cv_i1(di, dj)]
+#define A2(di, dj, dk) W.p2[(dk) + 1][cv_i2(di, dj)]
+#define A3(di, dj, dk) W.p3[(dk) + 1][cv_i3(di, dj)]
             double t1, t2, s1, s2, q1, q2;
             t1 = .5 * (A2(0, 0, 0) + A2(-1, 0, 0));
             t2 = .5 * (A2(0, 1, 0) + A2(-1, 1, 0));
@@ -1158,8 +1239,7 @@ static int convective(Ctx &c, const double *w1, const double *w2, const double *
             w[o] = nw;
         });
     } else if (c.s.convectype == 2) {
-        return launch_interior(g, c.stream, [=] __device__(int i, int j, int k) {
-            const int o = g.at(i, j, k);
+        return launch_convective_col(g, c.stream, w1, w2, w3, [=] __device__(const CvWin &W, int o) {
             const double nu = A1(0, 0, 0) +
                    tauh * (0.25 * (sq(A1(0, 0, 0) + A1(-1, 0, 0)) - sq(A1(1, 0, 0) + A1(0, 0, 0)) +
                                    (A1(0, -1, 0) + A1(0, 0, 0)) * (A2(0, 0, 0) + A2(-1, 0, 0)) -

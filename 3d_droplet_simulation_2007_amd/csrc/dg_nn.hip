@@ -148,7 +148,7 @@ struct Net {
     long accepted = 0, rejected = 0, skipped = 0;
     double last_alpha = 0.0, last_gain = 1.0, gain_sum = 0.0;
     // fallback controller
-    int probe_every = 8, skip_left = 0;
+    int probe_every = 8, skip_left = 0, weak_run = 0;
     double min_gain = 4.0;
     int eval = 0;
     int eval_iters[3] = {0, 0, 0};
@@ -613,8 +613,9 @@ int guess_solve_begin(Ctx &c, double *x, double *r, double h2i, bool apply, doub
     }
     const bool capture = N.capture && N.st.cap > 0;
     // fallback controller: after a learned step that cut |r|^2 by less than
-    // min_gain, the next probe_every solves use the plain guess alone
-    // (evaluation mode always runs the network)
+    // min_gain, the next probe_every * 2^(w-1) solves use the plain guess
+    // alone, w the number of weak steps in a row (at most 16 probe_every);
+    // evaluation mode always runs the network
     if (apply && !N.eval && N.skip_left > 0) {
         N.skip_left--;
         N.skipped++;
@@ -701,7 +702,15 @@ int guess_solve_begin(Ctx &c, double *x, double *r, double h2i, bool apply, doub
         N.accepted++;
     else
         N.rejected++;
-    if (!N.eval && N.probe_every > 0 && N.last_gain < N.min_gain) N.skip_left = N.probe_every;
+    if (!N.eval && N.probe_every > 0) {
+        if (N.last_gain < N.min_gain) {
+            N.weak_run = N.weak_run < 5 ? N.weak_run + 1 : 5;
+            const long long sk = (long long)N.probe_every << (N.weak_run - 1);
+            N.skip_left = sk > 0x7fffffffLL ? 0x7fffffff : (int)sk;
+        } else {
+            N.weak_run = 0;
+        }
+    }
     return 0;
 }
 
@@ -1077,6 +1086,7 @@ int dg_guess_set_controller(const int *probe_every, const double *min_gain)
     N.probe_every = *probe_every;
     N.min_gain = *min_gain;
     N.skip_left = 0;
+    N.weak_run = 0;
     return 0;
 }
 

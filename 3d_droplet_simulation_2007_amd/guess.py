@@ -140,7 +140,8 @@ def stats():
 
 def set_controller(probe_every=8, min_gain=4.0):
     """Fallback controller: a learned step dividing |r|^2 by less than
-    min_gain sends the next probe_every solves to the zero guess alone."""
+    min_gain sends the next probe_every * 2^(w-1) solves to the zero guess
+    alone, w the number of such steps in a row (capped at 16 probe_every)."""
     pe, mg = ctypes.c_int(int(probe_every)), ctypes.c_double(float(min_gain))
     _check(lib().dg_guess_set_controller(ctypes.byref(pe), ctypes.byref(mg)), "dg_guess_set_controller")
 

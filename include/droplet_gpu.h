@@ -196,8 +196,10 @@ int dg_guess_train(const int *epochs, const float *lr, const int *seed, double *
  * |r|^2 before / after the learned step over the solves that ran it */
 int dg_guess_stats(int *accepted, int *rejected, int *skipped, double *last_alpha, double *mean_gain);
 /* Fallback controller: a learned step that divides |r|^2 by less than
- * min_gain sends the next probe_every solves to the zero guess alone
- * (probe_every 0: the network runs on every solve).  Default 8, 4. */
+ * min_gain sends the next probe_every * 2^(w-1) solves to the zero guess
+ * alone, w the number of such steps in a row (capped at 16 probe_every; a
+ * step that reaches min_gain resets w).  probe_every 0: the network runs
+ * on every solve.  Default 8, 4. */
 int dg_guess_set_controller(const int *probe_every, const double *min_gain);
 /* on = 1: each solve runs from the zero, previous and learned guesses in
  * turn (same system); dg_guess_eval_iters returns the three counts of the

@@ -149,7 +149,7 @@ static void free_ctx(Ctx &c)
         L = Level();
     }
     void *fs[] = {c.u, c.v, c.w, c.s11, c.s22, c.s33, c.s12, c.s13, c.s23, c.rhon, c.mun, c.mut, c.ktur,
-                  c.eps, c.cmu, c.ccf, c.pprev, c.pcg_r, c.pcg_z, c.pcg_p, c.pcg_q, c.red_partial, c.dscal};
+                  c.eps, c.cmu, c.ccf, c.pprev, c.pcg_r, c.pcg_p, c.pcg_q, c.red_partial, c.dscal};
     for (void *p : fs)
         if (p) (void)hipFree(p);
     for (int q = 0; q < 2; q++)

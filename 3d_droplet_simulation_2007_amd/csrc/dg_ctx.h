@@ -49,7 +49,7 @@ struct Ctx {
     double *rhon = nullptr, *mun = nullptr, *mut = nullptr, *ktur = nullptr, *eps = nullptr, *cmu = nullptr;
     double *ccf = nullptr;     // csf's filtered volume fraction (automatic array in csf)
     double *pprev = nullptr;   // pressure of the previous step (tolerance-mode guess)
-    double *pcg_r = nullptr, *pcg_z = nullptr, *pcg_p = nullptr, *pcg_q = nullptr;
+    double *pcg_r = nullptr, *pcg_p = nullptr, *pcg_q = nullptr;  // pcg_p: the ring of search directions
     double *red_partial = nullptr;  // RED_BLOCKS doubles
     double *dscal = nullptr;        // small device scalar scratch (64 doubles)
     double *hscal = nullptr;        // pinned host mirror of dscal

@@ -662,7 +662,16 @@ __global__ void __launch_bounds__(256) k_final2(const double *partial, int nb, d
 }
 
 // device scalar slots
-enum { S_RZ = 16, S_ZQ, S_RZOLD, S_ALPHA, S_PQ, S_RR, S_BB, S_FIRST };
+enum { S_RZ = 16, S_ZQ, S_RZOLD, S_ALPHA, S_PQ, S_RR, S_BB, S_FIRST, S_IT, S_AH = 32 };
+
+// The search directions of a solve go to a ring of PCG_HIST buffers and
+// their step lengths to ds[S_AH ..]; x takes x + alpha_j p_j for the
+// buffered j in iteration order when the ring is full and at the end of
+// the solve (k_pcg_flush), the same additions in the same order as one
+// x += alpha p per iteration, without reading and writing x every
+// iteration.  ds[S_IT]: the solve's iteration count, read by the kernels.
+constexpr int PCG_HIST = 16;
+static_assert(S_AH + PCG_HIST <= 64, "device scalar slots");
 
 // K2: p_new = z + beta p_old (beta from device scalars), q = A p_new,
 // partial sums of p_new . q
@@ -679,10 +688,13 @@ constexpr int PCG_KCH = 8;
 // of the search direction; p_new is rounded to it before the stencil, so q
 // is A applied to the stored direction and x, r stay consistent.
 template <class Z, typename PT>
-__global__ void __launch_bounds__(256) k_pcg_dir(Grid g, Z z, const PT *pold, PT *pnew, double *q,
-                                                 const double *a, const double *b, const double *c, double h2i,
-                                                 const double *ds, double *partial)
+__global__ void __launch_bounds__(256) k_pcg_dir(Grid g, Z z, PT *hist, size_t nslot, double *q, const double *a,
+                                                 const double *b, const double *c, double h2i, const double *ds,
+                                                 double *partial)
 {
+    const int it = (int)ds[S_IT];
+    PT *pnew = hist + (size_t)(it % PCG_HIST) * nslot;
+    const PT *pold = hist + (size_t)((it + PCG_HIST - 1) % PCG_HIST) * nslot;
     constexpr int TW = 66, TH = 6;  // tile + ring
     __shared__ double pl[2][TH][TW];
     __shared__ double lds4[4];
@@ -758,8 +770,8 @@ __global__ void __launch_bounds__(256) k_pcg_dir(Grid g, Z z, const PT *pold, PT
 // (fs != null: also the FP32 red-black split copy of the new r, the next
 // V-cycle's right-hand side)
 template <typename PT>
-__global__ void __launch_bounds__(256) k_pcg_update(Grid g, double *x, double *r, const PT *p, const double *q,
-                                                    double *ds, double *partial, float *fs, RbGeom rg)
+__global__ void __launch_bounds__(256) k_pcg_update(Grid g, double *r, const double *q, double *ds, double *partial,
+                                                    float *fs, RbGeom rg)
 {
     __shared__ double lds4[4];
     unsigned lin;
@@ -773,7 +785,6 @@ __global__ void __launch_bounds__(256) k_pcg_update(Grid g, double *x, double *r
     double contrib = 0.0;
     if (i <= g.nx - 1 && j <= g.ny - 1) {
         const int o = g.at(i, j, k);
-        x[o] = x[o] + alpha * (double)p[o];
         const double rn = r[o] - alpha * q[o];
         r[o] = rn;
         if (fs) fs[rg.at32(i, j, k)] = (float)rn;
@@ -826,9 +837,24 @@ static int sum_partials(hipStream_t s, const double *partial, int nb, int stride
 // scalar bookkeeping after K3: alpha_old = rz/pq, rz_old = rz, first = 0
 __global__ void k_pcg_scalars(double *ds)
 {
+    const int it = (int)ds[S_IT];
     ds[S_ALPHA] = ds[S_RZ] / ds[S_PQ];
+    ds[S_AH + it % PCG_HIST] = ds[S_ALPHA];
+    ds[S_IT] = (double)(it + 1);
     ds[S_RZOLD] = ds[S_RZ];
     ds[S_FIRST] = 0.0;
+}
+
+// x += alpha_j p_j for the first count buffers of the ring, in order
+template <typename PT>
+__global__ void __launch_bounds__(256) k_pcg_flush(Grid g, double *x, const PT *hist, size_t nslot, const double *ds,
+                                                   int count)
+{
+    DG_INTERIOR_IJK(g);
+    const int o = g.at(i, j, k);
+    double v = x[o];
+    for (int q = 0; q < count; q++) v = v + ds[S_AH + q] * (double)hist[(size_t)q * nslot + o];
+    x[o] = v;
 }
 
 // the previous-step guess is dropped for the zero guess when its residual
@@ -935,12 +961,10 @@ static int ensure_tol_buffers(Ctx &c)
     const size_t nfine = c.g.n();
     if (!c.pcg_r) {
         DG_HIP(hipMalloc(&c.pcg_r, nfine * sizeof(double)));
-        DG_HIP(hipMalloc(&c.pcg_z, nfine * sizeof(double)));
-        DG_HIP(hipMalloc(&c.pcg_p, nfine * sizeof(double)));
+        DG_HIP(hipMalloc(&c.pcg_p, PCG_HIST * nfine * sizeof(double)));  // the direction ring
         DG_HIP(hipMalloc(&c.pcg_q, nfine * sizeof(double)));
         DG_HIP(hipMemset(c.pcg_r, 0, nfine * sizeof(double)));
-        DG_HIP(hipMemset(c.pcg_z, 0, nfine * sizeof(double)));
-        DG_HIP(hipMemset(c.pcg_p, 0, nfine * sizeof(double)));
+        DG_HIP(hipMemset(c.pcg_p, 0, PCG_HIST * nfine * sizeof(double)));
         DG_HIP(hipMemset(c.pcg_q, 0, nfine * sizeof(double)));
     }
     const Grid g0 = c.lev[0].g;
@@ -959,17 +983,19 @@ static int ensure_tol_buffers(Ctx &c)
 }
 
 // Enqueues one PCG iteration on c.stream (no host synchronisation): z = M r
-// (one V-cycle), r.z and z.q_old, p_new = z + beta p_old and q = A p_new,
-// p.q, x += alpha p, r -= alpha q, r.r, scalar bookkeeping, and the copy of
-// r.r to pinned host memory.  pa/pb: search-direction ping-pong buffers.
+// (one V-cycle), r.z and z.q_old, p_new = z + beta p_old into the
+// direction ring and q = A p_new, p.q, r -= alpha q, r.r, scalar
+// bookkeeping (alpha into the ring's step lengths), and the copy of r.r to
+// pinned host memory.  x is updated by k_pcg_flush.
 template <typename T>
-static int pcg_iteration(Ctx &c, double *pa, double *pb)
+static int pcg_iteration(Ctx &c)
 {
     hipStream_t s = c.stream;
     Level &F = c.lev[c.ng - 1];
     const Grid g = F.g;
     const double h2i = lev_h2i(c, g);
-    double *x = F.u, *r = c.pcg_r, *q = c.pcg_q, *ds = c.dscal, *partial = c.red_partial;
+    double *r = c.pcg_r, *q = c.pcg_q, *ds = c.dscal, *partial = c.red_partial;
+    const size_t nslot = g.n();
     const dim3 ig = interior_grid(g), ib = interior_block();
     const int nblk = (int)(ig.x * ig.y * ig.z);
     const dim3 dgr(ig.x, ig.y, (unsigned)((g.nz - 2 + PCG_KCH - 1) / PCG_KCH));  // k_pcg_dir: columns of planes
@@ -993,8 +1019,8 @@ static int pcg_iteration(Ctx &c, double *pa, double *pb)
         hipLaunchKernelGGL(k_final2, dim3(1), dim3(256), 0, s, partial, RED_BLOCKS, ds + S_RZ);
         DG_CHECK_LAUNCH();
         prof_begin(PK_PCG_DIR, s);
-        hipLaunchKernelGGL((k_pcg_dir<ZSplit, PT>), dgr, ib, 0, s, g, z, (const PT *)pa, (PT *)pb, q, F.a, F.b, F.c,
-                           h2i, ds, partial);
+        hipLaunchKernelGGL((k_pcg_dir<ZSplit, PT>), dgr, ib, 0, s, g, z, (PT *)c.pcg_p, nslot, q, F.a, F.b, F.c, h2i,
+                           ds, partial);
         prof_end(PK_PCG_DIR, s, (4.0 + 2.0 * sizeof(PT) + 4.0 * 8.0) * ncell);
         DG_CHECK_LAUNCH();
     } else {
@@ -1011,16 +1037,15 @@ static int pcg_iteration(Ctx &c, double *pa, double *pb)
         hipLaunchKernelGGL(k_final2, dim3(1), dim3(256), 0, s, partial, RED_BLOCKS, ds + S_RZ);
         DG_CHECK_LAUNCH();
         prof_begin(PK_PCG_DIR, s);
-        hipLaunchKernelGGL((k_pcg_dir<ZNatural<T>, PT>), dgr, ib, 0, s, g, z, (const PT *)pa, (PT *)pb, q, F.a, F.b,
-                           F.c, h2i, ds, partial);
+        hipLaunchKernelGGL((k_pcg_dir<ZNatural<T>, PT>), dgr, ib, 0, s, g, z, (PT *)c.pcg_p, nslot, q, F.a, F.b, F.c,
+                           h2i, ds, partial);
         prof_end(PK_PCG_DIR, s, ((double)sizeof(T) + 2.0 * sizeof(PT) + 4.0 * 8.0) * ncell);
         DG_CHECK_LAUNCH();
     }
     if (sum_partials(s, partial, nblk_dir, 1, partial + 2 * (size_t)nblk, ds + S_PQ)) return -1;
     prof_begin(PK_PCG_UPDATE, s);
-    hipLaunchKernelGGL(k_pcg_update<PT>, ig, ib, 0, s, g, x, r, (const PT *)pb, q, ds, partial,
-                       split ? F.rb_f : nullptr, rb_geom(g));
-    prof_end(PK_PCG_UPDATE, s, (5.0 * 8.0 + sizeof(PT) + (split ? 4.0 : 0.0)) * ncell);
+    hipLaunchKernelGGL(k_pcg_update<PT>, ig, ib, 0, s, g, r, q, ds, partial, split ? F.rb_f : nullptr, rb_geom(g));
+    prof_end(PK_PCG_UPDATE, s, (3.0 * 8.0 + (split ? 4.0 : 0.0)) * ncell);
     DG_CHECK_LAUNCH();
     if (sum_partials(s, partial, nblk, 1, partial + 2 * (size_t)nblk, ds + S_RR)) return -1;
     hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(1), 0, s, ds);
@@ -1032,8 +1057,9 @@ static int pcg_iteration(Ctx &c, double *pa, double *pb)
 // The iteration's kernel sequence and arguments depend only on the level
 // layout, the buffers and the solver options, so it is captured once per
 // (mode, nu1, nu2, fused) and replayed: one graph launch per iteration
-// instead of ~10 launches per level.  Two graphs: the search-direction
-// buffers swap roles every iteration.
+// instead of ~10 launches per level.  The kernels take the direction
+// ring's slots from the device iteration count, so one graph serves every
+// iteration.
 template <typename T>
 static int pcg_graphs(Ctx &c)
 {
@@ -1042,7 +1068,7 @@ static int pcg_graphs(Ctx &c)
     const int key[12] = {c.mg_mode, c.mg_nu1, c.mg_nu2, c.fused_smoother, c.ng, (int)c.g.nx,
                          rb, rb ? c.lev[c.ng - 1].rb_coef16 : -1, rb ? c.rb_tail : -1, c.mg_nuc,
                          (int)llround(c.mg_omega * 1e9), (int)c.g.nz};
-    bool ok = c.pcg_graph[0] && c.pcg_graph[1];
+    bool ok = c.pcg_graph[0] != nullptr;
     for (int q = 0; q < 12 && ok; q++) ok = c.pcg_graph_key[q] == key[q];
     if (ok) return 0;
     for (int q = 0; q < 2; q++) {
@@ -1051,18 +1077,17 @@ static int pcg_graphs(Ctx &c)
             c.pcg_graph[q] = nullptr;
         }
     }
-    for (int par = 0; par < 2; par++) {
-        double *pa = par == 0 ? c.pcg_p : c.pcg_z, *pb = par == 0 ? c.pcg_z : c.pcg_p;
+    {
         hipGraph_t graph;
         DG_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-        int rc = pcg_iteration<T>(c, pa, pb);
+        int rc = pcg_iteration<T>(c);
         hipError_t e = hipStreamEndCapture(c.stream, &graph);
         if (rc) return -1;
         if (e != hipSuccess) {
             set_error("PCG graph capture: %s", hipGetErrorString(e));
             return -1;
         }
-        DG_HIP(hipGraphInstantiate(&c.pcg_graph[par], graph, nullptr, nullptr, 0));
+        DG_HIP(hipGraphInstantiate(&c.pcg_graph[0], graph, nullptr, nullptr, 0));
         DG_HIP(hipGraphDestroy(graph));
     }
     for (int q = 0; q < 12; q++) c.pcg_graph_key[q] = key[q];
@@ -1113,7 +1138,8 @@ static int pcg_solve(Ctx &c)
                            g_tw.info);
         DG_CHECK_LAUNCH();
         c.hscal[S_FIRST] = 1.0;
-        DG_HIP(hipMemcpyAsync(ds + S_FIRST, c.hscal + S_FIRST, sizeof(double), hipMemcpyHostToDevice, s));
+        c.hscal[S_IT] = 0.0;
+        DG_HIP(hipMemcpyAsync(ds + S_FIRST, c.hscal + S_FIRST, 2 * sizeof(double), hipMemcpyHostToDevice, s));
     }
     DG_HIP(hipMemcpyAsync(c.hscal + S_RR, ds + S_RR, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
     DG_HIP(hipStreamSynchronize(s));
@@ -1122,17 +1148,30 @@ static int pcg_solve(Ctx &c)
     double rel = sqrt(c.hscal[S_RR]) / bnorm;
     const bool graphs = c.use_graphs && prof().kernel == PK_NONE;
     if (graphs && pcg_graphs<T>(c)) return -1;
+    // x += alpha_j p_j over the ring's first count slots
+    using PT = T;
+    auto flush = [&](int count) {
+        hipLaunchKernelGGL(k_pcg_flush<PT>, ig, ib, 0, s, g, x, (const PT *)c.pcg_p, g.n(), ds, count);
+        return hipGetLastError() == hipSuccess ? 0 : -1;
+    };
     int it = 0;
     while (rel >= c.mg_tol && it < c.mg_maxit) {
         if (graphs) {
-            DG_HIP(hipGraphLaunch(c.pcg_graph[it & 1], s));
+            DG_HIP(hipGraphLaunch(c.pcg_graph[0], s));
         } else {
-            double *pa = (it & 1) ? c.pcg_z : c.pcg_p, *pb = (it & 1) ? c.pcg_p : c.pcg_z;
-            if (pcg_iteration<T>(c, pa, pb)) return -1;
+            if (pcg_iteration<T>(c)) return -1;
+        }
+        it++;
+        if (it % PCG_HIST == 0 && flush(PCG_HIST)) {
+            set_error("k_pcg_flush launch failed");
+            return -1;
         }
         DG_HIP(hipStreamSynchronize(s));
         rel = sqrt(c.hscal[S_RR]) / bnorm;
-        it++;
+    }
+    if (it % PCG_HIST != 0 && flush(it % PCG_HIST)) {
+        set_error("k_pcg_flush launch failed");
+        return -1;
     }
     c.mg_iters_last = it;
     c.mg_relres_last = rel;

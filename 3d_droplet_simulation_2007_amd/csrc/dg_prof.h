@@ -46,14 +46,17 @@ inline void prof_begin(int id, hipStream_t s)
     (void)hipEventRecord(p.ev[p.used], s);
 }
 
-inline void prof_end(int id, hipStream_t s, double bytes)
+// closes the pair opened by prof_begin; n: the launches of the family the
+// pair brackets (a run of consecutive launches shares one pair, so the
+// events' own cost is spread over the run)
+inline void prof_end(int id, hipStream_t s, double bytes, int n = 1)
 {
     Prof &p = prof();
     if (p.kernel != id) return;
     (void)hipEventRecord(p.ev[p.used + 1], s);
     p.used += 2;
     p.bytes += bytes;
-    p.launches += 1;
+    p.launches += n;
 }
 
 }  // namespace dg

@@ -688,9 +688,10 @@ double lev_h2i(const Ctx &c, const Grid &g)
 }
 
 // zero: the iterate is zero on entry; zown: this pass's own cells are zero
-// (the second pass of the first sweep)
+// (the second pass of the first sweep).  Adds the pass's algorithmic bytes
+// to *bytes_sum.
 template <typename CT>
-int colour_pass(Ctx &c, int l, int colour, bool zero, int zown)
+int colour_pass(Ctx &c, int l, int colour, bool zero, int zown, double *bytes_sum)
 {
     const float om = (float)c.mg_omega;
     Level &L = c.lev[l];
@@ -702,8 +703,7 @@ int colour_pass(Ctx &c, int l, int colour, bool zero, int zown)
     // algorithmic bytes: own f read and u write (one colour, FP32), the other
     // colour's u (not read from a zero iterate) and all coefficients
     const double cells = (double)r.half;
-    const double bytes = cells * (zero ? 8.0 : (om == 1.f || zown ? 12.0 : 16.0)) + 6.0 * cells * (double)sizeof(CT);
-    if (top) prof_begin(PK_RB_COLOUR, c.stream);
+    *bytes_sum += cells * (zero ? 8.0 : (om == 1.f || zown ? 12.0 : 16.0)) + 6.0 * cells * (double)sizeof(CT);
     if (r.mh % 4 == 0) {
         const dim3 b4 = row_block(r.mh / 4), g4 = row_grid(b4, r.mh / 4, r.my, r.mz);
         if (top && zero)
@@ -722,7 +722,6 @@ int colour_pass(Ctx &c, int l, int colour, bool zero, int zown)
         hipLaunchKernelGGL((k_rb_colour<CT, true, false>), grid, b1, 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, colour, om, zown);
     else
         hipLaunchKernelGGL((k_rb_colour<CT, false, false>), grid, b1, 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, colour, om, zown);
-    if (top) prof_end(PK_RB_COLOUR, c.stream, bytes);
     DG_CHECK_LAUNCH();
     return 0;
 }
@@ -871,10 +870,15 @@ int vcycle_rb(Ctx &c, int l)
             cur = y;
         }
     } else {
+        // the finest level's pre-smoothing passes run back to back: one
+        // timing pair for the run when the family is profiled
+        double pb = 0.0;
+        if (top) prof_begin(PK_RB_COLOUR, c.stream);
         for (int q = 0; q < nu1; q++) {
-            if (colour_pass<CT>(c, l, 0, q == 0, q == 0)) return -1;
-            if (colour_pass<CT>(c, l, 1, false, q == 0)) return -1;
+            if (colour_pass<CT>(c, l, 0, q == 0, q == 0, &pb)) return -1;
+            if (colour_pass<CT>(c, l, 1, false, q == 0, &pb)) return -1;
         }
+        if (top) prof_end(PK_RB_COLOUR, c.stream, pb, 2 * nu1);
     }
     const RbGeom rf = rb_geom(L.g);
     const Coef<CT> cf{(const CT *)L.rb_coef, 2 * rf.half};
@@ -905,10 +909,13 @@ int vcycle_rb(Ctx &c, int l)
             return -1;
         }
     } else {
+        double pb = 0.0;
+        if (top) prof_begin(PK_RB_COLOUR, c.stream);
         for (int q = 0; q < nu2; q++) {
-            if (colour_pass<CT>(c, l, 1, false, 0)) return -1;
-            if (colour_pass<CT>(c, l, 0, false, 0)) return -1;
+            if (colour_pass<CT>(c, l, 1, false, 0, &pb)) return -1;
+            if (colour_pass<CT>(c, l, 0, false, 0, &pb)) return -1;
         }
+        if (top) prof_end(PK_RB_COLOUR, c.stream, pb, 2 * nu2);
     }
     return 0;
 }

@@ -89,6 +89,33 @@ def test_mixed_matches_fp64_and_converges(dg, n):
     assert np.linalg.norm(res[1:-1, 1:-1, 1:-1]) / np.linalg.norm(rhs[1:-1, 1:-1, 1:-1]) < 2e-8
 
 
+@pytest.mark.parametrize("mode", [1, 2])
+def test_long_solve_flushes_the_direction_ring(dg, mode):
+    """A solve to 1e-12 takes more iterations than the ring of 16 search
+    directions holds, so x is accumulated from the ring mid-solve as well
+    as at the end.  Graph replay and launch-by-launch agree to the bit, and
+    the returned p's true FP64 residual is the reported one."""
+    print("case long solve, mode", mode, flush=True)
+    rhs, a, b, c = sphere_system(34, 11)
+    out = []
+    try:
+        dg.set_smoother(1.0, 1)  # one sweep below the finest level: a slower cycle, more iterations
+        for graphs in (True, False):
+            dg.set_tuning(graphs=graphs)
+            out.append(dg.poisson_solve(rhs, a, b, c, mode=mode, tol=1e-12, maxit=200))
+    finally:
+        dg.set_tuning()
+        dg.set_smoother()
+    (p, it, rel), (p2, it2, rel2) = out
+    print("iterations", it, "relres", rel, flush=True)
+    assert it > 16 and rel < 1e-12
+    assert it2 == it and np.array_equal(p2, p)
+    res = np.zeros_like(rhs)
+    OK.resid(1.0, res, p, rhs, a, b, c)
+    true = np.linalg.norm(res[1:-1, 1:-1, 1:-1]) / np.linalg.norm(rhs[1:-1, 1:-1, 1:-1])
+    assert true < 2e-12, true
+
+
 def test_matches_oracle_solve(dg, oracle):
     print("case poisson vs oracle 34^3", flush=True)
     o = Oracle(inputdata_text(nx=34, ny=34, nz=34))

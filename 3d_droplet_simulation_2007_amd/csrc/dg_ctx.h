@@ -70,6 +70,7 @@ struct Ctx {
     int rb_split = 1;        // mode 2: red-black split storage of the V-cycle levels
     int rb_tail = 2;         // split levels of <= 4096 cells: 0 per-level launches, 1 one-workgroup tail, 2 LDS tail
     int coef16_ok = 0;       // face coefficients fit FP16 (set from the fluids' densities)
+    int top_coef_ready = 0;  // solvelpres's abcei wrote the finest level's split coefficients for this solve
     hipGraphExec_t pcg_graph[2] = {nullptr, nullptr};
     int pcg_graph_key[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     // diagnostics of the last step
@@ -107,8 +108,10 @@ int mg_interp(const Grid &gf, hipStream_t s, double *uf, const double *uc);
 int mg_addint(const Grid &gf, hipStream_t s, double *uf, const double *uc, double *res);
 int mg_acfromac(const Grid &gc, hipStream_t s, double *a, double *b, double *c, double *ei, const double *a2,
                 const double *b2, const double *c2);
+// split (when given): also the red-black split copy of a, b, c (FP16 when
+// split16, else FP32), three planes of 2 * half entries
 int mg_abcei(const Grid &g, hipStream_t s, double *a, double *b, double *c, double *ei, const double *cc, double r1,
-             double r2);
+             double r2, void *split = nullptr, int split16 = 0);
 int mg_filterc27(const Grid &g, hipStream_t s, const double *cc, double *cf);
 int mg_fill(size_t n, hipStream_t s, double *a, double v);
 int op_mglin(Ctx &c);

@@ -876,11 +876,12 @@ __global__ void k_guess_check_scalar(double *ds)
 
 // r = b - A x with folded ghosts; partial sums of r.r and b.b
 // ZERO: x is zero, so A x is +0 on every cell and r = b - h2i * 0 = b bit
-// for bit; x and the coefficients are not read
+// for bit; x and the coefficients are not read, and fs (when given) takes
+// the split FP32 copy of r, the V-cycle's first right-hand side
 template <bool ZERO>
 __global__ void __launch_bounds__(256) k_init_resid(Grid g, double *r, const double *x, const double *bvec,
                                                     const double *a, const double *b, const double *c, double h2i,
-                                                    double *partial)
+                                                    double *partial, float *fs, RbGeom rg)
 {
     __shared__ double lds4[4];
     __shared__ double lds4b[4];
@@ -895,6 +896,7 @@ __global__ void __launch_bounds__(256) k_init_resid(Grid g, double *r, const dou
     if (ZERO && i <= g.nx - 1 && j <= g.ny - 1) {
         const double bb = bvec[g.at(i, j, k)];
         r[g.at(i, j, k)] = bb;
+        if (fs) fs[rg.at32(i, j, k)] = (float)bb;
         c1 = bb * bb;
         c2 = bb * bb;
     } else if (i <= g.nx - 1 && j <= g.ny - 1) {
@@ -1113,10 +1115,15 @@ static int pcg_solve(Ctx &c)
     } else {
         if (mg_fill(g.n(), s, x, 0.0)) return -1;
     }
+    // the split V-cycle's first right-hand side: written with r = b from a
+    // zero guess, loaded below otherwise
+    const bool split32 = sizeof(T) == 4 && rb_enabled(c);
     if (c.mg_guess == 1)
-        hipLaunchKernelGGL(k_init_resid<false>, ig, ib, 0, s, g, r, x, bvec, F.a, F.b, F.c, h2i, partial);
+        hipLaunchKernelGGL(k_init_resid<false>, ig, ib, 0, s, g, r, x, bvec, F.a, F.b, F.c, h2i, partial, nullptr,
+                           rb_geom(g));
     else
-        hipLaunchKernelGGL(k_init_resid<true>, ig, ib, 0, s, g, r, x, bvec, F.a, F.b, F.c, h2i, partial);
+        hipLaunchKernelGGL(k_init_resid<true>, ig, ib, 0, s, g, r, x, bvec, F.a, F.b, F.c, h2i, partial,
+                           split32 ? F.rb_f : nullptr, rb_geom(g));
     DG_CHECK_LAUNCH();
     if (sum_partials(s, partial, nblk, 2, partial + 2 * (size_t)nblk, ds + S_RR)) return -1;  // r.r and b.b
     if (c.mg_guess == 1) {
@@ -1128,9 +1135,9 @@ static int pcg_solve(Ctx &c)
     }
     // learned guess (mg_guess 2) and sample capture
     const bool hooks = guess_wanted(c);
-    if (hooks && guess_solve_begin(c, x, r, h2i, c.mg_guess == 2, ds + S_RR)) return -1;
-    // the split V-cycle's right-hand side of the first iteration
-    if (sizeof(T) == 4 && rb_enabled(c) && rb_load_rhs(c, r)) return -1;
+    bool r_changed = false;
+    if (hooks && guess_solve_begin(c, x, r, h2i, c.mg_guess == 2, ds + S_RR, &r_changed)) return -1;
+    if (split32 && (c.mg_guess == 1 || r_changed) && rb_load_rhs(c, r)) return -1;
     // coarse factorisation for this solve, FIRST flag
     {
         Level &L0 = c.lev[0];

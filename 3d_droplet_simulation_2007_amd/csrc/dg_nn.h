@@ -15,7 +15,7 @@ namespace dg {
 // by x + alpha d and r by r - alpha A d if that lowers |r| (otherwise x, r
 // stay: the fallback is the guess x).  rr_slot receives the new r.r.  When
 // capture is on, the input is also written to the sample store.
-int guess_solve_begin(Ctx &c, double *x, double *r, double h2i, bool apply, double *rr_slot);
+int guess_solve_begin(Ctx &c, double *x, double *r, double h2i, bool apply, double *rr_slot, bool *r_changed);
 // Called after the solve: completes a captured sample with its target (x
 // averaged to the coarse level, normalised); a solve that did not converge
 // drops it.

@@ -592,8 +592,9 @@ bool guess_wanted(const Ctx &c)
     return c.mg_guess == 2 || N.capture || N.eval;
 }
 
-int guess_solve_begin(Ctx &c, double *x, double *r, double h2i, bool apply, double *rr_slot)
+int guess_solve_begin(Ctx &c, double *x, double *r, double h2i, bool apply, double *rr_slot, bool *r_changed)
 {
+    *r_changed = false;
     Net &N = net();
     hipStream_t s = c.stream;
     if (c.ng < 2) {
@@ -658,6 +659,7 @@ int guess_solve_begin(Ctx &c, double *x, double *r, double h2i, bool apply, doub
         N.pending = slot;
     }
     if (!apply) return 0;
+    *r_changed = true;  // the learned step below updates x and r
     if (net_forward(N.a.x0, s)) return -1;
     const double h2 = 1.0 / h2i;
     hipLaunchKernelGGL(k_prolong_guess, dim3(gridn((size_t)nfine)), dim3(256), 0, s, g, cx, cy, cz, N.a.out, w.scal,

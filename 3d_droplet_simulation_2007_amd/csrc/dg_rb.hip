@@ -959,6 +959,7 @@ int rb_prepare(Ctx &c)
             DG_HIP(hipMalloc(&L.rb_coef, 3 * n2 * (want16 ? sizeof(_Float16) : sizeof(float))));
             L.rb_coef16 = want16;
         }
+        if (l == c.ng - 1 && c.top_coef_ready) continue;  // written by solvelpres's abcei
         const dim3 bl = row_block(L.g.nx - 2), gr = row_grid(bl, L.g.nx - 2, L.g.ny - 2, L.g.nz - 2);
         if (want16)
             hipLaunchKernelGGL(k_rb_coef<_Float16>, gr, bl, 0, c.stream, L.g, r, L.a, L.b, L.c, (_Float16 *)L.rb_coef);
@@ -966,6 +967,7 @@ int rb_prepare(Ctx &c)
             hipLaunchKernelGGL(k_rb_coef<float>, gr, bl, 0, c.stream, L.g, r, L.a, L.b, L.c, (float *)L.rb_coef);
         DG_CHECK_LAUNCH();
     }
+    c.top_coef_ready = 0;
     return 0;
 }
 

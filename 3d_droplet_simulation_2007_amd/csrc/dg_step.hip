@@ -14,6 +14,7 @@
 #include <cmath>
 
 #include "dg_ctx.h"
+#include "dg_rb.h"
 #include "dg_util.h"
 
 namespace dg {
@@ -1047,9 +1048,11 @@ static int stress(Ctx &c)
 }
 
 int mg_abcei(const Grid &g, hipStream_t s, double *a, double *b, double *c, double *ei, const double *cc, double r1,
-             double r2)
+             double r2, void *split, int split16)
 {
     const int nx = g.nx, ny = g.ny, nz = g.nz, sy = g.sy, sz = g.sz;
+    const int mh = (nx - 2) / 2, my = ny - 2;
+    const size_t half = (size_t)mh * my * (nz - 2), n2 = 2 * half;
     // a, b, c and ei in one pass.  ei needs the coefficients of the faces
     // below the cell too; the thread forms them again with the same
     // expression, and at i = 2 takes bc_coef's a(1,j,k) = a(2,j,k).  The
@@ -1068,6 +1071,20 @@ int mg_abcei(const Grid &g, hipStream_t s, double *a, double *b, double *c, doub
             b[o] = nb;
             c[o] = nc;
             ei[o] = 1.0 / (na + aw + nb + bs + nc + cb);
+            if (split) {  // the conversion of the split V-cycle's set-up (k_rb_coef)
+                const size_t e = (size_t)((i + j + k) & 1) * half + ((size_t)(k - 2) * my + (j - 2)) * mh + ((i - 2) >> 1);
+                if (split16) {
+                    _Float16 *p = (_Float16 *)split;
+                    p[e] = (_Float16)na;
+                    p[n2 + e] = (_Float16)nb;
+                    p[2 * n2 + e] = (_Float16)nc;
+                } else {
+                    float *p = (float *)split;
+                    p[e] = (float)na;
+                    p[n2 + e] = (float)nb;
+                    p[2 * n2 + e] = (float)nc;
+                }
+            }
         }))
         return -1;
     // bc_coef (main.f:4465-4494): the three fills write different arrays
@@ -1295,7 +1312,20 @@ int op_solvelpres(Ctx &c, int t)
     if (op_bc_tensor(c)) return -1;
     const double r2 = c.s.rho2;
     const double r1 = .5 * (c.s.rho1 - c.s.rho2);
-    if (mg_abcei(c.g, c.stream, L.a, L.b, L.c, L.ei, L.cc, r1, r2)) return -1;
+    // with the split FP32 V-cycle already set up at the coefficient type it
+    // will use, abcei also writes the finest level's split copy, which the
+    // solve's set-up then skips
+    void *split = nullptr;
+    int split16 = 0;
+    if (c.mg_mode == 2 && rb_enabled(c)) {
+        const int want16 = (c.rb_split == 3 || (c.rb_split == 1 && c.coef16_ok)) ? 1 : 0;
+        if (L.rb_coef && L.rb_coef16 == want16) {
+            split = L.rb_coef;
+            split16 = want16;
+        }
+    }
+    if (mg_abcei(c.g, c.stream, L.a, L.b, L.c, L.ei, L.cc, r1, r2, split, split16)) return -1;
+    c.top_coef_ready = split != nullptr;
     if (predictor(c, w1, w2, w3)) return -1;
     if (op_bc_w(c, w1, w2, w3)) return -1;
     if (convective(c, w1, w2, w3)) return -1;

@@ -766,7 +766,7 @@ __global__ void __launch_bounds__(256) k_pcg_dir(Grid g, Z z, PT *hist, size_t n
     if (tid == 0) partial[bidx] = lds4[0] + lds4[1] + lds4[2] + lds4[3];
 }
 
-// K3: alpha = rz/pq; x += alpha p; r -= alpha q; partial sums of r.r
+// K3: alpha = rz/pq; r -= alpha q (x takes alpha p in k_pcg_flush); partial sums of r.r
 // (fs != null: also the FP32 red-black split copy of the new r, the next
 // V-cycle's right-hand side)
 template <typename PT>

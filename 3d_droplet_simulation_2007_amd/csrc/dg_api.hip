@@ -124,6 +124,9 @@ static int alloc_levels(Ctx &c, int nx, int ny, int nz, bool strict)
     if (dalloc(&c.dscal, 64)) return -1;
     DG_HIP(hipHostMalloc((void **)&c.hscal, 64 * sizeof(double), hipHostMallocDefault));
     memset(c.hscal, 0, 64 * sizeof(double));
+    DG_HIP(hipHostMalloc((void **)&c.hpoll, 4 * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent));
+    memset(c.hpoll, 0, 4 * sizeof(double));
+    DG_HIP(hipHostGetDevicePointer((void **)&c.dpoll, c.hpoll, 0));
     if (!c.stream) DG_HIP(hipStreamCreate(&c.stream));
     return 0;
 }
@@ -155,6 +158,7 @@ static void free_ctx(Ctx &c)
     for (int q = 0; q < 2; q++)
         if (c.pcg_graph[q]) (void)hipGraphExecDestroy(c.pcg_graph[q]);
     if (c.hscal) (void)hipHostFree(c.hscal);
+    if (c.hpoll) (void)hipHostFree(c.hpoll);
     if (c.stream) (void)hipStreamDestroy(c.stream);
     c = Ctx();
 }

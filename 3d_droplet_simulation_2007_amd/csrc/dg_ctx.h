@@ -53,6 +53,8 @@ struct Ctx {
     double *red_partial = nullptr;  // RED_BLOCKS doubles
     double *dscal = nullptr;        // small device scalar scratch (64 doubles)
     double *hscal = nullptr;        // pinned host mirror of dscal
+    double *hpoll = nullptr;        // mapped coherent host slots: [0] r.r of the last PCG iteration, [1] its count
+    double *dpoll = nullptr;        // the same slots' device address
     Scalars s{};
     // pressure-solver options
     int mg_mode = 0;       // 0 reference FMG schedule, 1 FP64 PCG, 2 FP32-preconditioned FP64 PCG

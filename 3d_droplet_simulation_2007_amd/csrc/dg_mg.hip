@@ -12,6 +12,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cmath>
 #include <vector>
 
@@ -835,7 +836,9 @@ static int sum_partials(hipStream_t s, const double *partial, int nb, int stride
 }
 
 // scalar bookkeeping after K3: alpha_old = rz/pq, rz_old = rz, first = 0
-__global__ void k_pcg_scalars(double *ds)
+// poll: mapped host memory; r.r and then, after a system-scope fence, the
+// iteration count (the host waits for the count, then reads r.r)
+__global__ void k_pcg_scalars(double *ds, double *poll)
 {
     const int it = (int)ds[S_IT];
     ds[S_ALPHA] = ds[S_RZ] / ds[S_PQ];
@@ -843,6 +846,9 @@ __global__ void k_pcg_scalars(double *ds)
     ds[S_IT] = (double)(it + 1);
     ds[S_RZOLD] = ds[S_RZ];
     ds[S_FIRST] = 0.0;
+    poll[0] = ds[S_RR];
+    __threadfence_system();
+    poll[1] = (double)(it + 1);
 }
 
 // x += alpha_j p_j for the first count buffers of the ring, in order
@@ -1050,9 +1056,8 @@ static int pcg_iteration(Ctx &c)
     prof_end(PK_PCG_UPDATE, s, (3.0 * 8.0 + (split ? 4.0 : 0.0)) * ncell);
     DG_CHECK_LAUNCH();
     if (sum_partials(s, partial, nblk, 1, partial + 2 * (size_t)nblk, ds + S_RR)) return -1;
-    hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(1), 0, s, ds);
+    hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(1), 0, s, ds, c.dpoll);
     DG_CHECK_LAUNCH();
-    DG_HIP(hipMemcpyAsync(c.hscal + S_RR, ds + S_RR, sizeof(double), hipMemcpyDeviceToHost, s));
     return 0;
 }
 
@@ -1161,6 +1166,16 @@ static int pcg_solve(Ctx &c)
         hipLaunchKernelGGL(k_pcg_flush<PT>, ig, ib, 0, s, g, x, (const PT *)c.pcg_p, g.n(), ds, count);
         return hipGetLastError() == hipSuccess ? 0 : -1;
     };
+    // the iteration's r.r arrives in mapped host memory: poll for its count
+    // (bounded), falling back to a stream synchronisation
+    volatile double *poll = c.hpoll;
+    poll[1] = 0.0;
+    auto wait_iteration = [&](int n) {
+        const auto t0 = std::chrono::steady_clock::now();
+        while (poll[1] < (double)n)
+            if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(2)) return false;
+        return true;
+    };
     int it = 0;
     while (rel >= c.mg_tol && it < c.mg_maxit) {
         if (graphs) {
@@ -1173,8 +1188,8 @@ static int pcg_solve(Ctx &c)
             set_error("k_pcg_flush launch failed");
             return -1;
         }
-        DG_HIP(hipStreamSynchronize(s));
-        rel = sqrt(c.hscal[S_RR]) / bnorm;
+        if (!wait_iteration(it)) DG_HIP(hipStreamSynchronize(s));
+        rel = sqrt(poll[0]) / bnorm;
     }
     if (it % PCG_HIST != 0 && flush(it % PCG_HIST)) {
         set_error("k_pcg_flush launch failed");

@@ -287,6 +287,13 @@ def set_tuning(fused=False, graphs=True, split=True, tail="lds"):
     _check(lib().dg_set_tuning(_i(fv), _i(1 if graphs else 0), _i(sv), _i(tv)), "dg_set_tuning")
 
 
+def set_lookahead(ratio=1e3):
+    """Queue PCG iteration n+1 before iteration n's residual reaches the host
+    while the last residual seen exceeds ratio * tol (<= 0: never);
+    bit-identical either way."""
+    _check(lib().dg_set_lookahead(_d(ratio)), "dg_set_lookahead")
+
+
 # kernel families dg_profile_start accepts
 PROFILE_KERNELS = {"gs_colour": 1, "resid": 2, "pcg_dir": 3, "pcg_update": 4, "relax": 5, "sweep": 6,
                    "rb_colour": 7}

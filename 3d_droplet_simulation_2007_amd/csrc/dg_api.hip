@@ -49,6 +49,7 @@ static bool g_ready = false;
 static int g_fused_default = 0, g_graphs_default = 1, g_split_default = 1, g_tail_default = 2;
 static double g_omega_default = 1.0;  // tolerance-mode smoother (dg_set_smoother)
 static int g_nuc_default = 0;
+static double g_lookahead_default = 1e3;  // dg_set_lookahead
 
 // ---- allocation -----------------------------------------------------------
 
@@ -124,8 +125,8 @@ static int alloc_levels(Ctx &c, int nx, int ny, int nz, bool strict)
     if (dalloc(&c.dscal, 64)) return -1;
     DG_HIP(hipHostMalloc((void **)&c.hscal, 64 * sizeof(double), hipHostMallocDefault));
     memset(c.hscal, 0, 64 * sizeof(double));
-    DG_HIP(hipHostMalloc((void **)&c.hpoll, 4 * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent));
-    memset(c.hpoll, 0, 4 * sizeof(double));
+    DG_HIP(hipHostMalloc((void **)&c.hpoll, 8 * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent));
+    memset(c.hpoll, 0, 8 * sizeof(double));
     DG_HIP(hipHostGetDevicePointer((void **)&c.dpoll, c.hpoll, 0));
     if (!c.stream) DG_HIP(hipStreamCreate(&c.stream));
     return 0;
@@ -394,6 +395,7 @@ int dg_init(const char *path)
     c.use_graphs = g_graphs_default;
     c.rb_split = g_split_default;
     c.rb_tail = g_tail_default;
+    c.pcg_lookahead = g_lookahead_default;
     c.mg_omega = g_omega_default;
     c.mg_nuc = g_nuc_default;
     nondim(in, c.s);
@@ -592,6 +594,17 @@ int dg_set_tuning(const int *fused, const int *graphs, const int *split, const i
     return 0;
 }
 
+int dg_set_lookahead(const double *ratio)
+{
+    if (!(*ratio == *ratio)) {
+        set_error("dg_set_lookahead: ratio is NaN");
+        return -1;
+    }
+    g_lookahead_default = *ratio;
+    ctx()->pcg_lookahead = g_lookahead_default;
+    return 0;
+}
+
 // ---- host-pointer kernels ----------------------------------------------------
 
 int dg_relax(double *u, const double *lengthsize, const double *rhs, const double *a, const double *b,
@@ -786,6 +799,7 @@ int dg_poisson_solve(const int *nx, const int *ny, const int *nz, const double *
     lc.use_graphs = g_graphs_default;
     lc.rb_split = g_split_default;
     lc.rb_tail = g_tail_default;
+    lc.pcg_lookahead = g_lookahead_default;
     lc.coef16_ok = 0;  // arbitrary coefficients: FP32 in the split V-cycle
     lc.mg_omega = g_omega_default;
     lc.mg_nuc = g_nuc_default;

@@ -53,7 +53,7 @@ struct Ctx {
     double *red_partial = nullptr;  // RED_BLOCKS doubles
     double *dscal = nullptr;        // small device scalar scratch (64 doubles)
     double *hscal = nullptr;        // pinned host mirror of dscal
-    double *hpoll = nullptr;        // mapped coherent host slots: [0] r.r of the last PCG iteration, [1] its count
+    double *hpoll = nullptr;        // mapped coherent host slots: [1] the count of PCG iterations done, [2 + 2 (n & 1)] and the next: r.r and the stopping flag of iteration n
     double *dpoll = nullptr;        // the same slots' device address
     Scalars s{};
     // pressure-solver options
@@ -68,6 +68,9 @@ struct Ctx {
                              // the split finest level (plane marching) and the split levels below it
                              // (tiles); 2 the split levels of <= 2^18 cells below the finest only
     int use_graphs = 1;      // replay each PCG iteration as a captured HIP graph
+    double pcg_lookahead = 1e3;  // PCG iteration n+1 is queued before n's residual reaches the host while
+                                 // the last residual the host has seen exceeds this multiple of the
+                                 // tolerance (<= 0: never)
     int ref_grid = 1;        // the grid has the reference's form nn = 2**ng + 2 (mglin's schedule may run)
     int rb_split = 1;        // mode 2: red-black split storage of the V-cycle levels
     int rb_tail = 2;         // split levels of <= 4096 cells: 0 per-level launches, 1 one-workgroup tail, 2 LDS tail

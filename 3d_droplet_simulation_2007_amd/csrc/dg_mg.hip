@@ -663,7 +663,7 @@ __global__ void __launch_bounds__(256) k_final2(const double *partial, int nb, d
 }
 
 // device scalar slots
-enum { S_RZ = 16, S_ZQ, S_RZOLD, S_ALPHA, S_PQ, S_RR, S_BB, S_FIRST, S_IT, S_AH = 32 };
+enum { S_RZ = 16, S_ZQ, S_RZOLD, S_ALPHA, S_PQ, S_RR, S_BB, S_FIRST, S_IT, S_STOP, S_TOL, S_AH = 32 };
 
 // The search directions of a solve go to a ring of PCG_HIST buffers and
 // their step lengths to ds[S_AH ..]; x takes x + alpha_j p_j for the
@@ -835,18 +835,31 @@ static int sum_partials(hipStream_t s, const double *partial, int nb, int stride
     return 0;
 }
 
-// scalar bookkeeping after K3: alpha_old = rz/pq, rz_old = rz, first = 0
-// poll: mapped host memory; r.r and then, after a system-scope fence, the
-// iteration count (the host waits for the count, then reads r.r)
+// scalar bookkeeping after K3: alpha_old = rz/pq, rz_old = rz, first = 0,
+// and the stopping test: stop = !(|r| / |b| >= tol), the host's own test on
+// the same doubles.  An iteration that finds stop set was queued ahead of
+// the host's test and comes after convergence: it records nothing, so the
+// flushes and the returned residual see only the iterations before it.
+// poll: mapped host memory; r.r and the stopping flag into the pair of
+// slots of the iteration number's parity (the host is at most one
+// iteration behind the GPU), then, after a system-scope fence, the
+// iteration count (the host waits for the count, then reads the pair)
 __global__ void k_pcg_scalars(double *ds, double *poll)
 {
+    if (ds[S_STOP] != 0.0) return;
+    double bnorm = sqrt(ds[S_BB]);
+    if (bnorm == 0.0) bnorm = 1.0;
+    const double stop = sqrt(ds[S_RR]) / bnorm >= ds[S_TOL] ? 0.0 : 1.0;
+    ds[S_STOP] = stop;
     const int it = (int)ds[S_IT];
+    double *pair = poll + 2 + 2 * ((it + 1) & 1);
+    pair[0] = ds[S_RR];
+    pair[1] = stop;
     ds[S_ALPHA] = ds[S_RZ] / ds[S_PQ];
     ds[S_AH + it % PCG_HIST] = ds[S_ALPHA];
     ds[S_IT] = (double)(it + 1);
     ds[S_RZOLD] = ds[S_RZ];
     ds[S_FIRST] = 0.0;
-    poll[0] = ds[S_RR];
     __threadfence_system();
     poll[1] = (double)(it + 1);
 }
@@ -1151,7 +1164,9 @@ static int pcg_solve(Ctx &c)
         DG_CHECK_LAUNCH();
         c.hscal[S_FIRST] = 1.0;
         c.hscal[S_IT] = 0.0;
-        DG_HIP(hipMemcpyAsync(ds + S_FIRST, c.hscal + S_FIRST, 2 * sizeof(double), hipMemcpyHostToDevice, s));
+        c.hscal[S_STOP] = 0.0;
+        c.hscal[S_TOL] = c.mg_tol;
+        DG_HIP(hipMemcpyAsync(ds + S_FIRST, c.hscal + S_FIRST, 4 * sizeof(double), hipMemcpyHostToDevice, s));
     }
     DG_HIP(hipMemcpyAsync(c.hscal + S_RR, ds + S_RR, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
     DG_HIP(hipStreamSynchronize(s));
@@ -1176,20 +1191,44 @@ static int pcg_solve(Ctx &c)
             if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(2)) return false;
         return true;
     };
-    int it = 0;
-    while (rel >= c.mg_tol && it < c.mg_maxit) {
+    // iterations queued (launched) and finished with their residual read (it)
+    int it = 0, launched = 0;
+    auto launch = [&]() {
         if (graphs) {
-            DG_HIP(hipGraphLaunch(c.pcg_graph[0], s));
+            if (hipGraphLaunch(c.pcg_graph[0], s) != hipSuccess) {
+                set_error("PCG graph launch failed");
+                return -1;
+            }
         } else {
             if (pcg_iteration<T>(c)) return -1;
         }
-        it++;
-        if (it % PCG_HIST == 0 && flush(PCG_HIST)) {
+        launched++;
+        if (launched % PCG_HIST == 0 && flush(PCG_HIST)) {
             set_error("k_pcg_flush launch failed");
             return -1;
         }
-        if (!wait_iteration(it)) DG_HIP(hipStreamSynchronize(s));
-        rel = sqrt(poll[0]) / bnorm;
+        return 0;
+    };
+    bool stop = !(rel >= c.mg_tol);
+    while (!stop && it < c.mg_maxit) {
+        if (launched == it && launch()) return -1;
+        // look ahead while the residual is far above the tolerance; never
+        // to an iteration that fills the ring, whose flush would take that
+        // iteration's slot whether it ran or was skipped
+        if (launched == it + 1 && launched < c.mg_maxit && (launched + 1) % PCG_HIST != 0 &&
+            c.pcg_lookahead > 0.0 && rel > c.pcg_lookahead * c.mg_tol && launch())
+            return -1;
+        if (!wait_iteration(it + 1)) {
+            DG_HIP(hipStreamSynchronize(s));
+            if (poll[1] < (double)(it + 1)) {
+                set_error("PCG iteration %d reported no residual", it + 1);
+                return -1;
+            }
+        }
+        it++;
+        const volatile double *pair = poll + 2 + 2 * (it & 1);
+        rel = sqrt(pair[0]) / bnorm;
+        stop = pair[1] != 0.0;
     }
     if (it % PCG_HIST != 0 && flush(it % PCG_HIST)) {
         set_error("k_pcg_flush launch failed");

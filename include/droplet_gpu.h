@@ -121,6 +121,15 @@ int dg_set_vof_guard(const int *on);
  * and to later dg_init / dg_poisson_solve calls. */
 int dg_set_tuning(const int *fused, const int *graphs, const int *split, const int *tail);
 
+/* Look-ahead of the tolerance-mode PCG loop: the host queues iteration n+1
+ * before iteration n's residual reaches it while the last residual it has
+ * seen exceeds ratio * tol (default 1e3; <= 0 never), so the GPU does not
+ * idle between the two.  Bit-identical either way: an iteration queued
+ * after convergence is skipped by the device's own stopping test (its
+ * V-cycle still runs) and changes nothing the solve returns.  Applies to
+ * the resident state and to later dg_init / dg_poisson_solve calls. */
+int dg_set_lookahead(const double *ratio);
+
 /* ---- host-pointer kernels with the reference's argument lists --------- */
 /* Each uploads its arrays, runs on the GPU and downloads the outputs. */
 int dg_relax(double *u, const double *lengthsize, const double *rhs, const double *a, const double *b,

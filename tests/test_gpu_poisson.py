@@ -116,6 +116,36 @@ def test_long_solve_flushes_the_direction_ring(dg, mode):
     assert true < 2e-12, true
 
 
+@pytest.mark.parametrize("mode", [1, 2])
+def test_lookahead_is_bit_identical(dg, mode):
+    """The host queues the next PCG iteration before it has the current
+    one's residual.  Never looking ahead, always looking ahead (every solve
+    then ends with one queued iteration past convergence, which the device's
+    stopping test skips) and the default give the same p, iteration count
+    and residual, in a short solve and in one long enough to flush the ring
+    of search directions mid-solve, with graph replay and without."""
+    print("case look-ahead, mode", mode, flush=True)
+    rhs, a, b, c = sphere_system(34, 5)
+    try:
+        for nuc, tol in ((0, 1e-8), (1, 1e-12)):
+            dg.set_smoother(1.0, nuc)
+            for graphs in (True, False):
+                dg.set_tuning(graphs=graphs)
+                out = []
+                for ratio in (0.0, 1e-300, 1e3):
+                    dg.set_lookahead(ratio)
+                    out.append(dg.poisson_solve(rhs, a, b, c, mode=mode, tol=tol, maxit=200))
+                (p, it, rel) = out[0]
+                print("tol", tol, "graphs", graphs, "iterations", it, "relres", rel, flush=True)
+                assert rel < tol and (it > 16 or tol > 1e-10)
+                for p2, it2, rel2 in out[1:]:
+                    assert it2 == it and rel2 == rel and np.array_equal(p2, p)
+    finally:
+        dg.set_lookahead()
+        dg.set_tuning()
+        dg.set_smoother()
+
+
 def test_matches_oracle_solve(dg, oracle):
     print("case poisson vs oracle 34^3", flush=True)
     o = Oracle(inputdata_text(nx=34, ny=34, nz=34))

@@ -864,16 +864,41 @@ __global__ void k_pcg_scalars(double *ds, double *poll)
     poll[1] = (double)(it + 1);
 }
 
-// x += alpha_j p_j for the first count buffers of the ring, in order
-template <typename PT>
-__global__ void __launch_bounds__(256) k_pcg_flush(Grid g, double *x, const PT *hist, size_t nslot, const double *ds,
-                                                   int count)
+// x += alpha_j p_j for the first N buffers of the ring, in order.  N is a
+// template parameter so that a cell's N direction loads are all in flight
+// before the first addition (a run-time count left one load per addition)
+template <typename PT, int N>
+__global__ void __launch_bounds__(256) k_pcg_flush(Grid g, double *x, const PT *hist, size_t nslot, const double *ds)
 {
     DG_INTERIOR_IJK(g);
     const int o = g.at(i, j, k);
+    PT h[N];
+#pragma unroll
+    for (int q = 0; q < N; q++) h[q] = hist[(size_t)q * nslot + o];
     double v = x[o];
-    for (int q = 0; q < count; q++) v = v + ds[S_AH + q] * (double)hist[(size_t)q * nslot + o];
+#pragma unroll
+    for (int q = 0; q < N; q++) v = v + ds[S_AH + q] * (double)h[q];
     x[o] = v;
+}
+
+template <typename PT>
+static int launch_flush(int count, hipStream_t s, const Grid &g, double *x, const PT *hist, size_t nslot,
+                        const double *ds)
+{
+    static_assert(PCG_HIST == 16, "one instantiation per ring fill");
+    const dim3 ig = interior_grid(g), ib = interior_block();
+    switch (count) {
+#define DG_FLUSH(N)                                                                                   \
+    case N:                                                                                           \
+        hipLaunchKernelGGL((k_pcg_flush<PT, N>), ig, ib, 0, s, g, x, hist, nslot, ds);                \
+        break;
+        DG_FLUSH(1) DG_FLUSH(2) DG_FLUSH(3) DG_FLUSH(4) DG_FLUSH(5) DG_FLUSH(6) DG_FLUSH(7) DG_FLUSH(8)
+        DG_FLUSH(9) DG_FLUSH(10) DG_FLUSH(11) DG_FLUSH(12) DG_FLUSH(13) DG_FLUSH(14) DG_FLUSH(15) DG_FLUSH(16)
+#undef DG_FLUSH
+    default:
+        return -1;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 // the previous-step guess is dropped for the zero guess when its residual
@@ -1178,8 +1203,7 @@ static int pcg_solve(Ctx &c)
     // x += alpha_j p_j over the ring's first count slots
     using PT = T;
     auto flush = [&](int count) {
-        hipLaunchKernelGGL(k_pcg_flush<PT>, ig, ib, 0, s, g, x, (const PT *)c.pcg_p, g.n(), ds, count);
-        return hipGetLastError() == hipSuccess ? 0 : -1;
+        return launch_flush<PT>(count, s, g, x, (const PT *)c.pcg_p, g.n(), ds);
     };
     // the iteration's r.r arrives in mapped host memory: poll for its count
     // (bounded), falling back to a stream synchronisation

@@ -628,8 +628,35 @@ __global__ void __launch_bounds__(256) k_dot_rz_zq(Grid g, const double *r, Z z,
     // block b takes the interior rows b, b + gridDim.x, ...; its threads
     // stride along each row (fixed assignment: deterministic sums)
     const int my = g.ny - 2, rows = my * (g.nz - 2);
+    const int G = (int)gridDim.x;
     double a1 = 0.0, a2 = 0.0;
-    for (int rr = (int)blockIdx.x; rr < rows; rr += (int)gridDim.x) {
+    int rr = (int)blockIdx.x;
+    if (g.nx - 2 <= 256) {
+        // at most one cell of a row per thread: four rows' loads in flight
+        // before their additions, which keep the row order
+        const int i = 2 + (int)threadIdx.x;
+        const bool act = i <= g.nx - 1;
+        for (; rr + 3 * G < rows; rr += 4 * G) {
+            double zv[4], rv[4], qv[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int ru = rr + u * G;
+                const int j = ru % my + 2, k = ru / my + 2;
+                const int o = g.at(act ? i : 2, j, k);
+                zv[u] = z(act ? i : 2, j, k, o);
+                rv[u] = r[o];
+                qv[u] = q[o];
+            }
+            if (act) {
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    a1 += rv[u] * zv[u];
+                    a2 += zv[u] * qv[u];
+                }
+            }
+        }
+    }
+    for (; rr < rows; rr += G) {
         const int j = rr % my + 2, k = rr / my + 2;
         for (int i = 2 + (int)threadIdx.x; i <= g.nx - 1; i += 256) {
             const int o = g.at(i, j, k);

@@ -170,7 +170,7 @@ struct NbGlobal {
 template <typename CT, bool ZERO, class NB>
 __device__ inline void rb_group4(const RbGeom &r, const NB &nb, const float *f, const float *uold_own,
                                  const Coef<CT> &cf, size_t own, size_t oth, float h2, int colour, float om, int zown,
-                                 int m0, int j, int k, float out[4])
+                                 int m0, int j, int k, float out[4], const float *eo = nullptr)
 {
     const int p = (colour + j + k) & 1;
     const size_t rs = (size_t)r.mh, ps = (size_t)r.mh * r.my;
@@ -219,7 +219,11 @@ __device__ inline void rb_group4(const RbGeom &r, const NB &nb, const float *f, 
     }
     float fv[4], uo[4] = {0.f, 0.f, 0.f, 0.f};
     ld4(f + own, fv);
-    if (!ZERO && !zown && om != 1.f) ld4(uold_own + own, uo);  // Gauss-Seidel needs no old value
+    if (!ZERO && !zown && om != 1.f) {  // Gauss-Seidel needs no old value
+        ld4(uold_own + own, uo);
+        if (eo)  // the coarse correction not yet added to the own colour
+            for (int q = 0; q < 4; q++) uo[q] = uo[q] + eo[q];
+    }
     for (int q = 0; q < 4; q++) {
         const int i = 2 + p + 2 * (m0 + q);
         const bool hw = i > 2, he = i < r.nx - 1;
@@ -450,11 +454,23 @@ __global__ void __launch_bounds__(256) k_rb_resid_restrict(RbGeom rf, const floa
 // fine u += coarse u of the parent cell (injection), fine in split order:
 // entry m of row (j, k) in both colours, cells i = 2 + p + 2m with p = 0, 1,
 // both children of coarse (m+2, j', k')
+struct CoarseE {
+    RbGeom rc;          // coarse level, split order (ucs), or
+    const float *ucs;
+    Grid gc;            // natural order (ucn)
+    const float *ucn;
+    // the correction of fine entry m of row (j, k), either colour
+    __device__ inline float at(int m, int j, int k) const
+    {
+        const int ic = m + 2, jc = (j - 2) / 2 + 2, kc = (k - 2) / 2 + 2;
+        return ucs ? ucs[rc.at(ic, jc, kc)] : ucn[gc.at(ic, jc, kc)];
+    }
+};
+
 __device__ inline void rb_prolong_entry(const RbGeom &rf, float *u, const RbGeom &rc, const float *ucs,
                                         const Grid &gc, const float *ucn, int m, int j, int k)
 {
-    const int ic = m + 2, jc = (j - 2) / 2 + 2, kc = (k - 2) / 2 + 2;
-    const float e = ucs ? ucs[rc.at(ic, jc, kc)] : ucn[gc.at(ic, jc, kc)];
+    const float e = CoarseE{rc, ucs, gc, ucn}.at(m, j, k);
     const size_t o = ((size_t)(k - 2) * rf.my + (j - 2)) * rf.mh + m;
     u[o] += e;
     u[rf.half + o] += e;
@@ -469,6 +485,57 @@ __global__ void __launch_bounds__(256) k_rb_prolong(RbGeom rf, float *u, RbGeom 
     const int k = (int)tile.z + 2;
     if (m >= rf.mh || j > rf.ny - 1) return;
     rb_prolong_entry(rf, u, rc, ucs, gc, ucn, m, j, k);
+}
+
+// Other-colour neighbours with the coarse correction added as they are
+// read: the values k_rb_prolong would have left in HBM.  eo: the
+// corrections of entries m0 .. m0+3 of the group's own row, which a
+// neighbour row with the same coarse parent row shares.
+struct NbProlong {
+    NbGlobal g;
+    CoarseE e;
+    int m0, j, k;
+    const float *eo;
+    __device__ inline void row4(int dj, int dk, float v[4]) const
+    {
+        g.row4(dj, dk, v);
+        if (((j - 2 + dj) >> 1) == ((j - 2) >> 1) && ((k - 2 + dk) >> 1) == ((k - 2) >> 1)) {
+            for (int q = 0; q < 4; q++) v[q] = v[q] + eo[q];
+        } else {
+            for (int q = 0; q < 4; q++) v[q] = v[q] + e.at(m0 + q, j + dj, k + dk);
+        }
+    }
+    __device__ inline float one(int dm) const { return g.one(dm) + e.at(m0 + dm, j, k); }
+};
+
+// The first post-smoothing sweep with the prolongation folded in: the pass
+// of colour 1 (NBE) reads both colours as u + e (e: the coarse correction
+// of the cell), the pass of colour 0 its own old values as u + e; each
+// colour's corrected values are written by its own pass.  The same values
+// and arithmetic as k_rb_prolong followed by the two colour passes, without
+// the prolongation's read and write of both colours.
+template <typename CT, bool TOP, bool NBE>
+__global__ void __launch_bounds__(256) k_rb_colour4_pro(RbGeom r, float *__restrict__ u, const float *__restrict__ f,
+                                                        Coef<CT> cf, float h2, int colour, float om, CoarseE ce)
+{
+    const uint3 tile = xcd_tile();
+    const int m0 = 4 * (int)(tile.x * blockDim.x + threadIdx.x);
+    const int j = (int)(tile.y * blockDim.y + threadIdx.y) + 2;
+    const int k = (int)tile.z + 2;
+    if (m0 >= r.mh || j > r.ny - 1) return;
+    const size_t row = ((size_t)(k - 2) * r.my + (j - 2)) * r.mh;
+    const size_t own = (size_t)colour * r.half + row + m0;
+    const size_t oth = (size_t)(1 - colour) * r.half + row + m0;
+    const NbGlobal nbg{u + oth, (size_t)r.mh, (size_t)r.mh * r.my};
+    float eo[4], out[4];
+    for (int q = 0; q < 4; q++) eo[q] = ce.at(m0 + q, j, k);
+    if (NBE) {
+        const NbProlong nb{nbg, ce, m0, j, k, eo};
+        rb_group4<CT, false>(r, nb, f, u, cf, own, oth, h2, colour, om, 0, m0, j, k, out, eo);
+    } else {
+        rb_group4<CT, false>(r, nbg, f, u, cf, own, oth, h2, colour, om, 0, m0, j, k, out, eo);
+    }
+    *reinterpret_cast<float4 *>(u + own) = make_float4(out[0], out[1], out[2], out[3]);
 }
 
 // The V-cycle below a small level in one workgroup: every colour pass,
@@ -726,6 +793,34 @@ int colour_pass(Ctx &c, int l, int colour, bool zero, int zown, double *bytes_su
     return 0;
 }
 
+// The first post-smoothing sweep of level l (colour 1, then colour 0) with
+// the prolongation from the level below folded in (rows of a multiple of 4
+// entries)
+template <typename CT>
+int colour_pass_pro(Ctx &c, int l, const CoarseE &ce, double *bytes_sum)
+{
+    const float om = (float)c.mg_omega;
+    Level &L = c.lev[l];
+    const RbGeom r = rb_geom(L.g);
+    const Coef<CT> cf{(const CT *)L.rb_coef, 2 * r.half};
+    const float h2 = (float)(1.0 / lev_h2i(c, L.g));
+    const bool top = l == c.ng - 1;
+    // two colour passes' bytes, each also reading the coarse level (FP32,
+    // a quarter as many cells as one fine colour)
+    const double cells = (double)r.half;
+    *bytes_sum += 2.0 * (cells * (om == 1.f ? 12.0 : 16.0) + 6.0 * cells * (double)sizeof(CT) + cells);
+    const dim3 b4 = row_block(r.mh / 4), g4 = row_grid(b4, r.mh / 4, r.my, r.mz);
+    if (top) {
+        hipLaunchKernelGGL((k_rb_colour4_pro<CT, true, true>), g4, b4, 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, 1, om, ce);
+        hipLaunchKernelGGL((k_rb_colour4_pro<CT, true, false>), g4, b4, 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, 0, om, ce);
+    } else {
+        hipLaunchKernelGGL((k_rb_colour4_pro<CT, false, true>), g4, b4, 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, 1, om, ce);
+        hipLaunchKernelGGL((k_rb_colour4_pro<CT, false, false>), g4, b4, 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, 0, om, ce);
+    }
+    DG_CHECK_LAUNCH();
+    return 0;
+}
+
 // highest level taken into the single-workgroup tail, or -1
 int tail_level(const Ctx &c)
 {
@@ -894,10 +989,15 @@ int vcycle_rb(Ctx &c, int l)
     } else {
         if (coarse_solve_f32(c)) return -1;
     }
-    const dim3 bf = row_block(rf.mh);
-    hipLaunchKernelGGL(k_rb_prolong, row_grid(bf, rf.mh, rf.my, rf.mz), bf, 0, c.stream, rf, cur, rc,
-                       csplit ? C.rb_u : nullptr, C.g, csplit ? nullptr : C.u32);
-    DG_CHECK_LAUNCH();
+    const CoarseE ce{rc, csplit ? C.rb_u : nullptr, C.g, csplit ? nullptr : C.u32};
+    // the colour passes take the prolongation into their first sweep
+    const bool pro_folded = !fused && nu2 >= 1 && rf.mh % 4 == 0;
+    if (!pro_folded) {
+        const dim3 bf = row_block(rf.mh);
+        hipLaunchKernelGGL(k_rb_prolong, row_grid(bf, rf.mh, rf.my, rf.mz), bf, 0, c.stream, rf, cur, rc,
+                           csplit ? C.rb_u : nullptr, C.g, csplit ? nullptr : C.u32);
+        DG_CHECK_LAUNCH();
+    }
     if (fused) {
         for (int q = 0; q < nu2; q++) {
             float *y = other(cur);
@@ -912,6 +1012,10 @@ int vcycle_rb(Ctx &c, int l)
         double pb = 0.0;
         if (top) prof_begin(PK_RB_COLOUR, c.stream);
         for (int q = 0; q < nu2; q++) {
+            if (q == 0 && pro_folded) {
+                if (colour_pass_pro<CT>(c, l, ce, &pb)) return -1;
+                continue;
+            }
             if (colour_pass<CT>(c, l, 1, false, 0, &pb)) return -1;
             if (colour_pass<CT>(c, l, 0, false, 0, &pb)) return -1;
         }

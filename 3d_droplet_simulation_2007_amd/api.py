@@ -77,6 +77,19 @@ def _p(a):
     return a.ctypes.data_as(_dp)
 
 
+def _shapes(what, shape, *arrays):
+    """Every array must have `shape`: the library reads and writes that many
+    doubles through each pointer."""
+    for q, a in enumerate(arrays):
+        if a.ndim != 3 or tuple(a.shape) != tuple(shape):
+            raise DropletError(f"{what}: array {q + 1} has shape {a.shape}, expected {tuple(shape)}")
+
+
+def _coarse(shape):
+    """Extents of the next-coarser level: n -> n/2 + 1 (memoryaloc, main.f:274-322)."""
+    return tuple(n // 2 + 1 for n in shape)
+
+
 def make_inputdata(nx=66, ny=66, nz=66, tmax=49, cycles=2, omega=1.5, npre=2, ntop=4, npost=2, maxcfl=0.25,
                    iturb=0, convectype=2, hswitch=0, xvel1=0.0, yvel1=-1.0, zvel1=0.0, xvel2=0.0, yvel2=0.0,
                    zvel2=0.0, rho2=1.2, rho1=1000.0, sigma=0.072, visc2=1.8e-5, visc1=1.0e-3, grav=-9.81,
@@ -179,62 +192,78 @@ class DropletGPU:
 # ---- host-pointer kernels with the reference's argument lists -------------
 
 def relax(u, lengthsize, rhs, a, b, c, ei, omega):
+    _shapes("relax", u.shape, u, rhs, a, b, c, ei)
     nz, ny, nx = u.shape
     _check(lib().dg_relax(_p(u), _d(lengthsize), _p(rhs), _p(a), _p(b), _p(c), _p(ei), _i(2), _i(nx), _i(ny),
                           _i(nz), _d(omega)), "dg_relax")
 
 
 def resid(lengthsize, res, u, rhs, a, b, c):
+    _shapes("resid", u.shape, res, u, rhs, a, b, c)
     nz, ny, nx = u.shape
     _check(lib().dg_resid(_d(lengthsize), _p(res), _p(u), _p(rhs), _p(a), _p(b), _p(c), _i(2), _i(nx), _i(ny),
                           _i(nz)), "dg_resid")
 
 
 def rstrct(uc, uf):
+    _shapes("rstrct", uc.shape, uc)
+    _shapes("rstrct", tuple(2 * n - 2 for n in uc.shape), uf)
     ncz, ncy, ncx = uc.shape
     _check(lib().dg_rstrct(_p(uc), _p(uf), _i(2), _i(ncx), _i(ncy), _i(ncz)), "dg_rstrct")
 
 
 def interp(uf, uc):
+    _shapes("interp", uf.shape, uf)
+    _shapes("interp", _coarse(uf.shape), uc)
     nz, ny, nx = uf.shape
     _check(lib().dg_interp(_p(uf), _p(uc), _i(2), _i(nx), _i(ny), _i(nz)), "dg_interp")
 
 
 def addint(uf, uc, res):
+    _shapes("addint", uf.shape, uf, res)
+    _shapes("addint", _coarse(uf.shape), uc)
     nz, ny, nx = uf.shape
     _check(lib().dg_addint(_p(uf), _p(uc), _p(res), _i(2), _i(nx), _i(ny), _i(nz)), "dg_addint")
 
 
 def acfromac(a, b, c, ei, a2, b2, c2):
+    _shapes("acfromac", a.shape, a, b, c, ei)
+    _shapes("acfromac", tuple(2 * n - 2 for n in a.shape), a2, b2, c2)
     nz, ny, nx = a.shape
     _check(lib().dg_acfromac(_p(a), _p(b), _p(c), _p(ei), _p(a2), _p(b2), _p(c2), _i(nx), _i(ny), _i(nz)),
            "dg_acfromac")
 
 
 def slvsml(u, rhs, a, b, c, ei, ntop, omega, lengthsize):
+    _shapes("slvsml", u.shape, u, rhs, a, b, c, ei)
     nz, ny, nx = u.shape
     _check(lib().dg_slvsml(_p(u), _p(rhs), _p(a), _p(b), _p(c), _p(ei), _i(2), _i(nx), _i(ny), _i(nz), _i(ntop),
                            _d(omega), _d(lengthsize)), "dg_slvsml")
 
 
 def abcei(a, b, c, ei, cc, r1, r2):
+    _shapes("abcei", a.shape, a, b, c, ei, cc)
     nz, ny, nx = a.shape
     _check(lib().dg_abcei(_p(a), _p(b), _p(c), _p(ei), _p(cc), _d(r1), _d(r2), _i(2), _i(nx), _i(ny), _i(nz)),
            "dg_abcei")
 
 
 def bc_scalar(s):
+    _shapes("bc_scalar", s.shape, s)
     nz, ny, nx = s.shape
     _check(lib().dg_bc_scalar(_p(s), _i(2), _i(nx), _i(ny), _i(nz)), "dg_bc_scalar")
 
 
 def filterc27(cc, cf):
+    _shapes("filterc27", cc.shape, cc, cf)
     nz, ny, nx = cc.shape
     _check(lib().dg_filterc27(_p(cc), _p(cf), _i(nx), _i(ny), _i(nz)), "dg_filterc27")
 
 
 def al3d(b1, b2, b3, cc):
     b1, b2, b3, cc = (np.ascontiguousarray(x, dtype=np.float64) for x in (b1, b2, b3, cc))
+    if not (b1.size == b2.size == b3.size == cc.size):
+        raise DropletError("al3d: inputs of different sizes")
     out = np.empty_like(b1)
     _check(lib().dg_al3d_batch(_i(b1.size), _p(b1), _p(b2), _p(b3), _p(cc), _p(out)), "dg_al3d_batch")
     return out
@@ -242,6 +271,8 @@ def al3d(b1, b2, b3, cc):
 
 def fl3d(m1, m2, m3, alpha, r0, dr0):
     m1, m2, m3, alpha, r0, dr0 = (np.ascontiguousarray(x, dtype=np.float64) for x in (m1, m2, m3, alpha, r0, dr0))
+    if len({x.size for x in (m1, m2, m3, alpha, r0, dr0)}) != 1:
+        raise DropletError("fl3d: inputs of different sizes")
     out = np.empty_like(m1)
     _check(lib().dg_fl3d_batch(_i(m1.size), _p(m1), _p(m2), _p(m3), _p(alpha), _p(r0), _p(dr0), _p(out)),
            "dg_fl3d_batch")
@@ -252,6 +283,7 @@ def poisson_solve(rhs, a, b, c, p0=None, lengthsize=1.0, mode=2, tol=1e-8, maxit
     """Tolerance-mode solve of resid's operator; returns (p, iterations, relres)."""
     nz, ny, nx = rhs.shape
     p = np.zeros_like(rhs) if p0 is None else np.array(p0, dtype=np.float64, copy=True)
+    _shapes("poisson_solve", rhs.shape, rhs, a, b, c, p)
     it = ctypes.c_int(0)
     rel = ctypes.c_double(0.0)
     _check(lib().dg_poisson_solve(_i(nx), _i(ny), _i(nz), _d(lengthsize), _p(rhs), _p(a), _p(b), _p(c), _p(p),

@@ -153,7 +153,7 @@ static void free_ctx(Ctx &c)
         L = Level();
     }
     void *fs[] = {c.u, c.v, c.w, c.s11, c.s22, c.s33, c.s12, c.s13, c.s23, c.rhon, c.mun, c.mut, c.ktur,
-                  c.eps, c.cmu, c.ccf, c.pprev, c.pcg_r, c.pcg_p, c.pcg_q, c.red_partial, c.dscal};
+                  c.eps, c.cmu, c.ccf, c.pprev, c.pcg_r, c.pcg_p, c.pcg_q, c.red_partial, c.dscal, c.coarse_M};
     for (void *p : fs)
         if (p) (void)hipFree(p);
     for (int q = 0; q < 2; q++)
@@ -803,7 +803,13 @@ int dg_poisson_solve(const int *nx, const int *ny, const int *nz, const double *
     lc.coef16_ok = 0;  // arbitrary coefficients: FP32 in the split V-cycle
     lc.mg_omega = g_omega_default;
     lc.mg_nuc = g_nuc_default;
-    if (op_mg_solve_tol(lc)) { free_ctx(lc); return -1; }
+    if (op_mg_solve_tol(lc)) {
+        // the iterations done and the residual reached stay readable
+        *iters = lc.mg_iters_last;
+        *relres = lc.mg_relres_last;
+        free_ctx(lc);
+        return -1;
+    }
     if (hipStreamSynchronize(lc.stream) != hipSuccess ||
         hipMemcpy(p, F.u, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
         set_error("dg_poisson_solve: download failed");

@@ -55,6 +55,8 @@ struct Ctx {
     double *hscal = nullptr;        // pinned host mirror of dscal
     double *hpoll = nullptr;        // mapped coherent host slots: [1] the count of PCG iterations done, [2 + 2 (n & 1)] and the next: r.r and the stopping flag of iteration n
     double *dpoll = nullptr;        // the same slots' device address
+    double *coarse_M = nullptr;     // Cholesky factor of the coarsest level's -A (tolerance mode), n0 x n0
+    size_t coarse_msize = 0;        // doubles allocated for coarse_M
     Scalars s{};
     // pressure-solver options
     int mg_mode = 0;       // 0 reference FMG schedule, 1 FP64 PCG, 2 FP32-preconditioned FP64 PCG

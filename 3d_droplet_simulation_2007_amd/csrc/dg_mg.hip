@@ -399,10 +399,14 @@ __global__ void k_to_f32(size_t n, const double *src, float *dst)
 // coarsest level: assemble -A (SPD) densely and factor it (Cholesky), one
 // block; the factor is kept for the forward/back substitutions of every
 // V-cycle of the solve
+// info: 0 on success, else the 1-based index of the first non-positive
+// pivot (written by thread 0 only; cleared here, so every solve reports its
+// own factorisation)
 __global__ void __launch_bounds__(256) k_coarse_factor(Grid g, const double *a, const double *b, const double *c,
-                                                      double h2i, double *M, int *info)
+                                                      double h2i, double *M, double *info)
 {
     const int mx = g.nx - 2, my = g.ny - 2, mz = g.nz - 2, n = mx * my * mz;
+    if (threadIdx.x == 0) *info = 0.0;
     for (int p = threadIdx.x; p < n * n; p += blockDim.x) M[p] = 0.0;
     __syncthreads();
     for (int row = threadIdx.x; row < n; row += blockDim.x) {
@@ -422,7 +426,7 @@ __global__ void __launch_bounds__(256) k_coarse_factor(Grid g, const double *a, 
     for (int p = 0; p < n; p++) {
         if (threadIdx.x == 0) {
             double dpp = M[p * n + p];
-            if (!(dpp > 0.0)) *info = p + 1;
+            if (!(dpp > 0.0) && *info == 0.0) *info = (double)(p + 1);
             M[p * n + p] = sqrt(dpp);
         }
         __syncthreads();
@@ -445,20 +449,16 @@ __global__ void __launch_bounds__(64) k_coarse_solve(Grid g, const double *M, co
     coarse_subst<T>(g, M, f, u, y, (int)threadIdx.x, 64);
 }
 
-struct TolWork {
-    double *M = nullptr;
-    int *info = nullptr;
-    size_t msize = 0;
-};
-static TolWork g_tw;
+// The coarse factor belongs to the context: a captured PCG graph bakes its
+// address into the coarse-solve kernels, so it lives and dies with the
+// context's graphs and is never moved while they exist.
+const double *coarse_factor(const Ctx &c) { return c.coarse_M; }
 
 // coarsest-level FP32 solve (natural layout f32 -> u32) for the split V-cycle
-const double *coarse_factor() { return g_tw.M; }
-
 int coarse_solve_f32(Ctx &c)
 {
     Level &L = c.lev[0];
-    hipLaunchKernelGGL(k_coarse_solve<float>, dim3(1), dim3(64), 0, c.stream, L.g, g_tw.M, L.f32, L.u32);
+    hipLaunchKernelGGL(k_coarse_solve<float>, dim3(1), dim3(64), 0, c.stream, L.g, c.coarse_M, L.f32, L.u32);
     DG_CHECK_LAUNCH();
     return 0;
 }
@@ -550,7 +550,7 @@ static int vcycle(Ctx &c, int l)
     const T *a, *b, *cc;
     level_ptrs<T>(L, u, f, r, a, b, cc, w);
     if (l == 0) {
-        hipLaunchKernelGGL(k_coarse_solve<T>, dim3(1), dim3(64), 0, c.stream, g, g_tw.M, f, u);
+        hipLaunchKernelGGL(k_coarse_solve<T>, dim3(1), dim3(64), 0, c.stream, g, c.coarse_M, f, u);
         DG_CHECK_LAUNCH();
         return 0;
     }
@@ -690,7 +690,7 @@ __global__ void __launch_bounds__(256) k_final2(const double *partial, int nb, d
 }
 
 // device scalar slots
-enum { S_RZ = 16, S_ZQ, S_RZOLD, S_ALPHA, S_PQ, S_RR, S_BB, S_FIRST, S_IT, S_STOP, S_TOL, S_AH = 32 };
+enum { S_RZ = 16, S_ZQ, S_RZOLD, S_ALPHA, S_PQ, S_RR, S_BB, S_FIRST, S_IT, S_STOP, S_TOL, S_CINFO, S_AH = 32 };
 
 // The search directions of a solve go to a ring of PCG_HIST buffers and
 // their step lengths to ds[S_AH ..]; x takes x + alpha_j p_j for the
@@ -863,8 +863,9 @@ static int sum_partials(hipStream_t s, const double *partial, int nb, int stride
 }
 
 // scalar bookkeeping after K3: alpha_old = rz/pq, rz_old = rz, first = 0,
-// and the stopping test: stop = !(|r| / |b| >= tol), the host's own test on
-// the same doubles.  An iteration that finds stop set was queued ahead of
+// and the stopping test on the same doubles as the host's: stop = 1 when
+// |r| / |b| < tol, 2 when r.r is not finite (the solve has failed; the host
+// reports it), 0 otherwise.  An iteration that finds stop set was queued ahead of
 // the host's test and comes after convergence: it records nothing, so the
 // flushes and the returned residual see only the iterations before it.
 // poll: mapped host memory; r.r and the stopping flag into the pair of
@@ -876,7 +877,8 @@ __global__ void k_pcg_scalars(double *ds, double *poll)
     if (ds[S_STOP] != 0.0) return;
     double bnorm = sqrt(ds[S_BB]);
     if (bnorm == 0.0) bnorm = 1.0;
-    const double stop = sqrt(ds[S_RR]) / bnorm >= ds[S_TOL] ? 0.0 : 1.0;
+    const double rr = ds[S_RR];
+    const double stop = !isfinite(rr) ? 2.0 : (sqrt(rr) / bnorm < ds[S_TOL] ? 1.0 : 0.0);
     ds[S_STOP] = stop;
     const int it = (int)ds[S_IT];
     double *pair = poll + 2 + 2 * ((it + 1) & 1);
@@ -1046,11 +1048,14 @@ static int ensure_tol_buffers(Ctx &c)
         set_error("coarsest multigrid level has %zu unknowns (limit 512)", n0);
         return -1;
     }
-    if (g_tw.msize < n0 * n0) {
-        if (g_tw.M) (void)hipFree(g_tw.M);
-        DG_HIP(hipMalloc(&g_tw.M, n0 * n0 * sizeof(double)));
-        g_tw.msize = n0 * n0;
-        if (!g_tw.info) DG_HIP(hipMalloc(&g_tw.info, sizeof(int)));
+    // allocated once per context: the coarsest level's size is fixed by the
+    // grid the context was created for
+    if (!c.coarse_M) {
+        DG_HIP(hipMalloc(&c.coarse_M, n0 * n0 * sizeof(double)));
+        c.coarse_msize = n0 * n0;
+    } else if (c.coarse_msize < n0 * n0) {
+        set_error("coarse factor of %zu doubles is too small for %zu unknowns", c.coarse_msize, n0);
+        return -1;
     }
     return 0;
 }
@@ -1211,8 +1216,8 @@ static int pcg_solve(Ctx &c)
     // coarse factorisation for this solve, FIRST flag
     {
         Level &L0 = c.lev[0];
-        hipLaunchKernelGGL(k_coarse_factor, dim3(1), dim3(256), 0, s, L0.g, L0.a, L0.b, L0.c, lev_h2i(c, L0.g), g_tw.M,
-                           g_tw.info);
+        hipLaunchKernelGGL(k_coarse_factor, dim3(1), dim3(256), 0, s, L0.g, L0.a, L0.b, L0.c, lev_h2i(c, L0.g),
+                           c.coarse_M, ds + S_CINFO);
         DG_CHECK_LAUNCH();
         c.hscal[S_FIRST] = 1.0;
         c.hscal[S_IT] = 0.0;
@@ -1220,11 +1225,25 @@ static int pcg_solve(Ctx &c)
         c.hscal[S_TOL] = c.mg_tol;
         DG_HIP(hipMemcpyAsync(ds + S_FIRST, c.hscal + S_FIRST, 4 * sizeof(double), hipMemcpyHostToDevice, s));
     }
-    DG_HIP(hipMemcpyAsync(c.hscal + S_RR, ds + S_RR, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    // r.r, b.b (S_RR, S_BB) and the factorisation's info (S_CINFO)
+    static_assert(S_CINFO == S_RR + 6, "one copy covers r.r .. the coarse factor's info");
+    DG_HIP(hipMemcpyAsync(c.hscal + S_RR, ds + S_RR, 7 * sizeof(double), hipMemcpyDeviceToHost, s));
     DG_HIP(hipStreamSynchronize(s));
     double bnorm = sqrt(c.hscal[S_RR + 1]);
     if (bnorm == 0.0) bnorm = 1.0;
     double rel = sqrt(c.hscal[S_RR]) / bnorm;
+    c.mg_iters_last = 0;
+    c.mg_relres_last = rel;
+    if (!std::isfinite(c.hscal[S_RR + 1]) || !std::isfinite(rel)) {
+        set_error("pressure solve: non-finite right-hand side or initial residual (|b|^2 = %g, |r|/|b| = %g)",
+                  c.hscal[S_RR + 1], rel);
+        return -1;
+    }
+    if (c.hscal[S_CINFO] != 0.0) {
+        set_error("pressure solve: coarsest-level Cholesky factorisation failed at pivot %d",
+                  (int)c.hscal[S_CINFO]);
+        return -1;
+    }
     const bool graphs = c.use_graphs && prof().kernel == PK_NONE;
     if (graphs && pcg_graphs<T>(c)) return -1;
     // x += alpha_j p_j over the ring's first count slots
@@ -1287,8 +1306,17 @@ static int pcg_solve(Ctx &c)
     }
     c.mg_iters_last = it;
     c.mg_relres_last = rel;
+    if (!std::isfinite(rel)) {
+        set_error("pressure solve: non-finite residual after %d PCG iterations", it);
+        return -1;
+    }
+    if (!(rel < c.mg_tol)) {
+        set_error("pressure solve: |r|/|b| = %.3e after %d PCG iterations (maxit %d) has not reached tol %.3e", rel,
+                  it, c.mg_maxit, c.mg_tol);
+        return -1;
+    }
     if (op_bc_scalar(g, s, x)) return -1;
-    if (hooks && guess_solve_end(c, x, h2i, rel < c.mg_tol)) return -1;
+    if (hooks && guess_solve_end(c, x, h2i, true)) return -1;
     DG_HIP(hipMemcpyAsync(c.pprev, x, g.n() * sizeof(double), hipMemcpyDeviceToDevice, s));
     return 0;
 }

@@ -15,7 +15,7 @@
 namespace dg {
 
 int coarse_solve_f32(Ctx &c);     // dg_mg.hip
-const double *coarse_factor();  // dg_mg.hip: Cholesky factor of the coarsest level
+const double *coarse_factor(const Ctx &c);  // dg_mg.hip: Cholesky factor of the coarsest level
 
 namespace {
 
@@ -855,7 +855,7 @@ int launch_tail(Ctx &c, int lt)
     t.g0 = g0;
     t.f0 = c.lev[0].f32;
     t.u0 = c.lev[0].u32;
-    t.M = coarse_factor();
+    t.M = coarse_factor(c);
     // LDS form when the device's per-workgroup LDS holds the staged levels
     // next to the 4 KB substitution buffer
     static int lds_max = -1;

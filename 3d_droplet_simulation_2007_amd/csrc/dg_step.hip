@@ -784,9 +784,7 @@ int op_cfl(Ctx &c, double *out_host)
     // max is order independent: one pass over the three components
     if (reduce_interior<true>(g, c.stream, c.red_partial, c.dscal + 8, [=] __device__(int i, int j, int k) {
             const int o = g.at(i, j, k);
-            double a = fabs(u[o]), b = fabs(v[o]), d = fabs(w[o]);
-            double m = a > b ? a : b;
-            return m > d ? m : d;
+            return max_nan(max_nan(fabs(u[o]), fabs(v[o])), fabs(w[o]));
         }))
         return -1;
     DG_HIP(hipMemcpyAsync(c.hscal + 8, c.dscal + 8, sizeof(double), hipMemcpyDeviceToHost, c.stream));
@@ -794,6 +792,12 @@ int op_cfl(Ctx &c, double *out_host)
     double m = c.hscal[8];
     if (m < 0.0) m = 0.0;  // CFL starts from 0 (main.f:4855)
     *out_host = m * c.s.tauh;
+    // the reference carries a NaN velocity on silently (both tests of
+    // checktime are false); here it ends the step with an error
+    if (!std::isfinite(*out_host)) {
+        set_error("CFL: non-finite velocity (max |u|,|v|,|w| = %g)", m);
+        return -1;
+    }
     return 0;
 }
 
@@ -1366,6 +1370,10 @@ int op_solvelpres(Ctx &c, int t)
         DG_HIP(hipStreamSynchronize(c.stream));
         const double epsdiv = c.s.h * c.hscal[9];
         c.epsdiv_last = epsdiv;
+        if (!std::isfinite(epsdiv)) {
+            set_error("solvelpres: non-finite divergence after the projection (step %d)", t);
+            return -1;
+        }
         if (epsdiv > 25.0 && c.s.ntop < 30) {
             c.s.ntop += 2; c.s.npre += 2; c.s.npost += 2;
         } else if (epsdiv < 2.5 && c.s.ntop > 4) {

@@ -80,6 +80,10 @@ inline int launch_linear(size_t n, hipStream_t s, F f)
     return 0;
 }
 
+// max that keeps a NaN operand (fmax drops it), so a NaN anywhere in a
+// maximum reduction (CFL, divergence) reaches the host
+__host__ __device__ inline double max_nan(double a, double b) { return (a != a || a > b) ? a : b; }
+
 // ---- block reductions (wave64 shuffles, then LDS across the 4 waves) ----
 __device__ inline double wave_sum(double v)
 {
@@ -88,7 +92,7 @@ __device__ inline double wave_sum(double v)
 }
 __device__ inline double wave_max(double v)
 {
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    for (int o = 32; o > 0; o >>= 1) v = max_nan(v, __shfl_xor(v, o, 64));
     return v;
 }
 
@@ -100,7 +104,7 @@ __device__ inline double block_reduce256(double v, double *lds4)
     if (l == 0) lds4[w] = v;
     __syncthreads();
     double r = lds4[0];
-    for (int q = 1; q < 4; q++) r = IS_MAX ? fmax(r, lds4[q]) : r + lds4[q];
+    for (int q = 1; q < 4; q++) r = IS_MAX ? max_nan(r, lds4[q]) : r + lds4[q];
     __syncthreads();
     return r;
 }
@@ -123,7 +127,7 @@ __global__ void __launch_bounds__(256) k_reduce_interior(Grid g, F value, double
         int j = (int)(q % my) + 2;
         int k = (int)(q / my) + 2;
         double v = value(i, j, k);
-        acc = IS_MAX ? fmax(acc, v) : acc + v;
+        acc = IS_MAX ? max_nan(acc, v) : acc + v;
     }
     double r = block_reduce256<IS_MAX>(acc, lds4);
     if (threadIdx.x == 0) partial[blockIdx.x] = r;
@@ -134,7 +138,7 @@ __global__ void __launch_bounds__(256) k_reduce_final(const double *partial, int
 {
     __shared__ double lds4[4];
     double acc = IS_MAX ? -1.0 : 0.0;
-    for (int p = threadIdx.x; p < n; p += 256) acc = IS_MAX ? fmax(acc, partial[p]) : acc + partial[p];
+    for (int p = threadIdx.x; p < n; p += 256) acc = IS_MAX ? max_nan(acc, partial[p]) : acc + partial[p];
     double r = block_reduce256<IS_MAX>(acc, lds4);
     if (threadIdx.x == 0) *out = r;
 }

@@ -56,6 +56,9 @@ CASES = [
     ("66x34x34 shipped-size grid", dict(nx=66, ny=34, nz=34, y0=0.0015, radius=0.0005), 6),
     ("18^3 upwind", dict(nx=18, ny=18, nz=18, convectype=1), 10),
     ("34x18x18 no convection, heavier drop", dict(nx=34, ny=18, nz=18, convectype=0, rho2=500.0, y0=0.002), 8),
+    # bc_v's inflow parabola (main.f:4078) and the predictor's x-gravity
+    # (main.f:3322-3326) non-zero
+    ("18^3 inflow xvel2=0.5, gravity on x", dict(nx=18, ny=18, nz=18, xvel2=0.5, grav1=-2.0), 6),
 ]
 
 
@@ -113,9 +116,10 @@ def test_step_parity_keps(dg):
 @pytest.mark.parametrize("mode", [1, 2])
 def test_step_tolerance_mode(dg, mode):
     """Tolerance-mode solver: each step's Poisson solve reaches 1e-8 and the
-    fields agree with the CPU statement of the same method (different
-    reduction order, coarsest factorisation and, in mode 2, FP16 face
-    coefficients in the preconditioner: agreement to a tolerance).
+    fields agree with the CPU statement of the same method.  Mode 1 (FP64
+    V-cycle) differs from it only in the dot products' summation order:
+    1e-12 (measured ~1e-14).  Mode 2's preconditioner stores FP16 face
+    coefficients on the GPU and FP32 in the statement: 1e-6.
 
     The VOF guard is on: without it a nearly full cell (1 - 1e-16) inside
     the drop has a vanishing Youngs normal, AL3D/FL3D's 0/0 empties it, and
@@ -128,7 +132,7 @@ def test_step_tolerance_mode(dg, mode):
     assert st["mg_relres"] < 1e-8 and st["mg_iters"] > 0
     print({k: f"{v:.2e}" for k, v in errs.items()}, "iters", st["mg_iters"], o.st.mg_cycles_last, flush=True)
     for f, e in errs.items():
-        assert e <= 1e-6, (f, e)
+        assert e <= (1e-12 if mode == 1 else 1e-9), (f, e)
     g.close()
 
 

@@ -71,6 +71,7 @@ def build(verbose=False):
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 HOST_DIR = os.path.join(HERE, "host")
 HOST_BIN = os.path.join(HERE, "droplet_host")
+DROPIN_BIN = os.path.join(HERE, "dropin_f77")
 
 
 def build_fortran_host(verbose=False):
@@ -88,13 +89,22 @@ def build_fortran_host(verbose=False):
         [FLANG, "-O2", "-J", OBJDIR, "-c", os.path.join(HOST_DIR, "droplet_host.f90"), "-o", host_o],
         [FLANG, "-o", HOST_BIN + ".tmp", host_o, mod_o, f"-L{HERE}", "-ldroplet_gpu", "-Wl,-rpath,$ORIGIN"],
     ]
+    # the Fortran-77 program on the reference's own call forms (implicit
+    # interfaces, COMMON blocks of input.h's layout)
+    f77_dir = os.path.join(HOST_DIR, "f77")
+    f77_o = os.path.join(OBJDIR, "dropin_f77.o")
+    steps += [
+        [FLANG, "-O2", "-I", f77_dir, "-c", os.path.join(f77_dir, "dropin.f"), "-o", f77_o],
+        [FLANG, "-o", DROPIN_BIN + ".tmp", f77_o, f"-L{HERE}", "-ldroplet_gpu", "-Wl,-rpath,$ORIGIN"],
+    ]
     for cmd in steps:
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"flang failed: {' '.join(cmd)}\n{r.stderr[-4000:]}")
     os.replace(HOST_BIN + ".tmp", HOST_BIN)
+    os.replace(DROPIN_BIN + ".tmp", DROPIN_BIN)
     if verbose:
-        print(f"built {HOST_BIN}")
+        print(f"built {HOST_BIN} and {DROPIN_BIN}")
     return HOST_BIN
 
 

@@ -16,6 +16,7 @@
 #include "dg_ctx.h"
 #include "dg_prof.h"
 #include "dg_util.h"
+#include "dg_resident.h"
 
 namespace dg {
 
@@ -166,19 +167,7 @@ static void free_ctx(Ctx &c)
 
 // ---- readinput (main.f:185-272): first token of each of the 38 lines ----
 
-struct Input {
-    int nx, ny, nz, tmax, nfilter, cycles;
-    double omega;
-    int npre, ntop, npost, restart;
-    double maxcfl;
-    int iturb, convectype, hswitch;
-    double xvel1, yvel1, zvel1, xvel2, yvel2, zvel2, rho2, rho1, sigma, visc2, visc1, grav, grav1;
-    int icase, ibc;
-    double length, radius, y0, z0;
-    int tprint, ntecplot, tsave, surfacet;
-};
-
-static int read_input(const char *path, Input &in)
+int read_input(const char *path, Input &in)
 {
     FILE *f = fopen(path, "r");
     if (!f) {
@@ -219,7 +208,7 @@ static int read_input(const char *path, Input &in)
 }
 
 // nondim (main.f:341-446): host arithmetic in the reference's order
-static void nondim(const Input &in, Scalars &s)
+void nondim(const Input &in, Scalars &s)
 {
     const double L = in.length;
     s.lengthscale = L;
@@ -355,7 +344,63 @@ extern "C" {
 int dg_status(void) { return g_status; }
 const char *dg_last_error(void) { return g_err; }
 
-int dg_init(const char *path)
+int dg_init(const char *path) { return init_from_input(path, nullptr); }
+
+int dg_finalize(void)
+{
+    Ctx &c = *ctx();
+    if (g_ready) free_ctx(c);
+    g_ready = false;
+    return 0;
+}
+}  // extern "C"
+
+namespace dg {
+
+// A fresh resident state of nx x ny x nz (every field zero, the tuning
+// defaults applied); the caller sets the scalars, then calls
+// resident_scalars_changed().
+int resident_setup(int nx, int ny, int nz)
+{
+    Ctx &c = *ctx();
+    if (g_ready) {
+        free_ctx(c);
+        g_ready = false;
+    }
+    g_status = 0;
+    g_err[0] = 0;
+    if (check_dims(nx, ny, nz)) return -1;
+    // m*2**k + 2 grids are accepted for the tolerance mode; the reference's
+    // 2**ng + 2 check moves to its FMG schedule (op_mglin)
+    if (alloc_levels(c, nx, ny, nz, false)) { free_ctx(c); return -1; }
+    if (alloc_fields(c)) { free_ctx(c); return -1; }
+    c.fused_smoother = g_fused_default;
+    c.use_graphs = g_graphs_default;
+    c.rb_split = g_split_default;
+    c.rb_tail = g_tail_default;
+    c.pcg_lookahead = g_lookahead_default;
+    c.mg_omega = g_omega_default;
+    c.mg_nuc = g_nuc_default;
+    g_ready = true;
+    return 0;
+}
+
+// what follows from the physical scalars: FP16 face coefficients are used
+// when 1/rho of both fluids fits FP16's normal range with margin (rho1 = 1
+// after nondim)
+void resident_scalars_changed()
+{
+    Ctx &c = *ctx();
+    const double rlo = c.s.rho2 < 1.0 ? c.s.rho2 : 1.0, rhi = c.s.rho2 > 1.0 ? c.s.rho2 : 1.0;
+    c.coef16_ok = (rlo > 0.0 && 1.0 / rlo <= 3.0e4 && 1.0 / rhi >= 1.0e-3) ? 1 : 0;
+}
+
+bool resident_ready() { return g_ready; }
+double *resident_field(int id) { return g_ready ? field_ptr(*ctx(), id) : nullptr; }
+
+// dg_init: readinput of the 38-line inputdata.in, memoryaloc, nondim and the
+// droplet IC on the device; the parsed input through in_out
+int init_from_input(const char *path, Input *in_out)
 {
     Ctx &c = *ctx();
     if (g_ready) {
@@ -386,40 +431,20 @@ int dg_init(const char *path)
         set_error("iturb=%d: only laminar (0) and k-epsilon (3) are on this path", in.iturb);
         return -1;
     }
-    if (check_dims(in.nx, in.ny, in.nz)) return -1;
-    // m*2**k + 2 grids are accepted for the tolerance mode; the reference's
-    // 2**ng + 2 check moves to its FMG schedule (op_mglin)
-    if (alloc_levels(c, in.nx, in.ny, in.nz, false)) return -1;
-    if (alloc_fields(c)) return -1;
-    c.fused_smoother = g_fused_default;
-    c.use_graphs = g_graphs_default;
-    c.rb_split = g_split_default;
-    c.rb_tail = g_tail_default;
-    c.pcg_lookahead = g_lookahead_default;
-    c.mg_omega = g_omega_default;
-    c.mg_nuc = g_nuc_default;
+    if (resident_setup(in.nx, in.ny, in.nz)) return -1;
     nondim(in, c.s);
-    {
-        // face coefficients lie between 1/rho of the two fluids (rho1 = 1
-        // after nondim): FP16 holds them when that range is moderate
-        const double rlo = c.s.rho2 < 1.0 ? c.s.rho2 : 1.0, rhi = c.s.rho2 > 1.0 ? c.s.rho2 : 1.0;
-        c.coef16_ok = (rlo > 0.0 && 1.0 / rlo <= 3.0e4 && 1.0 / rhi >= 1.0e-3) ? 1 : 0;
-    }
+    resident_scalars_changed();
     // cmu = 0.09 (main.f:347-353); mut, turbst and the Reynolds stresses start at 0
     if (mg_fill(c.g.n(), c.stream, c.cmu, 0.09)) return -1;
-    g_ready = true;
     if (op_smooth2(c) || op_bc_c(c, c.lev[c.ng - 1].cc) || op_bc_v(c) || op_calcrho(c)) return -1;
     DG_HIP(hipStreamSynchronize(c.stream));
+    if (in_out) *in_out = in;
     return 0;
 }
 
-int dg_finalize(void)
-{
-    Ctx &c = *ctx();
-    if (g_ready) free_ctx(c);
-    g_ready = false;
-    return 0;
-}
+}  // namespace dg
+
+extern "C" {
 
 int dg_vofmethod(const int *t)
 {

@@ -15,6 +15,7 @@
 
 #include "dg_ctx.h"
 #include "dg_rb.h"
+#include "dg_resident.h"
 #include "dg_util.h"
 
 namespace dg {
@@ -610,6 +611,15 @@ static int swp_step(Ctx &c, const double *us, double *vof1, double *vof3, int d)
     L.cc = c.ccf;
     c.ccf = t;
     return op_bc_c(c, L.cc);
+}
+
+int op_swp(Ctx &c, const double *us, double *cc, double *vof1, double *vof2, double *vof3, int d)
+{
+    if (d < 1 || d > 3) {
+        set_error("swp: direction %d is not 1, 2 or 3", d);
+        return -1;
+    }
+    return swp(c, us, cc, vof1, vof2, vof3, d);
 }
 
 // fturbdif (main.f:1239-1267): in-place update with k innermost, then j, so
@@ -1295,6 +1305,8 @@ static int convective(Ctx &c, const double *w1, const double *w2, const double *
         w[o] = nw;
     });
 }
+
+int op_convective(Ctx &c, const double *w1, const double *w2, const double *w3) { return convective(c, w1, w2, w3); }
 
 static int divergence(Ctx &c, double *w3)
 {

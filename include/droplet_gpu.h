@@ -227,6 +227,78 @@ int dg_conv3d_dgrad(const int *nx, const int *ny, const int *nz, const float *dy
 int dg_conv3d_wgrad(const int *nx, const int *ny, const int *nz, const float *x, const int *cinp, const float *dy,
                     const int *coutp, float *gw, float *gb);
 
+/* ---- the reference's own call forms (Fortran-77, implicit interfaces) --- */
+/* Definitions of the time step's procedures under their Fortran names, with
+ * their argument lists, for a program built from main.f: take these units
+ * out of main.f and link it against this library (oracle/link_dropin.py
+ * does exactly that).  State lives in the program's COMMON blocks
+ * (input.h:19-95), which the library binds through their symbols
+ * (memory_, pointers_, surfcom_, turbulence_, dimension_, diff_, cycling_,
+ * times_, dinem_, physics_).  The first call uploads every field and
+ * multigrid heap slot; afterwards the device copies are authoritative, an
+ * array argument names its resident copy by address (z(icc(ng)), u, ...;
+ * any other address is an error), the scalars tau, time, tauh, tauh2,
+ * tswap, NPRE, NTOP, NPOST are read from COMMON before each call and written
+ * back after it, and fields return to host memory through
+ * dg_sync_to_host_.  On an error these procedures print the message and
+ * stop the program with exit status 2, as the reference STOPs, unless
+ * dg_set_stop_on_error_(0) was called (then dg_status_ returns -1). */
+void vofmethod_(const int *t);                                   /* VOFmethod(t)        main.f:795 */
+void kepsmodel_(const int *t, double *kturnew, double *epsnew);  /* kepsmodel(t,..)     main.f:1888 */
+void checktime_(const int *t);                                   /* checktime(t)        main.f:2156 */
+double cfl_(void);                                               /* CFL()               main.f:4848 */
+void csf_(double *ccc, double *w11, double *w22, double *w33);   /* csf(ccc,w11,w22,w33) main.f:2363 */
+void solvelpres_(double *cc, double *w1, double *w2, double *w3, double *a, double *b, double *c, double *ei,
+                 const int *t);                                  /* solvelpres(...,t)   main.f:3236 */
+void calcrho_(double *cc);                                       /* calcrho(cc)         main.f:2177 */
+void bc_c_(double *cc);                                          /* bc_c(cc)            main.f:4331 */
+void mglin_(const double *r1, const double *r2);                 /* mglin(r1,r2)        main.f:3645 */
+void swp_(double *us, double *cc, double *vof1, double *vof2, double *vof3, const int *d); /* swp main.f:848 */
+void convective_(double *w1, double *w2, double *w3);            /* convective(w1,w2,w3) main.f:3441 */
+/* The fine-grained routines under their own names, on host arrays (upload,
+ * run, download; no COMMON needed), with the argument lists of the dg_*
+ * host-pointer kernels above.  For unit drop-in and tests. */
+void relax_(double *u, const double *lengthsize, const double *rhs, const double *a, const double *b,
+            const double *c, const double *ei, const int *ibc, const int *nx, const int *ny, const int *nz,
+            const double *omega);                                        /* relax      main.f:3956 */
+void resid_(const double *lengthsize, double *res, const double *u, const double *rhs, const double *a,
+            const double *b, const double *c, const int *ibc, const int *nx, const int *ny,
+            const int *nz);                                              /* resid      main.f:3902 */
+void rstrct_(double *uc, const double *uf, const int *ibc, const int *ncx, const int *ncy,
+             const int *ncz);                                            /* rstrct     main.f:3765 */
+void interp_(double *uf, const double *uc, const int *ibc, const int *nfx, const int *nfy,
+             const int *nfz);                                            /* interp     main.f:3866 */
+void addint_(double *uf, const double *uc, double *res, const int *ibc, const int *nfx, const int *nfy,
+             const int *nfz);                                            /* addint     main.f:3934 */
+void acfromac_(double *a, double *b, double *c, double *ei, const double *a2, const double *b2,
+               const double *c2, const int *nx, const int *ny, const int *nz); /* acfromac main.f:3793 */
+void slvsml_(double *u, const double *rhs, const double *a, const double *b, const double *c, const double *ei,
+             const int *ibc, const int *nx, const int *ny, const int *nz, const int *ntop, const double *omega,
+             const double *lengthsize);                                  /* slvsml     main.f:3846 */
+void abcei_(double *a, double *b, double *c, double *ei, const double *cc, const double *r1, const double *r2,
+            const int *ibc, const int *nx, const int *ny, const int *nz); /* abcei     main.f:3582 */
+void bc_scalar_(double *scal, const int *ibc, const int *nx, const int *ny, const int *nz); /* main.f:4298 */
+void filterc27_(const double *cc, double *cfilt, const int *nx, const int *ny, const int *nz); /* main.f:5473 */
+double al3d_(const double *b1, const double *b2, const double *b3, const double *cc); /* AL3D main.f:4562 */
+double fl3d_(const double *m1, const double *m2, const double *m3, const double *alpha, const double *r0,
+             const double *dr0);                                         /* FL3D       main.f:4639 */
+/* Lifecycle of the call forms (no reference counterpart).  dg_attach_:
+ * after the program's initialize (main.f:15), bind COMMON and upload
+ * (otherwise done by the first call).  dg_init_input_: for a program
+ * without the reference's initialize, the library reads ./inputdata.in and
+ * sets up the run (as dg_init) and writes the result into COMMON (scalars,
+ * /pointers/ with MYMALLOC's bookkeeping, every field).  dg_sync_to_host_ /
+ * dg_sync_to_device_(mask): bit id for each DG_F_* field that has a host
+ * array, bit DG_NFIELDS for cmu, bit 30 for every level's heap slots, -1
+ * for all. */
+void dg_attach_(void);
+void dg_init_input_(void);
+void dg_sync_to_host_(const int *mask);
+void dg_sync_to_device_(const int *mask);
+int dg_status_(void);
+void dg_set_stop_on_error_(const int *on);
+void dg_finalize_(void);
+
 /* ---- measurement (no reference counterpart) ---------------------------- */
 /* Times every launch of one kernel family with HIP events recorded on the
  * library's stream until dg_profile_stop, which returns the summed kernel

@@ -1,0 +1,83 @@
+"""A Fortran-77 program on the reference's own call forms
+(host/f77/dropin.f: implicit interfaces, arrays passed as z(icc(ng)) and the
+like, state in COMMON blocks of input.h's layout) runs the time loop of
+main.f:19-73 against libdroplet_gpu.so; its fields and scalars match the CPU
+restatement to the whole-step parity bar (relative L2 <= 1e-10 for cc, u,
+v, w, p; tau to 1e-12; NPRE/NTOP/NPOST and tswap equal).
+
+Variant 1 calls VOFmethod, checktime, csf and solvelpres as the main loop
+does; variant 2 writes VOFmethod's body out (three swp calls in tswap order,
+bc_c, calcrho) and calls the CFL function between them, as printout does.
+"""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from oracle_ffi import Oracle, inputdata_text
+
+pytestmark = pytest.mark.gpu
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PROG = os.path.join(REPO, "3d_droplet_simulation_2007_amd", "dropin_f77")
+NX, NY, NZ = 34, 18, 18  # the extents host/f77/dropin.h declares
+
+
+def run_program(variant, txt):
+    with tempfile.TemporaryDirectory() as d:
+        with open(os.path.join(d, "inputdata.in"), "w") as f:
+            f.write(txt)
+        out = os.path.join(d, "out.bin")
+        r = subprocess.run([PROG, str(variant), out], cwd=d, capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stdout + r.stderr
+        raw = open(out, "rb").read()
+    n = NX * NY * NZ
+    scal = np.frombuffer(raw[:16], dtype=np.float64)
+    ints = np.frombuffer(raw[16:32], dtype=np.int32)
+    f = np.frombuffer(raw[32:], dtype=np.float64)
+    assert f.size == 5 * n
+    fields = {nm: f[q * n:(q + 1) * n].reshape(NZ, NY, NX) for q, nm in enumerate(("cc", "u", "v", "w", "p"))}
+    return r.stdout, dict(tau=scal[0], time=scal[1], npre=int(ints[0]), ntop=int(ints[1]), npost=int(ints[2]),
+                          tswap=int(ints[3])), fields
+
+
+@pytest.mark.parametrize("variant", [1, 2])
+def test_f77_call_forms_match_restatement(dg, variant):
+    if not os.path.exists(PROG):
+        pytest.fail("dropin_f77 was not built (flang missing at build time?)")
+    nsteps = 6
+    print(f"case F77 call forms variant {variant}, {NX}x{NY}x{NZ}, {nsteps} steps", flush=True)
+    # a placement on which the reference stays finite (no vanishing Youngs normal)
+    txt = inputdata_text(nx=NX, ny=NY, nz=NZ, tmax=nsteps - 1, y0=0.0015, radius=0.0005)
+    stdout, sc, fields = run_program(variant, txt)
+    import oracle_ffi
+    L = oracle_ffi.lib()
+    o = Oracle(txt)
+    cfl_o = []
+    for t in range(nsteps):
+        # the step operator by operator (laminar CSF), the CFL where variant 2 prints it
+        L.or_vofmethod(o.h, t)
+        cfl_o.append(L.or_cfl(o.h))
+        L.or_checktime(o.h, t)
+        L.or_csf(o.h)
+        L.or_solvelpres(o.h, t)
+    cfls = []
+    if variant == 2:
+        cfls = [float(l.split()[3]) for l in stdout.splitlines() if l.strip().startswith("step")]
+        assert len(cfls) == nsteps
+        for a, b in zip(cfls, cfl_o):
+            assert abs(a - b) <= 1e-12 * abs(b), (a, b)
+    assert abs(sc["tau"] - o.st.tau) <= 1e-12 * o.st.tau, (sc["tau"], o.st.tau)
+    assert abs(sc["time"] - o.st.time) <= 1e-12 * o.st.time
+    assert (sc["npre"], sc["ntop"], sc["npost"], sc["tswap"]) == (o.st.npre, o.st.ntop, o.st.npost, o.st.tswap)
+    errs = {}
+    for nm, a in fields.items():
+        b = o.field(nm)
+        ai, bi = a[1:-1, 1:-1, 1:-1], b[1:-1, 1:-1, 1:-1]
+        errs[nm] = float(np.linalg.norm(ai - bi) / max(np.linalg.norm(bi), 1e-300))
+    print({k: f"{v:.2e}" for k, v in errs.items()}, "CFL", cfls[-1:] if cfls else "-", flush=True)
+    for nm, e in errs.items():
+        assert e <= 1e-10, (nm, e)
+    o.close()

@@ -128,3 +128,57 @@ def test_plic_functions(dg):
     got = dg.fl3d(mm[:, 0], mm[:, 1], mm[:, 2], al, r0, dr0)
     ref = np.array([OK.fl3d(*mm[q], al[q], r0[q], dr0[q]) for q in range(n)])
     np.testing.assert_allclose(got, ref, rtol=1e-13, atol=1e-15)
+
+
+def test_fortran_names_match_oracle(dg):
+    """The fine-grained routines under the reference's own symbols (relax_,
+    resid_, rstrct_, abcei_, filterc27_, al3d_, fl3d_), called by reference
+    as a Fortran program calls them, against the restatement: bit-exact, and
+    AL3D/FL3D to the same 2e-14 as their batch entry points."""
+    import ctypes
+    print("case Fortran-named kernels 18x18x18", flush=True)
+    L = dg.lib()
+    dp = ctypes.POINTER(ctypes.c_double)
+    P = lambda a: a.ctypes.data_as(dp)  # noqa: E731
+    I = lambda v: ctypes.byref(ctypes.c_int(v))  # noqa: E731
+    D = lambda v: ctypes.byref(ctypes.c_double(v))  # noqa: E731
+    L.dg_set_stop_on_error_(I(0))
+    shape = (18, 18, 18)
+    rng = np.random.default_rng(11)
+    u, rhs = _rand(shape, rng), _rand(shape, rng)
+    a, b, c = _coef(shape, rng), _coef(shape, rng), _coef(shape, rng)
+    ei = 1.0 / (a + b + c)
+    u1, u2 = u.copy(), u.copy()
+    OK.relax(u1, 1.0, rhs, a, b, c, ei, 1.5)
+    L.relax_(P(u2), D(1.0), P(rhs), P(a), P(b), P(c), P(ei), I(2), I(18), I(18), I(18), D(1.5))
+    assert np.array_equal(u1, u2)
+    r1, r2 = np.zeros_like(u), np.zeros_like(u)
+    OK.resid(1.0, r1, u1, rhs, a, b, c)
+    L.resid_(D(1.0), P(r2), P(u2), P(rhs), P(a), P(b), P(c), I(2), I(18), I(18), I(18))
+    assert np.array_equal(r1, r2)
+    uc1, uc2 = np.zeros((10, 10, 10)), np.zeros((10, 10, 10))
+    OK.rstrct(uc1, r1)
+    L.rstrct_(P(uc2), P(r2), I(2), I(10), I(10), I(10))
+    assert np.array_equal(uc1, uc2)
+    cc = rng.uniform(0.0, 1.0, size=(18, 18, 18))
+    f1, f2 = np.zeros_like(cc), np.zeros_like(cc)
+    OK.filterc27(cc, f1)
+    L.filterc27_(P(cc), P(f2), I(18), I(18), I(18))
+    assert np.array_equal(f1, f2)
+    outs = [np.zeros_like(cc) for _ in range(8)]
+    OK.abcei(*outs[:4], cc, 0.4994, 0.0012)
+    L.abcei_(*(P(x) for x in outs[4:]), P(cc), D(0.4994), D(0.0012), I(2), I(18), I(18), I(18))
+    for q in range(4):
+        assert np.array_equal(outs[q], outs[4 + q])
+    import oracle_ffi
+    O = oracle_ffi.lib()
+    L.al3d_.restype = ctypes.c_double
+    L.fl3d_.restype = ctypes.c_double
+    for m1, m2, m3, cv in ((0.1, 0.3, 0.6, 0.2), (0.2, 0.2, 0.6, 0.7), (0.0, 0.4, 0.6, 0.05)):
+        al_g = L.al3d_(D(m1), D(m2), D(m3), D(cv))
+        al_o = O.or_al3d(m1, m2, m3, cv)
+        assert abs(al_g - al_o) <= 2e-14 * abs(al_o), (al_g, al_o)
+        v_g = L.fl3d_(D(m1), D(m2), D(m3), D(al_o), D(0.0), D(1.0))
+        v_o = O.or_fl3d(m1, m2, m3, al_o, 0.0, 1.0)
+        assert abs(v_g - v_o) <= 2e-14 * max(abs(v_o), 1e-300), (v_g, v_o)
+    L.dg_set_stop_on_error_(I(1))

@@ -15,13 +15,14 @@ HEADER = os.path.join(REPO, "include", "droplet_gpu.h")
 
 def declared():
     txt = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(dg_\w+)\s*\(", txt, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|void|double|const char \*)\s*(\w+)\s*\(", txt, flags=re.M)))
 
 
 def test_header_declares_entry_points():
     names = declared()
     for must in ("dg_init", "dg_step", "dg_solvelpres", "dg_vofmethod", "dg_csf", "dg_checktime", "dg_relax",
-                 "dg_poisson_solve", "dg_get_field", "dg_last_error"):
+                 "dg_poisson_solve", "dg_get_field", "dg_last_error", "vofmethod_", "csf_", "solvelpres_",
+                 "checktime_", "kepsmodel_", "cfl_", "dg_sync_to_host_"):
         assert must in names
 
 

@@ -152,6 +152,12 @@ class DropletGPU:
     def calcrho(self):
         _check(lib().dg_calcrho(), "dg_calcrho")
 
+    def cfl(self):
+        """CFL() (main.f:4848): max |u|,|v|,|w| over the interior times tauh."""
+        v = ctypes.c_double()
+        _check(lib().dg_cfl(ctypes.byref(v)), "dg_cfl")
+        return v.value
+
     def clip_cc(self):
         _check(lib().dg_clip_cc(), "dg_clip_cc")
 

@@ -471,6 +471,11 @@ int dg_solvelpres(const int *t)
     if (require_ready()) return -1;
     return op_solvelpres(*ctx(), *t);
 }
+int dg_cfl(double *cfl)
+{
+    if (require_ready()) return -1;
+    return op_cfl(*ctx(), cfl);
+}
 int dg_calcrho(void)
 {
     if (require_ready()) return -1;

@@ -67,6 +67,9 @@ int dg_checktime(const int *t);     /* checktime main.f:2156-2175 */
 int dg_csf(void);                   /* csf main.f:2363-2481 (surfacet=2) */
 int dg_solvelpres(const int *t);    /* solvelpres main.f:3236-3439 */
 int dg_calcrho(void);               /* calcrho main.f:2177-2190 */
+/* CFL() main.f:4848-4881: max(|u|,|v|,|w|) over the interior times tauh,
+ * on the resident velocity; NaN velocities are an error */
+int dg_cfl(double *cfl);
 int dg_mglin(void);                 /* mglin main.f:3645-3763 on the resident finest level */
 
 /* ---- state exchange (replaces reading/writing the COMMON blocks) ------ */

@@ -302,6 +302,23 @@ int dg_status_(void);
 void dg_set_stop_on_error_(const int *on);
 void dg_finalize_(void);
 
+/* ---- no-allocation entry point ------------------------------------------ */
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): CFL() of main.f:4848-4881, as dg_cfl computes it, on
+ * velocity arrays the caller holds in device memory, nx*ny*nz doubles each
+ * in the reference layout.  The result, max(|u|,|v|,|w|) over the interior
+ * times tauh, is written to the device double *cfl; the call enqueues its
+ * kernels on `stream` (a hipStream_t, NULL for the null stream) and returns
+ * without waiting.  Allocates nothing: the reduction's partials live in a
+ * fixed 8200-byte device array that every GPU the library runs on holds
+ * until the process ends, shared by all calls on that GPU (calls that
+ * overlap on one GPU race).  Grids above DG_CFL_NOALLOC_MAX_CELLS cells
+ * (the kernels' 32-bit cell indices) are rejected with an error. */
+#define DG_CFL_NOALLOC_MAX_CELLS 2147483647LL
+int dg_cfl_noalloc(const double *u, const double *v, const double *w, const int *nx, const int *ny, const int *nz,
+                   const double *tauh, double *cfl, void *stream);
+
 /* ---- measurement (no reference counterpart) ---------------------------- */
 /* Times every launch of one kernel family with HIP events recorded on the
  * library's stream until dg_profile_stop, which returns the summed kernel

@@ -148,7 +148,8 @@ static void free_ctx(Ctx &c)
     for (int l = 0; l < c.ng; l++) {
         Level &L = c.lev[l];
         void *ps[] = {L.rhs, L.res, L.u, L.cc, L.a, L.b, L.c, L.ei, L.a32, L.b32, L.c32,
-                      L.u32, L.f32, L.r32, L.u64, L.f64, L.r64, L.w32, L.w64, L.rb_u, L.rb_f, L.rb_w, L.rb_coef};
+                      L.u32, L.f32, L.r32, L.u64, L.f64, L.r64, L.w32, L.w64, L.rb_u, L.rb_f, L.rb_w, L.rb_u64,
+                      L.rb_f64, L.rb_coef};
         for (void *p : ps)
             if (p) (void)hipFree(p);
         L = Level();

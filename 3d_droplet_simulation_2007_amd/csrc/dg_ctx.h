@@ -20,13 +20,18 @@ struct Level {
     float *a32 = nullptr, *b32 = nullptr, *c32 = nullptr;
     float *u32 = nullptr, *f32 = nullptr, *r32 = nullptr, *w32 = nullptr;
     double *u64 = nullptr, *f64 = nullptr, *r64 = nullptr, *w64 = nullptr;
-    // red-black split storage of the FP32 V-cycle (levels above the
-    // coarsest): u and f per colour, a/b/c per colour in FP16 or FP32
+    // red-black split storage of the V-cycle (levels above the coarsest):
+    // u and f per colour (FP32 with the FP32 V-cycle, FP64 with the FP64
+    // one), a/b/c per colour in FP16, FP32 or FP64
     float *rb_u = nullptr, *rb_f = nullptr;
-    float *rb_w = nullptr;  // finest level: second iterate buffer of the fused (out-of-place) sweep
+    float *rb_w = nullptr;  // second iterate buffer of the fused (out-of-place) sweeps
+    double *rb_u64 = nullptr, *rb_f64 = nullptr;
     void *rb_coef = nullptr;
-    int rb_coef16 = -1;  // type rb_coef was allocated for (-1: none)
+    int rb_ct = -1;  // coefficient type rb_coef was allocated for: RB_CT_F32, RB_CT_F16, RB_CT_F64 (-1: none)
 };
+
+// coefficient types of the split V-cycle
+enum { RB_CT_F32 = 0, RB_CT_F16 = 1, RB_CT_F64 = 2 };
 
 struct Scalars {
     // /PHYSICS/, /DIFF/, /DINEM/ values used on the path
@@ -74,7 +79,7 @@ struct Ctx {
                                  // the last residual the host has seen exceeds this multiple of the
                                  // tolerance (<= 0: never)
     int ref_grid = 1;        // the grid has the reference's form nn = 2**ng + 2 (mglin's schedule may run)
-    int rb_split = 1;        // mode 2: red-black split storage of the V-cycle levels
+    int rb_split = 1;        // red-black split storage of the V-cycle levels (modes 1 and 2)
     int rb_tail = 2;         // split levels of <= 4096 cells: 0 per-level launches, 1 one-workgroup tail, 2 LDS tail
     int coef16_ok = 0;       // face coefficients fit FP16 (set from the fluids' densities)
     int top_coef_ready = 0;  // solvelpres's abcei wrote the finest level's split coefficients for this solve
@@ -115,10 +120,10 @@ int mg_interp(const Grid &gf, hipStream_t s, double *uf, const double *uc);
 int mg_addint(const Grid &gf, hipStream_t s, double *uf, const double *uc, double *res);
 int mg_acfromac(const Grid &gc, hipStream_t s, double *a, double *b, double *c, double *ei, const double *a2,
                 const double *b2, const double *c2);
-// split (when given): also the red-black split copy of a, b, c (FP16 when
-// split16, else FP32), three planes of 2 * half entries
+// split (when given): also the red-black split copy of a, b, c of type
+// split_ct (RB_CT_*), three planes of 2 * half entries
 int mg_abcei(const Grid &g, hipStream_t s, double *a, double *b, double *c, double *ei, const double *cc, double r1,
-             double r2, void *split = nullptr, int split16 = 0);
+             double r2, void *split = nullptr, int split_ct = RB_CT_F32);
 int mg_filterc27(const Grid &g, hipStream_t s, const double *cc, double *cf);
 int mg_fill(size_t n, hipStream_t s, double *a, double v);
 int op_mglin(Ctx &c);

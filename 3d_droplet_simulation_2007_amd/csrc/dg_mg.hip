@@ -463,6 +463,15 @@ int coarse_solve_f32(Ctx &c)
     return 0;
 }
 
+// the same in FP64 (f64 -> u64) for the split FP64 V-cycle
+int coarse_solve_f64(Ctx &c)
+{
+    Level &L = c.lev[0];
+    hipLaunchKernelGGL(k_coarse_solve<double>, dim3(1), dim3(64), 0, c.stream, L.g, c.coarse_M, L.f64, L.u64);
+    DG_CHECK_LAUNCH();
+    return 0;
+}
+
 static inline double lev_h2i(const Ctx &c, const Grid &g)
 {
     const double h = c.s.lengthsize / (double)(g.nx - 2);
@@ -795,11 +804,11 @@ __global__ void __launch_bounds__(256) k_pcg_dir(Grid g, Z z, PT *hist, size_t n
 }
 
 // K3: alpha = rz/pq; r -= alpha q (x takes alpha p in k_pcg_flush); partial sums of r.r
-// (fs != null: also the FP32 red-black split copy of the new r, the next
-// V-cycle's right-hand side)
+// (fs != null: also the red-black split copy of the new r in the V-cycle's
+// type PT, the next V-cycle's right-hand side)
 template <typename PT>
 __global__ void __launch_bounds__(256) k_pcg_update(Grid g, double *r, const double *q, double *ds, double *partial,
-                                                    float *fs, RbGeom rg)
+                                                    PT *fs, RbGeom rg)
 {
     __shared__ double lds4[4];
     unsigned lin;
@@ -815,7 +824,7 @@ __global__ void __launch_bounds__(256) k_pcg_update(Grid g, double *r, const dou
         const int o = g.at(i, j, k);
         const double rn = r[o] - alpha * q[o];
         r[o] = rn;
-        if (fs) fs[rg.at32(i, j, k)] = (float)rn;
+        if (fs) fs[rg.at32(i, j, k)] = (PT)rn;
         contrib = rn * rn;
     }
     double v = wave_sum(contrib);
@@ -950,11 +959,11 @@ __global__ void k_guess_check_scalar(double *ds)
 // r = b - A x with folded ghosts; partial sums of r.r and b.b
 // ZERO: x is zero, so A x is +0 on every cell and r = b - h2i * 0 = b bit
 // for bit; x and the coefficients are not read, and fs (when given) takes
-// the split FP32 copy of r, the V-cycle's first right-hand side
-template <bool ZERO>
+// the split copy of r in the V-cycle's type FT, its first right-hand side
+template <bool ZERO, typename FT>
 __global__ void __launch_bounds__(256) k_init_resid(Grid g, double *r, const double *x, const double *bvec,
                                                     const double *a, const double *b, const double *c, double h2i,
-                                                    double *partial, float *fs, RbGeom rg)
+                                                    double *partial, FT *fs, RbGeom rg)
 {
     __shared__ double lds4[4];
     __shared__ double lds4b[4];
@@ -969,7 +978,7 @@ __global__ void __launch_bounds__(256) k_init_resid(Grid g, double *r, const dou
     if (ZERO && i <= g.nx - 1 && j <= g.ny - 1) {
         const double bb = bvec[g.at(i, j, k)];
         r[g.at(i, j, k)] = bb;
-        if (fs) fs[rg.at32(i, j, k)] = (float)bb;
+        if (fs) fs[rg.at32(i, j, k)] = (FT)bb;
         c1 = bb * bb;
         c2 = bb * bb;
     } else if (i <= g.nx - 1 && j <= g.ny - 1) {
@@ -1082,24 +1091,24 @@ static int pcg_iteration(Ctx &c)
     const T *a_, *b_, *c_;
     level_ptrs<T>(F, u, f, rr_, a_, b_, c_, w_);
     const double ncell = (double)(g.nx - 2) * (g.ny - 2) * (g.nz - 2);
-    const bool split = sizeof(T) == 4 && rb_enabled(c);
+    const bool split = rb_enabled(c);
     // search direction storage: FP32 with the FP32 V-cycle (mode 2), FP64
     // in the plain FP64 solver (mode 1)
     using PT = T;
     if (split) {
         // z = M r through the red-black split V-cycle; its right-hand side
-        // (the FP32 split copy of r) was written by the previous update or
-        // by the solve's set-up
+        // (the split copy of r) was written by the previous update or by the
+        // solve's set-up
         if (rb_vcycle(c, nullptr)) return -1;
-        const ZSplit z = rb_top_z(c);
-        hipLaunchKernelGGL(k_dot_rz_zq<ZSplit>, dim3(RED_BLOCKS), dim3(256), 0, s, g, r, z, q, partial);
+        const ZSplit<T> z = rb_top_z<T>(c);
+        hipLaunchKernelGGL(k_dot_rz_zq<ZSplit<T>>, dim3(RED_BLOCKS), dim3(256), 0, s, g, r, z, q, partial);
         DG_CHECK_LAUNCH();
         hipLaunchKernelGGL(k_final2, dim3(1), dim3(256), 0, s, partial, RED_BLOCKS, ds + S_RZ);
         DG_CHECK_LAUNCH();
         prof_begin(PK_PCG_DIR, s);
-        hipLaunchKernelGGL((k_pcg_dir<ZSplit, PT>), dgr, ib, 0, s, g, z, (PT *)c.pcg_p, nslot, q, F.a, F.b, F.c, h2i,
+        hipLaunchKernelGGL((k_pcg_dir<ZSplit<T>, PT>), dgr, ib, 0, s, g, z, (PT *)c.pcg_p, nslot, q, F.a, F.b, F.c, h2i,
                            ds, partial);
-        prof_end(PK_PCG_DIR, s, (4.0 + 2.0 * sizeof(PT) + 4.0 * 8.0) * ncell);
+        prof_end(PK_PCG_DIR, s, ((double)sizeof(T) + 2.0 * sizeof(PT) + 4.0 * 8.0) * ncell);
         DG_CHECK_LAUNCH();
     } else {
         if (sizeof(T) == 4)
@@ -1122,8 +1131,9 @@ static int pcg_iteration(Ctx &c)
     }
     if (sum_partials(s, partial, nblk_dir, 1, partial + 2 * (size_t)nblk, ds + S_PQ)) return -1;
     prof_begin(PK_PCG_UPDATE, s);
-    hipLaunchKernelGGL(k_pcg_update<PT>, ig, ib, 0, s, g, r, q, ds, partial, split ? F.rb_f : nullptr, rb_geom(g));
-    prof_end(PK_PCG_UPDATE, s, (3.0 * 8.0 + (split ? 4.0 : 0.0)) * ncell);
+    hipLaunchKernelGGL(k_pcg_update<PT>, ig, ib, 0, s, g, r, q, ds, partial, split ? rb_f_of<T>(F) : nullptr,
+                       rb_geom(g));
+    prof_end(PK_PCG_UPDATE, s, (3.0 * 8.0 + (split ? (double)sizeof(T) : 0.0)) * ncell);
     DG_CHECK_LAUNCH();
     if (sum_partials(s, partial, nblk, 1, partial + 2 * (size_t)nblk, ds + S_RR)) return -1;
     hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(1), 0, s, ds, c.dpoll);
@@ -1140,10 +1150,10 @@ static int pcg_iteration(Ctx &c)
 template <typename T>
 static int pcg_graphs(Ctx &c)
 {
-    const int rb = sizeof(T) == 4 && rb_enabled(c);
+    const int rb = rb_enabled(c);
     // everything that changes the captured kernel sequence or its arguments
     const int key[12] = {c.mg_mode, c.mg_nu1, c.mg_nu2, c.fused_smoother, c.ng, (int)c.g.nx,
-                         rb, rb ? c.lev[c.ng - 1].rb_coef16 : -1, rb ? c.rb_tail : -1, c.mg_nuc,
+                         rb, rb ? c.lev[c.ng - 1].rb_ct : -1, rb ? c.rb_tail : -1, c.mg_nuc,
                          (int)llround(c.mg_omega * 1e9), (int)c.g.nz};
     bool ok = c.pcg_graph[0] != nullptr;
     for (int q = 0; q < 12 && ok; q++) ok = c.pcg_graph_key[q] == key[q];
@@ -1192,13 +1202,13 @@ static int pcg_solve(Ctx &c)
     }
     // the split V-cycle's first right-hand side: written with r = b from a
     // zero guess, loaded below otherwise
-    const bool split32 = sizeof(T) == 4 && rb_enabled(c);
+    const bool split = rb_enabled(c);
     if (c.mg_guess == 1)
-        hipLaunchKernelGGL(k_init_resid<false>, ig, ib, 0, s, g, r, x, bvec, F.a, F.b, F.c, h2i, partial, nullptr,
-                           rb_geom(g));
+        hipLaunchKernelGGL((k_init_resid<false, T>), ig, ib, 0, s, g, r, x, bvec, F.a, F.b, F.c, h2i, partial,
+                           (T *)nullptr, rb_geom(g));
     else
-        hipLaunchKernelGGL(k_init_resid<true>, ig, ib, 0, s, g, r, x, bvec, F.a, F.b, F.c, h2i, partial,
-                           split32 ? F.rb_f : nullptr, rb_geom(g));
+        hipLaunchKernelGGL((k_init_resid<true, T>), ig, ib, 0, s, g, r, x, bvec, F.a, F.b, F.c, h2i, partial,
+                           split ? rb_f_of<T>(F) : nullptr, rb_geom(g));
     DG_CHECK_LAUNCH();
     if (sum_partials(s, partial, nblk, 2, partial + 2 * (size_t)nblk, ds + S_RR)) return -1;  // r.r and b.b
     if (c.mg_guess == 1) {
@@ -1212,7 +1222,7 @@ static int pcg_solve(Ctx &c)
     const bool hooks = guess_wanted(c);
     bool r_changed = false;
     if (hooks && guess_solve_begin(c, x, r, h2i, c.mg_guess == 2, ds + S_RR, &r_changed)) return -1;
-    if (split32 && (c.mg_guess == 1 || r_changed) && rb_load_rhs(c, r)) return -1;
+    if (split && (c.mg_guess == 1 || r_changed) && rb_load_rhs(c, r)) return -1;
     // coarse factorisation for this solve, FIRST flag
     {
         Level &L0 = c.lev[0];
@@ -1331,7 +1341,7 @@ int op_mg_solve_tol(Ctx &c)
         Level &C = c.lev[gi - 1], &F = c.lev[gi];
         if (mg_acfromac(C.g, s, C.a, C.b, C.c, C.ei, F.a, F.b, F.c)) return -1;
     }
-    if (c.mg_mode == 2 && rb_enabled(c)) {
+    if (rb_enabled(c)) {
         if (rb_prepare(c)) return -1;
     } else if (c.mg_mode == 2) {
         for (int l = 0; l < c.ng; l++) {

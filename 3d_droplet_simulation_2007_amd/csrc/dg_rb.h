@@ -1,4 +1,5 @@
-// dg_rb.h -- red-black split storage of the FP32 V-cycle (tolerance mode 2).
+// dg_rb.h -- red-black split storage of the tolerance-mode V-cycle (FP32
+// iterate in mode 2, FP64 iterate in mode 1).
 //
 // On every level above the coarsest, the unknowns u, the right-hand side f
 // and the face coefficients a, b, c of the interior cells are stored per
@@ -60,24 +61,47 @@ struct ZNatural {
     Grid g;
     __device__ inline double operator()(int i, int j, int k, int o) const { return (double)z[o]; }
 };
+template <typename T>
 struct ZSplit {
-    const float *z;
+    const T *z;
     RbGeom r;
     __device__ inline double operator()(int i, int j, int k, int o) const { return (double)z[r.at32(i, j, k)]; }
 };
 
-// whether the split path serves this context's mode-2 solves
+// the split iterate / right-hand side arrays of a level for iterate type T
+template <typename T>
+inline T *rb_u_of(Level &L);
+template <>
+inline float *rb_u_of<float>(Level &L) { return L.rb_u; }
+template <>
+inline double *rb_u_of<double>(Level &L) { return L.rb_u64; }
+template <typename T>
+inline T *rb_f_of(Level &L);
+template <>
+inline float *rb_f_of<float>(Level &L) { return L.rb_f; }
+template <>
+inline double *rb_f_of<double>(Level &L) { return L.rb_f64; }
+
+// whether the split path serves this context's tolerance-mode solves
+// (mode 2: FP32 iterate; mode 1: FP64 iterate with FP64 coefficients)
 bool rb_enabled(const Ctx &c);
+// the coefficient type (RB_CT_*) the split levels use for this context
+int rb_coef_type(const Ctx &c);
 // allocates the split arrays and builds the split coefficients of every
-// level above the coarsest for this solve (after the coarse operators and
-// their FP32 copies exist)
+// level above the coarsest for this solve (after the coarse operators exist)
 int rb_prepare(Ctx &c);
-// finest-level split right-hand side (FP32) from r (FP64, natural layout)
+// finest-level split right-hand side (the iterate's type) from r (FP64,
+// natural layout)
 int rb_load_rhs(Ctx &c, const double *r);
 // z = V-cycle(r) from a zero guess; r: FP64 finest-level residual (natural
 // layout), or null when the split right-hand side already holds it; the
 // result stays in the finest level's split u (rb_top_z)
 int rb_vcycle(Ctx &c, const double *r);
-ZSplit rb_top_z(const Ctx &c);
+template <typename T>
+ZSplit<T> rb_top_z(Ctx &c)
+{
+    Level &F = c.lev[c.ng - 1];
+    return ZSplit<T>{rb_u_of<T>(F), rb_geom(F.g)};
+}
 
 }  // namespace dg

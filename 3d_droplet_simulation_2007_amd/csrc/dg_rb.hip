@@ -1,11 +1,16 @@
-// dg_rb.hip -- the FP32 V-cycle of tolerance mode 2 on red-black split
-// storage (layout in dg_rb.h).  Same algorithm as the natural-layout
-// V-cycle in dg_mg.hip (symmetric red-black Gauss-Seidel V(nu1,nu2) with
-// the operator's own diagonal, 8-child average restriction, injection
-// prolongation, acfromac coarse operators, FP64 Cholesky on the coarsest
-// level); the coefficients may be rounded to FP16.
+// dg_rb.hip -- the tolerance-mode V-cycle on red-black split storage
+// (layout in dg_rb.h): FP32 iterate in mode 2 (coefficients FP16 or FP32),
+// FP64 iterate and coefficients in mode 1.  Same algorithm as the
+// natural-layout V-cycle in dg_mg.hip (symmetric red-black Gauss-Seidel
+// V(nu1,nu2) with the operator's own diagonal, 8-child average restriction,
+// injection prolongation, acfromac coarse operators, FP64 Cholesky on the
+// coarsest level).  Every kernel is templated on the coefficient type CT
+// and the iterate type T; the arithmetic is T's, in the same order for
+// either, so the FP64 form gives the natural-layout FP64 V-cycle's bits.
 
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "dg_coarse.h"
 #include "dg_prof.h"
@@ -15,18 +20,31 @@
 namespace dg {
 
 int coarse_solve_f32(Ctx &c);     // dg_mg.hip
+int coarse_solve_f64(Ctx &c);     // dg_mg.hip
 const double *coarse_factor(const Ctx &c);  // dg_mg.hip: Cholesky factor of the coarsest level
 
 namespace {
 
+// the type a stored coefficient is computed in: FP32 for FP16 and FP32
+// storage, FP64 for FP64
+template <typename CT>
+struct CoefVal {
+    using type = float;
+};
+template <>
+struct CoefVal<double> {
+    using type = double;
+};
+
 // three coefficient planes (a, b, c), each 2*half entries in split order
 template <typename CT>
 struct Coef {
+    using V = typename CoefVal<CT>::type;
     const CT *p;
     size_t n2;  // 2 * half
-    __device__ inline float a(size_t e) const { return (float)p[e]; }
-    __device__ inline float b(size_t e) const { return (float)p[n2 + e]; }
-    __device__ inline float c(size_t e) const { return (float)p[2 * n2 + e]; }
+    __device__ inline V a(size_t e) const { return (V)p[e]; }
+    __device__ inline V b(size_t e) const { return (V)p[n2 + e]; }
+    __device__ inline V c(size_t e) const { return (V)p[2 * n2 + e]; }
 };
 
 // thread (i, j, k) of the interior, one z plane per tile
@@ -47,15 +65,16 @@ __global__ void __launch_bounds__(256) k_rb_coef(Grid g, RbGeom r, const double 
     out[2 * n2 + e] = (CT)c[o];
 }
 
-// finest-level right-hand side: f (split FP32) = r (natural FP64)
-__global__ void __launch_bounds__(256) k_rb_in(Grid g, RbGeom r, const double *src, float *dst)
+// finest-level right-hand side: f (split, the iterate's type) = r (natural FP64)
+template <typename T>
+__global__ void __launch_bounds__(256) k_rb_in(Grid g, RbGeom r, const double *src, T *dst)
 {
     const uint3 tile = xcd_tile();
     const int i = (int)(tile.x * blockDim.x + threadIdx.x) + 2;
     const int j = (int)(tile.y * blockDim.y + threadIdx.y) + 2;
     const int k = (int)tile.z + 2;
     if (i > g.nx - 1 || j > g.ny - 1) return;
-    dst[r.at(i, j, k)] = (float)src[g.at(i, j, k)];
+    dst[r.at(i, j, k)] = (T)src[g.at(i, j, k)];
 }
 
 // One Gauss-Seidel colour pass: thread (m, j, k) updates the colour's cell
@@ -71,10 +90,9 @@ __global__ void __launch_bounds__(256) k_rb_in(Grid g, RbGeom r, const double *s
 // k+dk), uold the cell's own value before the pass.  Every split colour
 // update (the colour passes, the tail, the fused sweeps) goes through this
 // arithmetic, so their results agree to the bit.
-template <typename CT, bool ZERO, class NB>
-__device__ inline float rb_cell_value(const RbGeom &r, const NB &nb, const float *f, const float *uold_arr,
-                                      const Coef<CT> &cf, float h2, int colour, float om, int zown, int m, int j,
-                                      int k)
+template <typename CT, typename T, bool ZERO, class NB>
+__device__ inline T rb_cell_value(const RbGeom &r, const NB &nb, const T *f, const T *uold_arr, const Coef<CT> &cf,
+                                  T h2, int colour, T om, int zown, int m, int j, int k)
 {
     const int p = (colour + j + k) & 1;
     const int i = 2 + p + 2 * m;
@@ -82,12 +100,12 @@ __device__ inline float rb_cell_value(const RbGeom &r, const NB &nb, const float
     const size_t own = (size_t)colour * r.half + row + m;
     const size_t oth = (size_t)(1 - colour) * r.half;
     const size_t rs = (size_t)r.mh, ps = (size_t)r.mh * r.my;  // row and plane strides of one colour
-    const float ae = cf.a(own), bn = cf.b(own), ct = cf.c(own);
+    const T ae = cf.a(own), bn = cf.b(own), ct = cf.c(own);
     const bool hw = i > 2, hn = j < r.ny - 1, hs = j > 2, ht = k < r.nz - 1, hb = k > 2;
-    const float aw = hw ? cf.a(oth + row + m + p - 1) : 0.f;
-    const float bs = hs ? cf.b(oth + row - rs + m) : 0.f;
-    const float cb = hb ? cf.c(oth + row - ps + m) : 0.f;
-    float s = 0.f, d = ae;
+    const T aw = hw ? (T)cf.a(oth + row + m + p - 1) : (T)0;
+    const T bs = hs ? (T)cf.b(oth + row - rs + m) : (T)0;
+    const T cb = hb ? (T)cf.c(oth + row - ps + m) : (T)0;
+    T s = (T)0, d = ae;
     if (!ZERO) {
         if (i < r.nx - 1) s += ae * nb(m + p, 0, 0);
         if (hw) s += aw * nb(m + p - 1, 0, 0);
@@ -101,33 +119,33 @@ __device__ inline float rb_cell_value(const RbGeom &r, const NB &nb, const float
     if (hs) d += bs;
     if (ht) d += ct;
     if (hb) d += cb;
-    const float gsv = (s - h2 * f[own]) / d;
-    const float uold = (ZERO || zown || om == 1.f) ? 0.f : uold_arr[own];  // Gauss-Seidel needs no old value
-    return om * gsv + (1.f - om) * uold;
+    const T gsv = (s - h2 * f[own]) / d;
+    const T uold = (ZERO || zown || om == (T)1) ? (T)0 : uold_arr[own];  // Gauss-Seidel needs no old value
+    return om * gsv + ((T)1 - om) * uold;
 }
 
 // in-place colour update of one cell, neighbours from the iterate itself
-template <typename CT, bool ZERO>
-__device__ inline void rb_cell(const RbGeom &r, float *u, const float *f, const Coef<CT> &cf, float h2, int colour,
-                               float om, int zown, int m, int j, int k)
+template <typename CT, typename T, bool ZERO>
+__device__ inline void rb_cell(const RbGeom &r, T *u, const T *f, const Coef<CT> &cf, T h2, int colour, T om,
+                               int zown, int m, int j, int k)
 {
     const size_t row = ((size_t)(k - 2) * r.my + (j - 2)) * r.mh;
-    const float *o = u + (size_t)(1 - colour) * r.half + row;
+    const T *o = u + (size_t)(1 - colour) * r.half + row;
     const size_t rs = (size_t)r.mh, ps = (size_t)r.mh * r.my;
     auto nb = [&](int mm, int dj, int dk) { return o[(ptrdiff_t)dj * (ptrdiff_t)rs + (ptrdiff_t)dk * (ptrdiff_t)ps + mm]; };
-    u[(size_t)colour * r.half + row + m] = rb_cell_value<CT, ZERO>(r, nb, f, u, cf, h2, colour, om, zown, m, j, k);
+    u[(size_t)colour * r.half + row + m] = rb_cell_value<CT, T, ZERO>(r, nb, f, u, cf, h2, colour, om, zown, m, j, k);
 }
 
-template <typename CT, bool ZERO, bool TOP>
-__global__ void __launch_bounds__(256) k_rb_colour(RbGeom r, float *__restrict__ u, const float *__restrict__ f,
-                                                   Coef<CT> cf, float h2, int colour, float om, int zown)
+template <typename CT, typename T, bool ZERO, bool TOP>
+__global__ void __launch_bounds__(256) k_rb_colour(RbGeom r, T *__restrict__ u, const T *__restrict__ f, Coef<CT> cf,
+                                                   T h2, int colour, T om, int zown)
 {
     const uint3 tile = xcd_tile();
     const int m = (int)(tile.x * blockDim.x + threadIdx.x);
     const int j = (int)(tile.y * blockDim.y + threadIdx.y) + 2;
     const int k = (int)tile.z + 2;
     if (m >= r.mh || j > r.ny - 1) return;
-    rb_cell<CT, ZERO>(r, u, f, cf, h2, colour, om, zown, m, j, k);
+    rb_cell<CT, T, ZERO>(r, u, f, cf, h2, colour, om, zown, m, j, k);
 }
 
 // four consecutive entries (16-byte aligned for FP32, 8-byte for FP16)
@@ -148,46 +166,64 @@ __device__ inline void ld4(const _Float16 *p, float v[4])
     v[2] = (float)q[2];
     v[3] = (float)q[3];
 }
+// (32-byte aligned for FP64)
+__device__ inline void ld4(const double *p, double v[4])
+{
+    const double2 q0 = *reinterpret_cast<const double2 *>(p), q1 = *reinterpret_cast<const double2 *>(p + 2);
+    v[0] = q0.x;
+    v[1] = q0.y;
+    v[2] = q1.x;
+    v[3] = q1.y;
+}
+
+// four consecutive entries stored with vector stores
+__device__ inline void st4(float *p, const float v[4]) { *reinterpret_cast<float4 *>(p) = make_float4(v[0], v[1], v[2], v[3]); }
+__device__ inline void st4(double *p, const double v[4])
+{
+    *reinterpret_cast<double2 *>(p) = make_double2(v[0], v[1]);
+    *reinterpret_cast<double2 *>(p + 2) = make_double2(v[2], v[3]);
+}
 
 // Other-colour values around a group of four entries m0..m0+3 of row
 // (j, k): row4(dj, dk) the four entries of row j+dj, plane k+dk; one(dm)
 // the entry m0+dm of the group's own row (dm = -1 or 4).  NbGlobal reads
 // the iterate in HBM; NbRing reads a plane ring in LDS (the fused sweep).
+template <typename T>
 struct NbGlobal {
-    const float *o;  // other colour, row (j, k), entry m0
-    size_t rs, ps;   // row and plane strides of one colour
-    __device__ inline void row4(int dj, int dk, float v[4]) const
+    const T *o;     // other colour, row (j, k), entry m0
+    size_t rs, ps;  // row and plane strides of one colour
+    __device__ inline void row4(int dj, int dk, T v[4]) const
     {
         ld4(o + (ptrdiff_t)dj * (ptrdiff_t)rs + (ptrdiff_t)dk * (ptrdiff_t)ps, v);
     }
-    __device__ inline float one(int dm) const { return o[dm]; }
+    __device__ inline T one(int dm) const { return o[dm]; }
 };
 
 // The arithmetic of one colour update of four consecutive entries: the
 // same operations in the same order as rb_cell, per cell.  own / oth: the
 // group's offsets in its own and in the other colour's half (coefficients
 // of the group's faces and of the other colour's west/south/bottom faces).
-template <typename CT, bool ZERO, class NB>
-__device__ inline void rb_group4(const RbGeom &r, const NB &nb, const float *f, const float *uold_own,
-                                 const Coef<CT> &cf, size_t own, size_t oth, float h2, int colour, float om, int zown,
-                                 int m0, int j, int k, float out[4], const float *eo = nullptr)
+template <typename CT, typename T, bool ZERO, class NB>
+__device__ inline void rb_group4(const RbGeom &r, const NB &nb, const T *f, const T *uold_own, const Coef<CT> &cf,
+                                 size_t own, size_t oth, T h2, int colour, T om, int zown, int m0, int j, int k,
+                                 T out[4], const T *eo = nullptr)
 {
     const int p = (colour + j + k) & 1;
     const size_t rs = (size_t)r.mh, ps = (size_t)r.mh * r.my;
     const bool hn = j < r.ny - 1, hs = j > 2, ht = k < r.nz - 1, hb = k > 2;
-    float ae[4], bn[4], ct[4], aw[4] = {0.f, 0.f, 0.f, 0.f}, bs[4] = {0.f, 0.f, 0.f, 0.f},
-                                 cb[4] = {0.f, 0.f, 0.f, 0.f};
+    T ae[4], bn[4], ct[4], aw[4] = {(T)0, (T)0, (T)0, (T)0}, bs[4] = {(T)0, (T)0, (T)0, (T)0},
+                           cb[4] = {(T)0, (T)0, (T)0, (T)0};
     ld4(cf.p + own, ae);
     ld4(cf.p + cf.n2 + own, bn);
     ld4(cf.p + 2 * cf.n2 + own, ct);
     // west faces: the other colour's a at m+p-1
     {
-        float t[4];
+        T t[4];
         ld4(cf.p + oth, t);
         if (p) {
             for (int q = 0; q < 4; q++) aw[q] = t[q];
         } else {
-            aw[0] = m0 > 0 ? cf.a(oth - 1) : 0.f;
+            aw[0] = m0 > 0 ? (T)cf.a(oth - 1) : (T)0;
             aw[1] = t[0];
             aw[2] = t[1];
             aw[3] = t[2];
@@ -195,19 +231,19 @@ __device__ inline void rb_group4(const RbGeom &r, const NB &nb, const float *f, 
     }
     if (hs) ld4(cf.p + cf.n2 + oth - rs, bs);
     if (hb) ld4(cf.p + 2 * cf.n2 + oth - ps, cb);
-    float ue[4] = {0.f, 0.f, 0.f, 0.f}, uw[4] = {0.f, 0.f, 0.f, 0.f}, un[4], us[4], ut[4], ub[4];
+    T ue[4] = {(T)0, (T)0, (T)0, (T)0}, uw[4] = {(T)0, (T)0, (T)0, (T)0}, un[4], us[4], ut[4], ub[4];
     if (!ZERO) {
-        float t[4];
+        T t[4];
         nb.row4(0, 0, t);
         if (p) {  // west m .. , east m+1 ..
             for (int q = 0; q < 4; q++) uw[q] = t[q];
             ue[0] = t[1];
             ue[1] = t[2];
             ue[2] = t[3];
-            ue[3] = m0 + 4 < r.mh ? nb.one(4) : 0.f;
+            ue[3] = m0 + 4 < r.mh ? nb.one(4) : (T)0;
         } else {  // west m-1 .., east m ..
             for (int q = 0; q < 4; q++) ue[q] = t[q];
-            uw[0] = m0 > 0 ? nb.one(-1) : 0.f;
+            uw[0] = m0 > 0 ? nb.one(-1) : (T)0;
             uw[1] = t[0];
             uw[2] = t[1];
             uw[3] = t[2];
@@ -217,9 +253,9 @@ __device__ inline void rb_group4(const RbGeom &r, const NB &nb, const float *f, 
         if (ht) nb.row4(0, 1, ut);
         if (hb) nb.row4(0, -1, ub);
     }
-    float fv[4], uo[4] = {0.f, 0.f, 0.f, 0.f};
+    T fv[4], uo[4] = {(T)0, (T)0, (T)0, (T)0};
     ld4(f + own, fv);
-    if (!ZERO && !zown && om != 1.f) {  // Gauss-Seidel needs no old value
+    if (!ZERO && !zown && om != (T)1) {  // Gauss-Seidel needs no old value
         ld4(uold_own + own, uo);
         if (eo)  // the coarse correction not yet added to the own colour
             for (int q = 0; q < 4; q++) uo[q] = uo[q] + eo[q];
@@ -227,7 +263,7 @@ __device__ inline void rb_group4(const RbGeom &r, const NB &nb, const float *f, 
     for (int q = 0; q < 4; q++) {
         const int i = 2 + p + 2 * (m0 + q);
         const bool hw = i > 2, he = i < r.nx - 1;
-        float s = 0.f, d = ae[q];
+        T s = (T)0, d = ae[q];
         if (!ZERO) {
             if (he) s += ae[q] * ue[q];
             if (hw) s += aw[q] * uw[q];
@@ -241,17 +277,17 @@ __device__ inline void rb_group4(const RbGeom &r, const NB &nb, const float *f, 
         if (hs) d += bs[q];
         if (ht) d += ct[q];
         if (hb) d += cb[q];
-        const float gsv = (s - h2 * fv[q]) / d;
-        out[q] = om * gsv + (1.f - om) * uo[q];
+        const T gsv = (s - h2 * fv[q]) / d;
+        out[q] = om * gsv + ((T)1 - om) * uo[q];
     }
 }
 
 // The colour pass with four consecutive cells of a row per thread (rows of
 // a multiple of 4 cells): one vector load per neighbour row instead of
 // four scalar loads, same arithmetic in the same order per cell.
-template <typename CT, bool ZERO, bool TOP>
-__global__ void __launch_bounds__(256) k_rb_colour4(RbGeom r, float *__restrict__ u, const float *__restrict__ f,
-                                                    Coef<CT> cf, float h2, int colour, float om, int zown)
+template <typename CT, typename T, bool ZERO, bool TOP>
+__global__ void __launch_bounds__(256) k_rb_colour4(RbGeom r, T *__restrict__ u, const T *__restrict__ f, Coef<CT> cf,
+                                                    T h2, int colour, T om, int zown)
 {
     const uint3 tile = xcd_tile();
     const int m0 = 4 * (int)(tile.x * blockDim.x + threadIdx.x);
@@ -261,10 +297,10 @@ __global__ void __launch_bounds__(256) k_rb_colour4(RbGeom r, float *__restrict_
     const size_t row = ((size_t)(k - 2) * r.my + (j - 2)) * r.mh;
     const size_t own = (size_t)colour * r.half + row + m0;
     const size_t oth = (size_t)(1 - colour) * r.half + row + m0;  // other colour, same row, m0
-    const NbGlobal nb{u + oth, (size_t)r.mh, (size_t)r.mh * r.my};
-    float out[4];
-    rb_group4<CT, ZERO>(r, nb, f, u, cf, own, oth, h2, colour, om, zown, m0, j, k, out);
-    *reinterpret_cast<float4 *>(u + own) = make_float4(out[0], out[1], out[2], out[3]);
+    const NbGlobal<T> nb{u + oth, (size_t)r.mh, (size_t)r.mh * r.my};
+    T out[4];
+    rb_group4<CT, T, ZERO>(r, nb, f, u, cf, own, oth, h2, colour, om, zown, m0, j, k, out);
+    st4(u + own, out);
 }
 
 // One red-black sweep of the finest level in one pass (colour ca, then the
@@ -319,9 +355,9 @@ __global__ void __launch_bounds__(SWEEP_MAX_THREADS) k_rb_sweep4(RbGeom r, const
             const size_t row = ((size_t)(pl - 2) * r.my + (ja - 2)) * r.mh;
             const size_t own = (size_t)ca * r.half + row + m0;
             const size_t oth = (size_t)cb * r.half + row + m0;
-            const NbGlobal nb{X + oth, rs, ps};
+            const NbGlobal<float> nb{X + oth, rs, ps};
             float out[4];
-            rb_group4<CT, ZERO>(r, nb, f, X, cf, own, oth, h2, ca, om, ZERO ? 1 : 0, m0, ja, pl, out);
+            rb_group4<CT, float, ZERO>(r, nb, f, X, cf, own, oth, h2, ca, om, ZERO ? 1 : 0, m0, ja, pl, out);
             float *dst = ring + ((size_t)(pl % 3) * rows + rr) * r.mh + m0;
             *reinterpret_cast<float4 *>(dst) = make_float4(out[0], out[1], out[2], out[3]);
             if (rr >= 1 && rr <= TY && pl >= k0 && pl < k1)
@@ -336,7 +372,7 @@ __global__ void __launch_bounds__(SWEEP_MAX_THREADS) k_rb_sweep4(RbGeom r, const
             const size_t oth = (size_t)ca * r.half + row + m0;
             const NbRing nb{ring, r.mh, rows, q % 3, (q + 1) % 3, (q + 2) % 3, rr, m0};
             float out[4];
-            rb_group4<CT, false>(r, nb, f, X, cf, own, oth, h2, cb, om, ZERO ? 1 : 0, m0, j, q, out);
+            rb_group4<CT, float, false>(r, nb, f, X, cf, own, oth, h2, cb, om, ZERO ? 1 : 0, m0, j, q, out);
             *reinterpret_cast<float4 *>(Y + own) = make_float4(out[0], out[1], out[2], out[3]);
         }
         __syncthreads();
@@ -374,7 +410,7 @@ __global__ void __launch_bounds__(256) k_rb_sweep_tile(RbGeom r, const float *__
         const size_t row = ((size_t)(k - 2) * r.my + (j - 2)) * r.mh;
         const float *o = X + (size_t)cb * r.half + row;
         auto nb = [&](int mm, int dj, int dk) { return o[dj * rs + dk * ps + mm]; };
-        const float v = rb_cell_value<CT, ZERO>(r, nb, f, X, cf, h2, ca, om, ZERO ? 1 : 0, m, j, k);
+        const float v = rb_cell_value<CT, float, ZERO>(r, nb, f, X, cf, h2, ca, om, ZERO ? 1 : 0, m, j, k);
         A[e] = v;
         if (hx >= 1 && hx <= FT_X && hy >= 1 && hy <= FT_Y && hz >= 1 && hz <= FT_Z)
             Y[(size_t)ca * r.half + row + m] = v;
@@ -384,7 +420,7 @@ __global__ void __launch_bounds__(256) k_rb_sweep_tile(RbGeom r, const float *__
     const int m = m0 + tx, j = j0 + ty, k = k0 + tz;
     if (m < r.mh && j <= r.ny - 1 && k <= r.nz - 1) {
         auto nb = [&](int mm, int dj, int dk) { return A[((tz + 1 + dk) * HY + (ty + 1 + dj)) * HX + (mm - m0 + 1)]; };
-        const float v = rb_cell_value<CT, false>(r, nb, f, X, cf, h2, cb, om, ZERO ? 1 : 0, m, j, k);
+        const float v = rb_cell_value<CT, float, false>(r, nb, f, X, cf, h2, cb, om, ZERO ? 1 : 0, m, j, k);
         Y[(size_t)cb * r.half + ((size_t)(k - 2) * r.my + (j - 2)) * r.mh + m] = v;
     }
 }
@@ -396,10 +432,9 @@ __global__ void __launch_bounds__(256) k_rb_sweep_tile(RbGeom r, const float *__
 // and entry I + (dx>>1) of its row: every offset is the corner's row offset
 // plus constants.  Each child's residual f - A u folds the boundary like
 // the natural-layout resid (k_resid_fold), in the same operation order.
-template <typename CT>
-__device__ inline void rb_restrict_cell(const RbGeom &rf, const float *u, const float *f, const Coef<CT> &cf,
-                                        float h2i, const RbGeom &rc, float *fcs, const Grid &gc, float *fcn, int I,
-                                        int J, int K)
+template <typename CT, typename T>
+__device__ inline void rb_restrict_cell(const RbGeom &rf, const T *u, const T *f, const Coef<CT> &cf, T h2i,
+                                        const RbGeom &rc, T *fcs, const Grid &gc, T *fcn, int I, int J, int K)
 {
     const int i0 = 2 * I + 2, j0 = 2 * J + 2, k0 = 2 * K + 2;
     const int rs = rf.mh, ps = rf.mh * rf.my;
@@ -408,7 +443,7 @@ __device__ inline void rb_restrict_cell(const RbGeom &rf, const float *u, const 
     auto at = [&](int dx, int dy, int dz) {
         return (unsigned)((dx + dy + dz) & 1) * half + base + (unsigned)(dy * rs + dz * ps + (dx >> 1));
     };
-    float t = 0.f;
+    T t = (T)0;
 #pragma unroll
     for (int dz = 0; dz < 2; dz++)
 #pragma unroll
@@ -417,21 +452,21 @@ __device__ inline void rb_restrict_cell(const RbGeom &rf, const float *u, const 
             for (int dx = 0; dx < 2; dx++) {
                 const int i = i0 + dx, j = j0 + dy, k = k0 + dz;
                 const unsigned e = at(dx, dy, dz);
-                const float u0 = u[e];
-                const float ue = i < rf.nx - 1 ? u[at(dx + 1, dy, dz)] : 0.f;
-                const float uw = i > 2 ? u[at(dx - 1, dy, dz)] : u0;
-                const float un = j < rf.ny - 1 ? u[at(dx, dy + 1, dz)] : u0;
-                const float us = j > 2 ? u[at(dx, dy - 1, dz)] : u0;
-                const float ut = k < rf.nz - 1 ? u[at(dx, dy, dz + 1)] : u0;
-                const float ub = k > 2 ? u[at(dx, dy, dz - 1)] : u0;
-                const float aw = i > 2 ? cf.a(at(dx - 1, dy, dz)) : 0.f;
-                const float bs = j > 2 ? cf.b(at(dx, dy - 1, dz)) : 0.f;
-                const float cb = k > 2 ? cf.c(at(dx, dy, dz - 1)) : 0.f;
-                const float s = cf.a(e) * (ue - u0) + aw * (uw - u0) + cf.b(e) * (un - u0) + bs * (us - u0) +
-                                cf.c(e) * (ut - u0) + cb * (ub - u0);
+                const T u0 = u[e];
+                const T ue = i < rf.nx - 1 ? u[at(dx + 1, dy, dz)] : (T)0;
+                const T uw = i > 2 ? u[at(dx - 1, dy, dz)] : u0;
+                const T un = j < rf.ny - 1 ? u[at(dx, dy + 1, dz)] : u0;
+                const T us = j > 2 ? u[at(dx, dy - 1, dz)] : u0;
+                const T ut = k < rf.nz - 1 ? u[at(dx, dy, dz + 1)] : u0;
+                const T ub = k > 2 ? u[at(dx, dy, dz - 1)] : u0;
+                const T aw = i > 2 ? (T)cf.a(at(dx - 1, dy, dz)) : (T)0;
+                const T bs = j > 2 ? (T)cf.b(at(dx, dy - 1, dz)) : (T)0;
+                const T cb = k > 2 ? (T)cf.c(at(dx, dy, dz - 1)) : (T)0;
+                const T s = (T)cf.a(e) * (ue - u0) + aw * (uw - u0) + (T)cf.b(e) * (un - u0) + bs * (us - u0) +
+                            (T)cf.c(e) * (ut - u0) + cb * (ub - u0);
                 t += f[e] - h2i * s;
             }
-    const float v = 0.125f * t;
+    const T v = (T)0.125 * t;
     if (fcs)
         fcs[rc.at(I + 2, J + 2, K + 2)] = v;
     else
@@ -439,45 +474,47 @@ __device__ inline void rb_restrict_cell(const RbGeom &rf, const float *u, const 
 }
 
 // thread (I, J, K) of the coarse interior, one z plane per tile
-template <typename CT>
-__global__ void __launch_bounds__(256) k_rb_resid_restrict(RbGeom rf, const float *u, const float *f, Coef<CT> cf,
-                                                           float h2i, RbGeom rc, float *fcs, Grid gc, float *fcn)
+template <typename CT, typename T>
+__global__ void __launch_bounds__(256) k_rb_resid_restrict(RbGeom rf, const T *u, const T *f, Coef<CT> cf, T h2i,
+                                                           RbGeom rc, T *fcs, Grid gc, T *fcn)
 {
     const uint3 tile = xcd_tile();
     const int I = (int)(tile.x * blockDim.x + threadIdx.x);
     const int J = (int)(tile.y * blockDim.y + threadIdx.y);
     const int K = (int)tile.z;
     if (I >= gc.nx - 2 || J >= gc.ny - 2) return;
-    rb_restrict_cell<CT>(rf, u, f, cf, h2i, rc, fcs, gc, fcn, I, J, K);
+    rb_restrict_cell<CT, T>(rf, u, f, cf, h2i, rc, fcs, gc, fcn, I, J, K);
 }
 
 // fine u += coarse u of the parent cell (injection), fine in split order:
 // entry m of row (j, k) in both colours, cells i = 2 + p + 2m with p = 0, 1,
 // both children of coarse (m+2, j', k')
+template <typename T>
 struct CoarseE {
-    RbGeom rc;          // coarse level, split order (ucs), or
-    const float *ucs;
-    Grid gc;            // natural order (ucn)
-    const float *ucn;
+    RbGeom rc;  // coarse level, split order (ucs), or
+    const T *ucs;
+    Grid gc;    // natural order (ucn)
+    const T *ucn;
     // the correction of fine entry m of row (j, k), either colour
-    __device__ inline float at(int m, int j, int k) const
+    __device__ inline T at(int m, int j, int k) const
     {
         const int ic = m + 2, jc = (j - 2) / 2 + 2, kc = (k - 2) / 2 + 2;
         return ucs ? ucs[rc.at(ic, jc, kc)] : ucn[gc.at(ic, jc, kc)];
     }
 };
 
-__device__ inline void rb_prolong_entry(const RbGeom &rf, float *u, const RbGeom &rc, const float *ucs,
-                                        const Grid &gc, const float *ucn, int m, int j, int k)
+template <typename T>
+__device__ inline void rb_prolong_entry(const RbGeom &rf, T *u, const RbGeom &rc, const T *ucs, const Grid &gc,
+                                        const T *ucn, int m, int j, int k)
 {
-    const float e = CoarseE{rc, ucs, gc, ucn}.at(m, j, k);
+    const T e = CoarseE<T>{rc, ucs, gc, ucn}.at(m, j, k);
     const size_t o = ((size_t)(k - 2) * rf.my + (j - 2)) * rf.mh + m;
     u[o] += e;
     u[rf.half + o] += e;
 }
 
-__global__ void __launch_bounds__(256) k_rb_prolong(RbGeom rf, float *u, RbGeom rc, const float *ucs, Grid gc,
-                                                    const float *ucn)
+template <typename T>
+__global__ void __launch_bounds__(256) k_rb_prolong(RbGeom rf, T *u, RbGeom rc, const T *ucs, Grid gc, const T *ucn)
 {
     const uint3 tile = xcd_tile();
     const int m = (int)(tile.x * blockDim.x + threadIdx.x);
@@ -491,12 +528,13 @@ __global__ void __launch_bounds__(256) k_rb_prolong(RbGeom rf, float *u, RbGeom 
 // read: the values k_rb_prolong would have left in HBM.  eo: the
 // corrections of entries m0 .. m0+3 of the group's own row, which a
 // neighbour row with the same coarse parent row shares.
+template <typename T>
 struct NbProlong {
-    NbGlobal g;
-    CoarseE e;
+    NbGlobal<T> g;
+    CoarseE<T> e;
     int m0, j, k;
-    const float *eo;
-    __device__ inline void row4(int dj, int dk, float v[4]) const
+    const T *eo;
+    __device__ inline void row4(int dj, int dk, T v[4]) const
     {
         g.row4(dj, dk, v);
         if (((j - 2 + dj) >> 1) == ((j - 2) >> 1) && ((k - 2 + dk) >> 1) == ((k - 2) >> 1)) {
@@ -505,7 +543,7 @@ struct NbProlong {
             for (int q = 0; q < 4; q++) v[q] = v[q] + e.at(m0 + q, j + dj, k + dk);
         }
     }
-    __device__ inline float one(int dm) const { return g.one(dm) + e.at(m0 + dm, j, k); }
+    __device__ inline T one(int dm) const { return g.one(dm) + e.at(m0 + dm, j, k); }
 };
 
 // The first post-smoothing sweep with the prolongation folded in: the pass
@@ -514,9 +552,9 @@ struct NbProlong {
 // colour's corrected values are written by its own pass.  The same values
 // and arithmetic as k_rb_prolong followed by the two colour passes, without
 // the prolongation's read and write of both colours.
-template <typename CT, bool TOP, bool NBE>
-__global__ void __launch_bounds__(256) k_rb_colour4_pro(RbGeom r, float *__restrict__ u, const float *__restrict__ f,
-                                                        Coef<CT> cf, float h2, int colour, float om, CoarseE ce)
+template <typename CT, typename T, bool TOP, bool NBE>
+__global__ void __launch_bounds__(256) k_rb_colour4_pro(RbGeom r, T *__restrict__ u, const T *__restrict__ f,
+                                                        Coef<CT> cf, T h2, int colour, T om, CoarseE<T> ce)
 {
     const uint3 tile = xcd_tile();
     const int m0 = 4 * (int)(tile.x * blockDim.x + threadIdx.x);
@@ -526,16 +564,16 @@ __global__ void __launch_bounds__(256) k_rb_colour4_pro(RbGeom r, float *__restr
     const size_t row = ((size_t)(k - 2) * r.my + (j - 2)) * r.mh;
     const size_t own = (size_t)colour * r.half + row + m0;
     const size_t oth = (size_t)(1 - colour) * r.half + row + m0;
-    const NbGlobal nbg{u + oth, (size_t)r.mh, (size_t)r.mh * r.my};
-    float eo[4], out[4];
+    const NbGlobal<T> nbg{u + oth, (size_t)r.mh, (size_t)r.mh * r.my};
+    T eo[4], out[4];
     for (int q = 0; q < 4; q++) eo[q] = ce.at(m0 + q, j, k);
     if (NBE) {
-        const NbProlong nb{nbg, ce, m0, j, k, eo};
-        rb_group4<CT, false>(r, nb, f, u, cf, own, oth, h2, colour, om, 0, m0, j, k, out, eo);
+        const NbProlong<T> nb{nbg, ce, m0, j, k, eo};
+        rb_group4<CT, T, false>(r, nb, f, u, cf, own, oth, h2, colour, om, 0, m0, j, k, out, eo);
     } else {
-        rb_group4<CT, false>(r, nbg, f, u, cf, own, oth, h2, colour, om, 0, m0, j, k, out, eo);
+        rb_group4<CT, T, false>(r, nbg, f, u, cf, own, oth, h2, colour, om, 0, m0, j, k, out, eo);
     }
-    *reinterpret_cast<float4 *>(u + own) = make_float4(out[0], out[1], out[2], out[3]);
+    st4(u + own, out);
 }
 
 // The V-cycle below a small level in one workgroup: every colour pass,
@@ -551,60 +589,60 @@ constexpr int TAIL_THREADS = 1024;
 constexpr int TAIL_MAX_LEVELS = 8;
 constexpr size_t TAIL_MAX_CELLS = 4096;  // largest level taken into the tail
 
-template <typename CT>
+template <typename CT, typename T>
 struct RbTail {
     int top;  // highest level in the tail (>= 1)
     RbGeom r[TAIL_MAX_LEVELS];
-    float *u[TAIL_MAX_LEVELS], *f[TAIL_MAX_LEVELS];
+    T *u[TAIL_MAX_LEVELS], *f[TAIL_MAX_LEVELS];
     Coef<CT> cf[TAIL_MAX_LEVELS];
-    float h2[TAIL_MAX_LEVELS], h2i[TAIL_MAX_LEVELS];
+    T h2[TAIL_MAX_LEVELS], h2i[TAIL_MAX_LEVELS];
     int nu1[TAIL_MAX_LEVELS], nu2[TAIL_MAX_LEVELS];
-    float om;
+    T om;
     Grid g0;  // coarsest level (natural layout)
-    float *f0, *u0;
+    T *f0, *u0;
     const double *M;
 };
 
-// LDS bytes of the staged tail: per level u, f (FP32) and 3 coefficient
+// LDS bytes of the staged tail: per level u, f (type T) and 3 coefficient
 // planes, then the coarsest level's f and u, 16-byte aligned pieces
-template <typename CT>
-__host__ __device__ inline size_t tail_lds_bytes(const RbTail<CT> &t)
+template <typename CT, typename T>
+__host__ __device__ inline size_t tail_lds_bytes(const RbTail<CT, T> &t)
 {
     size_t b = 0;
     for (int l = 1; l <= t.top; l++) {
         const size_t n2 = 2 * t.r[l].half;
-        b += 2 * ((n2 * sizeof(float) + 15) & ~(size_t)15) + ((3 * n2 * sizeof(CT) + 15) & ~(size_t)15);
+        b += 2 * ((n2 * sizeof(T) + 15) & ~(size_t)15) + ((3 * n2 * sizeof(CT) + 15) & ~(size_t)15);
     }
     const size_t n0 = (size_t)t.g0.nx * t.g0.ny * t.g0.nz;
-    return b + 2 * ((n0 * sizeof(float) + 15) & ~(size_t)15);
+    return b + 2 * ((n0 * sizeof(T) + 15) & ~(size_t)15);
 }
 
 // where level l's arrays live for one form of the tail
-template <typename CT, bool LDS>
+template <typename CT, typename T, bool LDS>
 struct TailMem {
-    const RbTail<CT> &t;
+    const RbTail<CT, T> &t;
     unsigned char *lds;
     __device__ inline size_t base(int l) const  // LDS offset of level l's u
     {
         size_t b = 0;
         for (int q = 1; q < l; q++) {
             const size_t n2 = 2 * t.r[q].half;
-            b += 2 * ((n2 * sizeof(float) + 15) & ~(size_t)15) + ((3 * n2 * sizeof(CT) + 15) & ~(size_t)15);
+            b += 2 * ((n2 * sizeof(T) + 15) & ~(size_t)15) + ((3 * n2 * sizeof(CT) + 15) & ~(size_t)15);
         }
         return b;
     }
-    __device__ inline size_t fl(int l) const { return (2 * t.r[l].half * sizeof(float) + 15) & ~(size_t)15; }
-    __device__ inline float *u(int l) const { return LDS ? (float *)(lds + base(l)) : t.u[l]; }
-    __device__ inline float *f(int l) const { return LDS ? (float *)(lds + base(l) + fl(l)) : t.f[l]; }
+    __device__ inline size_t fl(int l) const { return (2 * t.r[l].half * sizeof(T) + 15) & ~(size_t)15; }
+    __device__ inline T *u(int l) const { return LDS ? (T *)(lds + base(l)) : t.u[l]; }
+    __device__ inline T *f(int l) const { return LDS ? (T *)(lds + base(l) + fl(l)) : t.f[l]; }
     __device__ inline Coef<CT> cf(int l) const
     {
         return LDS ? Coef<CT>{(const CT *)(lds + base(l) + 2 * fl(l)), 2 * t.r[l].half} : t.cf[l];
     }
-    __device__ inline float *f0() const { return LDS ? (float *)(lds + base(t.top + 1)) : t.f0; }
-    __device__ inline float *u0() const
+    __device__ inline T *f0() const { return LDS ? (T *)(lds + base(t.top + 1)) : t.f0; }
+    __device__ inline T *u0() const
     {
         const size_t n0 = (size_t)t.g0.nx * t.g0.ny * t.g0.nz;
-        return LDS ? (float *)(lds + base(t.top + 1) + ((n0 * sizeof(float) + 15) & ~(size_t)15)) : t.u0;
+        return LDS ? (T *)(lds + base(t.top + 1) + ((n0 * sizeof(T) + 15) & ~(size_t)15)) : t.u0;
     }
 };
 
@@ -635,87 +673,88 @@ __device__ inline TailCells tail_cells(const RbGeom &r)
     return c;
 }
 
-template <typename CT, bool LDS, bool ZERO>
-__device__ inline void tail_pass(const TailMem<CT, LDS> &mem, const TailCells &cells, int l, int colour, int zown)
+template <typename CT, typename T, bool LDS, bool ZERO>
+__device__ inline void tail_pass(const TailMem<CT, T, LDS> &mem, const TailCells &cells, int l, int colour, int zown)
 {
-    const RbTail<CT> &t = mem.t;
+    const RbTail<CT, T> &t = mem.t;
     const RbGeom &r = t.r[l];
-    float *u = mem.u(l);
-    const float *f = mem.f(l);
+    T *u = mem.u(l);
+    const T *f = mem.f(l);
     const Coef<CT> cf = mem.cf(l);
 #pragma unroll
     for (int s = 0; s < TAIL_CPT; s++)
-        if (s < cells.n) rb_cell<CT, ZERO>(r, u, f, cf, t.h2[l], colour, t.om, zown, cells.m[s], cells.j[s], cells.k[s]);
+        if (s < cells.n)
+            rb_cell<CT, T, ZERO>(r, u, f, cf, t.h2[l], colour, t.om, zown, cells.m[s], cells.j[s], cells.k[s]);
     __syncthreads();
 }
 
-template <typename CT, bool LDS>
-__device__ inline void tail_body(const TailMem<CT, LDS> &mem, double *y)
+template <typename CT, typename T, bool LDS>
+__device__ inline void tail_body(const TailMem<CT, T, LDS> &mem, double *y)
 {
-    const RbTail<CT> &t = mem.t;
+    const RbTail<CT, T> &t = mem.t;
     for (int l = t.top; l >= 1; l--) {
         const TailCells cells = tail_cells(t.r[l]);
         for (int q = 0; q < t.nu1[l]; q++) {
             if (q == 0)
-                tail_pass<CT, LDS, true>(mem, cells, l, 0, 1);
+                tail_pass<CT, T, LDS, true>(mem, cells, l, 0, 1);
             else
-                tail_pass<CT, LDS, false>(mem, cells, l, 0, 0);
-            tail_pass<CT, LDS, false>(mem, cells, l, 1, q == 0);
+                tail_pass<CT, T, LDS, false>(mem, cells, l, 0, 0);
+            tail_pass<CT, T, LDS, false>(mem, cells, l, 1, q == 0);
         }
         const RbGeom &rf = t.r[l];
         const bool csplit = l - 1 >= 1;
-        const float *u = mem.u(l), *f = mem.f(l);
+        const T *u = mem.u(l), *f = mem.f(l);
         const Coef<CT> cf = mem.cf(l);
-        float *fc = csplit ? mem.f(l - 1) : mem.f0();
+        T *fc = csplit ? mem.f(l - 1) : mem.f0();
         const int cx = rf.mh, cy = rf.my / 2, n = cx * cy * (rf.mz / 2);  // coarse interior
         for (int e = (int)threadIdx.x; e < n; e += TAIL_THREADS) {
             const int I = e % cx, q = e / cx;
-            rb_restrict_cell<CT>(rf, u, f, cf, t.h2i[l], t.r[csplit ? l - 1 : l], csplit ? fc : nullptr, t.g0,
-                                 csplit ? nullptr : fc, I, q % cy, q / cy);
+            rb_restrict_cell<CT, T>(rf, u, f, cf, t.h2i[l], t.r[csplit ? l - 1 : l], csplit ? fc : nullptr, t.g0,
+                                    csplit ? nullptr : fc, I, q % cy, q / cy);
         }
         __syncthreads();
     }
-    coarse_subst<float>(t.g0, t.M, mem.f0(), mem.u0(), y, (int)threadIdx.x, TAIL_THREADS);
+    coarse_subst<T>(t.g0, t.M, mem.f0(), mem.u0(), y, (int)threadIdx.x, TAIL_THREADS);
     __syncthreads();
     for (int l = 1; l <= t.top; l++) {
         const RbGeom &rf = t.r[l];
         const bool csplit = l - 1 >= 1;
-        float *u = mem.u(l);
-        const float *uc = csplit ? mem.u(l - 1) : mem.u0();
+        T *u = mem.u(l);
+        const T *uc = csplit ? mem.u(l - 1) : mem.u0();
         const int n = (int)rf.half;
         for (int e = (int)threadIdx.x; e < n; e += TAIL_THREADS) {
             const int m = e % rf.mh, q = e / rf.mh;
-            rb_prolong_entry(rf, u, csplit ? t.r[l - 1] : rf, csplit ? uc : nullptr, t.g0, csplit ? nullptr : uc, m,
-                             q % rf.my + 2, q / rf.my + 2);
+            rb_prolong_entry<T>(rf, u, csplit ? t.r[l - 1] : rf, csplit ? uc : nullptr, t.g0, csplit ? nullptr : uc,
+                                m, q % rf.my + 2, q / rf.my + 2);
         }
         __syncthreads();
         const TailCells cells = tail_cells(rf);
         for (int q = 0; q < t.nu2[l]; q++) {
-            tail_pass<CT, LDS, false>(mem, cells, l, 1, 0);
-            tail_pass<CT, LDS, false>(mem, cells, l, 0, 0);
+            tail_pass<CT, T, LDS, false>(mem, cells, l, 1, 0);
+            tail_pass<CT, T, LDS, false>(mem, cells, l, 0, 0);
         }
     }
 }
 
-template <typename CT>
-__global__ void __launch_bounds__(TAIL_THREADS) k_rb_tail(RbTail<CT> t)
+template <typename CT, typename T>
+__global__ void __launch_bounds__(TAIL_THREADS) k_rb_tail(RbTail<CT, T> t)
 {
     __shared__ double y[512];
-    const TailMem<CT, false> mem{t, nullptr};
-    tail_body<CT, false>(mem, y);
+    const TailMem<CT, T, false> mem{t, nullptr};
+    tail_body<CT, T, false>(mem, y);
 }
 
-template <typename CT>
-__global__ void __launch_bounds__(TAIL_THREADS) k_rb_tail_lds(RbTail<CT> t)
+template <typename CT, typename T>
+__global__ void __launch_bounds__(TAIL_THREADS) k_rb_tail_lds(RbTail<CT, T> t)
 {
     extern __shared__ __align__(16) unsigned char tail_lds[];
     __shared__ double y[512];
-    const TailMem<CT, true> mem{t, tail_lds};
+    const TailMem<CT, T, true> mem{t, tail_lds};
     // stage the top level's right-hand side and every level's coefficients
     {
         const int l = t.top;
         const int n2 = (int)(2 * t.r[l].half);
-        float *fd = mem.f(l);
+        T *fd = mem.f(l);
         for (int e = (int)threadIdx.x; e < n2; e += TAIL_THREADS) fd[e] = t.f[l][e];
     }
     for (int l = 1; l <= t.top; l++) {
@@ -724,11 +763,11 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_rb_tail_lds(RbTail<CT> t)
         for (int e = (int)threadIdx.x; e < n; e += TAIL_THREADS) cd[e] = t.cf[l].p[e];
     }
     __syncthreads();
-    tail_body<CT, true>(mem, y);
+    tail_body<CT, T, true>(mem, y);
     {
         const int l = t.top;
         const int n2 = (int)(2 * t.r[l].half);
-        const float *us = mem.u(l);
+        const T *us = mem.u(l);
         for (int e = (int)threadIdx.x; e < n2; e += TAIL_THREADS) t.u[l][e] = us[e];
     }
 }
@@ -757,38 +796,41 @@ double lev_h2i(const Ctx &c, const Grid &g)
 // zero: the iterate is zero on entry; zown: this pass's own cells are zero
 // (the second pass of the first sweep).  Adds the pass's algorithmic bytes
 // to *bytes_sum.
-template <typename CT>
+template <typename CT, typename T>
 int colour_pass(Ctx &c, int l, int colour, bool zero, int zown, double *bytes_sum)
 {
-    const float om = (float)c.mg_omega;
+    const T om = (T)c.mg_omega;
     Level &L = c.lev[l];
+    T *u = rb_u_of<T>(L), *f = rb_f_of<T>(L);
     const RbGeom r = rb_geom(L.g);
     const Coef<CT> cf{(const CT *)L.rb_coef, 2 * r.half};
-    const float h2 = (float)(1.0 / lev_h2i(c, L.g));
+    const T h2 = (T)(1.0 / lev_h2i(c, L.g));
     const dim3 b1 = row_block(r.mh), grid = row_grid(b1, r.mh, r.my, r.mz);
     const bool top = l == c.ng - 1;
-    // algorithmic bytes: own f read and u write (one colour, FP32), the other
-    // colour's u (not read from a zero iterate) and all coefficients
+    // algorithmic bytes: own f read and u write (one colour), the other
+    // colour's u (not read from a zero iterate), the own old u (SOR) and
+    // all coefficients
     const double cells = (double)r.half;
-    *bytes_sum += cells * (zero ? 8.0 : (om == 1.f || zown ? 12.0 : 16.0)) + 6.0 * cells * (double)sizeof(CT);
+    *bytes_sum += cells * (double)sizeof(T) * (zero ? 2.0 : (om == (T)1 || zown ? 3.0 : 4.0)) +
+                  6.0 * cells * (double)sizeof(CT);
     if (r.mh % 4 == 0) {
         const dim3 b4 = row_block(r.mh / 4), g4 = row_grid(b4, r.mh / 4, r.my, r.mz);
         if (top && zero)
-            hipLaunchKernelGGL((k_rb_colour4<CT, true, true>), g4, b4, 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, colour, om, zown);
+            hipLaunchKernelGGL((k_rb_colour4<CT, T, true, true>), g4, b4, 0, c.stream, r, u, f, cf, h2, colour, om, zown);
         else if (top)
-            hipLaunchKernelGGL((k_rb_colour4<CT, false, true>), g4, b4, 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, colour, om, zown);
+            hipLaunchKernelGGL((k_rb_colour4<CT, T, false, true>), g4, b4, 0, c.stream, r, u, f, cf, h2, colour, om, zown);
         else if (zero)
-            hipLaunchKernelGGL((k_rb_colour4<CT, true, false>), g4, b4, 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, colour, om, zown);
+            hipLaunchKernelGGL((k_rb_colour4<CT, T, true, false>), g4, b4, 0, c.stream, r, u, f, cf, h2, colour, om, zown);
         else
-            hipLaunchKernelGGL((k_rb_colour4<CT, false, false>), g4, b4, 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, colour, om, zown);
+            hipLaunchKernelGGL((k_rb_colour4<CT, T, false, false>), g4, b4, 0, c.stream, r, u, f, cf, h2, colour, om, zown);
     } else if (top && zero)
-        hipLaunchKernelGGL((k_rb_colour<CT, true, true>), grid, b1, 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, colour, om, zown);
+        hipLaunchKernelGGL((k_rb_colour<CT, T, true, true>), grid, b1, 0, c.stream, r, u, f, cf, h2, colour, om, zown);
     else if (top)
-        hipLaunchKernelGGL((k_rb_colour<CT, false, true>), grid, b1, 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, colour, om, zown);
+        hipLaunchKernelGGL((k_rb_colour<CT, T, false, true>), grid, b1, 0, c.stream, r, u, f, cf, h2, colour, om, zown);
     else if (zero)
-        hipLaunchKernelGGL((k_rb_colour<CT, true, false>), grid, b1, 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, colour, om, zown);
+        hipLaunchKernelGGL((k_rb_colour<CT, T, true, false>), grid, b1, 0, c.stream, r, u, f, cf, h2, colour, om, zown);
     else
-        hipLaunchKernelGGL((k_rb_colour<CT, false, false>), grid, b1, 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, colour, om, zown);
+        hipLaunchKernelGGL((k_rb_colour<CT, T, false, false>), grid, b1, 0, c.stream, r, u, f, cf, h2, colour, om, zown);
     DG_CHECK_LAUNCH();
     return 0;
 }
@@ -796,29 +838,46 @@ int colour_pass(Ctx &c, int l, int colour, bool zero, int zown, double *bytes_su
 // The first post-smoothing sweep of level l (colour 1, then colour 0) with
 // the prolongation from the level below folded in (rows of a multiple of 4
 // entries)
-template <typename CT>
-int colour_pass_pro(Ctx &c, int l, const CoarseE &ce, double *bytes_sum)
+template <typename CT, typename T>
+int colour_pass_pro(Ctx &c, int l, const CoarseE<T> &ce, double *bytes_sum)
 {
-    const float om = (float)c.mg_omega;
+    const T om = (T)c.mg_omega;
     Level &L = c.lev[l];
+    T *u = rb_u_of<T>(L), *f = rb_f_of<T>(L);
     const RbGeom r = rb_geom(L.g);
     const Coef<CT> cf{(const CT *)L.rb_coef, 2 * r.half};
-    const float h2 = (float)(1.0 / lev_h2i(c, L.g));
+    const T h2 = (T)(1.0 / lev_h2i(c, L.g));
     const bool top = l == c.ng - 1;
-    // two colour passes' bytes, each also reading the coarse level (FP32,
-    // a quarter as many cells as one fine colour)
-    const double cells = (double)r.half;
-    *bytes_sum += 2.0 * (cells * (om == 1.f ? 12.0 : 16.0) + 6.0 * cells * (double)sizeof(CT) + cells);
+    // two colour passes' bytes, each also reading the coarse level (a
+    // quarter as many cells as one fine colour)
+    const double cells = (double)r.half, ts = (double)sizeof(T);
+    *bytes_sum += 2.0 * (cells * ts * (om == (T)1 ? 3.0 : 4.0) + 6.0 * cells * (double)sizeof(CT) + cells * ts / 4.0);
     const dim3 b4 = row_block(r.mh / 4), g4 = row_grid(b4, r.mh / 4, r.my, r.mz);
     if (top) {
-        hipLaunchKernelGGL((k_rb_colour4_pro<CT, true, true>), g4, b4, 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, 1, om, ce);
-        hipLaunchKernelGGL((k_rb_colour4_pro<CT, true, false>), g4, b4, 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, 0, om, ce);
+        hipLaunchKernelGGL((k_rb_colour4_pro<CT, T, true, true>), g4, b4, 0, c.stream, r, u, f, cf, h2, 1, om, ce);
+        hipLaunchKernelGGL((k_rb_colour4_pro<CT, T, true, false>), g4, b4, 0, c.stream, r, u, f, cf, h2, 0, om, ce);
     } else {
-        hipLaunchKernelGGL((k_rb_colour4_pro<CT, false, true>), g4, b4, 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, 1, om, ce);
-        hipLaunchKernelGGL((k_rb_colour4_pro<CT, false, false>), g4, b4, 0, c.stream, r, L.rb_u, L.rb_f, cf, h2, 0, om, ce);
+        hipLaunchKernelGGL((k_rb_colour4_pro<CT, T, false, true>), g4, b4, 0, c.stream, r, u, f, cf, h2, 1, om, ce);
+        hipLaunchKernelGGL((k_rb_colour4_pro<CT, T, false, false>), g4, b4, 0, c.stream, r, u, f, cf, h2, 0, om, ce);
     }
     DG_CHECK_LAUNCH();
     return 0;
+}
+
+// the coarsest level's natural-layout right-hand side and iterate of type T
+template <typename T>
+void coarsest_arrays(Level &L0, T *&f0, T *&u0);
+template <>
+void coarsest_arrays<float>(Level &L0, float *&f0, float *&u0)
+{
+    f0 = L0.f32;
+    u0 = L0.u32;
+}
+template <>
+void coarsest_arrays<double>(Level &L0, double *&f0, double *&u0)
+{
+    f0 = L0.f64;
+    u0 = L0.u64;
 }
 
 // highest level taken into the single-workgroup tail, or -1
@@ -830,7 +889,7 @@ int tail_level(const Ctx &c)
     return lt;
 }
 
-template <typename CT>
+template <typename CT, typename T>
 int launch_tail(Ctx &c, int lt)
 {
     const Grid &g0 = c.lev[0].g;
@@ -838,23 +897,22 @@ int launch_tail(Ctx &c, int lt)
         set_error("split V-cycle tail: coarsest level above 512 unknowns");
         return -1;
     }
-    RbTail<CT> t{};
+    RbTail<CT, T> t{};
     t.top = lt;
     for (int l = 1; l <= lt; l++) {
         Level &L = c.lev[l];
         t.r[l] = rb_geom(L.g);
-        t.u[l] = L.rb_u;
-        t.f[l] = L.rb_f;
+        t.u[l] = rb_u_of<T>(L);
+        t.f[l] = rb_f_of<T>(L);
         t.cf[l] = Coef<CT>{(const CT *)L.rb_coef, 2 * t.r[l].half};
-        t.h2[l] = (float)(1.0 / lev_h2i(c, L.g));
-        t.h2i[l] = (float)lev_h2i(c, L.g);
+        t.h2[l] = (T)(1.0 / lev_h2i(c, L.g));
+        t.h2i[l] = (T)lev_h2i(c, L.g);
         t.nu1[l] = c.mg_nuc <= 0 ? c.mg_nu1 : c.mg_nuc;
         t.nu2[l] = c.mg_nuc <= 0 ? c.mg_nu2 : c.mg_nuc;
     }
-    t.om = (float)c.mg_omega;
+    t.om = (T)c.mg_omega;
     t.g0 = g0;
-    t.f0 = c.lev[0].f32;
-    t.u0 = c.lev[0].u32;
+    coarsest_arrays<T>(c.lev[0], t.f0, t.u0);
     t.M = coarse_factor(c);
     // LDS form when the device's per-workgroup LDS holds the staged levels
     // next to the 4 KB substitution buffer
@@ -867,11 +925,11 @@ int launch_tail(Ctx &c, int lt)
     const size_t lds = tail_lds_bytes(t);
     if (c.rb_tail == 2 && lds + 512 * sizeof(double) <= (size_t)lds_max) {
         if (lds > 48 * 1024)
-            DG_HIP(hipFuncSetAttribute((const void *)k_rb_tail_lds<CT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+            DG_HIP(hipFuncSetAttribute((const void *)k_rb_tail_lds<CT, T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)lds));
-        hipLaunchKernelGGL(k_rb_tail_lds<CT>, dim3(1), dim3(TAIL_THREADS), lds, c.stream, t);
+        hipLaunchKernelGGL((k_rb_tail_lds<CT, T>), dim3(1), dim3(TAIL_THREADS), lds, c.stream, t);
     } else {
-        hipLaunchKernelGGL(k_rb_tail<CT>, dim3(1), dim3(TAIL_THREADS), 0, c.stream, t);
+        hipLaunchKernelGGL((k_rb_tail<CT, T>), dim3(1), dim3(TAIL_THREADS), 0, c.stream, t);
     }
     DG_CHECK_LAUNCH();
     return 0;
@@ -892,7 +950,8 @@ constexpr size_t TILE_SWEEP_MAX_CELLS = 262144;
 // all with fused = 1, up to TILE_SWEEP_MAX_CELLS with fused = 2)
 int fused_kind(const Ctx &c, int l)
 {
-    if (c.fused_smoother == 0 || !c.lev[l].rb_w) return 0;
+    // the one-pass sweeps exist for the FP32 iterate only
+    if (c.fused_smoother == 0 || !c.lev[l].rb_w || c.mg_mode != 2) return 0;
     if (l != c.ng - 1)
         return (c.fused_smoother == 1 || 2 * rb_geom(c.lev[l].g).half <= TILE_SWEEP_MAX_CELLS) ? 2 : 0;
     if (c.fused_smoother != 1) return 0;
@@ -943,11 +1002,33 @@ int sweep_fused(Ctx &c, int l, const float *X, float *Y, int ca, bool zero, int 
     return 0;
 }
 
-template <typename CT>
+// the one-pass sweep for the FP32 iterate; never reached for FP64
+// (fused_kind is 0 there)
+template <typename CT, typename T>
+int sweep_fused_t(Ctx &c, int l, const T *X, T *Y, int ca, bool zero, int kind)
+{
+    if constexpr (std::is_same<T, float>::value) {
+        return sweep_fused<CT>(c, l, X, Y, ca, zero, kind);
+    } else {
+        set_error("split V-cycle: no one-pass sweep for the FP64 iterate");
+        return -1;
+    }
+}
+
+// the coarsest level's solve in the iterate's type
+template <typename T>
+int coarse_solve_t(Ctx &c);
+template <>
+int coarse_solve_t<float>(Ctx &c) { return coarse_solve_f32(c); }
+template <>
+int coarse_solve_t<double>(Ctx &c) { return coarse_solve_f64(c); }
+
+template <typename CT, typename T>
 int vcycle_rb(Ctx &c, int l)
 {
-    if (c.rb_tail > 0 && l == tail_level(c)) return launch_tail<CT>(c, l);
+    if (c.rb_tail > 0 && l == tail_level(c)) return launch_tail<CT, T>(c, l);
     Level &L = c.lev[l];
+    T *const lu = rb_u_of<T>(L), *const lf = rb_f_of<T>(L);
     const bool top = l == c.ng - 1;
     const int nu1 = (top || c.mg_nuc <= 0) ? c.mg_nu1 : c.mg_nuc;
     const int nu2 = (top || c.mg_nuc <= 0) ? c.mg_nu2 : c.mg_nuc;
@@ -955,13 +1036,14 @@ int vcycle_rb(Ctx &c, int l)
     // buffer that makes the last one land in rb_u
     const int fk = fused_kind(c, l);
     const bool fused = fk != 0;
-    float *cur = L.rb_u;
-    auto other = [&](float *b) { return b == L.rb_u ? L.rb_w : L.rb_u; };
+    T *cur = lu;
+    T *const lw = (T *)(void *)L.rb_w;  // FP32 only (fused_kind)
+    auto other = [&](T *b) { return b == lu ? lw : lu; };
     if (fused) {
-        float *w1 = ((nu1 + nu2) & 1) ? L.rb_u : L.rb_w;
+        T *w1 = ((nu1 + nu2) & 1) ? lu : lw;
         for (int q = 0; q < nu1; q++) {
-            float *y = q == 0 ? w1 : other(cur);
-            if (sweep_fused<CT>(c, l, cur, y, 0, q == 0, fk)) return -1;
+            T *y = q == 0 ? w1 : other(cur);
+            if (sweep_fused_t<CT, T>(c, l, cur, y, 0, q == 0, fk)) return -1;
             cur = y;
         }
     } else {
@@ -970,8 +1052,8 @@ int vcycle_rb(Ctx &c, int l)
         double pb = 0.0;
         if (top) prof_begin(PK_RB_COLOUR, c.stream);
         for (int q = 0; q < nu1; q++) {
-            if (colour_pass<CT>(c, l, 0, q == 0, q == 0, &pb)) return -1;
-            if (colour_pass<CT>(c, l, 1, false, q == 0, &pb)) return -1;
+            if (colour_pass<CT, T>(c, l, 0, q == 0, q == 0, &pb)) return -1;
+            if (colour_pass<CT, T>(c, l, 1, false, q == 0, &pb)) return -1;
         }
         if (top) prof_end(PK_RB_COLOUR, c.stream, pb, 2 * nu1);
     }
@@ -980,31 +1062,34 @@ int vcycle_rb(Ctx &c, int l)
     Level &C = c.lev[l - 1];
     const bool csplit = l - 1 >= 1;
     const RbGeom rc = rb_geom(C.g);
+    T *cf0 = nullptr, *cu0 = nullptr;
+    coarsest_arrays<T>(C, cf0, cu0);
+    T *const cfs = csplit ? rb_f_of<T>(C) : nullptr, *const cus = csplit ? rb_u_of<T>(C) : nullptr;
     const dim3 bc = row_block(C.g.nx - 2);
-    hipLaunchKernelGGL(k_rb_resid_restrict<CT>, row_grid(bc, C.g.nx - 2, C.g.ny - 2, C.g.nz - 2), bc, 0, c.stream, rf, cur, L.rb_f, cf,
-                       (float)lev_h2i(c, L.g), rc, csplit ? C.rb_f : nullptr, C.g, csplit ? nullptr : C.f32);
+    hipLaunchKernelGGL((k_rb_resid_restrict<CT, T>), row_grid(bc, C.g.nx - 2, C.g.ny - 2, C.g.nz - 2), bc, 0, c.stream,
+                       rf, cur, lf, cf, (T)lev_h2i(c, L.g), rc, cfs, C.g, csplit ? nullptr : cf0);
     DG_CHECK_LAUNCH();
     if (csplit) {
-        if (vcycle_rb<CT>(c, l - 1)) return -1;
+        if (vcycle_rb<CT, T>(c, l - 1)) return -1;
     } else {
-        if (coarse_solve_f32(c)) return -1;
+        if (coarse_solve_t<T>(c)) return -1;
     }
-    const CoarseE ce{rc, csplit ? C.rb_u : nullptr, C.g, csplit ? nullptr : C.u32};
+    const CoarseE<T> ce{rc, cus, C.g, csplit ? nullptr : cu0};
     // the colour passes take the prolongation into their first sweep
     const bool pro_folded = !fused && nu2 >= 1 && rf.mh % 4 == 0;
     if (!pro_folded) {
         const dim3 bf = row_block(rf.mh);
-        hipLaunchKernelGGL(k_rb_prolong, row_grid(bf, rf.mh, rf.my, rf.mz), bf, 0, c.stream, rf, cur, rc,
-                           csplit ? C.rb_u : nullptr, C.g, csplit ? nullptr : C.u32);
+        hipLaunchKernelGGL(k_rb_prolong<T>, row_grid(bf, rf.mh, rf.my, rf.mz), bf, 0, c.stream, rf, cur, rc, cus, C.g,
+                           csplit ? nullptr : cu0);
         DG_CHECK_LAUNCH();
     }
     if (fused) {
         for (int q = 0; q < nu2; q++) {
-            float *y = other(cur);
-            if (sweep_fused<CT>(c, l, cur, y, 1, false, fk)) return -1;
+            T *y = other(cur);
+            if (sweep_fused_t<CT, T>(c, l, cur, y, 1, false, fk)) return -1;
             cur = y;
         }
-        if (cur != L.rb_u) {
+        if (cur != lu) {
             set_error("split V-cycle: fused sweep buffer bookkeeping");
             return -1;
         }
@@ -1013,11 +1098,11 @@ int vcycle_rb(Ctx &c, int l)
         if (top) prof_begin(PK_RB_COLOUR, c.stream);
         for (int q = 0; q < nu2; q++) {
             if (q == 0 && pro_folded) {
-                if (colour_pass_pro<CT>(c, l, ce, &pb)) return -1;
+                if (colour_pass_pro<CT, T>(c, l, ce, &pb)) return -1;
                 continue;
             }
-            if (colour_pass<CT>(c, l, 1, false, 0, &pb)) return -1;
-            if (colour_pass<CT>(c, l, 0, false, 0, &pb)) return -1;
+            if (colour_pass<CT, T>(c, l, 1, false, 0, &pb)) return -1;
+            if (colour_pass<CT, T>(c, l, 0, false, 0, &pb)) return -1;
         }
         if (top) prof_end(PK_RB_COLOUR, c.stream, pb, 2 * nu2);
     }
@@ -1028,7 +1113,7 @@ int vcycle_rb(Ctx &c, int l)
 
 bool rb_enabled(const Ctx &c)
 {
-    if (!c.rb_split || c.mg_mode != 2 || c.ng < 2 || c.mg_nu1 < 1 || c.mg_nuc < 0) return false;
+    if (!c.rb_split || (c.mg_mode != 1 && c.mg_mode != 2) || c.ng < 2 || c.mg_nu1 < 1 || c.mg_nuc < 0) return false;
     for (int l = 1; l < c.ng; l++) {
         const Grid &g = c.lev[l].g;
         if ((g.nx - 2) & 1 || (g.ny - 2) & 1 || (g.nz - 2) & 1) return false;
@@ -1039,36 +1124,53 @@ bool rb_enabled(const Ctx &c)
     return true;
 }
 
+// mode 1: FP64; mode 2: rb_split 1 FP16 when the densities allow, 2 FP32, 3 FP16
+int rb_coef_type(const Ctx &c)
+{
+    if (c.mg_mode == 1) return RB_CT_F64;
+    return (c.rb_split == 3 || (c.rb_split == 1 && c.coef16_ok)) ? RB_CT_F16 : RB_CT_F32;
+}
+
+static size_t coef_bytes(int ct) { return ct == RB_CT_F16 ? 2 : (ct == RB_CT_F32 ? 4 : 8); }
+
 int rb_prepare(Ctx &c)
 {
-    // rb_split: 1 FP16 coefficients when the densities allow, 2 FP32, 3 FP16
-    const int want16 = (c.rb_split == 3 || (c.rb_split == 1 && c.coef16_ok)) ? 1 : 0;
+    const int want = rb_coef_type(c);
+    const bool f64 = c.mg_mode == 1;
     for (int l = 1; l < c.ng; l++) {
         Level &L = c.lev[l];
         const RbGeom r = rb_geom(L.g);
         const size_t n2 = 2 * r.half;
-        if (!L.rb_u) {
+        if (f64 && !L.rb_u64) {
+            DG_HIP(hipMalloc(&L.rb_u64, n2 * sizeof(double)));
+            DG_HIP(hipMalloc(&L.rb_f64, n2 * sizeof(double)));
+            DG_HIP(hipMemset(L.rb_u64, 0, n2 * sizeof(double)));
+            DG_HIP(hipMemset(L.rb_f64, 0, n2 * sizeof(double)));
+        }
+        if (!f64 && !L.rb_u) {
             DG_HIP(hipMalloc(&L.rb_u, n2 * sizeof(float)));
             DG_HIP(hipMalloc(&L.rb_f, n2 * sizeof(float)));
             DG_HIP(hipMemset(L.rb_u, 0, n2 * sizeof(float)));
             DG_HIP(hipMemset(L.rb_f, 0, n2 * sizeof(float)));
         }
-        if (c.fused_smoother != 0 && !L.rb_w) {
+        if (!f64 && c.fused_smoother != 0 && !L.rb_w) {
             DG_HIP(hipMalloc(&L.rb_w, n2 * sizeof(float)));
             DG_HIP(hipMemset(L.rb_w, 0, n2 * sizeof(float)));
         }
-        if (L.rb_coef16 != want16) {
+        if (L.rb_ct != want) {
             if (L.rb_coef) (void)hipFree(L.rb_coef);
             L.rb_coef = nullptr;
-            DG_HIP(hipMalloc(&L.rb_coef, 3 * n2 * (want16 ? sizeof(_Float16) : sizeof(float))));
-            L.rb_coef16 = want16;
+            DG_HIP(hipMalloc(&L.rb_coef, 3 * n2 * coef_bytes(want)));
+            L.rb_ct = want;
         }
         if (l == c.ng - 1 && c.top_coef_ready) continue;  // written by solvelpres's abcei
         const dim3 bl = row_block(L.g.nx - 2), gr = row_grid(bl, L.g.nx - 2, L.g.ny - 2, L.g.nz - 2);
-        if (want16)
+        if (want == RB_CT_F16)
             hipLaunchKernelGGL(k_rb_coef<_Float16>, gr, bl, 0, c.stream, L.g, r, L.a, L.b, L.c, (_Float16 *)L.rb_coef);
-        else
+        else if (want == RB_CT_F32)
             hipLaunchKernelGGL(k_rb_coef<float>, gr, bl, 0, c.stream, L.g, r, L.a, L.b, L.c, (float *)L.rb_coef);
+        else
+            hipLaunchKernelGGL(k_rb_coef<double>, gr, bl, 0, c.stream, L.g, r, L.a, L.b, L.c, (double *)L.rb_coef);
         DG_CHECK_LAUNCH();
     }
     c.top_coef_ready = 0;
@@ -1080,7 +1182,10 @@ int rb_load_rhs(Ctx &c, const double *r)
     Level &F = c.lev[c.ng - 1];
     const RbGeom rg = rb_geom(F.g);
     const dim3 bl = row_block(F.g.nx - 2), gr = row_grid(bl, F.g.nx - 2, F.g.ny - 2, F.g.nz - 2);
-    hipLaunchKernelGGL(k_rb_in, gr, bl, 0, c.stream, F.g, rg, r, F.rb_f);
+    if (c.mg_mode == 1)
+        hipLaunchKernelGGL(k_rb_in<double>, gr, bl, 0, c.stream, F.g, rg, r, F.rb_f64);
+    else
+        hipLaunchKernelGGL(k_rb_in<float>, gr, bl, 0, c.stream, F.g, rg, r, F.rb_f);
     DG_CHECK_LAUNCH();
     return 0;
 }
@@ -1089,13 +1194,12 @@ int rb_vcycle(Ctx &c, const double *r)
 {
     Level &F = c.lev[c.ng - 1];
     if (r && rb_load_rhs(c, r)) return -1;
-    return F.rb_coef16 == 1 ? vcycle_rb<_Float16>(c, c.ng - 1) : vcycle_rb<float>(c, c.ng - 1);
-}
-
-ZSplit rb_top_z(const Ctx &c)
-{
-    const Level &F = c.lev[c.ng - 1];
-    return ZSplit{F.rb_u, rb_geom(F.g)};
+    switch (F.rb_ct) {
+    case RB_CT_F16: return vcycle_rb<_Float16, float>(c, c.ng - 1);
+    case RB_CT_F32: return vcycle_rb<float, float>(c, c.ng - 1);
+    case RB_CT_F64: return vcycle_rb<double, double>(c, c.ng - 1);
+    default: set_error("split V-cycle: coefficients not prepared"); return -1;
+    }
 }
 
 }  // namespace dg

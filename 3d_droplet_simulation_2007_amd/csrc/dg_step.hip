@@ -1062,7 +1062,7 @@ static int stress(Ctx &c)
 }
 
 int mg_abcei(const Grid &g, hipStream_t s, double *a, double *b, double *c, double *ei, const double *cc, double r1,
-             double r2, void *split, int split16)
+             double r2, void *split, int split_ct)
 {
     const int nx = g.nx, ny = g.ny, nz = g.nz, sy = g.sy, sz = g.sz;
     const int mh = (nx - 2) / 2, my = ny - 2;
@@ -1087,11 +1087,16 @@ int mg_abcei(const Grid &g, hipStream_t s, double *a, double *b, double *c, doub
             ei[o] = 1.0 / (na + aw + nb + bs + nc + cb);
             if (split) {  // the conversion of the split V-cycle's set-up (k_rb_coef)
                 const size_t e = (size_t)((i + j + k) & 1) * half + ((size_t)(k - 2) * my + (j - 2)) * mh + ((i - 2) >> 1);
-                if (split16) {
+                if (split_ct == RB_CT_F16) {
                     _Float16 *p = (_Float16 *)split;
                     p[e] = (_Float16)na;
                     p[n2 + e] = (_Float16)nb;
                     p[2 * n2 + e] = (_Float16)nc;
+                } else if (split_ct == RB_CT_F64) {
+                    double *p = (double *)split;
+                    p[e] = na;
+                    p[n2 + e] = nb;
+                    p[2 * n2 + e] = nc;
                 } else {
                     float *p = (float *)split;
                     p[e] = (float)na;
@@ -1332,15 +1337,15 @@ int op_solvelpres(Ctx &c, int t)
     // will use, abcei also writes the finest level's split copy, which the
     // solve's set-up then skips
     void *split = nullptr;
-    int split16 = 0;
-    if (c.mg_mode == 2 && rb_enabled(c)) {
-        const int want16 = (c.rb_split == 3 || (c.rb_split == 1 && c.coef16_ok)) ? 1 : 0;
-        if (L.rb_coef && L.rb_coef16 == want16) {
+    int split_ct = RB_CT_F32;
+    if (rb_enabled(c)) {
+        const int want = rb_coef_type(c);
+        if (L.rb_coef && L.rb_ct == want) {
             split = L.rb_coef;
-            split16 = want16;
+            split_ct = want;
         }
     }
-    if (mg_abcei(c.g, c.stream, L.a, L.b, L.c, L.ei, L.cc, r1, r2, split, split16)) return -1;
+    if (mg_abcei(c.g, c.stream, L.a, L.b, L.c, L.ei, L.cc, r1, r2, split, split_ct)) return -1;
     c.top_coef_ready = split != nullptr;
     if (predictor(c, w1, w2, w3)) return -1;
     if (op_bc_w(c, w1, w2, w3)) return -1;

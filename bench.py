@@ -16,13 +16,20 @@ timed steps, abort the run.
 
 Measured paths, each timed over K steps after W warm-up steps:
   target       FP64 fields, PCG with an FP32 V-cycle preconditioner (FP64
-               residual/update), initial guess from the learned network
-               (BF16, on the 128^3 level) over the previous step's pressure,
-               residual-checked, with the fallback controller;
+               residual/update; red-black split storage, FP16 face
+               coefficients), initial guess: the learned network's pressure
+               prediction (BF16, on the 128^3 level) over a zero start,
+               accepted only when it lowers the residual, with the fallback
+               controller;
   mixed        the same solver without the network, with the faster of the
                zero and previous-step guesses;
   denominator  the same PCG with an FP64 V-cycle (the plain FP64 multigrid
-               solver), with the faster of the zero and previous-step guesses.
+               solver) on the same red-black split storage, launch shapes and
+               single-workgroup tail, FP64 coefficients, with the faster of
+               the zero and previous-step guesses;
+  fp64 natural the FP64 V-cycle on the natural layout (reported beside).
+The target and the denominator are also timed over a later window of the
+same run (steps 150..169 by default), where the splash has developed.
 Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
 """
 from __future__ import annotations
@@ -59,11 +66,14 @@ def splash_inputdata(dg, n):
                              length=0.006, radius=0.001, y0=0.0025, z0=1000.0, cycles=2, npre=2, ntop=4, npost=2)
 
 
-def run_path(dg, txt, mode, steps, warmup, tol, guess=1, profile_kernel=None, profile_steps=2):
+def run_path(dg, txt, mode, steps, warmup, tol, guess=1, profile_kernel=None, profile_steps=2, late_start=0,
+             late_steps=0):
     """W warm-up steps, K timed steps (headline, no instrumentation), then,
     when profile_kernel is given, profile_steps more steps of the same run
     with HIP events around every launch of that kernel on the library's
-    stream (graph replay is off while events are recorded)."""
+    stream (graph replay is off while events are recorded), then, when
+    late_steps > 0, untimed steps up to late_start and late_steps timed
+    steps from there."""
     import numpy as np
     gs = importlib.import_module("3d_droplet_simulation_2007_amd.guess")
     g = dg.DropletGPU(inputdata_text=txt)
@@ -109,9 +119,55 @@ def run_path(dg, txt, mode, steps, warmup, tol, guess=1, profile_kernel=None, pr
         prof = dg.profile_stop()
         if prof["launches"] == 0:
             raise RuntimeError(f"profiling recorded no launch of kernel family {profile_kernel!r}")
+    late = None
+    if late_steps > 0:
+        t = warmup + steps + (profile_steps if profile_kernel is not None else 0)
+        while t < late_start:
+            g.step(t)
+            t += 1
+        g.state()
+        lt0 = time.perf_counter()
+        its = []
+        for q in range(late_steps):
+            g.step(t + q)
+            s = g.state()
+            its.append(s["mg_iters"])
+            if not s["mg_relres"] < tol:
+                raise RuntimeError(f"mode {mode}: late-window step {t + q} did not converge")
+        lt1 = time.perf_counter()
+        late = dict(first_step=t, steps=late_steps, ms_per_step=1e3 * (lt1 - lt0) / late_steps,
+                    mg_iters_per_step=float(np.mean(its)))
     g.close()
     return dict(seconds=t1 - t0, steps=steps, iters=timed_iters, relres=st["mg_relres"], tau=st["tau"],
-                vol0=vol0, vol1=vol1, prof=prof, p=p, guess_stats=guess_stats)
+                vol0=vol0, vol1=vol1, prof=prof, p=p, guess_stats=guess_stats, late=late)
+
+
+def host_facts():
+    """CPU model, logical cores and the restatement's compiler and flags."""
+    import platform
+    import subprocess
+    model = platform.processor() or "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        cc = subprocess.run(["gcc", "--version"], capture_output=True, text=True).stdout.splitlines()[0]
+    except (OSError, IndexError):
+        cc = "gcc (version unknown)"
+    flags = ""
+    try:
+        with open(os.path.join(HERE, "oracle", "Makefile")) as f:
+            for ln in f:
+                if ln.startswith("CFLAGS"):
+                    flags = ln.split("=", 1)[1].strip()
+    except OSError:
+        pass
+    return dict(cpu_model=model, host_logical_cores=os.cpu_count(), compiler=cc, flags=flags)
 
 
 def cpu_baseline(n, steps, tol, omega=1.0, nu_coarse=0):
@@ -162,6 +218,8 @@ def main():
     ap.add_argument("--no-denominator", action="store_true")
     ap.add_argument("--omega", type=float, default=1.15, help="SOR weight of the V-cycle smoother (all paths)")
     ap.add_argument("--nu-coarse", type=int, default=4, help="smoothing sweeps below the finest level (all paths)")
+    ap.add_argument("--late-start", type=int, default=150, help="first step of the later timed window (0: none)")
+    ap.add_argument("--late-steps", type=int, default=20, help="steps in the later timed window")
     ap.add_argument("--weights", default=os.path.join(HERE, "3d_droplet_simulation_2007_amd", "weights",
                                                        "guess_unet.npz"), help="learned-guess network weights")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -185,7 +243,8 @@ def main():
     import numpy as np
     # target: learned guess + FP32-preconditioned FP64 PCG; dominant kernel:
     # the finest-level colour pass of the red-black split FP32 V-cycle
-    tgt = run_path(dg, txt, 2, args.steps, args.warmup, args.tol, guess=2, profile_kernel="rb_colour")
+    late = dict(late_start=args.late_start, late_steps=args.late_steps if args.late_start > 0 else 0)
+    tgt = run_path(dg, txt, 2, args.steps, args.warmup, args.tol, guess=2, profile_kernel="rb_colour", **late)
     out = {}
     den = mixed = None
     if not args.no_denominator:
@@ -195,11 +254,21 @@ def main():
                  for gi, gname in ((0, "zero"), (1, "previous step"))}
         mguess = min(mixes, key=lambda k: mixes[k]["seconds"])
         mixed = mixes[mguess]
-        # the plain FP64 solver with the better of its zero and previous-step guesses
+        # the plain FP64 solver with the better of its zero and previous-step
+        # guesses, on the split layout the mixed solver uses
         dens = {gname: run_path(dg, txt, 1, args.steps, args.warmup, args.tol, guess=gi)
                 for gi, gname in ((0, "zero"), (1, "previous step"))}
         dguess = min(dens, key=lambda k: dens[k]["seconds"])
         den = dens[dguess]
+        # its later window, with the faster guess, in a run of its own
+        den_late = run_path(dg, txt, 1, args.steps, args.warmup, args.tol, guess=(0 if dguess == "zero" else 1),
+                            **late)["late"]
+        # the FP64 V-cycle on the natural layout (stride-2 colour passes)
+        dg.set_tuning(split=False)
+        try:
+            nat = run_path(dg, txt, 1, args.steps, args.warmup, args.tol, guess=(0 if dguess == "zero" else 1))
+        finally:
+            dg.set_tuning()
         a, b = tgt["p"][1:-1, 1:-1, 1:-1], den["p"][1:-1, 1:-1, 1:-1]
         out["p_rel_l2_vs_fp64"] = float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
     value = tgt["steps"] / tgt["seconds"]
@@ -217,14 +286,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 fields and PCG, f32 V-cycle, bf16 guess network",
+        "dtype": "f64 fields and PCG, f32 V-cycle iterate, f16 face coefficients, bf16 guess network",
         "data": "synthetic (reference smooth2 droplet IC)",
         "config": {"workload": f"{n - 2}^3 droplet-on-wall splash, 6 mm cube, r=1 mm, v=-3 m/s, CSF, "
                                f"each step's Poisson solve to {args.tol:g} relative residual",
                    "grid_with_ghosts": [n, n, n], "solver": f"FP64 PCG + FP32 V(2,2) preconditioner, red-black SOR omega {args.omega:g}, "
                                                          f"{args.nu_coarse} sweeps below the finest level",
-                   "initial_guess": "learned (encoder-decoder on the next-coarser level) over the previous "
-                                    "step's pressure, residual-checked, with the fallback controller",
+                   "initial_guess": "learned pressure prediction (encoder-decoder on the next-coarser level) "
+                                    "over a zero start, kept only when it lowers the residual, fallback zero, "
+                                    "with the fallback controller",
                    "parallelism": "single GPU"},
         "guess_weights": {k: weights_meta.get(k) for k in ("grid", "samples", "epochs")},
         "learned_guess": tgt["guess_stats"],
@@ -244,9 +314,23 @@ def main():
         dv = den["steps"] / den["seconds"]
         line["denominator_fp64"] = {"value": dv, "ms_per_step": 1e3 * den["seconds"] / den["steps"],
                                     "mg_iters_per_step": float(np.mean(den["iters"])), "guess": dguess,
+                                    "solver": "FP64 PCG + FP64 V(2,2) on red-black split storage, FP64 "
+                                              "coefficients, the same launch shapes and single-workgroup tail",
                                     "other_guess_ms_per_step": {k: 1e3 * v["seconds"] / v["steps"]
-                                                                for k, v in dens.items() if k != dguess}}
+                                                                for k, v in dens.items() if k != dguess},
+                                    "late_window": den_late}
         line["speedup_vs_fp64"] = value / dv
+        nv = nat["steps"] / nat["seconds"]
+        line["fp64_natural_layout"] = {"value": nv, "ms_per_step": 1e3 * nat["seconds"] / nat["steps"],
+                                       "mg_iters_per_step": float(np.mean(nat["iters"])),
+                                       "speedup_of_target": value / nv}
+        if tgt["late"] and den_late:
+            line["late_window"] = {"first_step": tgt["late"]["first_step"], "steps": tgt["late"]["steps"],
+                                   "target_ms_per_step": tgt["late"]["ms_per_step"],
+                                   "target_mg_iters_per_step": tgt["late"]["mg_iters_per_step"],
+                                   "denominator_ms_per_step": den_late["ms_per_step"],
+                                   "denominator_mg_iters_per_step": den_late["mg_iters_per_step"],
+                                   "speedup_vs_fp64": den_late["ms_per_step"] / tgt["late"]["ms_per_step"]}
         mv = mixed["steps"] / mixed["seconds"]
         line["mixed_without_learned_guess"] = {"value": mv, "ms_per_step": 1e3 * mixed["seconds"] / mixed["steps"],
                                                "mg_iters_per_step": float(np.mean(mixed["iters"])), "guess": mguess,
@@ -256,7 +340,8 @@ def main():
         v, dt = cpu_baseline(n, args.cpu_steps, args.tol, args.omega, args.nu_coarse)
         line["cpu_baseline"] = {"value": v, "unit": "time steps/s", "cores": 1, "kind": "port",
                                 "sample": f"{args.cpu_steps} step(s) of the same {n - 2}^3 workload, FP64 PCG + "
-                                          f"FP64 V-cycle (the denominator's algorithm), {dt:.1f} s"}
+                                          f"FP64 V-cycle (the denominator's algorithm), {dt:.1f} s, one thread",
+                                **host_facts()}
     print(json.dumps(line))
 
 

@@ -72,6 +72,31 @@ def test_tuning_switches_are_bit_identical(dg, n):
         assert np.array_equal(p, outs[0][0]), combo
 
 
+@pytest.mark.parametrize("n", [34, 66])
+def test_fp64_split_is_bit_identical_to_natural(dg, n):
+    """Mode 1 (FP64 V-cycle) on red-black split storage with FP64
+    coefficients, its small levels in one workgroup (global memory: FP64
+    levels of 4096 cells exceed the LDS) or one launch per operation, graph
+    replay or not, against the natural-layout FP64 V-cycle: the same FP64
+    operations in the same order per cell, so the same bits."""
+    print("case FP64 split vs natural", n, flush=True)
+    rhs, a, b, c = sphere_system(n, 3)
+    combos = [(False, True, "lds"), (True, True, "lds"), (True, True, "global"), (True, True, None),
+              (True, False, "lds")]
+    outs = []
+    try:
+        for split, graphs, tail in combos:
+            dg.set_tuning(split=split, graphs=graphs, tail=tail)
+            outs.append(dg.poisson_solve(rhs, a, b, c, mode=1, tol=1e-10))
+    finally:
+        dg.set_tuning()
+    p0, it0, rel0 = outs[0]
+    assert rel0 < 1e-10
+    for (p, it, rel), combo in zip(outs[1:], combos[1:]):
+        assert it == it0 and rel == rel0, (combo, it, it0, rel, rel0)
+        assert np.array_equal(p, p0), combo
+
+
 @pytest.mark.parametrize("n", [34, 66, 130])
 def test_mixed_matches_fp64_and_converges(dg, n):
     print("case poisson", n, flush=True)

@@ -33,6 +33,7 @@
 static double dmax1(double a, double b) { return a > b ? a : b; }
 static double dmin1(double a, double b) { return a < b ? a : b; }
 static double sq(double a) { return a * a; }
+static double cube(double a) { return a * a * a; }
 
 /* REAL*4 literals promoted to REAL*8 */
 #define F4(x) ((double)(float)(x))
@@ -771,6 +772,363 @@ void or_csf(or_state *st)
 }
 
 /* ------------------------------------------------------------------ */
+/* Interface heights, CalcH3d main.f:1312-1810.  Per interior cell: the
+ * Youngs normal from the 27-point stencil (main.f:1332-1355), the cut of
+ * the unit cube by the plane of that normal holding the volume
+ * min(c, 1-c), classified by the sorted normal components n1 <= n2 <= n3
+ * into the cases of the reference (main.f:1377-1584: two-dimensional,
+ * triangle, quadrilateral A, pentagon and hexagon by Newton iteration,
+ * quadrilateral B), the six face heights h1a..h3b of that cut, mirrored
+ * for c > 0.5 (main.f:1590-1600) and assigned to the x/y/z faces by which
+ * normal component is the smallest and the middle one (main.f:1602-1728,
+ * including its use of niz's sign for the y heights in the last branch),
+ * swapped for c <= 0.5 (main.f:1733-1737), clipped to [0, 1].  The
+ * reference STOPs when no case matches; stop_code records that. */
+static void hset(int negative, double a, double b, double *lo, double *hi)
+{
+    if (negative) { *lo = a; *hi = b; }
+    else { *lo = b; *hi = a; }
+}
+
+void or_calch3d(or_state *st)
+{
+    int nx = st->nx, ny = st->ny, nz = st->nz;
+    const double *cc = st->lev[st->ng - 1].cc;
+    st->stop_code = 0;
+    for (int k = 2; k <= nz - 1; k++)
+        for (int j = 2; j <= ny - 1; j++)
+            for (int i = 2; i <= nx - 1; i++) {
+#define C(a, b, c) AT(cc, a, b, c)
+                double mm1 = C(i - 1, j - 1, k - 1) + C(i - 1, j - 1, k + 1) + C(i - 1, j + 1, k - 1) + C(i - 1, j + 1, k + 1) +
+                             2.0 * (C(i - 1, j - 1, k) + C(i - 1, j + 1, k) + C(i - 1, j, k - 1) + C(i - 1, j, k + 1)) +
+                             4.0 * C(i - 1, j, k);
+                double mm2 = C(i + 1, j - 1, k - 1) + C(i + 1, j - 1, k + 1) + C(i + 1, j + 1, k - 1) + C(i + 1, j + 1, k + 1) +
+                             2.0 * (C(i + 1, j - 1, k) + C(i + 1, j + 1, k) + C(i + 1, j, k - 1) + C(i + 1, j, k + 1)) +
+                             4.0 * C(i + 1, j, k);
+                const double normx = mm1 - mm2;
+                mm1 = C(i - 1, j - 1, k - 1) + C(i - 1, j - 1, k + 1) + C(i + 1, j - 1, k - 1) + C(i + 1, j - 1, k + 1) +
+                      2.0 * (C(i - 1, j - 1, k) + C(i + 1, j - 1, k) + C(i, j - 1, k - 1) + C(i, j - 1, k + 1)) +
+                      4.0 * C(i, j - 1, k);
+                mm2 = C(i - 1, j + 1, k - 1) + C(i - 1, j + 1, k + 1) + C(i + 1, j + 1, k - 1) + C(i + 1, j + 1, k + 1) +
+                      2.0 * (C(i - 1, j + 1, k) + C(i + 1, j + 1, k) + C(i, j + 1, k - 1) + C(i, j + 1, k + 1)) +
+                      4.0 * C(i, j + 1, k);
+                const double normy = mm1 - mm2;
+                mm1 = C(i - 1, j - 1, k - 1) + C(i - 1, j + 1, k - 1) + C(i + 1, j - 1, k - 1) + C(i + 1, j + 1, k - 1) +
+                      2.0 * (C(i - 1, j, k - 1) + C(i + 1, j, k - 1) + C(i, j - 1, k - 1) + C(i, j + 1, k - 1)) +
+                      4.0 * C(i, j, k - 1);
+                mm2 = C(i - 1, j - 1, k + 1) + C(i - 1, j + 1, k + 1) + C(i + 1, j - 1, k + 1) + C(i + 1, j + 1, k + 1) +
+                      2.0 * (C(i - 1, j, k + 1) + C(i + 1, j, k + 1) + C(i, j - 1, k + 1) + C(i, j + 1, k + 1)) +
+                      4.0 * C(i, j, k + 1);
+                const double normz = mm1 - mm2;
+                const double c0 = C(i, j, k);
+#undef C
+                const double nn = sqrt(normx * normx + normy * normy + normz * normz + 1.0e-50);
+                const double nix = normx / nn, niy = normy / nn, niz = normz / nn;
+                const double ccnew = c0 > 0.5 ? 1.0 - c0 : c0;
+                const double absxn = fabs(nix), absyn = fabs(niy), abszn = fabs(niz);
+                const double n1 = dmin1(dmin1(absxn, absyn), abszn);
+                const double n3 = dmax1(dmax1(absxn, absyn), abszn);
+                const double n2 = absxn + absyn + abszn - n1 - n3;
+                const double clim1 = sqrt(2.0 * ccnew * n3 * n2);
+                double h1a, h1b, h2a, h2b, h3a, h3b, diag, s1, s2, s3;
+                if (n1 == 0.0 && n2 > 0.0 && ccnew <= clim1) {
+                    s2 = sqrt(2.0 * ccnew * n3 / n2);
+                    s3 = sqrt(2.0 * ccnew * n2 / n3);
+                    h1a = s2 * s3 / 2.0; h1b = s2 * s3 / 2.0;
+                    h2a = s3; h2b = 0.0; h3a = s2; h3b = 0.0;
+                } else if (n1 == 0.0 && n2 > 0.0 && ccnew > clim1) {
+                    h1a = c0; h1b = c0;
+                    h2a = c0 + n2 / (2.0 * n3); h2b = c0 - n2 / (2.0 * n3);
+                    h3a = 1.0; h3b = 0.0;
+                } else if (n1 == 0.0 && n2 == 0.0) {
+                    h1a = ccnew; h1b = ccnew; h2a = ccnew; h2b = ccnew; h3a = 1.0; h3b = 0.0;
+                } else if (6.0 * n2 * n3 * ccnew < n1 * n1) {
+                    diag = pow(6.0 * n1 * n2 * n3 * ccnew, 1.0 / 3.0);
+                    s1 = diag / n1; s2 = diag / n2; s3 = diag / n3;
+                    h1a = (s2 * s3) / 2.0; h1b = 0.0; h2a = (s1 * s3) / 2.0; h2b = 0.0; h3a = (s1 * s2) / 2.0; h3b = 0.0;
+                } else if (6.0 * n2 * n3 * ccnew < 3.0 * (n2 * n2) - 3.0 * n1 * n2 + n1 * n1) {
+                    diag = (3.0 * n1 + sqrt(72.0 * n2 * n3 * ccnew - 3.0 * (n1 * n1))) / 6.0;
+                    s1 = diag / n1; s2 = diag / n2; s3 = diag / n3;
+                    h1a = (s2 * s3) / 2.0;
+                    h1b = (s2 * s3) / 2.0 * sq((s1 - 1.0) / s1);
+                    h2a = s3 * (2.0 * s1 - 1.0) / (2.0 * s1); h2b = 0.0;
+                    h3a = s2 * (2.0 * s1 - 1.0) / (2.0 * s1); h3b = 0.0;
+                } else if ((n1 + n2 >= n3 && 6.0 * n1 * n2 * n3 * ccnew < n3 * n3 * n3 - cube(n3 - n1) - cube(n3 - n2)) ||
+                           (n1 + n2 < n3 && 2.0 * n3 * ccnew < n1 + n2)) {
+                    const double rsixn3 = 1.0 / (6.0 * n1 * n2 * n3);
+                    diag = n2;
+                    for (int loop = 1; loop <= 1000; loop++) {
+                        const double delta = -(rsixn3 * (cube(diag) - cube(diag - n1) - cube(diag - n2)) - ccnew) /
+                                             (3.0 * rsixn3 * (diag * diag - sq(diag - n1) - sq(diag - n2)));
+                        diag = diag + delta;
+                        if (fabs(delta) < 1.0e-6) break;
+                    }
+                    s1 = diag / n1; s2 = diag / n2; s3 = diag / n3;
+                    h1a = s3 * (2.0 * s2 - 1.0) / (2.0 * s2);
+                    h1b = (s2 * s3) / 2.0 * sq((s1 - 1.0) / s1);
+                    h2a = s3 * (2.0 * s1 - 1.0) / (2.0 * s1);
+                    h2b = (s1 * s3) / 2.0 * sq((s2 - 1.0) / s2);
+                    h3a = 2.0 * (s1 + s2) - s1 / s2 - s2 / s1 - s1 * s2 - 1.0; h3b = 0.0;
+                } else if (n1 + n2 > n3 && 6.0 * n1 * n2 * n3 * ccnew >= n3 * n3 * n3 - cube(n3 - n1) - cube(n3 - n2)) {
+                    const double rsixn3 = 1.0 / (6.0 * n1 * n2 * n3);
+                    diag = n3;
+                    for (int loop = 1; loop <= 1000; loop++) {
+                        const double delta =
+                            -(rsixn3 * (cube(diag) - cube(diag - n1) - cube(diag - n2) - cube(diag - n3)) - ccnew) /
+                            (3.0 * rsixn3 * (diag * diag - sq(diag - n1) - sq(diag - n2) - sq(diag - n3)));
+                        diag = diag + delta;
+                        if (fabs(delta) < 1.0e-6) break;
+                    }
+                    s1 = diag / n1; s2 = diag / n2; s3 = diag / n3;
+                    h1a = 2.0 * (s2 + s3) - s2 / s3 - s3 / s2 - s2 * s3 - 1.0;
+                    h1b = (s2 * s3) / 2.0 * sq((s1 - 1.0) / s1);
+                    h2a = 2.0 * (s1 + s3) - s1 / s3 - s3 / s1 - s1 * s3 - 1.0;
+                    h2b = (s1 * s3) / 2.0 * sq((s2 - 1.0) / s2);
+                    h3a = 2.0 * (s1 + s2) - s1 / s2 - s2 / s1 - s1 * s2 - 1.0;
+                    h3b = (s1 * s2) / 2.0 * sq((s3 - 1.0) / s3);
+                } else if (n1 + n2 < n3 && 6.0 * n1 * n2 * n3 * ccnew >= cube(n1 + n2) - n1 * n1 * n1 - n2 * n2 * n2) {
+                    diag = 0.5 * (n1 + n2) + ccnew * n3;
+                    s1 = diag / n1; s2 = diag / n2; s3 = diag / n3;
+                    h1a = s3 * (2.0 * s2 - 1.0) / (2.0 * s2);
+                    h1b = s3 * (2.0 * s2 - 1.0) * (s1 - 1.0) / (2.0 * s1 * s2);
+                    h2a = s3 * (2.0 * s1 - 1.0) / (2.0 * s1);
+                    h2b = s3 * (2.0 * s1 - 1.0) * (s2 - 1.0) / (2.0 * s1 * s2);
+                    h3a = 1.0; h3b = 0.0;
+                } else {
+                    st->stop_code = 1;  /* "NO MATCH CASES FOUND" */
+                    return;
+                }
+                if (c0 > 0.5) {
+                    h1a = 1.0 - h1a; h1b = 1.0 - h1b; h2a = 1.0 - h2a;
+                    h2b = 1.0 - h2b; h3a = 1.0 - h3a; h3b = 1.0 - h3b;
+                }
+                double xl = 0, xh = 0, yl = 0, yh = 0, zl = 0, zh = 0;
+                if (absxn == n1) {
+                    hset(nix < 0, h1a, h1b, &xl, &xh);
+                    if (fabs(absyn - n2) < 1.0e-6) {
+                        hset(niy < 0, h2a, h2b, &yl, &yh);
+                        hset(niz < 0, h3a, h3b, &zl, &zh);
+                    } else if (fabs(abszn - n2) < 1.0e-6) {
+                        hset(niy < 0, h3a, h3b, &yl, &yh);
+                        hset(niz < 0, h2a, h2b, &zl, &zh);
+                    } else {
+                        st->stop_code = 2;
+                        return;
+                    }
+                } else if (absyn == n1) {
+                    hset(niy < 0, h1a, h1b, &yl, &yh);
+                    if (fabs(absxn - n2) < 1.0e-6) {
+                        hset(nix < 0, h2a, h2b, &xl, &xh);
+                        hset(niz < 0, h3a, h3b, &zl, &zh);
+                    } else if (fabs(abszn - n2) < 1.0e-6) {
+                        hset(nix < 0, h3a, h3b, &xl, &xh);
+                        hset(niz < 0, h2a, h2b, &zl, &zh);
+                    } else {
+                        st->stop_code = 2;
+                        return;
+                    }
+                } else if (abszn == n1) {
+                    hset(niz < 0, h1a, h1b, &zl, &zh);
+                    if (fabs(absxn - n2) < 1.0e-6) {
+                        hset(nix < 0, h2a, h2b, &xl, &xh);
+                        hset(niz < 0, h3a, h3b, &yl, &yh);  /* the y heights follow niz's sign (main.f:1691-1697) */
+                    } else if (fabs(absyn - n2) < 1.0e-6) {
+                        hset(nix < 0, h3a, h3b, &xl, &xh);
+                        hset(niz < 0, h2a, h2b, &yl, &yh);
+                    } else {
+                        st->stop_code = 2;
+                        return;
+                    }
+                } else {
+                    /* no component equals the minimum: a NaN normal; the
+                     * reference assigns nothing and keeps the old heights */
+                    xl = AT(st->hxmin, i, j, k); xh = AT(st->hxmax, i, j, k);
+                    yl = AT(st->hymin, i, j, k); yh = AT(st->hymax, i, j, k);
+                    zl = AT(st->hzmin, i, j, k); zh = AT(st->hzmax, i, j, k);
+                }
+                if (c0 <= 0.5) {
+                    double t;
+                    t = xl; xl = xh; xh = t;
+                    t = yl; yl = yh; yh = t;
+                    t = zl; zl = zh; zh = t;
+                }
+                double hv[6] = {xl, xh, yl, yh, zl, zh};
+                for (int q = 0; q < 6; q++) if (hv[q] < 0.0) hv[q] = 0.0;
+                for (int q = 0; q < 6; q++) if (hv[q] > 1.0) hv[q] = 1.0;
+                AT(st->hxmin, i, j, k) = hv[0]; AT(st->hxmax, i, j, k) = hv[1];
+                AT(st->hymin, i, j, k) = hv[2]; AT(st->hymax, i, j, k) = hv[3];
+                AT(st->hzmin, i, j, k) = hv[4]; AT(st->hzmax, i, j, k) = hv[5];
+            }
+}
+
+/* the curvature -div(grad c / |grad c|) of SGIP and SGIP2 (main.f:2878-2921,
+ * 3074-3117): csf's face-normalised differences, with the tangential sums
+ * of the x-down, y-up and z-down faces added in SGIP's own order */
+static void sgip_curvature(const double *ccf, double *w11, int nx, int ny, int nz)
+{
+    const double e50 = 1.0e-50;
+    for (int j = 2; j <= ny - 1; j++)
+        for (int i = 2; i <= nx - 1; i++)
+            for (int k = 2; k <= nz - 1; k++) {
+#define F(a, b, c) AT(ccf, a, b, c)
+                double nxu = sqrt(sq(F(i + 1, j, k) - F(i, j, k)) +
+                                  0.0625 * sq(F(i, j + 1, k) - F(i, j - 1, k) + F(i + 1, j + 1, k) - F(i + 1, j - 1, k)) +
+                                  0.0625 * sq(F(i, j, k + 1) - F(i, j, k - 1) + F(i + 1, j, k + 1) - F(i + 1, j, k - 1)) + e50);
+                double nxd = sqrt(sq(F(i, j, k) - F(i - 1, j, k)) +
+                                  0.0625 * sq(F(i - 1, j + 1, k) - F(i - 1, j - 1, k) + F(i, j + 1, k) - F(i, j - 1, k)) +
+                                  0.0625 * sq(F(i - 1, j, k + 1) - F(i - 1, j, k - 1) + F(i, j, k + 1) - F(i, j, k - 1)) + e50);
+                double nyu = sqrt(sq(F(i, j + 1, k) - F(i, j, k)) +
+                                  0.0625 * sq(F(i + 1, j + 1, k) - F(i - 1, j + 1, k) + F(i + 1, j, k) - F(i - 1, j, k)) +
+                                  0.0625 * sq(F(i, j + 1, k + 1) - F(i, j + 1, k - 1) + F(i, j, k + 1) - F(i, j, k - 1)) + e50);
+                double nyd = sqrt(sq(F(i, j, k) - F(i, j - 1, k)) +
+                                  0.0625 * sq(F(i + 1, j, k) - F(i - 1, j, k) + F(i + 1, j - 1, k) - F(i - 1, j - 1, k)) +
+                                  0.0625 * sq(F(i, j, k + 1) - F(i, j, k - 1) + F(i, j - 1, k + 1) - F(i, j - 1, k - 1)) + e50);
+                double nzu = sqrt(sq(F(i, j, k + 1) - F(i, j, k)) +
+                                  0.0625 * sq(F(i + 1, j, k) - F(i - 1, j, k) + F(i + 1, j, k + 1) - F(i - 1, j, k + 1)) +
+                                  0.0625 * sq(F(i, j + 1, k) - F(i, j - 1, k) + F(i, j + 1, k + 1) - F(i, j - 1, k + 1)) + e50);
+                double nzd = sqrt(sq(F(i, j, k) - F(i, j, k - 1)) +
+                                  0.0625 * sq(F(i + 1, j, k - 1) - F(i - 1, j, k - 1) + F(i + 1, j, k) - F(i - 1, j, k)) +
+                                  0.0625 * sq(F(i, j + 1, k - 1) - F(i, j - 1, k - 1) + F(i, j + 1, k) - F(i, j - 1, k)) + e50);
+                double d1x = (F(i + 1, j, k) - F(i, j, k)) / nxu;
+                double d2x = (F(i, j, k) - F(i - 1, j, k)) / nxd;
+                double d1y = (F(i, j + 1, k) - F(i, j, k)) / nyu;
+                double d2y = (F(i, j, k) - F(i, j - 1, k)) / nyd;
+                double d1z = (F(i, j, k + 1) - F(i, j, k)) / nzu;
+                double d2z = (F(i, j, k) - F(i, j, k - 1)) / nzd;
+#undef F
+                AT(w11, i, j, k) = -(d1x - d2x + d1y - d2y + d1z - d2z);
+            }
+}
+
+/* SIND / COSD of the reference, built with -Ddsind=sind -Ddcosd=cosd
+ * (oracle/build_ref.sh): the Fortran 2023 intrinsics, the sine and cosine
+ * of the argument times pi/180 */
+#define OR_PI 3.14159265358979323846
+static double sind_(double x) { return sin(x * (OR_PI / 180.0)); }
+static double cosd_(double x) { return cos(x * (OR_PI / 180.0)); }
+
+/* SGIP surface tension, main.f:2835-3027: one filter pass (as csf), the
+ * curvature, S_dd = sigh (w11_i + w11_{i-1}) (ccf_{i+1} - ccf_{i-1})
+ * |L - L_down| / 2 with L the mean interface height of CalcH3d along d;
+ * on the first row above the wall (j = 2) a contact-angle form with the
+ * angle theta from the contact-line velocity (main.f:2954-2999); bc_out
+ * of S11, S22, S33 and the copies w = S.  No bc_tensor. */
+void or_sgip(or_state *st)
+{
+    int nx = st->nx, ny = st->ny, nz = st->nz;
+    or_level *L = &st->lev[st->ng - 1];
+    double *ccc = L->cc, *w11 = L->u, *w22 = L->res, *w33 = L->rhs;
+    size_t n = (size_t)nx * ny * nz;
+    double *ccf = (double *)malloc(sizeof(double) * n);
+    copyarr(ccf, ccc, nx, ny, nz);
+    or_filterc27(ccf, w22, nx, ny, nz);
+    copyarr(ccf, w22, nx, ny, nz);
+    bc_out(ccf, nx, ny, nz);
+    if (st->sigma == 0.0) {
+        fillzero(st->s11, nx, ny, nz); fillzero(st->s22, nx, ny, nz); fillzero(st->s33, nx, ny, nz);
+        fillzero(st->s12, nx, ny, nz); fillzero(st->s13, nx, ny, nz); fillzero(st->s23, nx, ny, nz);
+        free(ccf);
+        return;
+    }
+    const double sigh = 0.5 * (st->tau * st->tau) * st->sigma / (st->h * st->h * st->h);
+    sgip_curvature(ccf, w11, nx, ny, nz);
+    bc_out(w11, nx, ny, nz);
+    const double *u = st->u, *w = st->w;
+    for (int k = 2; k <= nz - 1; k++)
+        for (int j = 2; j <= ny - 1; j++)
+            for (int i = 2; i <= nx - 1; i++) {
+                const double Lx = (AT(st->hxmax, i, j, k) + AT(st->hxmin, i, j, k)) / 2.0;
+                const double Lxd = (AT(st->hxmax, i - 1, j, k) + AT(st->hxmin, i - 1, j, k)) / 2.0;
+                const double Ly = (AT(st->hymax, i, j, k) + AT(st->hymin, i, j, k)) / 2.0;
+                const double Lyd = (AT(st->hymax, i, j - 1, k) + AT(st->hymin, i, j - 1, k)) / 2.0;
+                const double Lz = (AT(st->hzmax, i, j, k) + AT(st->hzmin, i, j, k)) / 2.0;
+                const double Lzd = (AT(st->hzmax, i, j, k - 1) + AT(st->hzmin, i, j, k - 1)) / 2.0;
+                AT(st->s11, i, j, k) = sigh * (AT(w11, i, j, k) + AT(w11, i - 1, j, k)) *
+                                       (AT(ccf, i + 1, j, k) - AT(ccf, i - 1, j, k)) * fabs(Lx - Lxd) / 2.0;
+                AT(st->s22, i, j, k) = sigh * (AT(w11, i, j, k) + AT(w11, i, j - 1, k)) *
+                                       (AT(ccf, i, j + 1, k) - AT(ccf, i, j - 1, k)) * fabs(Ly - Lyd) / 2.0;
+                AT(st->s33, i, j, k) = sigh * (AT(w11, i, j, k) + AT(w11, i, j, k - 1)) *
+                                       (AT(ccf, i, j, k + 1) - AT(ccf, i, j, k - 1)) * fabs(Lz - Lzd) / 2.0;
+                if (j == 2) {
+                    double mx = AT(ccf, i + 1, j, k) - AT(ccf, i - 1, j, k);
+                    double mz = AT(ccf, i, j, k + 1) - AT(ccf, i, j, k - 1);
+                    const double mmx = mx / sqrt(mz * mz + mx * mx + 1.0e-50);
+                    const double mmz = mz / sqrt(mz * mz + mx * mx + 1.0e-50);
+                    double normv = mmx * AT(u, i, j, k) + mmz * AT(w, i, j, k);
+                    normv = -normv;
+                    double theta = 0.0;
+                    if (fabs(normv) > (double)0.1f) {
+                        if (normv > 0) theta = 115.0;
+                        if (normv < 0) theta = 45.0;
+                    } else {
+                        theta = (double)350.0f * normv + 80.0;
+                    }
+                    const double my = AT(ccf, i, j + 1, k) - AT(ccf, i, j - 1, k);
+                    const double mmy = cosd_(theta);
+                    AT(st->s11, i, j, k) = sigh * (AT(w11, i, j, k) + AT(w11, i - 1, j, k)) * mx * sind_(theta) / 2.0;
+                    AT(st->s22, i, j, k) = sigh * (AT(w11, i, j, k) + AT(w11, i, j - 1, k)) * mmy *
+                                           sqrt(mz * mz + mx * mx + my * my) * -1.0;
+                    AT(st->s33, i, j, k) = sigh * (AT(w11, i, j, k) + AT(w11, i, j, k - 1)) * mz * sind_(theta) / 2.0;
+                }
+            }
+    bc_out(st->s11, nx, ny, nz);
+    bc_out(st->s22, nx, ny, nz);
+    bc_out(st->s33, nx, ny, nz);
+    copyarr(w11, st->s11, nx, ny, nz);
+    copyarr(w22, st->s22, nx, ny, nz);
+    copyarr(w33, st->s33, nx, ny, nz);
+    free(ccf);
+}
+
+/* SGIP2 (second order), main.f:3029-3166: three filter passes, sigh =
+ * 0.5 tau sigma / h^2, the centred form S_dd = sigh (w11_{d+1} + w11_{d-1})
+ * (ccf_{d+1} - ccf_{d-1}) (L_{d+1} - L_{d-1}) / 4; no bc_out of S, no
+ * bc_tensor. */
+void or_sgip2(or_state *st)
+{
+    int nx = st->nx, ny = st->ny, nz = st->nz;
+    or_level *L = &st->lev[st->ng - 1];
+    double *ccc = L->cc, *w11 = L->u, *w22 = L->res, *w33 = L->rhs;
+    size_t n = (size_t)nx * ny * nz;
+    double *ccf = (double *)malloc(sizeof(double) * n);
+    copyarr(ccf, ccc, nx, ny, nz);
+    for (int pass = 0; pass < 3; pass++) {
+        or_filterc27(ccf, w22, nx, ny, nz);
+        copyarr(ccf, w22, nx, ny, nz);
+        bc_out(ccf, nx, ny, nz);
+    }
+    if (st->sigma == 0.0) {
+        fillzero(st->s11, nx, ny, nz); fillzero(st->s22, nx, ny, nz); fillzero(st->s33, nx, ny, nz);
+        fillzero(st->s12, nx, ny, nz); fillzero(st->s13, nx, ny, nz); fillzero(st->s23, nx, ny, nz);
+        free(ccf);
+        return;
+    }
+    const double sigh = 0.5 * st->tau * st->sigma / (st->h * st->h);
+    sgip_curvature(ccf, w11, nx, ny, nz);
+    bc_out(w11, nx, ny, nz);
+    for (int k = 2; k <= nz - 1; k++)
+        for (int j = 2; j <= ny - 1; j++)
+            for (int i = 2; i <= nx - 1; i++) {
+                const double Lxu = (AT(st->hxmax, i + 1, j, k) + AT(st->hxmin, i + 1, j, k)) / 2.0;
+                const double Lxd = (AT(st->hxmax, i - 1, j, k) + AT(st->hxmin, i - 1, j, k)) / 2.0;
+                const double Lyu = (AT(st->hymax, i, j + 1, k) + AT(st->hymin, i, j + 1, k)) / 2.0;
+                const double Lyd = (AT(st->hymax, i, j - 1, k) + AT(st->hymin, i, j - 1, k)) / 2.0;
+                const double Lzu = (AT(st->hzmax, i, j, k + 1) + AT(st->hzmin, i, j, k + 1)) / 2.0;
+                const double Lzd = (AT(st->hzmax, i, j, k - 1) + AT(st->hzmin, i, j, k - 1)) / 2.0;
+                AT(st->s11, i, j, k) = sigh * (AT(w11, i + 1, j, k) + AT(w11, i - 1, j, k)) *
+                                       (AT(ccf, i + 1, j, k) - AT(ccf, i - 1, j, k)) * (Lxu - Lxd) * 0.25;
+                AT(st->s22, i, j, k) = sigh * (AT(w11, i, j + 1, k) + AT(w11, i, j - 1, k)) *
+                                       (AT(ccf, i, j + 1, k) - AT(ccf, i, j - 1, k)) * (Lyu - Lyd) * 0.25;
+                AT(st->s33, i, j, k) = sigh * (AT(w11, i, j, k + 1) + AT(w11, i, j, k - 1)) *
+                                       (AT(ccf, i, j, k + 1) - AT(ccf, i, j, k - 1)) * (Lzu - Lzd) * 0.25;
+            }
+    copyarr(w11, st->s11, nx, ny, nz);
+    copyarr(w22, st->s22, nx, ny, nz);
+    copyarr(w33, st->s33, nx, ny, nz);
+    free(ccf);
+}
+
+/* ------------------------------------------------------------------ */
 /* multigrid kernels, main.f:3765-3997                                  */
 
 void or_rstrct(double *uc, const double *uf, int ncx, int ncy, int ncz)
@@ -1186,9 +1544,12 @@ void or_solvelpres(or_state *st, int t)
 void or_step(or_state *st, int t)
 {
     or_vofmethod(st, t);
+    if (st->hswitch == 1) or_calch3d(st);  /* main.f:30 */
     if (st->iturb == 3) or_kepsmodel(st, t);
     or_checktime(st, t);
     if (st->surfacet == 2) or_csf(st);
+    else if (st->surfacet == 5) or_sgip(st);
+    else if (st->surfacet == 6) or_sgip2(st);
     or_solvelpres(st, t);
 }
 
@@ -1399,6 +1760,8 @@ or_state *or_create(const char *path, char *errbuf, int errlen)
     st->rhon = zalloc(n); st->mun = zalloc(n); st->mut = zalloc(n);
     st->ktur = zalloc(n); st->eps = zalloc(n); st->cmu = zalloc(n);
     st->p_prev = zalloc(n);
+    st->hxmin = zalloc(n); st->hxmax = zalloc(n); st->hymin = zalloc(n);
+    st->hymax = zalloc(n); st->hzmin = zalloc(n); st->hzmax = zalloc(n);
     st->mg_mode = 0;
     st->mg_nu1 = 2; st->mg_nu2 = 2; st->mg_maxcycles = 200; st->mg_tol = 1e-8; st->mg_guess = 0;
     nondim(st);
@@ -1420,6 +1783,7 @@ void or_destroy(or_state *st)
     free(st->s11); free(st->s22); free(st->s33); free(st->s12); free(st->s13); free(st->s23);
     free(st->rhon); free(st->mun); free(st->mut); free(st->ktur); free(st->eps); free(st->cmu);
     free(st->p_prev);
+    free(st->hxmin); free(st->hxmax); free(st->hymin); free(st->hymax); free(st->hzmin); free(st->hzmax);
     free(st);
 }
 
@@ -1450,6 +1814,12 @@ double *or_field(or_state *st, int id)
     case OR_F_RHS: return L->rhs;
     case OR_F_RES: return L->res;
     case OR_F_PPREV: return st->p_prev;
+    case OR_F_HXMIN: return st->hxmin;
+    case OR_F_HXMAX: return st->hxmax;
+    case OR_F_HYMIN: return st->hymin;
+    case OR_F_HYMAX: return st->hymax;
+    case OR_F_HZMIN: return st->hzmin;
+    case OR_F_HZMAX: return st->hzmax;
     default: return NULL;
     }
 }

@@ -88,6 +88,13 @@ typedef struct or_state {
      * check, main.f:274-322), so the FMG schedule (mglin) may run; 0: an
      * m*2**k + 2 grid, tolerance mode only */
     int ref_grid;
+    /* /NEWW/ interface heights of CalcH3d (main.f:1312-1810), read by SGIP
+     * and SGIP2; allocated zeroed, only the interior is ever written */
+    double *hxmin, *hxmax, *hymin, *hymax, *hzmin, *hzmax;
+    /* a STOP of the reference on the last step: 0 none, 1 CalcH3d's "NO
+     * MATCH CASES FOUND", 2 its "PROBLEM finding MIN and MAX unit vector
+     * components" (main.f:1582-1583, 1641-1644) */
+    int stop_code;
 } or_state;
 
 /* field ids shared with the GPU library's dg_field ids */
@@ -97,6 +104,7 @@ enum {
     OR_F_RHON, OR_F_MUN, OR_F_MUT, OR_F_KTUR, OR_F_EPS,
     OR_F_A, OR_F_B, OR_F_C, OR_F_EI, OR_F_RHS, OR_F_RES,
     OR_F_PPREV,
+    OR_F_HXMIN, OR_F_HXMAX, OR_F_HYMIN, OR_F_HYMAX, OR_F_HZMIN, OR_F_HZMAX,
     OR_NFIELDS
 };
 
@@ -113,6 +121,9 @@ void or_vofmethod(or_state *st, int t);           /* main.f:795-824 */
 void or_kepsmodel(or_state *st, int t);           /* main.f:1888-2026 */
 void or_checktime(or_state *st, int t);           /* main.f:2156-2175 */
 void or_csf(or_state *st);                        /* main.f:2363-2481 */
+void or_calch3d(or_state *st);                    /* main.f:1312-1810 */
+void or_sgip(or_state *st);                       /* main.f:2835-3027 */
+void or_sgip2(or_state *st);                      /* main.f:3029-3166 */
 void or_solvelpres(or_state *st, int t);          /* main.f:3236-3439 */
 void or_calcrho(or_state *st);                    /* main.f:2177-2190 */
 double or_cfl(or_state *st);                      /* main.f:4848-4881 */

@@ -15,6 +15,11 @@ Fixtures are data only: seeded inputs and the reference's outputs.
   run_66x34x34.npz   6 steps on the shipped input.h grid: history and fields
   run_18_upwind_keps.npz  upwind convection + k-epsilon from k0/eps0, 5 steps
   run_66_hist.npz    17 steps at 66^3: per-step history and field checksums
+  run_shipped_sgip.npz  the reference's shipped parameters (66x34x34, hswitch=1
+                     CalcH3d heights, surfacet=5 SGIP with its contact-angle
+                     wall row; initial cc clipped to [0, 1], see below): 4
+                     steps, history, fields and heights; then SGIP2 and CSF
+                     with CalcH3d, 3 steps each: history and checksums
 """
 import os
 import sys
@@ -138,10 +143,62 @@ def runs():
                         **chk)
 
 
+# the values of the reference's shipped inputdata.in (66x34x34, water and a
+# heavier second fluid, inflow 0.5 m/s, no gravity, a droplet centred just
+# below the wall, SGIP surface tension with CalcH3d heights)
+SHIPPED = dict(nx=66, ny=34, nz=34, tmax=400, cycles=2, omega=1.5, npre=5, ntop=4, npost=5, maxcfl=0.1, iturb=0,
+               convectype=2, hswitch=1, xvel1=0.0, yvel1=0.0, zvel1=0.0, xvel2=0.5, yvel2=0.0, zvel2=0.0,
+               rho2=500.0, rho1=1000.0, sigma=0.0728, visc2=0.001, visc1=0.001, grav=0.0, grav1=0.0,
+               length=0.008, radius=0.001, y0=-0.00017365, z0=1000.0, surfacet=5)
+
+
+def shipped_text(**over):
+    kw = dict(SHIPPED)
+    kw.update(over)
+    return inputdata_text(**kw)
+
+
+def clipped_cc(txt):
+    """smooth2's initial volume fraction clipped to [0, 1]: unclipped, 8
+    cells hold 1 + 4e-13, the first VOFmethod turns 100 cells to NaN (AL3D's
+    cube root of a negative number) and CalcH3d STOPs with NO MATCH CASES
+    FOUND at the first step."""
+    from oracle_ffi import Oracle
+    o = Oracle(txt)
+    cc = o.field("cc").clip(0.0, 1.0).copy()
+    o.close()
+    return cc
+
+
+def sgip_runs():
+    from refrun import FIELD_NAMES, HEIGHT_NAMES
+    out = {}
+    txt = shipped_text()
+    cc0 = clipped_cc(txt)
+    r = run_reference(txt, 4, init_fields={"cc": cc0}, want_fields=FIELD_NAMES + HEIGHT_NAMES)
+    out.update(inputdata=np.array(txt), init_cc=cc0, **hist_arrays(r["hist"]))
+    out.update({f"f_{k}": r["fields"][k] for k in ("cc", "u", "v", "w", "p", "s11", "s22", "s33") + HEIGHT_NAMES})
+    for tag, over in (("sgip2", dict(surfacet=6)), ("csfh", dict(surfacet=2))):
+        t2 = shipped_text(**over)
+        r = run_reference(t2, 3, init_fields={"cc": cc0}, want_fields=FIELD_NAMES + HEIGHT_NAMES)
+        out[f"{tag}_inputdata"] = np.array(t2)
+        out.update({f"{tag}_{k}": v for k, v in hist_arrays(r["hist"]).items()})
+        for k, v in r["fields"].items():
+            out[f"{tag}_sum_{k}"] = v[1:-1, 1:-1, 1:-1].sum()
+            out[f"{tag}_sumsq_{k}"] = (v[1:-1, 1:-1, 1:-1] ** 2).sum()
+    np.savez_compressed(os.path.join(HERE, "run_shipped_sgip.npz"), **out)
+
+
 if __name__ == "__main__":
-    plic()
-    kernels()
-    runs()
+    which = sys.argv[1:] or ["plic", "kernels", "runs", "sgip"]
+    if "plic" in which:
+        plic()
+    if "kernels" in which:
+        kernels()
+    if "runs" in which:
+        runs()
+    if "sgip" in which:
+        sgip_runs()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

@@ -18,7 +18,7 @@ ORACLE_SO = os.path.join(REPO, "oracle", "_build", "liboracle.so")
 
 FIELD_IDS = dict(cc=0, u=1, v=2, w=3, p=4, s11=5, s22=6, s33=7, s12=8, s13=9, s23=10,
                  rhon=11, mun=12, mut=13, ktur=14, eps=15, a=16, b=17, c=18, ei=19, rhs=20, res=21,
-                 pprev=22)
+                 pprev=22, hxmin=23, hxmax=24, hymin=25, hymax=26, hzmin=27, hzmax=28)
 
 _lib = None
 
@@ -44,6 +44,8 @@ def lib():
         for nm in ("or_step", "or_vofmethod", "or_kepsmodel", "or_checktime", "or_solvelpres"):
             getattr(L, nm).argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.or_csf.argtypes = [ctypes.c_void_p]
+        for nm in ("or_calch3d", "or_sgip", "or_sgip2"):
+            getattr(L, nm).argtypes = [ctypes.c_void_p]
         L.or_calcrho.argtypes = [ctypes.c_void_p]
         L.or_cfl.argtypes = [ctypes.c_void_p]
         L.or_cfl.restype = ctypes.c_double
@@ -165,6 +167,8 @@ def _state_fields():
                            "eps", "cmu")]
     f += [("mg_mode", i), ("mg_nu1", i), ("mg_nu2", i), ("mg_maxcycles", i), ("mg_guess", i), ("mg_tol", d),
           ("mg_cycles_last", i), ("mg_relres_last", d), ("mg_bnorm_last", d), ("p_prev", p), ("cfl_last", d), ("mg_outer", i), ("mg_prolong", i), ("vof_guard", i), ("mg_omega", d), ("mg_nuc", i), ("ref_grid", i)]
+    f += [(n, p) for n in ("hxmin", "hxmax", "hymin", "hymax", "hzmin", "hzmax")]
+    f += [("stop_code", i)]
     return f
 
 

@@ -24,6 +24,9 @@ REF_DIR = os.path.join(REPO, "oracle", "_ref")
 
 FIELD_NAMES = ("cc", "u", "v", "w", "p", "s11", "s22", "s33", "s12", "s13", "s23",
                "rhon", "mun", "mut", "ktur", "eps")
+# interface heights of CalcH3d, and their order in COMMON /NEWW/ (input.h:90-95)
+HEIGHT_NAMES = ("hxmin", "hxmax", "hymin", "hymax", "hzmin", "hzmax")
+_NEWW_ORDER = ("hxmax", "hxmin", "hymax", "hymin", "hzmax", "hzmin")
 
 
 def ref_lib_path(nx, ny, nz):
@@ -106,7 +109,11 @@ def _child(lib_path, inputdata, nsteps, init_fields, want_fields, conn, every_st
         # /DIFF/ ANGLEDIF,EPSC,EPSURR,tauh (double) then tauh2 (INTEGER)
         diff_i = np.frombuffer((ctypes.c_int * 9).in_dll(lib, "diff_"), dtype=np.int32)
         iturb = int(cyc_i[14])
+        hswitch = int(cyc_i[19])
         surfacet = int(cyc_i[20])
+        neww = np.frombuffer((ctypes.c_double * (6 * n)).in_dll(lib, "neww_"), dtype=np.float64)
+        for q, nm in enumerate(_NEWW_ORDER):
+            arrs[nm] = neww[q * n:(q + 1) * n].reshape(nz, ny, nx)
         for nm, val in (init_fields or {}).items():
             arrs[nm][...] = val
         hist = []
@@ -114,11 +121,17 @@ def _child(lib_path, inputdata, nsteps, init_fields, want_fields, conn, every_st
         for t in range(nsteps):
             ti = ctypes.c_int(t)
             lib.vofmethod_(ctypes.byref(ti))
+            if hswitch == 1:
+                lib.calch3d_(zp(icc[ng - 1]))
             if iturb in (3, 4):
                 lib.kepsmodel_(ctypes.byref(ti), zp(iu[ng - 1]), zp(ires[ng - 1]))
             lib.checktime_(ctypes.byref(ti))
             if surfacet == 2:
                 lib.csf_(zp(icc[ng - 1]), zp(iu[ng - 1]), zp(ires[ng - 1]), zp(irhs[ng - 1]))
+            elif surfacet == 5:
+                lib.sgip_(zp(icc[ng - 1]), zp(iu[ng - 1]), zp(ires[ng - 1]), zp(irhs[ng - 1]))
+            elif surfacet == 6:
+                lib.sgip2_(zp(icc[ng - 1]), zp(iu[ng - 1]), zp(ires[ng - 1]), zp(irhs[ng - 1]))
             else:
                 raise RuntimeError(f"surfacet={surfacet} not on this path")
             lib.solvelpres_(zp(icc[ng - 1]), zp(iu[ng - 1]), zp(ires[ng - 1]), zp(irhs[ng - 1]),

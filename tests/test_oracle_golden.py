@@ -108,3 +108,25 @@ def test_run_66_history_bitexact(oracle):
         assert inner.sum() == z[f"sum_{f}"]
         assert (inner * inner).sum() == z[f"sumsq_{f}"]
         assert np.abs(inner).max() == z[f"maxabs_{f}"]
+
+
+def test_shipped_sgip_calch3d_bitexact(oracle):
+    """The reference's shipped parameters: CalcH3d interface heights
+    (hswitch=1), SGIP surface tension with its contact-angle wall row, 4
+    steps; then SGIP2 and CSF with CalcH3d, 3 steps (checksums).  The
+    initial volume fraction is the clipped one the fixture holds (unclipped,
+    the reference STOPs at step 0, make_golden.py)."""
+    z = load("run_shipped_sgip.npz")
+    o = _check_run(z, 4, ("cc", "u", "v", "w", "p", "s11", "s22", "s33", "hxmin", "hxmax", "hymin", "hymax",
+                          "hzmin", "hzmax"), init=dict(cc=z["init_cc"]))
+    assert o.st.stop_code == 0
+    for tag in ("sgip2", "csfh"):
+        o = Oracle(str(z[f"{tag}_inputdata"]))
+        o.field("cc")[...] = z["init_cc"]
+        hist = o.run(3)
+        for key in ("tau", "time", "ntop", "tswap", "vol"):
+            assert np.array_equal(np.array([h[key] for h in hist]), z[f"{tag}_h_{key}"]), (tag, key)
+        for f in ("cc", "u", "v", "w", "p", "hxmin", "hymax", "hzmin"):
+            inner = o.field(f)[1:-1, 1:-1, 1:-1]
+            assert inner.sum() == z[f"{tag}_sum_{f}"], (tag, f)
+            assert (inner ** 2).sum() == z[f"{tag}_sumsq_{f}"], (tag, f)

@@ -94,3 +94,31 @@ def test_kernels_random_bitexact(shape):
     K.addint(f1, c1, r1b)
     OK.addint(f2, c2, r2b)
     assert np.array_equal(f1, f2) and np.array_equal(r1b, r2b)
+
+
+def test_shipped_sgip_sgip2_calch3d_bitexact():
+    """CalcH3d (main.f:1312-1810), SGIP (main.f:2835-3027, the contact-angle
+    wall row with SIND/COSD included) and SGIP2 (main.f:3029-3166) against
+    the live reference on its shipped parameters, initial cc clipped to
+    [0, 1] on both sides (unclipped the reference STOPs at step 0).  Every
+    field and the six height arrays, bit for bit."""
+    need(66, 34, 34)
+    import sys
+    import os
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    from make_golden import clipped_cc, shipped_text
+    for surfacet, nsteps in ((5, 3), (6, 2)):
+        txt = shipped_text(surfacet=surfacet)
+        cc0 = clipped_cc(txt)
+        ref = refrun.run_reference(txt, nsteps, init_fields={"cc": cc0},
+                                   want_fields=refrun.FIELD_NAMES + refrun.HEIGHT_NAMES)
+        o = Oracle(txt)
+        o.field("cc")[...] = cc0
+        hist = o.run(nsteps)
+        assert o.st.stop_code == 0
+        for hr, ho in zip(ref["hist"], hist):
+            for key in ("tau", "time", "npre", "ntop", "npost", "tswap", "vol"):
+                assert hr[key] == ho[key], (surfacet, hr["t"], key)
+        for f in refrun.FIELD_NAMES + refrun.HEIGHT_NAMES:
+            assert np.array_equal(ref["fields"][f], o.field(f), equal_nan=True), (surfacet, f)
+        o.close()

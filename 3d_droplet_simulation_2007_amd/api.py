@@ -18,7 +18,7 @@ LIBPATH = os.path.join(HERE, "libdroplet_gpu.so")
 
 FIELD_IDS = dict(cc=0, u=1, v=2, w=3, p=4, s11=5, s22=6, s33=7, s12=8, s13=9, s23=10,
                  rhon=11, mun=12, mut=13, ktur=14, eps=15, a=16, b=17, c=18, ei=19, rhs=20, res=21,
-                 pprev=22)
+                 pprev=22, hxmin=23, hxmax=24, hymin=25, hymax=26, hzmin=27, hzmax=28)
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -145,6 +145,15 @@ class DropletGPU:
 
     def csf(self):
         _check(lib().dg_csf(), "dg_csf")
+
+    def calch3d(self):
+        _check(lib().dg_calch3d(), "dg_calch3d")
+
+    def sgip(self):
+        _check(lib().dg_sgip(), "dg_sgip")
+
+    def sgip2(self):
+        _check(lib().dg_sgip2(), "dg_sgip2")
 
     def solvelpres(self, t):
         _check(lib().dg_solvelpres(_i(t)), "dg_solvelpres")

@@ -143,6 +143,18 @@ static int alloc_fields(Ctx &c)
     return 0;
 }
 
+// CalcH3d's six height arrays, zeroed (the reference's COMMON starts at 0
+// and its ghost cells are never written)
+int alloc_heights(Ctx &c)
+{
+    if (c.hxmin) return 0;
+    const size_t n = c.g.n();
+    double **arrs[] = {&c.hxmin, &c.hxmax, &c.hymin, &c.hymax, &c.hzmin, &c.hzmax};
+    for (double **p : arrs)
+        if (dalloc(p, n)) return -1;
+    return 0;
+}
+
 static void free_ctx(Ctx &c)
 {
     for (int l = 0; l < c.ng; l++) {
@@ -155,7 +167,8 @@ static void free_ctx(Ctx &c)
         L = Level();
     }
     void *fs[] = {c.u, c.v, c.w, c.s11, c.s22, c.s33, c.s12, c.s13, c.s23, c.rhon, c.mun, c.mut, c.ktur,
-                  c.eps, c.cmu, c.ccf, c.pprev, c.pcg_r, c.pcg_p, c.pcg_q, c.red_partial, c.dscal, c.coarse_M};
+                  c.eps, c.cmu, c.ccf, c.pprev, c.pcg_r, c.pcg_p, c.pcg_q, c.red_partial, c.dscal, c.coarse_M,
+                  c.hxmin, c.hxmax, c.hymin, c.hymax, c.hzmin, c.hzmax};
     for (void *p : fs)
         if (p) (void)hipFree(p);
     for (int q = 0; q < 2; q++)
@@ -288,6 +301,12 @@ static double *field_ptr(Ctx &c, int id)
     case DG_F_RHS: return L.rhs;
     case DG_F_RES: return L.res;
     case DG_F_PPREV: return c.pprev;
+    case DG_F_HXMIN: return c.hxmin;
+    case DG_F_HXMAX: return c.hxmax;
+    case DG_F_HYMIN: return c.hymin;
+    case DG_F_HYMAX: return c.hymax;
+    case DG_F_HZMIN: return c.hzmin;
+    case DG_F_HZMAX: return c.hzmax;
     default: return nullptr;
     }
 }
@@ -420,12 +439,14 @@ int init_from_input(const char *path, Input *in_out)
         set_error("restart=%d: restart files (iniget) are not on this path", in.restart);
         return -1;
     }
-    if (in.hswitch != 0) {
-        set_error("hswitch=%d: interface heights (CalcH3d) are not on this path", in.hswitch);
+    if (in.hswitch != 0 && in.hswitch != 1) {
+        set_error("hswitch=%d: 0 (off) or 1 (CalcH3d interface heights)", in.hswitch);
         return -1;
     }
-    if (in.surfacet != 2) {
-        set_error("surfacet=%d: only CSF (surfacet=2) is on this path", in.surfacet);
+    if (in.surfacet != 2 && in.surfacet != 5 && in.surfacet != 6) {
+        set_error("surfacet=%d: CSF (2), SGIP (5) and SGIP2 (6) are on this path; CSS, CSF-BKZ and Meier "
+                  "(1, 3, 4) are not",
+                  in.surfacet);
         return -1;
     }
     if (in.iturb != 0 && in.iturb != 3) {
@@ -437,6 +458,7 @@ int init_from_input(const char *path, Input *in_out)
     resident_scalars_changed();
     // cmu = 0.09 (main.f:347-353); mut, turbst and the Reynolds stresses start at 0
     if (mg_fill(c.g.n(), c.stream, c.cmu, 0.09)) return -1;
+    if ((in.hswitch == 1 || in.surfacet == 5 || in.surfacet == 6) && alloc_heights(c)) return -1;
     if (op_smooth2(c) || op_bc_c(c, c.lev[c.ng - 1].cc) || op_bc_v(c) || op_calcrho(c)) return -1;
     DG_HIP(hipStreamSynchronize(c.stream));
     if (in_out) *in_out = in;
@@ -467,6 +489,21 @@ int dg_csf(void)
     if (require_ready()) return -1;
     return op_csf(*ctx());
 }
+int dg_calch3d(void)
+{
+    if (require_ready()) return -1;
+    return op_calch3d(*ctx());
+}
+int dg_sgip(void)
+{
+    if (require_ready()) return -1;
+    return op_sgip(*ctx());
+}
+int dg_sgip2(void)
+{
+    if (require_ready()) return -1;
+    return op_sgip2(*ctx());
+}
 int dg_solvelpres(const int *t)
 {
     if (require_ready()) return -1;
@@ -493,9 +530,12 @@ int dg_step(const int *t)
     if (require_ready()) return -1;
     Ctx &c = *ctx();
     if (op_vofmethod(c, *t, true)) return -1;
+    if (c.s.hswitch == 1 && op_calch3d(c)) return -1;  // main.f:30
     if (c.s.iturb == 3 && op_kepsmodel(c, *t)) return -1;
     if (op_checktime(c, *t)) return -1;
     if (c.s.surfacet == 2 && op_csf(c, false)) return -1;
+    if (c.s.surfacet == 5 && op_sgip(c, false)) return -1;
+    if (c.s.surfacet == 6 && op_sgip2(c, false)) return -1;
     if (op_solvelpres(c, *t)) return -1;
     return 0;
 }

@@ -53,6 +53,9 @@ struct Ctx {
     double *s11 = nullptr, *s22 = nullptr, *s33 = nullptr, *s12 = nullptr, *s13 = nullptr, *s23 = nullptr;
     double *rhon = nullptr, *mun = nullptr, *mut = nullptr, *ktur = nullptr, *eps = nullptr, *cmu = nullptr;
     double *ccf = nullptr;     // csf's filtered volume fraction (automatic array in csf)
+    // CalcH3d's interface heights (/NEWW/, input.h:90-95); allocated with
+    // hswitch = 1 or SGIP/SGIP2 surface tension, only the interior written
+    double *hxmin = nullptr, *hxmax = nullptr, *hymin = nullptr, *hymax = nullptr, *hzmin = nullptr, *hzmax = nullptr;
     double *pprev = nullptr;   // pressure of the previous step (tolerance-mode guess)
     double *pcg_r = nullptr, *pcg_p = nullptr, *pcg_q = nullptr;  // pcg_p: the ring of search directions
     double *red_partial = nullptr;  // RED_BLOCKS doubles
@@ -107,6 +110,9 @@ int op_kepsmodel(Ctx &c, int t);
 int op_cfl(Ctx &c, double *out_host);
 int op_checktime(Ctx &c, int t);
 int op_csf(Ctx &c, bool w_interior = true);
+int op_calch3d(Ctx &c);                           // CalcH3d main.f:1312-1810
+int op_sgip(Ctx &c, bool w_interior = true);      // SGIP main.f:2835-3027
+int op_sgip2(Ctx &c, bool w_interior = true);     // SGIP2 main.f:3029-3166
 int op_solvelpres(Ctx &c, int t);
 int op_smooth2(Ctx &c);
 

@@ -58,6 +58,7 @@ extern char cycling_[] __attribute__((weak));
 extern char times_[] __attribute__((weak));
 extern char dinem_[] __attribute__((weak));
 extern char physics_[] __attribute__((weak));
+extern char neww_[] __attribute__((weak));  // CalcH3d heights (input.h:90-95), optional
 }
 
 namespace dg {
@@ -77,6 +78,7 @@ struct HostCommon {
     int *ptrs = nullptr;  // icc(NGM), iu, ires, irhs, ia, ib, ic, iei, ng
     double *surf = nullptr, *turb = nullptr, *phys = nullptr, *diffd = nullptr, *cycd = nullptr, *dinem = nullptr;
     int *dim = nullptr, *tauh2 = nullptr, *cyci = nullptr, *times = nullptr;
+    double *neww = nullptr;  // /NEWW/ hxmax, hxmin, hymax, hymin, hzmax, hzmin (null when absent)
 
     // /pointers/ entry of kind q (P_*) at Fortran level j (1-based)
     int &ptr(int q, int j) const { return ptrs[q * NGM + (j - 1)]; }
@@ -182,6 +184,7 @@ int bind_common(HostCommon &h)
     h.cyci = (int *)(cycling_ + 2 * sizeof(double));
     h.times = (int *)times_;
     h.dinem = (double *)dinem_;
+    h.neww = neww_ ? (double *)neww_ : nullptr;
     h.bound = true;
     return 0;
 }
@@ -244,6 +247,13 @@ double *host_field(int id)
     case DG_F_EI: return h.slot(P_EI, ng);
     case DG_F_RHS: return h.slot(P_RHS, ng);
     case DG_F_RES: return h.slot(P_RES, ng);
+    // /NEWW/ holds hxmax, hxmin, hymax, hymin, hzmax, hzmin in that order
+    case DG_F_HXMIN: return h.neww ? h.neww + 1 * h.N : nullptr;
+    case DG_F_HXMAX: return h.neww ? h.neww + 0 * h.N : nullptr;
+    case DG_F_HYMIN: return h.neww ? h.neww + 3 * h.N : nullptr;
+    case DG_F_HYMAX: return h.neww ? h.neww + 2 * h.N : nullptr;
+    case DG_F_HZMIN: return h.neww ? h.neww + 5 * h.N : nullptr;
+    case DG_F_HZMAX: return h.neww ? h.neww + 4 * h.N : nullptr;
     default: return nullptr;
     }
 }
@@ -432,6 +442,8 @@ int attach()
     }
     read_all_scalars();
     resident_scalars_changed();
+    if ((ctx()->s.hswitch == 1 || ctx()->s.surfacet == 5 || ctx()->s.surfacet == 6) && alloc_heights(*ctx()))
+        return -1;
     if (copy_heap(true) || copy_fields(ALL_FIELDS, true)) return -1;
     g_attached = true;
     return 0;
@@ -598,6 +610,39 @@ void solvelpres_(double *cc, double *w1, double *w2, double *w3, double *a, doub
             return -1;
         return op_solvelpres(cx, *t);
     });
+}
+
+// CalcH3d(cc), main.f:1312-1810 (called with z(icc(ng)), main.f:30)
+void calch3d_(double *cc)
+{
+    call_form("CalcH3d", [&](Ctx &c) {
+        if (expect_slot("CalcH3d", 1, cc, finest().cc, "z(icc(ng))")) return -1;
+        if (alloc_heights(c)) return -1;
+        return op_calch3d(c);
+    });
+}
+
+// SGIP(ccc, w11, w22, w33), main.f:2835-3027, and SGIP2, main.f:3029-3166
+// (called as csf, main.f:59-62)
+static void sgip_form(const char *who, double *ccc, double *w11, double *w22, double *w33,
+                      int (*op)(Ctx &, bool))
+{
+    call_form(who, [&](Ctx &c) {
+        Level &L = finest();
+        if (expect_slot(who, 1, ccc, L.cc, "z(icc(ng))") || expect_slot(who, 2, w11, L.u, "z(iu(ng))") ||
+            expect_slot(who, 3, w22, L.res, "z(ires(ng))") || expect_slot(who, 4, w33, L.rhs, "z(irhs(ng))"))
+            return -1;
+        if (alloc_heights(c)) return -1;
+        return op(c, true);
+    });
+}
+void sgip_(double *ccc, double *w11, double *w22, double *w33)
+{
+    sgip_form("SGIP", ccc, w11, w22, w33, op_sgip);
+}
+void sgip2_(double *ccc, double *w11, double *w22, double *w33)
+{
+    sgip_form("SGIP2", ccc, w11, w22, w33, op_sgip2);
 }
 
 // calcrho(cc), main.f:2177-2190

@@ -28,6 +28,8 @@ void nondim(const Input &in, Scalars &s);
 int init_from_input(const char *path, Input *in_out);
 // a fresh resident state of nx x ny x nz with every field zero
 int resident_setup(int nx, int ny, int nz);
+// CalcH3d's six interface-height arrays (zeroed), once
+int alloc_heights(Ctx &c);
 // recomputes what depends on the physical scalars (after they are set)
 void resident_scalars_changed();
 bool resident_ready();

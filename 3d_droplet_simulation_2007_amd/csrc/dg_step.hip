@@ -1019,6 +1019,370 @@ int op_csf(Ctx &c, bool w_interior)
 }
 
 // ---------------------------------------------------------------------------
+// Interface heights, CalcH3d (main.f:1312-1810).  Per interior cell: the
+// Youngs normal of the 27-point stencil, the plane cut of the unit cube
+// holding min(c, 1-c) classified by the sorted normal components (the
+// reference's cases: two-dimensional, triangle, quadrilateral A, pentagon
+// and hexagon by Newton iteration, quadrilateral B), the six face heights
+// mirrored for c > 0.5, assigned to the x/y/z faces by which component is
+// the smallest and the middle one (the y heights of the last branch follow
+// niz's sign, as in main.f:1691-1719), swapped for c <= 0.5 and clipped to
+// [0, 1].  The reference STOPs when no case matches (main.f:1582-1583) or
+// the middle component is not found (main.f:1641-1644): *stop takes 1 or 2.
+
+__device__ inline void hset(bool negative, double a, double b, double &lo, double &hi)
+{
+    if (negative) { lo = a; hi = b; }
+    else { lo = b; hi = a; }
+}
+
+__device__ inline double cube(double a) { return a * a * a; }
+
+int op_calch3d(Ctx &c)
+{
+    const Grid g = c.g;
+    if (!c.hxmin) {
+        set_error("CalcH3d: the interface-height arrays are not allocated (hswitch=0 and no SGIP)");
+        return -1;
+    }
+    double *hxl = c.hxmin, *hxh = c.hxmax, *hyl = c.hymin, *hyh = c.hymax, *hzl = c.hzmin, *hzh = c.hzmax;
+    double *stop = c.dscal + 12;
+    if (mg_fill(1, c.stream, stop, 0.0)) return -1;
+    if (launch_column27(g, c.stream, c.lev[c.ng - 1].cc, [=] __device__(const double (&P)[3][9], int o) {
+#define C DG_P27
+            double mm1 = C(-1, -1, -1) + C(-1, -1, 1) + C(-1, 1, -1) + C(-1, 1, 1) +
+                         2.0 * (C(-1, -1, 0) + C(-1, 1, 0) + C(-1, 0, -1) + C(-1, 0, 1)) + 4.0 * C(-1, 0, 0);
+            double mm2 = C(1, -1, -1) + C(1, -1, 1) + C(1, 1, -1) + C(1, 1, 1) +
+                         2.0 * (C(1, -1, 0) + C(1, 1, 0) + C(1, 0, -1) + C(1, 0, 1)) + 4.0 * C(1, 0, 0);
+            const double normx = mm1 - mm2;
+            mm1 = C(-1, -1, -1) + C(-1, -1, 1) + C(1, -1, -1) + C(1, -1, 1) +
+                  2.0 * (C(-1, -1, 0) + C(1, -1, 0) + C(0, -1, -1) + C(0, -1, 1)) + 4.0 * C(0, -1, 0);
+            mm2 = C(-1, 1, -1) + C(-1, 1, 1) + C(1, 1, -1) + C(1, 1, 1) +
+                  2.0 * (C(-1, 1, 0) + C(1, 1, 0) + C(0, 1, -1) + C(0, 1, 1)) + 4.0 * C(0, 1, 0);
+            const double normy = mm1 - mm2;
+            mm1 = C(-1, -1, -1) + C(-1, 1, -1) + C(1, -1, -1) + C(1, 1, -1) +
+                  2.0 * (C(-1, 0, -1) + C(1, 0, -1) + C(0, -1, -1) + C(0, 1, -1)) + 4.0 * C(0, 0, -1);
+            mm2 = C(-1, -1, 1) + C(-1, 1, 1) + C(1, -1, 1) + C(1, 1, 1) +
+                  2.0 * (C(-1, 0, 1) + C(1, 0, 1) + C(0, -1, 1) + C(0, 1, 1)) + 4.0 * C(0, 0, 1);
+            const double normz = mm1 - mm2;
+            const double c0 = C(0, 0, 0);
+#undef C
+            const double nn = sqrt(normx * normx + normy * normy + normz * normz + 1.0e-50);
+            const double nix = normx / nn, niy = normy / nn, niz = normz / nn;
+            const double ccnew = c0 > 0.5 ? 1.0 - c0 : c0;
+            const double absxn = fabs(nix), absyn = fabs(niy), abszn = fabs(niz);
+            const double n1 = dmin1(dmin1(absxn, absyn), abszn);
+            const double n3 = dmax1(dmax1(absxn, absyn), abszn);
+            const double n2 = absxn + absyn + abszn - n1 - n3;
+            const double clim1 = sqrt(2.0 * ccnew * n3 * n2);
+            double h1a, h1b, h2a, h2b, h3a, h3b, diag, s1, s2, s3;
+            if (n1 == 0.0 && n2 > 0.0 && ccnew <= clim1) {
+                s2 = sqrt(2.0 * ccnew * n3 / n2);
+                s3 = sqrt(2.0 * ccnew * n2 / n3);
+                h1a = s2 * s3 / 2.0; h1b = s2 * s3 / 2.0;
+                h2a = s3; h2b = 0.0; h3a = s2; h3b = 0.0;
+            } else if (n1 == 0.0 && n2 > 0.0 && ccnew > clim1) {
+                h1a = c0; h1b = c0;
+                h2a = c0 + n2 / (2.0 * n3); h2b = c0 - n2 / (2.0 * n3);
+                h3a = 1.0; h3b = 0.0;
+            } else if (n1 == 0.0 && n2 == 0.0) {
+                h1a = ccnew; h1b = ccnew; h2a = ccnew; h2b = ccnew; h3a = 1.0; h3b = 0.0;
+            } else if (6.0 * n2 * n3 * ccnew < n1 * n1) {
+                diag = pow(6.0 * n1 * n2 * n3 * ccnew, 1.0 / 3.0);
+                s1 = diag / n1; s2 = diag / n2; s3 = diag / n3;
+                h1a = (s2 * s3) / 2.0; h1b = 0.0; h2a = (s1 * s3) / 2.0; h2b = 0.0; h3a = (s1 * s2) / 2.0; h3b = 0.0;
+            } else if (6.0 * n2 * n3 * ccnew < 3.0 * (n2 * n2) - 3.0 * n1 * n2 + n1 * n1) {
+                diag = (3.0 * n1 + sqrt(72.0 * n2 * n3 * ccnew - 3.0 * (n1 * n1))) / 6.0;
+                s1 = diag / n1; s2 = diag / n2; s3 = diag / n3;
+                h1a = (s2 * s3) / 2.0;
+                h1b = (s2 * s3) / 2.0 * sq((s1 - 1.0) / s1);
+                h2a = s3 * (2.0 * s1 - 1.0) / (2.0 * s1); h2b = 0.0;
+                h3a = s2 * (2.0 * s1 - 1.0) / (2.0 * s1); h3b = 0.0;
+            } else if ((n1 + n2 >= n3 && 6.0 * n1 * n2 * n3 * ccnew < n3 * n3 * n3 - cube(n3 - n1) - cube(n3 - n2)) ||
+                       (n1 + n2 < n3 && 2.0 * n3 * ccnew < n1 + n2)) {
+                const double rsixn3 = 1.0 / (6.0 * n1 * n2 * n3);
+                diag = n2;
+                for (int loop = 1; loop <= 1000; loop++) {
+                    const double delta = -(rsixn3 * (cube(diag) - cube(diag - n1) - cube(diag - n2)) - ccnew) /
+                                         (3.0 * rsixn3 * (diag * diag - sq(diag - n1) - sq(diag - n2)));
+                    diag = diag + delta;
+                    if (fabs(delta) < 1.0e-6) break;
+                }
+                s1 = diag / n1; s2 = diag / n2; s3 = diag / n3;
+                h1a = s3 * (2.0 * s2 - 1.0) / (2.0 * s2);
+                h1b = (s2 * s3) / 2.0 * sq((s1 - 1.0) / s1);
+                h2a = s3 * (2.0 * s1 - 1.0) / (2.0 * s1);
+                h2b = (s1 * s3) / 2.0 * sq((s2 - 1.0) / s2);
+                h3a = 2.0 * (s1 + s2) - s1 / s2 - s2 / s1 - s1 * s2 - 1.0; h3b = 0.0;
+            } else if (n1 + n2 > n3 && 6.0 * n1 * n2 * n3 * ccnew >= n3 * n3 * n3 - cube(n3 - n1) - cube(n3 - n2)) {
+                const double rsixn3 = 1.0 / (6.0 * n1 * n2 * n3);
+                diag = n3;
+                for (int loop = 1; loop <= 1000; loop++) {
+                    const double delta =
+                        -(rsixn3 * (cube(diag) - cube(diag - n1) - cube(diag - n2) - cube(diag - n3)) - ccnew) /
+                        (3.0 * rsixn3 * (diag * diag - sq(diag - n1) - sq(diag - n2) - sq(diag - n3)));
+                    diag = diag + delta;
+                    if (fabs(delta) < 1.0e-6) break;
+                }
+                s1 = diag / n1; s2 = diag / n2; s3 = diag / n3;
+                h1a = 2.0 * (s2 + s3) - s2 / s3 - s3 / s2 - s2 * s3 - 1.0;
+                h1b = (s2 * s3) / 2.0 * sq((s1 - 1.0) / s1);
+                h2a = 2.0 * (s1 + s3) - s1 / s3 - s3 / s1 - s1 * s3 - 1.0;
+                h2b = (s1 * s3) / 2.0 * sq((s2 - 1.0) / s2);
+                h3a = 2.0 * (s1 + s2) - s1 / s2 - s2 / s1 - s1 * s2 - 1.0;
+                h3b = (s1 * s2) / 2.0 * sq((s3 - 1.0) / s3);
+            } else if (n1 + n2 < n3 && 6.0 * n1 * n2 * n3 * ccnew >= cube(n1 + n2) - n1 * n1 * n1 - n2 * n2 * n2) {
+                diag = 0.5 * (n1 + n2) + ccnew * n3;
+                s1 = diag / n1; s2 = diag / n2; s3 = diag / n3;
+                h1a = s3 * (2.0 * s2 - 1.0) / (2.0 * s2);
+                h1b = s3 * (2.0 * s2 - 1.0) * (s1 - 1.0) / (2.0 * s1 * s2);
+                h2a = s3 * (2.0 * s1 - 1.0) / (2.0 * s1);
+                h2b = s3 * (2.0 * s1 - 1.0) * (s2 - 1.0) / (2.0 * s1 * s2);
+                h3a = 1.0; h3b = 0.0;
+            } else {
+                *stop = 1.0;  // NO MATCH CASES FOUND
+                return;
+            }
+            if (c0 > 0.5) {
+                h1a = 1.0 - h1a; h1b = 1.0 - h1b; h2a = 1.0 - h2a;
+                h2b = 1.0 - h2b; h3a = 1.0 - h3a; h3b = 1.0 - h3b;
+            }
+            double xl, xh, yl, yh, zl, zh;
+            if (absxn == n1) {
+                hset(nix < 0, h1a, h1b, xl, xh);
+                if (fabs(absyn - n2) < 1.0e-6) {
+                    hset(niy < 0, h2a, h2b, yl, yh);
+                    hset(niz < 0, h3a, h3b, zl, zh);
+                } else if (fabs(abszn - n2) < 1.0e-6) {
+                    hset(niy < 0, h3a, h3b, yl, yh);
+                    hset(niz < 0, h2a, h2b, zl, zh);
+                } else {
+                    *stop = 2.0;
+                    return;
+                }
+            } else if (absyn == n1) {
+                hset(niy < 0, h1a, h1b, yl, yh);
+                if (fabs(absxn - n2) < 1.0e-6) {
+                    hset(nix < 0, h2a, h2b, xl, xh);
+                    hset(niz < 0, h3a, h3b, zl, zh);
+                } else if (fabs(abszn - n2) < 1.0e-6) {
+                    hset(nix < 0, h3a, h3b, xl, xh);
+                    hset(niz < 0, h2a, h2b, zl, zh);
+                } else {
+                    *stop = 2.0;
+                    return;
+                }
+            } else if (abszn == n1) {
+                hset(niz < 0, h1a, h1b, zl, zh);
+                if (fabs(absxn - n2) < 1.0e-6) {
+                    hset(nix < 0, h2a, h2b, xl, xh);
+                    hset(niz < 0, h3a, h3b, yl, yh);
+                } else if (fabs(absyn - n2) < 1.0e-6) {
+                    hset(nix < 0, h3a, h3b, xl, xh);
+                    hset(niz < 0, h2a, h2b, yl, yh);
+                } else {
+                    *stop = 2.0;
+                    return;
+                }
+            } else {
+                // a NaN normal: the reference assigns nothing here
+                xl = hxl[o]; xh = hxh[o]; yl = hyl[o]; yh = hyh[o]; zl = hzl[o]; zh = hzh[o];
+            }
+            if (c0 <= 0.5) {
+                double t;
+                t = xl; xl = xh; xh = t;
+                t = yl; yl = yh; yh = t;
+                t = zl; zl = zh; zh = t;
+            }
+            auto clip = [](double v) {
+                if (v < 0.0) v = 0.0;
+                if (v > 1.0) v = 1.0;
+                return v;
+            };
+            hxl[o] = clip(xl); hxh[o] = clip(xh); hyl[o] = clip(yl);
+            hyh[o] = clip(yh); hzl[o] = clip(zl); hzh[o] = clip(zh);
+        }))
+        return -1;
+    DG_HIP(hipMemcpyAsync(c.hscal + 12, stop, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    DG_HIP(hipStreamSynchronize(c.stream));
+    if (c.hscal[12] == 1.0) {
+        set_error("STOP: NO MATCH CASES FOUND (CalcH3d, main.f:1582)");
+        return -1;
+    }
+    if (c.hscal[12] != 0.0) {
+        set_error("STOP: PROBLEM finding MIN and MAX unit vector components (CalcH3d, main.f:1641)");
+        return -1;
+    }
+    return 0;
+}
+
+// SGIP / SGIP2 (main.f:2835-3166): the filter passes exactly as the
+// reference (ccf = ccc; nf times: w22 = filterc27(ccf), ccf = w22 over the
+// whole array, bc_out(ccf)), then the curvature into w11 with the
+// tangential sums of the x-down, y-up and z-down faces in SGIP's own order
+// (main.f:2878-2915; csf adds them the other way round), bc_out(w11).
+static int sgip_front(Ctx &c, int nf)
+{
+    const Grid g = c.g;
+    Level &L = c.lev[c.ng - 1];
+    double *ccc = L.cc, *w11 = L.u, *w22 = L.res, *ccf = c.ccf;
+    const size_t nb = sizeof(double) * g.n();
+    hipStream_t s = c.stream;
+    DG_HIP(hipMemcpyAsync(ccf, ccc, nb, hipMemcpyDeviceToDevice, s));
+    for (int q = 0; q < nf; q++) {
+        if (mg_filterc27(g, s, ccf, w22)) return -1;
+        DG_HIP(hipMemcpyAsync(ccf, w22, nb, hipMemcpyDeviceToDevice, s));
+        if (op_bc_out(g, s, ccf)) return -1;
+    }
+    if (c.s.sigma == 0.0) return 0;
+    if (launch_column27(g, s, ccf, [=] __device__(const double (&P)[3][9], int o) {
+            const double e50 = 1.0e-50;
+#define F DG_P27
+            double nxu = sqrt(sq(F(1, 0, 0) - F(0, 0, 0)) + 0.0625 * sq(F(0, 1, 0) - F(0, -1, 0) + F(1, 1, 0) - F(1, -1, 0)) +
+                              0.0625 * sq(F(0, 0, 1) - F(0, 0, -1) + F(1, 0, 1) - F(1, 0, -1)) + e50);
+            double nxd = sqrt(sq(F(0, 0, 0) - F(-1, 0, 0)) + 0.0625 * sq(F(-1, 1, 0) - F(-1, -1, 0) + F(0, 1, 0) - F(0, -1, 0)) +
+                              0.0625 * sq(F(-1, 0, 1) - F(-1, 0, -1) + F(0, 0, 1) - F(0, 0, -1)) + e50);
+            double nyu = sqrt(sq(F(0, 1, 0) - F(0, 0, 0)) + 0.0625 * sq(F(1, 1, 0) - F(-1, 1, 0) + F(1, 0, 0) - F(-1, 0, 0)) +
+                              0.0625 * sq(F(0, 1, 1) - F(0, 1, -1) + F(0, 0, 1) - F(0, 0, -1)) + e50);
+            double nyd = sqrt(sq(F(0, 0, 0) - F(0, -1, 0)) + 0.0625 * sq(F(1, 0, 0) - F(-1, 0, 0) + F(1, -1, 0) - F(-1, -1, 0)) +
+                              0.0625 * sq(F(0, 0, 1) - F(0, 0, -1) + F(0, -1, 1) - F(0, -1, -1)) + e50);
+            double nzu = sqrt(sq(F(0, 0, 1) - F(0, 0, 0)) + 0.0625 * sq(F(1, 0, 0) - F(-1, 0, 0) + F(1, 0, 1) - F(-1, 0, 1)) +
+                              0.0625 * sq(F(0, 1, 0) - F(0, -1, 0) + F(0, 1, 1) - F(0, -1, 1)) + e50);
+            double nzd = sqrt(sq(F(0, 0, 0) - F(0, 0, -1)) + 0.0625 * sq(F(1, 0, -1) - F(-1, 0, -1) + F(1, 0, 0) - F(-1, 0, 0)) +
+                              0.0625 * sq(F(0, 1, -1) - F(0, -1, -1) + F(0, 1, 0) - F(0, -1, 0)) + e50);
+            double d1x = (F(1, 0, 0) - F(0, 0, 0)) / nxu;
+            double d2x = (F(0, 0, 0) - F(-1, 0, 0)) / nxd;
+            double d1y = (F(0, 1, 0) - F(0, 0, 0)) / nyu;
+            double d2y = (F(0, 0, 0) - F(0, -1, 0)) / nyd;
+            double d1z = (F(0, 0, 1) - F(0, 0, 0)) / nzu;
+            double d2z = (F(0, 0, 0) - F(0, 0, -1)) / nzd;
+#undef F
+            w11[o] = -(d1x - d2x + d1y - d2y + d1z - d2z);
+        }))
+        return -1;
+    return op_bc_out(g, s, w11);
+}
+
+// sigma = 0: the six stresses are zeroed and nothing is copied (the w's keep
+// what they hold: w22 the last filter pass)
+static int zero_stresses(Ctx &c)
+{
+    double *arr[6] = {c.s11, c.s22, c.s33, c.s12, c.s13, c.s23};
+    for (double *a : arr)
+        if (mg_fill(c.g.n(), c.stream, a, 0.0)) return -1;
+    return 0;
+}
+
+// w11, w22, w33 = s11, s22, s33 over the whole array (main.f:3017-3019,
+// 3156-3158), or only their ghost shells when the caller rewrites the
+// interiors before reading them (the time step's predictor)
+static int copy_w_from_s(Ctx &c, bool w_interior)
+{
+    Level &L = c.lev[c.ng - 1];
+    double *dst[3] = {L.u, L.res, L.rhs};
+    const double *src[3] = {c.s11, c.s22, c.s33};
+    if (w_interior) {
+        const size_t nb = sizeof(double) * c.g.n();
+        for (int q = 0; q < 3; q++) DG_HIP(hipMemcpyAsync(dst[q], src[q], nb, hipMemcpyDeviceToDevice, c.stream));
+        return 0;
+    }
+    return copy_shell(c.g, c.stream, dst, src, 3);
+}
+
+// SIND / COSD of the reference (the Fortran 2023 intrinsics its build maps
+// dsind / dcosd to): sine and cosine of the argument times pi/180
+__device__ inline double sind_deg(double x) { return sin(x * (3.14159265358979323846 / 180.0)); }
+__device__ inline double cosd_deg(double x) { return cos(x * (3.14159265358979323846 / 180.0)); }
+
+// SGIP (main.f:2835-3027): S_dd = sigh (w11 + w11_down)(ccf_up - ccf_down)
+// |L - L_down| / 2 with L the mean CalcH3d height along d; on j = 2 the
+// contact-angle form, theta from the contact-line velocity (115 or 45
+// degrees beyond |v_n| > 0.1, else 350 v_n + 80); bc_out of S11..S33; no
+// bc_tensor.
+int op_sgip(Ctx &c, bool w_interior)
+{
+    if (!c.hxmin) {
+        set_error("SGIP: the interface-height arrays are not allocated");
+        return -1;
+    }
+    if (sgip_front(c, 1)) return -1;
+    if (c.s.sigma == 0.0) return zero_stresses(c);
+    const Grid g = c.g;
+    Level &L = c.lev[c.ng - 1];
+    const double *w11 = L.u, *ccf = c.ccf, *u = c.u, *w = c.w;
+    const double *hxl = c.hxmin, *hxh = c.hxmax, *hyl = c.hymin, *hyh = c.hymax, *hzl = c.hzmin, *hzh = c.hzmax;
+    double *s11 = c.s11, *s22 = c.s22, *s33 = c.s33;
+    const double sigh = 0.5 * (c.s.tau * c.s.tau) * c.s.sigma / (c.s.h * c.s.h * c.s.h);
+    const int sy = g.sy, sz = g.sz;
+    if (launch_interior(g, c.stream, [=] __device__(int i, int j, int k) {
+            const int o = g.at(i, j, k);
+            const double Lx = (hxh[o] + hxl[o]) / 2.0, Lxd = (hxh[o - 1] + hxl[o - 1]) / 2.0;
+            const double Ly = (hyh[o] + hyl[o]) / 2.0, Lyd = (hyh[o - sy] + hyl[o - sy]) / 2.0;
+            const double Lz = (hzh[o] + hzl[o]) / 2.0, Lzd = (hzh[o - sz] + hzl[o - sz]) / 2.0;
+            const double wv = w11[o];
+            double v11 = sigh * (wv + w11[o - 1]) * (ccf[o + 1] - ccf[o - 1]) * fabs(Lx - Lxd) / 2.0;
+            double v22 = sigh * (wv + w11[o - sy]) * (ccf[o + sy] - ccf[o - sy]) * fabs(Ly - Lyd) / 2.0;
+            double v33 = sigh * (wv + w11[o - sz]) * (ccf[o + sz] - ccf[o - sz]) * fabs(Lz - Lzd) / 2.0;
+            if (j == 2) {
+                const double mx = ccf[o + 1] - ccf[o - 1], mz = ccf[o + sz] - ccf[o - sz];
+                const double mmx = mx / sqrt(mz * mz + mx * mx + 1.0e-50);
+                const double mmz = mz / sqrt(mz * mz + mx * mx + 1.0e-50);
+                double normv = mmx * u[o] + mmz * w[o];
+                normv = -normv;
+                double theta = 0.0;
+                if (fabs(normv) > DG_F4(0.1)) {
+                    if (normv > 0) theta = 115.0;
+                    if (normv < 0) theta = 45.0;
+                } else {
+                    theta = DG_F4(350.0) * normv + 80.0;
+                }
+                const double my = ccf[o + sy] - ccf[o - sy];
+                const double mmy = cosd_deg(theta);
+                v11 = sigh * (wv + w11[o - 1]) * mx * sind_deg(theta) / 2.0;
+                v22 = sigh * (wv + w11[o - sy]) * mmy * sqrt(mz * mz + mx * mx + my * my) * -1.0;
+                v33 = sigh * (wv + w11[o - sz]) * mz * sind_deg(theta) / 2.0;
+            }
+            s11[o] = v11;
+            s22[o] = v22;
+            s33[o] = v33;
+        }))
+        return -1;
+    if (op_bc_out(g, c.stream, s11) || op_bc_out(g, c.stream, s22) || op_bc_out(g, c.stream, s33)) return -1;
+    return copy_w_from_s(c, w_interior);
+}
+
+// SGIP2 (main.f:3029-3166): three filter passes, sigh = 0.5 tau sigma / h^2,
+// S_dd = sigh (w11_up + w11_down)(ccf_up - ccf_down)(L_up - L_down) / 4; no
+// bc_out of S, no bc_tensor.
+int op_sgip2(Ctx &c, bool w_interior)
+{
+    if (!c.hxmin) {
+        set_error("SGIP2: the interface-height arrays are not allocated");
+        return -1;
+    }
+    if (sgip_front(c, 3)) return -1;
+    if (c.s.sigma == 0.0) return zero_stresses(c);
+    const Grid g = c.g;
+    Level &L = c.lev[c.ng - 1];
+    const double *w11 = L.u, *ccf = c.ccf;
+    const double *hxl = c.hxmin, *hxh = c.hxmax, *hyl = c.hymin, *hyh = c.hymax, *hzl = c.hzmin, *hzh = c.hzmax;
+    double *s11 = c.s11, *s22 = c.s22, *s33 = c.s33;
+    const double sigh = 0.5 * c.s.tau * c.s.sigma / (c.s.h * c.s.h);
+    const int sy = g.sy, sz = g.sz;
+    if (launch_interior(g, c.stream, [=] __device__(int i, int j, int k) {
+            const int o = g.at(i, j, k);
+            const double Lxu = (hxh[o + 1] + hxl[o + 1]) / 2.0, Lxd = (hxh[o - 1] + hxl[o - 1]) / 2.0;
+            const double Lyu = (hyh[o + sy] + hyl[o + sy]) / 2.0, Lyd = (hyh[o - sy] + hyl[o - sy]) / 2.0;
+            const double Lzu = (hzh[o + sz] + hzl[o + sz]) / 2.0, Lzd = (hzh[o - sz] + hzl[o - sz]) / 2.0;
+            s11[o] = sigh * (w11[o + 1] + w11[o - 1]) * (ccf[o + 1] - ccf[o - 1]) * (Lxu - Lxd) * 0.25;
+            s22[o] = sigh * (w11[o + sy] + w11[o - sy]) * (ccf[o + sy] - ccf[o - sy]) * (Lyu - Lyd) * 0.25;
+            s33[o] = sigh * (w11[o + sz] + w11[o - sz]) * (ccf[o + sz] - ccf[o - sz]) * (Lzu - Lzd) * 0.25;
+        }))
+        return -1;
+    return copy_w_from_s(c, w_interior);
+}
+
+// ---------------------------------------------------------------------------
 // solvelpres (main.f:3236-3439) with convective (main.f:3441-3580).
 // ftr = 0 (main.f:3255) and turbst = 0 on this path, so the mut and turbst
 // terms add +0.0 and are not evaluated; the Reynolds-stress arrays of

@@ -4,13 +4,14 @@ c reference, arrays passed as z(icc(ng)) and the like, state in COMMON.
 c The loop is PROGRAM THREE DIM's (main.f:19-73) without printout.
 c
 c usage: dropin_f77 VARIANT OUT     (run where ./inputdata.in is)
-c   VARIANT 1: the loop's calls VOFmethod, kepsmodel, checktime, csf,
+c   VARIANT 1: the loop's calls VOFmethod, CalcH3d (hswitch = 1),
+c              kepsmodel, checktime, csf / SGIP / SGIP2 (surfacet 2, 5, 6),
 c              solvelpres, as main.f:27-69;
 c   VARIANT 2: the same with VOFmethod's body written out (main.f:795-824):
 c              the three swp calls in tswap order, bc_c and calcrho, plus
 c              the CFL function as printout calls it (main.f:5048).
-c Writes tau, time, NPRE, NTOP, NPOST, tswap and cc, u, v, w, p (raw
-c doubles) to OUT after the last step.
+c Writes tau, time, NPRE, NTOP, NPOST, tswap and cc, u, v, w, p and the
+c heights hxmin, hymax, hzmin (raw doubles) to OUT after the last step.
       PROGRAM DROPIN
       INCLUDE 'dropin.h'
       INTEGER t, variant, q, i, j, k
@@ -46,11 +47,16 @@ c
           c1 = CFL()
           write (*,'(a,i6,a,es22.15)') ' step', t, '  CFL', c1
         endif
+        if (hswitch.eq.1) call CalcH3d(z(icc(ng)))
         if (iturb.eq.3.or.iturb.eq.4)
      &    call kepsmodel(t,z(iu(ng)),z(ires(ng)))
         call checktime(t)
         if (surfacet.eq.2) then
           call csf(z(icc(ng)),z(iu(ng)),z(ires(ng)),z(irhs(ng)))
+        else if (surfacet.eq.5) then
+          call SGIP(z(icc(ng)),z(iu(ng)),z(ires(ng)),z(irhs(ng)))
+        else if (surfacet.eq.6) then
+          call SGIP2(z(icc(ng)),z(iu(ng)),z(ires(ng)),z(irhs(ng)))
         else
           STOP 'capalg'
         endif
@@ -67,6 +73,7 @@ c
       write (11) u, v, w
       write (11) (((z(iu(ng)-1+i+nx*(j-1)+nx*ny*(k-1)),
      %             i=1,nx), j=1,ny), k=1,nz)
+      write (11) hxmin, hymax, hzmin
       close (11)
       write (*,'(a,i6,a,es22.15,a,i4)') ' final t=', tmax,
      %   '  tau=', tau, '  NTOP=', NTOP

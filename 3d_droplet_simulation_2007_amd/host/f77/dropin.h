@@ -53,3 +53,7 @@ c Fortran call forms bind to; the sizes follow input.h's PARAMETER rules.
      % rhon(NMX,NMY,NMZ),mun(NMX,NMY,NMZ)
       COMMON /PHYSICS/ tau,time,radius,h,rho1,rho2,mu1,mu2,sigma,g,
      % g1,y0,z0,xvel1,yvel1,zvel1,xvel2,yvel2,zvel2,rhon,mun
+      DOUBLE PRECISION hxmax(NMX,NMY,NMZ),hxmin(NMX,NMY,NMZ),
+     & hymax(NMX,NMY,NMZ),hymin(NMX,NMY,NMZ),
+     & hzmax(NMX,NMY,NMZ),hzmin(NMX,NMY,NMZ)
+      COMMON /NEWW/ hxmax,hxmin,hymax,hymin,hzmax,hzmin

@@ -29,6 +29,9 @@ enum {
     DG_F_RHON, DG_F_MUN, DG_F_MUT, DG_F_KTUR, DG_F_EPS,
     DG_F_A, DG_F_B, DG_F_C, DG_F_EI, DG_F_RHS, DG_F_RES,
     DG_F_PPREV,
+    /* CalcH3d's interface heights (/NEWW/), present with hswitch = 1 or
+     * surfacet = 5, 6 */
+    DG_F_HXMIN, DG_F_HXMAX, DG_F_HYMIN, DG_F_HYMAX, DG_F_HZMIN, DG_F_HZMAX,
     DG_NFIELDS
 };
 
@@ -65,6 +68,9 @@ int dg_vofmethod(const int *t);     /* VOFmethod main.f:795-824 */
 int dg_kepsmodel(const int *t);     /* kepsmodel main.f:1888-2026 (iturb=3) */
 int dg_checktime(const int *t);     /* checktime main.f:2156-2175 */
 int dg_csf(void);                   /* csf main.f:2363-2481 (surfacet=2) */
+int dg_calch3d(void);               /* CalcH3d main.f:1312-1810 (hswitch=1) */
+int dg_sgip(void);                  /* SGIP main.f:2835-3027 (surfacet=5) */
+int dg_sgip2(void);                 /* SGIP2 main.f:3029-3166 (surfacet=6) */
 int dg_solvelpres(const int *t);    /* solvelpres main.f:3236-3439 */
 int dg_calcrho(void);               /* calcrho main.f:2177-2190 */
 /* CFL() main.f:4848-4881: max(|u|,|v|,|w|) over the interior times tauh,
@@ -254,6 +260,9 @@ void csf_(double *ccc, double *w11, double *w22, double *w33);   /* csf(ccc,w11,
 void solvelpres_(double *cc, double *w1, double *w2, double *w3, double *a, double *b, double *c, double *ei,
                  const int *t);                                  /* solvelpres(...,t)   main.f:3236 */
 void calcrho_(double *cc);                                       /* calcrho(cc)         main.f:2177 */
+void calch3d_(double *cc);                                       /* CalcH3d(cc)         main.f:1312 */
+void sgip_(double *ccc, double *w11, double *w22, double *w33);  /* SGIP(ccc,...)       main.f:2835 */
+void sgip2_(double *ccc, double *w11, double *w22, double *w33); /* SGIP2(ccc,...)      main.f:3029 */
 void bc_c_(double *cc);                                          /* bc_c(cc)            main.f:4331 */
 void mglin_(const double *r1, const double *r2);                 /* mglin(r1,r2)        main.f:3645 */
 void swp_(double *us, double *cc, double *vof1, double *vof2, double *vof3, const int *d); /* swp main.f:848 */

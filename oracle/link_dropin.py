@@ -9,7 +9,7 @@ input.h, and links it against the library.  Nothing else of main.f is
 changed.  Output: oracle/_ref/three_dim_dropin (git-ignored; built to be
 linked, never run by the tests).  The temporary copy of main.f is deleted.
 
-usage: python oracle/link_dropin.py [--keep-going]
+usage: python oracle/link_dropin.py
 Exit status 0 when the link succeeds; prints the undefined symbols the
 library resolved.
 """
@@ -30,7 +30,8 @@ OUT = os.path.join(HERE, "_ref", "three_dim_dropin")
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 
 # program units the library replaces (its exported Fortran call forms)
-REPLACED = ("vofmethod", "swp", "kepsmodel", "checktime", "cfl", "csf", "solvelpres", "convective", "mglin")
+REPLACED = ("vofmethod", "swp", "calch3d", "kepsmodel", "checktime", "cfl", "csf", "sgip", "sgip2", "solvelpres",
+            "convective", "mglin")
 
 _UNIT = re.compile(r"^[ \t]+(?:(?:double\s+precision|integer|real|logical)\s+)?"
                    r"(subroutine|function|program)\s+(\w+)", re.IGNORECASE)

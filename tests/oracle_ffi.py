@@ -239,3 +239,29 @@ def inputdata_text(nx=18, ny=18, nz=18, tmax=5, cycles=2, omega=1.5, npre=2, nto
             xvel1, yvel1, zvel1, xvel2, yvel2, zvel2, rho2, rho1, sigma, visc2, visc1, grav, grav1, 2, 2,
             length, radius, y0, z0, 100000, 100000, 100000, surfacet]
     return "".join(f"{v!r}\n" if isinstance(v, float) else f"{v}\n" for v in vals)
+
+
+# the values of the reference's shipped inputdata.in (66x34x34, water and a
+# heavier second fluid, inflow 0.5 m/s, no gravity, a droplet centred just
+# below the wall, SGIP surface tension with CalcH3d heights)
+SHIPPED = dict(nx=66, ny=34, nz=34, tmax=400, cycles=2, omega=1.5, npre=5, ntop=4, npost=5, maxcfl=0.1, iturb=0,
+               convectype=2, hswitch=1, xvel1=0.0, yvel1=0.0, zvel1=0.0, xvel2=0.5, yvel2=0.0, zvel2=0.0,
+               rho2=500.0, rho1=1000.0, sigma=0.0728, visc2=0.001, visc1=0.001, grav=0.0, grav1=0.0,
+               length=0.008, radius=0.001, y0=-0.00017365, z0=1000.0, surfacet=5)
+
+
+def shipped_text(**over):
+    kw = dict(SHIPPED)
+    kw.update(over)
+    return inputdata_text(**kw)
+
+
+def clipped_cc(txt):
+    """smooth2's initial volume fraction clipped to [0, 1]: unclipped, 8
+    cells hold 1 + 4e-13, the first VOFmethod turns 100 cells to NaN (AL3D's
+    cube root of a negative number) and CalcH3d STOPs with NO MATCH CASES
+    FOUND at the first step."""
+    o = Oracle(txt)
+    cc = o.field("cc").clip(0.0, 1.0).copy()
+    o.close()
+    return cc

@@ -2,8 +2,8 @@
 
 oracle/link_dropin.py compiles main.f where it lies under /root/reference,
 with the program units whose call forms the library defines taken out
-(VOFmethod, swp, kepsmodel, checktime, CFL, csf, solvelpres, convective,
-mglin) and nothing else changed, and links it against the library.  The
+(VOFmethod, swp, CalcH3d, kepsmodel, checktime, CFL, csf, SGIP, SGIP2,
+solvelpres, convective, mglin) and nothing else changed, and links it against the library.  The
 program is linked, never run here: the test checks that every procedure the
 rest of main.f calls among those is left undefined in the program and
 defined by the library, and that the program exports the COMMON blocks the
@@ -36,7 +36,8 @@ def test_reference_program_links_against_library(dg):
                        text=True, timeout=600)
     assert r.returncode == 0, r.stdout + r.stderr
     undefined = {ln.split()[-1] for ln in _nm([PROG]) if ln.strip().startswith("U ")}
-    for sym in ("vofmethod_", "checktime_", "csf_", "solvelpres_", "kepsmodel_", "cfl_"):
+    for sym in ("vofmethod_", "checktime_", "csf_", "solvelpres_", "kepsmodel_", "cfl_", "calch3d_", "sgip_",
+                "sgip2_"):
         assert sym in undefined, sym
     defined = {ln.split()[-1] for ln in _nm(["-D", "--defined-only", LIB]) if ln.strip()}
     for sym in ("vofmethod_", "checktime_", "csf_", "solvelpres_", "kepsmodel_", "cfl_", "mglin_", "swp_",

@@ -37,8 +37,9 @@ def run_program(variant, txt):
     scal = np.frombuffer(raw[:16], dtype=np.float64)
     ints = np.frombuffer(raw[16:32], dtype=np.int32)
     f = np.frombuffer(raw[32:], dtype=np.float64)
-    assert f.size == 5 * n
-    fields = {nm: f[q * n:(q + 1) * n].reshape(NZ, NY, NX) for q, nm in enumerate(("cc", "u", "v", "w", "p"))}
+    names = ("cc", "u", "v", "w", "p", "hxmin", "hymax", "hzmin")
+    assert f.size == len(names) * n
+    fields = {nm: f[q * n:(q + 1) * n].reshape(NZ, NY, NX) for q, nm in enumerate(names)}
     return r.stdout, dict(tau=scal[0], time=scal[1], npre=int(ints[0]), ntop=int(ints[1]), npost=int(ints[2]),
                           tswap=int(ints[3])), fields
 
@@ -73,7 +74,7 @@ def test_f77_call_forms_match_restatement(dg, variant):
     assert abs(sc["time"] - o.st.time) <= 1e-12 * o.st.time
     assert (sc["npre"], sc["ntop"], sc["npost"], sc["tswap"]) == (o.st.npre, o.st.ntop, o.st.npost, o.st.tswap)
     errs = {}
-    for nm, a in fields.items():
+    for nm, a in ((k, v) for k, v in fields.items() if not k.startswith("h")):
         b = o.field(nm)
         ai, bi = a[1:-1, 1:-1, 1:-1], b[1:-1, 1:-1, 1:-1]
         errs[nm] = float(np.linalg.norm(ai - bi) / max(np.linalg.norm(bi), 1e-300))
@@ -81,3 +82,28 @@ def test_f77_call_forms_match_restatement(dg, variant):
     for nm, e in errs.items():
         assert e <= 1e-10, (nm, e)
     o.close()
+
+
+@pytest.mark.parametrize("surfacet", [5, 6], ids=["SGIP", "SGIP2"])
+def test_f77_call_forms_with_heights_match_the_c_abi(dg, surfacet):
+    """The main loop's CalcH3d and SGIP / SGIP2 branches (main.f:30, 59-62)
+    through the call forms, against the same steps through the C-ABI
+    (dg_step) on the GPU: the same kernels on the same state, so the same
+    bits."""
+    if not os.path.exists(PROG):
+        pytest.fail("dropin_f77 was not built (flang missing at build time?)")
+    nsteps = 3
+    print(f"case F77 call forms hswitch=1 surfacet={surfacet}, {NX}x{NY}x{NZ}, {nsteps} steps", flush=True)
+    txt = inputdata_text(nx=NX, ny=NY, nz=NZ, tmax=nsteps - 1, y0=0.0015, radius=0.0005, hswitch=1,
+                         surfacet=surfacet)
+    _, sc, fields = run_program(1, txt)
+    g = dg.DropletGPU(inputdata_text=txt)
+    try:
+        for t in range(nsteps):
+            g.step(t)
+        st = g.state()
+        assert sc["tau"] == st["tau"] and sc["ntop"] == st["ntop"] and sc["tswap"] == st["tswap"]
+        for nm, a in fields.items():
+            assert np.array_equal(a, g.get(nm)), nm
+    finally:
+        g.close()

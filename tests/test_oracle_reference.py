@@ -103,10 +103,7 @@ def test_shipped_sgip_sgip2_calch3d_bitexact():
     [0, 1] on both sides (unclipped the reference STOPs at step 0).  Every
     field and the six height arrays, bit for bit."""
     need(66, 34, 34)
-    import sys
-    import os
-    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
-    from make_golden import clipped_cc, shipped_text
+    from oracle_ffi import clipped_cc, shipped_text
     for surfacet, nsteps in ((5, 3), (6, 2)):
         txt = shipped_text(surfacet=surfacet)
         cc0 = clipped_cc(txt)

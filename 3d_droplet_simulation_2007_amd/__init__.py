@@ -6,5 +6,5 @@ The package directory name starts with a digit; import it with
 ``importlib.import_module("3d_droplet_simulation_2007_amd")``.
 """
 from .api import (FIELD_IDS, DropletError, DropletGPU, abcei, acfromac, addint, al3d, bc_scalar, fl3d,  # noqa: F401
-                  filterc27, interp, lib, make_inputdata, poisson_solve, profile_start, profile_stop, relax, resid,
-                  rstrct, set_lookahead, set_smoother, set_tuning, slvsml)
+                  filterc27, format_real, interp, lib, make_inputdata, poisson_solve, profile_start, profile_stop,
+                  relax, resid, rstrct, set_lookahead, set_smoother, set_tuning, slvsml)

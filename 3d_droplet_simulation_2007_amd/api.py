@@ -203,6 +203,55 @@ class DropletGPU:
             setattr(s, k, v)
         _check(lib().dg_set_state(ctypes.byref(s)), "dg_set_state")
 
+    # ---- output, checkpoint and restart
+    def run_limits(self):
+        """(t0, tmax) of the run: 0 and inputdata.in's tmax, or save.dat's
+        t0 and t0 + its tmax after a restart (iniget, main.f:698-793)."""
+        t0, tmax = ctypes.c_int(), ctypes.c_int()
+        _check(lib().dg_run_limits(ctypes.byref(t0), ctypes.byref(tmax)), "dg_run_limits")
+        return t0.value, tmax.value
+
+    def output_open(self, directory=".", echo_stdout=False):
+        """openfiles (main.f:105-118) in `directory`; starts the writer thread."""
+        _check(lib().dg_output_open(os.fspath(directory).encode(), _i(1 if echo_stdout else 0)), "dg_output_open")
+
+    def printout(self, t):
+        """printout(t, cc, p) (main.f:5020-5100): queues the downloads and the
+        files of step t; returns before they are written."""
+        _check(lib().dg_printout(_i(t)), "dg_printout")
+
+    def output_flush(self):
+        _check(lib().dg_output_flush(), "dg_output_flush")
+
+    def output_close(self):
+        _check(lib().dg_output_close(), "dg_output_close")
+
+    def output_stats(self):
+        jobs = ctypes.c_int()
+        b, w, t = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _check(lib().dg_output_stats(ctypes.byref(jobs), ctypes.byref(b), ctypes.byref(w), ctypes.byref(t)),
+               "dg_output_stats")
+        return dict(jobs=jobs.value, bytes=b.value, wait_ms=w.value, write_ms=t.value)
+
+    def savedata(self, path):
+        """savedata (main.f:5414-5467): the restart records iniget reads."""
+        _check(lib().dg_savedata(os.fspath(path).encode()), "dg_savedata")
+
+    def save_state(self, path):
+        """The full resident state, for a bit-identical resume."""
+        _check(lib().dg_save_state(os.fspath(path).encode()), "dg_save_state")
+
+    def load_state(self, path):
+        _check(lib().dg_load_state(os.fspath(path).encode()), "dg_load_state")
+
+
+def format_real(x, mode=0):
+    """x as flang writes it list-directed (mode 0) or with E15.7 (mode 1);
+    host code, no GPU."""
+    buf = ctypes.create_string_buffer(64)
+    _check(lib().dg_format_real(_d(x), _i(mode), buf, _i(64)), "dg_format_real")
+    return buf.value.decode()
+
 
 # ---- host-pointer kernels with the reference's argument lists -------------
 

@@ -22,15 +22,25 @@ namespace dg {
 
 static char g_err[1024] = "";
 static int g_status = 0;
+static thread_local std::string *t_err_sink = nullptr;
 
 void set_error(const char *fmt, ...)
 {
     va_list ap;
     va_start(ap, fmt);
+    if (t_err_sink) {
+        // a worker thread's error: kept for the thread (the first one)
+        char buf[1024];
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        if (t_err_sink->empty()) *t_err_sink = buf;
+        return;
+    }
     vsnprintf(g_err, sizeof g_err, fmt, ap);
     va_end(ap);
     g_status = -1;
 }
+void set_error_sink(std::string *sink) { t_err_sink = sink; }
 const char *last_error() { return g_err; }
 int status() { return g_status; }
 
@@ -369,9 +379,10 @@ int dg_init(const char *path) { return init_from_input(path, nullptr); }
 int dg_finalize(void)
 {
     Ctx &c = *ctx();
+    const int rc = output_shutdown();
     if (g_ready) free_ctx(c);
     g_ready = false;
-    return 0;
+    return rc;
 }
 }  // extern "C"
 
@@ -418,8 +429,10 @@ void resident_scalars_changed()
 bool resident_ready() { return g_ready; }
 double *resident_field(int id) { return g_ready ? field_ptr(*ctx(), id) : nullptr; }
 
-// dg_init: readinput of the 38-line inputdata.in, memoryaloc, nondim and the
-// droplet IC on the device; the parsed input through in_out
+// dg_init: initialize (main.f:121-183) -- readinput of the 38-line
+// inputdata.in, memoryaloc, nondim, then the droplet IC (smooth2) on the
+// device or, with restart = 1, iniget's records from save.dat beside
+// inputdata.in; bc_c, bc_v, calcrho.  The parsed input through in_out.
 int init_from_input(const char *path, Input *in_out)
 {
     Ctx &c = *ctx();
@@ -429,40 +442,73 @@ int init_from_input(const char *path, Input *in_out)
     }
     g_status = 0;
     g_err[0] = 0;
+    RunInfo &ri = run_info();
+    ri.valid = false;
     Input in;
     if (read_input(path, in)) return -1;
-    if (in.icase != 2) {
-        set_error("icase=%d: only the droplet case (icase=2, smooth2) is on this path", in.icase);
-        return -1;
-    }
+    // iniget runs after nondim (main.f:139-174): the scales come from
+    // inputdata.in, the solver settings and output cadences from the file
+    Input run = in;
+    int t0 = 0;
+    std::vector<double> uvwcc[4];
     if (in.restart != 0) {
-        set_error("restart=%d: restart files (iniget) are not on this path", in.restart);
+        const std::string p(path);
+        const size_t slash = p.find_last_of('/');
+        const std::string save = (slash == std::string::npos ? std::string() : p.substr(0, slash + 1)) + "save.dat";
+        if (read_restart_file(save.c_str(), run, &t0, uvwcc)) return -1;
+    }
+    if (run.icase != 2) {
+        set_error("icase=%d: only the droplet case (icase=2) is on this path", run.icase);
         return -1;
     }
-    if (in.hswitch != 0 && in.hswitch != 1) {
-        set_error("hswitch=%d: 0 (off) or 1 (CalcH3d interface heights)", in.hswitch);
+    if (run.hswitch != 0 && run.hswitch != 1) {
+        set_error("hswitch=%d: 0 (off) or 1 (CalcH3d interface heights)", run.hswitch);
         return -1;
     }
-    if (in.surfacet != 2 && in.surfacet != 5 && in.surfacet != 6) {
+    if (run.surfacet != 2 && run.surfacet != 5 && run.surfacet != 6) {
         set_error("surfacet=%d: CSF (2), SGIP (5) and SGIP2 (6) are on this path; CSS, CSF-BKZ and Meier "
                   "(1, 3, 4) are not",
-                  in.surfacet);
+                  run.surfacet);
         return -1;
     }
-    if (in.iturb != 0 && in.iturb != 3) {
-        set_error("iturb=%d: only laminar (0) and k-epsilon (3) are on this path", in.iturb);
+    if (run.iturb != 0 && run.iturb != 3) {
+        set_error("iturb=%d: only laminar (0) and k-epsilon (3) are on this path", run.iturb);
         return -1;
     }
     if (resident_setup(in.nx, in.ny, in.nz)) return -1;
     nondim(in, c.s);
+    if (in.restart != 0) {
+        // the values iniget reads into COMMON that the time step uses as they are
+        c.s.maxcfl = run.maxcfl;
+        c.s.omega = run.omega;
+        c.s.cycles = run.cycles; c.s.npre = run.npre; c.s.ntop = run.ntop; c.s.npost = run.npost;
+        c.s.iturb = run.iturb; c.s.convectype = run.convectype; c.s.hswitch = run.hswitch;
+        c.s.tprint = run.tprint;
+    }
     resident_scalars_changed();
     // cmu = 0.09 (main.f:347-353); mut, turbst and the Reynolds stresses start at 0
     if (mg_fill(c.g.n(), c.stream, c.cmu, 0.09)) return -1;
-    if ((in.hswitch == 1 || in.surfacet == 5 || in.surfacet == 6) && alloc_heights(c)) return -1;
-    if (op_smooth2(c) || op_bc_c(c, c.lev[c.ng - 1].cc) || op_bc_v(c) || op_calcrho(c)) return -1;
+    if ((run.hswitch == 1 || run.surfacet == 5 || run.surfacet == 6) && alloc_heights(c)) return -1;
+    if (in.restart != 0) {
+        const size_t bytes = c.g.n() * sizeof(double);
+        double *dst[4] = {c.u, c.v, c.w, c.lev[c.ng - 1].cc};
+        for (int q = 0; q < 4; q++) DG_HIP(hipMemcpy(dst[q], uvwcc[q].data(), bytes, hipMemcpyHostToDevice));
+    } else if (op_smooth2(c)) {
+        return -1;
+    }
+    if (op_bc_c(c, c.lev[c.ng - 1].cc) || op_bc_v(c) || op_calcrho(c)) return -1;
     DG_HIP(hipStreamSynchronize(c.stream));
-    if (in_out) *in_out = in;
+    ri.in = run;
+    ri.t0 = t0;
+    ri.valid = true;
+    if (in_out) *in_out = run;
     return 0;
+}
+
+RunInfo &run_info()
+{
+    static RunInfo r;
+    return r;
 }
 
 }  // namespace dg
@@ -542,8 +588,9 @@ int dg_step(const int *t)
 
 int dg_run(const int *t0, const int *nsteps)
 {
+    const bool out = output_active();
     for (int t = *t0; t < *t0 + *nsteps; t++)
-        if (dg_step(&t)) return -1;
+        if ((out && dg_printout(&t)) || dg_step(&t)) return -1;
     Ctx &c = *ctx();
     DG_HIP(hipStreamSynchronize(c.stream));
     return 0;

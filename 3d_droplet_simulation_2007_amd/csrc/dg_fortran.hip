@@ -25,8 +25,10 @@
 // error.  Scalars are the host's: before each call the library reads tau,
 // time, tauh, tauh2, tswap, NPRE, NTOP, NPOST from COMMON, and after it
 // writes back those the routine changes (the 8-byte hand-backs).  Fields
-// reach host memory only through dg_sync_to_host_(mask), which a program
-// calls before its own readers (printout, main.f:5020-5153).
+// reach host memory through dg_sync_to_host_(mask), and at the end of
+// solvelpres_ for step t the library downloads into COMMON the fields the
+// program's own printout reads at the start of step t + 1 (main.f:21-23,
+// 5020-5100), so the program's output files need no change to its loop.
 //
 // Errors: the reference STOPs on a fatal condition; these entry points
 // print the library's error message and stop the program the same way
@@ -355,7 +357,7 @@ void write_all_scalars(const Input &in)
     ci[HostCommon::C_CONVECTYPE] = s.convectype; ci[HostCommon::C_HSWITCH] = s.hswitch;
     ci[HostCommon::C_SURFACET] = s.surfacet;
     // /TIMES/ t0, tmax, tstore, ntecplot, tsave, tprint
-    h.times[0] = 0; h.times[1] = in.tmax; h.times[3] = in.ntecplot; h.times[4] = in.tsave; h.times[5] = in.tprint;
+    h.times[0] = run_info().t0; h.times[1] = in.tmax; h.times[3] = in.ntecplot; h.times[4] = in.tsave; h.times[5] = in.tprint;
     double *d = h.dinem;
     d[0] = s.tau_si; d[1] = s.timescale; d[2] = s.lengthscale; d[3] = in.length; d[4] = in.rho1; d[5] = in.rho2;
     d[6] = in.visc1; d[7] = in.visc2; d[8] = in.sigma; d[9] = in.grav; d[10] = s.lengthsize; d[12] = in.grav1;
@@ -425,6 +427,23 @@ int copy_fields(unsigned mask, bool to_device)
         else DG_HIP(hipMemcpy(g_hc.cmu(), c.cmu, bytes, hipMemcpyDeviceToHost));
     }
     return 0;
+}
+
+// The fields the program's own printout (main.f:5020-5100) reads at the
+// start of step t, into COMMON: u, v, w, cc and mut every tprint steps
+// (velocityc, FINDNINF, SUMFIELD, FINDMAX, FINDN2), u, v, w, cc every
+// ntecplot steps (printtecp) and at t = tsave (savedata), the pressure
+// every 100 steps.  /TIMES/ holds ntecplot, tsave, tprint (input.h:67).
+int printout_fields_to_host(int t)
+{
+    const int ntecplot = g_hc.times[3], tsave = g_hc.times[4], tprint = g_hc.times[5];
+    const unsigned uvwc = (1u << DG_F_U) | (1u << DG_F_V) | (1u << DG_F_W) | (1u << DG_F_CC);
+    unsigned m = 0;
+    if (tprint > 0 && t % tprint == 0) m |= uvwc | (1u << DG_F_MUT);
+    if (ntecplot > 0 && t % ntecplot == 0) m |= uvwc;
+    if (t % 100 == 0) m |= 1u << DG_F_P;
+    if (t == tsave) m |= uvwc;
+    return m ? copy_fields(m, false) : 0;
 }
 
 constexpr unsigned ALL_FIELDS = (1u << (DG_NFIELDS + 1)) - 1;  // every field id and cmu
@@ -608,7 +627,8 @@ void solvelpres_(double *cc, double *w1, double *w2, double *w3, double *a, doub
             expect_slot("solvelpres", 5, a, L.a, "z(ia(ng))") || expect_slot("solvelpres", 6, b, L.b, "z(ib(ng))") ||
             expect_slot("solvelpres", 7, c, L.c, "z(ic(ng))") || expect_slot("solvelpres", 8, ei, L.ei, "z(iei(ng))"))
             return -1;
-        return op_solvelpres(cx, *t);
+        if (op_solvelpres(cx, *t)) return -1;
+        return printout_fields_to_host(*t + 1);
     });
 }
 

@@ -34,6 +34,9 @@ inline Grid make_grid(int nx, int ny, int nz)
 void set_error(const char *fmt, ...);
 const char *last_error();
 int status();
+// errors raised on the calling thread go to *sink (null: the library's
+// error state); for worker threads, whose errors the caller reports later
+void set_error_sink(std::string *sink);
 
 #define DG_HIP(call)                                                                      \
     do {                                                                                  \

@@ -2,6 +2,8 @@
 // C-ABI (dg_api.hip) and the Fortran call forms (dg_fortran.hip).
 #pragma once
 
+#include <vector>
+
 #include "dg_ctx.h"
 
 namespace dg {
@@ -35,6 +37,26 @@ void resident_scalars_changed();
 bool resident_ready();
 // device array of a field id (DG_F_*), nullptr when not ready
 double *resident_field(int id);
+
+// The run's inputdata.in values as printout and savedata use them (after a
+// restart: with the values iniget read from the file) and its first step.
+struct RunInfo {
+    bool valid = false;
+    Input in{};
+    int t0 = 0;
+};
+RunInfo &run_info();
+
+// iniget (main.f:698-793): header values into `in`, t0, u/v/w/cc records
+int read_restart_file(const char *path, Input &in, int *t0, std::vector<double> uvwcc[4]);
+// savedata's records (main.f:5414-5467) of host arrays u, v, w, cc
+int write_restart_records(const char *path, const Input &in, int t0, const Scalars &s, const double *u,
+                          const double *v, const double *w, const double *cc);
+
+// printout's writer (dg_output.hip): waits for queued output and stops its
+// thread (dg_finalize); output_active: dg_output_open has opened the files
+int output_shutdown();
+bool output_active();
 
 // single operators of the time step on resident arrays
 int op_swp(Ctx &c, const double *us, double *cc, double *vof1, double *vof2, double *vof3, int d);

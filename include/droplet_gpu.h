@@ -53,9 +53,16 @@ typedef struct dg_state {
 /* Replaces initialize (main.f:121-183): readinput (main.f:185-272) of the
  * unchanged inputdata.in format, memoryaloc (main.f:274-322), nondim
  * (main.f:341-446), the droplet IC smooth2 (main.f:5678-5752), bc_c, bc_v,
- * calcrho.  Allocates every field in HBM. */
+ * calcrho.  With restart = 1 in inputdata.in, iniget (main.f:698-793)
+ * replaces smooth2: u, v, w, cc and the values the reference reads from
+ * save.dat, the file in inputdata.in's directory.  Allocates every field in
+ * HBM. */
 int dg_init(const char *inputdata_path);
-/* Frees device memory (the reference has no counterpart; process exit). */
+/* The loop bounds of the run: t0 (0, or save.dat's t0 on a restart) and
+ * tmax (t0 + the file's tmax on a restart, main.f:791). */
+int dg_run_limits(int *t0, int *tmax);
+/* Frees device memory and finishes queued output (the reference has no
+ * counterpart; process exit). */
 int dg_finalize(void);
 /* 0 if the library is usable, -1 after an error. */
 int dg_status(void);
@@ -63,6 +70,8 @@ const char *dg_last_error(void);
 
 /* ---- the time-step operators (device resident) ------------------------ */
 int dg_step(const int *t);          /* loop body main.f:21-71 (printout excluded) */
+/* the loop main.f:19-73 for nsteps steps from t0; printout at the start of
+ * each step when dg_output_open has opened the output files */
 int dg_run(const int *t0, const int *nsteps);
 int dg_vofmethod(const int *t);     /* VOFmethod main.f:795-824 */
 int dg_kepsmodel(const int *t);     /* kepsmodel main.f:1888-2026 (iturb=3) */
@@ -138,6 +147,48 @@ int dg_set_tuning(const int *fused, const int *graphs, const int *split, const i
  * V-cycle still runs) and changes nothing the solve returns.  Applies to
  * the resident state and to later dg_init / dg_poisson_solve calls. */
 int dg_set_lookahead(const double *ratio);
+
+/* ---- output, checkpoint and restart ------------------------------------ */
+/* openfiles (main.f:105-118) in directory dir ("" or NULL: the current
+ * one): creates outputdata.out, spurious.plt, normv.plt, hysteresis.plt and
+ * velocity.plt with their header lines, and starts the writer thread.
+ * echo_stdout = 1 also writes printout's screen lines to stdout. */
+int dg_output_open(const char *dir, const int *echo_stdout);
+/* printout(t, cc, p) (main.f:5020-5100) at the start of step t, writing the
+ * reference's files in its formats: every tprint steps velocityc's record
+ * (velocity.plt), the diagnostics lines (outputdata.out, and stdout when
+ * echoed) and the spurious.plt and normv.plt records; every ntecplot steps
+ * mainNNNN.plt (printtecp, main.f:5106-5153); every 100 steps
+ * pressureNNNN.plt; at t = tsave the restart records (savedata) into
+ * save.dat -- the file iniget reads (the reference writes them to unit 2
+ * after opening saved.dat on unit 3, main.f:5418-5419).  The fields are
+ * snapshotted on the device in stream order and downloaded and written by
+ * the writer thread: the call returns before the files are written. */
+int dg_printout(const int *t);
+/* Waits until every queued printout is written and flushes the files;
+ * returns -1 with the writer's first error. */
+int dg_output_flush(void);
+/* Finishes queued output, closes the files and stops the writer thread. */
+int dg_output_close(void);
+/* The writer's counters since the process started: printouts queued,
+ * bytes downloaded, milliseconds dg_printout waited for a free snapshot
+ * slot, milliseconds the writer thread spent on files. */
+int dg_output_stats(int *jobs, double *bytes, double *wait_ms, double *write_ms);
+/* savedata (main.f:5414-5467): the restart records of the current u, v, w,
+ * cc and the run's values into the file `path` (Fortran unformatted
+ * sequential records, the layout iniget reads). */
+int dg_savedata(const char *path);
+/* The full resident state -- every scalar, every field that carries over
+ * between steps and every level's multigrid heap arrays -- into `path`,
+ * and back into a run of the same grid (no reference counterpart: the
+ * reference's restart file holds u, v, w, cc only).  A run resumed from
+ * dg_save_state after step t continues bit-identically with step t + 1. */
+int dg_save_state(const char *path);
+int dg_load_state(const char *path);
+/* Host formatting, no GPU (no reference counterpart): x as flang writes it
+ * list-directed (mode 0) or with the reference's E15.7 (mode 1), into buf
+ * of len bytes, NUL-terminated. */
+int dg_format_real(const double *x, const int *mode, char *buf, const int *len);
 
 /* ---- host-pointer kernels with the reference's argument lists --------- */
 /* Each uploads its arrays, runs on the GPU and downloads the outputs. */

@@ -233,11 +233,12 @@ class Oracle:
 def inputdata_text(nx=18, ny=18, nz=18, tmax=5, cycles=2, omega=1.5, npre=2, ntop=4, npost=2, maxcfl=0.25,
                    iturb=0, convectype=2, hswitch=0, xvel1=0.0, yvel1=-1.0, zvel1=0.0, xvel2=0.0, yvel2=0.0,
                    zvel2=0.0, rho2=1.2, rho1=1000.0, sigma=0.072, visc2=1.8e-5, visc1=1.0e-3, grav=-9.81,
-                   grav1=0.0, length=0.004, radius=0.0008, y0=0.0025, z0=1000.0, surfacet=2):
+                   grav1=0.0, length=0.004, radius=0.0008, y0=0.0025, z0=1000.0, surfacet=2, restart=0,
+                   tprint=100000, ntecplot=100000, tsave=100000):
     """inputdata.in in the reference's 38-line order (main.f:188-234)."""
-    vals = [nx, ny, nz, tmax, 0, cycles, omega, npre, ntop, npost, 0, maxcfl, iturb, convectype, hswitch,
+    vals = [nx, ny, nz, tmax, 0, cycles, omega, npre, ntop, npost, restart, maxcfl, iturb, convectype, hswitch,
             xvel1, yvel1, zvel1, xvel2, yvel2, zvel2, rho2, rho1, sigma, visc2, visc1, grav, grav1, 2, 2,
-            length, radius, y0, z0, 100000, 100000, 100000, surfacet]
+            length, radius, y0, z0, tprint, ntecplot, tsave, surfacet]
     return "".join(f"{v!r}\n" if isinstance(v, float) else f"{v}\n" for v in vals)
 
 

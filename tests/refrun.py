@@ -58,11 +58,13 @@ class _Layout:
         self.n = nx * ny * nz
 
 
-def _child(lib_path, inputdata, nsteps, init_fields, want_fields, conn, every_step_fields):
+def _child(lib_path, inputdata, nsteps, init_fields, want_fields, conn, every_step_fields, workdir=None,
+           printout_ts=(), loop_printout=False):
     try:
-        devnull = os.open(os.devnull, os.O_WRONLY)
-        os.dup2(devnull, 1)
-        tmp = tempfile.mkdtemp(prefix="refrun_")
+        tmp = workdir or tempfile.mkdtemp(prefix="refrun_")
+        # stdout (printout's list-directed lines) into the work directory
+        out = os.open(os.path.join(tmp, "stdout.txt"), os.O_WRONLY | os.O_CREAT | os.O_TRUNC, 0o644)
+        os.dup2(out, 1)
         with open(os.path.join(tmp, "inputdata.in"), "w") as f:
             f.write(inputdata)
         os.chdir(tmp)
@@ -118,8 +120,12 @@ def _child(lib_path, inputdata, nsteps, init_fields, want_fields, conn, every_st
             arrs[nm][...] = val
         hist = []
         snaps = []
-        for t in range(nsteps):
+        tim = np.frombuffer((ctypes.c_int * 6).in_dll(lib, "times_"), dtype=np.int32)
+        t0 = int(tim[0])
+        for t in range(t0, t0 + nsteps):
             ti = ctypes.c_int(t)
+            if loop_printout:
+                lib.printout_(ctypes.byref(ti), zp(icc[ng - 1]), zp(iu[ng - 1]))
             lib.vofmethod_(ctypes.byref(ti))
             if hswitch == 1:
                 lib.calch3d_(zp(icc[ng - 1]))
@@ -143,7 +149,14 @@ def _child(lib_path, inputdata, nsteps, init_fields, want_fields, conn, every_st
             if every_step_fields:
                 snaps.append({nm: arrs[nm].copy() for nm in every_step_fields})
         out = {nm: arrs[nm].copy() for nm in want_fields}
-        scal = dict(tau=float(physd[0]), time=float(physd[1]), h=float(physd[3]))
+        scal = dict(tau=float(physd[0]), time=float(physd[1]), h=float(physd[3]), tauh=float(diff[3]),
+                    tauh2=int(diff_i[8]), npre=int(cyc_i[5]), npost=int(cyc_i[6]), ntop=int(cyc_i[7]),
+                    tswap=int(cyc_i[17]), t0=t0)
+        # printout (main.f:5020-5100) of the final state at the given steps
+        for t in printout_ts:
+            ti = ctypes.c_int(t)
+            lib.printout_(ctypes.byref(ti), zp(icc[ng - 1]), zp(iu[ng - 1]))
+        _flush_units(lib, (2, 6, 23, 24, 26, 27, 28))
         conn.send(dict(fields=out, hist=hist, scalars=scal, snaps=snaps))
     except Exception as e:  # pragma: no cover - reported to the parent
         conn.send(dict(error=repr(e)))
@@ -151,8 +164,27 @@ def _child(lib_path, inputdata, nsteps, init_fields, want_fields, conn, every_st
         conn.close()
 
 
+def _flush_units(lib, units):
+    """FLUSH of the reference's open units through the Fortran runtime that
+    the library carries (the child ends without the runtime's exit handler,
+    which would otherwise write the buffers)."""
+    begin = lib._FortranAioBeginFlush
+    begin.restype = ctypes.c_void_p
+    begin.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+    end = lib._FortranAioEndIoStatement
+    end.argtypes = [ctypes.c_void_p]
+    for u in units:
+        end(begin(u, b"refrun.py", 0))
+
+
 def run_reference(inputdata: str, nsteps: int, init_fields=None, want_fields=FIELD_NAMES,
-                  every_step_fields=None, timeout=600):
+                  every_step_fields=None, timeout=600, workdir=None, printout_ts=(), loop_printout=False):
+    """Runs initialize and nsteps of the loop from t0 (0, or save.dat's t0
+    with restart = 1: put save.dat into workdir).  workdir (default: a fresh
+    temporary directory) receives inputdata.in, stdout.txt and the files
+    the reference writes; loop_printout calls printout at the start of each
+    step as the reference's loop does, printout_ts calls it on the final
+    state for each listed step."""
     nx, ny, nz = (int(l.split()[0]) for l in inputdata.splitlines()[:3])
     lib_path = ref_lib_path(nx, ny, nz)
     if not os.path.exists(lib_path):
@@ -160,7 +192,7 @@ def run_reference(inputdata: str, nsteps: int, init_fields=None, want_fields=FIE
     ctx = mp.get_context("fork")
     a, b = ctx.Pipe(duplex=False)
     proc = ctx.Process(target=_child, args=(lib_path, inputdata, nsteps, init_fields, want_fields, b,
-                                            every_step_fields))
+                                            every_step_fields, workdir, tuple(printout_ts), loop_printout))
     proc.start()
     b.close()
     if not a.poll(timeout):

@@ -18,7 +18,8 @@ LIBPATH = os.path.join(HERE, "libdroplet_gpu.so")
 
 FIELD_IDS = dict(cc=0, u=1, v=2, w=3, p=4, s11=5, s22=6, s33=7, s12=8, s13=9, s23=10,
                  rhon=11, mun=12, mut=13, ktur=14, eps=15, a=16, b=17, c=18, ei=19, rhs=20, res=21,
-                 pprev=22, hxmin=23, hxmax=24, hymin=25, hymax=26, hzmin=27, hzmax=28)
+                 pprev=22, hxmin=23, hxmax=24, hymin=25, hymax=26, hzmin=27, hzmax=28,
+                 u2p=29, v2p=30, w2p=31, uvp=32, uwp=33, vwp=34, cmu=35)
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -145,6 +146,27 @@ class DropletGPU:
 
     def csf(self):
         _check(lib().dg_csf(), "dg_csf")
+
+    def eddyvisc(self):
+        """eddyvisc (main.f:1828-1885), the mixing-length model of iturb 1, 2."""
+        _check(lib().dg_eddyvisc(), "dg_eddyvisc")
+
+    def uv2p(self):
+        """uv2p (main.f:2032-2155), the anisotropic stresses of iturb 4."""
+        _check(lib().dg_uv2p(), "dg_uv2p")
+
+    def smooth2_droplet(self):
+        """The droplet dg_init placed (smooth2, main.f:5678-5752) as one row of
+        init_droplets' parameters: x0, y0, z0, r, u, v, w in solver units."""
+        p = np.zeros(7)
+        _check(lib().dg_smooth2_droplet(_p(p)), "dg_smooth2_droplet")
+        return p
+
+    def init_droplets(self, params):
+        """smooth2 generalised to several droplets: params is (n, 7) with rows
+        (x0, y0, z0, r, u, v, w) in solver units (see smooth2_droplet)."""
+        a = np.ascontiguousarray(params, dtype=np.float64).reshape(-1, 7)
+        _check(lib().dg_init_droplets(_i(a.shape[0]), _p(a)), "dg_init_droplets")
 
     def calch3d(self):
         _check(lib().dg_calch3d(), "dg_calch3d")

@@ -150,11 +150,21 @@ static int alloc_fields(Ctx &c)
                        &c.mun, &c.mut, &c.ktur, &c.eps, &c.cmu, &c.ccf, &c.pprev};
     for (double **p : arrs)
         if (dalloc(p, n)) return -1;
-    return 0;
+    return dalloc(&c.lmix, 2 * (size_t)c.g.nx);  // lmix, then eddyvisc's per-station scratch
 }
 
 // CalcH3d's six height arrays, zeroed (the reference's COMMON starts at 0
 // and its ghost cells are never written)
+int alloc_reynolds(Ctx &c)
+{
+    if (c.u2p) return 0;
+    const size_t n = c.g.n();
+    double **arrs[] = {&c.u2p, &c.v2p, &c.w2p, &c.uvp, &c.uwp, &c.vwp};
+    for (double **p : arrs)
+        if (dalloc(p, n)) return -1;
+    return 0;
+}
+
 int alloc_heights(Ctx &c)
 {
     if (c.hxmin) return 0;
@@ -178,7 +188,8 @@ static void free_ctx(Ctx &c)
     }
     void *fs[] = {c.u, c.v, c.w, c.s11, c.s22, c.s33, c.s12, c.s13, c.s23, c.rhon, c.mun, c.mut, c.ktur,
                   c.eps, c.cmu, c.ccf, c.pprev, c.pcg_r, c.pcg_p, c.pcg_q, c.red_partial, c.dscal, c.coarse_M,
-                  c.hxmin, c.hxmax, c.hymin, c.hymax, c.hzmin, c.hzmax};
+                  c.hxmin, c.hxmax, c.hymin, c.hymax, c.hzmin, c.hzmax, c.u2p, c.v2p, c.w2p, c.uvp, c.uwp,
+                  c.vwp, c.lmix};
     for (void *p : fs)
         if (p) (void)hipFree(p);
     for (int q = 0; q < 2; q++)
@@ -317,6 +328,13 @@ static double *field_ptr(Ctx &c, int id)
     case DG_F_HYMAX: return c.hymax;
     case DG_F_HZMIN: return c.hzmin;
     case DG_F_HZMAX: return c.hzmax;
+    case DG_F_U2P: return c.u2p;
+    case DG_F_V2P: return c.v2p;
+    case DG_F_W2P: return c.w2p;
+    case DG_F_UVP: return c.uvp;
+    case DG_F_UWP: return c.uwp;
+    case DG_F_VWP: return c.vwp;
+    case DG_F_CMU: return c.cmu;
     default: return nullptr;
     }
 }
@@ -471,8 +489,9 @@ int init_from_input(const char *path, Input *in_out)
                   run.surfacet);
         return -1;
     }
-    if (run.iturb != 0 && run.iturb != 3) {
-        set_error("iturb=%d: only laminar (0) and k-epsilon (3) are on this path", run.iturb);
+    if (run.iturb < 0 || run.iturb > 4) {
+        // kepsmodel STOPs with 'iturb problem' (main.f:1966-1968)
+        set_error("iturb=%d: 0 laminar, 1 and 2 mixing length, 3 k-epsilon, 4 anisotropic k-epsilon", run.iturb);
         return -1;
     }
     if (resident_setup(in.nx, in.ny, in.nz)) return -1;
@@ -489,7 +508,12 @@ int init_from_input(const char *path, Input *in_out)
     // cmu = 0.09 (main.f:347-353); mut, turbst and the Reynolds stresses start at 0
     if (mg_fill(c.g.n(), c.stream, c.cmu, 0.09)) return -1;
     if ((run.hswitch == 1 || run.surfacet == 5 || run.surfacet == 6) && alloc_heights(c)) return -1;
+    if (run.iturb == 4 && alloc_reynolds(c)) return -1;
     if (in.restart != 0) {
+        // no smooth2 on a restart: y0 and z0 keep nondim's channel centre
+        // (main.f:409-410), which eddyvisc reads
+        c.s.y0 = (double)(in.ny - 2) * c.s.h / 2.0;
+        c.s.z0 = (double)(in.nz - 2) * c.s.h / 2.0;
         const size_t bytes = c.g.n() * sizeof(double);
         double *dst[4] = {c.u, c.v, c.w, c.lev[c.ng - 1].cc};
         for (int q = 0; q < 4; q++) DG_HIP(hipMemcpy(dst[q], uvwcc[q].data(), bytes, hipMemcpyHostToDevice));
@@ -524,6 +548,51 @@ int dg_kepsmodel(const int *t)
 {
     if (require_ready()) return -1;
     return op_kepsmodel(*ctx(), *t);
+}
+int dg_eddyvisc(void)
+{
+    if (require_ready()) return -1;
+    return op_eddyvisc(*ctx());
+}
+int dg_uv2p(void)
+{
+    if (require_ready()) return -1;
+    Ctx &c = *ctx();
+    if (alloc_reynolds(c)) return -1;
+    return op_uv2p(c);
+}
+int dg_init_droplets(const int *n, const double *params)
+{
+    if (require_ready()) return -1;
+    Ctx &c = *ctx();
+    if (*n < 1 || *n > 4096) {
+        set_error("dg_init_droplets: %d droplets (1 to 4096)", *n);
+        return -1;
+    }
+    for (int d = 0; d < *n; d++)
+        if (!(params[7 * d + 3] > 0.0) || !std::isfinite(params[7 * d + 0]) || !std::isfinite(params[7 * d + 1]) ||
+            !std::isfinite(params[7 * d + 2])) {
+            set_error("dg_init_droplets: droplet %d has radius %g at (%g, %g, %g)", d, params[7 * d + 3],
+                      params[7 * d + 0], params[7 * d + 1], params[7 * d + 2]);
+            return -1;
+        }
+    if (op_droplets(c, *n, params) || op_bc_v(c) || op_calcrho(c)) return -1;
+    DG_HIP(hipStreamSynchronize(c.stream));
+    return 0;
+}
+int dg_smooth2_droplet(double *params)
+{
+    if (require_ready()) return -1;
+    const Ctx &c = *ctx();
+    // smooth2's centre and velocities (main.f:5684-5690, 5739-5745)
+    params[0] = (double)(c.g.nx - 2) * c.s.h / 4.0;
+    params[1] = c.s.y0;
+    params[2] = c.s.z0;
+    params[3] = c.s.radius;
+    params[4] = 0.0;
+    params[5] = c.s.yvel1;
+    params[6] = c.s.zvel1;
+    return 0;
 }
 int dg_checktime(const int *t)
 {
@@ -577,7 +646,8 @@ int dg_step(const int *t)
     Ctx &c = *ctx();
     if (op_vofmethod(c, *t, true)) return -1;
     if (c.s.hswitch == 1 && op_calch3d(c)) return -1;  // main.f:30
-    if (c.s.iturb == 3 && op_kepsmodel(c, *t)) return -1;
+    if ((c.s.iturb == 1 || c.s.iturb == 2) && op_eddyvisc(c)) return -1;  // main.f:35
+    if ((c.s.iturb == 3 || c.s.iturb == 4) && op_kepsmodel(c, *t)) return -1;  // main.f:39-40
     if (op_checktime(c, *t)) return -1;
     if (c.s.surfacet == 2 && op_csf(c, false)) return -1;
     if (c.s.surfacet == 5 && op_sgip(c, false)) return -1;

@@ -56,6 +56,10 @@ struct Ctx {
     // CalcH3d's interface heights (/NEWW/, input.h:90-95); allocated with
     // hswitch = 1 or SGIP/SGIP2 surface tension, only the interior written
     double *hxmin = nullptr, *hxmax = nullptr, *hymin = nullptr, *hymax = nullptr, *hzmin = nullptr, *hzmax = nullptr;
+    // uv2p's Reynolds stresses (/TURBULENCE/, input.h:36-43); allocated
+    // with iturb = 4, zero otherwise in the reference (nondim main.f:436-441)
+    double *u2p = nullptr, *v2p = nullptr, *w2p = nullptr, *uvp = nullptr, *uwp = nullptr, *vwp = nullptr;
+    double *lmix = nullptr;    // eddyvisc's mixing length lmix(NMX): nx doubles, then nx of scratch
     double *pprev = nullptr;   // pressure of the previous step (tolerance-mode guess)
     double *pcg_r = nullptr, *pcg_p = nullptr, *pcg_q = nullptr;  // pcg_p: the ring of search directions
     double *red_partial = nullptr;  // RED_BLOCKS doubles
@@ -111,6 +115,11 @@ int op_cfl(Ctx &c, double *out_host);
 int op_checktime(Ctx &c, int t);
 int op_csf(Ctx &c, bool w_interior = true);
 int op_calch3d(Ctx &c);                           // CalcH3d main.f:1312-1810
+int op_eddyvisc(Ctx &c);                          // eddyvisc main.f:1828-1885 (iturb 1, 2)
+int op_uv2p(Ctx &c);                              // uv2p main.f:2032-2155 (iturb 4)
+// smooth2 (main.f:5678-5752) for n droplets: centres, radii (nondimensional)
+// and liquid velocities in device-visible host order (x0, y0, z0, r, u, v, w)
+int op_droplets(Ctx &c, int n, const double *params);
 int op_sgip(Ctx &c, bool w_interior = true);      // SGIP main.f:2835-3027
 int op_sgip2(Ctx &c, bool w_interior = true);     // SGIP2 main.f:3029-3166
 int op_solvelpres(Ctx &c, int t);

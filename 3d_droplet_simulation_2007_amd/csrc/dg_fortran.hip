@@ -256,6 +256,10 @@ double *host_field(int id)
     case DG_F_HYMAX: return h.neww ? h.neww + 2 * h.N : nullptr;
     case DG_F_HZMIN: return h.neww ? h.neww + 5 * h.N : nullptr;
     case DG_F_HZMAX: return h.neww ? h.neww + 4 * h.N : nullptr;
+    // /TURBULENCE/ u2p, v2p, w2p, uvp, uwp, vwp follow cmu (input.h:36-43)
+    case DG_F_U2P: case DG_F_V2P: case DG_F_W2P: case DG_F_UVP: case DG_F_UWP: case DG_F_VWP:
+        return h.cmu() + (size_t)(id - DG_F_U2P + 1) * h.N;
+    case DG_F_CMU: return h.cmu();
     default: return nullptr;
     }
 }
@@ -269,7 +273,6 @@ double *resident_of(const double *host)
         const double *hp = host_field(id);
         if (hp && hp == host) return resident_field(id);
     }
-    if (host == g_hc.cmu()) return ctx()->cmu;
     return nullptr;
 }
 
@@ -410,21 +413,25 @@ int copy_heap(bool to_device)
     return 0;
 }
 
+// bit of a field id in the sync masks: ids below DG_F_U2P their own bit,
+// cmu bit 29, the six Reynolds-stress arrays bit 31 (bit 30: the heap)
+unsigned sync_bit(int id)
+{
+    if (id < DG_F_U2P) return 1u << id;
+    return id == DG_F_CMU ? 1u << 29 : 1u << 31;
+}
+
 int copy_fields(unsigned mask, bool to_device)
 {
     Ctx &c = *ctx();
     DG_HIP(hipStreamSynchronize(c.stream));
     const size_t bytes = g_hc.N * sizeof(double);
     for (int id = 0; id < DG_NFIELDS; id++) {
-        if (!(mask & (1u << id))) continue;
+        if (!(mask & sync_bit(id))) continue;
         double *hp = host_field(id), *d = resident_field(id);
         if (!hp || !d) continue;
         if (to_device) DG_HIP(hipMemcpy(d, hp, bytes, hipMemcpyHostToDevice));
         else DG_HIP(hipMemcpy(hp, d, bytes, hipMemcpyDeviceToHost));
-    }
-    if (mask & (1u << DG_NFIELDS)) {
-        if (to_device) DG_HIP(hipMemcpy(c.cmu, g_hc.cmu(), bytes, hipMemcpyHostToDevice));
-        else DG_HIP(hipMemcpy(g_hc.cmu(), c.cmu, bytes, hipMemcpyDeviceToHost));
     }
     return 0;
 }
@@ -446,8 +453,9 @@ int printout_fields_to_host(int t)
     return m ? copy_fields(m, false) : 0;
 }
 
-constexpr unsigned ALL_FIELDS = (1u << (DG_NFIELDS + 1)) - 1;  // every field id and cmu
-constexpr unsigned HEAP_BIT = 1u << 30;                         // every level's heap slots
+constexpr unsigned HEAP_BIT = 1u << 30;        // every level's heap slots
+constexpr unsigned ALL_FIELDS = ~HEAP_BIT;     // every field id, cmu and the Reynolds stresses
+static_assert(DG_F_U2P == 29, "sync-mask bits 0-28 are the field ids below the Reynolds stresses");
 
 int attach()
 {
@@ -463,6 +471,7 @@ int attach()
     resident_scalars_changed();
     if ((ctx()->s.hswitch == 1 || ctx()->s.surfacet == 5 || ctx()->s.surfacet == 6) && alloc_heights(*ctx()))
         return -1;
+    if (ctx()->s.iturb == 4 && alloc_reynolds(*ctx())) return -1;
     if (copy_heap(true) || copy_fields(ALL_FIELDS, true)) return -1;
     g_attached = true;
     return 0;
@@ -541,8 +550,9 @@ void dg_init_input_(void)
 }
 
 // Fields to host memory before the program's own readers.  mask: bit id
-// for each DG_F_* field with a host array, bit DG_NFIELDS for cmu, bit 30
-// every level's heap slots; -1 all of them.
+// for each DG_F_* field id below DG_F_U2P with a host array, bit 29 cmu,
+// bit 31 the six Reynolds stresses, bit 30 every level's heap slots; -1
+// all of them.
 void dg_sync_to_host_(const int *mask)
 {
     if (!enter()) { fail("dg_sync_to_host"); return; }
@@ -630,6 +640,25 @@ void solvelpres_(double *cc, double *w1, double *w2, double *w3, double *a, doub
         if (op_solvelpres(cx, *t)) return -1;
         return printout_fields_to_host(*t + 1);
     });
+}
+
+// eddyvisc(cc, ufilt), main.f:1828-1885; called with z(icc(ng)), z(iu(ng))
+// (main.f:35): the filtered velocity goes into the pressure slot
+void eddyvisc_(double *cc, double *ufilt)
+{
+    call_form("eddyvisc", [&](Ctx &c) {
+        Level &L = finest();
+        if (expect_slot("eddyvisc", 1, cc, L.cc, "z(icc(ng))") ||
+            expect_slot("eddyvisc", 2, ufilt, L.u, "z(iu(ng))"))
+            return -1;
+        return op_eddyvisc(c);
+    });
+}
+
+// uv2p(), main.f:2032-2155 (called by kepsmodel with iturb = 4, main.f:2023)
+void uv2p_(void)
+{
+    call_form("uv2p", [&](Ctx &c) { return op_uv2p(c); });
 }
 
 // CalcH3d(cc), main.f:1312-1810 (called with z(icc(ng)), main.f:30)

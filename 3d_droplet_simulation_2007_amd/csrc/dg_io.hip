@@ -16,9 +16,10 @@
 //  - the full state, for a bit-exact resume (no reference counterpart).  It
 //    holds every scalar of the path and every field that carries
 //    information from one step to the next: velocity, stresses, densities,
-//    viscosities, turbulence fields, interface heights and the previous
-//    pressure.  It also holds all eight multigrid heap arrays of every
-//    level, because the reference reads stale values of its scratch slots.
+//    viscosities, turbulence fields (with cmu and the Reynolds stresses),
+//    interface heights and the previous pressure.  It also holds all eight
+//    multigrid heap arrays of every level, because the reference reads
+//    stale values of its scratch slots.
 //    Layout, little-endian: "DGSTATE1"; int32 nx, ny, nz, ng, nd, ni; nd
 //    doubles and ni int32 scalars; an int32 count of arrays; then per array
 //    an int32 tag (a DG_F_* id, 999 for cmu, 1000 + 8 * level + slot for the
@@ -64,10 +65,11 @@ void scalars_in(Scalars &s, const std::vector<double> &d, const std::vector<int3
     s.iturb = i[q++]; s.convectype = i[q++]; s.surfacet = i[q++]; s.hswitch = i[q++]; s.tprint = i[q++];
 }
 
-// the fields a state file holds (absent height arrays are skipped)
+// the fields a state file holds (absent height and Reynolds-stress arrays are skipped)
 const int STATE_FIELDS[] = {DG_F_U, DG_F_V, DG_F_W, DG_F_S11, DG_F_S22, DG_F_S33, DG_F_S12, DG_F_S13, DG_F_S23,
                             DG_F_RHON, DG_F_MUN, DG_F_MUT, DG_F_KTUR, DG_F_EPS, DG_F_PPREV, DG_F_HXMIN, DG_F_HXMAX,
-                            DG_F_HYMIN, DG_F_HYMAX, DG_F_HZMIN, DG_F_HZMAX};
+                            DG_F_HYMIN, DG_F_HYMAX, DG_F_HZMIN, DG_F_HZMAX, DG_F_U2P, DG_F_V2P, DG_F_W2P,
+                            DG_F_UVP, DG_F_UWP, DG_F_VWP};
 constexpr int TAG_CMU = 999;
 constexpr int TAG_HEAP = 1000;
 
@@ -77,6 +79,11 @@ bool is_height(int id)
            id == DG_F_HZMAX;
 }
 
+bool is_reynolds(int id) { return id >= DG_F_U2P && id <= DG_F_VWP; }
+
+// arrays allocated on first use: present in a state file, allocated on load
+bool is_lazy(int id) { return is_height(id) || is_reynolds(id); }
+
 // heap slot `slot` of level l, in /pointers/ order: rhs, res, u, cc, a, b, c, ei
 double *heap_array(Ctx &c, int l, int slot)
 {
@@ -85,7 +92,7 @@ double *heap_array(Ctx &c, int l, int slot)
     return a[slot];
 }
 
-// device array of a state-file tag (heights resolve once allocated)
+// device array of a state-file tag (lazily allocated arrays resolve once allocated)
 double *tag_array(Ctx &c, int tag, size_t *len)
 {
     *len = c.g.n();
@@ -403,8 +410,8 @@ int dg_load_state(const char *path)
             return -1;
         }
         size_t want = 0;
-        const bool known = is_height(tag) || tag_array(c, tag, &want) != nullptr;
-        if (is_height(tag)) want = c.g.n();
+        const bool known = is_lazy(tag) || tag_array(c, tag, &want) != nullptr;
+        if (is_lazy(tag)) want = c.g.n();
         if (!known || len < 0 || (size_t)len != want) {
             set_error("state file %s: array tag %d of %lld values does not fit this run", path, tag, (long long)len);
             return -1;
@@ -417,7 +424,7 @@ int dg_load_state(const char *path)
         arrs.push_back({tag, std::move(data)});
     }
     for (auto &a : arrs)
-        if (is_height(a.first) && alloc_heights(c)) return -1;
+        if ((is_height(a.first) && alloc_heights(c)) || (is_reynolds(a.first) && alloc_reynolds(c))) return -1;
     for (auto &a : arrs) {
         size_t len = 0;
         double *dst = tag_array(c, a.first, &len);

@@ -32,6 +32,8 @@ int init_from_input(const char *path, Input *in_out);
 int resident_setup(int nx, int ny, int nz);
 // CalcH3d's six interface-height arrays (zeroed), once
 int alloc_heights(Ctx &c);
+// uv2p's six Reynolds-stress arrays (zeroed), once
+int alloc_reynolds(Ctx &c);
 // recomputes what depends on the physical scalars (after they are set)
 void resident_scalars_changed();
 bool resident_ready();

@@ -689,7 +689,11 @@ int op_vofmethod(Ctx &c, int t, bool step)
 
 int op_kepsmodel(Ctx &c, int t)
 {
-    (void)t;
+    if (c.s.iturb == 4 && alloc_reynolds(c)) return -1;
+    // the production term from the Reynolds stresses (iturb = 4, after the
+    // first step) or from the eddy viscosity (main.f:1955-1969)
+    const bool rey = c.s.iturb == 4 && t != 0;
+    const double *u2p = c.u2p, *v2p = c.v2p, *w2p = c.w2p, *uvp = c.uvp, *uwp = c.uwp, *vwp = c.vwp;
     const Grid g = c.g;
     Level &L = c.lev[c.ng - 1];
     double *kn = L.u, *en = L.res;
@@ -737,9 +741,14 @@ int op_kepsmodel(Ctx &c, int t)
             double dwdy = 0.25 * (W(0, 0, 0) - W(0, -1, 0) + W(0, 0, 1) - W(0, -1, 1) + W(0, 1, 0) - W(0, 0, 0) +
                                   W(0, 1, 1) - W(0, 0, 1));
             double dwdz = W(0, 0, 1) - W(0, 0, 0);
-            prod = F(mut, 0, 0, 0) / F(rh, 0, 0, 0) / h2 *
-                   (2.0 * (dudx * dudx + dvdy * dvdy + dwdz * dwdz + dvdx * dudy + dwdx * dudz + dvdz * dwdy) +
-                    dudy * dudy + dudz * dudz + dvdx * dvdx + dvdz * dvdz + dwdx * dwdx + dwdy * dwdy);
+            if (!rey)
+                prod = F(mut, 0, 0, 0) / F(rh, 0, 0, 0) / h2 *
+                       (2.0 * (dudx * dudx + dvdy * dvdy + dwdz * dwdz + dvdx * dudy + dwdx * dudz + dvdz * dwdy) +
+                        dudy * dudy + dudz * dudz + dvdx * dvdx + dvdz * dvdz + dwdx * dwdx + dwdy * dwdy);
+            else
+                prod = -(u2p[o] * dudx + v2p[o] * dvdy + w2p[o] * dwdz + uvp[o] * (dudy + dvdx) +
+                         uwp[o] * (dudz + dwdx) + vwp[o] * (dvdz + dwdy)) /
+                       h;
             const double knew = F(kt, 0, 0, 0) + tau * (-conv + diff + prod - F(ep, 0, 0, 0));
             conv = (+dmax1(U(1, 0, 0), 0.0) * F(ep, 0, 0, 0) + dmin1(U(1, 0, 0), 0.0) * F(ep, 1, 0, 0) -
                     dmax1(U(0, 0, 0), 0.0) * F(ep, -1, 0, 0) - dmin1(U(0, 0, 0), 0.0) * F(ep, 0, 0, 0) +
@@ -781,7 +790,176 @@ int op_kepsmodel(Ctx &c, int t)
         return -1;
     if (op_bc_scalar(g, c.stream, c.ktur)) return -1;
     if (op_bc_scalar(g, c.stream, c.eps)) return -1;
-    return op_bc_scalar(g, c.stream, c.mut);
+    if (op_bc_scalar(g, c.stream, c.mut)) return -1;
+    return c.s.iturb == 4 ? op_uv2p(c) : 0;  // main.f:2023
+}
+
+// eddyvisc (main.f:1828-1885), iturb = 1, 2: u filtered into the finest
+// pressure slot, which the reference passes as ufilt (main.f:35); the
+// mixing length of each x station from the filtered profile through
+// (j1, k1); mut from it and the y and z shear of u.
+//
+// The station scan carries j2 from one station to the next when the second
+// branch finds no match (main.f:1855-1861), so one workgroup finds each
+// station's first matching j in parallel and one thread then walks the
+// stations in order.  j2 is uninitialised in the reference before that
+// branch's first station; it starts at 0 here.
+__global__ void __launch_bounds__(256) k_eddyvisc_lmix(Grid g, const double *uf, int j1, int k1, int jet,
+                                                       double xvel2, double alfa_h, double *lmix, int *found)
+{
+    for (int i = (int)threadIdx.x + 1; i <= g.nx; i += (int)blockDim.x) {
+        const double centre = uf[g.at(i, j1, k1)];
+        const double umax = jet ? centre : xvel2, umin = jet ? xvel2 : centre;
+        const double lim = DG_F4(0.1) * (umax - umin);
+        int f = 0;
+        for (int j = 2; j <= g.ny - 1; j++) {
+            const double d = uf[g.at(i, j, k1)] - umin;
+            if (jet ? d > lim : d <= lim) {
+                f = j;
+                break;
+            }
+        }
+        found[i - 1] = f;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int j2 = 0;
+        for (int i = 1; i <= g.nx; i++) {
+            if (jet) j2 = found[i - 1];
+            else if (found[i - 1]) j2 = found[i - 1];
+            if (j2 == 2) j2 = j1;
+            lmix[i - 1] = alfa_h * (double)abs(j2 - j1);
+        }
+    }
+}
+
+int op_eddyvisc(Ctx &c)
+{
+    const Grid g = c.g;
+    double *uf = c.lev[c.ng - 1].u;
+    const double *u = c.u;
+    const int sy = g.sy, sz = g.sz;
+    // filter (main.f:1271-1284)
+    if (launch_interior(g, c.stream, [=] __device__(int i, int j, int k) {
+            const int o = g.at(i, j, k);
+            uf[o] = 0.25 * u[o] + 0.125 * (u[o - sz] + u[o + sz] + u[o - sy] + u[o + sy]) +
+                    0.0625 * (u[o - sy - sz] + u[o - sy + sz] + u[o + sy - sz] + u[o + sy + sz]);
+        }))
+        return -1;
+    const double h = c.s.h;
+    const int j1 = (int)(c.s.y0 / h), k1 = (int)(c.s.z0 / h);
+    if (j1 < 1 || j1 > g.ny || k1 < 1 || k1 > g.nz) {
+        set_error("eddyvisc: centre station (j1, k1) = (%d, %d) is outside the %d x %d x %d grid (main.f:1841-1842)",
+                  j1, k1, g.nx, g.ny, g.nz);
+        return -1;
+    }
+    // the stations' first matches, as ints in the second half of lmix's allocation
+    int *found = reinterpret_cast<int *>(c.lmix + g.nx);
+    hipLaunchKernelGGL(k_eddyvisc_lmix, dim3(1), dim3(256), 0, c.stream, g, uf, j1, k1, c.s.xvel1 > c.s.xvel2 ? 1 : 0,
+                       c.s.xvel2, DG_F4(0.08) * h, c.lmix, found);
+    DG_CHECK_LAUNCH();
+    const double *rh = c.rhon, *lmix = c.lmix;
+    double *mut = c.mut;
+    const double cmu1 = DG_F4(1.0);
+    if (launch_interior(g, c.stream, [=] __device__(int i, int j, int k) {
+            const int o = g.at(i, j, k);
+            const double dudy = 0.25 * (u[o] - u[o - sy] + u[o + 1] - u[o + 1 - sy] + u[o + sy] - u[o] +
+                                        u[o + 1 + sy] - u[o + 1]) /
+                                h;
+            const double dudz = 0.25 * (u[o] - u[o - sz] + u[o + 1] - u[o + 1 - sz] + u[o + sz] - u[o] +
+                                        u[o + 1 + sz] - u[o + 1]) /
+                                h;
+            const double lm = lmix[i - 1] + lmix[i];
+            mut[o] = cmu1 * rh[o] * 0.25 * lm * lm * sqrt(dudy * dudy + dudz * dudz);
+        }))
+        return -1;
+    return op_bc_scalar(g, c.stream, mut);
+}
+
+// uv2p (main.f:2032-2155), iturb = 4: Shih-Zhu-Lumley eddy viscosity and
+// Reynolds stresses.  The reference's REAL constants are rounded to single
+// precision: 1./3., 2./3., 1.E-30 and sqrt(6.).
+int op_uv2p(Ctx &c)
+{
+    if (alloc_reynolds(c)) return -1;
+    const Grid g = c.g;
+    const double *u = c.u, *v = c.v, *w = c.w, *kt = c.ktur, *ep = c.eps, *rh = c.rhon;
+    double *cmu = c.cmu, *mut = c.mut, *u2p = c.u2p, *v2p = c.v2p, *w2p = c.w2p, *uvp = c.uvp, *uwp = c.uwp,
+           *vwp = c.vwp;
+    const double h = c.s.h;
+    const double third = (double)(1.0f / 3.0f), twothird = (double)(2.0f / 3.0f), tiny = DG_F4(1.E-30);
+    const double sixsq = (double)sqrtf(6.0f);
+    const int sy = g.sy, sz = g.sz;
+    if (launch_interior(g, c.stream, [=] __device__(int i, int j, int k) {
+            const int o = g.at(i, j, k);
+#define U(di, dj, dk) u[o + (di) + sy * (dj) + sz * (dk)]
+#define V(di, dj, dk) v[o + (di) + sy * (dj) + sz * (dk)]
+#define W(di, dj, dk) w[o + (di) + sy * (dj) + sz * (dk)]
+            const double keps = kt[o] / ep[o];
+            const double k2eps = keps * kt[o];
+            const double k2eps2 = keps * keps;
+            const double k3eps2 = k2eps2 * kt[o];
+            const double dudx = (U(1, 0, 0) - U(0, 0, 0)) / h;
+            const double dudy = 0.25 * (U(0, 0, 0) - U(0, -1, 0) + U(1, 0, 0) - U(1, -1, 0) + U(0, 1, 0) - U(0, 0, 0) +
+                                        U(1, 1, 0) - U(1, 0, 0)) /
+                                h;
+            const double dudz = 0.25 * (U(0, 0, 0) - U(0, 0, -1) + U(1, 0, 0) - U(1, 0, -1) + U(0, 0, 1) - U(0, 0, 0) +
+                                        U(1, 0, 1) - U(1, 0, 0)) /
+                                h;
+            const double dvdx = 0.25 * (V(0, 0, 0) - V(-1, 0, 0) + V(1, 0, 0) - V(0, 0, 0) + V(0, 1, 0) - V(-1, 1, 0) +
+                                        V(1, 1, 0) - V(0, 1, 0)) /
+                                h;
+            const double dvdy = (V(0, 1, 0) - V(0, 0, 0)) / h;
+            const double dvdz = 0.25 * (V(0, 0, 0) - V(0, 0, -1) + V(0, 0, 1) - V(0, 0, 0) + V(0, 1, 0) - V(0, 1, -1) +
+                                        V(0, 1, 1) - V(0, 1, 0)) /
+                                h;
+            const double dwdx = 0.25 * (W(0, 0, 0) - W(-1, 0, 0) + W(1, 0, 0) - W(0, 0, 0) + W(0, 0, 1) - W(-1, 0, 1) +
+                                        W(1, 0, 1) - W(0, 0, 1)) /
+                                h;
+            const double dwdy = 0.25 * (W(0, 0, 0) - W(0, -1, 0) + W(0, 0, 1) - W(0, -1, 1) + W(0, 1, 0) - W(0, 0, 0) +
+                                        W(0, 1, 1) - W(0, 0, 1)) /
+                                h;
+            const double dwdz = (W(0, 0, 1) - W(0, 0, 0)) / h;
+#undef U
+#undef V
+#undef W
+            const double ss11 = dudx, ss12 = 0.5 * (dudy + dvdx), ss13 = 0.5 * (dudz + dwdx), ss22 = dvdy,
+                         ss23 = 0.5 * (dvdz + dwdy), ss33 = dwdz;
+            const double sskk = ss11 + ss22 + ss33;
+            const double sstar11 = ss11 - third * sskk, sstar22 = ss22 - third * sskk, sstar33 = ss33 - third * sskk;
+            const double sstar = sqrt(sstar11 * sstar11 + sstar22 * sstar22 + sstar33 * sstar33 + 2.0 * ss12 * ss12 +
+                                      2.0 * ss13 * ss13 + 2.0 * ss23 * ss23 + tiny);
+            const double omeg12 = 0.5 * (dudy - dvdx), omeg13 = 0.5 * (dudz - dwdx), omeg23 = 0.5 * (dvdz - dwdy);
+            const double omegastar = sqrt(2.0 * omeg12 * omeg12 + 2.0 * omeg13 * omeg13 + 2.0 * omeg23 * omeg23);
+            const double wstar = (sstar11 * sstar11 * sstar11 + sstar22 * sstar22 * sstar22 + sstar33 * sstar33 * sstar33 +
+                                  3.0 * sstar11 * ss12 * ss12 + 3.0 * ss12 * ss12 * sstar22 + 3.0 * sstar11 * ss13 * ss13 +
+                                  6.0 * ss12 * ss13 * ss23 + 3.0 * ss13 * ss13 * sstar33 + 3.0 * sstar22 * ss23 * ss23 +
+                                  3.0 * ss23 * ss23 * sstar33) /
+                                 sstar / sstar / sstar;
+            const double ustar = sqrt(sstar * sstar + omegastar * omegastar);
+            double sixsqws = sixsq * wstar;
+            if ((1.0 - fabs(sixsqws)) < 1.e-4) sixsqws = sixsqws < 0.0 ? -1.0 : 1.0;
+            const double asstar = sixsq * cos(acos(sixsqws) / 3.0);
+            const double cm = 1.0 / (6.5 + asstar * ustar * keps);
+            cmu[o] = cm;
+            mut[o] = cm * rh[o] * kt[o] * kt[o] / ep[o];
+            double c2 = 1.0 - 9.0 * cm * cm * sstar * sstar * k2eps2;
+            if (c2 < 0.0) c2 = 0.0;
+            c2 = sqrt(c2) / (1.0 + 6.0 * sstar * k2eps2 * omegastar);
+            u2p[o] = twothird * kt[o] - 2.0 * cm * k2eps * sstar11 + 4.0 * c2 * k3eps2 * (ss12 * omeg12 + ss13 * omeg13);
+            v2p[o] = twothird * kt[o] - 2.0 * cm * k2eps * sstar22 - 4.0 * c2 * k3eps2 * (ss12 * omeg12 - ss23 * omeg23);
+            w2p[o] = twothird * kt[o] - 2.0 * cm * k2eps * sstar33 - 4.0 * c2 * k3eps2 * (ss13 * omeg13 + ss23 * omeg23);
+            uvp[o] = -(2.0 * cm * k2eps * ss12) +
+                     2.0 * c2 * k3eps2 * (-(sstar11 * omeg12) + sstar22 * omeg12 + ss13 * omeg23 + ss23 * omeg13);
+            uwp[o] = -(2.0 * cm * k2eps * ss13) +
+                     2.0 * c2 * k3eps2 * (-(sstar11 * omeg13) + sstar33 * omeg13 - ss12 * omeg23 + ss23 * omeg12);
+            vwp[o] = -(2.0 * cm * k2eps * ss23) +
+                     2.0 * c2 * k3eps2 * (-(sstar22 * omeg23) + sstar33 * omeg23 - ss12 * omeg13 - ss13 * omeg12);
+        }))
+        return -1;
+    for (double *a : {mut, u2p, v2p, w2p, uvp, uwp, vwp})
+        if (op_bc_scalar(g, c.stream, a)) return -1;
+    return 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1587,11 +1765,131 @@ static int launch_convective_col(const Grid &g, hipStream_t s, const double *w1,
     return 0;
 }
 
+// the 16 Reynolds-stress terms of one component, which the reference
+// writes with alternating signs (+t0 - t1 + t2 ...): added one at a time to
+// acc in that order
+__device__ inline double add_alternating(double acc, const double (&t)[16], int first)
+{
+    for (int q = first; q < 16; q++) acc = (q & 1) ? acc - t[q] : acc + t[q];
+    return acc;
+}
+
+// convective with the Reynolds-stress terms of the anisotropic model
+// (iturb = 4, main.f:3469-3472, 3492-3495, 3513-3516 and 3536-3563), in the
+// reference's order of evaluation: in the upwind form the terms are a
+// parenthesised group of their own, in the central form they continue the
+// 0.25 * (...) sum; one cell per thread
+static int convective_reynolds(Ctx &c, const double *w1, const double *w2, const double *w3)
+{
+    const Grid g = c.g;
+    double *u = c.u, *v = c.v, *w = c.w;
+    const double tauh = c.s.tauh;
+    const double *u2p = c.u2p, *v2p = c.v2p, *w2p = c.w2p, *uvp = c.uvp, *uwp = c.uwp, *vwp = c.vwp;
+    const int sy = g.sy, sz = g.sz;
+    const bool upwind = c.s.convectype == 1;
+    return launch_interior(g, c.stream, [=] __device__(int i, int j, int k) {
+        const int o = g.at(i, j, k);
+#define A(a, di, dj, dk) a[o + (di) + sy * (dj) + sz * (dk)]
+        const double ru[16] = {A(uvp, 0, 0, 0),  A(uvp, 0, 1, 0),   A(uvp, 0, -1, 0),  A(uvp, 0, 0, 0),
+                               A(uvp, -1, 0, 0), A(uvp, -1, 1, 0),  A(uvp, -1, -1, 0), A(uvp, -1, 0, 0),
+                               A(uwp, 0, 0, 0),  A(uwp, 0, 0, 1),   A(uwp, 0, 0, -1),  A(uwp, 0, 0, 0),
+                               A(uwp, -1, 0, 0), A(uwp, -1, 0, 1),  A(uwp, -1, 0, -1), A(uwp, -1, 0, 0)};
+        const double rv[16] = {A(uvp, -1, 0, 0),  A(uvp, 0, 0, 0),   A(uvp, 0, 0, 0),  A(uvp, 1, 0, 0),
+                               A(uvp, -1, -1, 0), A(uvp, 0, -1, 0),  A(uvp, 0, -1, 0), A(uvp, 1, -1, 0),
+                               A(vwp, 0, 0, -1),  A(vwp, 0, 0, 0),   A(vwp, 0, 0, 0),  A(vwp, 0, 0, 1),
+                               A(vwp, 0, -1, -1), A(vwp, 0, -1, 0),  A(vwp, 0, -1, 0), A(vwp, 0, -1, 1)};
+        const double rw[16] = {A(uwp, -1, 0, 0),  A(uwp, 0, 0, 0),   A(uwp, 0, 0, 0),  A(uwp, 1, 0, 0),
+                               A(uwp, -1, 0, -1), A(uwp, 0, 0, -1),  A(uwp, 0, 0, -1), A(uwp, 1, 0, -1),
+                               A(vwp, 0, -1, 0),  A(vwp, 0, 0, 0),   A(vwp, 0, 0, 0),  A(vwp, 0, 1, 0),
+                               A(vwp, 0, -1, -1), A(vwp, 0, 0, -1),  A(vwp, 0, 0, -1), A(vwp, 0, 1, -1)};
+        double nu, nv, nw;
+        if (upwind) {
+            double t1, t2, s1, s2, q1, q2;
+            t1 = .5 * (A(w2, 0, 0, 0) + A(w2, -1, 0, 0));
+            t2 = .5 * (A(w2, 0, 1, 0) + A(w2, -1, 1, 0));
+            s1 = .5 * (A(w1, -1, 0, 0) + A(w1, 0, 0, 0));
+            s2 = .5 * (A(w1, 0, 0, 0) + A(w1, 1, 0, 0));
+            q1 = .5 * (A(w3, 0, 0, 0) + A(w3, -1, 0, 0));
+            q2 = .5 * (A(w3, 0, 0, 1) + A(w3, -1, 0, 1));
+            nu = A(w1, 0, 0, 0) +
+                 tauh * (+dmax1(t1, 0.0) * A(w1, 0, -1, 0) + dmin1(t1, 0.0) * A(w1, 0, 0, 0) -
+                         dmax1(t2, 0.0) * A(w1, 0, 0, 0) - dmin1(t2, 0.0) * A(w1, 0, 1, 0) +
+                         dmax1(s1, 0.0) * A(w1, -1, 0, 0) + dmin1(s1, 0.0) * A(w1, 0, 0, 0) -
+                         dmax1(s2, 0.0) * A(w1, 0, 0, 0) - dmin1(s2, 0.0) * A(w1, 1, 0, 0) +
+                         dmax1(q1, 0.0) * A(w1, 0, 0, -1) + dmin1(q1, 0.0) * A(w1, 0, 0, 0) -
+                         dmax1(q2, 0.0) * A(w1, 0, 0, 0) - dmin1(q2, 0.0) * A(w1, 0, 0, 1) + A(u2p, -1, 0, 0) -
+                         A(u2p, 0, 0, 0) + 0.25 * add_alternating(ru[0], ru, 1));
+            t1 = .5 * (A(w1, 0, 0, 0) + A(w1, 0, -1, 0));
+            t2 = .5 * (A(w1, 1, 0, 0) + A(w1, 1, -1, 0));
+            s1 = .5 * (A(w2, 0, -1, 0) + A(w2, 0, 0, 0));
+            s2 = .5 * (A(w2, 0, 0, 0) + A(w2, 0, 1, 0));
+            q1 = .5 * (A(w3, 0, 0, 0) + A(w3, 0, -1, 0));
+            q2 = .5 * (A(w3, 0, 0, 1) + A(w3, 0, -1, 1));
+            nv = A(w2, 0, 0, 0) +
+                 tauh * (+dmax1(t1, 0.0) * A(w2, -1, 0, 0) + dmin1(t1, 0.0) * A(w2, 0, 0, 0) -
+                         dmax1(t2, 0.0) * A(w2, 0, 0, 0) - dmin1(t2, 0.0) * A(w2, 1, 0, 0) +
+                         dmax1(s1, 0.0) * A(w2, 0, -1, 0) + dmin1(s1, 0.0) * A(w2, 0, 0, 0) -
+                         dmax1(s2, 0.0) * A(w2, 0, 0, 0) - dmin1(s2, 0.0) * A(w2, 0, 1, 0) +
+                         dmax1(q1, 0.0) * A(w2, 0, 0, -1) + dmin1(q1, 0.0) * A(w2, 0, 0, 0) -
+                         dmax1(q2, 0.0) * A(w2, 0, 0, 0) - dmin1(q2, 0.0) * A(w2, 0, 0, 1) + A(v2p, 0, -1, 0) -
+                         A(v2p, 0, 0, 0) + 0.25 * add_alternating(rv[0], rv, 1));
+            t1 = .5 * (A(w1, 0, 0, 0) + A(w1, 0, 0, -1));
+            t2 = .5 * (A(w1, 1, 0, 0) + A(w1, 1, 0, -1));
+            s1 = .5 * (A(w3, 0, 0, -1) + A(w3, 0, 0, 0));
+            s2 = .5 * (A(w3, 0, 0, 0) + A(w3, 0, 0, 1));
+            q1 = .5 * (A(w2, 0, 0, 0) + A(w2, 0, 0, -1));
+            q2 = .5 * (A(w2, 0, 1, 0) + A(w2, 0, 1, -1));
+            nw = A(w3, 0, 0, 0) +
+                 tauh * (+dmax1(t1, 0.0) * A(w3, -1, 0, 0) + dmin1(t1, 0.0) * A(w3, 0, 0, 0) -
+                         dmax1(t2, 0.0) * A(w3, 0, 0, 0) - dmin1(t2, 0.0) * A(w3, 1, 0, 0) +
+                         dmax1(s1, 0.0) * A(w3, 0, 0, -1) + dmin1(s1, 0.0) * A(w3, 0, 0, 0) -
+                         dmax1(s2, 0.0) * A(w3, 0, 0, 0) - dmin1(s2, 0.0) * A(w3, 0, 0, 1) +
+                         dmax1(q1, 0.0) * A(w3, 0, -1, 0) + dmin1(q1, 0.0) * A(w3, 0, 0, 0) -
+                         dmax1(q2, 0.0) * A(w3, 0, 0, 0) - dmin1(q2, 0.0) * A(w3, 0, 1, 0) + A(w2p, 0, 0, -1) -
+                         A(w2p, 0, 0, 0) + 0.25 * add_alternating(rw[0], rw, 1));
+        } else {
+            nu = A(w1, 0, 0, 0) +
+                 tauh * (0.25 * add_alternating(
+                                    sq(A(w1, 0, 0, 0) + A(w1, -1, 0, 0)) - sq(A(w1, 1, 0, 0) + A(w1, 0, 0, 0)) +
+                                        (A(w1, 0, -1, 0) + A(w1, 0, 0, 0)) * (A(w2, 0, 0, 0) + A(w2, -1, 0, 0)) -
+                                        (A(w1, 0, 1, 0) + A(w1, 0, 0, 0)) * (A(w2, 0, 1, 0) + A(w2, -1, 1, 0)) +
+                                        (A(w1, 0, 0, -1) + A(w1, 0, 0, 0)) * (A(w3, 0, 0, 0) + A(w3, -1, 0, 0)) -
+                                        (A(w1, 0, 0, 1) + A(w1, 0, 0, 0)) * (A(w3, 0, 0, 1) + A(w3, -1, 0, 1)),
+                                    ru, 0) +
+                         A(u2p, -1, 0, 0) - A(u2p, 0, 0, 0));
+            nv = A(w2, 0, 0, 0) +
+                 tauh * (0.25 * add_alternating(
+                                    sq(A(w2, 0, 0, 0) + A(w2, 0, -1, 0)) - sq(A(w2, 0, 1, 0) + A(w2, 0, 0, 0)) +
+                                        (A(w2, -1, 0, 0) + A(w2, 0, 0, 0)) * (A(w1, 0, 0, 0) + A(w1, 0, -1, 0)) -
+                                        (A(w2, 1, 0, 0) + A(w2, 0, 0, 0)) * (A(w1, 1, 0, 0) + A(w1, 1, -1, 0)) +
+                                        (A(w2, 0, 0, -1) + A(w2, 0, 0, 0)) * (A(w3, 0, 0, 0) + A(w3, 0, -1, 0)) -
+                                        (A(w2, 0, 0, 1) + A(w2, 0, 0, 0)) * (A(w3, 0, 0, 1) + A(w3, 0, -1, 1)),
+                                    rv, 0) +
+                         A(v2p, 0, -1, 0) - A(v2p, 0, 0, 0));
+            nw = A(w3, 0, 0, 0) +
+                 tauh * (0.25 * add_alternating(
+                                    sq(A(w3, 0, 0, 0) + A(w3, 0, 0, -1)) - sq(A(w3, 0, 0, 1) + A(w3, 0, 0, 0)) +
+                                        (A(w3, -1, 0, 0) + A(w3, 0, 0, 0)) * (A(w1, 0, 0, 0) + A(w1, 0, 0, -1)) -
+                                        (A(w3, 1, 0, 0) + A(w3, 0, 0, 0)) * (A(w1, 1, 0, 0) + A(w1, 1, 0, -1)) +
+                                        (A(w3, 0, -1, 0) + A(w3, 0, 0, 0)) * (A(w2, 0, 0, 0) + A(w2, 0, 0, -1)) -
+                                        (A(w3, 0, 1, 0) + A(w3, 0, 0, 0)) * (A(w2, 0, 1, 0) + A(w2, 0, 1, -1)),
+                                    rw, 0) +
+                         A(w2p, 0, 0, -1) - A(w2p, 0, 0, 0));
+        }
+#undef A
+        u[o] = nu;
+        v[o] = nv;
+        w[o] = nw;
+    });
+}
+
 static int convective(Ctx &c, const double *w1, const double *w2, const double *w3)
 {
     const Grid g = c.g;
     double *u = c.u, *v = c.v, *w = c.w;
     const double tauh = c.s.tauh;
+    if (c.s.iturb == 4 && c.u2p && (c.s.convectype == 1 || c.s.convectype == 2))
+        return convective_reynolds(c, w1, w2, w3);
     if (c.s.convectype == 1) {
         return launch_convective_col(g, c.stream, w1, w2, w3, [=] __device__(const CvWin &W, int o) {
 #define A1(di, dj, dk) W.p1[(dk) + 1][cv_i1(di, dj)]
@@ -1816,6 +2114,135 @@ int op_smooth2(Ctx &c)
         v[p] = yv1 * cc[p] + yv2 * (1.0 - cc[p]);
         w[p] = zv1 * cc[p] + zv2 * (1.0 - cc[p]);
     });
+}
+
+// smooth2 (main.f:5678-5752) for several droplets.  params holds 7 values
+// per droplet in the solver's units (lengths in units of length_SI, the
+// grid's coordinates xx(i) = (i - 1.5) h of smooth2; velocities in units of
+// vscale): centre x0, y0, z0, radius r, liquid velocity u, v, w.  A cell is
+// full when it lies inside one droplet, empty when it meets none, and
+// otherwise counts the 30^3 sub-points of smooth2 that lie in any droplet
+// (a point in two droplets counts once, for the first).  Velocities follow
+// smooth2's mixture rule with the velocity of the cell's droplet (the full
+// cell's, or the one holding most of its points; ghost cells take that of
+// the nearest interior cell): v = v_d cc + yvel2 (1 - cc), w likewise, and
+// u = u_d cc (smooth2 leaves u = 0, main.f:5741).  With one droplet at
+// smooth2's centre and (0, yvel1, zvel1) this is smooth2, bit for bit.
+constexpr int DROP_MAX_CANDIDATES = 8;
+
+int op_droplets(Ctx &c, int n, const double *params)
+{
+    const Grid g = c.g;
+    Level &L = c.lev[c.ng - 1];
+    double *cc = L.cc;
+    const double h = c.s.h;
+    double *dp = nullptr;
+    int *owner = nullptr, *flag = nullptr;
+    DG_HIP(hipMalloc(&dp, sizeof(double) * 7 * (size_t)n));
+    if (hipMalloc(&owner, sizeof(int) * g.n()) != hipSuccess || hipMalloc(&flag, sizeof(int)) != hipSuccess) {
+        (void)hipFree(dp);
+        if (owner) (void)hipFree(owner);
+        set_error("droplets: out of device memory for the owner map");
+        return -1;
+    }
+    int rc = 0;
+    auto fail = [&](const char *what) {
+        if (!rc) set_error("droplets: %s", what);
+        rc = -1;
+    };
+    if (hipMemcpy(dp, params, sizeof(double) * 7 * (size_t)n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemsetAsync(flag, 0, sizeof(int), c.stream) != hipSuccess)
+        fail("parameter upload");
+    if (!rc && launch_interior_rr(g, c.stream, [=] __device__(int i, int j, int k) {
+            const double xa = ((double)(i - 1) - 1.5) * h, xb = ((double)i - 1.5) * h;
+            const double ya = ((double)(j - 1) - 1.5) * h, yb = ((double)j - 1.5) * h;
+            const double za = ((double)(k - 1) - 1.5) * h, zb = ((double)k - 1.5) * h;
+            const int o = g.at(i, j, k);
+            int cand[DROP_MAX_CANDIDATES];
+            int nc = 0;
+            for (int d = 0; d < n; d++) {
+                const double x0 = dp[7 * d], y0 = dp[7 * d + 1], z0 = dp[7 * d + 2], r2 = dp[7 * d + 3] * dp[7 * d + 3];
+                const double dmin =
+                    dmin1(sq(xa - x0), sq(xb - x0)) + dmin1(sq(ya - y0), sq(yb - y0)) + dmin1(sq(za - z0), sq(zb - z0));
+                if (!(dmin < r2)) continue;
+                const double dmax =
+                    dmax1(sq(xa - x0), sq(xb - x0)) + dmax1(sq(ya - y0), sq(yb - y0)) + dmax1(sq(za - z0), sq(zb - z0));
+                if (!(dmax > r2)) {
+                    cc[o] = 1.0;
+                    owner[o] = d;
+                    return;
+                }
+                if (nc == DROP_MAX_CANDIDATES) {
+                    atomicOr(flag, 1);
+                    continue;
+                }
+                cand[nc++] = d;
+            }
+            if (nc == 0) {
+                cc[o] = 0.0;
+                owner[o] = -1;
+                return;
+            }
+            const int index = 30;
+            int cnt[DROP_MAX_CANDIDATES];
+            for (int q = 0; q < nc; q++) cnt[q] = 0;
+            for (int l = 1; l <= index; l++) {
+                const double tx = xa + ((double)l - .5) / (double)index * h;
+                for (int m = 1; m <= index; m++) {
+                    const double ty = ya + ((double)m - .5) / (double)index * h;
+                    for (int q3 = 1; q3 <= index; q3++) {
+                        const double tz = za + ((double)q3 - .5) / (double)index * h;
+                        for (int q = 0; q < nc; q++) {
+                            const int d = cand[q];
+                            const double px = tx - dp[7 * d], py = ty - dp[7 * d + 1], pz = tz - dp[7 * d + 2];
+                            if (sq(px) + sq(py) + sq(pz) < dp[7 * d + 3] * dp[7 * d + 3]) {
+                                cnt[q]++;
+                                break;
+                            }
+                        }
+                    }
+                }
+            }
+            int total = 0, best = 0;
+            for (int q = 0; q < nc; q++) {
+                total += cnt[q];
+                if (cnt[q] > cnt[best]) best = q;
+            }
+            const double inc = 1.0 / (double)(index * index * index);
+            double acc = 0.0;
+            for (int q = 0; q < total; q++) acc = acc + inc;
+            cc[o] = acc;
+            owner[o] = cand[best];
+        }))
+        fail("launch");
+    int hflag = 0;
+    if (!rc && hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, c.stream) != hipSuccess) fail("flag");
+    if (!rc && hipStreamSynchronize(c.stream) != hipSuccess) fail("synchronisation");
+    if (!rc && hflag)
+        fail("more than 8 droplets reach into one cell");
+    if (!rc && op_bc_c(c, cc)) rc = -1;
+    double *u = c.u, *v = c.v, *w = c.w;
+    const double yv2 = c.s.yvel2, zv2 = c.s.zvel2;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    if (!rc && launch_linear(g.n(), c.stream, [=] __device__(size_t p) {
+            const int i = (int)(p % (size_t)nx) + 1, j = (int)((p / (size_t)nx) % (size_t)ny) + 1,
+                      k = (int)(p / ((size_t)nx * ny)) + 1;
+            const int ii = i < 2 ? 2 : (i > nx - 1 ? nx - 1 : i), jj = j < 2 ? 2 : (j > ny - 1 ? ny - 1 : j),
+                      kk = k < 2 ? 2 : (k > nz - 1 ? nz - 1 : k);
+            const int d = owner[g.at(ii, jj, kk)];
+            const double ud = d >= 0 ? dp[7 * d + 4] : 0.0, vd = d >= 0 ? dp[7 * d + 5] : 0.0,
+                         wd = d >= 0 ? dp[7 * d + 6] : 0.0;
+            const double cv = cc[p];
+            u[p] = ud == 0.0 ? 0.0 : ud * cv;
+            v[p] = vd * cv + yv2 * (1.0 - cv);
+            w[p] = wd * cv + zv2 * (1.0 - cv);
+        }))
+        rc = -1;
+    if (hipStreamSynchronize(c.stream) != hipSuccess) fail("synchronisation");
+    (void)hipFree(dp);
+    (void)hipFree(owner);
+    (void)hipFree(flag);
+    return rc;
 }
 
 }  // namespace dg

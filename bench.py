@@ -58,22 +58,25 @@ ROOF_KERNEL = "k_rb_colour4<_Float16,*,true>"
 PMC_TRAFFIC = {(ROOF_KERNEL, 258): (5 * 232.1e6 + 167.7e6 + 251.9e6 + 246.6e6) / 8}
 
 
-def splash_inputdata(dg, n):
+def splash_inputdata(dg, n, **kw):
     # 6 mm cube; droplet radius 1 mm, its lowest point 1.5 mm above the wall
     # (centre 2.5 mm); the reference centres z and forces x0 = L/4.
     return dg.make_inputdata(nx=n, ny=n, nz=n, tmax=200, maxcfl=0.25, iturb=0, convectype=2, yvel1=-3.0,
                              rho2=1.2, rho1=1000.0, sigma=0.072, visc2=1.8e-5, visc1=1.0e-3, grav=-9.81,
-                             length=0.006, radius=0.001, y0=0.0025, z0=1000.0, cycles=2, npre=2, ntop=4, npost=2)
+                             length=0.006, radius=0.001, y0=0.0025, z0=1000.0, cycles=2, npre=2, ntop=4, npost=2,
+                             **kw)
 
 
 def run_path(dg, txt, mode, steps, warmup, tol, guess=1, profile_kernel=None, profile_steps=2, late_start=0,
-             late_steps=0):
+             late_steps=0, output_dir=None):
     """W warm-up steps, K timed steps (headline, no instrumentation), then,
     when profile_kernel is given, profile_steps more steps of the same run
     with HIP events around every launch of that kernel on the library's
     stream (graph replay is off while events are recorded), then, when
     late_steps > 0, untimed steps up to late_start and late_steps timed
-    steps from there."""
+    steps from there.  With output_dir, printout runs before every step (the
+    reference's loop, main.f:21-23) into that directory; the time to finish
+    the queued files after the timed steps is measured apart."""
     import numpy as np
     gs = importlib.import_module("3d_droplet_simulation_2007_amd.guess")
     g = dg.DropletGPU(inputdata_text=txt)
@@ -82,18 +85,34 @@ def run_path(dg, txt, mode, steps, warmup, tol, guess=1, profile_kernel=None, pr
     g.set_vof_guard(True)
     g.set_solver(mode=mode, tol=tol, nu1=2, nu2=2, maxit=200, guess=guess)
     vol0 = float(g.get("cc")[1:-1, 1:-1, 1:-1].sum())
+    if output_dir:
+        g.output_open(output_dir)
     for t in range(warmup):
+        if output_dir:
+            g.printout(t)
         g.step(t)
     g.state()  # synchronises the stream
+    if output_dir:
+        g.output_flush()
     gs0 = gs.stats()
+    o0 = g.output_stats()
     t0 = time.perf_counter()
     timed = []
     for t in range(warmup, warmup + steps):
+        if output_dir:
+            g.printout(t)
         g.step(t)
         s = g.state()
         timed.append((s["mg_iters"], s["mg_relres"]))
     st = g.state()  # synchronises
     t1 = time.perf_counter()
+    output = None
+    if output_dir:
+        g.output_flush()
+        t2 = time.perf_counter()
+        o1 = g.output_stats()
+        g.output_close()
+        output = dict(drain_ms=1e3 * (t2 - t1), **{k: o1[k] - o0[k] for k in o1})
     gs1 = gs.stats()
     guess_stats = {k: gs1[k] - gs0[k] for k in ("accepted", "rejected", "skipped")}
     ran = guess_stats["accepted"] + guess_stats["rejected"]
@@ -138,8 +157,10 @@ def run_path(dg, txt, mode, steps, warmup, tol, guess=1, profile_kernel=None, pr
         late = dict(first_step=t, steps=late_steps, ms_per_step=1e3 * (lt1 - lt0) / late_steps,
                     mg_iters_per_step=float(np.mean(its)))
     g.close()
+    print(f"bench: mode {mode} guess {guess}: {1e3 * (t1 - t0) / steps:.2f} ms/step over {steps} steps"
+          + (f", later window {late['ms_per_step']:.2f} ms/step" if late else ""), file=sys.stderr, flush=True)
     return dict(seconds=t1 - t0, steps=steps, iters=timed_iters, relres=st["mg_relres"], tau=st["tau"],
-                vol0=vol0, vol1=vol1, prof=prof, p=p, guess_stats=guess_stats, late=late)
+                vol0=vol0, vol1=vol1, prof=prof, p=p, guess_stats=guess_stats, late=late, output=output)
 
 
 def host_facts():
@@ -222,6 +243,7 @@ def main():
     ap.add_argument("--late-steps", type=int, default=20, help="steps in the later timed window")
     ap.add_argument("--weights", default=os.path.join(HERE, "3d_droplet_simulation_2007_amd", "weights",
                                                        "guess_unet.npz"), help="learned-guess network weights")
+    ap.add_argument("--no-output", action="store_true", help="skip the run with printout's files")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (scale for HBM byte counters) and exit")
     args = ap.parse_args()
@@ -336,6 +358,22 @@ def main():
                                                "mg_iters_per_step": float(np.mean(mixed["iters"])), "guess": mguess,
                                                "speedup_vs_fp64": mv / dv}
     line.update(out)
+    if not args.no_output and mixed is not None:
+        # the mixed path again, with printout's files at tprint 5 and
+        # ntecplot 10 (a full-field Tecplot file every 10 steps), written to
+        # a scratch directory by the writer thread
+        import tempfile
+        otxt = splash_inputdata(dg, n, tprint=5, ntecplot=10)
+        with tempfile.TemporaryDirectory() as d:
+            o = run_path(dg, otxt, 2, args.steps, args.warmup, args.tol, guess=(0 if mguess == "zero" else 1),
+                         output_dir=d)
+        st = o["output"]
+        line["output"] = {"cadence": "printout every step: diagnostics every 5 steps, mainNNNN.plt every 10",
+                          "ms_per_step": 1e3 * o["seconds"] / o["steps"],
+                          "ms_per_step_without_output": 1e3 * mixed["seconds"] / mixed["steps"],
+                          "printouts_queued": st["jobs"], "bytes_downloaded": st["bytes"],
+                          "step_wait_for_writer_ms": st["wait_ms"], "writer_thread_ms": st["write_ms"],
+                          "drain_after_timed_steps_ms": st["drain_ms"]}
     if not args.no_cpu_baseline:
         v, dt = cpu_baseline(n, args.cpu_steps, args.tol, args.omega, args.nu_coarse)
         line["cpu_baseline"] = {"value": v, "unit": "time steps/s", "cores": 1, "kind": "port",

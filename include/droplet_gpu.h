@@ -32,6 +32,11 @@ enum {
     /* CalcH3d's interface heights (/NEWW/), present with hswitch = 1 or
      * surfacet = 5, 6 */
     DG_F_HXMIN, DG_F_HXMAX, DG_F_HYMIN, DG_F_HYMAX, DG_F_HZMIN, DG_F_HZMAX,
+    /* /TURBULENCE/ Reynolds stresses of the anisotropic k-epsilon model
+     * (uv2p, iturb = 4; present with iturb = 4) and the model coefficient
+     * cmu */
+    DG_F_U2P, DG_F_V2P, DG_F_W2P, DG_F_UVP, DG_F_UWP, DG_F_VWP,
+    DG_F_CMU,
     DG_NFIELDS
 };
 
@@ -58,6 +63,20 @@ typedef struct dg_state {
  * save.dat, the file in inputdata.in's directory.  Allocates every field in
  * HBM. */
 int dg_init(const char *inputdata_path);
+/* smooth2 (main.f:5678-5752) generalised to n droplets, replacing the
+ * volume fraction and velocity of the resident state (then bc_c, bc_v,
+ * calcrho).  params: 7 doubles per droplet in the solver's units (lengths
+ * in units of length_SI, on smooth2's grid coordinates (i - 1.5) h;
+ * velocities in units of the run's velocity scale): centre x0, y0, z0,
+ * radius, liquid velocity u, v, w.  A cell inside one droplet is full, a
+ * cell that meets none empty, others count smooth2's 30^3 sub-points in any
+ * droplet; velocities are smooth2's mixture rule with the velocity of the
+ * cell's droplet (u = u_d cc).  At most 8 droplets may reach into one cell.
+ * dg_smooth2_droplet gives the single droplet dg_init placed, in these
+ * units: dg_init_droplets with it alone reproduces dg_init's state.  No
+ * reference counterpart for n > 1 (the reference has one droplet). */
+int dg_init_droplets(const int *n, const double *params);
+int dg_smooth2_droplet(double *params);
 /* The loop bounds of the run: t0 (0, or save.dat's t0 on a restart) and
  * tmax (t0 + the file's tmax on a restart, main.f:791). */
 int dg_run_limits(int *t0, int *tmax);
@@ -74,7 +93,9 @@ int dg_step(const int *t);          /* loop body main.f:21-71 (printout excluded
  * each step when dg_output_open has opened the output files */
 int dg_run(const int *t0, const int *nsteps);
 int dg_vofmethod(const int *t);     /* VOFmethod main.f:795-824 */
-int dg_kepsmodel(const int *t);     /* kepsmodel main.f:1888-2026 (iturb=3) */
+int dg_kepsmodel(const int *t);     /* kepsmodel main.f:1888-2026 (iturb=3, 4) */
+int dg_eddyvisc(void);               /* eddyvisc main.f:1828-1885 (iturb=1, 2) */
+int dg_uv2p(void);                   /* uv2p main.f:2032-2155 (iturb=4) */
 int dg_checktime(const int *t);     /* checktime main.f:2156-2175 */
 int dg_csf(void);                   /* csf main.f:2363-2481 (surfacet=2) */
 int dg_calch3d(void);               /* CalcH3d main.f:1312-1810 (hswitch=1) */
@@ -312,6 +333,8 @@ void solvelpres_(double *cc, double *w1, double *w2, double *w3, double *a, doub
                  const int *t);                                  /* solvelpres(...,t)   main.f:3236 */
 void calcrho_(double *cc);                                       /* calcrho(cc)         main.f:2177 */
 void calch3d_(double *cc);                                       /* CalcH3d(cc)         main.f:1312 */
+void eddyvisc_(double *cc, double *ufilt);                       /* eddyvisc(cc,ufilt)  main.f:1828 */
+void uv2p_(void);                                                /* uv2p()              main.f:2032 */
 void sgip_(double *ccc, double *w11, double *w22, double *w33);  /* SGIP(ccc,...)       main.f:2835 */
 void sgip2_(double *ccc, double *w11, double *w22, double *w33); /* SGIP2(ccc,...)      main.f:3029 */
 void bc_c_(double *cc);                                          /* bc_c(cc)            main.f:4331 */
@@ -351,9 +374,9 @@ double fl3d_(const double *m1, const double *m2, const double *m3, const double 
  * without the reference's initialize, the library reads ./inputdata.in and
  * sets up the run (as dg_init) and writes the result into COMMON (scalars,
  * /pointers/ with MYMALLOC's bookkeeping, every field).  dg_sync_to_host_ /
- * dg_sync_to_device_(mask): bit id for each DG_F_* field that has a host
- * array, bit DG_NFIELDS for cmu, bit 30 for every level's heap slots, -1
- * for all. */
+ * dg_sync_to_device_(mask): bit id for each DG_F_* field id below
+ * DG_F_U2P that has a host array, bit 29 for cmu, bit 31 for the six
+ * Reynolds-stress arrays, bit 30 for every level's heap slots, -1 for all. */
 void dg_attach_(void);
 void dg_init_input_(void);
 void dg_sync_to_host_(const int *mask);

@@ -628,10 +628,15 @@ void or_kepsmodel(or_state *st, int t)
                 double dwdy = 0.25 * (AT(w, i, j, k) - AT(w, i, j - 1, k) + AT(w, i, j, k + 1) - AT(w, i, j - 1, k + 1) +
                                       AT(w, i, j + 1, k) - AT(w, i, j, k) + AT(w, i, j + 1, k + 1) - AT(w, i, j, k + 1));
                 double dwdz = AT(w, i, j, k + 1) - AT(w, i, j, k);
-                (void)t;
-                prod = AT(mut, i, j, k) / AT(rh, i, j, k) / h2 *
-                       (2.0 * (dudx * dudx + dvdy * dvdy + dwdz * dwdz + dvdx * dudy + dwdx * dudz + dvdz * dwdy) +
-                        dudy * dudy + dudz * dudz + dvdx * dvdx + dvdz * dvdz + dwdx * dwdx + dwdy * dwdy);
+                if (t == 0 || st->iturb == 3)
+                    prod = AT(mut, i, j, k) / AT(rh, i, j, k) / h2 *
+                           (2.0 * (dudx * dudx + dvdy * dvdy + dwdz * dwdz + dvdx * dudy + dwdx * dudz + dvdz * dwdy) +
+                            dudy * dudy + dudz * dudz + dvdx * dvdx + dvdz * dvdz + dwdx * dwdx + dwdy * dwdy);
+                else /* iturb = 4: production from the Reynolds stresses (main.f:1961-1964) */
+                    prod = -(AT(st->u2p, i, j, k) * dudx + AT(st->v2p, i, j, k) * dvdy + AT(st->w2p, i, j, k) * dwdz +
+                             AT(st->uvp, i, j, k) * (dudy + dvdx) + AT(st->uwp, i, j, k) * (dudz + dwdx) +
+                             AT(st->vwp, i, j, k) * (dvdz + dwdy)) /
+                           h;
                 AT(kn, i, j, k) = AT(kt, i, j, k) + tau * (-conv + diff + prod - AT(ep, i, j, k));
                 conv = (+dmax1(AT(u, i + 1, j, k), 0.0) * AT(ep, i, j, k) + dmin1(AT(u, i + 1, j, k), 0.0) * AT(ep, i + 1, j, k) -
                         dmax1(AT(u, i, j, k), 0.0) * AT(ep, i - 1, j, k) - dmin1(AT(u, i, j, k), 0.0) * AT(ep, i, j, k) +
@@ -665,6 +670,163 @@ void or_kepsmodel(or_state *st, int t)
     or_bc_scalar(st->ktur, nx, ny, nz);
     or_bc_scalar(st->eps, nx, ny, nz);
     or_bc_scalar(st->mut, nx, ny, nz);
+    if (st->iturb == 4) or_uv2p(st); /* main.f:2023 */
+}
+
+/* eddyvisc, main.f:1828-1885 (iturb = 1, 2): filter u into ufilt (the
+ * finest pressure slot z(iu(ng)), main.f:35) on the interior, the mixing
+ * length of each x station from the filtered profile through (j1, k1), then
+ * mut from the mixing length and the y and z shear of u.  j2 is left
+ * uninitialised by the reference before the first station's second branch;
+ * it starts at 0 here.  A centre index (j1, k1) outside the grid, which the
+ * reference would read out of bounds, leaves lmix at 0. */
+void or_eddyvisc(or_state *st)
+{
+    int nx = st->nx, ny = st->ny, nz = st->nz;
+    double *uf = st->lev[st->ng - 1].u;
+    const double *u = st->u;
+    const double h = st->h;
+    /* filter, main.f:1271-1284 */
+    for (int j = 2; j <= ny - 1; j++)
+        for (int i = 2; i <= nx - 1; i++)
+            for (int k = 2; k <= nz - 1; k++)
+                AT(uf, i, j, k) = 0.25 * AT(u, i, j, k) +
+                                  0.125 * (AT(u, i, j, k - 1) + AT(u, i, j, k + 1) + AT(u, i, j - 1, k) + AT(u, i, j + 1, k)) +
+                                  0.0625 * (AT(u, i, j - 1, k - 1) + AT(u, i, j - 1, k + 1) + AT(u, i, j + 1, k - 1) +
+                                            AT(u, i, j + 1, k + 1));
+    const double alfa = F4(0.08), cmu1 = F4(1.0);
+    const int j1 = (int)(st->y0 / h), k1 = (int)(st->z0 / h);
+    const int inside = j1 >= 1 && j1 <= ny && k1 >= 1 && k1 <= nz;
+    int j2 = 0;
+    for (int i = 1; i <= nx; i++) {
+        if (!inside) {
+            st->lmix[i - 1] = 0.0;
+            continue;
+        }
+        double umax, umin;
+        if (st->xvel1 > st->xvel2) {
+            umax = AT(uf, i, j1, k1);
+            umin = st->xvel2;
+            j2 = 0;
+            for (int j = 2; j <= ny - 1; j++)
+                if (AT(uf, i, j, k1) - umin > F4(0.1) * (umax - umin)) {
+                    j2 = j;
+                    break;
+                }
+        } else {
+            umax = st->xvel2;
+            umin = AT(uf, i, j1, k1);
+            for (int j = 2; j <= ny - 1; j++)
+                if (AT(uf, i, j, k1) - umin <= F4(0.1) * (umax - umin)) {
+                    j2 = j;
+                    break;
+                }
+        }
+        if (j2 == 2) j2 = j1;
+        st->lmix[i - 1] = alfa * h * (double)abs(j2 - j1);
+    }
+    const double *rh = st->rhon;
+    double *mut = st->mut;
+    for (int i = 2; i <= nx - 1; i++)
+        for (int k = 2; k <= nz - 1; k++)
+            for (int j = 2; j <= ny - 1; j++) {
+                const double dudy = 0.25 *
+                                    (AT(u, i, j, k) - AT(u, i, j - 1, k) + AT(u, i + 1, j, k) - AT(u, i + 1, j - 1, k) +
+                                     AT(u, i, j + 1, k) - AT(u, i, j, k) + AT(u, i + 1, j + 1, k) - AT(u, i + 1, j, k)) /
+                                    h;
+                const double dudz = 0.25 *
+                                    (AT(u, i, j, k) - AT(u, i, j, k - 1) + AT(u, i + 1, j, k) - AT(u, i + 1, j, k - 1) +
+                                     AT(u, i, j, k + 1) - AT(u, i, j, k) + AT(u, i + 1, j, k + 1) - AT(u, i + 1, j, k)) /
+                                    h;
+                const double lm = st->lmix[i - 1] + st->lmix[i];
+                AT(mut, i, j, k) = cmu1 * AT(rh, i, j, k) * 0.25 * lm * lm * sqrt(dudy * dudy + dudz * dudz);
+            }
+    or_bc_scalar(mut, nx, ny, nz);
+}
+
+/* uv2p, main.f:2032-2155 (iturb = 4): the anisotropic (Shih-Zhu-Lumley)
+ * eddy viscosity and Reynolds stresses from the strain and rotation
+ * rates.  REAL literals of the reference are rounded to single precision
+ * (F4): 1./3., 2./3., 1.E-30 and sqrt(6.), which the reference evaluates
+ * in REAL arithmetic. */
+void or_uv2p(or_state *st)
+{
+    int nx = st->nx, ny = st->ny, nz = st->nz;
+    const double *u = st->u, *v = st->v, *w = st->w, *kt = st->ktur, *ep = st->eps, *rh = st->rhon;
+    const double h = st->h;
+    const double third = (double)(1.0f / 3.0f), twothird = (double)(2.0f / 3.0f), tiny = F4(1.E-30);
+    const double sixsq = (double)sqrtf(6.0f);
+    for (int k = 2; k <= nz - 1; k++)
+        for (int j = 2; j <= ny - 1; j++)
+            for (int i = 2; i <= nx - 1; i++) {
+                const double keps = AT(kt, i, j, k) / AT(ep, i, j, k);
+                const double k2eps = keps * AT(kt, i, j, k);
+                const double k2eps2 = keps * keps;
+                const double k3eps2 = k2eps2 * AT(kt, i, j, k);
+                const double dudx = (AT(u, i + 1, j, k) - AT(u, i, j, k)) / h;
+                const double dudy = 0.25 * (AT(u, i, j, k) - AT(u, i, j - 1, k) + AT(u, i + 1, j, k) - AT(u, i + 1, j - 1, k) +
+                                            AT(u, i, j + 1, k) - AT(u, i, j, k) + AT(u, i + 1, j + 1, k) - AT(u, i + 1, j, k)) /
+                                    h;
+                const double dudz = 0.25 * (AT(u, i, j, k) - AT(u, i, j, k - 1) + AT(u, i + 1, j, k) - AT(u, i + 1, j, k - 1) +
+                                            AT(u, i, j, k + 1) - AT(u, i, j, k) + AT(u, i + 1, j, k + 1) - AT(u, i + 1, j, k)) /
+                                    h;
+                const double dvdx = 0.25 * (AT(v, i, j, k) - AT(v, i - 1, j, k) + AT(v, i + 1, j, k) - AT(v, i, j, k) +
+                                            AT(v, i, j + 1, k) - AT(v, i - 1, j + 1, k) + AT(v, i + 1, j + 1, k) - AT(v, i, j + 1, k)) /
+                                    h;
+                const double dvdy = (AT(v, i, j + 1, k) - AT(v, i, j, k)) / h;
+                const double dvdz = 0.25 * (AT(v, i, j, k) - AT(v, i, j, k - 1) + AT(v, i, j, k + 1) - AT(v, i, j, k) +
+                                            AT(v, i, j + 1, k) - AT(v, i, j + 1, k - 1) + AT(v, i, j + 1, k + 1) - AT(v, i, j + 1, k)) /
+                                    h;
+                const double dwdx = 0.25 * (AT(w, i, j, k) - AT(w, i - 1, j, k) + AT(w, i + 1, j, k) - AT(w, i, j, k) +
+                                            AT(w, i, j, k + 1) - AT(w, i - 1, j, k + 1) + AT(w, i + 1, j, k + 1) - AT(w, i, j, k + 1)) /
+                                    h;
+                const double dwdy = 0.25 * (AT(w, i, j, k) - AT(w, i, j - 1, k) + AT(w, i, j, k + 1) - AT(w, i, j - 1, k + 1) +
+                                            AT(w, i, j + 1, k) - AT(w, i, j, k) + AT(w, i, j + 1, k + 1) - AT(w, i, j, k + 1)) /
+                                    h;
+                const double dwdz = (AT(w, i, j, k + 1) - AT(w, i, j, k)) / h;
+                const double ss11 = dudx, ss12 = 0.5 * (dudy + dvdx), ss13 = 0.5 * (dudz + dwdx), ss22 = dvdy,
+                             ss23 = 0.5 * (dvdz + dwdy), ss33 = dwdz;
+                const double sskk = ss11 + ss22 + ss33;
+                const double sstar11 = ss11 - third * sskk, sstar22 = ss22 - third * sskk, sstar33 = ss33 - third * sskk;
+                const double sstar = sqrt(sstar11 * sstar11 + sstar22 * sstar22 + sstar33 * sstar33 + 2.0 * ss12 * ss12 +
+                                          2.0 * ss13 * ss13 + 2.0 * ss23 * ss23 + tiny);
+                const double omeg12 = 0.5 * (dudy - dvdx), omeg13 = 0.5 * (dudz - dwdx), omeg23 = 0.5 * (dvdz - dwdy);
+                const double omegastar = sqrt(2.0 * omeg12 * omeg12 + 2.0 * omeg13 * omeg13 + 2.0 * omeg23 * omeg23);
+                const double wstar = (sstar11 * sstar11 * sstar11 + sstar22 * sstar22 * sstar22 + sstar33 * sstar33 * sstar33 +
+                                      3.0 * sstar11 * ss12 * ss12 + 3.0 * ss12 * ss12 * sstar22 + 3.0 * sstar11 * ss13 * ss13 +
+                                      6.0 * ss12 * ss13 * ss23 + 3.0 * ss13 * ss13 * sstar33 + 3.0 * sstar22 * ss23 * ss23 +
+                                      3.0 * ss23 * ss23 * sstar33) /
+                                     sstar / sstar / sstar;
+                const double ustar = sqrt(sstar * sstar + omegastar * omegastar);
+                double sixsqws = sixsq * wstar;
+                if ((1.0 - fabs(sixsqws)) < 1.e-4) sixsqws = sixsqws < 0.0 ? -1.0 : 1.0;
+                const double asstar = sixsq * cos(acos(sixsqws) / 3.0);
+                const double cm = 1.0 / (6.5 + asstar * ustar * keps);
+                AT(st->cmu, i, j, k) = cm;
+                AT(st->mut, i, j, k) = cm * AT(rh, i, j, k) * AT(kt, i, j, k) * AT(kt, i, j, k) / AT(ep, i, j, k);
+                double c2 = 1.0 - 9.0 * cm * cm * sstar * sstar * k2eps2;
+                if (c2 < 0.0) c2 = 0.0;
+                c2 = sqrt(c2) / (1.0 + 6.0 * sstar * k2eps2 * omegastar);
+                AT(st->u2p, i, j, k) = twothird * AT(kt, i, j, k) - 2.0 * cm * k2eps * sstar11 +
+                                       4.0 * c2 * k3eps2 * (ss12 * omeg12 + ss13 * omeg13);
+                AT(st->v2p, i, j, k) = twothird * AT(kt, i, j, k) - 2.0 * cm * k2eps * sstar22 -
+                                       4.0 * c2 * k3eps2 * (ss12 * omeg12 - ss23 * omeg23);
+                AT(st->w2p, i, j, k) = twothird * AT(kt, i, j, k) - 2.0 * cm * k2eps * sstar33 -
+                                       4.0 * c2 * k3eps2 * (ss13 * omeg13 + ss23 * omeg23);
+                AT(st->uvp, i, j, k) = -(2.0 * cm * k2eps * ss12) +
+                                       2.0 * c2 * k3eps2 * (-(sstar11 * omeg12) + sstar22 * omeg12 + ss13 * omeg23 + ss23 * omeg13);
+                AT(st->uwp, i, j, k) = -(2.0 * cm * k2eps * ss13) +
+                                       2.0 * c2 * k3eps2 * (-(sstar11 * omeg13) + sstar33 * omeg13 - ss12 * omeg23 + ss23 * omeg12);
+                AT(st->vwp, i, j, k) = -(2.0 * cm * k2eps * ss23) +
+                                       2.0 * c2 * k3eps2 * (-(sstar22 * omeg23) + sstar33 * omeg23 - ss12 * omeg13 - ss13 * omeg12);
+            }
+    or_bc_scalar(st->mut, nx, ny, nz);
+    or_bc_scalar(st->u2p, nx, ny, nz);
+    or_bc_scalar(st->v2p, nx, ny, nz);
+    or_bc_scalar(st->w2p, nx, ny, nz);
+    or_bc_scalar(st->uvp, nx, ny, nz);
+    or_bc_scalar(st->uwp, nx, ny, nz);
+    or_bc_scalar(st->vwp, nx, ny, nz);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1326,15 +1488,14 @@ void or_mglin(or_state *st)
 }
 
 /* ------------------------------------------------------------------ */
-/* convective, main.f:3441-3580 (Reynolds-stress arrays u2p.. are zero
- * unless iturb = 4, which is outside this path; they are kept as zero
- * terms so the evaluation order matches the reference exactly)          */
+/* convective, main.f:3441-3580, with the Reynolds-stress terms of
+ * u2p..vwp in the reference's evaluation order (the arrays are zero unless
+ * iturb = 4)                                                             */
 static void convective(or_state *st, const double *w1, const double *w2, const double *w3)
 {
     int nx = st->nx, ny = st->ny, nz = st->nz;
     double *u = st->u, *v = st->v, *w = st->w;
     const double tauh = st->tauh;
-    const double z0 = 0.0; /* value of every u2p/uvp/.. entry on this path */
     if (st->convectype == 1) {
         for (int k = 2; k <= nz - 1; k++)
             for (int j = 2; j <= ny - 1; j++)
@@ -1353,8 +1514,8 @@ static void convective(or_state *st, const double *w1, const double *w2, const d
                                 dmax1(s1, 0.0) * AT(w1, i - 1, j, k) + dmin1(s1, 0.0) * AT(w1, i, j, k) -
                                 dmax1(s2, 0.0) * AT(w1, i, j, k) - dmin1(s2, 0.0) * AT(w1, i + 1, j, k) +
                                 dmax1(q1, 0.0) * AT(w1, i, j, k - 1) + dmin1(q1, 0.0) * AT(w1, i, j, k) -
-                                dmax1(q2, 0.0) * AT(w1, i, j, k) - dmin1(q2, 0.0) * AT(w1, i, j, k + 1) + z0 - z0 +
-                                0.25 * (+z0 - z0 + z0 - z0 + z0 - z0 + z0 - z0 + z0 - z0 + z0 - z0 + z0 - z0 + z0 - z0));
+                                dmax1(q2, 0.0) * AT(w1, i, j, k) - dmin1(q2, 0.0) * AT(w1, i, j, k + 1) + AT(st->u2p, i - 1, j, k) - AT(st->u2p, i, j, k) +
+                                0.25 * (+AT(st->uvp, i, j, k) - AT(st->uvp, i, j + 1, k) + AT(st->uvp, i, j - 1, k) - AT(st->uvp, i, j, k) + AT(st->uvp, i - 1, j, k) - AT(st->uvp, i - 1, j + 1, k) + AT(st->uvp, i - 1, j - 1, k) - AT(st->uvp, i - 1, j, k) + AT(st->uwp, i, j, k) - AT(st->uwp, i, j, k + 1) + AT(st->uwp, i, j, k - 1) - AT(st->uwp, i, j, k) + AT(st->uwp, i - 1, j, k) - AT(st->uwp, i - 1, j, k + 1) + AT(st->uwp, i - 1, j, k - 1) - AT(st->uwp, i - 1, j, k)));
                     t1 = .5 * (AT(w1, i, j, k) + AT(w1, i, j - 1, k));
                     t2 = .5 * (AT(w1, i + 1, j, k) + AT(w1, i + 1, j - 1, k));
                     s1 = .5 * (AT(w2, i, j - 1, k) + AT(w2, i, j, k));
@@ -1368,8 +1529,8 @@ static void convective(or_state *st, const double *w1, const double *w2, const d
                                 dmax1(s1, 0.0) * AT(w2, i, j - 1, k) + dmin1(s1, 0.0) * AT(w2, i, j, k) -
                                 dmax1(s2, 0.0) * AT(w2, i, j, k) - dmin1(s2, 0.0) * AT(w2, i, j + 1, k) +
                                 dmax1(q1, 0.0) * AT(w2, i, j, k - 1) + dmin1(q1, 0.0) * AT(w2, i, j, k) -
-                                dmax1(q2, 0.0) * AT(w2, i, j, k) - dmin1(q2, 0.0) * AT(w2, i, j, k + 1) + z0 - z0 +
-                                0.25 * (+z0 - z0 + z0 - z0 + z0 - z0 + z0 - z0 + z0 - z0 + z0 - z0 + z0 - z0 + z0 - z0));
+                                dmax1(q2, 0.0) * AT(w2, i, j, k) - dmin1(q2, 0.0) * AT(w2, i, j, k + 1) + AT(st->v2p, i, j - 1, k) - AT(st->v2p, i, j, k) +
+                                0.25 * (+AT(st->uvp, i - 1, j, k) - AT(st->uvp, i, j, k) + AT(st->uvp, i, j, k) - AT(st->uvp, i + 1, j, k) + AT(st->uvp, i - 1, j - 1, k) - AT(st->uvp, i, j - 1, k) + AT(st->uvp, i, j - 1, k) - AT(st->uvp, i + 1, j - 1, k) + AT(st->vwp, i, j, k - 1) - AT(st->vwp, i, j, k) + AT(st->vwp, i, j, k) - AT(st->vwp, i, j, k + 1) + AT(st->vwp, i, j - 1, k - 1) - AT(st->vwp, i, j - 1, k) + AT(st->vwp, i, j - 1, k) - AT(st->vwp, i, j - 1, k + 1)));
                     t1 = .5 * (AT(w1, i, j, k) + AT(w1, i, j, k - 1));
                     t2 = .5 * (AT(w1, i + 1, j, k) + AT(w1, i + 1, j, k - 1));
                     s1 = .5 * (AT(w3, i, j, k - 1) + AT(w3, i, j, k));
@@ -1383,8 +1544,8 @@ static void convective(or_state *st, const double *w1, const double *w2, const d
                                 dmax1(s1, 0.0) * AT(w3, i, j, k - 1) + dmin1(s1, 0.0) * AT(w3, i, j, k) -
                                 dmax1(s2, 0.0) * AT(w3, i, j, k) - dmin1(s2, 0.0) * AT(w3, i, j, k + 1) +
                                 dmax1(q1, 0.0) * AT(w3, i, j - 1, k) + dmin1(q1, 0.0) * AT(w3, i, j, k) -
-                                dmax1(q2, 0.0) * AT(w3, i, j, k) - dmin1(q2, 0.0) * AT(w3, i, j + 1, k) + z0 - z0 +
-                                0.25 * (+z0 - z0 + z0 - z0 + z0 - z0 + z0 - z0 + z0 - z0 + z0 - z0 + z0 - z0 + z0 - z0));
+                                dmax1(q2, 0.0) * AT(w3, i, j, k) - dmin1(q2, 0.0) * AT(w3, i, j + 1, k) + AT(st->w2p, i, j, k - 1) - AT(st->w2p, i, j, k) +
+                                0.25 * (+AT(st->uwp, i - 1, j, k) - AT(st->uwp, i, j, k) + AT(st->uwp, i, j, k) - AT(st->uwp, i + 1, j, k) + AT(st->uwp, i - 1, j, k - 1) - AT(st->uwp, i, j, k - 1) + AT(st->uwp, i, j, k - 1) - AT(st->uwp, i + 1, j, k - 1) + AT(st->vwp, i, j - 1, k) - AT(st->vwp, i, j, k) + AT(st->vwp, i, j, k) - AT(st->vwp, i, j + 1, k) + AT(st->vwp, i, j - 1, k - 1) - AT(st->vwp, i, j, k - 1) + AT(st->vwp, i, j, k - 1) - AT(st->vwp, i, j + 1, k - 1)));
                 }
     } else if (st->convectype == 2) {
         for (int k = 2; k <= nz - 1; k++)
@@ -1397,8 +1558,8 @@ static void convective(or_state *st, const double *w1, const double *w2, const d
                                         (AT(w1, i, j + 1, k) + AT(w1, i, j, k)) * (AT(w2, i, j + 1, k) + AT(w2, i - 1, j + 1, k)) +
                                         (AT(w1, i, j, k - 1) + AT(w1, i, j, k)) * (AT(w3, i, j, k) + AT(w3, i - 1, j, k)) -
                                         (AT(w1, i, j, k + 1) + AT(w1, i, j, k)) * (AT(w3, i, j, k + 1) + AT(w3, i - 1, j, k + 1)) +
-                                        z0 - z0 + z0 - z0 + z0 - z0 + z0 - z0 + z0 - z0 + z0 - z0 + z0 - z0 + z0 - z0) +
-                                z0 - z0);
+                                        AT(st->uvp, i, j, k) - AT(st->uvp, i, j + 1, k) + AT(st->uvp, i, j - 1, k) - AT(st->uvp, i, j, k) + AT(st->uvp, i - 1, j, k) - AT(st->uvp, i - 1, j + 1, k) + AT(st->uvp, i - 1, j - 1, k) - AT(st->uvp, i - 1, j, k) + AT(st->uwp, i, j, k) - AT(st->uwp, i, j, k + 1) + AT(st->uwp, i, j, k - 1) - AT(st->uwp, i, j, k) + AT(st->uwp, i - 1, j, k) - AT(st->uwp, i - 1, j, k + 1) + AT(st->uwp, i - 1, j, k - 1) - AT(st->uwp, i - 1, j, k)) +
+                                AT(st->u2p, i - 1, j, k) - AT(st->u2p, i, j, k));
                     AT(v, i, j, k) =
                         AT(w2, i, j, k) +
                         tauh * (0.25 * (sq(AT(w2, i, j, k) + AT(w2, i, j - 1, k)) - sq(AT(w2, i, j + 1, k) + AT(w2, i, j, k)) +
@@ -1406,8 +1567,8 @@ static void convective(or_state *st, const double *w1, const double *w2, const d
                                         (AT(w2, i + 1, j, k) + AT(w2, i, j, k)) * (AT(w1, i + 1, j, k) + AT(w1, i + 1, j - 1, k)) +
                                         (AT(w2, i, j, k - 1) + AT(w2, i, j, k)) * (AT(w3, i, j, k) + AT(w3, i, j - 1, k)) -
                                         (AT(w2, i, j, k + 1) + AT(w2, i, j, k)) * (AT(w3, i, j, k + 1) + AT(w3, i, j - 1, k + 1)) +
-                                        z0 - z0 + z0 - z0 + z0 - z0 + z0 - z0 + z0 - z0 + z0 - z0 + z0 - z0 + z0 - z0) +
-                                z0 - z0);
+                                        AT(st->uvp, i - 1, j, k) - AT(st->uvp, i, j, k) + AT(st->uvp, i, j, k) - AT(st->uvp, i + 1, j, k) + AT(st->uvp, i - 1, j - 1, k) - AT(st->uvp, i, j - 1, k) + AT(st->uvp, i, j - 1, k) - AT(st->uvp, i + 1, j - 1, k) + AT(st->vwp, i, j, k - 1) - AT(st->vwp, i, j, k) + AT(st->vwp, i, j, k) - AT(st->vwp, i, j, k + 1) + AT(st->vwp, i, j - 1, k - 1) - AT(st->vwp, i, j - 1, k) + AT(st->vwp, i, j - 1, k) - AT(st->vwp, i, j - 1, k + 1)) +
+                                AT(st->v2p, i, j - 1, k) - AT(st->v2p, i, j, k));
                     AT(w, i, j, k) =
                         AT(w3, i, j, k) +
                         tauh * (0.25 * (sq(AT(w3, i, j, k) + AT(w3, i, j, k - 1)) - sq(AT(w3, i, j, k + 1) + AT(w3, i, j, k)) +
@@ -1415,8 +1576,8 @@ static void convective(or_state *st, const double *w1, const double *w2, const d
                                         (AT(w3, i + 1, j, k) + AT(w3, i, j, k)) * (AT(w1, i + 1, j, k) + AT(w1, i + 1, j, k - 1)) +
                                         (AT(w3, i, j - 1, k) + AT(w3, i, j, k)) * (AT(w2, i, j, k) + AT(w2, i, j, k - 1)) -
                                         (AT(w3, i, j + 1, k) + AT(w3, i, j, k)) * (AT(w2, i, j + 1, k) + AT(w2, i, j + 1, k - 1)) +
-                                        z0 - z0 + z0 - z0 + z0 - z0 + z0 - z0 + z0 - z0 + z0 - z0 + z0 - z0 + z0 - z0) +
-                                z0 - z0);
+                                        AT(st->uwp, i - 1, j, k) - AT(st->uwp, i, j, k) + AT(st->uwp, i, j, k) - AT(st->uwp, i + 1, j, k) + AT(st->uwp, i - 1, j, k - 1) - AT(st->uwp, i, j, k - 1) + AT(st->uwp, i, j, k - 1) - AT(st->uwp, i + 1, j, k - 1) + AT(st->vwp, i, j - 1, k) - AT(st->vwp, i, j, k) + AT(st->vwp, i, j, k) - AT(st->vwp, i, j + 1, k) + AT(st->vwp, i, j - 1, k - 1) - AT(st->vwp, i, j, k - 1) + AT(st->vwp, i, j, k - 1) - AT(st->vwp, i, j + 1, k - 1)) +
+                                AT(st->w2p, i, j, k - 1) - AT(st->w2p, i, j, k));
                 }
     } else {
         for (int k = 2; k <= nz - 1; k++)
@@ -1545,7 +1706,8 @@ void or_step(or_state *st, int t)
 {
     or_vofmethod(st, t);
     if (st->hswitch == 1) or_calch3d(st);  /* main.f:30 */
-    if (st->iturb == 3) or_kepsmodel(st, t);
+    if (st->iturb == 1 || st->iturb == 2) or_eddyvisc(st);  /* main.f:35 */
+    if (st->iturb == 3 || st->iturb == 4) or_kepsmodel(st, t);  /* main.f:39-40 */
     or_checktime(st, t);
     if (st->surfacet == 2) or_csf(st);
     else if (st->surfacet == 5) or_sgip(st);
@@ -1762,6 +1924,9 @@ or_state *or_create(const char *path, char *errbuf, int errlen)
     st->p_prev = zalloc(n);
     st->hxmin = zalloc(n); st->hxmax = zalloc(n); st->hymin = zalloc(n);
     st->hymax = zalloc(n); st->hzmin = zalloc(n); st->hzmax = zalloc(n);
+    st->u2p = zalloc(n); st->v2p = zalloc(n); st->w2p = zalloc(n);
+    st->uvp = zalloc(n); st->uwp = zalloc(n); st->vwp = zalloc(n);
+    st->lmix = zalloc((size_t)st->nx);
     st->mg_mode = 0;
     st->mg_nu1 = 2; st->mg_nu2 = 2; st->mg_maxcycles = 200; st->mg_tol = 1e-8; st->mg_guess = 0;
     nondim(st);
@@ -1784,6 +1949,7 @@ void or_destroy(or_state *st)
     free(st->rhon); free(st->mun); free(st->mut); free(st->ktur); free(st->eps); free(st->cmu);
     free(st->p_prev);
     free(st->hxmin); free(st->hxmax); free(st->hymin); free(st->hymax); free(st->hzmin); free(st->hzmax);
+    free(st->u2p); free(st->v2p); free(st->w2p); free(st->uvp); free(st->uwp); free(st->vwp); free(st->lmix);
     free(st);
 }
 
@@ -1820,6 +1986,13 @@ double *or_field(or_state *st, int id)
     case OR_F_HYMAX: return st->hymax;
     case OR_F_HZMIN: return st->hzmin;
     case OR_F_HZMAX: return st->hzmax;
+    case OR_F_U2P: return st->u2p;
+    case OR_F_V2P: return st->v2p;
+    case OR_F_W2P: return st->w2p;
+    case OR_F_UVP: return st->uvp;
+    case OR_F_UWP: return st->uwp;
+    case OR_F_VWP: return st->vwp;
+    case OR_F_CMU: return st->cmu;
     default: return NULL;
     }
 }

@@ -95,6 +95,11 @@ typedef struct or_state {
      * MATCH CASES FOUND", 2 its "PROBLEM finding MIN and MAX unit vector
      * components" (main.f:1582-1583, 1641-1644) */
     int stop_code;
+    /* /TURBULENCE/ Reynolds stresses of the anisotropic k-epsilon model
+     * (uv2p, main.f:2032-2155; zero unless iturb = 4, nondim main.f:436-441)
+     * and eddyvisc's mixing length lmix(NMX) (main.f:1828-1885) */
+    double *u2p, *v2p, *w2p, *uvp, *uwp, *vwp;
+    double *lmix;
 } or_state;
 
 /* field ids shared with the GPU library's dg_field ids */
@@ -105,6 +110,8 @@ enum {
     OR_F_A, OR_F_B, OR_F_C, OR_F_EI, OR_F_RHS, OR_F_RES,
     OR_F_PPREV,
     OR_F_HXMIN, OR_F_HXMAX, OR_F_HYMIN, OR_F_HYMAX, OR_F_HZMIN, OR_F_HZMAX,
+    OR_F_U2P, OR_F_V2P, OR_F_W2P, OR_F_UVP, OR_F_UWP, OR_F_VWP,
+    OR_F_CMU,
     OR_NFIELDS
 };
 
@@ -118,7 +125,9 @@ int       or_level_dims(or_state *st, int level, int *dims3);
 /* ---- one time step and its parts (main.f:19-73) */
 void or_step(or_state *st, int t);
 void or_vofmethod(or_state *st, int t);           /* main.f:795-824 */
-void or_kepsmodel(or_state *st, int t);           /* main.f:1888-2026 */
+void or_kepsmodel(or_state *st, int t);           /* main.f:1888-2026 (iturb 3, 4) */
+void or_eddyvisc(or_state *st);                   /* main.f:1828-1885 (iturb 1, 2) */
+void or_uv2p(or_state *st);                       /* main.f:2032-2155 (iturb 4) */
 void or_checktime(or_state *st, int t);           /* main.f:2156-2175 */
 void or_csf(or_state *st);                        /* main.f:2363-2481 */
 void or_calch3d(or_state *st);                    /* main.f:1312-1810 */

@@ -18,7 +18,8 @@ ORACLE_SO = os.path.join(REPO, "oracle", "_build", "liboracle.so")
 
 FIELD_IDS = dict(cc=0, u=1, v=2, w=3, p=4, s11=5, s22=6, s33=7, s12=8, s13=9, s23=10,
                  rhon=11, mun=12, mut=13, ktur=14, eps=15, a=16, b=17, c=18, ei=19, rhs=20, res=21,
-                 pprev=22, hxmin=23, hxmax=24, hymin=25, hymax=26, hzmin=27, hzmax=28)
+                 pprev=22, hxmin=23, hxmax=24, hymin=25, hymax=26, hzmin=27, hzmax=28,
+                 u2p=29, v2p=30, w2p=31, uvp=32, uwp=33, vwp=34, cmu=35)
 
 _lib = None
 
@@ -44,7 +45,7 @@ def lib():
         for nm in ("or_step", "or_vofmethod", "or_kepsmodel", "or_checktime", "or_solvelpres"):
             getattr(L, nm).argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.or_csf.argtypes = [ctypes.c_void_p]
-        for nm in ("or_calch3d", "or_sgip", "or_sgip2"):
+        for nm in ("or_calch3d", "or_sgip", "or_sgip2", "or_eddyvisc", "or_uv2p"):
             getattr(L, nm).argtypes = [ctypes.c_void_p]
         L.or_calcrho.argtypes = [ctypes.c_void_p]
         L.or_cfl.argtypes = [ctypes.c_void_p]
@@ -169,6 +170,7 @@ def _state_fields():
           ("mg_cycles_last", i), ("mg_relres_last", d), ("mg_bnorm_last", d), ("p_prev", p), ("cfl_last", d), ("mg_outer", i), ("mg_prolong", i), ("vof_guard", i), ("mg_omega", d), ("mg_nuc", i), ("ref_grid", i)]
     f += [(n, p) for n in ("hxmin", "hxmax", "hymin", "hymax", "hzmin", "hzmax")]
     f += [("stop_code", i)]
+    f += [(n, p) for n in ("u2p", "v2p", "w2p", "uvp", "uwp", "vwp", "lmix")]
     return f
 
 
@@ -265,4 +267,51 @@ def clipped_cc(txt):
     o = Oracle(txt)
     cc = o.field("cc").clip(0.0, 1.0).copy()
     o.close()
+    return cc
+
+
+def sampled_cc(shape, h, drops):
+    """The sub-point count of smooth2 (main.f:5693-5731) for several
+    droplets: volume fractions of the interior, numpy restatement."""
+    nz, ny, nx = shape
+    index = 30
+    inc = 1.0 / float(index ** 3)
+    table = np.zeros(index ** 3 + 1)
+    acc = 0.0
+    for q in range(1, index ** 3 + 1):
+        acc = acc + inc
+        table[q] = acc
+    cc = np.zeros(shape)
+    off = (np.arange(1, index + 1, dtype=np.float64) - 0.5) / float(index) * h
+    for k in range(2, nz):
+        for j in range(2, ny):
+            for i in range(2, nx):
+                xa, xb = ((i - 1) - 1.5) * h, (i - 1.5) * h
+                ya, yb = ((j - 1) - 1.5) * h, (j - 1.5) * h
+                za, zb = ((k - 1) - 1.5) * h, (k - 1.5) * h
+                cand = []
+                full = False
+                for (x0, y0, z0, r, *_v) in drops:
+                    dmin = min((xa - x0) ** 2, (xb - x0) ** 2) + min((ya - y0) ** 2, (yb - y0) ** 2) + \
+                        min((za - z0) ** 2, (zb - z0) ** 2)
+                    if not dmin < r * r:
+                        continue
+                    dmax = max((xa - x0) ** 2, (xb - x0) ** 2) + max((ya - y0) ** 2, (yb - y0) ** 2) + \
+                        max((za - z0) ** 2, (zb - z0) ** 2)
+                    if not dmax > r * r:
+                        full = True
+                        break
+                    cand.append((x0, y0, z0, r))
+                if full:
+                    cc[k - 1, j - 1, i - 1] = 1.0
+                    continue
+                if not cand:
+                    continue
+                tx = (xa + off)[:, None, None]
+                ty = (ya + off)[None, :, None]
+                tz = (za + off)[None, None, :]
+                inside = np.zeros((index, index, index), dtype=bool)
+                for (x0, y0, z0, r) in cand:
+                    inside |= (tx - x0) ** 2 + (ty - y0) ** 2 + (tz - z0) ** 2 < r * r
+                cc[k - 1, j - 1, i - 1] = table[int(inside.sum())]
     return cc

@@ -96,10 +96,11 @@ def _child(lib_path, inputdata, nsteps, init_fields, want_fields, conn, every_st
         arrs["rhon"] = phys[19:19 + n].reshape(nz, ny, nx)
         arrs["mun"] = phys[19 + n:19 + 2 * n].reshape(nz, ny, nx)
         # /TURBULENCE/: mut, lmix(NMX), turbst, ktur, eps, cmu, ...
-        turb = np.frombuffer((ctypes.c_double * (5 * n + nx)).in_dll(lib, "turbulence_"), dtype=np.float64)
+        turb = np.frombuffer((ctypes.c_double * (11 * n + nx)).in_dll(lib, "turbulence_"), dtype=np.float64)
         arrs["mut"] = turb[0:n].reshape(nz, ny, nx)
-        arrs["ktur"] = turb[n + nx + n:n + nx + 2 * n].reshape(nz, ny, nx)
-        arrs["eps"] = turb[n + nx + 2 * n:n + nx + 3 * n].reshape(nz, ny, nx)
+        arrs["lmix"] = turb[n:n + nx]
+        for q, nm in enumerate(("ktur", "eps", "cmu", "u2p", "v2p", "w2p", "uvp", "uwp", "vwp")):
+            arrs[nm] = turb[n + nx + (q + 1) * n:n + nx + (q + 2) * n].reshape(nz, ny, nx)
         arrs["cc"] = zslot(icc[ng - 1])
         arrs["p"] = zslot(iu[ng - 1])
         # /CYCLING/ MAXCFL,OMEGA (2 doubles = 4 ints), then CYCLES,NPRE,NPOST,NTOP,
@@ -129,6 +130,8 @@ def _child(lib_path, inputdata, nsteps, init_fields, want_fields, conn, every_st
             lib.vofmethod_(ctypes.byref(ti))
             if hswitch == 1:
                 lib.calch3d_(zp(icc[ng - 1]))
+            if iturb in (1, 2):
+                lib.eddyvisc_(zp(icc[ng - 1]), zp(iu[ng - 1]))
             if iturb in (3, 4):
                 lib.kepsmodel_(ctypes.byref(ti), zp(iu[ng - 1]), zp(ires[ng - 1]))
             lib.checktime_(ctypes.byref(ti))

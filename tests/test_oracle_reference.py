@@ -22,7 +22,18 @@ RUNS = [
     ("18^3 more sweeps, omega 1.2", dict(nx=18, ny=18, nz=18, npre=4, ntop=6, npost=3, omega=1.2, cycles=3), 6, None),
     ("18^3 k-epsilon", dict(nx=18, ny=18, nz=18, iturb=3), 5, dict(ktur=2e-4, eps=3e-3 * 0.004)),
     ("34^3 surface tension off", dict(nx=34, ny=34, nz=34, sigma=0.0), 4, None),
+    ("18^3 mixing length (iturb 1), liquid moving in x", dict(nx=18, ny=18, nz=18, iturb=1, xvel1=1.0, yvel1=0.0,
+                                                             y0=0.002), 5, None),
+    ("34x18x18 mixing length (iturb 1), inflow 0.2", dict(nx=34, ny=18, nz=18, iturb=1, xvel1=1.0, yvel1=0.0,
+                                                         xvel2=0.2, y0=0.002, radius=0.0006), 5, None),
+    ("34x18x18 mixing length (iturb 2), faster drop", dict(nx=34, ny=18, nz=18, iturb=2, xvel1=0.8, yvel1=0.0,
+                                                          y0=0.002, radius=0.0005), 5, None),
+    ("18^3 anisotropic k-epsilon (iturb 4)", dict(nx=18, ny=18, nz=18, iturb=4), 5, dict(ktur=2e-4, eps=3e-3 * 0.004)),
+    ("34x18x18 anisotropic k-epsilon (iturb 4), upwind, inflow",
+     dict(nx=34, ny=18, nz=18, iturb=4, convectype=1, xvel2=0.5, y0=0.002, radius=0.0005), 4,
+     dict(ktur=1e-4, eps=1e-3 * 0.004)),
 ]
+TURB_FIELDS = ("cmu", "u2p", "v2p", "w2p", "uvp", "uwp", "vwp")
 
 
 @pytest.mark.parametrize("name,kw,nsteps,init", RUNS, ids=[r[0] for r in RUNS])
@@ -30,7 +41,8 @@ def test_whole_steps_bitexact(name, kw, nsteps, init):
     nx, ny, nz = kw["nx"], kw["ny"], kw["nz"]
     need(nx, ny, nz)
     txt = inputdata_text(**kw)
-    ref = refrun.run_reference(txt, nsteps, init_fields=init)
+    fields = refrun.FIELD_NAMES + (TURB_FIELDS if kw.get("iturb", 0) else ())
+    ref = refrun.run_reference(txt, nsteps, init_fields=init, want_fields=fields)
     o = Oracle(txt)
     for k, v in (init or {}).items():
         o.field(k)[...] = v
@@ -38,8 +50,13 @@ def test_whole_steps_bitexact(name, kw, nsteps, init):
     for hr, ho in zip(ref["hist"], hist):
         for key in ("tau", "time", "tauh", "tauh2", "npre", "ntop", "npost", "tswap", "vol"):
             assert hr[key] == ho[key], (hr["t"], key, hr[key], ho[key])
-    for f in refrun.FIELD_NAMES:
+    for f in fields:
         assert np.array_equal(ref["fields"][f], o.field(f), equal_nan=True), f
+    if kw.get("iturb", 0):
+        # the model did something: a nonzero eddy viscosity in the interior
+        assert np.abs(o.field("mut")[1:-1, 1:-1, 1:-1]).max() > 0.0
+    if kw.get("iturb") == 4:
+        assert np.abs(o.field("uvp")[1:-1, 1:-1, 1:-1]).max() > 0.0
 
 
 def test_nan_from_smooth2_overshoot_reproduced():

@@ -414,7 +414,7 @@ def set_lookahead(ratio=1e3):
 
 # kernel families dg_profile_start accepts
 PROFILE_KERNELS = {"gs_colour": 1, "resid": 2, "pcg_dir": 3, "pcg_update": 4, "relax": 5, "sweep": 6,
-                   "rb_colour": 7}
+                   "rb_colour": 7, "conv_fwd": 8}
 
 
 def profile_start(kernel):
@@ -426,7 +426,8 @@ def profile_start(kernel):
 
 def profile_stop():
     """Stop timing; returns dict(total_ms, launches, bytes) where bytes is the
-    algorithmic byte count of the timed launches."""
+    algorithmic byte count of the timed launches (for conv_fwd: their
+    algorithmic FLOPs)."""
     ms, nl, by = ctypes.c_double(), ctypes.c_int(), ctypes.c_double()
     _check(lib().dg_profile_stop(ctypes.byref(ms), ctypes.byref(nl), ctypes.byref(by)), "dg_profile_stop")
     return dict(total_ms=ms.value, launches=nl.value, bytes=by.value)

@@ -5,6 +5,7 @@
 
 #include "dg_conv.h"
 #include "dg_internal.h"
+#include "dg_prof.h"
 
 namespace dg {
 
@@ -128,6 +129,7 @@ int conv3d_fwd(int nx, int ny, int nz, const bf16_t *x, int xs, int cinp, const 
     nb = (nb + 7) / 8 * 8;
     const dim3 grid((unsigned)nb), block(256);
     const int cg = cinp / 8;
+    prof_begin(PK_CONV_FWD, s);
     switch (coutp / 16) {
     case 1: hipLaunchKernelGGL(k_conv_fwd<1>, grid, block, 0, s, nx, ny, nz, x, xs, cg, wB, coutp, bias, cout, relu, y, ys, yf, yfs); break;
     case 2: hipLaunchKernelGGL(k_conv_fwd<2>, grid, block, 0, s, nx, ny, nz, x, xs, cg, wB, coutp, bias, cout, relu, y, ys, yf, yfs); break;
@@ -140,6 +142,9 @@ int conv3d_fwd(int nx, int ny, int nz, const bf16_t *x, int xs, int cinp, const 
         return -1;
     }
     DG_CHECK_LAUNCH();
+    // algorithmic FLOPs: a multiply and an add per tap, input channel (as
+    // padded) and output channel of every voxel
+    prof_end(PK_CONV_FWD, s, 2.0 * (double)V * 27.0 * (double)cinp * (double)cout);
     return 0;
 }
 

@@ -18,6 +18,7 @@ enum ProfKernel {
     PK_RELAX = 5,       // reference relax colour pass (parity mode)
     PK_SWEEP = 6,       // fused two-colour smoother sweep
     PK_RB_COLOUR = 7,   // finest-level colour pass of the red-black split FP32 V-cycle
+    PK_CONV_FWD = 8,    // BF16 MFMA convolution forward (the guess network); counts FLOPs, not bytes
 };
 
 struct Prof {

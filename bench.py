@@ -46,6 +46,7 @@ sys.path.insert(0, HERE)
 
 METRIC = "time steps/s at 256^3 to 1e-8 Poisson residual: learned+mixed MG vs FP64 MG"
 HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK_TFLOPS = 2500.0  # dense BF16 MFMA of one MI355X
 # HBM bytes per launch of the dominant kernel family at 258^3, from
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this script scaled by
 # the same counters on a plain 256 MiB copy (fetch x1.9998, write x1.0):
@@ -358,6 +359,22 @@ def main():
                                                "mg_iters_per_step": float(np.mean(mixed["iters"])), "guess": mguess,
                                                "speedup_vs_fp64": mv / dv}
     line.update(out)
+    # the guess network's BF16 MFMA convolutions (forward) on an input of
+    # the workload's next-coarser level, timed with HIP events around their
+    # launches on the network's stream
+    nc = (n - 2) // 2
+    feat = np.random.default_rng(5).standard_normal((nc, nc, nc, 4)).astype(np.float32)
+    gs.forward(feat)
+    dg.profile_start("conv_fwd")
+    for _ in range(3):
+        gs.forward(feat)
+    cp = dg.profile_stop()
+    conv_tf = cp["bytes"] / (cp["total_ms"] * 1e-3) / 1e12
+    line["mfma_conv"] = {"kernel": "k_conv_fwd (BF16 MFMA 16x16x32, FP32 accumulate), every layer of the network",
+                         "bound": "mfma", "achieved": conv_tf, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": conv_tf / MFMA_BF16_PEAK_TFLOPS, "input": [nc, nc, nc, 4],
+                         "flops_per_forward": cp["bytes"] / 3, "ms_per_forward": cp["total_ms"] / 3,
+                         "launches_per_forward": cp["launches"] // 3}
     if not args.no_output and mixed is not None:
         # the mixed path again, with printout's files at tprint 5 and
         # ntecplot 10 (a full-field Tecplot file every 10 steps), written to

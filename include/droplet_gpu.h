@@ -408,7 +408,9 @@ int dg_cfl_noalloc(const double *u, const double *v, const double *w, const int 
  * time, the launch count and the summed algorithmic bytes of those launches.
  * kernel ids: 1 tolerance-mode Gauss-Seidel colour pass, 2 its residual,
  * 3 PCG direction + A.p, 4 PCG update, 5 reference relax colour pass,
- * 6 fused two-colour smoother sweep. */
+ * 6 fused two-colour smoother sweep, 7 finest-level colour pass of the
+ * red-black split FP32 V-cycle, 8 BF16 MFMA convolution forward of the guess
+ * network (total_bytes then holds the launches' algorithmic FLOPs). */
 int dg_profile_start(const int *kernel_id);
 int dg_profile_stop(double *total_ms, int *launches, double *total_bytes);
 

@@ -142,3 +142,45 @@ def test_fallback_controller_skips_after_a_weak_step(dg, gs):
     finally:
         gs.set_controller()
         g.close()
+
+
+def test_committed_weights_on_held_out_cases(dg, gs):
+    """The committed network (weights/guess_unet.npz, trained on 256^3 splash
+    cases) on held-out cases of its training distribution, in evaluation
+    mode: the three iteration counts of the same system.  The 100-case
+    evaluation (weights/guess_unet_eval.json: zero 9.03, previous 9.03,
+    learned 9.19 mean iterations, no case better, 16 worse) shows that the
+    network does not save iterations; this test runs two cases and checks
+    that every start reaches the tolerance and that the learned start costs
+    at most one iteration more than the zero start."""
+    import os
+    import sys
+    here = os.path.dirname(os.path.abspath(__file__))
+    sys.path.insert(0, os.path.join(os.path.dirname(here), "3d_droplet_simulation_2007_amd"))
+    import train_guess as tg
+    gs.load_weights()
+    dg.set_smoother(1.15, 4)
+    rng = np.random.default_rng(424242)
+    try:
+        for ci in range(2):
+            p = tg.case_params(rng)
+            print(f"case held-out {ci}: r={p['radius']*1e3:.2f} mm y0={p['y0']*1e3:.2f} mm v={p['yvel1']:.2f}",
+                  flush=True)
+            g = tg.open_case(dg, 258, p, 1e-8, 1)
+            try:
+                for t in range(3):
+                    g.step(t)
+                gs.eval_mode(True)
+                try:
+                    g.step(3)
+                finally:
+                    gs.eval_mode(False)
+                it = gs.eval_iters()
+                print(it, flush=True)
+                assert g.state()["mg_relres"] < 1e-8
+                assert min(it.values()) > 0
+                assert it["learned"] <= it["zero"] + 1
+            finally:
+                g.close()
+    finally:
+        dg.set_smoother()

@@ -263,9 +263,74 @@ std::string list_real(double x)
     return s + std::string(d, (size_t)e) + "." + (d + e);
 }
 
-// Ew.d, right-justified in w characters (w asterisks when it does not fit)
-void edit_e(char *out, double x, int w, int dd)
+// E15.7 of a finite nonzero x without printf: the 7 digits are the integer
+// nearest to m = |x| 10^(6 - e), formed with one correctly rounded
+// multiplication or division by an exact power of ten (relative error below
+// 2^-53, so below 1.2e-9 on m).  When m lies within 1e-6 of a rounding tie
+// the nearest integer is not certain and the caller takes the exact path;
+// likewise for scales beyond 10^22.  Returns false then.
+static bool fast_e15_7(double x, char *out)
 {
+    static const double P10[23] = {1e0,  1e1,  1e2,  1e3,  1e4,  1e5,  1e6,  1e7,  1e8,  1e9,  1e10, 1e11,
+                                   1e12, 1e13, 1e14, 1e15, 1e16, 1e17, 1e18, 1e19, 1e20, 1e21, 1e22};
+    const double a = fabs(x);
+    int e = (int)floor(log10(a));
+    for (int pass = 0; pass < 3; pass++) {
+        const int sc = 6 - e;
+        if (sc > 22 || sc < -22) return false;
+        const double m = sc >= 0 ? a * P10[sc] : a / P10[-sc];
+        // the decade with 10^6 <= m < 10^7; a value on the edge of two
+        // decades gives the same digits from either (it rounds to 10^6 x
+        // 10^k both ways), unless it is a tie, which is tested below
+        if (m < 1e6) {
+            e -= 1;
+            continue;
+        }
+        if (m >= 1e7) {
+            e += 1;
+            continue;
+        }
+        const double fl = floor(m), fr = m - fl;
+        if (fabs(fr - 0.5) < 1e-6) return false;
+        long long dg = (long long)fl + (fr > 0.5 ? 1 : 0);
+        int ex = e + 1;
+        if (dg > 9999999) {  // rounded up to 10^7: 0.1000000 of the next decade
+            dg /= 10;
+            ex += 1;
+        }
+        char s[24];
+        int n = 0;
+        if (std::signbit(x)) s[n++] = '-';
+        s[n++] = '0';
+        s[n++] = '.';
+        for (int q = 6; q >= 0; q--) {
+            s[n + q] = (char)('0' + dg % 10);
+            dg /= 10;
+        }
+        n += 7;
+        const int ae = ex < 0 ? -ex : ex;
+        if (ae <= 99) {
+            s[n++] = 'E';
+            s[n++] = ex < 0 ? '-' : '+';
+        } else {
+            s[n++] = ex < 0 ? '-' : '+';
+            s[n++] = (char)('0' + ae / 100);
+        }
+        s[n++] = (char)('0' + (ae / 10) % 10);
+        s[n++] = (char)('0' + ae % 10);
+        memset(out, ' ', (size_t)(15 - n));
+        memcpy(out + (15 - n), s, (size_t)n);
+        return true;
+    }
+    return false;
+}
+
+// Ew.d, right-justified in w characters (w asterisks when it does not fit).
+// exact: always through printf's correctly rounded digits (E15.7 otherwise
+// takes fast_e15_7 where it is certain).
+void edit_e(char *out, double x, int w, int dd, bool exact = false)
+{
+    if (!exact && w == 15 && dd == 7 && x != 0.0 && std::isfinite(x) && fast_e15_7(x, out)) return;
     char s[64];
     if (std::isnan(x)) {
         snprintf(s, sizeof s, "NaN");
@@ -594,8 +659,10 @@ int write_tecplot(Writer &W, const Job &j, const Slot &sl)
             }
         }
     };
-    unsigned nth = std::thread::hardware_concurrency();
-    nth = std::max(1u, std::min(nth, 16u));
+    // formatting threads: the host's cores but two, kept for the thread
+    // that queues the time steps and for the runtime's own
+    const unsigned hw = std::thread::hardware_concurrency();
+    const unsigned nth = std::max(1u, std::min(hw > 2 ? hw - 2 : 1u, 16u));
     // planes in batches of nth, each plane formatted by one thread
     std::vector<std::string> bufs(nth);
     for (int k0 = 1; k0 <= nz && rc == 0; k0 += (int)nth) {
@@ -908,11 +975,11 @@ int dg_format_real(const double *x, const int *mode, char *buf, const int *len)
     std::string s;
     if (*mode == 0) {
         s = fmt::list_real(*x);
-    } else if (*mode == 1) {
+    } else if (*mode == 1 || *mode == 2) {
         s.assign(15, ' ');
-        fmt::edit_e(&s[0], *x, 15, 7);
+        fmt::edit_e(&s[0], *x, 15, 7, *mode == 2);
     } else {
-        set_error("dg_format_real: mode %d (0 list-directed, 1 E15.7)", *mode);
+        set_error("dg_format_real: mode %d (0 list-directed, 1 E15.7, 2 E15.7 through printf's digits)", *mode);
         return -1;
     }
     if ((int)s.size() + 1 > *len) {

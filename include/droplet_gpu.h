@@ -207,8 +207,9 @@ int dg_savedata(const char *path);
 int dg_save_state(const char *path);
 int dg_load_state(const char *path);
 /* Host formatting, no GPU (no reference counterpart): x as flang writes it
- * list-directed (mode 0) or with the reference's E15.7 (mode 1), into buf
- * of len bytes, NUL-terminated. */
+ * list-directed (mode 0) or with the reference's E15.7 (mode 1: the
+ * writers' form; mode 2: the same through printf's correctly rounded
+ * digits, a check of mode 1), into buf of len bytes, NUL-terminated. */
 int dg_format_real(const double *x, const int *mode, char *buf, const int *len);
 
 /* ---- host-pointer kernels with the reference's argument lists --------- */

@@ -98,3 +98,30 @@ def test_formats_match_flang(dg, tmp_path):
             if got != want:
                 bad.append((mode, repr(float(x)), got, want))
     assert not bad, bad[:20]
+
+
+def test_fast_e15_7_equals_exact_digits(dg):
+    """The writers' E15.7 (mode 1, integer rounding of a scaled value where
+    it is certain) against the same edit through printf's correctly rounded
+    digits (mode 2) on 200000 values: log-uniform magnitudes, values on and
+    next to 7-digit rounding ties, powers of ten and their neighbours."""
+    lib = dg.lib()
+    rng = np.random.default_rng(7)
+    mags = 10.0 ** rng.uniform(-40, 40, 100000) * rng.choice([-1.0, 1.0], 100000)
+    ties = (rng.integers(1000000, 9999999, 40000) + 0.5) * 10.0 ** rng.integers(-30, 30, 40000).astype(float)
+    near = np.concatenate([np.nextafter(ties[:20000], 0.0), np.nextafter(ties[:20000], np.inf)])
+    p10 = 10.0 ** np.arange(-30, 31, dtype=float)
+    edge = np.concatenate([p10, np.nextafter(p10, 0.0), np.nextafter(p10, np.inf), 9.9999995 * p10,
+                           np.nextafter(9.9999995 * p10, 0.0), 0.99999995 * p10])
+    vals = np.concatenate([mags, ties, near, edge, rng.standard_normal(20000)])
+    buf1, buf2 = ctypes.create_string_buffer(32), ctypes.create_string_buffer(32)
+    n = ctypes.c_int(32)
+    m1, m2 = ctypes.c_int(1), ctypes.c_int(2)
+    bad = []
+    for x in vals:
+        xd = ctypes.c_double(float(x))
+        lib.dg_format_real(ctypes.byref(xd), ctypes.byref(m1), buf1, ctypes.byref(n))
+        lib.dg_format_real(ctypes.byref(xd), ctypes.byref(m2), buf2, ctypes.byref(n))
+        if buf1.value != buf2.value:
+            bad.append((repr(float(x)), buf1.value, buf2.value))
+    assert not bad, bad[:10]

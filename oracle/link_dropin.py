@@ -6,8 +6,9 @@ units whose call forms libdroplet_gpu.so defines (the time step's routines
 that PROGRAM THREE DIM's loop calls, main.f:19-73, and those only they
 call), compiles the rest with ROCm flang next to the reference's unmodified
 input.h, and links it against the library.  Nothing else of main.f is
-changed.  Output: oracle/_ref/three_dim_dropin (git-ignored; built to be
-linked, never run by the tests).  The temporary copy of main.f is deleted.
+changed.  Output: oracle/_ref/three_dim_dropin (git-ignored), which the
+GPU test of printout's files runs.  The temporary copy of main.f is
+deleted.
 
 usage: python oracle/link_dropin.py
 Exit status 0 when the link succeeds; prints the undefined symbols the
@@ -30,8 +31,8 @@ OUT = os.path.join(HERE, "_ref", "three_dim_dropin")
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 
 # program units the library replaces (its exported Fortran call forms)
-REPLACED = ("vofmethod", "swp", "calch3d", "kepsmodel", "checktime", "cfl", "csf", "sgip", "sgip2", "solvelpres",
-            "convective", "mglin")
+REPLACED = ("vofmethod", "swp", "calch3d", "eddyvisc", "kepsmodel", "uv2p", "checktime", "cfl", "csf", "sgip",
+            "sgip2", "solvelpres", "convective", "mglin")
 
 _UNIT = re.compile(r"^[ \t]+(?:(?:double\s+precision|integer|real|logical)\s+)?"
                    r"(subroutine|function|program)\s+(\w+)", re.IGNORECASE)

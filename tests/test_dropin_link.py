@@ -2,9 +2,10 @@
 
 oracle/link_dropin.py compiles main.f where it lies under /root/reference,
 with the program units whose call forms the library defines taken out
-(VOFmethod, swp, CalcH3d, kepsmodel, checktime, CFL, csf, SGIP, SGIP2,
-solvelpres, convective, mglin) and nothing else changed, and links it against the library.  The
-program is linked, never run here: the test checks that every procedure the
+(VOFmethod, swp, CalcH3d, eddyvisc, kepsmodel, uv2p, checktime, CFL, csf,
+SGIP, SGIP2, solvelpres, convective, mglin) and nothing else changed, and
+links it against the library.  The program is linked, not run, here (the
+GPU test of printout's files runs it): the test checks that every procedure the
 rest of main.f calls among those is left undefined in the program and
 defined by the library, and that the program exports the COMMON blocks the
 library binds.  Skipped where the reference or flang is absent.
@@ -37,12 +38,16 @@ def test_reference_program_links_against_library(dg):
     assert r.returncode == 0, r.stdout + r.stderr
     undefined = {ln.split()[-1] for ln in _nm([PROG]) if ln.strip().startswith("U ")}
     for sym in ("vofmethod_", "checktime_", "csf_", "solvelpres_", "kepsmodel_", "cfl_", "calch3d_", "sgip_",
-                "sgip2_"):
+                "sgip2_", "eddyvisc_"):
         assert sym in undefined, sym
     defined = {ln.split()[-1] for ln in _nm(["-D", "--defined-only", LIB]) if ln.strip()}
     for sym in ("vofmethod_", "checktime_", "csf_", "solvelpres_", "kepsmodel_", "cfl_", "mglin_", "swp_",
-                "convective_"):
+                "convective_", "calch3d_", "sgip_", "sgip2_", "eddyvisc_", "uv2p_"):
         assert sym in defined, sym
+    # no unit of the program defines a replaced procedure any more
+    own = {ln.split()[-1] for ln in _nm(["--defined-only", PROG]) if ln.strip()}
+    for sym in ("vofmethod_", "solvelpres_", "kepsmodel_", "eddyvisc_", "uv2p_", "calch3d_", "sgip_"):
+        assert sym not in own, sym
     exported = {ln.split()[-1] for ln in _nm(["-D", PROG]) if ln.strip()}
     for blk in COMMONS:
         assert blk in exported, blk

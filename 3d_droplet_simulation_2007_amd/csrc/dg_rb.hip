@@ -89,36 +89,44 @@ __global__ void __launch_bounds__(256) k_rb_in(Grid g, RbGeom r, const double *s
 // nb(m', dj, dk) is the other colour's value at entry m' of row (j+dj,
 // k+dk), uold the cell's own value before the pass.  Every split colour
 // update (the colour passes, the tail, the fused sweeps) goes through this
-// arithmetic, so their results agree to the bit.
+// arithmetic, so their results agree to the bit.  Every load is issued:
+// a missing neighbour reads the cell's own entry (or its own row) instead,
+// and the value is then selected away, so the loads of a cell are in flight
+// together rather than each behind a branch.  Offsets are 32-bit (every
+// split level has fewer than 2^31 entries).
 template <typename CT, typename T, bool ZERO, class NB>
 __device__ inline T rb_cell_value(const RbGeom &r, const NB &nb, const T *f, const T *uold_arr, const Coef<CT> &cf,
                                   T h2, int colour, T om, int zown, int m, int j, int k)
 {
     const int p = (colour + j + k) & 1;
     const int i = 2 + p + 2 * m;
-    const size_t row = ((size_t)(k - 2) * r.my + (j - 2)) * r.mh;
-    const size_t own = (size_t)colour * r.half + row + m;
-    const size_t oth = (size_t)(1 - colour) * r.half;
-    const size_t rs = (size_t)r.mh, ps = (size_t)r.mh * r.my;  // row and plane strides of one colour
+    const int rs = r.mh, ps = r.mh * r.my;  // row and plane strides of one colour
+    const int row = ((k - 2) * r.my + (j - 2)) * r.mh;
+    const int own = colour * (int)r.half + row + m;
+    const int oth = (1 - colour) * (int)r.half + row;
+    const bool hw = i > 2, he = i < r.nx - 1, hn = j < r.ny - 1, hs = j > 2, ht = k < r.nz - 1, hb = k > 2;
     const T ae = cf.a(own), bn = cf.b(own), ct = cf.c(own);
-    const bool hw = i > 2, hn = j < r.ny - 1, hs = j > 2, ht = k < r.nz - 1, hb = k > 2;
-    const T aw = hw ? (T)cf.a(oth + row + m + p - 1) : (T)0;
-    const T bs = hs ? (T)cf.b(oth + row - rs + m) : (T)0;
-    const T cb = hb ? (T)cf.c(oth + row - ps + m) : (T)0;
+    const T awv = cf.a(hw ? oth + m + p - 1 : own);
+    const T bsv = cf.b(hs ? oth - rs + m : own);
+    const T cbv = cf.c(hb ? oth - ps + m : own);
+    const T aw = hw ? awv : (T)0, bs = hs ? bsv : (T)0, cb = hb ? cbv : (T)0;
     T s = (T)0, d = ae;
     if (!ZERO) {
-        if (i < r.nx - 1) s += ae * nb(m + p, 0, 0);
-        if (hw) s += aw * nb(m + p - 1, 0, 0);
-        if (hn) s += bn * nb(m, 1, 0);
-        if (hs) s += bs * nb(m, -1, 0);
-        if (ht) s += ct * nb(m, 0, 1);
-        if (hb) s += cb * nb(m, 0, -1);
+        const T ue = nb(he ? m + p : m, 0, 0), uw = nb(hw ? m + p - 1 : m, 0, 0);
+        const T un = nb(m, hn ? 1 : 0, 0), us = nb(m, hs ? -1 : 0, 0);
+        const T ut = nb(m, 0, ht ? 1 : 0), ub = nb(m, 0, hb ? -1 : 0);
+        s = he ? s + ae * ue : s;
+        s = hw ? s + aw * uw : s;
+        s = hn ? s + bn * un : s;
+        s = hs ? s + bs * us : s;
+        s = ht ? s + ct * ut : s;
+        s = hb ? s + cb * ub : s;
     }
-    if (hw) d += aw;
-    if (hn) d += bn;
-    if (hs) d += bs;
-    if (ht) d += ct;
-    if (hb) d += cb;
+    d = hw ? d + aw : d;
+    d = hn ? d + bn : d;
+    d = hs ? d + bs : d;
+    d = ht ? d + ct : d;
+    d = hb ? d + cb : d;
     const T gsv = (s - h2 * f[own]) / d;
     const T uold = (ZERO || zown || om == (T)1) ? (T)0 : uold_arr[own];  // Gauss-Seidel needs no old value
     return om * gsv + ((T)1 - om) * uold;
@@ -129,11 +137,11 @@ template <typename CT, typename T, bool ZERO>
 __device__ inline void rb_cell(const RbGeom &r, T *u, const T *f, const Coef<CT> &cf, T h2, int colour, T om,
                                int zown, int m, int j, int k)
 {
-    const size_t row = ((size_t)(k - 2) * r.my + (j - 2)) * r.mh;
-    const T *o = u + (size_t)(1 - colour) * r.half + row;
-    const size_t rs = (size_t)r.mh, ps = (size_t)r.mh * r.my;
-    auto nb = [&](int mm, int dj, int dk) { return o[(ptrdiff_t)dj * (ptrdiff_t)rs + (ptrdiff_t)dk * (ptrdiff_t)ps + mm]; };
-    u[(size_t)colour * r.half + row + m] = rb_cell_value<CT, T, ZERO>(r, nb, f, u, cf, h2, colour, om, zown, m, j, k);
+    const int row = ((k - 2) * r.my + (j - 2)) * r.mh;
+    const T *o = u + (1 - colour) * (int)r.half + row;
+    const int rs = r.mh, ps = r.mh * r.my;
+    auto nb = [&](int mm, int dj, int dk) { return o[dj * rs + dk * ps + mm]; };
+    u[colour * (int)r.half + row + m] = rb_cell_value<CT, T, ZERO>(r, nb, f, u, cf, h2, colour, om, zown, m, j, k);
 }
 
 template <typename CT, typename T, bool ZERO, bool TOP>
@@ -452,16 +460,19 @@ __device__ inline void rb_restrict_cell(const RbGeom &rf, const T *u, const T *f
             for (int dx = 0; dx < 2; dx++) {
                 const int i = i0 + dx, j = j0 + dy, k = k0 + dz;
                 const unsigned e = at(dx, dy, dz);
+                const bool he = i < rf.nx - 1, hw = i > 2, hn = j < rf.ny - 1, hs = j > 2, ht = k < rf.nz - 1,
+                           hb = k > 2;
+                // every load issued, a missing neighbour reading the child
+                // itself; the value is selected afterwards
                 const T u0 = u[e];
-                const T ue = i < rf.nx - 1 ? u[at(dx + 1, dy, dz)] : (T)0;
-                const T uw = i > 2 ? u[at(dx - 1, dy, dz)] : u0;
-                const T un = j < rf.ny - 1 ? u[at(dx, dy + 1, dz)] : u0;
-                const T us = j > 2 ? u[at(dx, dy - 1, dz)] : u0;
-                const T ut = k < rf.nz - 1 ? u[at(dx, dy, dz + 1)] : u0;
-                const T ub = k > 2 ? u[at(dx, dy, dz - 1)] : u0;
-                const T aw = i > 2 ? (T)cf.a(at(dx - 1, dy, dz)) : (T)0;
-                const T bs = j > 2 ? (T)cf.b(at(dx, dy - 1, dz)) : (T)0;
-                const T cb = k > 2 ? (T)cf.c(at(dx, dy, dz - 1)) : (T)0;
+                const T uev = u[he ? at(dx + 1, dy, dz) : e], uwv = u[hw ? at(dx - 1, dy, dz) : e];
+                const T unv = u[hn ? at(dx, dy + 1, dz) : e], usv = u[hs ? at(dx, dy - 1, dz) : e];
+                const T utv = u[ht ? at(dx, dy, dz + 1) : e], ubv = u[hb ? at(dx, dy, dz - 1) : e];
+                const T awv = (T)cf.a(hw ? at(dx - 1, dy, dz) : e), bsv = (T)cf.b(hs ? at(dx, dy - 1, dz) : e),
+                        cbv = (T)cf.c(hb ? at(dx, dy, dz - 1) : e);
+                const T ue = he ? uev : (T)0, uw = hw ? uwv : u0, un = hn ? unv : u0, us = hs ? usv : u0,
+                        ut = ht ? utv : u0, ub = hb ? ubv : u0;
+                const T aw = hw ? awv : (T)0, bs = hs ? bsv : (T)0, cb = hb ? cbv : (T)0;
                 const T s = (T)cf.a(e) * (ue - u0) + aw * (uw - u0) + (T)cf.b(e) * (un - u0) + bs * (us - u0) +
                             (T)cf.c(e) * (ut - u0) + cb * (ub - u0);
                 t += f[e] - h2i * s;
@@ -601,48 +612,62 @@ struct RbTail {
     Grid g0;  // coarsest level (natural layout)
     T *f0, *u0;
     const double *M;
+    // LDS form: byte offsets of level l's u, f and coefficients (index l),
+    // and of the coarsest level's f and u (index 0: f0, index top + 1: u0)
+    unsigned lu[TAIL_MAX_LEVELS + 1], lf[TAIL_MAX_LEVELS + 1], lc[TAIL_MAX_LEVELS];
 };
 
-// LDS bytes of the staged tail: per level u, f (type T) and 3 coefficient
-// planes, then the coarsest level's f and u, 16-byte aligned pieces
+// LDS layout of the staged tail: per level u, f (type T) and 3 coefficient
+// planes, then the coarsest level's f and u, 16-byte aligned pieces.
+// Fills the offsets of t and returns the total.
 template <typename CT, typename T>
-__host__ __device__ inline size_t tail_lds_bytes(const RbTail<CT, T> &t)
+inline size_t tail_lds_layout(RbTail<CT, T> &t)
 {
+    auto al = [](size_t b) { return (b + 15) & ~(size_t)15; };
     size_t b = 0;
     for (int l = 1; l <= t.top; l++) {
         const size_t n2 = 2 * t.r[l].half;
-        b += 2 * ((n2 * sizeof(T) + 15) & ~(size_t)15) + ((3 * n2 * sizeof(CT) + 15) & ~(size_t)15);
+        t.lu[l] = (unsigned)b;
+        b += al(n2 * sizeof(T));
+        t.lf[l] = (unsigned)b;
+        b += al(n2 * sizeof(T));
+        t.lc[l] = (unsigned)b;
+        b += al(3 * n2 * sizeof(CT));
     }
     const size_t n0 = (size_t)t.g0.nx * t.g0.ny * t.g0.nz;
-    return b + 2 * ((n0 * sizeof(T) + 15) & ~(size_t)15);
+    t.lf[0] = (unsigned)b;
+    b += al(n0 * sizeof(T));
+    t.lu[t.top + 1] = (unsigned)b;
+    b += al(n0 * sizeof(T));
+    return b;
 }
+
+// one level of the tail as a pass works on it: geometry, arrays (in LDS or
+// HBM) and constants, read from the kernel arguments once per visit
+template <typename CT, typename T>
+struct TailLev {
+    RbGeom r;
+    T *u, *f;
+    Coef<CT> cf;
+    T h2, h2i;
+};
 
 // where level l's arrays live for one form of the tail
 template <typename CT, typename T, bool LDS>
 struct TailMem {
     const RbTail<CT, T> &t;
     unsigned char *lds;
-    __device__ inline size_t base(int l) const  // LDS offset of level l's u
-    {
-        size_t b = 0;
-        for (int q = 1; q < l; q++) {
-            const size_t n2 = 2 * t.r[q].half;
-            b += 2 * ((n2 * sizeof(T) + 15) & ~(size_t)15) + ((3 * n2 * sizeof(CT) + 15) & ~(size_t)15);
-        }
-        return b;
-    }
-    __device__ inline size_t fl(int l) const { return (2 * t.r[l].half * sizeof(T) + 15) & ~(size_t)15; }
-    __device__ inline T *u(int l) const { return LDS ? (T *)(lds + base(l)) : t.u[l]; }
-    __device__ inline T *f(int l) const { return LDS ? (T *)(lds + base(l) + fl(l)) : t.f[l]; }
+    __device__ inline T *u(int l) const { return LDS ? (T *)(lds + t.lu[l]) : t.u[l]; }
+    __device__ inline T *f(int l) const { return LDS ? (T *)(lds + t.lf[l]) : t.f[l]; }
     __device__ inline Coef<CT> cf(int l) const
     {
-        return LDS ? Coef<CT>{(const CT *)(lds + base(l) + 2 * fl(l)), 2 * t.r[l].half} : t.cf[l];
+        return LDS ? Coef<CT>{(const CT *)(lds + t.lc[l]), 2 * t.r[l].half} : t.cf[l];
     }
-    __device__ inline T *f0() const { return LDS ? (T *)(lds + base(t.top + 1)) : t.f0; }
-    __device__ inline T *u0() const
+    __device__ inline T *f0() const { return LDS ? (T *)(lds + t.lf[0]) : t.f0; }
+    __device__ inline T *u0() const { return LDS ? (T *)(lds + t.lu[t.top + 1]) : t.u0; }
+    __device__ inline TailLev<CT, T> level(int l) const
     {
-        const size_t n0 = (size_t)t.g0.nx * t.g0.ny * t.g0.nz;
-        return LDS ? (T *)(lds + base(t.top + 1) + ((n0 * sizeof(T) + 15) & ~(size_t)15)) : t.u0;
+        return TailLev<CT, T>{t.r[l], u(l), f(l), cf(l), t.h2[l], t.h2i[l]};
     }
 };
 
@@ -673,18 +698,13 @@ __device__ inline TailCells tail_cells(const RbGeom &r)
     return c;
 }
 
-template <typename CT, typename T, bool LDS, bool ZERO>
-__device__ inline void tail_pass(const TailMem<CT, T, LDS> &mem, const TailCells &cells, int l, int colour, int zown)
+template <typename CT, typename T, bool ZERO>
+__device__ inline void tail_pass(const TailLev<CT, T> &L, const TailCells &cells, T om, int colour, int zown)
 {
-    const RbTail<CT, T> &t = mem.t;
-    const RbGeom &r = t.r[l];
-    T *u = mem.u(l);
-    const T *f = mem.f(l);
-    const Coef<CT> cf = mem.cf(l);
 #pragma unroll
     for (int s = 0; s < TAIL_CPT; s++)
         if (s < cells.n)
-            rb_cell<CT, T, ZERO>(r, u, f, cf, t.h2[l], colour, t.om, zown, cells.m[s], cells.j[s], cells.k[s]);
+            rb_cell<CT, T, ZERO>(L.r, L.u, L.f, L.cf, L.h2, colour, om, zown, cells.m[s], cells.j[s], cells.k[s]);
     __syncthreads();
 }
 
@@ -692,46 +712,52 @@ template <typename CT, typename T, bool LDS>
 __device__ inline void tail_body(const TailMem<CT, T, LDS> &mem, double *y)
 {
     const RbTail<CT, T> &t = mem.t;
+    const T om = t.om;
     for (int l = t.top; l >= 1; l--) {
-        const TailCells cells = tail_cells(t.r[l]);
-        for (int q = 0; q < t.nu1[l]; q++) {
+        const TailLev<CT, T> L = mem.level(l);
+        const TailCells cells = tail_cells(L.r);
+        const int nu1 = t.nu1[l];
+        for (int q = 0; q < nu1; q++) {
             if (q == 0)
-                tail_pass<CT, T, LDS, true>(mem, cells, l, 0, 1);
+                tail_pass<CT, T, true>(L, cells, om, 0, 1);
             else
-                tail_pass<CT, T, LDS, false>(mem, cells, l, 0, 0);
-            tail_pass<CT, T, LDS, false>(mem, cells, l, 1, q == 0);
+                tail_pass<CT, T, false>(L, cells, om, 0, 0);
+            tail_pass<CT, T, false>(L, cells, om, 1, q == 0);
         }
-        const RbGeom &rf = t.r[l];
+        const RbGeom &rf = L.r;
         const bool csplit = l - 1 >= 1;
-        const T *u = mem.u(l), *f = mem.f(l);
-        const Coef<CT> cf = mem.cf(l);
         T *fc = csplit ? mem.f(l - 1) : mem.f0();
+        const RbGeom rc = t.r[csplit ? l - 1 : l];
+        const Grid g0 = t.g0;
         const int cx = rf.mh, cy = rf.my / 2, n = cx * cy * (rf.mz / 2);  // coarse interior
         for (int e = (int)threadIdx.x; e < n; e += TAIL_THREADS) {
             const int I = e % cx, q = e / cx;
-            rb_restrict_cell<CT, T>(rf, u, f, cf, t.h2i[l], t.r[csplit ? l - 1 : l], csplit ? fc : nullptr, t.g0,
-                                    csplit ? nullptr : fc, I, q % cy, q / cy);
+            rb_restrict_cell<CT, T>(rf, L.u, L.f, L.cf, L.h2i, rc, csplit ? fc : nullptr, g0, csplit ? nullptr : fc, I,
+                                    q % cy, q / cy);
         }
         __syncthreads();
     }
     coarse_subst<T>(t.g0, t.M, mem.f0(), mem.u0(), y, (int)threadIdx.x, TAIL_THREADS);
     __syncthreads();
     for (int l = 1; l <= t.top; l++) {
-        const RbGeom &rf = t.r[l];
+        const TailLev<CT, T> L = mem.level(l);
+        const RbGeom &rf = L.r;
         const bool csplit = l - 1 >= 1;
-        T *u = mem.u(l);
         const T *uc = csplit ? mem.u(l - 1) : mem.u0();
+        const RbGeom rc = t.r[csplit ? l - 1 : l];
+        const Grid g0 = t.g0;
         const int n = (int)rf.half;
         for (int e = (int)threadIdx.x; e < n; e += TAIL_THREADS) {
             const int m = e % rf.mh, q = e / rf.mh;
-            rb_prolong_entry<T>(rf, u, csplit ? t.r[l - 1] : rf, csplit ? uc : nullptr, t.g0, csplit ? nullptr : uc,
-                                m, q % rf.my + 2, q / rf.my + 2);
+            rb_prolong_entry<T>(rf, L.u, rc, csplit ? uc : nullptr, g0, csplit ? nullptr : uc, m, q % rf.my + 2,
+                                q / rf.my + 2);
         }
         __syncthreads();
         const TailCells cells = tail_cells(rf);
-        for (int q = 0; q < t.nu2[l]; q++) {
-            tail_pass<CT, T, LDS, false>(mem, cells, l, 1, 0);
-            tail_pass<CT, T, LDS, false>(mem, cells, l, 0, 0);
+        const int nu2 = t.nu2[l];
+        for (int q = 0; q < nu2; q++) {
+            tail_pass<CT, T, false>(L, cells, om, 1, 0);
+            tail_pass<CT, T, false>(L, cells, om, 0, 0);
         }
     }
 }
@@ -922,7 +948,7 @@ int launch_tail(Ctx &c, int lt)
         DG_HIP(hipGetDevice(&dev));
         DG_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     }
-    const size_t lds = tail_lds_bytes(t);
+    const size_t lds = tail_lds_layout(t);
     if (c.rb_tail == 2 && lds + 512 * sizeof(double) <= (size_t)lds_max) {
         if (lds > 48 * 1024)
             DG_HIP(hipFuncSetAttribute((const void *)k_rb_tail_lds<CT, T>, hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -211,6 +211,10 @@ struct NbGlobal {
 // same operations in the same order as rb_cell, per cell.  own / oth: the
 // group's offsets in its own and in the other colour's half (coefficients
 // of the group's faces and of the other colour's west/south/bottom faces).
+// As in rb_cell_value, every load is issued (a missing neighbour row reads
+// the group's own row, a missing end entry the group's first one) and the
+// values are selected afterwards, so the group's loads are in flight
+// together instead of one round trip per boundary test.
 template <typename CT, typename T, bool ZERO, class NB>
 __device__ inline void rb_group4(const RbGeom &r, const NB &nb, const T *f, const T *uold_own, const Coef<CT> &cf,
                                  size_t own, size_t oth, T h2, int colour, T om, int zown, int m0, int j, int k,
@@ -219,72 +223,77 @@ __device__ inline void rb_group4(const RbGeom &r, const NB &nb, const T *f, cons
     const int p = (colour + j + k) & 1;
     const size_t rs = (size_t)r.mh, ps = (size_t)r.mh * r.my;
     const bool hn = j < r.ny - 1, hs = j > 2, ht = k < r.nz - 1, hb = k > 2;
-    T ae[4], bn[4], ct[4], aw[4] = {(T)0, (T)0, (T)0, (T)0}, bs[4] = {(T)0, (T)0, (T)0, (T)0},
-                           cb[4] = {(T)0, (T)0, (T)0, (T)0};
+    const bool wfirst = m0 > 0, elast = m0 + 4 < r.mh;
+    const bool need_old = !ZERO && !zown && om != (T)1;  // Gauss-Seidel needs no old value
+    T ae[4], bn[4], ct[4], tw[4], bsv[4], cbv[4], fv[4], uo[4] = {(T)0, (T)0, (T)0, (T)0};
     ld4(cf.p + own, ae);
     ld4(cf.p + cf.n2 + own, bn);
     ld4(cf.p + 2 * cf.n2 + own, ct);
-    // west faces: the other colour's a at m+p-1
-    {
-        T t[4];
-        ld4(cf.p + oth, t);
-        if (p) {
-            for (int q = 0; q < 4; q++) aw[q] = t[q];
-        } else {
-            aw[0] = m0 > 0 ? (T)cf.a(oth - 1) : (T)0;
-            aw[1] = t[0];
-            aw[2] = t[1];
-            aw[3] = t[2];
-        }
-    }
-    if (hs) ld4(cf.p + cf.n2 + oth - rs, bs);
-    if (hb) ld4(cf.p + 2 * cf.n2 + oth - ps, cb);
-    T ue[4] = {(T)0, (T)0, (T)0, (T)0}, uw[4] = {(T)0, (T)0, (T)0, (T)0}, un[4], us[4], ut[4], ub[4];
-    if (!ZERO) {
-        T t[4];
-        nb.row4(0, 0, t);
-        if (p) {  // west m .. , east m+1 ..
-            for (int q = 0; q < 4; q++) uw[q] = t[q];
-            ue[0] = t[1];
-            ue[1] = t[2];
-            ue[2] = t[3];
-            ue[3] = m0 + 4 < r.mh ? nb.one(4) : (T)0;
-        } else {  // west m-1 .., east m ..
-            for (int q = 0; q < 4; q++) ue[q] = t[q];
-            uw[0] = m0 > 0 ? nb.one(-1) : (T)0;
-            uw[1] = t[0];
-            uw[2] = t[1];
-            uw[3] = t[2];
-        }
-        if (hn) nb.row4(1, 0, un);
-        if (hs) nb.row4(-1, 0, us);
-        if (ht) nb.row4(0, 1, ut);
-        if (hb) nb.row4(0, -1, ub);
-    }
-    T fv[4], uo[4] = {(T)0, (T)0, (T)0, (T)0};
+    ld4(cf.p + oth, tw);
+    const T aw0v = (T)cf.a(wfirst ? oth - 1 : oth);
+    ld4(cf.p + cf.n2 + (hs ? oth - rs : oth), bsv);
+    ld4(cf.p + 2 * cf.n2 + (hb ? oth - ps : oth), cbv);
     ld4(f + own, fv);
-    if (!ZERO && !zown && om != (T)1) {  // Gauss-Seidel needs no old value
-        ld4(uold_own + own, uo);
-        if (eo)  // the coarse correction not yet added to the own colour
-            for (int q = 0; q < 4; q++) uo[q] = uo[q] + eo[q];
+    if (need_old) ld4(uold_own + own, uo);
+    T uc[4], unv[4], usv[4], utv[4], ubv[4], u1v = (T)0;
+    if (!ZERO) {
+        nb.row4(0, 0, uc);
+        // the entry beyond the group: east (m0+4) when p = 1, west (m0-1) when p = 0
+        u1v = nb.one(p ? (elast ? 4 : 0) : (wfirst ? -1 : 0));
+        nb.row4(hn ? 1 : 0, 0, unv);
+        nb.row4(hs ? -1 : 0, 0, usv);
+        nb.row4(0, ht ? 1 : 0, utv);
+        nb.row4(0, hb ? -1 : 0, ubv);
+    }
+    // west faces: the other colour's a at m+p-1
+    T aw[4], bs[4], cb[4];
+    if (p) {
+        for (int q = 0; q < 4; q++) aw[q] = tw[q];
+    } else {
+        aw[0] = wfirst ? aw0v : (T)0;
+        aw[1] = tw[0];
+        aw[2] = tw[1];
+        aw[3] = tw[2];
+    }
+    for (int q = 0; q < 4; q++) {
+        bs[q] = hs ? bsv[q] : (T)0;
+        cb[q] = hb ? cbv[q] : (T)0;
+    }
+    if (need_old && eo)  // the coarse correction not yet added to the own colour
+        for (int q = 0; q < 4; q++) uo[q] = uo[q] + eo[q];
+    T ue[4] = {(T)0, (T)0, (T)0, (T)0}, uw[4] = {(T)0, (T)0, (T)0, (T)0};
+    if (!ZERO) {
+        if (p) {  // west m .. , east m+1 ..
+            for (int q = 0; q < 4; q++) uw[q] = uc[q];
+            ue[0] = uc[1];
+            ue[1] = uc[2];
+            ue[2] = uc[3];
+            ue[3] = elast ? u1v : (T)0;
+        } else {  // west m-1 .., east m ..
+            for (int q = 0; q < 4; q++) ue[q] = uc[q];
+            uw[0] = wfirst ? u1v : (T)0;
+            uw[1] = uc[0];
+            uw[2] = uc[1];
+            uw[3] = uc[2];
+        }
     }
     for (int q = 0; q < 4; q++) {
         const int i = 2 + p + 2 * (m0 + q);
         const bool hw = i > 2, he = i < r.nx - 1;
         T s = (T)0, d = ae[q];
         if (!ZERO) {
-            if (he) s += ae[q] * ue[q];
-            if (hw) s += aw[q] * uw[q];
-            if (hn) s += bn[q] * un[q];
-            if (hs) s += bs[q] * us[q];
-            if (ht) s += ct[q] * ut[q];
-            if (hb) s += cb[q] * ub[q];
+            s = he ? s + ae[q] * ue[q] : s;
+            s = hw ? s + aw[q] * uw[q] : s;
+            s = hn ? s + bn[q] * unv[q] : s;
+            s = hs ? s + bs[q] * usv[q] : s;
+            s = ht ? s + ct[q] * utv[q] : s;
+            s = hb ? s + cb[q] * ubv[q] : s;
         }
-        if (hw) d += aw[q];
-        if (hn) d += bn[q];
-        if (hs) d += bs[q];
-        if (ht) d += ct[q];
-        if (hb) d += cb[q];
+        d = hw ? d + aw[q] : d;
+        d = hn ? d + bn[q] : d;
+        d = hs ? d + bs[q] : d;
+        d = ht ? d + ct[q] : d;
+        d = hb ? d + cb[q] : d;
         const T gsv = (s - h2 * fv[q]) / d;
         out[q] = om * gsv + ((T)1 - om) * uo[q];
     }

@@ -460,32 +460,45 @@ __device__ inline void rb_restrict_cell(const RbGeom &rf, const T *u, const T *f
     auto at = [&](int dx, int dy, int dz) {
         return (unsigned)((dx + dy + dz) & 1) * half + base + (unsigned)(dy * rs + dz * ps + (dx >> 1));
     };
+    // one child's residual, added to t in the children's order
+    auto child = [&](T &t, int dx, int dy, int dz) {
+        const int i = i0 + dx, j = j0 + dy, k = k0 + dz;
+        const unsigned e = at(dx, dy, dz);
+        const bool he = i < rf.nx - 1, hw = i > 2, hn = j < rf.ny - 1, hs = j > 2, ht = k < rf.nz - 1,
+                   hb = k > 2;
+        // every load issued, a missing neighbour reading the child
+        // itself; the value is selected afterwards
+        const T u0 = u[e];
+        const T uev = u[he ? at(dx + 1, dy, dz) : e], uwv = u[hw ? at(dx - 1, dy, dz) : e];
+        const T unv = u[hn ? at(dx, dy + 1, dz) : e], usv = u[hs ? at(dx, dy - 1, dz) : e];
+        const T utv = u[ht ? at(dx, dy, dz + 1) : e], ubv = u[hb ? at(dx, dy, dz - 1) : e];
+        const T awv = (T)cf.a(hw ? at(dx - 1, dy, dz) : e), bsv = (T)cf.b(hs ? at(dx, dy - 1, dz) : e),
+                cbv = (T)cf.c(hb ? at(dx, dy, dz - 1) : e);
+        const T ue = he ? uev : (T)0, uw = hw ? uwv : u0, un = hn ? unv : u0, us = hs ? usv : u0,
+                ut = ht ? utv : u0, ub = hb ? ubv : u0;
+        const T aw = hw ? awv : (T)0, bs = hs ? bsv : (T)0, cb = hb ? cbv : (T)0;
+        const T s = (T)cf.a(e) * (ue - u0) + aw * (uw - u0) + (T)cf.b(e) * (un - u0) + bs * (us - u0) +
+                    (T)cf.c(e) * (ut - u0) + cb * (ub - u0);
+        t += f[e] - h2i * s;
+    };
     T t = (T)0;
+    if constexpr (sizeof(T) == 4) {
 #pragma unroll
-    for (int dz = 0; dz < 2; dz++)
+        for (int dz = 0; dz < 2; dz++)
 #pragma unroll
-        for (int dy = 0; dy < 2; dy++)
+            for (int dy = 0; dy < 2; dy++)
 #pragma unroll
-            for (int dx = 0; dx < 2; dx++) {
-                const int i = i0 + dx, j = j0 + dy, k = k0 + dz;
-                const unsigned e = at(dx, dy, dz);
-                const bool he = i < rf.nx - 1, hw = i > 2, hn = j < rf.ny - 1, hs = j > 2, ht = k < rf.nz - 1,
-                           hb = k > 2;
-                // every load issued, a missing neighbour reading the child
-                // itself; the value is selected afterwards
-                const T u0 = u[e];
-                const T uev = u[he ? at(dx + 1, dy, dz) : e], uwv = u[hw ? at(dx - 1, dy, dz) : e];
-                const T unv = u[hn ? at(dx, dy + 1, dz) : e], usv = u[hs ? at(dx, dy - 1, dz) : e];
-                const T utv = u[ht ? at(dx, dy, dz + 1) : e], ubv = u[hb ? at(dx, dy, dz - 1) : e];
-                const T awv = (T)cf.a(hw ? at(dx - 1, dy, dz) : e), bsv = (T)cf.b(hs ? at(dx, dy - 1, dz) : e),
-                        cbv = (T)cf.c(hb ? at(dx, dy, dz - 1) : e);
-                const T ue = he ? uev : (T)0, uw = hw ? uwv : u0, un = hn ? unv : u0, us = hs ? usv : u0,
-                        ut = ht ? utv : u0, ub = hb ? ubv : u0;
-                const T aw = hw ? awv : (T)0, bs = hs ? bsv : (T)0, cb = hb ? cbv : (T)0;
-                const T s = (T)cf.a(e) * (ue - u0) + aw * (uw - u0) + (T)cf.b(e) * (un - u0) + bs * (us - u0) +
-                            (T)cf.c(e) * (ut - u0) + cb * (ub - u0);
-                t += f[e] - h2i * s;
-            }
+                for (int dx = 0; dx < 2; dx++) child(t, dx, dy, dz);
+    } else {
+        // FP64: four children's loads at a time (all eight in flight need
+        // over 200 registers and halve the occupancy)
+#pragma unroll 1
+        for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+            for (int dy = 0; dy < 2; dy++)
+#pragma unroll
+                for (int dx = 0; dx < 2; dx++) child(t, dx, dy, dz);
+    }
     const T v = (T)0.125 * t;
     if (fcs)
         fcs[rc.at(I + 2, J + 2, K + 2)] = v;
@@ -604,7 +617,8 @@ __global__ void __launch_bounds__(256) k_rb_colour4_pro(RbGeom r, T *__restrict_
 // natural-layout direct solve).  LDS form: the levels' u, f and face
 // coefficients and the coarsest level's f and u are staged in LDS (loaded
 // once per launch: the top level's f and every coefficient; the top level's
-// u written back at the end); the global form works in place.
+// u written back at the end), or only u and f when the coefficients do not
+// fit (FP64); the global form works in place.
 constexpr int TAIL_THREADS = 1024;
 constexpr int TAIL_MAX_LEVELS = 8;
 constexpr size_t TAIL_MAX_CELLS = 4096;  // largest level taken into the tail
@@ -624,6 +638,10 @@ struct RbTail {
     // LDS form: byte offsets of level l's u, f and coefficients (index l),
     // and of the coarsest level's f and u (index 0: f0, index top + 1: u0)
     unsigned lu[TAIL_MAX_LEVELS + 1], lf[TAIL_MAX_LEVELS + 1], lc[TAIL_MAX_LEVELS];
+    // 1: the LDS form keeps u and f in LDS but reads the coefficients where
+    // they lie (FP64 coefficients of a 16^3 level exceed the LDS); the
+    // kernel is instantiated for either case (its CG parameter)
+    int cglob;
 };
 
 // LDS layout of the staged tail: per level u, f (type T) and 3 coefficient
@@ -641,7 +659,7 @@ inline size_t tail_lds_layout(RbTail<CT, T> &t)
         t.lf[l] = (unsigned)b;
         b += al(n2 * sizeof(T));
         t.lc[l] = (unsigned)b;
-        b += al(3 * n2 * sizeof(CT));
+        if (!t.cglob) b += al(3 * n2 * sizeof(CT));
     }
     const size_t n0 = (size_t)t.g0.nx * t.g0.ny * t.g0.nz;
     t.lf[0] = (unsigned)b;
@@ -661,8 +679,9 @@ struct TailLev {
     T h2, h2i;
 };
 
-// where level l's arrays live for one form of the tail
-template <typename CT, typename T, bool LDS>
+// where level l's arrays live for one form of the tail (CG: coefficients
+// in HBM even in the LDS form)
+template <typename CT, typename T, bool LDS, bool CG = false>
 struct TailMem {
     const RbTail<CT, T> &t;
     unsigned char *lds;
@@ -670,7 +689,7 @@ struct TailMem {
     __device__ inline T *f(int l) const { return LDS ? (T *)(lds + t.lf[l]) : t.f[l]; }
     __device__ inline Coef<CT> cf(int l) const
     {
-        return LDS ? Coef<CT>{(const CT *)(lds + t.lc[l]), 2 * t.r[l].half} : t.cf[l];
+        return (LDS && !CG) ? Coef<CT>{(const CT *)(lds + t.lc[l]), 2 * t.r[l].half} : t.cf[l];
     }
     __device__ inline T *f0() const { return LDS ? (T *)(lds + t.lf[0]) : t.f0; }
     __device__ inline T *u0() const { return LDS ? (T *)(lds + t.lu[t.top + 1]) : t.u0; }
@@ -717,8 +736,8 @@ __device__ inline void tail_pass(const TailLev<CT, T> &L, const TailCells &cells
     __syncthreads();
 }
 
-template <typename CT, typename T, bool LDS>
-__device__ inline void tail_body(const TailMem<CT, T, LDS> &mem, double *y)
+template <typename CT, typename T, bool LDS, bool CG>
+__device__ inline void tail_body(const TailMem<CT, T, LDS, CG> &mem, double *y)
 {
     const RbTail<CT, T> &t = mem.t;
     const T om = t.om;
@@ -776,15 +795,15 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_rb_tail(RbTail<CT, T> t)
 {
     __shared__ double y[512];
     const TailMem<CT, T, false> mem{t, nullptr};
-    tail_body<CT, T, false>(mem, y);
+    tail_body<CT, T, false, false>(mem, y);
 }
 
-template <typename CT, typename T>
+template <typename CT, typename T, bool CG>
 __global__ void __launch_bounds__(TAIL_THREADS) k_rb_tail_lds(RbTail<CT, T> t)
 {
     extern __shared__ __align__(16) unsigned char tail_lds[];
     __shared__ double y[512];
-    const TailMem<CT, T, true> mem{t, tail_lds};
+    const TailMem<CT, T, true, CG> mem{t, tail_lds};
     // stage the top level's right-hand side and every level's coefficients
     {
         const int l = t.top;
@@ -792,13 +811,13 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_rb_tail_lds(RbTail<CT, T> t)
         T *fd = mem.f(l);
         for (int e = (int)threadIdx.x; e < n2; e += TAIL_THREADS) fd[e] = t.f[l][e];
     }
-    for (int l = 1; l <= t.top; l++) {
+    for (int l = 1; l <= t.top && !CG; l++) {
         const int n = (int)(6 * t.r[l].half);
         CT *cd = (CT *)mem.cf(l).p;
         for (int e = (int)threadIdx.x; e < n; e += TAIL_THREADS) cd[e] = t.cf[l].p[e];
     }
     __syncthreads();
-    tail_body<CT, T, true>(mem, y);
+    tail_body<CT, T, true, CG>(mem, y);
     {
         const int l = t.top;
         const int n2 = (int)(2 * t.r[l].half);
@@ -957,12 +976,24 @@ int launch_tail(Ctx &c, int lt)
         DG_HIP(hipGetDevice(&dev));
         DG_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     }
-    const size_t lds = tail_lds_layout(t);
+    size_t lds = tail_lds_layout(t);
+    if (lds + 512 * sizeof(double) > (size_t)lds_max) {
+        // u and f in LDS, coefficients read where they lie
+        t.cglob = 1;
+        lds = tail_lds_layout(t);
+    }
     if (c.rb_tail == 2 && lds + 512 * sizeof(double) <= (size_t)lds_max) {
-        if (lds > 48 * 1024)
-            DG_HIP(hipFuncSetAttribute((const void *)k_rb_tail_lds<CT, T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)lds));
-        hipLaunchKernelGGL((k_rb_tail_lds<CT, T>), dim3(1), dim3(TAIL_THREADS), lds, c.stream, t);
+        if (t.cglob) {
+            if (lds > 48 * 1024)
+                DG_HIP(hipFuncSetAttribute((const void *)k_rb_tail_lds<CT, T, true>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            hipLaunchKernelGGL((k_rb_tail_lds<CT, T, true>), dim3(1), dim3(TAIL_THREADS), lds, c.stream, t);
+        } else {
+            if (lds > 48 * 1024)
+                DG_HIP(hipFuncSetAttribute((const void *)k_rb_tail_lds<CT, T, false>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            hipLaunchKernelGGL((k_rb_tail_lds<CT, T, false>), dim3(1), dim3(TAIL_THREADS), lds, c.stream, t);
+        }
     } else {
         hipLaunchKernelGGL((k_rb_tail<CT, T>), dim3(1), dim3(TAIL_THREADS), 0, c.stream, t);
     }

@@ -56,7 +56,7 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0  # dense BF16 MFMA of one MI355X
 # them; keyed by kernel and grid extent, re-measured whenever the kernel
 # changes
 ROOF_KERNEL = "k_rb_colour4<_Float16,*,true>"
-PMC_TRAFFIC = {(ROOF_KERNEL, 258): (5 * 232.1e6 + 167.7e6 + 251.9e6 + 246.6e6) / 8}
+PMC_TRAFFIC = {(ROOF_KERNEL, 258): (5 * 230.0e6 + 167.7e6 + 249.4e6 + 245.1e6) / 8}
 
 
 def splash_inputdata(dg, n, **kw):
@@ -245,6 +245,9 @@ def main():
     ap.add_argument("--weights", default=os.path.join(HERE, "3d_droplet_simulation_2007_amd", "weights",
                                                        "guess_unet.npz"), help="learned-guess network weights")
     ap.add_argument("--no-output", action="store_true", help="skip the run with printout's files")
+    ap.add_argument("--only-path", choices=("fp64", "mixed"), default=None,
+                    help="run one path alone (zero guess) for a profile: the FP64 denominator or the mixed "
+                         "solver without the network; prints its ms/step and no JSON line")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (scale for HBM byte counters) and exit")
     args = ap.parse_args()
@@ -261,6 +264,9 @@ def main():
     dg.set_smoother(args.omega, args.nu_coarse)
     n = args.grid
     txt = splash_inputdata(dg, n)
+    if args.only_path:
+        run_path(dg, txt, 1 if args.only_path == "fp64" else 2, args.steps, args.warmup, args.tol, guess=0)
+        return
     gs = importlib.import_module("3d_droplet_simulation_2007_amd.guess")
     weights_meta = gs.load_weights(args.weights)
     import numpy as np

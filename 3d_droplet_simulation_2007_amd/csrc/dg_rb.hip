@@ -528,11 +528,13 @@ struct CoarseE {
     const T *ucs;
     Grid gc;    // natural order (ucn)
     const T *ucn;
-    // the correction of fine entry m of row (j, k), either colour
+    // the correction of fine entry m of row (j, k), either colour (one load
+    // from the selected array, no branch around it)
     __device__ inline T at(int m, int j, int k) const
     {
         const int ic = m + 2, jc = (j - 2) / 2 + 2, kc = (k - 2) / 2 + 2;
-        return ucs ? ucs[rc.at(ic, jc, kc)] : ucn[gc.at(ic, jc, kc)];
+        const size_t e = ucs ? (size_t)rc.at32(ic, jc, kc) : (size_t)gc.at(ic, jc, kc);
+        return (ucs ? ucs : ucn)[e];
     }
 };
 
@@ -560,21 +562,35 @@ __global__ void __launch_bounds__(256) k_rb_prolong(RbGeom rf, T *u, RbGeom rc, 
 // Other-colour neighbours with the coarse correction added as they are
 // read: the values k_rb_prolong would have left in HBM.  eo: the
 // corrections of entries m0 .. m0+3 of the group's own row, which a
-// neighbour row with the same coarse parent row shares.
+// neighbour row with the same coarse parent row shares; ey / ez: those of
+// the one y / z neighbour row with another parent (loaded up front, so a
+// row read selects among registers instead of loading behind a branch).
 template <typename T>
 struct NbProlong {
     NbGlobal<T> g;
     CoarseE<T> e;
     int m0, j, k;
     const T *eo;
+    T ey[4], ez[4];
+    __device__ inline NbProlong(const NbGlobal<T> &g_, const CoarseE<T> &e_, int m0_, int j_, int k_, const T *eo_,
+                                const RbGeom &r)
+        : g(g_), e(e_), m0(m0_), j(j_), k(k_), eo(eo_)
+    {
+        // the y / z neighbour row whose parent differs (clamped to the own
+        // row where it does not exist; its values are then never used)
+        int jo = ((j - 2) & 1) ? j + 1 : j - 1, ko = ((k - 2) & 1) ? k + 1 : k - 1;
+        jo = (jo < 2 || jo > r.ny - 1) ? j : jo;
+        ko = (ko < 2 || ko > r.nz - 1) ? k : ko;
+        for (int q = 0; q < 4; q++) {
+            ey[q] = e.at(m0 + q, jo, k);
+            ez[q] = e.at(m0 + q, j, ko);
+        }
+    }
     __device__ inline void row4(int dj, int dk, T v[4]) const
     {
         g.row4(dj, dk, v);
-        if (((j - 2 + dj) >> 1) == ((j - 2) >> 1) && ((k - 2 + dk) >> 1) == ((k - 2) >> 1)) {
-            for (int q = 0; q < 4; q++) v[q] = v[q] + eo[q];
-        } else {
-            for (int q = 0; q < 4; q++) v[q] = v[q] + e.at(m0 + q, j + dj, k + dk);
-        }
+        const bool same = ((j - 2 + dj) >> 1) == ((j - 2) >> 1) && ((k - 2 + dk) >> 1) == ((k - 2) >> 1);
+        for (int q = 0; q < 4; q++) v[q] = v[q] + (same ? eo[q] : (dj != 0 ? ey[q] : ez[q]));
     }
     __device__ inline T one(int dm) const { return g.one(dm) + e.at(m0 + dm, j, k); }
 };
@@ -601,7 +617,7 @@ __global__ void __launch_bounds__(256) k_rb_colour4_pro(RbGeom r, T *__restrict_
     T eo[4], out[4];
     for (int q = 0; q < 4; q++) eo[q] = ce.at(m0 + q, j, k);
     if (NBE) {
-        const NbProlong<T> nb{nbg, ce, m0, j, k, eo};
+        const NbProlong<T> nb(nbg, ce, m0, j, k, eo, r);
         rb_group4<CT, T, false>(r, nb, f, u, cf, own, oth, h2, colour, om, 0, m0, j, k, out, eo);
     } else {
         rb_group4<CT, T, false>(r, nbg, f, u, cf, own, oth, h2, colour, om, 0, m0, j, k, out, eo);

@@ -90,6 +90,7 @@ struct Ctx {
     int rb_tail = 2;         // split levels of <= 4096 cells: 0 per-level launches, 1 one-workgroup tail, 2 LDS tail
     int coef16_ok = 0;       // face coefficients fit FP16 (set from the fluids' densities)
     int top_coef_ready = 0;  // solvelpres's abcei wrote the finest level's split coefficients for this solve
+    int coarse_coef_ready = 0;  // acfromac wrote the coarse levels' split coefficients for this solve
     hipGraphExec_t pcg_graph[2] = {nullptr, nullptr};
     int pcg_graph_key[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     // diagnostics of the last step
@@ -133,8 +134,10 @@ int mg_resid(const Grid &g, hipStream_t s, double lengthsize, double *res, const
 int mg_rstrct(const Grid &gc, hipStream_t s, double *uc, const double *uf);
 int mg_interp(const Grid &gf, hipStream_t s, double *uf, const double *uc);
 int mg_addint(const Grid &gf, hipStream_t s, double *uf, const double *uc, double *res);
+// split (when given): also the red-black split copy of the coarse a, b, c
+// of type split_ct (RB_CT_*), as mg_abcei
 int mg_acfromac(const Grid &gc, hipStream_t s, double *a, double *b, double *c, double *ei, const double *a2,
-                const double *b2, const double *c2);
+                const double *b2, const double *c2, void *split = nullptr, int split_ct = -1);
 // split (when given): also the red-black split copy of a, b, c of type
 // split_ct (RB_CT_*), three planes of 2 * half entries
 int mg_abcei(const Grid &g, hipStream_t s, double *a, double *b, double *c, double *ei, const double *cc, double r1,

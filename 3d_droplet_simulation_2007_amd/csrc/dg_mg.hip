@@ -118,10 +118,12 @@ int mg_addint(const Grid &gf, hipStream_t s, double *uf, const double *uc, doubl
 // the lower ghost faces also write the ghost-face averages a(1,j,k),
 // b(i,1,k) and c(i,j,1) that the reference's loops include.
 int mg_acfromac(const Grid &gc, hipStream_t s, double *a, double *b, double *c, double *ei, const double *a2,
-                const double *b2, const double *c2)
+                const double *b2, const double *c2, void *split, int split_ct)
 {
     const Grid gf = make_grid(2 * gc.nx - 2, 2 * gc.ny - 2, 2 * gc.nz - 2);
     const int sy = gc.sy, sz = gc.sz;
+    const int mh = (gc.nx - 2) / 2, my = gc.ny - 2;
+    const size_t half = (size_t)mh * my * (gc.nz - 2), n2 = 2 * half;
     return launch_interior(gc, s, [=] __device__(int i, int j, int k) {
         auto A = [&](int ii) {
             return 0.25 * (a2[gf.at(2 * ii - 1, 2 * j - 1, 2 * k - 1)] + a2[gf.at(2 * ii - 1, 2 * j - 2, 2 * k - 1)] +
@@ -144,6 +146,25 @@ int mg_acfromac(const Grid &gc, hipStream_t s, double *a, double *b, double *c, 
         if (j == 2) b[o - sy] = bs;
         if (k == 2) c[o - sz] = cb;
         ei[o] = 1.0 / (av + aw + bv + bs + cv + cb);
+        if (split) {  // the split V-cycle's set-up conversion (k_rb_coef) of this level
+            const size_t e = (size_t)((i + j + k) & 1) * half + ((size_t)(k - 2) * my + (j - 2)) * mh + ((i - 2) >> 1);
+            if (split_ct == RB_CT_F16) {
+                _Float16 *p = (_Float16 *)split;
+                p[e] = (_Float16)av;
+                p[n2 + e] = (_Float16)bv;
+                p[2 * n2 + e] = (_Float16)cv;
+            } else if (split_ct == RB_CT_F64) {
+                double *p = (double *)split;
+                p[e] = av;
+                p[n2 + e] = bv;
+                p[2 * n2 + e] = cv;
+            } else {
+                float *p = (float *)split;
+                p[e] = (float)av;
+                p[n2 + e] = (float)bv;
+                p[2 * n2 + e] = (float)cv;
+            }
+        }
     });
 }
 
@@ -881,7 +902,7 @@ static int sum_partials(hipStream_t s, const double *partial, int nb, int stride
 // slots of the iteration number's parity (the host is at most one
 // iteration behind the GPU), then, after a system-scope fence, the
 // iteration count (the host waits for the count, then reads the pair)
-__global__ void k_pcg_scalars(double *ds, double *poll)
+__device__ inline void pcg_scalars(double *ds, double *poll)
 {
     if (ds[S_STOP] != 0.0) return;
     double bnorm = sqrt(ds[S_BB]);
@@ -900,6 +921,38 @@ __global__ void k_pcg_scalars(double *ds, double *poll)
     ds[S_FIRST] = 0.0;
     __threadfence_system();
     poll[1] = (double)(it + 1);
+}
+
+// the second stage of r.r (its chunk sums, in order) and the scalar
+// bookkeeping in one launch
+__global__ void __launch_bounds__(256) k_final_scalars(const double *chunks, int nchunk, double *ds, double *poll)
+{
+    __shared__ double lds4[4];
+    double a1 = 0.0;
+    for (int p = threadIdx.x; p < nchunk; p += 256) a1 += chunks[p];
+    const double s1 = block_reduce256<false>(a1, lds4);
+    if (threadIdx.x == 0) {
+        ds[S_RR] = s1;
+        pcg_scalars(ds, poll);
+    }
+}
+
+// a one-launch deterministic sum of nb partials by one workgroup of 1024
+// threads (thread t adds entries t, t + 1024, ...; the 16 wave sums are
+// added in order)
+__global__ void __launch_bounds__(1024) k_sum_wide(const double *partial, int nb, double *out)
+{
+    __shared__ double lds16[16];
+    double a1 = 0.0;
+    for (int p = threadIdx.x; p < nb; p += 1024) a1 += partial[p];
+    a1 = wave_sum(a1);
+    if ((threadIdx.x & 63) == 0) lds16[threadIdx.x >> 6] = a1;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = lds16[0];
+        for (int q = 1; q < 16; q++) r += lds16[q];
+        *out = r;
+    }
 }
 
 // x += alpha_j p_j for the first N buffers of the ring, in order.  N is a
@@ -1129,15 +1182,23 @@ static int pcg_iteration(Ctx &c)
         prof_end(PK_PCG_DIR, s, ((double)sizeof(T) + 2.0 * sizeof(PT) + 4.0 * 8.0) * ncell);
         DG_CHECK_LAUNCH();
     }
-    if (sum_partials(s, partial, nblk_dir, 1, partial + 2 * (size_t)nblk, ds + S_PQ)) return -1;
+    // p.q: k_pcg_dir's per-workgroup partials summed in one launch
+    hipLaunchKernelGGL(k_sum_wide, dim3(1), dim3(1024), 0, s, partial, nblk_dir, ds + S_PQ);
+    DG_CHECK_LAUNCH();
     prof_begin(PK_PCG_UPDATE, s);
     hipLaunchKernelGGL(k_pcg_update<PT>, ig, ib, 0, s, g, r, q, ds, partial, split ? rb_f_of<T>(F) : nullptr,
                        rb_geom(g));
     prof_end(PK_PCG_UPDATE, s, (3.0 * 8.0 + (split ? (double)sizeof(T) : 0.0)) * ncell);
     DG_CHECK_LAUNCH();
-    if (sum_partials(s, partial, nblk, 1, partial + 2 * (size_t)nblk, ds + S_RR)) return -1;
-    hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(1), 0, s, ds, c.dpoll);
-    DG_CHECK_LAUNCH();
+    // r.r: chunk sums, then their sum with the scalar bookkeeping
+    {
+        const int nchunk = (nblk + RP_CHUNK - 1) / RP_CHUNK;
+        double *chunks = partial + 2 * (size_t)nblk;
+        hipLaunchKernelGGL(k_partial_chunks, dim3((unsigned)nchunk), dim3(256), 0, s, partial, nblk, 1, chunks);
+        DG_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_final_scalars, dim3(1), dim3(256), 0, s, chunks, nchunk, ds, c.dpoll);
+        DG_CHECK_LAUNCH();
+    }
     return 0;
 }
 
@@ -1337,11 +1398,18 @@ int op_mg_solve_tol(Ctx &c)
     if (ensure_tol_buffers(c)) return -1;
     const Grid g = c.g;
     (void)g;  // the partial-sum buffer is sized for this grid by alloc_levels
+    // with split storage the coarse levels' split coefficients are written
+    // by acfromac itself (the arrays are allocated first)
+    const bool split = rb_enabled(c);
+    if (split && rb_alloc(c)) return -1;
     for (int gi = c.ng - 1; gi >= 1; gi--) {
         Level &C = c.lev[gi - 1], &F = c.lev[gi];
-        if (mg_acfromac(C.g, s, C.a, C.b, C.c, C.ei, F.a, F.b, F.c)) return -1;
+        const bool cs = split && gi - 1 >= 1;
+        if (mg_acfromac(C.g, s, C.a, C.b, C.c, C.ei, F.a, F.b, F.c, cs ? C.rb_coef : nullptr, cs ? C.rb_ct : -1))
+            return -1;
     }
-    if (rb_enabled(c)) {
+    if (split) {
+        c.coarse_coef_ready = 1;
         if (rb_prepare(c)) return -1;
     } else if (c.mg_mode == 2) {
         for (int l = 0; l < c.ng; l++) {

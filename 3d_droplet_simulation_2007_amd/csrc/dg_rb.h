@@ -87,8 +87,13 @@ inline double *rb_f_of<double>(Level &L) { return L.rb_f64; }
 bool rb_enabled(const Ctx &c);
 // the coefficient type (RB_CT_*) the split levels use for this context
 int rb_coef_type(const Ctx &c);
-// allocates the split arrays and builds the split coefficients of every
-// level above the coarsest for this solve (after the coarse operators exist)
+// allocates the split arrays (and the coefficient planes of the type this
+// solve uses) of every level above the coarsest
+int rb_alloc(Ctx &c);
+// rb_alloc, then the split coefficients of every level above the coarsest
+// for this solve (after the coarse operators exist), except those written
+// already (the finest by abcei, top_coef_ready; the coarse ones by
+// acfromac, coarse_coef_ready); clears both flags
 int rb_prepare(Ctx &c);
 // finest-level split right-hand side (the iterate's type) from r (FP64,
 // natural layout)

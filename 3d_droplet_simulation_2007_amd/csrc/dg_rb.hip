@@ -1215,7 +1215,7 @@ int rb_coef_type(const Ctx &c)
 
 static size_t coef_bytes(int ct) { return ct == RB_CT_F16 ? 2 : (ct == RB_CT_F32 ? 4 : 8); }
 
-int rb_prepare(Ctx &c)
+int rb_alloc(Ctx &c)
 {
     const int want = rb_coef_type(c);
     const bool f64 = c.mg_mode == 1;
@@ -1245,7 +1245,19 @@ int rb_prepare(Ctx &c)
             DG_HIP(hipMalloc(&L.rb_coef, 3 * n2 * coef_bytes(want)));
             L.rb_ct = want;
         }
+    }
+    return 0;
+}
+
+int rb_prepare(Ctx &c)
+{
+    if (rb_alloc(c)) return -1;
+    const int want = rb_coef_type(c);
+    for (int l = 1; l < c.ng; l++) {
+        Level &L = c.lev[l];
+        const RbGeom r = rb_geom(L.g);
         if (l == c.ng - 1 && c.top_coef_ready) continue;  // written by solvelpres's abcei
+        if (l < c.ng - 1 && c.coarse_coef_ready) continue;  // written by acfromac
         const dim3 bl = row_block(L.g.nx - 2), gr = row_grid(bl, L.g.nx - 2, L.g.ny - 2, L.g.nz - 2);
         if (want == RB_CT_F16)
             hipLaunchKernelGGL(k_rb_coef<_Float16>, gr, bl, 0, c.stream, L.g, r, L.a, L.b, L.c, (_Float16 *)L.rb_coef);
@@ -1256,6 +1268,7 @@ int rb_prepare(Ctx &c)
         DG_CHECK_LAUNCH();
     }
     c.top_coef_ready = 0;
+    c.coarse_coef_ready = 0;
     return 0;
 }
 

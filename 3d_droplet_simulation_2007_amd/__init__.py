@@ -7,4 +7,4 @@ The package directory name starts with a digit; import it with
 """
 from .api import (FIELD_IDS, DropletError, DropletGPU, abcei, acfromac, addint, al3d, bc_scalar, fl3d,  # noqa: F401
                   filterc27, format_real, interp, lib, make_inputdata, poisson_solve, profile_start, profile_stop,
-                  relax, resid, rstrct, set_lookahead, set_smoother, set_tuning, slvsml)
+                  relax, resid, rstrct, set_lookahead, set_smoother, set_tile_smoother, set_tuning, slvsml)

@@ -384,6 +384,15 @@ def set_smoother(omega=1.0, nu_coarse=0):
     _check(lib().dg_set_smoother(_d(omega), _i(nu_coarse)), "dg_set_smoother")
 
 
+def set_tile_smoother(on=0, nu=0):
+    """Tile smoother on the V-cycle levels between the finest and the tail
+    (one launch per level visit, restricted additive Schwarz with red-black
+    SOR inside each 16x8x8 tile): 1 (or True) on those of at most 2^18
+    cells, 2 on all of them, 0 (False): global red-black passes.  nu: the
+    sweeps before and after the coarse correction there (0: nu_coarse)."""
+    _check(lib().dg_set_tile_smoother(_i(int(on)), _i(int(nu))), "dg_set_tile_smoother")
+
+
 def set_tuning(fused=False, graphs=True, split=True, tail="lds"):
     """Tolerance-mode execution switches:
     fused -- one-launch two-colour smoother sweeps instead of two colour

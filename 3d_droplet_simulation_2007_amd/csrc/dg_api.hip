@@ -60,6 +60,7 @@ static bool g_ready = false;
 static int g_fused_default = 0, g_graphs_default = 1, g_split_default = 1, g_tail_default = 2;
 static double g_omega_default = 1.0;  // tolerance-mode smoother (dg_set_smoother)
 static int g_nuc_default = 0;
+static int g_tiles_default = 0, g_tile_nu_default = 0;  // tile smoother (dg_set_tile_smoother)
 static double g_lookahead_default = 1e3;  // dg_set_lookahead
 
 // ---- allocation -----------------------------------------------------------
@@ -181,7 +182,7 @@ static void free_ctx(Ctx &c)
         Level &L = c.lev[l];
         void *ps[] = {L.rhs, L.res, L.u, L.cc, L.a, L.b, L.c, L.ei, L.a32, L.b32, L.c32,
                       L.u32, L.f32, L.r32, L.u64, L.f64, L.r64, L.w32, L.w64, L.rb_u, L.rb_f, L.rb_w, L.rb_u64,
-                      L.rb_f64, L.rb_coef};
+                      L.rb_f64, L.rb_w64, L.rb_coef};
         for (void *p : ps)
             if (p) (void)hipFree(p);
         L = Level();
@@ -427,6 +428,8 @@ int resident_setup(int nx, int ny, int nz)
     c.use_graphs = g_graphs_default;
     c.rb_split = g_split_default;
     c.rb_tail = g_tail_default;
+    c.rb_tiles = g_tiles_default;
+    c.rb_tile_nu = g_tile_nu_default;
     c.pcg_lookahead = g_lookahead_default;
     c.mg_omega = g_omega_default;
     c.mg_nuc = g_nuc_default;
@@ -779,6 +782,21 @@ int dg_set_tuning(const int *fused, const int *graphs, const int *split, const i
     c.use_graphs = g_graphs_default;
     c.rb_split = g_split_default;
     c.rb_tail = g_tail_default;
+    c.rb_tiles = g_tiles_default;
+    c.rb_tile_nu = g_tile_nu_default;
+    return 0;
+}
+
+int dg_set_tile_smoother(const int *on, const int *nu)
+{
+    if (*on < 0 || *on > 2 || *nu < 0) {
+        set_error("dg_set_tile_smoother: on must be 0, 1 or 2 and nu >= 0");
+        return -1;
+    }
+    g_tiles_default = *on;
+    g_tile_nu_default = *nu;
+    ctx()->rb_tiles = g_tiles_default;
+    ctx()->rb_tile_nu = g_tile_nu_default;
     return 0;
 }
 
@@ -987,6 +1005,8 @@ int dg_poisson_solve(const int *nx, const int *ny, const int *nz, const double *
     lc.use_graphs = g_graphs_default;
     lc.rb_split = g_split_default;
     lc.rb_tail = g_tail_default;
+    lc.rb_tiles = g_tiles_default;
+    lc.rb_tile_nu = g_tile_nu_default;
     lc.pcg_lookahead = g_lookahead_default;
     lc.coef16_ok = 0;  // arbitrary coefficients: FP32 in the split V-cycle
     lc.mg_omega = g_omega_default;

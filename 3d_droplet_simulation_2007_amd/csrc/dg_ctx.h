@@ -24,8 +24,9 @@ struct Level {
     // u and f per colour (FP32 with the FP32 V-cycle, FP64 with the FP64
     // one), a/b/c per colour in FP16, FP32 or FP64
     float *rb_u = nullptr, *rb_f = nullptr;
-    float *rb_w = nullptr;  // second iterate buffer of the fused (out-of-place) sweeps
-    double *rb_u64 = nullptr, *rb_f64 = nullptr;
+    float *rb_w = nullptr;  // second iterate buffer of the fused (out-of-place) sweeps and of the tile
+                            // smoother's post-smoothing (whose result it holds)
+    double *rb_u64 = nullptr, *rb_f64 = nullptr, *rb_w64 = nullptr;
     void *rb_coef = nullptr;
     int rb_ct = -1;  // coefficient type rb_coef was allocated for: RB_CT_F32, RB_CT_F16, RB_CT_F64 (-1: none)
 };
@@ -88,11 +89,15 @@ struct Ctx {
     int ref_grid = 1;        // the grid has the reference's form nn = 2**ng + 2 (mglin's schedule may run)
     int rb_split = 1;        // red-black split storage of the V-cycle levels (modes 1 and 2)
     int rb_tail = 2;         // split levels of <= 4096 cells: 0 per-level launches, 1 one-workgroup tail, 2 LDS tail
+    int rb_tiles = 0;        // split levels between the finest and the tail smoothed as tiles (one launch per
+                             // level visit, restricted additive Schwarz with red-black SOR in each tile):
+                             // 0 none, 1 those of at most 2^18 cells, 2 all
+    int rb_tile_nu = 0;      // sweeps (pre and post) on the tile levels; 0: the levels' own count
     int coef16_ok = 0;       // face coefficients fit FP16 (set from the fluids' densities)
     int top_coef_ready = 0;  // solvelpres's abcei wrote the finest level's split coefficients for this solve
     int coarse_coef_ready = 0;  // acfromac wrote the coarse levels' split coefficients for this solve
     hipGraphExec_t pcg_graph[2] = {nullptr, nullptr};
-    int pcg_graph_key[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    int pcg_graph_key[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     // diagnostics of the last step
     int mg_iters_last = 0;
     double mg_relres_last = 0.0, cfl_last = 0.0, epsdiv_last = 0.0;

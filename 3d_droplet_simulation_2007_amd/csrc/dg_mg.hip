@@ -1213,11 +1213,12 @@ static int pcg_graphs(Ctx &c)
 {
     const int rb = rb_enabled(c);
     // everything that changes the captured kernel sequence or its arguments
-    const int key[12] = {c.mg_mode, c.mg_nu1, c.mg_nu2, c.fused_smoother, c.ng, (int)c.g.nx,
+    const int key[14] = {c.mg_mode, c.mg_nu1, c.mg_nu2, c.fused_smoother, c.ng, (int)c.g.nx,
                          rb, rb ? c.lev[c.ng - 1].rb_ct : -1, rb ? c.rb_tail : -1, c.mg_nuc,
-                         (int)llround(c.mg_omega * 1e9), (int)c.g.nz};
+                         (int)llround(c.mg_omega * 1e9), (int)c.g.nz, rb ? c.rb_tiles : -1,
+                         rb ? c.rb_tile_nu : -1};
     bool ok = c.pcg_graph[0] != nullptr;
-    for (int q = 0; q < 12 && ok; q++) ok = c.pcg_graph_key[q] == key[q];
+    for (int q = 0; q < 14 && ok; q++) ok = c.pcg_graph_key[q] == key[q];
     if (ok) return 0;
     for (int q = 0; q < 2; q++) {
         if (c.pcg_graph[q]) {
@@ -1238,7 +1239,7 @@ static int pcg_graphs(Ctx &c)
         DG_HIP(hipGraphInstantiate(&c.pcg_graph[0], graph, nullptr, nullptr, 0));
         DG_HIP(hipGraphDestroy(graph));
     }
-    for (int q = 0; q < 12; q++) c.pcg_graph_key[q] = key[q];
+    for (int q = 0; q < 14; q++) c.pcg_graph_key[q] = key[q];
     return 0;
 }
 

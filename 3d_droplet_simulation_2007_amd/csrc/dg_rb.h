@@ -82,6 +82,13 @@ inline float *rb_f_of<float>(Level &L) { return L.rb_f; }
 template <>
 inline double *rb_f_of<double>(Level &L) { return L.rb_f64; }
 
+template <typename T>
+inline T *rb_w_of(Level &L);
+template <>
+inline float *rb_w_of<float>(Level &L) { return L.rb_w; }
+template <>
+inline double *rb_w_of<double>(Level &L) { return L.rb_w64; }
+
 // whether the split path serves this context's tolerance-mode solves
 // (mode 2: FP32 iterate; mode 1: FP64 iterate with FP64 coefficients)
 bool rb_enabled(const Ctx &c);

@@ -625,6 +625,173 @@ __global__ void __launch_bounds__(256) k_rb_colour4_pro(RbGeom r, T *__restrict_
     st4(u + own, out);
 }
 
+// Tile smoother of the levels between the finest and the tail: all of a
+// level's pre- (or post-) smoothing sweeps in one launch.  Each workgroup
+// owns a tile of TS_X x TS_Y x TS_Z cells; it keeps the iterate on the tile
+// grown by two cells on every side (the box) in LDS, runs the red-black
+// sweeps on the tile grown by one cell, and writes back the tile's own
+// cells.  The grown cells' neighbours outside it keep their values from
+// before the sweeps (restricted additive Schwarz with an overlap of one
+// cell): neighbouring tiles do not wait for each other, so the sweeps need
+// no grid-wide synchronisation.  A different smoother from the global
+// passes (tiles see each other's values once per launch); the CPU
+// statement carries the same smoother.  Per cell the arithmetic is
+// rb_cell_value's, in the same order.
+//   POST = false: pre-smoothing from a zero iterate, colour 0 then 1; the
+//   iterate is not read, u is written;
+//   POST = true: post-smoothing, colour 1 then 0, the iterate read as
+//   u + e (the coarse correction of the cell's parent), so the
+//   prolongation is folded in as in k_rb_colour4_pro; out of place into
+//   uout (the level's second iterate buffer), since the boxes read cells
+//   that other tiles write.
+// Thread t < GT_PAIRS owns pair t of the grown tile: the cell of colour 0
+// and the cell of colour 1 at that place, whose face coefficients, diagonal
+// and h^2 f it loads once into registers; a sweep then reads only the
+// iterate's neighbours from LDS.
+constexpr int TS_X = 16, TS_Y = 8, TS_Z = 8;
+constexpr int TB_X = TS_X + 4, TB_Y = TS_Y + 4, TB_Z = TS_Z + 4;
+constexpr int TB_N = TB_X * TB_Y * TB_Z;
+constexpr int GT_PX = (TS_X + 2) / 2, GT_Y = TS_Y + 2, GT_Z = TS_Z + 2;  // grown tile: pairs per row, rows, planes
+constexpr int GT_PAIRS = GT_PX * GT_Y * GT_Z;
+constexpr int TS_THREADS = 1024;
+constexpr size_t TS_MIN_CELLS = 4096;  // levels of more cells than this (at most this many: the tail)
+constexpr size_t TS_MAX_CELLS_1 = (size_t)1 << 18;  // rb_tiles = 1: levels of at most this many cells
+static_assert(GT_PAIRS <= TS_THREADS && TS_X * TS_Y * TS_Z == TS_THREADS, "one pair and one tile cell per thread");
+
+template <typename CT, typename T>
+constexpr size_t tsmooth_lds_bytes()
+{
+    return (size_t)TB_N * sizeof(T);
+}
+
+// one cell of the grown tile as a thread smooths it
+template <typename T>
+struct TsCell {
+    int o;  // LDS offset
+    bool ok, he, hw, hn, hs, ht, hb;
+    T ae, aw, bn, bs, ct, cb, d, hf;  // faces (missing ones 0), diagonal, h^2 f
+};
+
+template <typename CT, typename T>
+__device__ inline TsCell<T> ts_cell(const RbGeom &r, const T *f, const Coef<CT> &cf, T h2, int i, int j, int k,
+                                    int bi, int bj, int bk, bool own)
+{
+    TsCell<T> c;
+    c.ok = own && i >= 2 && i <= r.nx - 1 && j >= 2 && j <= r.ny - 1 && k >= 2 && k <= r.nz - 1;
+    c.o = ((k - bk) * TB_Y + (j - bj)) * TB_X + (i - bi);
+    c.hw = i > 2;
+    c.he = i < r.nx - 1;
+    c.hn = j < r.ny - 1;
+    c.hs = j > 2;
+    c.ht = k < r.nz - 1;
+    c.hb = k > 2;
+    // every load issued (a cell or face outside the level reads the nearest
+    // cell inside it) and selected afterwards
+    const int ic = min(max(i, 2), r.nx - 1), jc = min(max(j, 2), r.ny - 1), kc = min(max(k, 2), r.nz - 1);
+    const unsigned e = r.at32(ic, jc, kc);
+    const unsigned ew = r.at32(max(ic - 1, 2), jc, kc), es = r.at32(ic, max(jc - 1, 2), kc),
+                   eb = r.at32(ic, jc, max(kc - 1, 2));
+    const T fv = f[e];
+    c.ae = (T)cf.a(e);
+    c.bn = (T)cf.b(e);
+    c.ct = (T)cf.c(e);
+    const T awv = (T)cf.a(ew), bsv = (T)cf.b(es), cbv = (T)cf.c(eb);
+    c.aw = c.hw ? awv : (T)0;
+    c.bs = c.hs ? bsv : (T)0;
+    c.cb = c.hb ? cbv : (T)0;
+    T d = c.ae;
+    d = c.hw ? d + c.aw : d;
+    d = c.hn ? d + c.bn : d;
+    d = c.hs ? d + c.bs : d;
+    d = c.ht ? d + c.ct : d;
+    d = c.hb ? d + c.cb : d;
+    c.d = d;
+    c.hf = h2 * fv;
+    return c;
+}
+
+template <typename T>
+__device__ inline void ts_update(T *su, const TsCell<T> &c, T om)
+{
+    if (!c.ok) return;
+    const int o = c.o;
+    T s = (T)0;
+    s = c.he ? s + c.ae * su[o + 1] : s;
+    s = c.hw ? s + c.aw * su[o - 1] : s;
+    s = c.hn ? s + c.bn * su[o + TB_X] : s;
+    s = c.hs ? s + c.bs * su[o - TB_X] : s;
+    s = c.ht ? s + c.ct * su[o + TB_X * TB_Y] : s;
+    s = c.hb ? s + c.cb * su[o - TB_X * TB_Y] : s;
+    const T gsv = (s - c.hf) / c.d;
+    const T uold = om == (T)1 ? (T)0 : su[o];
+    su[o] = om * gsv + ((T)1 - om) * uold;
+}
+
+template <typename CT, typename T, bool POST>
+__global__ void __launch_bounds__(TS_THREADS) k_rb_tsmooth(RbGeom r, const T *u, T *uout, const T *__restrict__ f,
+                                                           Coef<CT> cf, T h2, T om, int nsw, CoarseE<T> ce, int ntx,
+                                                           int nty)
+{
+    extern __shared__ __align__(16) unsigned char ts_lds[];
+    T *const su = reinterpret_cast<T *>(ts_lds);
+    // tiles in XCD-contiguous runs (the dispatcher deals blocks round-robin
+    // to the 8 XCDs), so the boxes' shared halo cells sit in one L2
+    const unsigned n = gridDim.x;
+    unsigned id = blockIdx.x;
+    const unsigned per = n / 8;
+    if (id < per * 8) id = (id % 8) * per + id / 8;
+    const int tx = (int)(id % (unsigned)ntx), ty = (int)((id / (unsigned)ntx) % (unsigned)nty),
+              tz = (int)(id / ((unsigned)ntx * (unsigned)nty));
+    const int i0 = 2 + tx * TS_X, j0 = 2 + ty * TS_Y, k0 = 2 + tz * TS_Z;
+    const int bi = i0 - 2, bj = j0 - 2, bk = k0 - 2;  // the box's first cell
+    const int tid = (int)threadIdx.x;
+    // the box: zero (pre-smoothing) or u + e (post-smoothing); every load
+    // issued before the first store to LDS, a place outside the level
+    // loading the nearest cell inside it and storing 0
+    constexpr int NIT = (TB_N + TS_THREADS - 1) / TS_THREADS;
+    if (POST) {
+        T ru[NIT], re[NIT];
+#pragma unroll
+        for (int it = 0; it < NIT; it++) {
+            const int q = min(tid + it * TS_THREADS, TB_N - 1);
+            const int i = min(max(bi + q % TB_X, 2), r.nx - 1), j = min(max(bj + (q / TB_X) % TB_Y, 2), r.ny - 1),
+                      k = min(max(bk + q / (TB_X * TB_Y), 2), r.nz - 1);
+            ru[it] = u[r.at32(i, j, k)];
+            re[it] = ce.at((i - 2) >> 1, j, k);
+        }
+#pragma unroll
+        for (int it = 0; it < NIT; it++) {
+            const int q = tid + it * TS_THREADS;
+            if (q < TB_N) {
+                const int i = bi + q % TB_X, j = bj + (q / TB_X) % TB_Y, k = bk + q / (TB_X * TB_Y);
+                const bool in = i >= 2 && i <= r.nx - 1 && j >= 2 && j <= r.ny - 1 && k >= 2 && k <= r.nz - 1;
+                su[q] = in ? ru[it] + re[it] : (T)0;
+            }
+        }
+    } else {
+        for (int q = tid; q < TB_N; q += TS_THREADS) su[q] = (T)0;
+    }
+    // this thread's pair of the grown tile, both colours
+    const bool own = tid < GT_PAIRS;
+    const int pq = tid % GT_PX, j = j0 - 1 + (tid / GT_PX) % GT_Y, k = k0 - 1 + tid / (GT_PX * GT_Y);
+    const int gi0 = i0 - 1;
+    const int p0 = (gi0 + j + k) & 1;  // colour 0's cell is the pair's first when (i + j + k) is even there
+    const TsCell<T> c0 = ts_cell<CT, T>(r, f, cf, h2, gi0 + 2 * pq + p0, j, k, bi, bj, bk, own);
+    const TsCell<T> c1 = ts_cell<CT, T>(r, f, cf, h2, gi0 + 2 * pq + (1 - p0), j, k, bi, bj, bk, own);
+    __syncthreads();
+    for (int sw = 0; sw < nsw; sw++) {
+        ts_update(su, POST ? c1 : c0, om);
+        __syncthreads();
+        ts_update(su, POST ? c0 : c1, om);
+        __syncthreads();
+    }
+    {
+        const int i = i0 + tid % TS_X, jj = j0 + (tid / TS_X) % TS_Y, kk = k0 + tid / (TS_X * TS_Y);
+        if (i <= r.nx - 1 && jj <= r.ny - 1 && kk <= r.nz - 1)
+            uout[r.at32(i, jj, kk)] = su[((kk - bk) * TB_Y + (jj - bj)) * TB_X + (i - bi)];
+    }
+}
+
 // The V-cycle below a small level in one workgroup: every colour pass,
 // restriction, the coarsest solve and every prolongation of the levels
 // 1..top as loops of TAIL_THREADS threads separated by barriers, with the
@@ -638,6 +805,7 @@ __global__ void __launch_bounds__(256) k_rb_colour4_pro(RbGeom r, T *__restrict_
 constexpr int TAIL_THREADS = 1024;
 constexpr int TAIL_MAX_LEVELS = 8;
 constexpr size_t TAIL_MAX_CELLS = 4096;  // largest level taken into the tail
+static_assert(TS_MIN_CELLS == TAIL_MAX_CELLS, "the tile smoother serves the levels above the tail");
 
 template <typename CT, typename T>
 struct RbTail {
@@ -1097,6 +1265,44 @@ int sweep_fused_t(Ctx &c, int l, const T *X, T *Y, int ca, bool zero, int kind)
     }
 }
 
+// whether level l's sweeps run as the tile smoother: on request, below the
+// finest level, above the tail's size (rb_tiles 1: and at most 2^18 cells,
+// where the global passes are launch-bound), and when the box fits the LDS
+template <typename CT, typename T>
+bool tile_level(const Ctx &c, int l)
+{
+    if (!c.rb_tiles || l >= c.ng - 1 || l < 1) return false;
+    const size_t cells = 2 * rb_geom(c.lev[l].g).half;
+    if (cells <= TS_MIN_CELLS || (c.rb_tiles == 1 && cells > TS_MAX_CELLS_1)) return false;
+    static int lds_max = -1;
+    if (lds_max < 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+            lds_max = 0;
+    }
+    return tsmooth_lds_bytes<CT, T>() <= (size_t)lds_max;
+}
+
+// all nsw pre- (POST false) or post-smoothing sweeps of level l in one launch
+template <typename CT, typename T, bool POST>
+int tile_smooth(Ctx &c, int l, int nsw, const CoarseE<T> &ce)
+{
+    Level &L = c.lev[l];
+    const RbGeom r = rb_geom(L.g);
+    const Coef<CT> cf{(const CT *)L.rb_coef, 2 * r.half};
+    const int ntx = (2 * r.mh + TS_X - 1) / TS_X, nty = (r.my + TS_Y - 1) / TS_Y, ntz = (r.mz + TS_Z - 1) / TS_Z;
+    const size_t lds = tsmooth_lds_bytes<CT, T>();
+    if (lds > 48 * 1024)
+        DG_HIP(hipFuncSetAttribute((const void *)k_rb_tsmooth<CT, T, POST>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds));
+    hipLaunchKernelGGL((k_rb_tsmooth<CT, T, POST>), dim3((unsigned)(ntx * nty * ntz)), dim3(TS_THREADS), lds,
+                       c.stream, r, rb_u_of<T>(L), POST ? rb_w_of<T>(L) : rb_u_of<T>(L), rb_f_of<T>(L), cf,
+                       (T)(1.0 / lev_h2i(c, L.g)), (T)c.mg_omega, nsw, ce, ntx, nty);
+    DG_CHECK_LAUNCH();
+    return 0;
+}
+
 // the coarsest level's solve in the iterate's type
 template <typename T>
 int coarse_solve_t(Ctx &c);
@@ -1116,12 +1322,16 @@ int vcycle_rb(Ctx &c, int l)
     const int nu2 = (top || c.mg_nuc <= 0) ? c.mg_nu2 : c.mg_nuc;
     // fused sweeps alternate between rb_u and rb_w; the first one writes the
     // buffer that makes the last one land in rb_u
-    const int fk = fused_kind(c, l);
+    const bool tiles = tile_level<CT, T>(c, l);
+    const int fk = tiles ? 0 : fused_kind(c, l);
     const bool fused = fk != 0;
     T *cur = lu;
     T *const lw = (T *)(void *)L.rb_w;  // FP32 only (fused_kind)
     auto other = [&](T *b) { return b == lu ? lw : lu; };
-    if (fused) {
+    const int nut1 = c.rb_tile_nu > 0 ? c.rb_tile_nu : nu1, nut2 = c.rb_tile_nu > 0 ? c.rb_tile_nu : nu2;
+    if (tiles) {
+        if (tile_smooth<CT, T, false>(c, l, nut1, CoarseE<T>{})) return -1;
+    } else if (fused) {
         T *w1 = ((nu1 + nu2) & 1) ? lu : lw;
         for (int q = 0; q < nu1; q++) {
             T *y = q == 0 ? w1 : other(cur);
@@ -1146,7 +1356,10 @@ int vcycle_rb(Ctx &c, int l)
     const RbGeom rc = rb_geom(C.g);
     T *cf0 = nullptr, *cu0 = nullptr;
     coarsest_arrays<T>(C, cf0, cu0);
-    T *const cfs = csplit ? rb_f_of<T>(C) : nullptr, *const cus = csplit ? rb_u_of<T>(C) : nullptr;
+    // the coarse level's result: rb_u, or rb_w where the tile smoother's
+    // post-smoothing wrote it
+    T *const cfs = csplit ? rb_f_of<T>(C) : nullptr;
+    T *const cus = csplit ? (tile_level<CT, T>(c, l - 1) ? rb_w_of<T>(C) : rb_u_of<T>(C)) : nullptr;
     const dim3 bc = row_block(C.g.nx - 2);
     hipLaunchKernelGGL((k_rb_resid_restrict<CT, T>), row_grid(bc, C.g.nx - 2, C.g.ny - 2, C.g.nz - 2), bc, 0, c.stream,
                        rf, cur, lf, cf, (T)lev_h2i(c, L.g), rc, cfs, C.g, csplit ? nullptr : cf0);
@@ -1158,6 +1371,7 @@ int vcycle_rb(Ctx &c, int l)
     }
     const CoarseE<T> ce{rc, cus, C.g, csplit ? nullptr : cu0};
     // the colour passes take the prolongation into their first sweep
+    if (tiles) return tile_smooth<CT, T, true>(c, l, nut2, ce);
     const bool pro_folded = !fused && nu2 >= 1 && rf.mh % 4 == 0;
     if (!pro_folded) {
         const dim3 bf = row_block(rf.mh);
@@ -1235,9 +1449,13 @@ int rb_alloc(Ctx &c)
             DG_HIP(hipMemset(L.rb_u, 0, n2 * sizeof(float)));
             DG_HIP(hipMemset(L.rb_f, 0, n2 * sizeof(float)));
         }
-        if (!f64 && c.fused_smoother != 0 && !L.rb_w) {
+        if (!f64 && (c.fused_smoother != 0 || (c.rb_tiles && l < c.ng - 1)) && !L.rb_w) {
             DG_HIP(hipMalloc(&L.rb_w, n2 * sizeof(float)));
             DG_HIP(hipMemset(L.rb_w, 0, n2 * sizeof(float)));
+        }
+        if (f64 && c.rb_tiles && l < c.ng - 1 && !L.rb_w64) {
+            DG_HIP(hipMalloc(&L.rb_w64, n2 * sizeof(double)));
+            DG_HIP(hipMemset(L.rb_w64, 0, n2 * sizeof(double)));
         }
         if (L.rb_ct != want) {
             if (L.rb_coef) (void)hipFree(L.rb_coef);

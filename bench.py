@@ -28,6 +28,10 @@ Measured paths, each timed over K steps after W warm-up steps:
                single-workgroup tail, FP64 coefficients, with the faster of
                the zero and previous-step guesses;
   fp64 natural the FP64 V-cycle on the natural layout (reported beside).
+Every path uses the same smoother: V(2,2), SOR weight 1.15, 4 sweeps below
+the finest level, and (--tiles 1, the default) the levels of 4096 < cells
+<= 2^18 (64^3 and 32^3 here) smoothed as 16x8x8 tiles, one launch per level
+visit (the natural-layout FP64 path has no tiles).
 The target and the denominator are also timed over a later window of the
 same run (steps 150..169 by default), where the splash has developed.
 Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
@@ -57,6 +61,12 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0  # dense BF16 MFMA of one MI355X
 # changes
 ROOF_KERNEL = "k_rb_colour4<_Float16,*,true>"
 PMC_TRAFFIC = {(ROOF_KERNEL, 258): (5 * 230.0e6 + 167.7e6 + 246.0e6 + 245.1e6) / 8}
+
+
+TILE_LABEL = {0: "", 1: "; levels of 4096 < cells <= 2^18 as 16x8x8 tiles (restricted additive Schwarz, "
+                        "one launch per level visit)",
+              2: "; every level of more than 4096 cells below the finest as 16x8x8 tiles (restricted "
+                 "additive Schwarz, one launch per level visit)"}
 
 
 def splash_inputdata(dg, n, **kw):
@@ -158,7 +168,8 @@ def run_path(dg, txt, mode, steps, warmup, tol, guess=1, profile_kernel=None, pr
         late = dict(first_step=t, steps=late_steps, ms_per_step=1e3 * (lt1 - lt0) / late_steps,
                     mg_iters_per_step=float(np.mean(its)))
     g.close()
-    print(f"bench: mode {mode} guess {guess}: {1e3 * (t1 - t0) / steps:.2f} ms/step over {steps} steps"
+    print(f"bench: mode {mode} guess {guess}: {1e3 * (t1 - t0) / steps:.2f} ms/step over {steps} steps, "
+          f"{float(np.mean([q[0] for q in timed])):.2f} PCG iterations/step"
           + (f", later window {late['ms_per_step']:.2f} ms/step" if late else ""), file=sys.stderr, flush=True)
     return dict(seconds=t1 - t0, steps=steps, iters=timed_iters, relres=st["mg_relres"], tau=st["tau"],
                 vol0=vol0, vol1=vol1, prof=prof, p=p, guess_stats=guess_stats, late=late, output=output)
@@ -240,6 +251,11 @@ def main():
     ap.add_argument("--no-denominator", action="store_true")
     ap.add_argument("--omega", type=float, default=1.15, help="SOR weight of the V-cycle smoother (all paths)")
     ap.add_argument("--nu-coarse", type=int, default=4, help="smoothing sweeps below the finest level (all paths)")
+    ap.add_argument("--tiles", type=int, choices=(0, 1, 2), default=1,
+                    help="tile smoother on the V-cycle levels between the finest and the tail (all paths): "
+                         "1 on those of at most 2^18 cells, 2 on all")
+    ap.add_argument("--tile-sweeps", type=int, default=0,
+                    help="sweeps before and after the coarse correction on the tile levels (0: --nu-coarse)")
     ap.add_argument("--late-start", type=int, default=150, help="first step of the later timed window (0: none)")
     ap.add_argument("--late-steps", type=int, default=20, help="steps in the later timed window")
     ap.add_argument("--weights", default=os.path.join(HERE, "3d_droplet_simulation_2007_amd", "weights",
@@ -262,6 +278,7 @@ def main():
     dg.lib()
     # one smoother for every path (target, mixed and the FP64 denominator)
     dg.set_smoother(args.omega, args.nu_coarse)
+    dg.set_tile_smoother(args.tiles, args.tile_sweeps)
     n = args.grid
     txt = splash_inputdata(dg, n)
     if args.only_path:
@@ -320,7 +337,8 @@ def main():
         "config": {"workload": f"{n - 2}^3 droplet-on-wall splash, 6 mm cube, r=1 mm, v=-3 m/s, CSF, "
                                f"each step's Poisson solve to {args.tol:g} relative residual",
                    "grid_with_ghosts": [n, n, n], "solver": f"FP64 PCG + FP32 V(2,2) preconditioner, red-black SOR omega {args.omega:g}, "
-                                                         f"{args.nu_coarse} sweeps below the finest level",
+                                                         f"{args.nu_coarse} sweeps below the finest level"
+                                                         + TILE_LABEL[args.tiles],
                    "initial_guess": "learned pressure prediction (encoder-decoder on the next-coarser level) "
                                     "over a zero start, kept only when it lowers the residual, fallback zero, "
                                     "with the fallback controller",
@@ -344,7 +362,8 @@ def main():
         line["denominator_fp64"] = {"value": dv, "ms_per_step": 1e3 * den["seconds"] / den["steps"],
                                     "mg_iters_per_step": float(np.mean(den["iters"])), "guess": dguess,
                                     "solver": "FP64 PCG + FP64 V(2,2) on red-black split storage, FP64 "
-                                              "coefficients, the same launch shapes and single-workgroup tail",
+                                              "coefficients, the same launch shapes, tile smoother and "
+                                              "single-workgroup tail",
                                     "other_guess_ms_per_step": {k: 1e3 * v["seconds"] / v["steps"]
                                                                 for k, v in dens.items() if k != dguess},
                                     "late_window": den_late}

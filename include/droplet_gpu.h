@@ -160,6 +160,21 @@ int dg_set_vof_guard(const int *on);
  * and to later dg_init / dg_poisson_solve calls. */
 int dg_set_tuning(const int *fused, const int *graphs, const int *split, const int *tail);
 
+/* Tile smoother of the tolerance-mode V-cycle (no reference counterpart;
+ * split storage only): on = 2 runs each level strictly between the finest
+ * level and the tail (levels of more than 4096 cells), on = 1 those of them
+ * with at most 2^18 cells, as tiles of 16x8x8
+ * cells, all of a visit's pre- or post-smoothing sweeps in one launch: each
+ * tile smooths itself grown by one cell in LDS with the cells around that
+ * held at their values from before the sweeps, and writes back its own cells
+ * (restricted additive Schwarz, red-black SOR inside; the prolongation is
+ * folded into the post-smoothing launch).  nu: the sweeps before and after
+ * the coarse correction on those levels (0: the levels' own count, the
+ * smoother's nu_coarse).  A different preconditioner from the global
+ * passes (on = 0, the default), the same tolerance reached.  Applies
+ * to the resident state and to later dg_init / dg_poisson_solve calls. */
+int dg_set_tile_smoother(const int *on, const int *nu);
+
 /* Look-ahead of the tolerance-mode PCG loop: the host queues iteration n+1
  * before iteration n's residual reaches it while the last residual it has
  * seen exceeds ratio * tol (default 1e3; <= 0 never), so the GPU does not

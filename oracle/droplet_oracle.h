@@ -100,6 +100,19 @@ typedef struct or_state {
      * and eddyvisc's mixing length lmix(NMX) (main.f:1828-1885) */
     double *u2p, *v2p, *w2p, *uvp, *uwp, *vwp;
     double *lmix;
+    /* tolerance-mode smoother on the levels strictly between the finest and
+     * the coarsest: mg_blk[0..2] > 0 smooths each tile of that many cells
+     * (x, y, z) on its own: the tile grown by mg_blk_ov cells is smoothed
+     * with the cells around it held at their values from before the sweeps,
+     * and the tile's own cells are kept (restricted additive Schwarz;
+     * ov = 0: block Jacobi), red-black SOR inside; only on levels of more
+     * than mg_blk_min and (mg_blk_max > 0) at most mg_blk_max interior
+     * cells.  0 (default): global red-black passes. */
+    int mg_blk[3];
+    int mg_blk_min;
+    int mg_blk_ov;
+    int mg_blk_max;
+    int mg_blk_nu;  /* sweeps (pre and post) on the tile levels; 0: the levels' own count */
 } or_state;
 
 /* field ids shared with the GPU library's dg_field ids */
@@ -160,6 +173,11 @@ void or_mglin(or_state *st);                                                    
 /* ---- tolerance-mode pressure solve (no reference counterpart): solves the
  * finest-level system held in lev[ng-1] (rhs, a, b, c, ei) into lev[ng-1].u */
 int  or_mg_solve_tol(or_state *st);
+/* the tile smoother of the levels between the finest and the coarsest (0, 0, 0: global passes) */
+void or_set_block_smoother(or_state *st, int bx, int by, int bz, int min_cells);
+void or_set_block_overlap(or_state *st, int ov);
+void or_set_block_max(or_state *st, int max_cells);
+void or_set_block_sweeps(or_state *st, int nu);
 double or_resid_norm(or_state *st, double *bnorm_out);
 
 #ifdef __cplusplus

@@ -79,6 +79,61 @@ static double tw1(int i, int c, int nc, int dirichlet_hi)
                     u[IDX(i, j, k)] = om * g + ((T)1 - om) * u[IDX(i, j, k)];                                     \
                 }                                                                                                 \
     }                                                                                                             \
+    /* nsw red-black sweeps, colour c0 first, on each tile of blk[0..2] interior cells on its own: the   \
+     * tile grown by ov cells on every side (within the domain) is smoothed in a private copy whose        \
+     * neighbours outside it hold the iterate before the sweeps, and only the tile's own cells are         \
+     * written back (restricted additive Schwarz; ov = 0 is block Jacobi).  The per-cell arithmetic is     \
+     * gs_colour's.                                                                                         \
+     */                                                                                                   \
+    static void blk_smooth_##SFX(T *u, T *uo, const T *rhs, const T *a, const T *b, const T *c, int nx, int ny, \
+                                 int nz, T h2, T om, int nsw, int c0, const int *blk, int ov)                     \
+    {                                                                                                         \
+        memcpy(uo, u, sizeof(T) * (size_t)nx * ny * nz);                                                      \
+        int wx = blk[0] + 2 * ov + 2, wy = blk[1] + 2 * ov + 2, wz = blk[2] + 2 * ov + 2;                     \
+        T *w = (T *)malloc(sizeof(T) * (size_t)wx * wy * wz);                                                 \
+        for (int k0 = 2; k0 <= nz - 1; k0 += blk[2])                                                          \
+            for (int j0 = 2; j0 <= ny - 1; j0 += blk[1])                                                      \
+                for (int i0 = 2; i0 <= nx - 1; i0 += blk[0]) {                                                \
+                    int i1 = i0 + blk[0] - 1 < nx - 1 ? i0 + blk[0] - 1 : nx - 1;                             \
+                    int j1 = j0 + blk[1] - 1 < ny - 1 ? j0 + blk[1] - 1 : ny - 1;                             \
+                    int k1 = k0 + blk[2] - 1 < nz - 1 ? k0 + blk[2] - 1 : nz - 1;                             \
+                    int I0 = i0 - ov > 2 ? i0 - ov : 2, I1 = i1 + ov < nx - 1 ? i1 + ov : nx - 1;             \
+                    int J0 = j0 - ov > 2 ? j0 - ov : 2, J1 = j1 + ov < ny - 1 ? j1 + ov : ny - 1;             \
+                    int K0 = k0 - ov > 2 ? k0 - ov : 2, K1 = k1 + ov < nz - 1 ? k1 + ov : nz - 1;             \
+                    /* w(i,j,k) for i in I0-1..I1+1 etc. */                                                   \
+                    for (int k = K0 - 1; k <= K1 + 1; k++)                                                    \
+                        for (int j = J0 - 1; j <= J1 + 1; j++)                                                \
+                            for (int i = I0 - 1; i <= I1 + 1; i++)                                            \
+                                w[(i - I0 + 1) + (size_t)wx * ((j - J0 + 1) + (size_t)wy * (k - K0 + 1))] =   \
+                                    uo[IDX(i, j, k)];                                                         \
+                    for (int sw = 0; sw < 2 * nsw; sw++) {                                                    \
+                        int col = (sw & 1) ? 1 - c0 : c0;                                                     \
+                        for (int k = K0; k <= K1; k++)                                                        \
+                            for (int j = J0; j <= J1; j++)                                                    \
+                                for (int i = I0 + ((col + I0 + j + k) & 1); i <= I1; i += 2) {                 \
+                                    size_t q = (i - I0 + 1) + (size_t)wx * ((j - J0 + 1) + (size_t)wy * (k - K0 + 1)); \
+                                    size_t sy = (size_t)wx, sz = (size_t)wx * wy;                             \
+                                    T aw = a[IDX(i - 1, j, k)], ae = a[IDX(i, j, k)];                         \
+                                    T bs = b[IDX(i, j - 1, k)], bn = b[IDX(i, j, k)];                         \
+                                    T cb = c[IDX(i, j, k - 1)], ct = c[IDX(i, j, k)];                         \
+                                    T s = (T)0, d = ae;                                                       \
+                                    if (i < nx - 1) s += ae * w[q + 1];                                       \
+                                    if (i > 2) { s += aw * w[q - 1]; d += aw; }                               \
+                                    if (j < ny - 1) { s += bn * w[q + sy]; d += bn; }                         \
+                                    if (j > 2) { s += bs * w[q - sy]; d += bs; }                              \
+                                    if (k < nz - 1) { s += ct * w[q + sz]; d += ct; }                         \
+                                    if (k > 2) { s += cb * w[q - sz]; d += cb; }                              \
+                                    T g = (s - h2 * rhs[IDX(i, j, k)]) / d;                                   \
+                                    w[q] = om * g + ((T)1 - om) * w[q];                                       \
+                                }                                                                             \
+                    }                                                                                         \
+                    for (int k = k0; k <= k1; k++)                                                            \
+                        for (int j = j0; j <= j1; j++)                                                        \
+                            for (int i = i0; i <= i1; i++)                                                    \
+                                u[IDX(i, j, k)] = w[(i - I0 + 1) + (size_t)wx * ((j - J0 + 1) + (size_t)wy * (k - K0 + 1))]; \
+                }                                                                                             \
+        free(w);                                                                                              \
+    }                                                                                                         \
     static void resid_##SFX(T *res, const T *u, const T *rhs, const T *a, const T *b, const T *c, int nx, int ny,  \
                             int nz, T h2i)                                                                        \
     {                                                                                                             \
@@ -215,6 +270,16 @@ typedef struct tol_ws {
     float *af[OR_MAXLEV], *bf[OR_MAXLEV], *cf[OR_MAXLEV];
 } tol_ws;
 
+/* the tile smoother serves a level below the finest (and above the
+ * coarsest) of more than mg_blk_min and (mg_blk_max > 0) at most mg_blk_max
+ * interior cells */
+static int use_blk(const or_state *st, int nx, int ny, int nz)
+{
+    const double cells = (double)(nx - 2) * (ny - 2) * (nz - 2);
+    return st->mg_blk[0] > 0 && st->mg_blk[1] > 0 && st->mg_blk[2] > 0 && cells > (double)st->mg_blk_min &&
+           (st->mg_blk_max <= 0 || cells <= (double)st->mg_blk_max);
+}
+
 static double lev_h2i(or_state *st, int nx) { double h = st->lengthsize / (double)(nx - 2); return 1.0 / (h * h); }
 
 /* symmetric V-cycle on levels l..0 in FP64: approximately solves A e = f */
@@ -231,7 +296,10 @@ static void vcycle_d(or_state *st, tol_ws *W, int l)
     const int nu1 = (top || st->mg_nuc <= 0) ? st->mg_nu1 : st->mg_nuc;
     const int nu2 = (top || st->mg_nuc <= 0) ? st->mg_nu2 : st->mg_nuc;
     const double om = st->mg_omega > 0.0 ? st->mg_omega : 1.0;
-    for (int s = 0; s < nu1; s++) {
+    const int blk = !top && use_blk(st, nx, ny, nz);
+    const int nub = st->mg_blk_nu > 0 ? st->mg_blk_nu : nu1;
+    if (blk) blk_smooth_d(W->u[l], W->r[l], W->f[l], L->a, L->b, L->c, nx, ny, nz, h2, om, nub, 0, st->mg_blk, st->mg_blk_ov);
+    for (int s = 0; s < nu1 && !blk; s++) {
         gs_colour_d(W->u[l], W->f[l], L->a, L->b, L->c, nx, ny, nz, h2, om, 0);
         gs_colour_d(W->u[l], W->f[l], L->a, L->b, L->c, nx, ny, nz, h2, om, 1);
     }
@@ -240,7 +308,10 @@ static void vcycle_d(or_state *st, tol_ws *W, int l)
     memset(W->u[l - 1], 0, sizeof(double) * (size_t)W->nx[l - 1] * W->ny[l - 1] * W->nz[l - 1]);
     vcycle_d(st, W, l - 1);
     prolong_add_d(W->u[l], W->u[l - 1], nx, ny, nz, st->mg_prolong);
-    for (int s = 0; s < nu2; s++) {
+    if (blk)
+        blk_smooth_d(W->u[l], W->r[l], W->f[l], L->a, L->b, L->c, nx, ny, nz, h2, om, st->mg_blk_nu > 0 ? st->mg_blk_nu : nu2,
+                     1, st->mg_blk, st->mg_blk_ov);
+    for (int s = 0; s < nu2 && !blk; s++) {
         gs_colour_d(W->u[l], W->f[l], L->a, L->b, L->c, nx, ny, nz, h2, om, 1);
         gs_colour_d(W->u[l], W->f[l], L->a, L->b, L->c, nx, ny, nz, h2, om, 0);
     }
@@ -263,7 +334,11 @@ static void vcycle_f(or_state *st, tol_ws *W, int l)
     const int nu1 = (top || st->mg_nuc <= 0) ? st->mg_nu1 : st->mg_nuc;
     const int nu2 = (top || st->mg_nuc <= 0) ? st->mg_nu2 : st->mg_nuc;
     const float om = st->mg_omega > 0.0 ? (float)st->mg_omega : 1.0f;
-    for (int s = 0; s < nu1; s++) {
+    const int blk = !top && use_blk(st, nx, ny, nz);
+    if (blk)
+        blk_smooth_f(W->uf[l], W->rf[l], W->ff[l], W->af[l], W->bf[l], W->cf[l], nx, ny, nz, h2f, om,
+                     st->mg_blk_nu > 0 ? st->mg_blk_nu : nu1, 0, st->mg_blk, st->mg_blk_ov);
+    for (int s = 0; s < nu1 && !blk; s++) {
         gs_colour_f(W->uf[l], W->ff[l], W->af[l], W->bf[l], W->cf[l], nx, ny, nz, h2f, om, 0);
         gs_colour_f(W->uf[l], W->ff[l], W->af[l], W->bf[l], W->cf[l], nx, ny, nz, h2f, om, 1);
     }
@@ -272,7 +347,10 @@ static void vcycle_f(or_state *st, tol_ws *W, int l)
     memset(W->uf[l - 1], 0, sizeof(float) * (size_t)W->nx[l - 1] * W->ny[l - 1] * W->nz[l - 1]);
     vcycle_f(st, W, l - 1);
     prolong_add_f(W->uf[l], W->uf[l - 1], nx, ny, nz, st->mg_prolong);
-    for (int s = 0; s < nu2; s++) {
+    if (blk)
+        blk_smooth_f(W->uf[l], W->rf[l], W->ff[l], W->af[l], W->bf[l], W->cf[l], nx, ny, nz, h2f, om,
+                     st->mg_blk_nu > 0 ? st->mg_blk_nu : nu2, 1, st->mg_blk, st->mg_blk_ov);
+    for (int s = 0; s < nu2 && !blk; s++) {
         gs_colour_f(W->uf[l], W->ff[l], W->af[l], W->bf[l], W->cf[l], nx, ny, nz, h2f, om, 1);
         gs_colour_f(W->uf[l], W->ff[l], W->af[l], W->bf[l], W->cf[l], nx, ny, nz, h2f, om, 0);
     }
@@ -413,3 +491,17 @@ int or_mg_solve_tol(or_state *st)
     }
     return it;
 }
+
+void or_set_block_smoother(or_state *st, int bx, int by, int bz, int min_cells)
+{
+    st->mg_blk[0] = bx;
+    st->mg_blk[1] = by;
+    st->mg_blk[2] = bz;
+    st->mg_blk_min = min_cells;
+}
+
+void or_set_block_overlap(or_state *st, int ov) { st->mg_blk_ov = ov; }
+
+void or_set_block_max(or_state *st, int max_cells) { st->mg_blk_max = max_cells; }
+
+void or_set_block_sweeps(or_state *st, int nu) { st->mg_blk_nu = nu; }

@@ -55,6 +55,10 @@ def lib():
         L.or_fl3d.restype = ctypes.c_double
         L.or_fl3d.argtypes = [ctypes.c_double] * 6
         L.or_mg_solve_tol.argtypes = [ctypes.c_void_p]
+        L.or_set_block_smoother.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 4
+        L.or_set_block_overlap.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.or_set_block_max.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.or_set_block_sweeps.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.or_mglin.argtypes = [ctypes.c_void_p]
         L.or_resid_norm.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
         L.or_resid_norm.restype = ctypes.c_double

@@ -114,3 +114,53 @@ def test_reference_relax_converges_to_direct_solution(oracle):
         OK.relax(u, 1.0, rhs, a, b, c, ei, 1.0)
     got = u[1:-1, 1:-1, 1:-1].ravel()
     assert np.linalg.norm(got - x) / np.linalg.norm(x) < 1e-6
+
+
+def _tol_solve(oracle, o, blk=(0, 0, 0), ov=0, minc=0, mode=1, tol=1e-11, nuc=2):
+    o.st.mg_mode, o.st.mg_outer, o.st.mg_prolong = mode, 1, 0
+    o.st.mg_tol, o.st.mg_nu1, o.st.mg_nu2, o.st.mg_maxcycles, o.st.mg_guess = tol, 2, 2, 100, 0
+    o.st.mg_omega, o.st.mg_nuc = 1.15, nuc
+    oracle.lib().or_set_block_smoother(o.h, *blk, minc)
+    oracle.lib().or_set_block_overlap(o.h, ov)
+    it = oracle.lib().or_mg_solve_tol(o.h)
+    return o.field("p").copy(), it
+
+
+def test_tile_smoother_with_one_tile_is_the_global_smoother(oracle):
+    """A tile that covers the whole level (with or without overlap) sees no
+    neighbour outside itself, so the tile smoother is the global red-black
+    smoother: the same p to the bit, the same iteration count."""
+    o = droplet_system(n=34, seed=6)
+    p0, it0 = _tol_solve(oracle, o)
+    for ov in (0, 1, 3):
+        p1, it1 = _tol_solve(oracle, o, blk=(64, 64, 64), ov=ov)
+        assert it1 == it0 and np.array_equal(p1, p0), ov
+
+
+def test_tile_smoother_with_full_overlap_is_the_global_smoother(oracle):
+    """With an overlap at least the level's extent every tile smooths the
+    whole level and keeps its own cells: again the global smoother, so the
+    write-back of each tile's own cells and the grown region are pinned."""
+    o = droplet_system(n=34, seed=8)
+    p0, it0 = _tol_solve(oracle, o)
+    p1, it1 = _tol_solve(oracle, o, blk=(8, 4, 4), ov=32)
+    assert it1 == it0 and np.array_equal(p1, p0)
+    # a small overlap is a different (Schwarz) smoother
+    p2, it2 = _tol_solve(oracle, o, blk=(8, 4, 4), ov=1)
+    assert not np.array_equal(p2, p0)
+
+
+def test_tile_smoother_pcg_matches_direct_solve(oracle):
+    """The PCG with the tile smoother (overlap 0 and 1, FP64 and FP32
+    V-cycles, tiles clipped at the level's ends) reaches the direct solution."""
+    o = droplet_system(n=34, seed=9)
+    nx = o.nx
+    h2i = 1.0 / (1.0 / (nx - 2)) ** 2
+    a, b, c, rhs = (o.field(k).copy() for k in ("a", "b", "c", "rhs"))
+    x = spla.spsolve(assemble(a, b, c, h2i).tocsc(), rhs[1:-1, 1:-1, 1:-1].ravel()).reshape(nx - 2, nx - 2, nx - 2)
+    for mode in (1, 2):
+        for ov in (0, 1):
+            p, it = _tol_solve(oracle, o, blk=(12, 6, 6), ov=ov, minc=64, mode=mode)
+            err = np.linalg.norm(p[1:-1, 1:-1, 1:-1] - x) / np.linalg.norm(x)
+            assert it < 40 and o.st.mg_relres_last < 1e-11, (mode, ov, it)
+            assert err < 1e-7, (mode, ov, err)

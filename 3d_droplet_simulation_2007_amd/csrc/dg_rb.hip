@@ -627,7 +627,7 @@ __global__ void __launch_bounds__(256) k_rb_colour4_pro(RbGeom r, T *__restrict_
 
 // Tile smoother of the levels between the finest and the tail: all of a
 // level's pre- (or post-) smoothing sweeps in one launch.  Each workgroup
-// owns a tile of TS_X x TS_Y x TS_Z cells; it keeps the iterate on the tile
+// owns a tile of cells (TsShape); it keeps the iterate on the tile
 // grown by two cells on every side (the box) in LDS, runs the red-black
 // sweeps on the tile grown by one cell, and writes back the tile's own
 // cells.  The grown cells' neighbours outside it keep their values from
@@ -636,7 +636,8 @@ __global__ void __launch_bounds__(256) k_rb_colour4_pro(RbGeom r, T *__restrict_
 // no grid-wide synchronisation.  A different smoother from the global
 // passes (tiles see each other's values once per launch); the CPU
 // statement carries the same smoother.  Per cell the arithmetic is
-// rb_cell_value's, in the same order.
+// rb_cell_value's, in the same order, except that the Gauss-Seidel value
+// multiplies by the reciprocal of the diagonal, formed once per launch.
 //   POST = false: pre-smoothing from a zero iterate, colour 0 then 1; the
 //   iterate is not read, u is written;
 //   POST = true: post-smoothing, colour 1 then 0, the iterate read as
@@ -644,24 +645,31 @@ __global__ void __launch_bounds__(256) k_rb_colour4_pro(RbGeom r, T *__restrict_
 //   prolongation is folded in as in k_rb_colour4_pro; out of place into
 //   uout (the level's second iterate buffer), since the boxes read cells
 //   that other tiles write.
-// Thread t < GT_PAIRS owns pair t of the grown tile: the cell of colour 0
-// and the cell of colour 1 at that place, whose face coefficients, diagonal
-// and h^2 f it loads once into registers; a sweep then reads only the
-// iterate's neighbours from LDS.
-constexpr int TS_X = 16, TS_Y = 8, TS_Z = 8;
-constexpr int TB_X = TS_X + 4, TB_Y = TS_Y + 4, TB_Z = TS_Z + 4;
-constexpr int TB_N = TB_X * TB_Y * TB_Z;
-constexpr int GT_PX = (TS_X + 2) / 2, GT_Y = TS_Y + 2, GT_Z = TS_Z + 2;  // grown tile: pairs per row, rows, planes
-constexpr int GT_PAIRS = GT_PX * GT_Y * GT_Z;
+// Thread t owns pairs t, t + 1024, ... of the grown tile (a pair: the cell
+// of colour 0 and the cell of colour 1 at one place), whose face
+// coefficients, diagonal and h^2 f it loads once into registers; a sweep
+// then reads only the iterate's neighbours from LDS.
+//
+// The tile shapes: 16x8x8 cells on the levels of at most 2^18 cells,
+// 32x16x8 above (fewer, larger tiles: one round of workgroups at 128^3, a
+// smaller share of recomputed halo, and a smoother closer to the global one).
+template <int X, int Y, int Z>
+struct TsShape {
+    static constexpr int TX = X, TY = Y, TZ = Z;
+    static constexpr int BX = X + 4, BY = Y + 4, BZ = Z + 4, BN = BX * BY * BZ;  // box
+    static constexpr int PX = (X + 2) / 2, GY = Y + 2, GZ = Z + 2, PAIRS = PX * GY * GZ;  // grown tile
+};
+using TsSmall = TsShape<16, 8, 8>;
+using TsLarge = TsShape<32, 16, 8>;
 constexpr int TS_THREADS = 1024;
 constexpr size_t TS_MIN_CELLS = 4096;  // levels of more cells than this (at most this many: the tail)
 constexpr size_t TS_MAX_CELLS_1 = (size_t)1 << 18;  // rb_tiles = 1: levels of at most this many cells
-static_assert(GT_PAIRS <= TS_THREADS && TS_X * TS_Y * TS_Z == TS_THREADS, "one pair and one tile cell per thread");
+constexpr size_t TS_LARGE_CELLS = (size_t)1 << 18;  // levels of more cells than this take TsLarge
 
-template <typename CT, typename T>
+template <class S, typename T>
 constexpr size_t tsmooth_lds_bytes()
 {
-    return (size_t)TB_N * sizeof(T);
+    return (size_t)S::BN * sizeof(T);
 }
 
 // one cell of the grown tile as a thread smooths it
@@ -669,16 +677,16 @@ template <typename T>
 struct TsCell {
     int o;  // LDS offset
     bool ok, he, hw, hn, hs, ht, hb;
-    T ae, aw, bn, bs, ct, cb, d, hf;  // faces (missing ones 0), diagonal, h^2 f
+    T ae, aw, bn, bs, ct, cb, rd, hf;  // faces (missing ones 0), 1 / diagonal, h^2 f
 };
 
-template <typename CT, typename T>
+template <class S, typename CT, typename T>
 __device__ inline TsCell<T> ts_cell(const RbGeom &r, const T *f, const Coef<CT> &cf, T h2, int i, int j, int k,
                                     int bi, int bj, int bk, bool own)
 {
     TsCell<T> c;
     c.ok = own && i >= 2 && i <= r.nx - 1 && j >= 2 && j <= r.ny - 1 && k >= 2 && k <= r.nz - 1;
-    c.o = ((k - bk) * TB_Y + (j - bj)) * TB_X + (i - bi);
+    c.o = ((k - bk) * S::BY + (j - bj)) * S::BX + (i - bi);
     c.hw = i > 2;
     c.he = i < r.nx - 1;
     c.hn = j < r.ny - 1;
@@ -705,12 +713,12 @@ __device__ inline TsCell<T> ts_cell(const RbGeom &r, const T *f, const Coef<CT> 
     d = c.hs ? d + c.bs : d;
     d = c.ht ? d + c.ct : d;
     d = c.hb ? d + c.cb : d;
-    c.d = d;
+    c.rd = (T)1 / d;  // a multiplication per sweep instead of a division
     c.hf = h2 * fv;
     return c;
 }
 
-template <typename T>
+template <class S, typename T>
 __device__ inline void ts_update(T *su, const TsCell<T> &c, T om)
 {
     if (!c.ok) return;
@@ -718,16 +726,16 @@ __device__ inline void ts_update(T *su, const TsCell<T> &c, T om)
     T s = (T)0;
     s = c.he ? s + c.ae * su[o + 1] : s;
     s = c.hw ? s + c.aw * su[o - 1] : s;
-    s = c.hn ? s + c.bn * su[o + TB_X] : s;
-    s = c.hs ? s + c.bs * su[o - TB_X] : s;
-    s = c.ht ? s + c.ct * su[o + TB_X * TB_Y] : s;
-    s = c.hb ? s + c.cb * su[o - TB_X * TB_Y] : s;
-    const T gsv = (s - c.hf) / c.d;
+    s = c.hn ? s + c.bn * su[o + S::BX] : s;
+    s = c.hs ? s + c.bs * su[o - S::BX] : s;
+    s = c.ht ? s + c.ct * su[o + S::BX * S::BY] : s;
+    s = c.hb ? s + c.cb * su[o - S::BX * S::BY] : s;
+    const T gsv = (s - c.hf) * c.rd;
     const T uold = om == (T)1 ? (T)0 : su[o];
     su[o] = om * gsv + ((T)1 - om) * uold;
 }
 
-template <typename CT, typename T, bool POST>
+template <class S, typename CT, typename T, bool POST>
 __global__ void __launch_bounds__(TS_THREADS) k_rb_tsmooth(RbGeom r, const T *u, T *uout, const T *__restrict__ f,
                                                            Coef<CT> cf, T h2, T om, int nsw, CoarseE<T> ce, int ntx,
                                                            int nty)
@@ -742,53 +750,66 @@ __global__ void __launch_bounds__(TS_THREADS) k_rb_tsmooth(RbGeom r, const T *u,
     if (id < per * 8) id = (id % 8) * per + id / 8;
     const int tx = (int)(id % (unsigned)ntx), ty = (int)((id / (unsigned)ntx) % (unsigned)nty),
               tz = (int)(id / ((unsigned)ntx * (unsigned)nty));
-    const int i0 = 2 + tx * TS_X, j0 = 2 + ty * TS_Y, k0 = 2 + tz * TS_Z;
+    const int i0 = 2 + tx * S::TX, j0 = 2 + ty * S::TY, k0 = 2 + tz * S::TZ;
     const int bi = i0 - 2, bj = j0 - 2, bk = k0 - 2;  // the box's first cell
     const int tid = (int)threadIdx.x;
     // the box: zero (pre-smoothing) or u + e (post-smoothing); every load
     // issued before the first store to LDS, a place outside the level
     // loading the nearest cell inside it and storing 0
-    constexpr int NIT = (TB_N + TS_THREADS - 1) / TS_THREADS;
+    constexpr int NIT = (S::BN + TS_THREADS - 1) / TS_THREADS;
     if (POST) {
         T ru[NIT], re[NIT];
 #pragma unroll
         for (int it = 0; it < NIT; it++) {
-            const int q = min(tid + it * TS_THREADS, TB_N - 1);
-            const int i = min(max(bi + q % TB_X, 2), r.nx - 1), j = min(max(bj + (q / TB_X) % TB_Y, 2), r.ny - 1),
-                      k = min(max(bk + q / (TB_X * TB_Y), 2), r.nz - 1);
+            const int q = min(tid + it * TS_THREADS, S::BN - 1);
+            const int i = min(max(bi + q % S::BX, 2), r.nx - 1), j = min(max(bj + (q / S::BX) % S::BY, 2), r.ny - 1),
+                      k = min(max(bk + q / (S::BX * S::BY), 2), r.nz - 1);
             ru[it] = u[r.at32(i, j, k)];
             re[it] = ce.at((i - 2) >> 1, j, k);
         }
 #pragma unroll
         for (int it = 0; it < NIT; it++) {
             const int q = tid + it * TS_THREADS;
-            if (q < TB_N) {
-                const int i = bi + q % TB_X, j = bj + (q / TB_X) % TB_Y, k = bk + q / (TB_X * TB_Y);
+            if (q < S::BN) {
+                const int i = bi + q % S::BX, j = bj + (q / S::BX) % S::BY, k = bk + q / (S::BX * S::BY);
                 const bool in = i >= 2 && i <= r.nx - 1 && j >= 2 && j <= r.ny - 1 && k >= 2 && k <= r.nz - 1;
                 su[q] = in ? ru[it] + re[it] : (T)0;
             }
         }
     } else {
-        for (int q = tid; q < TB_N; q += TS_THREADS) su[q] = (T)0;
+        for (int q = tid; q < S::BN; q += TS_THREADS) su[q] = (T)0;
     }
-    // this thread's pair of the grown tile, both colours
-    const bool own = tid < GT_PAIRS;
-    const int pq = tid % GT_PX, j = j0 - 1 + (tid / GT_PX) % GT_Y, k = k0 - 1 + tid / (GT_PX * GT_Y);
+    // this thread's pairs of the grown tile (pair tid + p TS_THREADS), both
+    // colours
+    constexpr int PP = (S::PAIRS + TS_THREADS - 1) / TS_THREADS;
+    TsCell<T> c0[PP], c1[PP];
     const int gi0 = i0 - 1;
-    const int p0 = (gi0 + j + k) & 1;  // colour 0's cell is the pair's first when (i + j + k) is even there
-    const TsCell<T> c0 = ts_cell<CT, T>(r, f, cf, h2, gi0 + 2 * pq + p0, j, k, bi, bj, bk, own);
-    const TsCell<T> c1 = ts_cell<CT, T>(r, f, cf, h2, gi0 + 2 * pq + (1 - p0), j, k, bi, bj, bk, own);
+#pragma unroll
+    for (int pp = 0; pp < PP; pp++) {
+        const int e = tid + pp * TS_THREADS;
+        const bool own = e < S::PAIRS;
+        const int pq = e % S::PX, j = j0 - 1 + (e / S::PX) % S::GY, k = k0 - 1 + e / (S::PX * S::GY);
+        const int p0 = (gi0 + j + k) & 1;  // colour 0's cell is the pair's first when (i + j + k) is even there
+        c0[pp] = ts_cell<S, CT, T>(r, f, cf, h2, gi0 + 2 * pq + p0, j, k, bi, bj, bk, own);
+        c1[pp] = ts_cell<S, CT, T>(r, f, cf, h2, gi0 + 2 * pq + (1 - p0), j, k, bi, bj, bk, own);
+    }
     __syncthreads();
     for (int sw = 0; sw < nsw; sw++) {
-        ts_update(su, POST ? c1 : c0, om);
+#pragma unroll
+        for (int pp = 0; pp < PP; pp++) ts_update<S>(su, POST ? c1[pp] : c0[pp], om);
         __syncthreads();
-        ts_update(su, POST ? c0 : c1, om);
+#pragma unroll
+        for (int pp = 0; pp < PP; pp++) ts_update<S>(su, POST ? c0[pp] : c1[pp], om);
         __syncthreads();
     }
-    {
-        const int i = i0 + tid % TS_X, jj = j0 + (tid / TS_X) % TS_Y, kk = k0 + tid / (TS_X * TS_Y);
+    constexpr int CPT = S::TX * S::TY * S::TZ / TS_THREADS;
+    static_assert(CPT * TS_THREADS == S::TX * S::TY * S::TZ, "whole tile cells per thread");
+#pragma unroll
+    for (int q0 = 0; q0 < CPT; q0++) {
+        const int q = tid + q0 * TS_THREADS;
+        const int i = i0 + q % S::TX, jj = j0 + (q / S::TX) % S::TY, kk = k0 + q / (S::TX * S::TY);
         if (i <= r.nx - 1 && jj <= r.ny - 1 && kk <= r.nz - 1)
-            uout[r.at32(i, jj, kk)] = su[((kk - bk) * TB_Y + (jj - bj)) * TB_X + (i - bi)];
+            uout[r.at32(i, jj, kk)] = su[((kk - bk) * S::BY + (jj - bj)) * S::BX + (i - bi)];
     }
 }
 
@@ -1267,7 +1288,8 @@ int sweep_fused_t(Ctx &c, int l, const T *X, T *Y, int ca, bool zero, int kind)
 
 // whether level l's sweeps run as the tile smoother: on request, below the
 // finest level, above the tail's size (rb_tiles 1: and at most 2^18 cells,
-// where the global passes are launch-bound), and when the box fits the LDS
+// where the global passes are launch-bound), and when its tile's box fits
+// the LDS
 template <typename CT, typename T>
 bool tile_level(const Ctx &c, int l)
 {
@@ -1281,26 +1303,34 @@ bool tile_level(const Ctx &c, int l)
             hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
             lds_max = 0;
     }
-    return tsmooth_lds_bytes<CT, T>() <= (size_t)lds_max;
+    const size_t lds = cells > TS_LARGE_CELLS ? tsmooth_lds_bytes<TsLarge, T>() : tsmooth_lds_bytes<TsSmall, T>();
+    return lds <= (size_t)lds_max;
+}
+
+template <class S, typename CT, typename T, bool POST>
+int tile_smooth_s(Ctx &c, int l, int nsw, const CoarseE<T> &ce)
+{
+    Level &L = c.lev[l];
+    const RbGeom r = rb_geom(L.g);
+    const Coef<CT> cf{(const CT *)L.rb_coef, 2 * r.half};
+    const int ntx = (2 * r.mh + S::TX - 1) / S::TX, nty = (r.my + S::TY - 1) / S::TY, ntz = (r.mz + S::TZ - 1) / S::TZ;
+    const size_t lds = tsmooth_lds_bytes<S, T>();
+    if (lds > 48 * 1024)
+        DG_HIP(hipFuncSetAttribute((const void *)k_rb_tsmooth<S, CT, T, POST>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((k_rb_tsmooth<S, CT, T, POST>), dim3((unsigned)(ntx * nty * ntz)), dim3(TS_THREADS), lds,
+                       c.stream, r, rb_u_of<T>(L), POST ? rb_w_of<T>(L) : rb_u_of<T>(L), rb_f_of<T>(L), cf,
+                       (T)(1.0 / lev_h2i(c, L.g)), (T)c.mg_omega, nsw, ce, ntx, nty);
+    DG_CHECK_LAUNCH();
+    return 0;
 }
 
 // all nsw pre- (POST false) or post-smoothing sweeps of level l in one launch
 template <typename CT, typename T, bool POST>
 int tile_smooth(Ctx &c, int l, int nsw, const CoarseE<T> &ce)
 {
-    Level &L = c.lev[l];
-    const RbGeom r = rb_geom(L.g);
-    const Coef<CT> cf{(const CT *)L.rb_coef, 2 * r.half};
-    const int ntx = (2 * r.mh + TS_X - 1) / TS_X, nty = (r.my + TS_Y - 1) / TS_Y, ntz = (r.mz + TS_Z - 1) / TS_Z;
-    const size_t lds = tsmooth_lds_bytes<CT, T>();
-    if (lds > 48 * 1024)
-        DG_HIP(hipFuncSetAttribute((const void *)k_rb_tsmooth<CT, T, POST>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)lds));
-    hipLaunchKernelGGL((k_rb_tsmooth<CT, T, POST>), dim3((unsigned)(ntx * nty * ntz)), dim3(TS_THREADS), lds,
-                       c.stream, r, rb_u_of<T>(L), POST ? rb_w_of<T>(L) : rb_u_of<T>(L), rb_f_of<T>(L), cf,
-                       (T)(1.0 / lev_h2i(c, L.g)), (T)c.mg_omega, nsw, ce, ntx, nty);
-    DG_CHECK_LAUNCH();
-    return 0;
+    if (2 * rb_geom(c.lev[l].g).half > TS_LARGE_CELLS) return tile_smooth_s<TsLarge, CT, T, POST>(c, l, nsw, ce);
+    return tile_smooth_s<TsSmall, CT, T, POST>(c, l, nsw, ce);
 }
 
 // the coarsest level's solve in the iterate's type

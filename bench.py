@@ -28,10 +28,12 @@ Measured paths, each timed over K steps after W warm-up steps:
                single-workgroup tail, FP64 coefficients, with the faster of
                the zero and previous-step guesses;
   fp64 natural the FP64 V-cycle on the natural layout (reported beside).
-Every path uses the same smoother: V(2,2), SOR weight 1.15, 4 sweeps below
-the finest level, and (--tiles 1, the default) the levels of 4096 < cells
-<= 2^18 (64^3 and 32^3 here) smoothed as 16x8x8 tiles, one launch per level
-visit (the natural-layout FP64 path has no tiles).
+Every path uses the same smoother: V(2,2), SOR weight 1.15, 4 sweeps on the
+levels in the single-workgroup tail, and (--tiles 2, the default) the levels
+between the finest and the tail (128^3, 64^3 and 32^3 here) smoothed as
+tiles, 5 sweeps before and after the coarse correction in one launch each
+(the natural-layout FP64 path has no tiles and 4 sweeps on every level
+below the finest).
 The target and the denominator are also timed over a later window of the
 same run (steps 150..169 by default), where the splash has developed.
 Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
@@ -65,8 +67,8 @@ PMC_TRAFFIC = {(ROOF_KERNEL, 258): (5 * 230.0e6 + 167.7e6 + 246.0e6 + 245.1e6) /
 
 TILE_LABEL = {0: "", 1: "; levels of 4096 < cells <= 2^18 as 16x8x8 tiles (restricted additive Schwarz, "
                         "one launch per level visit)",
-              2: "; every level of more than 4096 cells below the finest as 16x8x8 tiles (restricted "
-                 "additive Schwarz, one launch per level visit)"}
+              2: "; every level of more than 4096 cells below the finest as tiles of 16x8x8 cells "
+                 "(32x16x8 above 2^18 cells; restricted additive Schwarz, one launch per level visit)"}
 
 
 def splash_inputdata(dg, n, **kw):
@@ -251,10 +253,10 @@ def main():
     ap.add_argument("--no-denominator", action="store_true")
     ap.add_argument("--omega", type=float, default=1.15, help="SOR weight of the V-cycle smoother (all paths)")
     ap.add_argument("--nu-coarse", type=int, default=4, help="smoothing sweeps below the finest level (all paths)")
-    ap.add_argument("--tiles", type=int, choices=(0, 1, 2), default=1,
+    ap.add_argument("--tiles", type=int, choices=(0, 1, 2), default=2,
                     help="tile smoother on the V-cycle levels between the finest and the tail (all paths): "
                          "1 on those of at most 2^18 cells, 2 on all")
-    ap.add_argument("--tile-sweeps", type=int, default=0,
+    ap.add_argument("--tile-sweeps", type=int, default=5,
                     help="sweeps before and after the coarse correction on the tile levels (0: --nu-coarse)")
     ap.add_argument("--late-start", type=int, default=150, help="first step of the later timed window (0: none)")
     ap.add_argument("--late-steps", type=int, default=20, help="steps in the later timed window")
@@ -338,7 +340,9 @@ def main():
                                f"each step's Poisson solve to {args.tol:g} relative residual",
                    "grid_with_ghosts": [n, n, n], "solver": f"FP64 PCG + FP32 V(2,2) preconditioner, red-black SOR omega {args.omega:g}, "
                                                          f"{args.nu_coarse} sweeps below the finest level"
-                                                         + TILE_LABEL[args.tiles],
+                                                         + TILE_LABEL[args.tiles]
+                                                         + (f", {args.tile_sweeps} sweeps on the tile levels"
+                                                            if args.tiles and args.tile_sweeps else ""),
                    "initial_guess": "learned pressure prediction (encoder-decoder on the next-coarser level) "
                                     "over a zero start, kept only when it lowers the residual, fallback zero, "
                                     "with the fallback controller",

@@ -113,6 +113,8 @@ typedef struct or_state {
     int mg_blk_ov;
     int mg_blk_max;
     int mg_blk_nu;  /* sweeps (pre and post) on the tile levels; 0: the levels' own count */
+    int mg_blk2[3]; /* the tile of the tile levels of more than mg_blk2_min cells (0: mg_blk) */
+    int mg_blk2_min;
 } or_state;
 
 /* field ids shared with the GPU library's dg_field ids */
@@ -178,6 +180,7 @@ void or_set_block_smoother(or_state *st, int bx, int by, int bz, int min_cells);
 void or_set_block_overlap(or_state *st, int ov);
 void or_set_block_max(or_state *st, int max_cells);
 void or_set_block_sweeps(or_state *st, int nu);
+void or_set_block_large(or_state *st, int bx, int by, int bz, int min_cells);
 double or_resid_norm(or_state *st, double *bnorm_out);
 
 #ifdef __cplusplus

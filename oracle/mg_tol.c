@@ -83,7 +83,7 @@ static double tw1(int i, int c, int nc, int dirichlet_hi)
      * tile grown by ov cells on every side (within the domain) is smoothed in a private copy whose        \
      * neighbours outside it hold the iterate before the sweeps, and only the tile's own cells are         \
      * written back (restricted additive Schwarz; ov = 0 is block Jacobi).  The per-cell arithmetic is     \
-     * gs_colour's.                                                                                         \
+     * gs_colour's (the GPU multiplies by the reciprocal of the diagonal: last-bit differences).         \
      */                                                                                                   \
     static void blk_smooth_##SFX(T *u, T *uo, const T *rhs, const T *a, const T *b, const T *c, int nx, int ny, \
                                  int nz, T h2, T om, int nsw, int c0, const int *blk, int ov)                     \
@@ -273,6 +273,15 @@ typedef struct tol_ws {
 /* the tile smoother serves a level below the finest (and above the
  * coarsest) of more than mg_blk_min and (mg_blk_max > 0) at most mg_blk_max
  * interior cells */
+/* the tile of a level: mg_blk2 on levels of more than mg_blk2_min cells
+ * (when set), mg_blk otherwise */
+static const int *blk_dims(const or_state *st, int nx, int ny, int nz)
+{
+    const double cells = (double)(nx - 2) * (ny - 2) * (nz - 2);
+    if (st->mg_blk2[0] > 0 && cells > (double)st->mg_blk2_min) return st->mg_blk2;
+    return st->mg_blk;
+}
+
 static int use_blk(const or_state *st, int nx, int ny, int nz)
 {
     const double cells = (double)(nx - 2) * (ny - 2) * (nz - 2);
@@ -298,7 +307,7 @@ static void vcycle_d(or_state *st, tol_ws *W, int l)
     const double om = st->mg_omega > 0.0 ? st->mg_omega : 1.0;
     const int blk = !top && use_blk(st, nx, ny, nz);
     const int nub = st->mg_blk_nu > 0 ? st->mg_blk_nu : nu1;
-    if (blk) blk_smooth_d(W->u[l], W->r[l], W->f[l], L->a, L->b, L->c, nx, ny, nz, h2, om, nub, 0, st->mg_blk, st->mg_blk_ov);
+    if (blk) blk_smooth_d(W->u[l], W->r[l], W->f[l], L->a, L->b, L->c, nx, ny, nz, h2, om, nub, 0, blk_dims(st, nx, ny, nz), st->mg_blk_ov);
     for (int s = 0; s < nu1 && !blk; s++) {
         gs_colour_d(W->u[l], W->f[l], L->a, L->b, L->c, nx, ny, nz, h2, om, 0);
         gs_colour_d(W->u[l], W->f[l], L->a, L->b, L->c, nx, ny, nz, h2, om, 1);
@@ -310,7 +319,7 @@ static void vcycle_d(or_state *st, tol_ws *W, int l)
     prolong_add_d(W->u[l], W->u[l - 1], nx, ny, nz, st->mg_prolong);
     if (blk)
         blk_smooth_d(W->u[l], W->r[l], W->f[l], L->a, L->b, L->c, nx, ny, nz, h2, om, st->mg_blk_nu > 0 ? st->mg_blk_nu : nu2,
-                     1, st->mg_blk, st->mg_blk_ov);
+                     1, blk_dims(st, nx, ny, nz), st->mg_blk_ov);
     for (int s = 0; s < nu2 && !blk; s++) {
         gs_colour_d(W->u[l], W->f[l], L->a, L->b, L->c, nx, ny, nz, h2, om, 1);
         gs_colour_d(W->u[l], W->f[l], L->a, L->b, L->c, nx, ny, nz, h2, om, 0);
@@ -337,7 +346,7 @@ static void vcycle_f(or_state *st, tol_ws *W, int l)
     const int blk = !top && use_blk(st, nx, ny, nz);
     if (blk)
         blk_smooth_f(W->uf[l], W->rf[l], W->ff[l], W->af[l], W->bf[l], W->cf[l], nx, ny, nz, h2f, om,
-                     st->mg_blk_nu > 0 ? st->mg_blk_nu : nu1, 0, st->mg_blk, st->mg_blk_ov);
+                     st->mg_blk_nu > 0 ? st->mg_blk_nu : nu1, 0, blk_dims(st, nx, ny, nz), st->mg_blk_ov);
     for (int s = 0; s < nu1 && !blk; s++) {
         gs_colour_f(W->uf[l], W->ff[l], W->af[l], W->bf[l], W->cf[l], nx, ny, nz, h2f, om, 0);
         gs_colour_f(W->uf[l], W->ff[l], W->af[l], W->bf[l], W->cf[l], nx, ny, nz, h2f, om, 1);
@@ -349,7 +358,7 @@ static void vcycle_f(or_state *st, tol_ws *W, int l)
     prolong_add_f(W->uf[l], W->uf[l - 1], nx, ny, nz, st->mg_prolong);
     if (blk)
         blk_smooth_f(W->uf[l], W->rf[l], W->ff[l], W->af[l], W->bf[l], W->cf[l], nx, ny, nz, h2f, om,
-                     st->mg_blk_nu > 0 ? st->mg_blk_nu : nu2, 1, st->mg_blk, st->mg_blk_ov);
+                     st->mg_blk_nu > 0 ? st->mg_blk_nu : nu2, 1, blk_dims(st, nx, ny, nz), st->mg_blk_ov);
     for (int s = 0; s < nu2 && !blk; s++) {
         gs_colour_f(W->uf[l], W->ff[l], W->af[l], W->bf[l], W->cf[l], nx, ny, nz, h2f, om, 1);
         gs_colour_f(W->uf[l], W->ff[l], W->af[l], W->bf[l], W->cf[l], nx, ny, nz, h2f, om, 0);
@@ -505,3 +514,11 @@ void or_set_block_overlap(or_state *st, int ov) { st->mg_blk_ov = ov; }
 void or_set_block_max(or_state *st, int max_cells) { st->mg_blk_max = max_cells; }
 
 void or_set_block_sweeps(or_state *st, int nu) { st->mg_blk_nu = nu; }
+
+void or_set_block_large(or_state *st, int bx, int by, int bz, int min_cells)
+{
+    st->mg_blk2[0] = bx;
+    st->mg_blk2[1] = by;
+    st->mg_blk2[2] = bz;
+    st->mg_blk2_min = min_cells;
+}

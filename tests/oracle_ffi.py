@@ -59,6 +59,7 @@ def lib():
         L.or_set_block_overlap.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.or_set_block_max.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.or_set_block_sweeps.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.or_set_block_large.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 4
         L.or_mglin.argtypes = [ctypes.c_void_p]
         L.or_resid_norm.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
         L.or_resid_norm.restype = ctypes.c_double

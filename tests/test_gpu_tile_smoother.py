@@ -1,10 +1,12 @@
 """Tile smoother of the tolerance-mode V-cycle (dg_set_tile_smoother): the
-levels between the finest and the single-workgroup tail smooth each 16x8x8
-tile on its own (restricted additive Schwarz with an overlap of one cell,
-red-black SOR inside), one launch per level visit.
+levels between the finest and the single-workgroup tail smooth each tile
+(16x8x8 cells, 32x16x8 on levels of more than 2^18 cells) on its own
+(restricted additive Schwarz with an overlap of one cell, red-black SOR
+inside), one launch per level visit.
 
 The FP64 solve follows the CPU statement of the same smoother
-(oracle/mg_tol.c, or_set_block_smoother(16, 8, 8, 4096) with overlap 1):
+(oracle/mg_tol.c, or_set_block_smoother(16, 8, 8, 4096),
+or_set_block_large(32, 16, 8, 2^18), overlap 1):
 solutions to 1e-8 and iterations within one.  The FP32-preconditioned solve
 with tiles matches the FP64 solve without them to 1e-6 and returns its true
 FP64 residual; graph replay, look-ahead and launch-by-launch agree to the
@@ -67,6 +69,7 @@ def oracle_solve(oracle, n, rhs, a, b, c, tiles, tol=1e-10, nuc=4, omega=1.15, s
         L.or_set_block_overlap(o.h, 1)
         L.or_set_block_max(o.h, 1 << 18 if tiles == 1 else 0)
         L.or_set_block_sweeps(o.h, tile_nu)
+        L.or_set_block_large(o.h, 32, 16, 8, 1 << 18)
     it = L.or_mg_solve_tol(o.h)
     p = o.field("p").copy()
     o.close()

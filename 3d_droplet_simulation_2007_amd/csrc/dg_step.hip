@@ -1064,6 +1064,18 @@ static int launch_column27(const Grid &g, hipStream_t s, const double *src, F f)
 
 #define DG_P27(di, dj, dk) P[(dk) + 1][((dj) + 1) * 3 + (di) + 1]
 
+// the 27 values of a column kernel's block are one value, bit for bit
+__device__ inline bool uniform27(const double (&P)[3][9])
+{
+    const long long v = __double_as_longlong(P[1][4]);
+    bool u = true;
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int q = 0; q < 9; q++) u = u && __double_as_longlong(P[p][q]) == v;
+    return u;
+}
+
 int mg_filterc27(const Grid &g, hipStream_t s, const double *cc, double *cf)
 {
     return launch_column27(g, s, cc, [=] __device__(const double (&P)[3][9], int o) {
@@ -1143,6 +1155,13 @@ int op_csf(Ctx &c, bool w_interior)
     const double sigh = 0.5 * (c.s.tau * c.s.tau) * c.s.sigma / (c.s.h * c.s.h * c.s.h);
     const int sy = g.sy, sz = g.sz;
     if (launch_column27(g, s, ccf, [=] __device__(const double (&P)[3][9], int o) {
+            // a wavefront whose cells all see one value (bit for bit) on
+            // their 3x3x3 blocks: every difference below is +0, every
+            // normalised gradient +0 / sqrt(e50), and the result -(+0)
+            if (__all(uniform27(P))) {
+                w11[o] = -0.0;
+                return;
+            }
             const double e50 = 1.0e-50;
 #define F DG_P27
             double nxu = sqrt(sq(F(1, 0, 0) - F(0, 0, 0)) + 0.0625 * sq(F(0, 1, 0) - F(0, -1, 0) + F(1, 1, 0) - F(1, -1, 0)) +
@@ -1414,6 +1433,13 @@ static int sgip_front(Ctx &c, int nf)
     }
     if (c.s.sigma == 0.0) return 0;
     if (launch_column27(g, s, ccf, [=] __device__(const double (&P)[3][9], int o) {
+            // a wavefront whose cells all see one value (bit for bit) on
+            // their 3x3x3 blocks: every difference below is +0, every
+            // normalised gradient +0 / sqrt(e50), and the result -(+0)
+            if (__all(uniform27(P))) {
+                w11[o] = -0.0;
+                return;
+            }
             const double e50 = 1.0e-50;
 #define F DG_P27
             double nxu = sqrt(sq(F(1, 0, 0) - F(0, 0, 0)) + 0.0625 * sq(F(0, 1, 0) - F(0, -1, 0) + F(1, 1, 0) - F(1, -1, 0)) +

@@ -111,6 +111,139 @@ __global__ void __launch_bounds__(256) k_conv_fwd(int nx, int ny, int nz, const 
     }
 }
 
+// The same convolution with the input staged in LDS: a workgroup owns a
+// tile of 32 x 2R x 2 voxels (each wave R rows of 32 voxels: 2R 16-row M
+// tiles) and first copies the tile's input with its one-voxel halo, 34 x
+// (2R+2) x 4 voxels of all cinp channels, from HBM into LDS with 16-byte
+// loads (zeros outside the grid).  The K loop then reads each A fragment
+// from LDS instead of gathering it from global memory, and loads each B
+// fragment (the packed weights, from global memory: small, cached) once for
+// all of the wave's M tiles.  Same K order, same MFMAs: the same bits.
+constexpr int CF_TX = 32, CF_TZ = 2;
+
+template <int R>
+struct CfTile {
+    static constexpr int TY = 2 * R, HX = CF_TX + 2, HY = TY + 2, HZ = CF_TZ + 2, HN = HX * HY * HZ;
+};
+
+template <int NT, int R>
+__global__ void __launch_bounds__(256) k_conv_fwd_lds(int nx, int ny, int nz, const bf16_t *__restrict__ x, int xs,
+                                                      int cg, const bf16_t *__restrict__ wB, int coutp,
+                                                      const float *__restrict__ bias, int cout, int relu,
+                                                      bf16_t *__restrict__ y, int ys, float *__restrict__ yf, int yfs,
+                                                      int ntx, int nty)
+{
+    using TL = CfTile<R>;
+    constexpr int MT = 2 * R;  // M tiles per wave
+    extern __shared__ __align__(16) unsigned char cf_lds[];
+    uint4 *const hal = reinterpret_cast<uint4 *>(cf_lds);  // [voxel of the halo tile][channel group]
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row = lane & 15, kg = lane >> 4;
+    const int blk = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const int tx = blk % ntx, ty = (blk / ntx) % nty, tz = blk / (ntx * nty);
+    const int x0 = tx * CF_TX, y0 = ty * TL::TY, z0 = tz * CF_TZ;
+    if (z0 >= nz) return;  // the grid's rounding to a multiple of 8 blocks
+    // the halo tile, eight 16-byte loads in flight per thread before their
+    // stores; a place outside the grid loads voxel 0 and stores zeros
+    constexpr int UB = 8;
+    const int ne = TL::HN * cg;
+    for (int e0 = (int)threadIdx.x; e0 < ne; e0 += 256 * UB) {
+        uint4 v[UB];
+        bool in[UB];
+#pragma unroll
+        for (int u = 0; u < UB; u++) {
+            const int e = min(e0 + u * 256, ne - 1);
+            const int hv = e / cg, c8 = e - hv * cg;
+            const int hx = hv % TL::HX, hy = (hv / TL::HX) % TL::HY, hz = hv / (TL::HX * TL::HY);
+            const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
+            in[u] = (unsigned)gx < (unsigned)nx && (unsigned)gy < (unsigned)ny && (unsigned)gz < (unsigned)nz;
+            const size_t src = in[u] ? ((size_t)((long long)gz * ny + gy) * nx + gx) * xs + c8 * 8 : 0;
+            v[u] = *reinterpret_cast<const uint4 *>(x + src);
+        }
+#pragma unroll
+        for (int u = 0; u < UB; u++)
+            if (e0 + u * 256 < ne) hal[e0 + u * 256] = in[u] ? v[u] : make_uint4(0u, 0u, 0u, 0u);
+    }
+    __syncthreads();
+    // this wave's rows: tile rows wave * R + q, q < R (row = (y, z) of the tile)
+    int rbase[R];
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+        const int tr = wave * R + q, ry = tr % TL::TY, rz = tr / TL::TY;
+        rbase[q] = ((rz + 1) * TL::HY + (ry + 1)) * TL::HX + 1;  // halo voxel of the row's first voxel
+    }
+    f32x4 acc[MT][NT];
+#pragma unroll
+    for (int m = 0; m < MT; m++)
+#pragma unroll
+        for (int n = 0; n < NT; n++) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int G = 27 * cg;
+    int g = kg, t = kg / cg, c8 = kg - (kg / cg) * cg;
+    for (int g0 = 0; g0 < G; g0 += 4, g += 4) {
+        const bool gv = g < G;
+        const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+        const int toff = (dz * TL::HY + dy) * TL::HX + dx;
+        bf16x8 b[NT];
+        const bf16_t *wg = wB + ((size_t)(gv ? g : 0) * coutp + row) * 8;
+#pragma unroll
+        for (int n = 0; n < NT; n++) b[n] = gv ? ld8(wg + n * 128) : zero8();
+#pragma unroll
+        for (int m = 0; m < MT; m++) {
+            // voxel (m & 1) * 16 + row of the wave's row m >> 1, its tap neighbour
+            const int hv = rbase[m >> 1] + toff + (m & 1) * 16 + row;
+            const bf16x8 a = gv ? __builtin_bit_cast(bf16x8, hal[hv * cg + c8]) : zero8();
+#pragma unroll
+            for (int n = 0; n < NT; n++) acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[n], acc[m][n], 0, 0, 0);
+        }
+        c8 += 4;
+        while (c8 >= cg) {
+            c8 -= cg;
+            t++;
+        }
+    }
+#pragma unroll
+    for (int n = 0; n < NT; n++) {
+        const int co = n * 16 + row;
+        if (co >= cout) continue;
+        const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+        for (int m = 0; m < MT; m++) {
+            const int tr = wave * R + (m >> 1), gyw = y0 + tr % TL::TY, gzw = z0 + tr / TL::TY;
+            if (gyw >= ny || gzw >= nz) continue;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int gx = x0 + (m & 1) * 16 + kg * 4 + r;
+                if (gx >= nx) continue;
+                const size_t v = ((size_t)gzw * ny + gyw) * nx + gx;
+                float val = acc[m][n][r] + bv;
+                if (relu) val = fmaxf(val, 0.f);
+                if (yf)
+                    yf[v * yfs + co] = val;
+                else
+                    y[v * ys + co] = f2bf(val);
+            }
+        }
+    }
+}
+
+template <int NT, int R>
+static int launch_conv_fwd_lds(int nx, int ny, int nz, const bf16_t *x, int xs, int cg, const bf16_t *wB, int coutp,
+                               const float *bias, int cout, int relu, bf16_t *y, int ys, float *yf, int yfs,
+                               hipStream_t s)
+{
+    using TL = CfTile<R>;
+    const int ntx = (nx + CF_TX - 1) / CF_TX, nty = (ny + TL::TY - 1) / TL::TY, ntz = (nz + CF_TZ - 1) / CF_TZ;
+    long long nbl = (long long)ntx * nty * ntz;
+    nbl = (nbl + 7) / 8 * 8;
+    const size_t lds = (size_t)TL::HN * cg * 16;
+    if (lds > 48 * 1024)
+        DG_HIP(hipFuncSetAttribute((const void *)k_conv_fwd_lds<NT, R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds));
+    hipLaunchKernelGGL((k_conv_fwd_lds<NT, R>), dim3((unsigned)nbl), dim3(256), lds, s, nx, ny, nz, x, xs, cg, wB,
+                       coutp, bias, cout, relu, y, ys, yf, yfs, ntx, nty);
+    return 0;
+}
+
 int conv3d_fwd(int nx, int ny, int nz, const bf16_t *x, int xs, int cinp, const bf16_t *wB, int coutp,
                const float *bias, int cout, int relu, bf16_t *y, int ys, float *yf, int yfs, hipStream_t s)
 {
@@ -125,10 +258,30 @@ int conv3d_fwd(int nx, int ny, int nz, const bf16_t *x, int xs, int cinp, const 
     }
     const long long V = (long long)nx * ny * nz;
     if (V <= 0) return 0;
+    const int cg = cinp / 8;
+    // the LDS-staged form where a row of 32 voxels is mostly inside the grid
+    // and the halo tile fits the LDS: two rows per wave (B reused over four
+    // M tiles) up to 48 input channels, one row per wave up to 64
+    if (nx >= CF_TX && cinp <= 64 && (coutp / 16 == 1 || coutp / 16 == 2 || coutp / 16 == 4)) {
+        prof_begin(PK_CONV_FWD, s);
+        int rc;
+        if (cinp <= 48) {
+            if (coutp == 16) rc = launch_conv_fwd_lds<1, 2>(nx, ny, nz, x, xs, cg, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s);
+            else if (coutp == 32) rc = launch_conv_fwd_lds<2, 2>(nx, ny, nz, x, xs, cg, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s);
+            else rc = launch_conv_fwd_lds<4, 2>(nx, ny, nz, x, xs, cg, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s);
+        } else {
+            if (coutp == 16) rc = launch_conv_fwd_lds<1, 1>(nx, ny, nz, x, xs, cg, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s);
+            else if (coutp == 32) rc = launch_conv_fwd_lds<2, 1>(nx, ny, nz, x, xs, cg, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s);
+            else rc = launch_conv_fwd_lds<4, 1>(nx, ny, nz, x, xs, cg, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s);
+        }
+        if (rc) return -1;
+        DG_CHECK_LAUNCH();
+        prof_end(PK_CONV_FWD, s, 2.0 * (double)V * 27.0 * (double)cinp * (double)cout);
+        return 0;
+    }
     long long nb = (V + 127) / 128;
     nb = (nb + 7) / 8 * 8;
     const dim3 grid((unsigned)nb), block(256);
-    const int cg = cinp / 8;
     prof_begin(PK_CONV_FWD, s);
     switch (coutp / 16) {
     case 1: hipLaunchKernelGGL(k_conv_fwd<1>, grid, block, 0, s, nx, ny, nz, x, xs, cg, wB, coutp, bias, cout, relu, y, ys, yf, yfs); break;

@@ -113,12 +113,15 @@ __global__ void __launch_bounds__(256) k_conv_fwd(int nx, int ny, int nz, const 
 
 // The same convolution with the input staged in LDS: a workgroup owns a
 // tile of 32 x 2R x 2 voxels (each wave R rows of 32 voxels: 2R 16-row M
-// tiles) and first copies the tile's input with its one-voxel halo, 34 x
-// (2R+2) x 4 voxels of all cinp channels, from HBM into LDS with 16-byte
-// loads (zeros outside the grid).  The K loop then reads each A fragment
-// from LDS instead of gathering it from global memory, and loads each B
-// fragment (the packed weights, from global memory: small, cached) once for
-// all of the wave's M tiles.  Same K order, same MFMAs: the same bits.
+// tiles; R = 2 up to 48 input channels, else 1) and first copies the tile's
+// input with its one-voxel halo, 34 x (2R+2) x 4 voxels of all cinp
+// channels, from HBM into LDS with 16-byte loads (zeros outside the grid).
+// The K loop is unrolled at compile time (CG = cinp / 8 channel groups, NT =
+// coutp / 16 output tiles): each step reads the lane's A offset from a small
+// LDS table (k group -> tap and channel group), its A fragments from the
+// halo tile at constant distances, and its B fragment (packed weights, from
+// global memory: small, cached) D steps ahead of its use, once for all of
+// the wave's M tiles.  Same K order, same MFMAs: the same bits.
 constexpr int CF_TX = 32, CF_TZ = 2;
 
 template <int R>
@@ -126,34 +129,58 @@ struct CfTile {
     static constexpr int TY = 2 * R, HX = CF_TX + 2, HY = TY + 2, HZ = CF_TZ + 2, HN = HX * HY * HZ;
 };
 
-template <int NT, int R>
-__global__ void __launch_bounds__(256) k_conv_fwd_lds(int nx, int ny, int nz, const bf16_t *__restrict__ x, int xs,
-                                                      int cg, const bf16_t *__restrict__ wB, int coutp,
-                                                      const float *__restrict__ bias, int cout, int relu,
-                                                      bf16_t *__restrict__ y, int ys, float *__restrict__ yf, int yfs,
-                                                      int ntx, int nty)
-{
+template <int CG>
+struct CfShape {
+    static constexpr int R = CG <= 6 ? 2 : 1, G = 27 * CG, NS = (G + 3) / 4, NSP = (NS + 3) / 4 * 4,
+                         REM = G - 4 * (NS - 1);
     using TL = CfTile<R>;
-    constexpr int MT = 2 * R;  // M tiles per wave
+    static constexpr size_t lds = (size_t)TL::HN * CG * 16 + (size_t)4 * NSP * 4;
+};
+
+// halo-tile offset, in voxels, of tap t (x fastest, then y, then z)
+template <int HX, int HY>
+__device__ constexpr int cf_toff(int t)
+{
+    return (t / 9 - 1) * HX * HY + ((t / 3) % 3 - 1) * HX + (t % 3 - 1);
+}
+
+template <int NT, int CG>
+__global__ void __launch_bounds__(256) k_conv_fwd_lds(int nx, int ny, int nz, const bf16_t *__restrict__ x, int xs,
+                                                      const bf16_t *__restrict__ wB, const float *__restrict__ bias,
+                                                      int cout, int relu, bf16_t *__restrict__ y, int ys,
+                                                      float *__restrict__ yf, int yfs, int ntx, int nty)
+{
+    using S = CfShape<CG>;
+    using TL = typename S::TL;
+    constexpr int R = S::R, MT = 2 * R, NS = S::NS, NSP = S::NSP, REM = S::REM;
+    constexpr int COUTP = 16 * NT;
+    constexpr int D = NT >= 4 ? 2 : 4;  // B prefetch distance in steps
     extern __shared__ __align__(16) unsigned char cf_lds[];
     uint4 *const hal = reinterpret_cast<uint4 *>(cf_lds);  // [voxel of the halo tile][channel group]
+    int *const tab = reinterpret_cast<int *>(hal + TL::HN * CG);  // [k group mod 4][step]
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int row = lane & 15, kg = lane >> 4;
     const int blk = xcd_tile((int)blockIdx.x, (int)gridDim.x);
     const int tx = blk % ntx, ty = (blk / ntx) % nty, tz = blk / (ntx * nty);
     const int x0 = tx * CF_TX, y0 = ty * TL::TY, z0 = tz * CF_TZ;
     if (z0 >= nz) return;  // the grid's rounding to a multiple of 8 blocks
+    // offset table: k group g = 4 s + q reads channel group g % CG of tap
+    // g / CG, at this many uint4 from the output voxel's own first group
+    if ((int)threadIdx.x < 4 * NSP) {
+        const int q = (int)threadIdx.x / NSP, s = (int)threadIdx.x % NSP, g = min(4 * s + q, S::G - 1);
+        const int t = g / CG;
+        tab[threadIdx.x] = cf_toff<TL::HX, TL::HY>(t) * CG + (g - t * CG);
+    }
     // the halo tile, eight 16-byte loads in flight per thread before their
     // stores; a place outside the grid loads voxel 0 and stores zeros
-    constexpr int UB = 8;
-    const int ne = TL::HN * cg;
-    for (int e0 = (int)threadIdx.x; e0 < ne; e0 += 256 * UB) {
+    constexpr int UB = 8, NE = TL::HN * CG;
+    for (int e0 = (int)threadIdx.x; e0 < NE; e0 += 256 * UB) {
         uint4 v[UB];
         bool in[UB];
 #pragma unroll
         for (int u = 0; u < UB; u++) {
-            const int e = min(e0 + u * 256, ne - 1);
-            const int hv = e / cg, c8 = e - hv * cg;
+            const int e = min(e0 + u * 256, NE - 1);
+            const int hv = e / CG, c8 = e - hv * CG;
             const int hx = hv % TL::HX, hy = (hv / TL::HX) % TL::HY, hz = hv / (TL::HX * TL::HY);
             const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
             in[u] = (unsigned)gx < (unsigned)nx && (unsigned)gy < (unsigned)ny && (unsigned)gz < (unsigned)nz;
@@ -162,44 +189,48 @@ __global__ void __launch_bounds__(256) k_conv_fwd_lds(int nx, int ny, int nz, co
         }
 #pragma unroll
         for (int u = 0; u < UB; u++)
-            if (e0 + u * 256 < ne) hal[e0 + u * 256] = in[u] ? v[u] : make_uint4(0u, 0u, 0u, 0u);
+            if (e0 + u * 256 < NE) hal[e0 + u * 256] = in[u] ? v[u] : make_uint4(0u, 0u, 0u, 0u);
     }
-    __syncthreads();
-    // this wave's rows: tile rows wave * R + q, q < R (row = (y, z) of the tile)
-    int rbase[R];
+    // B fragments of the first D steps while the tile lands
+    const bf16_t *const wl = wB + ((size_t)kg * COUTP + row) * 8;  // step s adds s * 4 * COUTP * 8
+    auto load_b = [&](int s, bf16x8 *b) {
+        const bool ok = s < NS - 1 || REM == 4 || kg < REM;
 #pragma unroll
-    for (int q = 0; q < R; q++) {
-        const int tr = wave * R + q, ry = tr % TL::TY, rz = tr / TL::TY;
-        rbase[q] = ((rz + 1) * TL::HY + (ry + 1)) * TL::HX + 1;  // halo voxel of the row's first voxel
-    }
+        for (int n = 0; n < NT; n++) b[n] = ok ? ld8(wl + (size_t)(ok ? s : 0) * 4 * COUTP * 8 + n * 128) : zero8();
+    };
+    bf16x8 bq[D][NT];
+#pragma unroll
+    for (int s = 0; s < D; s++)
+        if (s < NS) load_b(s, bq[s]);
+    __syncthreads();
+    // the wave's rows are tile rows wave * R + q, q < R, which share z (TY
+    // is even): row q starts q * HX halo voxels after row 0
+    const int tr0 = wave * R, ry = tr0 % TL::TY, rz = tr0 / TL::TY;
+    const uint4 *const ab = hal + (((rz + 1) * TL::HY + (ry + 1)) * TL::HX + 1 + row) * CG;
+    const int4 *const tp = reinterpret_cast<const int4 *>(tab + kg * NSP);
     f32x4 acc[MT][NT];
 #pragma unroll
     for (int m = 0; m < MT; m++)
 #pragma unroll
         for (int n = 0; n < NT; n++) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const int G = 27 * cg;
-    int g = kg, t = kg / cg, c8 = kg - (kg / cg) * cg;
-    for (int g0 = 0; g0 < G; g0 += 4, g += 4) {
-        const bool gv = g < G;
-        const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
-        const int toff = (dz * TL::HY + dy) * TL::HX + dx;
-        bf16x8 b[NT];
-        const bf16_t *wg = wB + ((size_t)(gv ? g : 0) * coutp + row) * 8;
+    int4 tv = tp[0];
 #pragma unroll
-        for (int n = 0; n < NT; n++) b[n] = gv ? ld8(wg + n * 128) : zero8();
+    for (int s = 0; s < NS; s++) {
+        const int off = (s & 3) == 0 ? tv.x : (s & 3) == 1 ? tv.y : (s & 3) == 2 ? tv.z : tv.w;
+        if ((s & 3) == 3 && s + 1 < NS) tv = tp[(s + 1) >> 2];
+        const bool ok = s < NS - 1 || REM == 4 || kg < REM;
+        bf16x8 a[MT];
 #pragma unroll
         for (int m = 0; m < MT; m++) {
-            // voxel (m & 1) * 16 + row of the wave's row m >> 1, its tap neighbour
-            const int hv = rbase[m >> 1] + toff + (m & 1) * 16 + row;
-            const bf16x8 a = gv ? __builtin_bit_cast(bf16x8, hal[hv * cg + c8]) : zero8();
+            const bf16x8 av = __builtin_bit_cast(bf16x8, ab[off + ((m >> 1) * TL::HX + (m & 1) * 16) * CG]);
+            a[m] = ok ? av : zero8();
+        }
+        bf16x8 *const b = bq[s % D];
 #pragma unroll
-            for (int n = 0; n < NT; n++) acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[n], acc[m][n], 0, 0, 0);
-        }
-        c8 += 4;
-        while (c8 >= cg) {
-            c8 -= cg;
-            t++;
-        }
+        for (int m = 0; m < MT; m++)
+#pragma unroll
+            for (int n = 0; n < NT; n++) acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b[n], acc[m][n], 0, 0, 0);
+        if (s + D < NS) load_b(s + D, b);
     }
 #pragma unroll
     for (int n = 0; n < NT; n++) {
@@ -208,7 +239,7 @@ __global__ void __launch_bounds__(256) k_conv_fwd_lds(int nx, int ny, int nz, co
         const float bv = bias ? bias[co] : 0.f;
 #pragma unroll
         for (int m = 0; m < MT; m++) {
-            const int tr = wave * R + (m >> 1), gyw = y0 + tr % TL::TY, gzw = z0 + tr / TL::TY;
+            const int tr = tr0 + (m >> 1), gyw = y0 + tr % TL::TY, gzw = z0 + tr / TL::TY;
             if (gyw >= ny || gzw >= nz) continue;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
@@ -226,22 +257,32 @@ __global__ void __launch_bounds__(256) k_conv_fwd_lds(int nx, int ny, int nz, co
     }
 }
 
-template <int NT, int R>
-static int launch_conv_fwd_lds(int nx, int ny, int nz, const bf16_t *x, int xs, int cg, const bf16_t *wB, int coutp,
-                               const float *bias, int cout, int relu, bf16_t *y, int ys, float *yf, int yfs,
-                               hipStream_t s)
+template <int NT, int CG>
+static int launch_conv_fwd_lds(int nx, int ny, int nz, const bf16_t *x, int xs, const bf16_t *wB, const float *bias,
+                               int cout, int relu, bf16_t *y, int ys, float *yf, int yfs, hipStream_t s)
 {
-    using TL = CfTile<R>;
+    using S = CfShape<CG>;
+    using TL = typename S::TL;
     const int ntx = (nx + CF_TX - 1) / CF_TX, nty = (ny + TL::TY - 1) / TL::TY, ntz = (nz + CF_TZ - 1) / CF_TZ;
     long long nbl = (long long)ntx * nty * ntz;
     nbl = (nbl + 7) / 8 * 8;
-    const size_t lds = (size_t)TL::HN * cg * 16;
-    if (lds > 48 * 1024)
-        DG_HIP(hipFuncSetAttribute((const void *)k_conv_fwd_lds<NT, R>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)lds));
-    hipLaunchKernelGGL((k_conv_fwd_lds<NT, R>), dim3((unsigned)nbl), dim3(256), lds, s, nx, ny, nz, x, xs, cg, wB,
-                       coutp, bias, cout, relu, y, ys, yf, yfs, ntx, nty);
+    if (S::lds > 48 * 1024)
+        DG_HIP(hipFuncSetAttribute((const void *)k_conv_fwd_lds<NT, CG>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)S::lds));
+    hipLaunchKernelGGL((k_conv_fwd_lds<NT, CG>), dim3((unsigned)nbl), dim3(256), S::lds, s, nx, ny, nz, x, xs, wB,
+                       bias, cout, relu, y, ys, yf, yfs, ntx, nty);
     return 0;
+}
+
+template <int CG>
+static int conv_fwd_lds_nt(int nx, int ny, int nz, const bf16_t *x, int xs, const bf16_t *wB, int coutp,
+                           const float *bias, int cout, int relu, bf16_t *y, int ys, float *yf, int yfs, hipStream_t s)
+{
+    switch (coutp) {
+    case 16: return launch_conv_fwd_lds<1, CG>(nx, ny, nz, x, xs, wB, bias, cout, relu, y, ys, yf, yfs, s);
+    case 32: return launch_conv_fwd_lds<2, CG>(nx, ny, nz, x, xs, wB, bias, cout, relu, y, ys, yf, yfs, s);
+    default: return launch_conv_fwd_lds<4, CG>(nx, ny, nz, x, xs, wB, bias, cout, relu, y, ys, yf, yfs, s);
+    }
 }
 
 int conv3d_fwd(int nx, int ny, int nz, const bf16_t *x, int xs, int cinp, const bf16_t *wB, int coutp,
@@ -260,19 +301,16 @@ int conv3d_fwd(int nx, int ny, int nz, const bf16_t *x, int xs, int cinp, const 
     if (V <= 0) return 0;
     const int cg = cinp / 8;
     // the LDS-staged form where a row of 32 voxels is mostly inside the grid
-    // and the halo tile fits the LDS: two rows per wave (B reused over four
-    // M tiles) up to 48 input channels, one row per wave up to 64
-    if (nx >= CF_TX && cinp <= 64 && (coutp / 16 == 1 || coutp / 16 == 2 || coutp / 16 == 4)) {
+    // and the halo tile fits the LDS (up to 64 input channels)
+    if (nx >= CF_TX && (cg <= 2 || cg == 4 || cg == 6 || cg == 8) && (coutp == 16 || coutp == 32 || coutp == 64)) {
         prof_begin(PK_CONV_FWD, s);
         int rc;
-        if (cinp <= 48) {
-            if (coutp == 16) rc = launch_conv_fwd_lds<1, 2>(nx, ny, nz, x, xs, cg, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s);
-            else if (coutp == 32) rc = launch_conv_fwd_lds<2, 2>(nx, ny, nz, x, xs, cg, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s);
-            else rc = launch_conv_fwd_lds<4, 2>(nx, ny, nz, x, xs, cg, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s);
-        } else {
-            if (coutp == 16) rc = launch_conv_fwd_lds<1, 1>(nx, ny, nz, x, xs, cg, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s);
-            else if (coutp == 32) rc = launch_conv_fwd_lds<2, 1>(nx, ny, nz, x, xs, cg, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s);
-            else rc = launch_conv_fwd_lds<4, 1>(nx, ny, nz, x, xs, cg, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s);
+        switch (cg) {
+        case 1: rc = conv_fwd_lds_nt<1>(nx, ny, nz, x, xs, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s); break;
+        case 2: rc = conv_fwd_lds_nt<2>(nx, ny, nz, x, xs, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s); break;
+        case 4: rc = conv_fwd_lds_nt<4>(nx, ny, nz, x, xs, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s); break;
+        case 6: rc = conv_fwd_lds_nt<6>(nx, ny, nz, x, xs, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s); break;
+        default: rc = conv_fwd_lds_nt<8>(nx, ny, nz, x, xs, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s); break;
         }
         if (rc) return -1;
         DG_CHECK_LAUNCH();

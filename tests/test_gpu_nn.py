@@ -44,7 +44,26 @@ CONV_CASES = [
 ]
 
 
-@pytest.mark.parametrize("case", CONV_CASES, ids=[f"{c[0]}_{c[1]}to{c[3]}" for c in CONV_CASES])
+# grids of at least 32 cells in x take the LDS-staged forward kernel (a
+# workgroup per 32 x 2R x 2 tile): every (input groups, output tiles) form it
+# is built for, full tiles and tiles clipped in x, y and z
+LDS_CASES = [
+    ((32, 4, 2), 8, 4, 16, 16, True),
+    ((35, 5, 3), 16, 16, 16, 16, True),
+    ((33, 6, 5), 16, 16, 32, 32, False),
+    ((34, 4, 3), 32, 32, 32, 32, True),
+    ((40, 3, 3), 32, 32, 64, 64, True),
+    ((38, 7, 3), 48, 48, 16, 16, True),
+    ((32, 4, 3), 48, 48, 32, 32, True),
+    ((33, 2, 2), 48, 40, 64, 64, True),
+    ((32, 5, 2), 64, 64, 16, 16, True),
+    ((36, 3, 4), 64, 64, 64, 64, True),
+    ((64, 2, 3), 16, 16, 16, 1, False),
+]
+
+
+@pytest.mark.parametrize("case", CONV_CASES + LDS_CASES,
+                         ids=[f"{c[0]}_{c[1]}to{c[3]}" for c in CONV_CASES + LDS_CASES])
 def test_conv3d_forward(gs, case):
     (nx, ny, nz), cinp, cin, coutp, cout, relu = case
     print("conv fwd", case, flush=True)
@@ -141,6 +160,28 @@ def test_network_forward_and_gradient(gs):
     errs = np.array(errs)
     assert errs[:, :2].max() <= 4e-2, errs[:, :2].max()
     assert errs[:, 2:].max() <= 0.1, errs[:, 2:].max()
+
+
+def test_network_forward_at_32cube(gs):
+    """Whole-network forward at 32^3, where the finest level's five layers
+    (the output layer's FP32 result among them) take the LDS-staged
+    kernel: against the statement with the device's BF16 storage points,
+    the same bar as at 16^3."""
+    print("network 32^3 forward", flush=True)
+    rng = np.random.default_rng(23)
+    p = nn.random_params(24).astype(np.float32)
+    gs.set_params(p)
+    pb = nn.bf16(p.astype(np.float64))
+    offs, _ = nn.param_layout()
+    for (wo, bo), spec in zip(offs, nn.LAYERS):
+        pb[bo:bo + spec[3]] = p[bo:bo + spec[3]]
+    params = nn.unflatten(pb)
+    x0 = rng.normal(size=(32, 32, 32, 4))
+    xb = nn.bf16(x0)
+    y = gs.forward(x0).astype(np.float64)
+    y_dev, _ = nn.forward(params, xb, round_acts=True)
+    print("forward rel L2 vs device-rounded", rel_l2(y, y_dev), flush=True)
+    assert rel_l2(y, y_dev) <= 1e-2
 
 
 def test_training_lowers_the_loss(gs):

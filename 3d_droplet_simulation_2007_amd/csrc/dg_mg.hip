@@ -736,15 +736,23 @@ static_assert(S_AH + PCG_HIST <= 64, "device scalar slots");
 // planes per workgroup of k_pcg_dir
 constexpr int PCG_KCH = 8;
 
+// Barrier for LDS hand-offs only: waits for this wave's LDS operations,
+// not for its global loads in flight (which __syncthreads' workgroup fence
+// would drain), so loads issued before it overlap the wait.
+__device__ inline void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 // A workgroup of 64 x 4 threads owns a 64 x 4 column tile and marches up
 // PCG_KCH planes.  Per plane it forms p_new = z + beta p_old once for the
-// tile and a one-cell ring around it in LDS (two plane buffers: the
-// current plane and the next), so the in-plane neighbours come from LDS;
-// each thread carries its column's plane-below value and c-coefficient in
-// registers.  The per-cell arithmetic is the stencil of resid
-// (main.f:3917-3923) on p_new with bc_scalar folding.  PT: the storage type
-// of the search direction; p_new is rounded to it before the stencil, so q
-// is A applied to the stored direction and x, r stay consistent.
+// tile and a one-cell ring around it in LDS, so the in-plane neighbours
+// come from LDS; each thread carries its column's plane-below value and
+// c-coefficient in registers.  Three plane buffers: plane k + 1 is stored
+// while plane k - 1's buffer is free, so one barrier per plane separates
+// the stores from the reads; the loads of plane k + 2's z and p_old and of
+// plane k's coefficients are issued before that barrier and land while it
+// waits.  The per-cell arithmetic is the stencil of resid (main.f:3917-3923)
+// on p_new with bc_scalar folding.  PT: the storage type of the search
+// direction; p_new is rounded to it before the stencil, so q is A applied
+// to the stored direction and x, r stay consistent.
 template <class Z, typename PT>
 __global__ void __launch_bounds__(256) k_pcg_dir(Grid g, Z z, PT *hist, size_t nslot, double *q, const double *a,
                                                  const double *b, const double *c, double h2i, const double *ds,
@@ -753,8 +761,8 @@ __global__ void __launch_bounds__(256) k_pcg_dir(Grid g, Z z, PT *hist, size_t n
     const int it = (int)ds[S_IT];
     PT *pnew = hist + (size_t)(it % PCG_HIST) * nslot;
     const PT *pold = hist + (size_t)((it + PCG_HIST - 1) % PCG_HIST) * nslot;
-    constexpr int TW = 66, TH = 6;  // tile + ring
-    __shared__ double pl[2][TH][TW];
+    constexpr int TW = 66, TH = 6, NE = TH * TW, NFE = (NE + 255) / 256;  // tile + ring
+    __shared__ double pl[3][TH][TW];
     __shared__ double lds4[4];
     unsigned lin;
     const uint3 tile = xcd_tile(&lin);
@@ -764,48 +772,76 @@ __global__ void __launch_bounds__(256) k_pcg_dir(Grid g, Z z, PT *hist, size_t n
     const int i = i0 + tx, j = j0 + ty;
     const int k0 = (int)tile.z * PCG_KCH + 2;
     const int k1 = min(k0 + PCG_KCH, g.nz);  // interior planes k0 .. k1 - 1 (the last is nz - 1)
+    const int kl = min(k1, g.nz - 1);        // the last plane whose p_new is formed
     const bool first = ds[S_FIRST] != 0.0;
     const double beta = first ? 0.0 : -ds[S_ALPHA] * ds[S_ZQ] / ds[S_RZOLD];
     const int sy = g.sy, sz = g.sz;
     // the first iteration's direction is z itself (p_old is not read)
-    auto P = [&](int ii, int jj, int kk) {
-        const int oo = g.at(ii, jj, kk);
-        const double zz = z(ii, jj, kk, oo);
-        return (double)(PT)(first ? zz : zz + beta * (double)pold[oo]);
+    auto pdir = [&](double zz, PT po) { return (double)(PT)(first ? zz : zz + beta * (double)po); };
+    // this thread's ring places e = tid + 256 u: loads of plane kk (places
+    // outside the interior load the cell (2, 2, kk) and store 0: no
+    // stencil reads them) and their stores into buffer buf
+    int ei[NFE], ej[NFE];
+    bool eok[NFE];
+#pragma unroll
+    for (int u = 0; u < NFE; u++) {
+        const int e = tid + 256 * u, yy = e / TW, xx = e - yy * TW;
+        const int ii = i0 - 1 + xx, jj = j0 - 1 + yy;
+        eok[u] = e < NE && ii >= 2 && ii <= g.nx - 1 && jj >= 2 && jj <= g.ny - 1;
+        ei[u] = eok[u] ? ii : 2;
+        ej[u] = eok[u] ? jj : 2;
+    }
+    double zr[NFE];
+    PT pr[NFE];
+    auto load = [&](int kk) {
+#pragma unroll
+        for (int u = 0; u < NFE; u++) {
+            const int oo = g.at(ei[u], ej[u], kk);
+            zr[u] = z(ei[u], ej[u], kk, oo);
+            pr[u] = pold[oo];  // a ring slot in every iteration; unused in the first
+        }
     };
-    // p_new of plane kk on the tile and its ring (0 outside the interior,
-    // where no stencil reads it)
-    auto fill = [&](int buf, int kk) {
-        for (int e = tid; e < TH * TW; e += 256) {
-            const int yy = e / TW, xx = e - yy * TW;
-            const int ii = i0 - 1 + xx, jj = j0 - 1 + yy;
-            pl[buf][yy][xx] = (ii >= 2 && ii <= g.nx - 1 && jj >= 2 && jj <= g.ny - 1) ? P(ii, jj, kk) : 0.0;
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < NFE; u++) {
+            const int e = tid + 256 * u;
+            if (e < NE) (&pl[buf][0][0])[e] = eok[u] ? pdir(zr[u], pr[u]) : 0.0;
         }
     };
     const bool in = i <= g.nx - 1 && j <= g.ny - 1;
     double contrib = 0.0, p0 = 0.0, pb = 0.0, cb = 0.0;
-    fill(0, k0);
-    __syncthreads();
+    load(k0);
+    store(0);
+    load(min(k0 + 1, kl));
     if (in) {
         const int o = g.at(i, j, k0);
-        p0 = pl[0][ty + 1][tx + 1];
-        pb = k0 > 2 ? P(i, j, k0 - 1) : p0;
+        if (k0 > 2) {
+            const int ob = o - sz;
+            pb = pdir(z(i, j, k0 - 1, ob), pold[ob]);
+        }
         cb = c[o - sz];
+    }
+    lds_barrier();
+    if (in) {
+        p0 = pl[0][ty + 1][tx + 1];
+        if (k0 == 2) pb = p0;
     }
     int cur = 0;
     for (int k = k0; k < k1; k++) {
-        if (k < g.nz - 1) fill(cur ^ 1, k + 1);
-        __syncthreads();
+        const int nxt = cur == 2 ? 0 : cur + 1;
+        const int o = g.at(in ? i : 2, in ? j : 2, k);
+        const double ae = a[o], aw = a[o - 1], bn = b[o], bs = b[o - sy], ct = c[o];
+        if (k + 1 <= kl) store(nxt);
+        load(min(k + 2, kl));  // the last iterations reload plane kl: no branch around the loads
+        lds_barrier();
         if (in) {
-            const int o = g.at(i, j, k);
-            const double pt = k < g.nz - 1 ? pl[cur ^ 1][ty + 1][tx + 1] : p0;
+            const double pt = k < g.nz - 1 ? pl[nxt][ty + 1][tx + 1] : p0;
             const double pe = i < g.nx - 1 ? pl[cur][ty + 1][tx + 2] : 0.0;
             const double pw = i > 2 ? pl[cur][ty + 1][tx] : p0;
             const double pn = j < g.ny - 1 ? pl[cur][ty + 2][tx + 1] : p0;
             const double ps = j > 2 ? pl[cur][ty][tx + 1] : p0;
-            const double ct = c[o];
-            const double s = a[o] * (pe - p0) + a[o - 1] * (pw - p0) + b[o] * (pn - p0) + b[o - sy] * (ps - p0) +
-                             ct * (pt - p0) + cb * (pb - p0);
+            const double s = ae * (pe - p0) + aw * (pw - p0) + bn * (pn - p0) + bs * (ps - p0) + ct * (pt - p0) +
+                             cb * (pb - p0);
             const double qq = h2i * s;
             pnew[o] = (PT)p0;
             q[o] = qq;
@@ -814,8 +850,7 @@ __global__ void __launch_bounds__(256) k_pcg_dir(Grid g, Z z, PT *hist, size_t n
             p0 = pt;
             cb = ct;
         }
-        __syncthreads();
-        cur ^= 1;
+        cur = nxt;
     }
     // block of 64x4: reduce
     double v = wave_sum(contrib);

@@ -393,12 +393,8 @@ def set_tile_smoother(on=0, nu=0):
     _check(lib().dg_set_tile_smoother(_i(int(on)), _i(int(nu))), "dg_set_tile_smoother")
 
 
-def set_tuning(fused=False, graphs=True, split=True, tail="lds"):
+def set_tuning(graphs=True, split=True, tail="lds"):
     """Tolerance-mode execution switches:
-    fused -- one-launch two-colour smoother sweeps instead of two colour
-             passes (bit-identical): False nowhere, True on the natural layout
-             (Gauss-Seidel only) and every split level, "split" on the split
-             levels below the finest of at most 2^18 cells;
     graphs -- replay each PCG iteration as a captured HIP graph (bit-identical);
     split -- mode 2's FP32 V-cycle on red-black split storage with FP16
              coefficients where the densities allow (same tolerance reached);
@@ -410,8 +406,7 @@ def set_tuning(fused=False, graphs=True, split=True, tail="lds"):
             they fit (bit-identical)."""
     sv = {False: 0, True: 1, "fp32": 2, "fp16": 3}[split]
     tv = {None: 0, "global": 1, "lds": 2}[tail]
-    fv = {False: 0, True: 1, "split": 2}[fused]
-    _check(lib().dg_set_tuning(_i(fv), _i(1 if graphs else 0), _i(sv), _i(tv)), "dg_set_tuning")
+    _check(lib().dg_set_tuning(_i(1 if graphs else 0), _i(sv), _i(tv)), "dg_set_tuning")
 
 
 def set_lookahead(ratio=1e3):

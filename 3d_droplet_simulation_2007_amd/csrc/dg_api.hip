@@ -57,7 +57,7 @@ Prof &prof()
 }
 
 static bool g_ready = false;
-static int g_fused_default = 0, g_graphs_default = 1, g_split_default = 1, g_tail_default = 2;
+static int g_graphs_default = 1, g_split_default = 1, g_tail_default = 2;
 static double g_omega_default = 1.0;  // tolerance-mode smoother (dg_set_smoother)
 static int g_nuc_default = 0;
 static int g_tiles_default = 0, g_tile_nu_default = 0;  // tile smoother (dg_set_tile_smoother)
@@ -181,7 +181,7 @@ static void free_ctx(Ctx &c)
     for (int l = 0; l < c.ng; l++) {
         Level &L = c.lev[l];
         void *ps[] = {L.rhs, L.res, L.u, L.cc, L.a, L.b, L.c, L.ei, L.a32, L.b32, L.c32,
-                      L.u32, L.f32, L.r32, L.u64, L.f64, L.r64, L.w32, L.w64, L.rb_u, L.rb_f, L.rb_w, L.rb_u64,
+                      L.u32, L.f32, L.r32, L.u64, L.f64, L.r64, L.rb_u, L.rb_f, L.rb_w, L.rb_u64,
                       L.rb_f64, L.rb_w64, L.rb_coef};
         for (void *p : ps)
             if (p) (void)hipFree(p);
@@ -424,7 +424,6 @@ int resident_setup(int nx, int ny, int nz)
     // 2**ng + 2 check moves to its FMG schedule (op_mglin)
     if (alloc_levels(c, nx, ny, nz, false)) { free_ctx(c); return -1; }
     if (alloc_fields(c)) { free_ctx(c); return -1; }
-    c.fused_smoother = g_fused_default;
     c.use_graphs = g_graphs_default;
     c.rb_split = g_split_default;
     c.rb_tail = g_tail_default;
@@ -767,18 +766,16 @@ int dg_set_vof_guard(const int *on)
     return 0;
 }
 
-int dg_set_tuning(const int *fused, const int *graphs, const int *split, const int *tail)
+int dg_set_tuning(const int *graphs, const int *split, const int *tail)
 {
-    if (*fused < 0 || *fused > 2 || *split < 0 || *split > 3 || *tail < 0 || *tail > 2) {
-        set_error("dg_set_tuning: fused must be 0..2, split 0..3 and tail 0..2");
+    if (*split < 0 || *split > 3 || *tail < 0 || *tail > 2) {
+        set_error("dg_set_tuning: split must be 0..3 and tail 0..2");
         return -1;
     }
-    g_fused_default = *fused;
     g_graphs_default = *graphs ? 1 : 0;
     g_split_default = *split;
     g_tail_default = *tail;
     Ctx &c = *ctx();
-    c.fused_smoother = g_fused_default;
     c.use_graphs = g_graphs_default;
     c.rb_split = g_split_default;
     c.rb_tail = g_tail_default;
@@ -1001,7 +998,6 @@ int dg_poisson_solve(const int *nx, const int *ny, const int *nz, const double *
     if (rc) { set_error("dg_poisson_solve: upload failed"); free_ctx(lc); return -1; }
     lc.s.lengthsize = *lengthsize;
     lc.mg_mode = *mode; lc.mg_tol = *tol; lc.mg_maxit = *maxit; lc.mg_guess = 1;
-    lc.fused_smoother = g_fused_default;
     lc.use_graphs = g_graphs_default;
     lc.rb_split = g_split_default;
     lc.rb_tail = g_tail_default;

@@ -18,14 +18,13 @@ struct Level {
     double *a = nullptr, *b = nullptr, *c = nullptr, *ei = nullptr;
     // tolerance-mode work arrays
     float *a32 = nullptr, *b32 = nullptr, *c32 = nullptr;
-    float *u32 = nullptr, *f32 = nullptr, *r32 = nullptr, *w32 = nullptr;
-    double *u64 = nullptr, *f64 = nullptr, *r64 = nullptr, *w64 = nullptr;
+    float *u32 = nullptr, *f32 = nullptr, *r32 = nullptr;
+    double *u64 = nullptr, *f64 = nullptr, *r64 = nullptr;
     // red-black split storage of the V-cycle (levels above the coarsest):
     // u and f per colour (FP32 with the FP32 V-cycle, FP64 with the FP64
     // one), a/b/c per colour in FP16, FP32 or FP64
     float *rb_u = nullptr, *rb_f = nullptr;
-    float *rb_w = nullptr;  // second iterate buffer of the fused (out-of-place) sweeps and of the tile
-                            // smoother's post-smoothing (whose result it holds)
+    float *rb_w = nullptr;  // the tile smoother's post-smoothing result (out of place)
     double *rb_u64 = nullptr, *rb_f64 = nullptr, *rb_w64 = nullptr;
     void *rb_coef = nullptr;
     int rb_ct = -1;  // coefficient type rb_coef was allocated for: RB_CT_F32, RB_CT_F16, RB_CT_F64 (-1: none)
@@ -79,9 +78,6 @@ struct Ctx {
     int mg_nuc = 0;         // sweeps (pre and post) below the finest level; 0: mg_nu1 / mg_nu2
     int ordered_sums = 1;  // reproduce the reference's sequential outflow sums
     int vof_guard = 0;     // 1: plane normal to the sweep where the Youngs normal vanishes
-    int fused_smoother = 0;  // one-pass red-black sweeps: 0 nowhere; 1 the natural layout (Gauss-Seidel),
-                             // the split finest level (plane marching) and the split levels below it
-                             // (tiles); 2 the split levels of <= 2^18 cells below the finest only
     int use_graphs = 1;      // replay each PCG iteration as a captured HIP graph
     double pcg_lookahead = 1e3;  // PCG iteration n+1 is queued before n's residual reaches the host while
                                  // the last residual the host has seen exceeds this multiple of the
@@ -97,7 +93,7 @@ struct Ctx {
     int top_coef_ready = 0;  // solvelpres's abcei wrote the finest level's split coefficients for this solve
     int coarse_coef_ready = 0;  // acfromac wrote the coarse levels' split coefficients for this solve
     hipGraphExec_t pcg_graph[2] = {nullptr, nullptr};
-    int pcg_graph_key[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    int pcg_graph_key[13] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     // diagnostics of the last step
     int mg_iters_last = 0;
     double mg_relres_last = 0.0, cfl_last = 0.0, epsdiv_last = 0.0;

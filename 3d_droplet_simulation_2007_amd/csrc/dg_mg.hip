@@ -260,111 +260,6 @@ __global__ void __launch_bounds__(256) k_gs_colour(Grid g, T *u, const T *f, con
     u[o] = om * gsv + ((T)1 - om) * uold;
 }
 
-// One full red-black sweep (colour `first` then the other) in a single pass,
-// out of place: u_out = sweep(u_in).  Identical arithmetic to two
-// k_gs_colour launches on one array: the first colour's new values depend
-// only on old second-colour values (read from u_in); the second colour's
-// depend only on new first-colour values, which each block computes itself
-// on its tile plus a one-cell halo ring and exchanges through LDS.
-//
-// Block: 64 x 4 threads over an (x,y) tile of 64 x 16 positions (62 x 14
-// owned columns plus the halo ring), marching up a chunk of RB_KC planes.
-// Each thread owns four consecutive rows of one x column; the rows alternate
-// in colour, so at every plane step p each thread computes two first-colour
-// cells (x,y,p) (written to LDS slot p&1) and, after a barrier, finishes two
-// second-colour cells (x,y,p-1) below them, whose six neighbours are all
-// first-colour cells computed at steps p-2 (registers), p-1 (LDS) and p
-// (registers).  Ghost neighbours are folded as bc_scalar: dropped faces on
-// x-min, y and z, value 0 on x-max.
-constexpr int RB_TX = 64, RB_TY = 16, RB_ROWS = 4, RB_KC = 32;
-
-template <typename T, bool ZERO>
-__device__ __forceinline__ T rb_first(const Grid &g, const T *__restrict__ u_in, const T *__restrict__ f,
-                                      const T *__restrict__ a, const T *__restrict__ b, const T *__restrict__ c, T h2,
-                                      int x, int y, int p)
-{
-    const int sy = g.sy, sz = g.sz;
-    const int o = g.at(x, y, p);
-    const T ae = a[o], aw = a[o - 1], bn = b[o], bs = b[o - sy], ct = c[o], cb = c[o - sz];
-    T s = (T)0, d = ae;
-    if (!ZERO) {
-        if (x < g.nx - 1) s += ae * u_in[o + 1];
-        if (x > 2) s += aw * u_in[o - 1];
-        if (y < g.ny - 1) s += bn * u_in[o + sy];
-        if (y > 2) s += bs * u_in[o - sy];
-        if (p < g.nz - 1) s += ct * u_in[o + sz];
-        if (p > 2) s += cb * u_in[o - sz];
-    }
-    if (x > 2) d += aw;
-    if (y < g.ny - 1) d += bn;
-    if (y > 2) d += bs;
-    if (p < g.nz - 1) d += ct;
-    if (p > 2) d += cb;
-    return (s - h2 * f[o]) / d;
-}
-
-template <typename T, bool ZERO>
-__global__ void __launch_bounds__(RB_TX *(RB_TY / RB_ROWS)) k_rbgs_sweep(
-    Grid g, const T *__restrict__ u_in, T *__restrict__ u_out, const T *__restrict__ f, const T *__restrict__ a,
-    const T *__restrict__ b, const T *__restrict__ c, T h2, int first)
-{
-    __shared__ T lds[2][RB_TY][RB_TX];
-    const int tx = (int)threadIdx.x, ty0 = (int)threadIdx.y * RB_ROWS;
-    const int x = 2 + (RB_TX - 2) * (int)blockIdx.x - 1 + tx;
-    const int ybase = 2 + (RB_TY - 2) * (int)blockIdx.y - 1 + ty0;
-    const int k0 = 2 + RB_KC * (int)blockIdx.z;
-    const int kend = min(k0 + RB_KC - 1, g.nz - 1);
-    const bool xin = x >= 2 && x <= g.nx - 1;
-    const bool xown = xin && tx >= 1 && tx <= RB_TX - 2;
-    const int sy = g.sy, sz = g.sz;
-    T a_prev[RB_ROWS], a_new[RB_ROWS];
-#pragma unroll
-    for (int r = 0; r < RB_ROWS; r++) a_prev[r] = (T)0;
-    for (int p = k0 - 1; p <= kend + 1; p++) {
-        const bool pin = p >= 2 && p <= g.nz - 1;
-#pragma unroll
-        for (int r = 0; r < RB_ROWS; r++) {
-            const int y = ybase + r;
-            a_new[r] = (T)0;
-            const bool is_first = ((x + y + p) & 1) == first;
-            if (xin && y >= 2 && y <= g.ny - 1 && pin && is_first) {
-                a_new[r] = rb_first<T, ZERO>(g, u_in, f, a, b, c, h2, x, y, p);
-                lds[p & 1][ty0 + r][tx] = a_new[r];
-                const bool own = xown && ty0 + r >= 1 && ty0 + r <= RB_TY - 2;
-                if (own && p >= k0 && p <= kend) u_out[g.at(x, y, p)] = a_new[r];
-            }
-        }
-        __syncthreads();
-        const int q = p - 1;
-#pragma unroll
-        for (int r = 0; r < RB_ROWS; r++) {
-            const int y = ybase + r, ty = ty0 + r;
-            const bool is_first = ((x + y + p) & 1) == first;
-            const bool own = xown && y >= 2 && y <= g.ny - 1 && ty >= 1 && ty <= RB_TY - 2;
-            if (own && is_first && q >= k0 && q <= kend) {
-                // (x,y,q) has the second colour; its neighbours have the first
-                const int o = g.at(x, y, q);
-                const T ae = a[o], aw = a[o - 1], bn = b[o], bs = b[o - sy], ct = c[o], cb = c[o - sz];
-                T s = (T)0, d = ae;
-                if (x < g.nx - 1) s += ae * lds[q & 1][ty][tx + 1];
-                if (x > 2) s += aw * lds[q & 1][ty][tx - 1];
-                if (y < g.ny - 1) s += bn * lds[q & 1][ty + 1][tx];
-                if (y > 2) s += bs * lds[q & 1][ty - 1][tx];
-                if (q < g.nz - 1) s += ct * a_new[r];
-                if (q > 2) s += cb * a_prev[r];
-                if (x > 2) d += aw;
-                if (y < g.ny - 1) d += bn;
-                if (y > 2) d += bs;
-                if (q < g.nz - 1) d += ct;
-                if (q > 2) d += cb;
-                u_out[o] = (s - h2 * f[o]) / d;
-            }
-            if (is_first) a_prev[r] = a_new[r];
-        }
-        __syncthreads();
-    }
-}
-
 // r = f - A u
 template <typename T>
 __global__ void __launch_bounds__(256) k_resid_fold(Grid g, T *r, const T *u, const T *f, const T *a, const T *b,
@@ -500,42 +395,17 @@ static inline double lev_h2i(const Ctx &c, const Grid &g)
 }
 
 template <typename T>
-static void level_ptrs(Level &L, T *&u, T *&f, T *&r, const T *&a, const T *&b, const T *&cc, T *&w);
+static void level_ptrs(Level &L, T *&u, T *&f, T *&r, const T *&a, const T *&b, const T *&cc);
 template <>
 void level_ptrs<double>(Level &L, double *&u, double *&f, double *&r, const double *&a, const double *&b,
-                        const double *&cc, double *&w)
+                        const double *&cc)
 {
-    u = L.u64; f = L.f64; r = L.r64; a = L.a; b = L.b; cc = L.c; w = L.w64;
+    u = L.u64; f = L.f64; r = L.r64; a = L.a; b = L.b; cc = L.c;
 }
 template <>
-void level_ptrs<float>(Level &L, float *&u, float *&f, float *&r, const float *&a, const float *&b, const float *&cc,
-                       float *&w)
+void level_ptrs<float>(Level &L, float *&u, float *&f, float *&r, const float *&a, const float *&b, const float *&cc)
 {
-    u = L.u32; f = L.f32; r = L.r32; a = L.a32; b = L.b32; cc = L.c32; w = L.w32;
-}
-
-// one fused red-black sweep u_out = sweep(u_in); the finest level's launches
-// are the profiled kernel family PK_SWEEP
-template <typename T>
-static int fused_sweep(Ctx &c, const Grid &g, const T *u_in, T *u_out, const T *f, const T *a, const T *b,
-                       const T *cc, T h2, int first, bool zero, bool top)
-{
-    dim3 grid((unsigned)((g.nx - 2 + RB_TX - 3) / (RB_TX - 2)), (unsigned)((g.ny - 2 + RB_TY - 3) / (RB_TY - 2)),
-              (unsigned)((g.nz - 2 + RB_KC - 1) / RB_KC));
-    // algorithmic bytes of one red-black sweep: 7 words per cell (relax:
-    // a,b,c,ei,rhs,u -> u)
-    const double bytes = 7.0 * (double)sizeof(T) * (double)(g.nx - 2) * (g.ny - 2) * (g.nz - 2);
-    const bool profiled = top && !zero;  // the zero-guess sweep reads no u and is a different kernel
-    if (profiled) prof_begin(PK_SWEEP, c.stream);
-    if (zero)
-        hipLaunchKernelGGL((k_rbgs_sweep<T, true>), grid, dim3(RB_TX, RB_TY / RB_ROWS), 0, c.stream, g, u_in, u_out, f, a, b, cc,
-                           h2, first);
-    else
-        hipLaunchKernelGGL((k_rbgs_sweep<T, false>), grid, dim3(RB_TX, RB_TY / RB_ROWS), 0, c.stream, g, u_in, u_out, f, a, b,
-                           cc, h2, first);
-    if (profiled) prof_end(PK_SWEEP, c.stream, bytes);
-    DG_CHECK_LAUNCH();
-    return 0;
+    u = L.u32; f = L.f32; r = L.r32; a = L.a32; b = L.b32; cc = L.c32;
 }
 
 template <typename T>
@@ -576,9 +446,9 @@ static int vcycle(Ctx &c, int l)
 {
     Level &L = c.lev[l];
     const Grid g = L.g;
-    T *u, *f, *r, *w;
+    T *u, *f, *r;
     const T *a, *b, *cc;
-    level_ptrs<T>(L, u, f, r, a, b, cc, w);
+    level_ptrs<T>(L, u, f, r, a, b, cc);
     if (l == 0) {
         hipLaunchKernelGGL(k_coarse_solve<T>, dim3(1), dim3(64), 0, c.stream, g, c.coarse_M, f, u);
         DG_CHECK_LAUNCH();
@@ -590,60 +460,27 @@ static int vcycle(Ctx &c, int l)
     const int nu1 = (top || c.mg_nuc <= 0) ? c.mg_nu1 : c.mg_nuc;
     const int nu2 = (top || c.mg_nuc <= 0) ? c.mg_nu2 : c.mg_nuc;
     const T om = (T)c.mg_omega;
-    const bool fused = c.fused_smoother == 1;  // natural layout: one-pass sweeps on request only
-    if (fused && (c.mg_omega != 1.0 || nu1 != c.mg_nu1)) {
-        set_error("the fused smoother sweep supports Gauss-Seidel with equal sweep counts on all levels only");
-        return -1;
-    }
-    // cur: buffer holding the level's iterate.  With the fused (out-of-place)
-    // sweeps the buffers alternate; the first one is chosen so that the last
-    // post-smoothing sweep lands in u, where the caller reads the result.
-    T *cur = u;
-    if (fused) {
-        T *target = (c.mg_nu2 & 1) ? w : u;  // buffer that must hold the pre-smoothed iterate
-        T *other = target == u ? w : u;
-        cur = (c.mg_nu1 & 1) ? target : other;  // the first sweep writes here
-        if (c.mg_nu1 == 0) cur = target;
-        T *prev = cur == u ? w : u;  // unread by the zero-guess sweep
-        for (int sidx = 0; sidx < c.mg_nu1; sidx++) {
-            if (sidx > 0) { T *t = prev; prev = cur; cur = t; }
-            if (fused_sweep<T>(c, g, prev, cur, f, a, b, cc, h2, 0, sidx == 0, top)) return -1;
-        }
-    } else {
-        for (int sidx = 0; sidx < nu1; sidx++)
-            if (gs_sweep<T>(c, g, u, f, a, b, cc, h2, 0, 1, sidx == 0, top, om)) return -1;
-    }
+    for (int sidx = 0; sidx < nu1; sidx++)
+        if (gs_sweep<T>(c, g, u, f, a, b, cc, h2, 0, 1, sidx == 0, top, om)) return -1;
     if (nu1 == 0) {
-        hipLaunchKernelGGL(k_zero<T>, dim3((unsigned)((g.n() + 255) / 256)), dim3(256), 0, c.stream, g.n(), cur);
+        hipLaunchKernelGGL(k_zero<T>, dim3((unsigned)((g.n() + 255) / 256)), dim3(256), 0, c.stream, g.n(), u);
         DG_CHECK_LAUNCH();
     }
     prof_begin(PK_RESID, c.stream);
-    hipLaunchKernelGGL(k_resid_fold<T>, interior_grid(g), interior_block(), 0, c.stream, g, r, cur, f, a, b, cc, th2i);
+    hipLaunchKernelGGL(k_resid_fold<T>, interior_grid(g), interior_block(), 0, c.stream, g, r, u, f, a, b, cc, th2i);
     prof_end(PK_RESID, c.stream, 6.0 * (double)sizeof(T) * (double)(g.nx - 2) * (g.ny - 2) * (g.nz - 2));
     DG_CHECK_LAUNCH();
     Level &C = c.lev[l - 1];
-    T *uc, *fc, *rc, *wc;
+    T *uc, *fc, *rc;
     const T *ac, *bc, *ccc;
-    level_ptrs<T>(C, uc, fc, rc, ac, bc, ccc, wc);
+    level_ptrs<T>(C, uc, fc, rc, ac, bc, ccc);
     hipLaunchKernelGGL(k_restrict_avg<T>, interior_grid(C.g), interior_block(), 0, c.stream, C.g, g, fc, r);
     DG_CHECK_LAUNCH();
     if (vcycle<T>(c, l - 1)) return -1;
-    hipLaunchKernelGGL(k_prolong_inj_add<T>, interior_grid(g), interior_block(), 0, c.stream, g, C.g, cur, uc);
+    hipLaunchKernelGGL(k_prolong_inj_add<T>, interior_grid(g), interior_block(), 0, c.stream, g, C.g, u, uc);
     DG_CHECK_LAUNCH();
-    if (fused) {
-        for (int sidx = 0; sidx < c.mg_nu2; sidx++) {
-            T *nxt = cur == u ? w : u;
-            if (fused_sweep<T>(c, g, cur, nxt, f, a, b, cc, h2, 1, false, top)) return -1;
-            cur = nxt;
-        }
-        if (cur != u) {
-            set_error("V-cycle buffer bookkeeping");
-            return -1;
-        }
-    } else {
-        for (int sidx = 0; sidx < nu2; sidx++)
-            if (gs_sweep<T>(c, g, u, f, a, b, cc, h2, 1, 0, 0, top, om)) return -1;
-    }
+    for (int sidx = 0; sidx < nu2; sidx++)
+        if (gs_sweep<T>(c, g, u, f, a, b, cc, h2, 1, 0, 0, top, om)) return -1;
     return 0;
 }
 
@@ -1113,8 +950,6 @@ static int ensure_tol_buffers(Ctx &c)
             DG_HIP(hipMalloc(&L.u32, n * sizeof(float)));
             DG_HIP(hipMalloc(&L.f32, n * sizeof(float)));
             DG_HIP(hipMalloc(&L.r32, n * sizeof(float)));
-            DG_HIP(hipMalloc(&L.w32, n * sizeof(float)));
-            DG_HIP(hipMemset(L.w32, 0, n * sizeof(float)));
             DG_HIP(hipMemset(L.u32, 0, n * sizeof(float)));
             DG_HIP(hipMemset(L.f32, 0, n * sizeof(float)));
             DG_HIP(hipMemset(L.r32, 0, n * sizeof(float)));
@@ -1123,8 +958,6 @@ static int ensure_tol_buffers(Ctx &c)
             DG_HIP(hipMalloc(&L.u64, n * sizeof(double)));
             DG_HIP(hipMalloc(&L.f64, n * sizeof(double)));
             DG_HIP(hipMalloc(&L.r64, n * sizeof(double)));
-            DG_HIP(hipMalloc(&L.w64, n * sizeof(double)));
-            DG_HIP(hipMemset(L.w64, 0, n * sizeof(double)));
             DG_HIP(hipMemset(L.u64, 0, n * sizeof(double)));
             DG_HIP(hipMemset(L.f64, 0, n * sizeof(double)));
             DG_HIP(hipMemset(L.r64, 0, n * sizeof(double)));
@@ -1175,9 +1008,9 @@ static int pcg_iteration(Ctx &c)
     const int nblk = (int)(ig.x * ig.y * ig.z);
     const dim3 dgr(ig.x, ig.y, (unsigned)((g.nz - 2 + PCG_KCH - 1) / PCG_KCH));  // k_pcg_dir: columns of planes
     const int nblk_dir = (int)(dgr.x * dgr.y * dgr.z);
-    T *u, *f, *rr_, *w_;
+    T *u, *f, *rr_;
     const T *a_, *b_, *c_;
-    level_ptrs<T>(F, u, f, rr_, a_, b_, c_, w_);
+    level_ptrs<T>(F, u, f, rr_, a_, b_, c_);
     const double ncell = (double)(g.nx - 2) * (g.ny - 2) * (g.nz - 2);
     const bool split = rb_enabled(c);
     // search direction storage: FP32 with the FP32 V-cycle (mode 2), FP64
@@ -1248,12 +1081,12 @@ static int pcg_graphs(Ctx &c)
 {
     const int rb = rb_enabled(c);
     // everything that changes the captured kernel sequence or its arguments
-    const int key[14] = {c.mg_mode, c.mg_nu1, c.mg_nu2, c.fused_smoother, c.ng, (int)c.g.nx,
+    const int key[13] = {c.mg_mode, c.mg_nu1, c.mg_nu2, c.ng, (int)c.g.nx,
                          rb, rb ? c.lev[c.ng - 1].rb_ct : -1, rb ? c.rb_tail : -1, c.mg_nuc,
                          (int)llround(c.mg_omega * 1e9), (int)c.g.nz, rb ? c.rb_tiles : -1,
                          rb ? c.rb_tile_nu : -1};
     bool ok = c.pcg_graph[0] != nullptr;
-    for (int q = 0; q < 14 && ok; q++) ok = c.pcg_graph_key[q] == key[q];
+    for (int q = 0; q < 13 && ok; q++) ok = c.pcg_graph_key[q] == key[q];
     if (ok) return 0;
     for (int q = 0; q < 2; q++) {
         if (c.pcg_graph[q]) {
@@ -1274,7 +1107,7 @@ static int pcg_graphs(Ctx &c)
         DG_HIP(hipGraphInstantiate(&c.pcg_graph[0], graph, nullptr, nullptr, 0));
         DG_HIP(hipGraphDestroy(graph));
     }
-    for (int q = 0; q < 14; q++) c.pcg_graph_key[q] = key[q];
+    for (int q = 0; q < 13; q++) c.pcg_graph_key[q] = key[q];
     return 0;
 }
 

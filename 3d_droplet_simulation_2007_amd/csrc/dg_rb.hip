@@ -87,9 +87,9 @@ __global__ void __launch_bounds__(256) k_rb_in(Grid g, RbGeom r, const double *s
 // TOP only separates the finest level's launches into their own symbol.
 // The new value of one cell of colour `colour` (entry m of row (j, k)):
 // nb(m', dj, dk) is the other colour's value at entry m' of row (j+dj,
-// k+dk), uold the cell's own value before the pass.  Every split colour
-// update (the colour passes, the tail, the fused sweeps) goes through this
-// arithmetic, so their results agree to the bit.  Every load is issued:
+// k+dk), uold the cell's own value before the pass.  The colour passes
+// and the tail both go through this arithmetic, so their results agree to
+// the bit.  Every load is issued:
 // a missing neighbour reads the cell's own entry (or its own row) instead,
 // and the value is then selected away, so the loads of a cell are in flight
 // together rather than each behind a branch.  Offsets are 32-bit (every
@@ -195,7 +195,7 @@ __device__ inline void st4(double *p, const double v[4])
 // Other-colour values around a group of four entries m0..m0+3 of row
 // (j, k): row4(dj, dk) the four entries of row j+dj, plane k+dk; one(dm)
 // the entry m0+dm of the group's own row (dm = -1 or 4).  NbGlobal reads
-// the iterate in HBM; NbRing reads a plane ring in LDS (the fused sweep).
+// the iterate in HBM.
 template <typename T>
 struct NbGlobal {
     const T *o;     // other colour, row (j, k), entry m0
@@ -318,128 +318,6 @@ __global__ void __launch_bounds__(256) k_rb_colour4(RbGeom r, T *__restrict__ u,
     T out[4];
     rb_group4<CT, T, ZERO>(r, nb, f, u, cf, own, oth, h2, colour, om, zown, m0, j, k, out);
     st4(u + own, out);
-}
-
-// One red-black sweep of the finest level in one pass (colour ca, then the
-// other colour cb), out of place: X holds the iterate before the sweep, Y
-// receives it after.  A workgroup of TX x (TY + 2) threads owns TY whole
-// rows (TX groups of four entries each, TX = mh / 4) and a chunk of KC
-// planes, and marches up the planes: at step p every thread row updates
-// colour ca on plane p for one of the TY rows or the halo row on either
-// side (from X, exactly the first colour pass), keeps those values in a
-// three-plane LDS ring, and then the inner TY thread rows update colour cb
-// on plane p - 1 from the ring (exactly the second colour pass, which reads
-// the new ca values).  The ca planes just outside the chunk and the halo
-// rows are computed again by the neighbouring workgroups, which write them;
-// every cell is written by one workgroup.  Same results as the two colour
-// passes; the sweep reads f, X and the coefficients once and writes Y once.
-// ZERO: X is the zero iterate (the first sweep of a V-cycle: colour ca reads
-// no neighbour, colour cb no old own value).
-struct NbRing {
-    const float *ring;  // [3][TY + 2][mh]
-    int mh, rows;       // entries per row, rows per ring plane
-    int slot0, slot1, slotm;  // ring slots of planes k, k + 1, k - 1
-    int rr;             // ring row of the group's row
-    int m0;
-    __device__ inline const float *at(int dj, int dk) const
-    {
-        const int sl = dk == 0 ? slot0 : (dk > 0 ? slot1 : slotm);
-        return ring + ((size_t)sl * rows + (rr + dj)) * mh + m0;
-    }
-    __device__ inline void row4(int dj, int dk, float v[4]) const { ld4(at(dj, dk), v); }
-    __device__ inline float one(int dm) const { return at(0, 0)[dm]; }
-};
-
-constexpr int SWEEP_MAX_THREADS = 384;
-
-template <typename CT, bool ZERO>
-__global__ void __launch_bounds__(SWEEP_MAX_THREADS) k_rb_sweep4(RbGeom r, const float *__restrict__ X,
-                                                                 float *__restrict__ Y, const float *__restrict__ f,
-                                                                 Coef<CT> cf, float h2, int ca, float om, int kc)
-{
-    extern __shared__ __align__(16) float ring[];
-    const int rows = (int)blockDim.y, TY = rows - 2;
-    const uint3 tile = xcd_tile();
-    const int m0 = 4 * (int)threadIdx.x;  // blockDim.x * 4 == mh
-    const int j0 = (int)tile.y * TY + 2;
-    const int k0 = (int)tile.z * kc + 2, k1 = min(k0 + kc, r.nz);  // own planes k0 .. k1 - 1
-    const int cb = 1 - ca;
-    const size_t rs = (size_t)r.mh, ps = (size_t)r.mh * r.my;
-    for (int pl = k0 - 1; pl <= k1; pl++) {
-        // colour ca on plane pl: thread row rr takes grid row j0 - 1 + rr
-        const int rr = (int)threadIdx.y, ja = j0 - 1 + rr;
-        if (pl >= 2 && pl <= r.nz - 1 && ja >= 2 && ja <= r.ny - 1) {
-            const size_t row = ((size_t)(pl - 2) * r.my + (ja - 2)) * r.mh;
-            const size_t own = (size_t)ca * r.half + row + m0;
-            const size_t oth = (size_t)cb * r.half + row + m0;
-            const NbGlobal<float> nb{X + oth, rs, ps};
-            float out[4];
-            rb_group4<CT, float, ZERO>(r, nb, f, X, cf, own, oth, h2, ca, om, ZERO ? 1 : 0, m0, ja, pl, out);
-            float *dst = ring + ((size_t)(pl % 3) * rows + rr) * r.mh + m0;
-            *reinterpret_cast<float4 *>(dst) = make_float4(out[0], out[1], out[2], out[3]);
-            if (rr >= 1 && rr <= TY && pl >= k0 && pl < k1)
-                *reinterpret_cast<float4 *>(Y + own) = make_float4(out[0], out[1], out[2], out[3]);
-        }
-        __syncthreads();
-        // colour cb on plane pl - 1, the inner thread rows, from the ring
-        const int q = pl - 1, j = ja;
-        if (rr >= 1 && rr <= TY && q >= k0 && q < k1 && j <= r.ny - 1) {
-            const size_t row = ((size_t)(q - 2) * r.my + (j - 2)) * r.mh;
-            const size_t own = (size_t)cb * r.half + row + m0;
-            const size_t oth = (size_t)ca * r.half + row + m0;
-            const NbRing nb{ring, r.mh, rows, q % 3, (q + 1) % 3, (q + 2) % 3, rr, m0};
-            float out[4];
-            rb_group4<CT, float, false>(r, nb, f, X, cf, own, oth, h2, cb, om, ZERO ? 1 : 0, m0, j, q, out);
-            *reinterpret_cast<float4 *>(Y + own) = make_float4(out[0], out[1], out[2], out[3]);
-        }
-        __syncthreads();
-    }
-}
-
-// One red-black sweep of a level below the finest in one launch, out of
-// place (X before the sweep, Y after).  A workgroup owns a tile of the
-// second colour cb (FT_X entries x FT_Y rows x FT_Z planes); it first
-// updates the first colour ca on the tile and a one-entry, one-row,
-// one-plane ring around it from X (exactly the first colour pass) into
-// LDS, then cb on the tile from those values (exactly the second pass).
-// Ring cells are computed again by the workgroups that own them, which
-// write them; every cell is written once.  Two dependent phases per sweep
-// instead of two launches: on the small levels, where a launch costs more
-// than its bytes, that halves the sweep's time.
-constexpr int FT_X = 16, FT_Y = 8, FT_Z = 2;
-static_assert(FT_X * FT_Y * FT_Z == 256, "one thread per cell of the tile");
-
-template <typename CT, bool ZERO>
-__global__ void __launch_bounds__(256) k_rb_sweep_tile(RbGeom r, const float *__restrict__ X, float *__restrict__ Y,
-                                                       const float *__restrict__ f, Coef<CT> cf, float h2, int ca,
-                                                       float om)
-{
-    constexpr int HX = FT_X + 2, HY = FT_Y + 2, HZ = FT_Z + 2, HN = HX * HY * HZ;
-    __shared__ float A[HN];
-    const uint3 tile = xcd_tile();
-    const int m0 = (int)tile.x * FT_X, j0 = (int)tile.y * FT_Y + 2, k0 = (int)tile.z * FT_Z + 2;
-    const int tid = (int)threadIdx.x, cb = 1 - ca;
-    const ptrdiff_t rs = r.mh, ps = (ptrdiff_t)r.mh * r.my;
-    for (int e = tid; e < HN; e += 256) {
-        const int hx = e % HX, t = e / HX, hy = t % HY, hz = t / HY;
-        const int m = m0 - 1 + hx, j = j0 - 1 + hy, k = k0 - 1 + hz;
-        if (m < 0 || m >= r.mh || j < 2 || j > r.ny - 1 || k < 2 || k > r.nz - 1) continue;
-        const size_t row = ((size_t)(k - 2) * r.my + (j - 2)) * r.mh;
-        const float *o = X + (size_t)cb * r.half + row;
-        auto nb = [&](int mm, int dj, int dk) { return o[dj * rs + dk * ps + mm]; };
-        const float v = rb_cell_value<CT, float, ZERO>(r, nb, f, X, cf, h2, ca, om, ZERO ? 1 : 0, m, j, k);
-        A[e] = v;
-        if (hx >= 1 && hx <= FT_X && hy >= 1 && hy <= FT_Y && hz >= 1 && hz <= FT_Z)
-            Y[(size_t)ca * r.half + row + m] = v;
-    }
-    __syncthreads();
-    const int tx = tid % FT_X, ty = (tid / FT_X) % FT_Y, tz = tid / (FT_X * FT_Y);
-    const int m = m0 + tx, j = j0 + ty, k = k0 + tz;
-    if (m < r.mh && j <= r.ny - 1 && k <= r.nz - 1) {
-        auto nb = [&](int mm, int dj, int dk) { return A[((tz + 1 + dk) * HY + (ty + 1 + dj)) * HX + (mm - m0 + 1)]; };
-        const float v = rb_cell_value<CT, float, false>(r, nb, f, X, cf, h2, cb, om, ZERO ? 1 : 0, m, j, k);
-        Y[(size_t)cb * r.half + ((size_t)(k - 2) * r.my + (j - 2)) * r.mh + m] = v;
-    }
 }
 
 // coarse right-hand side = average of the 8 children's residuals; the
@@ -1206,86 +1084,6 @@ int launch_tail(Ctx &c, int lt)
     return 0;
 }
 
-// planes per workgroup of the fused sweep
-constexpr int SWEEP_KC = 4;
-
-// largest level (cells) that the tile sweep serves by default (fused = 2):
-// measured at 256^3, 7.6 us against two 4.4 us passes at 32^3 and 8.4
-// against 2 x 4.5 at 64^3, but 31.5 against 2 x 5.6 at 128^3, where its
-// recomputed ring costs more than the launch it saves
-constexpr size_t TILE_SWEEP_MAX_CELLS = 262144;
-
-// how level l sweeps (levels above the tail): 0 two colour passes, 1 the
-// plane-marching one-pass sweep (finest level, rows of 8..64 groups of four
-// entries, fused = 1), 2 the tile one-pass sweep (levels below the finest:
-// all with fused = 1, up to TILE_SWEEP_MAX_CELLS with fused = 2)
-int fused_kind(const Ctx &c, int l)
-{
-    // the one-pass sweeps exist for the FP32 iterate only
-    if (c.fused_smoother == 0 || !c.lev[l].rb_w || c.mg_mode != 2) return 0;
-    if (l != c.ng - 1)
-        return (c.fused_smoother == 1 || 2 * rb_geom(c.lev[l].g).half <= TILE_SWEEP_MAX_CELLS) ? 2 : 0;
-    if (c.fused_smoother != 1) return 0;
-    const RbGeom r = rb_geom(c.lev[l].g);
-    const int tx = r.mh / 4;
-    return r.mh % 4 == 0 && (tx == 8 || tx == 16 || tx == 32 || tx == 64) &&
-                   tx * (256 / tx + 2) <= SWEEP_MAX_THREADS
-               ? 1
-               : 0;
-}
-
-// one fused red-black sweep of level l: colour ca then the other, X -> Y
-template <typename CT>
-int sweep_fused(Ctx &c, int l, const float *X, float *Y, int ca, bool zero, int kind)
-{
-    Level &L = c.lev[l];
-    const RbGeom r = rb_geom(L.g);
-    const Coef<CT> cf{(const CT *)L.rb_coef, 2 * r.half};
-    const float h2 = (float)(1.0 / lev_h2i(c, L.g));
-    const float om = (float)c.mg_omega;
-    if (kind == 2) {
-        const dim3 grd((unsigned)((r.mh + FT_X - 1) / FT_X), (unsigned)((r.my + FT_Y - 1) / FT_Y),
-                       (unsigned)((r.mz + FT_Z - 1) / FT_Z));
-        if (zero)
-            hipLaunchKernelGGL((k_rb_sweep_tile<CT, true>), grd, dim3(256), 0, c.stream, r, X, Y, L.rb_f, cf, h2, ca, om);
-        else
-            hipLaunchKernelGGL((k_rb_sweep_tile<CT, false>), grd, dim3(256), 0, c.stream, r, X, Y, L.rb_f, cf, h2, ca,
-                               om);
-        DG_CHECK_LAUNCH();
-        return 0;
-    }
-    const int tx = r.mh / 4, ty = 256 / tx;  // ty owned rows, ty + 2 thread rows
-    const dim3 blk((unsigned)tx, (unsigned)(ty + 2));
-    const dim3 grd(1, (unsigned)((r.my + ty - 1) / ty), (unsigned)((r.mz + SWEEP_KC - 1) / SWEEP_KC));
-    const size_t lds = 3 * (size_t)(ty + 2) * r.mh * sizeof(float);
-    // algorithmic bytes: f, Y (and X unless zero) 4 B per cell, coefficients
-    const double cells = 2.0 * (double)r.half;
-    const double bytes = cells * (zero ? 8.0 : 12.0) + 3.0 * cells * (double)sizeof(CT);
-    prof_begin(PK_RB_COLOUR, c.stream);
-    if (zero)
-        hipLaunchKernelGGL((k_rb_sweep4<CT, true>), grd, blk, lds, c.stream, r, X, Y, L.rb_f, cf, h2, ca, om,
-                           SWEEP_KC);
-    else
-        hipLaunchKernelGGL((k_rb_sweep4<CT, false>), grd, blk, lds, c.stream, r, X, Y, L.rb_f, cf, h2, ca, om,
-                           SWEEP_KC);
-    prof_end(PK_RB_COLOUR, c.stream, bytes);
-    DG_CHECK_LAUNCH();
-    return 0;
-}
-
-// the one-pass sweep for the FP32 iterate; never reached for FP64
-// (fused_kind is 0 there)
-template <typename CT, typename T>
-int sweep_fused_t(Ctx &c, int l, const T *X, T *Y, int ca, bool zero, int kind)
-{
-    if constexpr (std::is_same<T, float>::value) {
-        return sweep_fused<CT>(c, l, X, Y, ca, zero, kind);
-    } else {
-        set_error("split V-cycle: no one-pass sweep for the FP64 iterate");
-        return -1;
-    }
-}
-
 // whether level l's sweeps run as the tile smoother: on request, below the
 // finest level, above the tail's size (rb_tiles 1: and at most 2^18 cells,
 // where the global passes are launch-bound), and when its tile's box fits
@@ -1350,24 +1148,10 @@ int vcycle_rb(Ctx &c, int l)
     const bool top = l == c.ng - 1;
     const int nu1 = (top || c.mg_nuc <= 0) ? c.mg_nu1 : c.mg_nuc;
     const int nu2 = (top || c.mg_nuc <= 0) ? c.mg_nu2 : c.mg_nuc;
-    // fused sweeps alternate between rb_u and rb_w; the first one writes the
-    // buffer that makes the last one land in rb_u
     const bool tiles = tile_level<CT, T>(c, l);
-    const int fk = tiles ? 0 : fused_kind(c, l);
-    const bool fused = fk != 0;
-    T *cur = lu;
-    T *const lw = (T *)(void *)L.rb_w;  // FP32 only (fused_kind)
-    auto other = [&](T *b) { return b == lu ? lw : lu; };
     const int nut1 = c.rb_tile_nu > 0 ? c.rb_tile_nu : nu1, nut2 = c.rb_tile_nu > 0 ? c.rb_tile_nu : nu2;
     if (tiles) {
         if (tile_smooth<CT, T, false>(c, l, nut1, CoarseE<T>{})) return -1;
-    } else if (fused) {
-        T *w1 = ((nu1 + nu2) & 1) ? lu : lw;
-        for (int q = 0; q < nu1; q++) {
-            T *y = q == 0 ? w1 : other(cur);
-            if (sweep_fused_t<CT, T>(c, l, cur, y, 0, q == 0, fk)) return -1;
-            cur = y;
-        }
     } else {
         // the finest level's pre-smoothing passes run back to back: one
         // timing pair for the run when the family is profiled
@@ -1392,7 +1176,7 @@ int vcycle_rb(Ctx &c, int l)
     T *const cus = csplit ? (tile_level<CT, T>(c, l - 1) ? rb_w_of<T>(C) : rb_u_of<T>(C)) : nullptr;
     const dim3 bc = row_block(C.g.nx - 2);
     hipLaunchKernelGGL((k_rb_resid_restrict<CT, T>), row_grid(bc, C.g.nx - 2, C.g.ny - 2, C.g.nz - 2), bc, 0, c.stream,
-                       rf, cur, lf, cf, (T)lev_h2i(c, L.g), rc, cfs, C.g, csplit ? nullptr : cf0);
+                       rf, lu, lf, cf, (T)lev_h2i(c, L.g), rc, cfs, C.g, csplit ? nullptr : cf0);
     DG_CHECK_LAUNCH();
     if (csplit) {
         if (vcycle_rb<CT, T>(c, l - 1)) return -1;
@@ -1402,24 +1186,14 @@ int vcycle_rb(Ctx &c, int l)
     const CoarseE<T> ce{rc, cus, C.g, csplit ? nullptr : cu0};
     // the colour passes take the prolongation into their first sweep
     if (tiles) return tile_smooth<CT, T, true>(c, l, nut2, ce);
-    const bool pro_folded = !fused && nu2 >= 1 && rf.mh % 4 == 0;
+    const bool pro_folded = nu2 >= 1 && rf.mh % 4 == 0;
     if (!pro_folded) {
         const dim3 bf = row_block(rf.mh);
-        hipLaunchKernelGGL(k_rb_prolong<T>, row_grid(bf, rf.mh, rf.my, rf.mz), bf, 0, c.stream, rf, cur, rc, cus, C.g,
+        hipLaunchKernelGGL(k_rb_prolong<T>, row_grid(bf, rf.mh, rf.my, rf.mz), bf, 0, c.stream, rf, lu, rc, cus, C.g,
                            csplit ? nullptr : cu0);
         DG_CHECK_LAUNCH();
     }
-    if (fused) {
-        for (int q = 0; q < nu2; q++) {
-            T *y = other(cur);
-            if (sweep_fused_t<CT, T>(c, l, cur, y, 1, false, fk)) return -1;
-            cur = y;
-        }
-        if (cur != lu) {
-            set_error("split V-cycle: fused sweep buffer bookkeeping");
-            return -1;
-        }
-    } else {
+    {
         double pb = 0.0;
         if (top) prof_begin(PK_RB_COLOUR, c.stream);
         for (int q = 0; q < nu2; q++) {
@@ -1479,7 +1253,7 @@ int rb_alloc(Ctx &c)
             DG_HIP(hipMemset(L.rb_u, 0, n2 * sizeof(float)));
             DG_HIP(hipMemset(L.rb_f, 0, n2 * sizeof(float)));
         }
-        if (!f64 && (c.fused_smoother != 0 || (c.rb_tiles && l < c.ng - 1)) && !L.rb_w) {
+        if (!f64 && (c.rb_tiles && l < c.ng - 1) && !L.rb_w) {
             DG_HIP(hipMalloc(&L.rb_w, n2 * sizeof(float)));
             DG_HIP(hipMemset(L.rb_w, 0, n2 * sizeof(float)));
         }

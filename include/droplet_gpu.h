@@ -141,13 +141,9 @@ int dg_set_ordered_sums(const int *on);
  * the run turns to NaN (swp main.f:848-1000).  0 (default): the reference's
  * arithmetic. */
 int dg_set_vof_guard(const int *on);
-/* Tolerance-mode execution switches.  Bit-identical either way: fused
- * selects one out-of-place pass per red-black sweep instead of two in-place
- * colour passes: 0 (default) nowhere; 1 on the natural layout (Gauss-Seidel,
- * equal sweep counts), the split finest level and the split levels below
- * it; 2 on the split levels below the finest of at most 2^18 cells; graphs = 1
- * (default) each PCG iteration replayed as a
- * captured HIP graph, 0 launched kernel by kernel.  split = 1 (default):
+/* Tolerance-mode execution switches.  graphs = 1 (default): each PCG
+ * iteration replayed as a captured HIP graph, 0 launched kernel by kernel
+ * (bit-identical).  split = 1 (default):
  * mode 2's FP32 V-cycle keeps each level above the coarsest in red-black
  * split storage with FP16 face coefficients when the densities allow
  * (different rounding of the preconditioner, same tolerance reached);
@@ -158,7 +154,7 @@ int dg_set_vof_guard(const int *on);
  * single-workgroup kernel in global memory, 2 (default) the same kernel on
  * LDS copies of those levels when they fit.  Applies to the resident state
  * and to later dg_init / dg_poisson_solve calls. */
-int dg_set_tuning(const int *fused, const int *graphs, const int *split, const int *tail);
+int dg_set_tuning(const int *graphs, const int *split, const int *tail);
 
 /* Tile smoother of the tolerance-mode V-cycle (no reference counterpart;
  * split storage only): on = 2 runs each level strictly between the finest

@@ -48,21 +48,18 @@ def sphere_system(n, seed):
 
 @pytest.mark.parametrize("n", [34, 66])
 def test_tuning_switches_are_bit_identical(dg, n):
-    """One-pass (fused) or two-pass sweeps on either layout, graph replay or
-    not, red-black split or natural storage, the small levels as one
-    workgroup (LDS or global memory) or one launch per operation: the same
-    FP32 arithmetic in the same order (the standalone solve keeps FP32
-    coefficients), so the same bits."""
+    """Graph replay or not, red-black split or natural storage, the small
+    levels as one workgroup (LDS or global memory) or one launch per
+    operation: the same FP32 arithmetic in the same order (the standalone
+    solve keeps FP32 coefficients), so the same bits."""
     print("case tuning", n, flush=True)
     rhs, a, b, c = sphere_system(n, 1)
     outs = []
-    combos = [("split", True, True, "lds"), (False, True, True, "lds"), ("split", False, True, "lds"),
-              (False, False, True, "lds"), ("split", True, True, "global"), (False, True, True, None),
-              ("split", False, True, None), (False, True, False, "lds"), (True, True, False, "lds"),
-              (True, False, False, "lds"), (False, False, False, "lds")]
+    combos = [(True, True, "lds"), (False, True, "lds"), (True, True, "global"), (True, True, None),
+              (False, True, None), (True, False, "lds"), (False, False, "lds")]
     try:
-        for fused, graphs, split, tail in combos:
-            dg.set_tuning(fused=fused, graphs=graphs, split=split, tail=tail)
+        for graphs, split, tail in combos:
+            dg.set_tuning(graphs=graphs, split=split, tail=tail)
             p, it, rel = dg.poisson_solve(rhs, a, b, c, mode=2, tol=1e-9)
             outs.append((p, it, rel))
     finally:
@@ -191,8 +188,8 @@ def test_matches_oracle_solve(dg, oracle):
 def test_smoother_options_match_oracle(dg, oracle, omega, nuc):
     """SOR weight and coarse-level sweep count: the FP64 solve follows the
     CPU statement of the same V-cycle (iterations within one, solutions to
-    1e-8), and the split FP32 V-cycle, in two colour passes or one fused
-    pass per sweep, stays bit-identical to the natural one."""
+    1e-8), and the split FP32 V-cycle stays bit-identical to the natural
+    one."""
     print("case smoother", omega, nuc, flush=True)
     rhs, a, b, c = sphere_system(34, 5)
     o = Oracle(inputdata_text(nx=34, ny=34, nz=34))
@@ -206,8 +203,8 @@ def test_smoother_options_match_oracle(dg, oracle, omega, nuc):
         dg.set_smoother(omega, nuc)
         pg, it, rel = dg.poisson_solve(rhs, a, b, c, mode=1, tol=1e-10)
         outs = []
-        for split, fused in ((True, False), (True, "split"), (False, False)):
-            dg.set_tuning(split=split, fused=fused)
+        for split in (True, False):
+            dg.set_tuning(split=split)
             outs.append(dg.poisson_solve(rhs, a, b, c, mode=2, tol=1e-9))
     finally:
         dg.set_smoother()

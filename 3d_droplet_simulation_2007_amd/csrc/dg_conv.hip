@@ -166,10 +166,10 @@ __global__ void __launch_bounds__(256) k_conv_fwd_lds(int nx, int ny, int nz, co
     if (z0 >= nz) return;  // the grid's rounding to a multiple of 8 blocks
     // offset table: k group g = 4 s + q reads channel group g % CG of tap
     // g / CG, at this many uint4 from the output voxel's own first group
-    if ((int)threadIdx.x < 4 * NSP) {
-        const int q = (int)threadIdx.x / NSP, s = (int)threadIdx.x % NSP, g = min(4 * s + q, S::G - 1);
+    for (int e = (int)threadIdx.x; e < 4 * NSP; e += 256) {
+        const int q = e / NSP, s = e % NSP, g = min(4 * s + q, S::G - 1);
         const int t = g / CG;
-        tab[threadIdx.x] = cf_toff<TL::HX, TL::HY>(t) * CG + (g - t * CG);
+        tab[e] = cf_toff<TL::HX, TL::HY>(t) * CG + (g - t * CG);
     }
     // the halo tile, eight 16-byte loads in flight per thread before their
     // stores; a place outside the grid loads voxel 0 and stores zeros
@@ -301,8 +301,9 @@ int conv3d_fwd(int nx, int ny, int nz, const bf16_t *x, int xs, int cinp, const 
     if (V <= 0) return 0;
     const int cg = cinp / 8;
     // the LDS-staged form where a row of 32 voxels is mostly inside the grid
-    // and the halo tile fits the LDS (up to 64 input channels)
-    if (nx >= CF_TX && (cg <= 2 || cg == 4 || cg == 6 || cg == 8) && (coutp == 16 || coutp == 32 || coutp == 64)) {
+    // and the halo tile fits the LDS (up to 128 input channels: 139 KB)
+    if (nx >= CF_TX && (cg <= 2 || cg == 4 || cg == 6 || cg == 8 || cg == 12 || cg == 16) &&
+        (coutp == 16 || coutp == 32 || coutp == 64)) {
         prof_begin(PK_CONV_FWD, s);
         int rc;
         switch (cg) {
@@ -310,7 +311,9 @@ int conv3d_fwd(int nx, int ny, int nz, const bf16_t *x, int xs, int cinp, const 
         case 2: rc = conv_fwd_lds_nt<2>(nx, ny, nz, x, xs, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s); break;
         case 4: rc = conv_fwd_lds_nt<4>(nx, ny, nz, x, xs, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s); break;
         case 6: rc = conv_fwd_lds_nt<6>(nx, ny, nz, x, xs, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s); break;
-        default: rc = conv_fwd_lds_nt<8>(nx, ny, nz, x, xs, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s); break;
+        case 8: rc = conv_fwd_lds_nt<8>(nx, ny, nz, x, xs, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s); break;
+        case 12: rc = conv_fwd_lds_nt<12>(nx, ny, nz, x, xs, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s); break;
+        default: rc = conv_fwd_lds_nt<16>(nx, ny, nz, x, xs, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s); break;
         }
         if (rc) return -1;
         DG_CHECK_LAUNCH();

@@ -59,6 +59,10 @@ LDS_CASES = [
     ((32, 5, 2), 64, 64, 16, 16, True),
     ((36, 3, 4), 64, 64, 64, 64, True),
     ((64, 2, 3), 16, 16, 16, 1, False),
+    ((33, 3, 3), 96, 96, 32, 32, True),
+    ((32, 2, 3), 96, 90, 64, 64, True),
+    ((34, 3, 2), 128, 128, 64, 64, True),
+    ((32, 2, 2), 128, 128, 16, 16, False),
 ]
 
 

@@ -570,15 +570,15 @@ static_assert(S_AH + PCG_HIST <= 64, "device scalar slots");
 
 // K2: p_new = z + beta p_old (beta from device scalars), q = A p_new,
 // partial sums of p_new . q
-// planes per workgroup of k_pcg_dir
-constexpr int PCG_KCH = 8;
+// planes per workgroup of k_pcg_dir, and rows of its column tile
+constexpr int PCG_KCH = 8, PCG_TY = 4;
 
 // Barrier for LDS hand-offs only: waits for this wave's LDS operations,
 // not for its global loads in flight (which __syncthreads' workgroup fence
 // would drain), so loads issued before it overlap the wait.
 __device__ inline void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
-// A workgroup of 64 x 4 threads owns a 64 x 4 column tile and marches up
+// A workgroup of 64 x PCG_TY threads owns a 64 x PCG_TY column tile and marches up
 // PCG_KCH planes.  Per plane it forms p_new = z + beta p_old once for the
 // tile and a one-cell ring around it in LDS, so the in-plane neighbours
 // come from LDS; each thread carries its column's plane-below value and
@@ -591,21 +591,21 @@ __device__ inline void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_b
 // direction; p_new is rounded to it before the stencil, so q is A applied
 // to the stored direction and x, r stay consistent.
 template <class Z, typename PT>
-__global__ void __launch_bounds__(256) k_pcg_dir(Grid g, Z z, PT *hist, size_t nslot, double *q, const double *a,
+__global__ void __launch_bounds__(64 * PCG_TY) k_pcg_dir(Grid g, Z z, PT *hist, size_t nslot, double *q, const double *a,
                                                  const double *b, const double *c, double h2i, const double *ds,
                                                  double *partial)
 {
     const int it = (int)ds[S_IT];
     PT *pnew = hist + (size_t)(it % PCG_HIST) * nslot;
     const PT *pold = hist + (size_t)((it + PCG_HIST - 1) % PCG_HIST) * nslot;
-    constexpr int TW = 66, TH = 6, NE = TH * TW, NFE = (NE + 255) / 256;  // tile + ring
+    constexpr int NT = 64 * PCG_TY, TW = 66, TH = PCG_TY + 2, NE = TH * TW, NFE = (NE + NT - 1) / NT;  // tile + ring
     __shared__ double pl[3][TH][TW];
-    __shared__ double lds4[4];
+    __shared__ double ldsw[PCG_TY];
     unsigned lin;
     const uint3 tile = xcd_tile(&lin);
     const int bidx = (int)lin;
     const int tx = (int)threadIdx.x, ty = (int)threadIdx.y, tid = tx + 64 * ty;
-    const int i0 = (int)tile.x * 64 + 2, j0 = (int)tile.y * 4 + 2;
+    const int i0 = (int)tile.x * 64 + 2, j0 = (int)tile.y * PCG_TY + 2;
     const int i = i0 + tx, j = j0 + ty;
     const int k0 = (int)tile.z * PCG_KCH + 2;
     const int k1 = min(k0 + PCG_KCH, g.nz);  // interior planes k0 .. k1 - 1 (the last is nz - 1)
@@ -615,14 +615,14 @@ __global__ void __launch_bounds__(256) k_pcg_dir(Grid g, Z z, PT *hist, size_t n
     const int sy = g.sy, sz = g.sz;
     // the first iteration's direction is z itself (p_old is not read)
     auto pdir = [&](double zz, PT po) { return (double)(PT)(first ? zz : zz + beta * (double)po); };
-    // this thread's ring places e = tid + 256 u: loads of plane kk (places
+    // this thread's ring places e = tid + NT u: loads of plane kk (places
     // outside the interior load the cell (2, 2, kk) and store 0: no
     // stencil reads them) and their stores into buffer buf
     int ei[NFE], ej[NFE];
     bool eok[NFE];
 #pragma unroll
     for (int u = 0; u < NFE; u++) {
-        const int e = tid + 256 * u, yy = e / TW, xx = e - yy * TW;
+        const int e = tid + NT * u, yy = e / TW, xx = e - yy * TW;
         const int ii = i0 - 1 + xx, jj = j0 - 1 + yy;
         eok[u] = e < NE && ii >= 2 && ii <= g.nx - 1 && jj >= 2 && jj <= g.ny - 1;
         ei[u] = eok[u] ? ii : 2;
@@ -641,7 +641,7 @@ __global__ void __launch_bounds__(256) k_pcg_dir(Grid g, Z z, PT *hist, size_t n
     auto store = [&](int buf) {
 #pragma unroll
         for (int u = 0; u < NFE; u++) {
-            const int e = tid + 256 * u;
+            const int e = tid + NT * u;
             if (e < NE) (&pl[buf][0][0])[e] = eok[u] ? pdir(zr[u], pr[u]) : 0.0;
         }
     };
@@ -689,11 +689,15 @@ __global__ void __launch_bounds__(256) k_pcg_dir(Grid g, Z z, PT *hist, size_t n
         }
         cur = nxt;
     }
-    // block of 64x4: reduce
+    // block of 64 x PCG_TY: reduce
     double v = wave_sum(contrib);
-    if ((tid & 63) == 0) lds4[tid >> 6] = v;
+    if ((tid & 63) == 0) ldsw[tid >> 6] = v;
     __syncthreads();
-    if (tid == 0) partial[bidx] = lds4[0] + lds4[1] + lds4[2] + lds4[3];
+    if (tid == 0) {
+        double t = ldsw[0];
+        for (int w = 1; w < PCG_TY; w++) t += ldsw[w];
+        partial[bidx] = t;
+    }
 }
 
 // K3: alpha = rz/pq; r -= alpha q (x takes alpha p in k_pcg_flush); partial sums of r.r
@@ -1006,7 +1010,10 @@ static int pcg_iteration(Ctx &c)
     const size_t nslot = g.n();
     const dim3 ig = interior_grid(g), ib = interior_block();
     const int nblk = (int)(ig.x * ig.y * ig.z);
-    const dim3 dgr(ig.x, ig.y, (unsigned)((g.nz - 2 + PCG_KCH - 1) / PCG_KCH));  // k_pcg_dir: columns of planes
+    // k_pcg_dir: 64 x PCG_TY columns of PCG_KCH planes
+    const dim3 dgr((unsigned)((g.nx - 2 + 63) / 64), (unsigned)((g.ny - 2 + PCG_TY - 1) / PCG_TY),
+                   (unsigned)((g.nz - 2 + PCG_KCH - 1) / PCG_KCH)),
+        dib(64, PCG_TY);
     const int nblk_dir = (int)(dgr.x * dgr.y * dgr.z);
     T *u, *f, *rr_;
     const T *a_, *b_, *c_;
@@ -1027,7 +1034,7 @@ static int pcg_iteration(Ctx &c)
         hipLaunchKernelGGL(k_final2, dim3(1), dim3(256), 0, s, partial, RED_BLOCKS, ds + S_RZ);
         DG_CHECK_LAUNCH();
         prof_begin(PK_PCG_DIR, s);
-        hipLaunchKernelGGL((k_pcg_dir<ZSplit<T>, PT>), dgr, ib, 0, s, g, z, (PT *)c.pcg_p, nslot, q, F.a, F.b, F.c, h2i,
+        hipLaunchKernelGGL((k_pcg_dir<ZSplit<T>, PT>), dgr, dib, 0, s, g, z, (PT *)c.pcg_p, nslot, q, F.a, F.b, F.c, h2i,
                            ds, partial);
         prof_end(PK_PCG_DIR, s, ((double)sizeof(T) + 2.0 * sizeof(PT) + 4.0 * 8.0) * ncell);
         DG_CHECK_LAUNCH();
@@ -1045,7 +1052,7 @@ static int pcg_iteration(Ctx &c)
         hipLaunchKernelGGL(k_final2, dim3(1), dim3(256), 0, s, partial, RED_BLOCKS, ds + S_RZ);
         DG_CHECK_LAUNCH();
         prof_begin(PK_PCG_DIR, s);
-        hipLaunchKernelGGL((k_pcg_dir<ZNatural<T>, PT>), dgr, ib, 0, s, g, z, (PT *)c.pcg_p, nslot, q, F.a, F.b, F.c,
+        hipLaunchKernelGGL((k_pcg_dir<ZNatural<T>, PT>), dgr, dib, 0, s, g, z, (PT *)c.pcg_p, nslot, q, F.a, F.b, F.c,
                            h2i, ds, partial);
         prof_end(PK_PCG_DIR, s, ((double)sizeof(T) + 2.0 * sizeof(PT) + 4.0 * 8.0) * ncell);
         DG_CHECK_LAUNCH();

@@ -112,28 +112,29 @@ __global__ void __launch_bounds__(256) k_conv_fwd(int nx, int ny, int nz, const 
 }
 
 // The same convolution with the input staged in LDS: a workgroup owns a
-// tile of 32 x 2R x 2 voxels (each wave R rows of 32 voxels: 2R 16-row M
-// tiles; R = 2 up to 48 input channels, else 1) and first copies the tile's
-// input with its one-voxel halo, 34 x (2R+2) x 4 voxels of all cinp
-// channels, from HBM into LDS with 16-byte loads (zeros outside the grid).
+// tile of TX x 2R x 2 voxels (TX = 32, or 16 on grids of 16 to 31 cells in
+// x; each wave R rows of TX voxels: TX / 16 * R M tiles of 16 voxels; R = 2
+// up to 48 input channels, else 1) and first copies the tile's input with
+// its one-voxel halo, (TX+2) x (2R+2) x 4 voxels of all cinp channels, from
+// HBM into LDS with 16-byte loads (zeros outside the grid).
 // The K loop is unrolled at compile time (CG = cinp / 8 channel groups, NT =
 // coutp / 16 output tiles): each step reads the lane's A offset from a small
 // LDS table (k group -> tap and channel group), its A fragments from the
 // halo tile at constant distances, and its B fragment (packed weights, from
 // global memory: small, cached) D steps ahead of its use, once for all of
 // the wave's M tiles.  Same K order, same MFMAs: the same bits.
-constexpr int CF_TX = 32, CF_TZ = 2;
+constexpr int CF_TZ = 2;
 
-template <int R>
+template <int R, int TX>
 struct CfTile {
-    static constexpr int TY = 2 * R, HX = CF_TX + 2, HY = TY + 2, HZ = CF_TZ + 2, HN = HX * HY * HZ;
+    static constexpr int MPR = TX / 16, TY = 2 * R, HX = TX + 2, HY = TY + 2, HZ = CF_TZ + 2, HN = HX * HY * HZ;
 };
 
-template <int CG>
+template <int CG, int TX>
 struct CfShape {
     static constexpr int R = CG <= 6 ? 2 : 1, G = 27 * CG, NS = (G + 3) / 4, NSP = (NS + 3) / 4 * 4,
                          REM = G - 4 * (NS - 1);
-    using TL = CfTile<R>;
+    using TL = CfTile<R, TX>;
     static constexpr size_t lds = (size_t)TL::HN * CG * 16 + (size_t)4 * NSP * 4;
 };
 
@@ -144,15 +145,15 @@ __device__ constexpr int cf_toff(int t)
     return (t / 9 - 1) * HX * HY + ((t / 3) % 3 - 1) * HX + (t % 3 - 1);
 }
 
-template <int NT, int CG>
+template <int NT, int CG, int TX>
 __global__ void __launch_bounds__(256) k_conv_fwd_lds(int nx, int ny, int nz, const bf16_t *__restrict__ x, int xs,
                                                       const bf16_t *__restrict__ wB, const float *__restrict__ bias,
                                                       int cout, int relu, bf16_t *__restrict__ y, int ys,
                                                       float *__restrict__ yf, int yfs, int ntx, int nty)
 {
-    using S = CfShape<CG>;
+    using S = CfShape<CG, TX>;
     using TL = typename S::TL;
-    constexpr int R = S::R, MT = 2 * R, NS = S::NS, NSP = S::NSP, REM = S::REM;
+    constexpr int R = S::R, MPR = TL::MPR, MT = MPR * R, NS = S::NS, NSP = S::NSP, REM = S::REM;
     constexpr int COUTP = 16 * NT;
     constexpr int D = NT >= 4 ? 2 : 4;  // B prefetch distance in steps
     extern __shared__ __align__(16) unsigned char cf_lds[];
@@ -162,7 +163,7 @@ __global__ void __launch_bounds__(256) k_conv_fwd_lds(int nx, int ny, int nz, co
     const int row = lane & 15, kg = lane >> 4;
     const int blk = xcd_tile((int)blockIdx.x, (int)gridDim.x);
     const int tx = blk % ntx, ty = (blk / ntx) % nty, tz = blk / (ntx * nty);
-    const int x0 = tx * CF_TX, y0 = ty * TL::TY, z0 = tz * CF_TZ;
+    const int x0 = tx * TX, y0 = ty * TL::TY, z0 = tz * CF_TZ;
     if (z0 >= nz) return;  // the grid's rounding to a multiple of 8 blocks
     // offset table: k group g = 4 s + q reads channel group g % CG of tap
     // g / CG, at this many uint4 from the output voxel's own first group
@@ -222,7 +223,7 @@ __global__ void __launch_bounds__(256) k_conv_fwd_lds(int nx, int ny, int nz, co
         bf16x8 a[MT];
 #pragma unroll
         for (int m = 0; m < MT; m++) {
-            const bf16x8 av = __builtin_bit_cast(bf16x8, ab[off + ((m >> 1) * TL::HX + (m & 1) * 16) * CG]);
+            const bf16x8 av = __builtin_bit_cast(bf16x8, ab[off + ((m / MPR) * TL::HX + (m % MPR) * 16) * CG]);
             a[m] = ok ? av : zero8();
         }
         bf16x8 *const b = bq[s % D];
@@ -239,11 +240,11 @@ __global__ void __launch_bounds__(256) k_conv_fwd_lds(int nx, int ny, int nz, co
         const float bv = bias ? bias[co] : 0.f;
 #pragma unroll
         for (int m = 0; m < MT; m++) {
-            const int tr = tr0 + (m >> 1), gyw = y0 + tr % TL::TY, gzw = z0 + tr / TL::TY;
+            const int tr = tr0 + m / MPR, gyw = y0 + tr % TL::TY, gzw = z0 + tr / TL::TY;
             if (gyw >= ny || gzw >= nz) continue;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int gx = x0 + (m & 1) * 16 + kg * 4 + r;
+                const int gx = x0 + (m % MPR) * 16 + kg * 4 + r;
                 if (gx >= nx) continue;
                 const size_t v = ((size_t)gzw * ny + gyw) * nx + gx;
                 float val = acc[m][n][r] + bv;
@@ -257,32 +258,40 @@ __global__ void __launch_bounds__(256) k_conv_fwd_lds(int nx, int ny, int nz, co
     }
 }
 
-template <int NT, int CG>
+template <int NT, int CG, int TX>
 static int launch_conv_fwd_lds(int nx, int ny, int nz, const bf16_t *x, int xs, const bf16_t *wB, const float *bias,
                                int cout, int relu, bf16_t *y, int ys, float *yf, int yfs, hipStream_t s)
 {
-    using S = CfShape<CG>;
+    using S = CfShape<CG, TX>;
     using TL = typename S::TL;
-    const int ntx = (nx + CF_TX - 1) / CF_TX, nty = (ny + TL::TY - 1) / TL::TY, ntz = (nz + CF_TZ - 1) / CF_TZ;
+    const int ntx = (nx + TX - 1) / TX, nty = (ny + TL::TY - 1) / TL::TY, ntz = (nz + CF_TZ - 1) / CF_TZ;
     long long nbl = (long long)ntx * nty * ntz;
     nbl = (nbl + 7) / 8 * 8;
     if (S::lds > 48 * 1024)
-        DG_HIP(hipFuncSetAttribute((const void *)k_conv_fwd_lds<NT, CG>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)S::lds));
-    hipLaunchKernelGGL((k_conv_fwd_lds<NT, CG>), dim3((unsigned)nbl), dim3(256), S::lds, s, nx, ny, nz, x, xs, wB,
+        DG_HIP(hipFuncSetAttribute((const void *)k_conv_fwd_lds<NT, CG, TX>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)S::lds));
+    hipLaunchKernelGGL((k_conv_fwd_lds<NT, CG, TX>), dim3((unsigned)nbl), dim3(256), S::lds, s, nx, ny, nz, x, xs, wB,
                        bias, cout, relu, y, ys, yf, yfs, ntx, nty);
     return 0;
 }
 
-template <int CG>
+template <int CG, int TX>
 static int conv_fwd_lds_nt(int nx, int ny, int nz, const bf16_t *x, int xs, const bf16_t *wB, int coutp,
                            const float *bias, int cout, int relu, bf16_t *y, int ys, float *yf, int yfs, hipStream_t s)
 {
     switch (coutp) {
-    case 16: return launch_conv_fwd_lds<1, CG>(nx, ny, nz, x, xs, wB, bias, cout, relu, y, ys, yf, yfs, s);
-    case 32: return launch_conv_fwd_lds<2, CG>(nx, ny, nz, x, xs, wB, bias, cout, relu, y, ys, yf, yfs, s);
-    default: return launch_conv_fwd_lds<4, CG>(nx, ny, nz, x, xs, wB, bias, cout, relu, y, ys, yf, yfs, s);
+    case 16: return launch_conv_fwd_lds<1, CG, TX>(nx, ny, nz, x, xs, wB, bias, cout, relu, y, ys, yf, yfs, s);
+    case 32: return launch_conv_fwd_lds<2, CG, TX>(nx, ny, nz, x, xs, wB, bias, cout, relu, y, ys, yf, yfs, s);
+    default: return launch_conv_fwd_lds<4, CG, TX>(nx, ny, nz, x, xs, wB, bias, cout, relu, y, ys, yf, yfs, s);
     }
+}
+
+template <int CG>
+static int conv_fwd_lds_tx(int nx, int ny, int nz, const bf16_t *x, int xs, const bf16_t *wB, int coutp,
+                           const float *bias, int cout, int relu, bf16_t *y, int ys, float *yf, int yfs, hipStream_t s)
+{
+    return nx >= 32 ? conv_fwd_lds_nt<CG, 32>(nx, ny, nz, x, xs, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s)
+                    : conv_fwd_lds_nt<CG, 16>(nx, ny, nz, x, xs, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s);
 }
 
 int conv3d_fwd(int nx, int ny, int nz, const bf16_t *x, int xs, int cinp, const bf16_t *wB, int coutp,
@@ -300,20 +309,21 @@ int conv3d_fwd(int nx, int ny, int nz, const bf16_t *x, int xs, int cinp, const 
     const long long V = (long long)nx * ny * nz;
     if (V <= 0) return 0;
     const int cg = cinp / 8;
-    // the LDS-staged form where a row of 32 voxels is mostly inside the grid
-    // and the halo tile fits the LDS (up to 128 input channels: 139 KB)
-    if (nx >= CF_TX && (cg <= 2 || cg == 4 || cg == 6 || cg == 8 || cg == 12 || cg == 16) &&
+    // the LDS-staged form where a row of 16 or 32 voxels is mostly inside
+    // the grid and the halo tile fits the LDS (up to 128 input channels:
+    // 139 KB)
+    if (nx >= 16 && (cg <= 2 || cg == 4 || cg == 6 || cg == 8 || cg == 12 || cg == 16) &&
         (coutp == 16 || coutp == 32 || coutp == 64)) {
         prof_begin(PK_CONV_FWD, s);
         int rc;
         switch (cg) {
-        case 1: rc = conv_fwd_lds_nt<1>(nx, ny, nz, x, xs, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s); break;
-        case 2: rc = conv_fwd_lds_nt<2>(nx, ny, nz, x, xs, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s); break;
-        case 4: rc = conv_fwd_lds_nt<4>(nx, ny, nz, x, xs, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s); break;
-        case 6: rc = conv_fwd_lds_nt<6>(nx, ny, nz, x, xs, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s); break;
-        case 8: rc = conv_fwd_lds_nt<8>(nx, ny, nz, x, xs, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s); break;
-        case 12: rc = conv_fwd_lds_nt<12>(nx, ny, nz, x, xs, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s); break;
-        default: rc = conv_fwd_lds_nt<16>(nx, ny, nz, x, xs, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s); break;
+        case 1: rc = conv_fwd_lds_tx<1>(nx, ny, nz, x, xs, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s); break;
+        case 2: rc = conv_fwd_lds_tx<2>(nx, ny, nz, x, xs, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s); break;
+        case 4: rc = conv_fwd_lds_tx<4>(nx, ny, nz, x, xs, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s); break;
+        case 6: rc = conv_fwd_lds_tx<6>(nx, ny, nz, x, xs, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s); break;
+        case 8: rc = conv_fwd_lds_tx<8>(nx, ny, nz, x, xs, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s); break;
+        case 12: rc = conv_fwd_lds_tx<12>(nx, ny, nz, x, xs, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s); break;
+        default: rc = conv_fwd_lds_tx<16>(nx, ny, nz, x, xs, wB, coutp, bias, cout, relu, y, ys, yf, yfs, s); break;
         }
         if (rc) return -1;
         DG_CHECK_LAUNCH();

@@ -44,9 +44,10 @@ CONV_CASES = [
 ]
 
 
-# grids of at least 32 cells in x take the LDS-staged forward kernel (a
-# workgroup per 32 x 2R x 2 tile): every (input groups, output tiles) form it
-# is built for, full tiles and tiles clipped in x, y and z
+# grids of at least 16 cells in x take the LDS-staged forward kernel (a
+# workgroup per 32 x 2R x 2 tile, 16 x 2R x 2 below 32 cells): every (input
+# groups, output tiles) form it is built for, full tiles and tiles clipped in
+# x, y and z
 LDS_CASES = [
     ((32, 4, 2), 8, 4, 16, 16, True),
     ((35, 5, 3), 16, 16, 16, 16, True),
@@ -63,6 +64,12 @@ LDS_CASES = [
     ((32, 2, 3), 96, 90, 64, 64, True),
     ((34, 3, 2), 128, 128, 64, 64, True),
     ((32, 2, 2), 128, 128, 16, 16, False),
+    ((16, 4, 3), 64, 64, 64, 64, True),
+    ((20, 3, 2), 64, 64, 64, 64, True),
+    ((17, 2, 3), 16, 16, 16, 16, True),
+    ((24, 5, 3), 48, 48, 32, 32, True),
+    ((16, 2, 2), 128, 128, 64, 64, True),
+    ((31, 3, 3), 8, 4, 16, 16, True),
 ]
 
 

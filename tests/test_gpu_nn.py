@@ -45,9 +45,9 @@ CONV_CASES = [
 
 
 # grids of at least 16 cells in x take the LDS-staged forward kernel (a
-# workgroup per 32 x 2R x 2 tile, 16 x 2R x 2 below 32 cells): every (input
-# groups, output tiles) form it is built for, full tiles and tiles clipped in
-# x, y and z
+# workgroup per 32 x 2R x 2 tile with 8, 16 or 48 input channels on grids of
+# 32+ cells in x, 16 x 2R x 2 otherwise): every (input groups, output tiles,
+# row width) form it is built for, full tiles and tiles clipped in x, y and z
 LDS_CASES = [
     ((32, 4, 2), 8, 4, 16, 16, True),
     ((35, 5, 3), 16, 16, 16, 16, True),

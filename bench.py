@@ -399,7 +399,8 @@ def main():
         gs.forward(feat)
     cp = dg.profile_stop()
     conv_tf = cp["bytes"] / (cp["total_ms"] * 1e-3) / 1e12
-    line["mfma_conv"] = {"kernel": "k_conv_fwd (BF16 MFMA 16x16x32, FP32 accumulate), every layer of the network",
+    line["mfma_conv"] = {"kernel": "k_conv_fwd_lds (BF16 MFMA 16x16x32, FP32 accumulate, input tile staged in LDS), "
+                                   "every layer of the network",
                          "bound": "mfma", "achieved": conv_tf, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                          "frac": conv_tf / MFMA_BF16_PEAK_TFLOPS, "input": [nc, nc, nc, 4],
                          "flops_per_forward": cp["bytes"] / 3, "ms_per_forward": cp["total_ms"] / 3,
